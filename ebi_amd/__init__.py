@@ -1,0 +1,538 @@
+"""ebi_amd -- MI355X (gfx950) chunked decode and in-situ query path for EBI1 files.
+
+Python mirror of the reference's public API (lemolatoon/ebi, ebi/src/api/
+decoder.rs and api/encoder.rs) over the C ABI of ``libebi_hip.so``
+(include/ebi_hip.h).  Every query runs in HIP kernels on the GPU; there is no
+CPU fallback: if the library or a GPU is missing, the calls raise.
+
+    cfg = CompressorConfig.buff(scale=100)
+    file = encode(values, cfg, ChunkOption.record_count(131072))
+    with Context(0) as ctx:
+        dec = Decoder(ctx, file)
+        ids = dec.filter(Predicate.range(RangeValue.inclusive(99.5),
+                                         RangeValue.inclusive(100.5)))
+        total = dec.sum(bitmask=ids)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Iterable, Optional
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libebi_hip.so"
+_lib = None
+
+OK, UNEXPECTED_END, CONVERSION, PRECONDITION, HIP, OOM, REJECTED, INVALID_ARGUMENT, UNSUPPORTED = range(9)
+_STATUS = {
+    1: "UnexpectedEndOfChunk",
+    2: "ConversionError",
+    3: "PreconditionsNotMet",
+    4: "HipError",
+    5: "OutOfMemory",
+    6: "Rejected",
+    7: "InvalidArgument",
+    8: "Unsupported",
+}
+
+SCHEMES = {
+    "uncompressed": 0,
+    "rle": 1,
+    "buff": 2,
+    "gorilla": 3,
+    "chimp": 4,
+    "chimp128": 5,
+    "elf_on_chimp": 6,
+    "elf": 7,
+    "delta_sprintz": 8,
+}
+SCHEME_NAMES = {v: k for k, v in SCHEMES.items()}
+
+
+class EbiError(RuntimeError):
+    """A non-zero ebi_status; mirrors decoder::DecoderError (decoder/error.rs:6-21)."""
+
+    def __init__(self, status: int, message: str = ""):
+        self.status = status
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+
+
+class _Pred(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("start_kind", C.c_int32),
+        ("end_kind", C.c_int32),
+        ("reserved", C.c_int32),
+        ("value", C.c_double),
+        ("start", C.c_double),
+        ("end", C.c_double),
+    ]
+
+
+class _EncCfg(C.Structure):
+    _fields_ = [
+        ("scheme", C.c_int32),
+        ("chunk_kind", C.c_int32),
+        ("chunk_value", C.c_uint64),
+        ("capacity", C.c_uint64),
+        ("scale", C.c_uint64),
+        ("n_threads", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class _FileInfo(C.Structure):
+    _fields_ = [
+        ("number_of_records", C.c_uint64),
+        ("number_of_chunks", C.c_uint64),
+        ("footer_offset", C.c_uint64),
+        ("chunk_option_value", C.c_uint64),
+        ("config_capacity", C.c_uint64),
+        ("config_scale", C.c_uint64),
+        ("scheme", C.c_int32),
+        ("chunk_option_kind", C.c_int32),
+        ("version", C.c_uint16 * 3),
+        ("reserved", C.c_uint16),
+    ]
+
+
+_P = C.c_void_p
+_SIGS = {
+    "ebi_encode": (C.c_int, [_P, C.c_size_t, C.POINTER(_EncCfg), C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "ebi_buffer_free": (None, [_P]),
+    "ebi_file_parse": (C.c_int, [_P, C.c_size_t, C.POINTER(_FileInfo)]),
+    "ebi_file_chunks": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P]),
+    "ebi_hip_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "ebi_hip_ctx_destroy": (C.c_int, [_P]),
+    "ebi_hip_last_error": (C.c_char_p, [_P]),
+    "ebi_hip_last_device_ms": (C.c_double, [_P]),
+    "ebi_hip_column_load": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(_P)]),
+    "ebi_hip_column_free": (C.c_int, [_P, _P]),
+    "ebi_hip_column_info": (C.c_int, [_P, C.POINTER(_FileInfo)]),
+    "ebi_hip_column_compressed_bytes": (C.c_uint64, [_P]),
+    "ebi_hip_decode_device": (C.c_int, [_P, _P, _P]),
+    "ebi_hip_decode_per_chunk": (C.c_int, [_P, _P, _P]),
+    "ebi_hip_decode_batch": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "ebi_hip_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "ebi_hip_device_free": (C.c_int, [_P, _P]),
+    "ebi_hip_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "ebi_hip_materialize": (C.c_int, [_P, _P, _P, C.c_int64, _P, C.POINTER(C.c_size_t)]),
+    "ebi_hip_filter": (C.c_int, [_P, _P, C.POINTER(_Pred), _P, C.c_int64, _P]),
+    "ebi_hip_filter_materialize": (C.c_int, [_P, _P, C.POINTER(_Pred), _P, C.c_int64, _P, C.POINTER(C.c_size_t)]),
+    "ebi_hip_sum": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
+    "ebi_hip_min": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
+    "ebi_hip_max": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
+}
+
+
+def lib():
+    """Load libebi_hip.so from this package directory (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise EbiError(HIP, f"{_LIB_PATH} is not built; run `python -m ebi_amd.build`")
+        L = C.CDLL(str(_LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, ctx: Optional["Context"] = None, what: str = ""):
+    if rc != OK:
+        msg = what
+        if ctx is not None and ctx._h:
+            m = lib().ebi_hip_last_error(ctx._h)
+            if m:
+                msg = f"{what}: {m.decode(errors='replace')}"
+        raise EbiError(rc, msg)
+
+
+# ----------------------------------------------------------------- config
+@dataclass(frozen=True)
+class ChunkOption:
+    """encoder::ChunkOption (ebi/src/encoder.rs:26-64)."""
+
+    kind: int
+    value: int = 0
+
+    @staticmethod
+    def full() -> "ChunkOption":
+        return ChunkOption(0, 0)
+
+    @staticmethod
+    def record_count(n: int) -> "ChunkOption":
+        return ChunkOption(1, int(n))
+
+    @staticmethod
+    def byte_size_best_effort(n: int) -> "ChunkOption":
+        return ChunkOption(2, int(n))
+
+
+@dataclass(frozen=True)
+class CompressorConfig:
+    """compressor::CompressorConfig builders (ebi/src/compressor.rs:404-458)."""
+
+    scheme: int
+    scale: int = 1
+    capacity: int = 8192
+
+    @staticmethod
+    def uncompressed(capacity: int = 8192):
+        return CompressorConfig(0, 1, capacity)
+
+    @staticmethod
+    def rle(capacity: int = 8192):
+        return CompressorConfig(1, 1, capacity)
+
+    @staticmethod
+    def buff(scale: int = 1):
+        return CompressorConfig(2, scale, 0)
+
+    @staticmethod
+    def gorilla(capacity: int = 8192):
+        return CompressorConfig(3, 1, capacity)
+
+    @staticmethod
+    def chimp(capacity: int = 8192):
+        return CompressorConfig(4, 1, capacity)
+
+    @staticmethod
+    def chimp128(capacity: int = 8192):
+        return CompressorConfig(5, 1, capacity)
+
+    @staticmethod
+    def elf(capacity: int = 8192):
+        return CompressorConfig(7, 1, capacity)
+
+    @staticmethod
+    def delta_sprintz(scale: int = 1, capacity: int = 8192):
+        return CompressorConfig(8, scale, capacity)
+
+    @staticmethod
+    def by_name(name: str, scale: int = 1) -> "CompressorConfig":
+        return CompressorConfig(SCHEMES[name], scale, 8192)
+
+
+def encode(values, config: CompressorConfig, chunk_option: ChunkOption, threads: int = 0) -> bytes:
+    """Encoder::encode (ebi/src/api/encoder.rs:140-178) on the host; returns the EBI1 file."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    cfg = _EncCfg(config.scheme, chunk_option.kind, chunk_option.value, config.capacity,
+                  config.scale, threads or min(16, os.cpu_count() or 1), 0)
+    out = _P()
+    n = C.c_size_t()
+    L = lib()
+    _check(L.ebi_encode(v.ctypes.data if v.size else None, v.size, C.byref(cfg), C.byref(out), C.byref(n)),
+           None, "encode")
+    try:
+        return C.string_at(out, n.value)
+    finally:
+        L.ebi_buffer_free(out)
+
+
+@dataclass
+class FileInfo:
+    number_of_records: int
+    number_of_chunks: int
+    scheme: int
+    footer_offset: int
+    chunk_option_kind: int
+    chunk_option_value: int
+    capacity: int
+    scale: int
+    version: tuple
+
+
+def file_info(data: bytes) -> FileInfo:
+    fi = _FileInfo()
+    buf = np.frombuffer(data, dtype=np.uint8)
+    _check(lib().ebi_file_parse(buf.ctypes.data, len(data), C.byref(fi)), None, "file_parse")
+    return FileInfo(fi.number_of_records, fi.number_of_chunks, fi.scheme, fi.footer_offset,
+                    fi.chunk_option_kind, fi.chunk_option_value, fi.config_capacity,
+                    fi.config_scale, tuple(fi.version))
+
+
+def file_chunks(data: bytes):
+    """Per-chunk (physical_offset, size, logical_offset, number_of_records) arrays."""
+    n = file_info(data).number_of_chunks
+    arrs = [np.zeros(n, dtype=np.uint64) for _ in range(4)]
+    buf = np.frombuffer(data, dtype=np.uint8)
+    _check(lib().ebi_file_chunks(buf.ctypes.data, len(data), *[a.ctypes.data for a in arrs]), None, "file_chunks")
+    return tuple(arrs)
+
+
+# -------------------------------------------------------------- predicates
+@dataclass(frozen=True)
+class RangeValue:
+    """decoder::query::RangeValue (decoder/query.rs:291-305)."""
+
+    kind: int  # 0 None, 1 Inclusive, 2 Exclusive
+    value: float = 0.0
+
+    @staticmethod
+    def inclusive(v: float) -> "RangeValue":
+        return RangeValue(1, float(v))
+
+    @staticmethod
+    def exclusive(v: float) -> "RangeValue":
+        return RangeValue(2, float(v))
+
+    @staticmethod
+    def none() -> "RangeValue":
+        return RangeValue(0, 0.0)
+
+
+@dataclass(frozen=True)
+class Predicate:
+    """decoder::query::Predicate (decoder/query.rs:396-446)."""
+
+    kind: int
+    value: float = 0.0
+    start: RangeValue = RangeValue(0)
+    end: RangeValue = RangeValue(0)
+
+    @staticmethod
+    def eq(v: float) -> "Predicate":
+        return Predicate(0, float(v))
+
+    @staticmethod
+    def ne(v: float) -> "Predicate":
+        return Predicate(1, float(v))
+
+    @staticmethod
+    def range(start: RangeValue, end: RangeValue) -> "Predicate":
+        return Predicate(2, 0.0, start, end)
+
+    def to_c(self) -> _Pred:
+        return _Pred(self.kind, self.start.kind, self.end.kind, 0, self.value, self.start.value, self.end.value)
+
+    def eval(self, x: np.ndarray) -> np.ndarray:
+        """Predicate::eval on f64 values (decoder/query.rs:425-433)."""
+        x = np.asarray(x, dtype=np.float64)
+        if self.kind == 0:
+            return x == self.value
+        if self.kind == 1:
+            return x != self.value
+        s = np.ones(x.shape, bool)
+        if self.start.kind == 1:
+            s = x >= self.start.value
+        elif self.start.kind == 2:
+            s = x > self.start.value
+        e = np.ones(x.shape, bool)
+        if self.end.kind == 1:
+            e = x <= self.end.value
+        elif self.end.kind == 2:
+            e = x < self.end.value
+        return s & e
+
+
+# ----------------------------------------------------------------- bitmasks
+def ids_to_bitset(ids, n_records: int) -> np.ndarray:
+    """Global record ids -> dense u64 words (bit r % 64 of word r // 64)."""
+    words = np.zeros((n_records + 63) // 64 + 1, dtype=np.uint64)
+    ids = np.asarray(list(ids) if not isinstance(ids, np.ndarray) else ids, dtype=np.int64)
+    ids = ids[(ids >= 0) & (ids < n_records)]
+    if ids.size:
+        np.bitwise_or.at(words, ids >> 6, np.left_shift(np.uint64(1), (ids & 63).astype(np.uint64)))
+    return words
+
+
+def bitset_to_ids(words: np.ndarray, n_records: int) -> np.ndarray:
+    bits = np.unpackbits(words.view(np.uint8), bitorder="little")[:n_records]
+    return np.flatnonzero(bits).astype(np.uint32)
+
+
+# ------------------------------------------------------------------ device
+class Context:
+    """A device context: replaces cuda_binding::Context (cuda-binding/src/lib.rs:32-181)."""
+
+    def __init__(self, device: int = 0):
+        self._h = None
+        h = _P()
+        rc = lib().ebi_hip_ctx_create(int(device), C.byref(h))
+        if rc != OK:
+            raise EbiError(rc, f"ebi_hip_ctx_create(device={device})")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().ebi_hip_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def last_device_ms(self) -> float:
+        return float(lib().ebi_hip_last_device_ms(self._h))
+
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        p = _P()
+        _check(lib().ebi_hip_device_alloc(self._h, int(nbytes), C.byref(p)), self, "device_alloc")
+        return DeviceBuffer(self, p, int(nbytes))
+
+
+class DeviceBuffer:
+    def __init__(self, ctx: Context, ptr, nbytes: int):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def to_host(self, dtype=np.float64, count: Optional[int] = None) -> np.ndarray:
+        it = np.dtype(dtype).itemsize
+        n = self.nbytes // it if count is None else count
+        out = np.empty(n, dtype=dtype)
+        _check(lib().ebi_hip_copy_to_host(self.ctx._h, out.ctypes.data, self.ptr, n * it), self.ctx, "copy_to_host")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().ebi_hip_device_free(self.ctx._h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ctx._h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Decoder:
+    """api::decoder::Decoder (ebi/src/api/decoder.rs:121-753) over a resident column.
+
+    Bitmask arguments are iterables of global record ids (the reference's
+    RoaringBitmap) or dense u64 word arrays; filter results are sorted u32 id
+    arrays.  ``chunk_id`` selects one chunk (ChunkId, api/decoder.rs:755-780).
+    """
+
+    def __init__(self, ctx: Context, data: bytes):
+        self.ctx = ctx
+        self._data = np.frombuffer(data, dtype=np.uint8)
+        h = _P()
+        _check(lib().ebi_hip_column_load(ctx._h, self._data.ctypes.data, len(data), C.byref(h)), ctx, "column_load")
+        self._h = h
+        fi = _FileInfo()
+        _check(lib().ebi_hip_column_info(h, C.byref(fi)), ctx, "column_info")
+        self.number_of_records = int(fi.number_of_records)
+        self.number_of_chunks = int(fi.number_of_chunks)
+        self.scheme = int(fi.scheme)
+
+    def close(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            lib().ebi_hip_column_free(self.ctx._h, self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def compressed_bytes(self) -> int:
+        return int(lib().ebi_hip_column_compressed_bytes(self._h))
+
+    def _mask(self, bitmask):
+        if bitmask is None:
+            return None, None
+        if isinstance(bitmask, np.ndarray) and bitmask.dtype == np.uint64:
+            w = np.ascontiguousarray(bitmask)
+            if w.size < (self.number_of_records + 63) // 64:
+                raise EbiError(INVALID_ARGUMENT, "bitset shorter than ceil(N/64) words")
+        else:
+            w = ids_to_bitset(bitmask, self.number_of_records)
+        return w, w.ctypes.data
+
+    @staticmethod
+    def _cid(chunk_id):
+        return -1 if chunk_id is None else int(chunk_id)
+
+    def materialize(self, bitmask=None, chunk_id=None) -> np.ndarray:
+        _, mp = self._mask(bitmask)
+        out = np.empty(max(self.number_of_records, 1), dtype=np.float64)
+        n = C.c_size_t()
+        _check(lib().ebi_hip_materialize(self.ctx._h, self._h, mp, self._cid(chunk_id), out.ctypes.data, C.byref(n)),
+               self.ctx, "materialize")
+        return out[: n.value].copy()
+
+    def filter_bitset(self, predicate: Predicate, bitmask=None, chunk_id=None) -> np.ndarray:
+        _, mp = self._mask(bitmask)
+        out = np.zeros((self.number_of_records + 63) // 64 + 1, dtype=np.uint64)
+        p = predicate.to_c()
+        _check(lib().ebi_hip_filter(self.ctx._h, self._h, C.byref(p), mp, self._cid(chunk_id), out.ctypes.data),
+               self.ctx, "filter")
+        return out
+
+    def filter(self, predicate: Predicate, bitmask=None, chunk_id=None) -> np.ndarray:
+        return bitset_to_ids(self.filter_bitset(predicate, bitmask, chunk_id), self.number_of_records)
+
+    def filter_materialize(self, predicate: Predicate, bitmask=None, chunk_id=None) -> np.ndarray:
+        _, mp = self._mask(bitmask)
+        out = np.empty(max(self.number_of_records, 1), dtype=np.float64)
+        n = C.c_size_t()
+        p = predicate.to_c()
+        _check(lib().ebi_hip_filter_materialize(self.ctx._h, self._h, C.byref(p), mp, self._cid(chunk_id),
+                                                out.ctypes.data, C.byref(n)), self.ctx, "filter_materialize")
+        return out[: n.value].copy()
+
+    def _agg(self, fn, bitmask, chunk_id) -> float:
+        _, mp = self._mask(bitmask)
+        r = C.c_double()
+        _check(fn(self.ctx._h, self._h, mp, self._cid(chunk_id), C.byref(r)), self.ctx, "aggregate")
+        return float(r.value)
+
+    def sum(self, bitmask=None, chunk_id=None) -> float:
+        return self._agg(lib().ebi_hip_sum, bitmask, chunk_id)
+
+    def min(self, bitmask=None, chunk_id=None) -> float:
+        return self._agg(lib().ebi_hip_min, bitmask, chunk_id)
+
+    def max(self, bitmask=None, chunk_id=None) -> float:
+        return self._agg(lib().ebi_hip_max, bitmask, chunk_id)
+
+    def decode_device(self, out: DeviceBuffer) -> None:
+        """Whole-column decode into a device buffer of N doubles (stays in HBM)."""
+        if out.nbytes < 8 * self.number_of_records:
+            raise EbiError(INVALID_ARGUMENT, "device buffer too small")
+        _check(lib().ebi_hip_decode_device(self.ctx._h, self._h, out.ptr), self.ctx, "decode_device")
+
+    def decode_per_chunk(self, out: DeviceBuffer) -> None:
+        """The plain per-chunk-launch comparator (one launch + host sync per chunk)."""
+        if out.nbytes < 8 * self.number_of_records:
+            raise EbiError(INVALID_ARGUMENT, "device buffer too small")
+        _check(lib().ebi_hip_decode_per_chunk(self.ctx._h, self._h, out.ptr), self.ctx, "decode_per_chunk")
+
+
+def decode_batch(ctx: Context, decoders, outs) -> None:
+    """One persistent launch over every chunk of several resident columns."""
+    n = len(decoders)
+    cols = (_P * n)(*[d._h for d in decoders])
+    ptrs = (_P * n)(*[o.ptr for o in outs])
+    _check(lib().ebi_hip_decode_batch(ctx._h, cols, n, ptrs), ctx, "decode_batch")
+
+
+def gpu_available() -> bool:
+    try:
+        h = _P()
+        rc = lib().ebi_hip_ctx_create(0, C.byref(h))
+        if rc == OK:
+            lib().ebi_hip_ctx_destroy(h)
+            return True
+    except Exception:
+        pass
+    return False

@@ -1,0 +1,108 @@
+// Resident column layout and work descriptors shared by the host loader and
+// the gfx950 kernels of libebi_hip.
+//
+// HBM layout of a resident column (one allocation per column):
+//   * every chunk body starts on a 256-byte boundary and is followed by at
+//     least kPad zero bytes, so wave loads never cross into another chunk
+//     and bit readers may over-read a little past the end;
+//   * BUFF bodies are re-laid out: each byte plane and the LSB tail start on
+//     their own 256-byte boundary (plane_stride = round_up(n, 256)), the
+//     20-byte header lives in the descriptor;
+//   * DeltaSprintz bodies start at the first W-bit field (byte 13 of the
+//     chunk), the 13-byte header lives in the descriptor;
+//   * RLE bodies are split into a u8 run array and an 8-byte aligned f64
+//     value array (structure of arrays);
+//   * Uncompressed, Gorilla, Chimp, Chimp128 and Elf bodies are the chunk
+//     bytes unchanged.
+// Next to the bodies the column keeps a small tile index computed once at
+// load time by the GPU (prefix state every kTile records or fields).
+#pragma once
+
+#include <cstdint>
+
+namespace ebi {
+
+constexpr uint32_t kAlign = 256;
+constexpr uint32_t kPad = 256;
+constexpr uint32_t kTile = 512;  // records (fields for RLE) per wave job: 64 lanes x 8
+constexpr uint32_t kSeg = 64;    // records per checkpoint of the XOR index
+constexpr uint32_t kWaveLanes = 64;  // gfx950 wavefront size
+
+enum Codec : uint32_t {
+  kCodecPlain = 0,
+  kCodecRLE = 1,
+  kCodecBUFF = 2,
+  kCodecGorilla = 3,
+  kCodecChimp = 4,
+  kCodecChimp128 = 5,
+  kCodecElf = 7,
+  kCodecSprintz = 8,
+};
+
+struct alignas(16) DevChunk {
+  uint64_t data;      // absolute device address of the chunk body
+  uint64_t rec_off;   // global id of the first record (ChunkFooter::logical_offset)
+  uint32_t n;         // records in the chunk (footer geometry)
+  uint32_t size;      // bytes of the chunk in the file
+  uint32_t codec;     // Codec
+  uint32_t tile0;     // first tile of this chunk in the column's tile index
+  // BUFF: B (bits), r (tail bits), n_planes, frac, base, plane_stride
+  // Sprintz: W (bits), scale, q0
+  // RLE: fields, value array offset from data
+  uint32_t bits;      // BUFF B / Sprintz W
+  uint32_t r;         // BUFF tail bits
+  uint32_t n_planes;  // BUFF byte planes
+  uint32_t frac;      // BUFF fractional bits
+  int64_t base;       // BUFF base fixed value / Sprintz q0
+  uint32_t scale;     // Sprintz scale (u32 from the header)
+  uint32_t plane_stride;  // BUFF bytes between planes
+  uint64_t fields;    // RLE field count
+  uint64_t values_off;    // RLE: byte offset of the f64 array from data
+  uint64_t xidx;      // absolute address of this chunk's XOR checkpoints (0 = none)
+};
+static_assert(sizeof(DevChunk) == 96, "DevChunk layout");
+
+// XOR-codec checkpoint (every kSeg records): enough state to restart the
+// serial decoder at record k*kSeg.
+struct alignas(16) XorCkpt {
+  uint64_t value;   // bit pattern of the previous record (prefix-XOR state)
+  uint32_t bitpos;  // bit offset of the record's first bit within the chunk
+  uint8_t a, b;     // Gorilla L, T | Chimp stored lead | Elf L, T
+  int8_t beta;      // Elf last beta* (-1 = unset)
+  uint8_t pad;
+};
+static_assert(sizeof(XorCkpt) == 16, "XorCkpt layout");
+
+// One wave's unit of work in a decode launch.
+enum JobKind : uint32_t {
+  kJobTile = 0,    // kTile records of a tile codec (plain, BUFF, Sprintz, RLE)
+  kJobSerial = 1,  // up to 64 XOR chunks, one lane per chunk, serial decode
+  kJobSeg = 2,     // 64 checkpoint segments of one XOR chunk, one lane each
+};
+
+struct alignas(16) Job {
+  uint32_t kind;
+  uint32_t col;    // column slot in the launch (output pointer / table)
+  uint32_t chunk;  // chunk index within the column (first chunk for kJobSerial)
+  uint32_t tile;   // tile / first segment index; lane count for kJobSerial
+};
+
+// Per-launch view of one column.
+struct alignas(16) ColView {
+  const DevChunk *chunks;
+  int64_t *tile_index;   // per-tile prefix state (Sprintz q, RLE start record)
+  double *out;           // decode output base (record r -> out[r])
+  uint32_t *chunk_err;   // per-chunk DevError flags found by the index pass
+  uint64_t n_records;
+  uint64_t n_chunks;
+};
+
+// Device error flags (OR-ed by kernels).
+enum DevError : uint32_t {
+  kErrNone = 0,
+  kErrOverrun = 1,      // a reader went past the chunk's bytes
+  kErrCount = 2,        // terminator or run total disagrees with the record count
+  kErrCorrupt = 4,      // field combination the reference panics on
+};
+
+}  // namespace ebi

@@ -1,0 +1,551 @@
+// C ABI of libebi_hip: context, resident columns and the decode entry points.
+// Query entry points live in query_api.cpp.
+//
+// Boundary citations (lemolatoon/ebi): the context replaces
+// cuda_binding::Context (cuda-binding/src/lib.rs:32-181); column_load is
+// Decoder::new (api/decoder.rs:136-156) plus the upload the reference never
+// does; decode_device is the whole-column materialize (api/decoder.rs:202-237)
+// written to HBM.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "../../../include/ebi_hip.h"
+#include "../kernels/launch.hpp"
+#include "column.hpp"
+#include "internal.hpp"
+
+namespace ebi {
+
+int DevBuf::ensure(size_t bytes) {
+  if (bytes <= cap && p) return 0;
+  if (p) hipFree(p);
+  p = nullptr;
+  cap = 0;
+  size_t want = std::max<size_t>(bytes, 1 << 20);
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return (int)e;
+  }
+  cap = want;
+  return 0;
+}
+void DevBuf::release() {
+  if (p) hipFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+int fail(ebi_hip_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->last_error = msg;
+  return code;
+}
+
+int hip_fail(ebi_hip_ctx *ctx, hipError_t e, const char *what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(ctx, e == hipErrorOutOfMemory ? EBI_OOM : EBI_HIP, m);
+}
+
+int map_dev_err(uint32_t flags) {
+  if (!flags) return EBI_OK;
+  if (flags & kErrCorrupt) return EBI_CONVERSION;
+  return EBI_UNEXPECTED_END;
+}
+
+static uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+static uint64_t be64(const uint8_t *p) {
+  uint64_t v = 0;
+  for (int i = 0; i < 8; ++i) v = (v << 8) | p[i];
+  return v;
+}
+static uint32_t be32(const uint8_t *p) {
+  return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
+}
+
+bool is_xor_codec(int scheme) {
+  return scheme == kGorilla || scheme == kChimp || scheme == kElf;
+}
+
+int upload_view(ebi_hip_ctx *ctx, Column &c, double *out) {
+  ColView v{c.d_chunks, c.d_tile_index, out, c.d_chunk_err, c.meta.number_of_records,
+            (uint64_t)c.chunks.size()};
+  *ctx->h_view = v;
+  hipError_t e = hipMemcpyAsync(c.d_view, ctx->h_view, sizeof(ColView), hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(ColView)");
+  // The pinned staging slot is reused by the next call: wait for the copy.
+  e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+  return EBI_OK;
+}
+
+int sync_and_check(ebi_hip_ctx *ctx, const char *what) {
+  hipError_t e = hipMemcpyAsync(ctx->h_flags, ctx->d_flags, 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, what);
+  uint32_t f = ctx->h_flags[0];
+  if (f) {
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "%s: device reported chunk error flags 0x%x", what, f);
+    return fail(ctx, map_dev_err(f), buf);
+  }
+  return EBI_OK;
+}
+
+int check_chunk_errors(ebi_hip_ctx *ctx, const Column &c, int64_t chunk_id) {
+  for (size_t k = 0; k < c.chunks.size(); ++k) {
+    if (chunk_id >= 0 && (uint64_t)chunk_id != k) continue;
+    if (c.chunks[k].err) {
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "chunk %zu failed validation at load (flags 0x%x)", k, c.chunks[k].err);
+      return fail(ctx, map_dev_err(c.chunks[k].err), buf);
+    }
+  }
+  return EBI_OK;
+}
+
+// Queue the kernels that write every record of the selected chunks of `c`
+// to out (ColView already uploaded).  chunk_id < 0: all chunks.
+int enqueue_decode(ebi_hip_ctx *ctx, Column &c) {
+  launch::tiles(ctx->stream, c.d_view, c.d_tile_jobs, c.n_tile_jobs, ctx->d_flags + 1, ctx->d_flags,
+                (uint32_t)ctx->cu_count * 8);
+  if (c.chimp128) {
+    launch::chimp128(ctx->stream, c.d_view, c.d_serial_jobs, c.n_serial_jobs, ctx->d_flags, true);
+  } else if (c.xor_codec) {
+    if (ctx->xor_mode == 0)
+      launch::segments(ctx->stream, c.d_view, c.d_seg_jobs, c.n_seg_jobs, ctx->d_flags);
+    else
+      launch::serial(ctx->stream, c.d_view, c.d_serial_jobs, c.n_serial_jobs, ctx->d_flags, true, false);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(ctx, e, "decode launch");
+  return EBI_OK;
+}
+
+static void free_column(Column &c) {
+  hipFree(c.d_body);
+  hipFree(c.d_chunks);
+  hipFree(c.d_tile_index);
+  hipFree(c.d_ckpt);
+  hipFree(c.d_chunk_err);
+  hipFree(c.d_view);
+  hipFree(c.d_tile_jobs);
+  hipFree(c.d_serial_jobs);
+  hipFree(c.d_seg_jobs);
+  hipFree(c.d_single_jobs);
+  c = Column();
+}
+
+template <class T>
+static hipError_t upload_vec(T **dst, const std::vector<T> &v) {
+  *dst = nullptr;
+  if (v.empty()) return hipSuccess;
+  hipError_t e = hipMalloc((void **)dst, v.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// Parse, validate codec headers, repack into the device layout, upload and
+// build the tile / checkpoint index.
+static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column &col) {
+  std::string err;
+  int rc = parse_file(file, len, &col.meta, &err);
+  if (rc) return fail(ctx, rc, err);
+  const int scheme = col.meta.scheme;
+  if (!(scheme == kUncompressed || scheme == kRLE || scheme == kBUFF || scheme == kGorilla ||
+        scheme == kChimp || scheme == kChimp128 || scheme == kElf || scheme == kDeltaSprintz))
+    return fail(ctx, EBI_UNSUPPORTED, "compression scheme outside the accelerated set");
+  col.xor_codec = is_xor_codec(scheme);
+  col.chimp128 = scheme == kChimp128;
+  const size_t nc = col.meta.chunks.size();
+  col.chunks.resize(nc);
+  struct Body { const uint8_t *src; uint64_t off, len; };
+  std::vector<Body> bodies(nc);
+  uint64_t dev_bytes = 0, tiles = 0, ckpts = 0;
+  for (size_t k = 0; k < nc; ++k) {
+    const ChunkGeom &g = col.meta.chunks[k];
+    HostChunk &h = col.chunks[k];
+    h.geom = g;
+    const uint8_t *p = file + g.physical_offset;
+    if (g.size >= (1ull << 29) || g.number_of_records >= (1ull << 31))
+      return fail(ctx, EBI_UNSUPPORTED, "chunk larger than 512 MiB or 2^31 records");
+    const uint32_t n = (uint32_t)g.number_of_records;
+    DevChunk d{};
+    d.rec_off = g.logical_offset;
+    d.n = n;
+    d.size = (uint32_t)g.size;
+    d.codec = (uint32_t)scheme;
+    col.compressed_bytes += g.size;
+    uint64_t body_len = g.size;
+    const uint8_t *src = p;
+    switch (scheme) {
+      case kUncompressed:
+        if ((uint64_t)n * 8 > g.size) return fail(ctx, EBI_UNEXPECTED_END, "uncompressed chunk shorter than its records");
+        body_len = (uint64_t)n * 8;
+        h.n_tiles = (n + kTile - 1) / kTile;
+        break;
+      case kRLE: {
+        if (g.size < 8) return fail(ctx, EBI_UNEXPECTED_END, "RLE chunk without field count");
+        const uint64_t f = rd_u64(p);
+        if (f > (g.size - 8) / 9) return fail(ctx, EBI_UNEXPECTED_END, "RLE fields run past the chunk");
+        d.fields = f;
+        d.values_off = round_up(f, kAlign);
+        body_len = d.values_off + 8 * f;
+        h.n_tiles = (uint32_t)((f + kTile - 1) / kTile);
+        break;
+      }
+      case kBUFF: {
+        if (g.size < 20) return fail(ctx, EBI_CONVERSION, "BUFF chunk shorter than its header");
+        const uint64_t lo = rd_u32(p), hi = rd_u32(p + 4);
+        d.base = (int64_t)(lo | (hi << 32));
+        const uint32_t nh = rd_u32(p + 8);
+        d.bits = rd_u32(p + 12);
+        d.frac = rd_u32(p + 16);
+        if (nh != n) return fail(ctx, EBI_CONVERSION, "BUFF header record count differs from the footer");
+        if (d.bits > 64) return fail(ctx, EBI_CONVERSION, "BUFF bit width above 64");
+        if (d.frac > 1074) return fail(ctx, EBI_CONVERSION, "BUFF fractional bits out of range");
+        d.n_planes = d.bits >= 8 ? d.bits / 8 : 0;
+        d.r = d.bits >= 8 ? d.bits % 8 : d.bits;
+        d.plane_stride = (uint32_t)round_up(std::max<uint32_t>(n, 1), kAlign);
+        const uint64_t tail = ((uint64_t)n * d.r + 7) / 8;
+        if (20 + (uint64_t)d.n_planes * n + tail > g.size)
+          return fail(ctx, EBI_CONVERSION, "BUFF planes run past the chunk");
+        body_len = (uint64_t)d.n_planes * d.plane_stride + tail;
+        h.n_tiles = (n + kTile - 1) / kTile;
+        break;
+      }
+      case kDeltaSprintz: {
+        if (g.size < 13) return fail(ctx, EBI_UNEXPECTED_END, "DeltaSprintz chunk shorter than its header");
+        d.base = (int64_t)be64(p);
+        d.scale = be32(p + 8);
+        d.bits = p[12];
+        if (d.bits > 64) return fail(ctx, EBI_CONVERSION, "DeltaSprintz bit width above 64");
+        if (13 + ((uint64_t)n * d.bits + 7) / 8 > g.size)
+          return fail(ctx, EBI_UNEXPECTED_END, "DeltaSprintz fields run past the chunk");
+        src = p + 13;
+        body_len = g.size - 13;
+        h.n_tiles = (n + kTile - 1) / kTile;
+        break;
+      }
+      default:  // XOR codecs: the bit stream as written
+        h.n_segs = col.chimp128 ? 0 : (n + kSeg - 1) / kSeg;
+        break;
+    }
+    bodies[k] = {src, dev_bytes, body_len};
+    d.data = dev_bytes;  // relative until the allocation exists
+    if (scheme == kDeltaSprintz || scheme == kRLE) {
+      d.tile0 = (uint32_t)tiles;
+      tiles += h.n_tiles;
+    }
+    if (h.n_segs) {
+      d.xidx = ckpts;  // relative
+      ckpts += h.n_segs;
+    }
+    h.dev = d;
+    dev_bytes += round_up(std::max<uint64_t>(body_len, 1), kAlign) + kPad;
+  }
+  col.device_bytes = dev_bytes;
+  col.tile_index_entries = tiles;
+  col.ckpt_entries = ckpts;
+
+  hipError_t e;
+  if ((e = hipMalloc((void **)&col.d_body, std::max<uint64_t>(dev_bytes, kAlign))) != hipSuccess)
+    return hip_fail(ctx, e, "hipMalloc(column body)");
+  // Repack into the device layout in a host staging buffer, then one copy.
+  {
+    uint8_t *stage = nullptr;
+    e = hipHostMalloc((void **)&stage, std::max<uint64_t>(dev_bytes, kAlign), hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipHostMalloc(staging)");
+    std::memset(stage, 0, dev_bytes);
+    for (size_t k = 0; k < nc; ++k) {
+      const Body &b = bodies[k];
+      HostChunk &h = col.chunks[k];
+      uint8_t *dst = stage + b.off;
+      const uint8_t *p = file + h.geom.physical_offset;
+      if (scheme == kRLE) {
+        const uint64_t f = h.dev.fields;
+        for (uint64_t i = 0; i < f; ++i) {
+          dst[i] = p[8 + 9 * i];
+          std::memcpy(dst + h.dev.values_off + 8 * i, p + 8 + 9 * i + 1, 8);
+        }
+      } else if (scheme == kBUFF) {
+        const uint32_t n = h.dev.n;
+        for (uint32_t pl = 0; pl < h.dev.n_planes; ++pl)
+          std::memcpy(dst + (size_t)pl * h.dev.plane_stride, p + 20 + (size_t)pl * n, n);
+        const uint64_t tail = ((uint64_t)n * h.dev.r + 7) / 8;
+        std::memcpy(dst + (size_t)h.dev.n_planes * h.dev.plane_stride, p + 20 + (size_t)h.dev.n_planes * n,
+                    tail);
+      } else if (b.len) {
+        std::memcpy(dst, b.src, b.len);
+      }
+    }
+    e = hipMemcpy(col.d_body, stage, dev_bytes, hipMemcpyHostToDevice);
+    hipHostFree(stage);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(column body)");
+  }
+  if (tiles && (e = hipMalloc((void **)&col.d_tile_index, tiles * 8)) != hipSuccess)
+    return hip_fail(ctx, e, "hipMalloc(tile index)");
+  if (ckpts && (e = hipMalloc((void **)&col.d_ckpt, ckpts * sizeof(XorCkpt))) != hipSuccess)
+    return hip_fail(ctx, e, "hipMalloc(checkpoints)");
+  if ((e = hipMalloc((void **)&col.d_chunk_err, std::max<size_t>(nc, 1) * 4)) != hipSuccess)
+    return hip_fail(ctx, e, "hipMalloc(chunk errors)");
+  if ((e = hipMalloc((void **)&col.d_view, sizeof(ColView))) != hipSuccess)
+    return hip_fail(ctx, e, "hipMalloc(view)");
+
+  std::vector<DevChunk> table(nc);
+  for (size_t k = 0; k < nc; ++k) {
+    DevChunk &d = col.chunks[k].dev;
+    d.data = (uint64_t)(uintptr_t)col.d_body + d.data;
+    if (col.chunks[k].n_segs) d.xidx = (uint64_t)(uintptr_t)(col.d_ckpt + d.xidx);
+    table[k] = d;
+  }
+  if ((e = upload_vec(&col.d_chunks, table)) != hipSuccess) return hip_fail(ctx, e, "upload chunk table");
+
+  // Resident job lists.
+  col.tile_job_first.assign(nc + 1, 0);
+  for (size_t k = 0; k < nc; ++k) {
+    col.tile_job_first[k] = (uint32_t)col.h_tile_jobs.size();
+    if (!col.xor_codec && !col.chimp128)
+      for (uint32_t t = 0; t < col.chunks[k].n_tiles; ++t)
+        col.h_tile_jobs.push_back(Job{kJobTile, 0, (uint32_t)k, t});
+    for (uint32_t s = 0; s < col.chunks[k].n_segs; s += kWaveLanes)
+      col.h_seg_jobs.push_back(Job{kJobSeg, 0, (uint32_t)k, s});
+  }
+  col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
+  std::vector<Job> single;
+  if (col.xor_codec || col.chimp128) {
+    for (size_t k = 0; k < nc; k += kWaveLanes)
+      col.h_serial_jobs.push_back(Job{kJobSerial, 0, (uint32_t)k, (uint32_t)std::min<size_t>(kWaveLanes, nc - k)});
+    for (size_t k = 0; k < nc; ++k) single.push_back(Job{kJobSerial, 0, (uint32_t)k, 1});
+  }
+  col.n_tile_jobs = (uint32_t)col.h_tile_jobs.size();
+  col.n_serial_jobs = (uint32_t)col.h_serial_jobs.size();
+  col.n_seg_jobs = (uint32_t)col.h_seg_jobs.size();
+  if ((e = upload_vec(&col.d_tile_jobs, col.h_tile_jobs)) != hipSuccess ||
+      (e = upload_vec(&col.d_serial_jobs, col.h_serial_jobs)) != hipSuccess ||
+      (e = upload_vec(&col.d_seg_jobs, col.h_seg_jobs)) != hipSuccess ||
+      (e = upload_vec(&col.d_single_jobs, single)) != hipSuccess)
+    return hip_fail(ctx, e, "upload job lists");
+
+  // Load-time index pass.
+  launch::zero_u32(ctx->stream, col.d_chunk_err, (uint32_t)nc);
+  launch::zero_u32(ctx->stream, ctx->d_flags, 2);
+  if ((rc = upload_view(ctx, col, nullptr))) return rc;
+  if (scheme == kDeltaSprintz || scheme == kRLE)
+    launch::tile_index(ctx->stream, col.d_view, 0, col.d_tile_jobs, col.n_tile_jobs, (uint32_t)nc, ctx->d_flags);
+  if (col.xor_codec)
+    launch::serial(ctx->stream, col.d_view, col.d_serial_jobs, col.n_serial_jobs, ctx->d_flags, false, true);
+  if (col.chimp128)
+    launch::chimp128(ctx->stream, col.d_view, col.d_serial_jobs, col.n_serial_jobs, ctx->d_flags, false);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "index launch");
+  std::vector<uint32_t> errs(nc);
+  if (nc) {
+    e = hipMemcpyAsync(errs.data(), col.d_chunk_err, nc * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "index pass");
+  }
+  for (size_t k = 0; k < nc; ++k) col.chunks[k].err = errs[k];
+  return EBI_OK;
+}
+
+}  // namespace ebi
+
+using namespace ebi;
+
+#define EBI_GUARD(ctx)                                   \
+  if (!(ctx)) return EBI_INVALID_ARGUMENT;               \
+  std::lock_guard<std::recursive_mutex> _lk((ctx)->mu); \
+  (ctx)->last_error.clear();
+
+extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
+  if (!out) return EBI_INVALID_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return EBI_HIP;
+  if (device < 0 || device >= n) return EBI_INVALID_ARGUMENT;
+  if ((e = hipSetDevice(device)) != hipSuccess) return EBI_HIP;
+  ebi_hip_ctx *c = new (std::nothrow) ebi_hip_ctx();
+  if (!c) return EBI_OOM;
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    c->cu_count = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipMalloc((void **)&c->d_flags, 64 * sizeof(uint32_t)) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_flags, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_view, 64 * sizeof(ColView), hipHostMallocDefault) != hipSuccess) {
+    ebi_hip_ctx_destroy(c);
+    return EBI_HIP;
+  }
+  c->h_view_cap = 64;
+  // Powers of ten for the Elf restore, parsed exactly as the reference does.
+  static double p10[400], n10[400];
+  char buf[32];
+  for (int i = 0; i < 400; ++i) {
+    std::snprintf(buf, sizeof buf, "1.0E%d", i);
+    p10[i] = std::strtod(buf, nullptr);
+    std::snprintf(buf, sizeof buf, "1.0E-%d", i);
+    n10[i] = std::strtod(buf, nullptr);
+  }
+  if (launch::set_pow10(p10, n10) != hipSuccess) {
+    ebi_hip_ctx_destroy(c);
+    return EBI_HIP;
+  }
+  const char *mode = std::getenv("EBI_HIP_XOR_MODE");
+  if (mode && std::strcmp(mode, "serial") == 0) c->xor_mode = 1;
+  *out = c;
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
+  if (!ctx) return EBI_INVALID_ARGUMENT;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  ctx->scratch_values.release();
+  ctx->scratch_bits.release();
+  ctx->scratch_misc.release();
+  ctx->scratch_out.release();
+  ctx->batch_jobs.release();
+  if (ctx->d_views) hipFree(ctx->d_views);
+  if (ctx->d_flags) hipFree(ctx->d_flags);
+  if (ctx->h_flags) hipHostFree(ctx->h_flags);
+  if (ctx->h_view) hipHostFree(ctx->h_view);
+  if (ctx->ev0) hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return EBI_OK;
+}
+
+extern "C" const char *ebi_hip_last_error(const ebi_hip_ctx *ctx) {
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+extern "C" double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx) { return ctx ? ctx->last_ms : 0.0; }
+
+extern "C" int ebi_hip_column_load(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, ebi_hip_col **out) {
+  EBI_GUARD(ctx);
+  if (!file || !out) return EBI_INVALID_ARGUMENT;
+  *out = nullptr;
+  hipSetDevice(ctx->device);
+  ebi_hip_col *c = new (std::nothrow) ebi_hip_col();
+  if (!c) return EBI_OOM;
+  int rc = load_column(ctx, file, len, c->c);
+  if (rc) {
+    hipStreamSynchronize(ctx->stream);
+    free_column(c->c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_column_free(ebi_hip_ctx *ctx, ebi_hip_col *col) {
+  EBI_GUARD(ctx);
+  if (!col) return EBI_INVALID_ARGUMENT;
+  hipStreamSynchronize(ctx->stream);
+  free_column(col->c);
+  delete col;
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_column_info(const ebi_hip_col *col, ebi_file_info *info) {
+  if (!col || !info) return EBI_INVALID_ARGUMENT;
+  const FileMeta &m = col->c.meta;
+  info->number_of_records = m.number_of_records;
+  info->number_of_chunks = m.chunks.size();
+  info->footer_offset = m.footer_offset;
+  info->chunk_option_value = m.chunk_value;
+  info->config_capacity = m.capacity;
+  info->config_scale = m.scale;
+  info->scheme = m.scheme;
+  info->chunk_option_kind = m.chunk_kind;
+  for (int i = 0; i < 3; ++i) info->version[i] = m.version[i];
+  info->reserved = 0;
+  return EBI_OK;
+}
+
+extern "C" uint64_t ebi_hip_column_compressed_bytes(const ebi_hip_col *col) {
+  return col ? col->c.compressed_bytes : 0;
+}
+
+extern "C" int ebi_hip_decode_device(ebi_hip_ctx *ctx, const ebi_hip_col *colc, double *d_out) {
+  EBI_GUARD(ctx);
+  if (!colc || (!d_out && colc->c.meta.number_of_records)) return EBI_INVALID_ARGUMENT;
+  Column &c = const_cast<ebi_hip_col *>(colc)->c;
+  int rc = check_chunk_errors(ctx, c, -1);
+  if (rc) return rc;
+  if ((rc = upload_view(ctx, c, d_out))) return rc;
+  launch::zero_u32(ctx->stream, ctx->d_flags, 2);
+  hipEventRecord(ctx->ev0, ctx->stream);
+  if ((rc = enqueue_decode(ctx, c))) return rc;
+  hipEventRecord(ctx->ev1, ctx->stream);
+  if ((rc = sync_and_check(ctx, "decode"))) return rc;
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  ctx->last_ms = ms;
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *colc, double *d_out) {
+  EBI_GUARD(ctx);
+  if (!colc || (!d_out && colc->c.meta.number_of_records)) return EBI_INVALID_ARGUMENT;
+  Column &c = const_cast<ebi_hip_col *>(colc)->c;
+  int rc = check_chunk_errors(ctx, c, -1);
+  if (rc) return rc;
+  if ((rc = upload_view(ctx, c, d_out))) return rc;
+  launch::zero_u32(ctx->stream, ctx->d_flags, 2);
+  hipEventRecord(ctx->ev0, ctx->stream);
+  // The plain form: one launch per chunk and a host synchronisation after
+  // each (lane-serial for the XOR codecs, the chunk's tiles otherwise).
+  for (size_t k = 0; k < c.chunks.size(); ++k) {
+    if (c.xor_codec || c.chimp128) {
+      if (c.chimp128)
+        launch::chimp128(ctx->stream, c.d_view, c.d_single_jobs + k, 1, ctx->d_flags, true);
+      else
+        launch::serial(ctx->stream, c.d_view, c.d_single_jobs + k, 1, ctx->d_flags, true, false);
+    } else {
+      const uint32_t a = c.tile_job_first[k], b = c.tile_job_first[k + 1];
+      launch::tiles(ctx->stream, c.d_view, c.d_tile_jobs + a, b - a, ctx->d_flags + 1, ctx->d_flags,
+                    (uint32_t)ctx->cu_count * 8);
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "per-chunk decode");
+  }
+  hipEventRecord(ctx->ev1, ctx->stream);
+  if ((rc = sync_and_check(ctx, "per-chunk decode"))) return rc;
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  ctx->last_ms = ms;
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_device_alloc(ebi_hip_ctx *ctx, size_t bytes, void **d_ptr) {
+  EBI_GUARD(ctx);
+  if (!d_ptr) return EBI_INVALID_ARGUMENT;
+  hipError_t e = hipMalloc(d_ptr, std::max<size_t>(bytes, 1));
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_device_free(ebi_hip_ctx *ctx, void *d_ptr) {
+  EBI_GUARD(ctx);
+  hipError_t e = hipFree(d_ptr);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipFree");
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_copy_to_host(ebi_hip_ctx *ctx, void *h_dst, const void *d_src, size_t bytes) {
+  EBI_GUARD(ctx);
+  if (!bytes) return EBI_OK;
+  hipError_t e = hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "copy to host");
+  return EBI_OK;
+}

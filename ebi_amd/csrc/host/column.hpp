@@ -1,0 +1,88 @@
+// Host-side state of libebi_hip: the device context and resident columns.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../device.hpp"
+#include "format.hpp"
+
+struct ebi_hip_ctx;
+
+namespace ebi {
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes);  // 0 or hip error
+  void release();
+};
+
+// Per-chunk host copy of the parsed codec header (also uploaded in DevChunk).
+struct HostChunk {
+  ChunkGeom geom;
+  DevChunk dev;       // with absolute device addresses filled in
+  uint32_t err = 0;   // DevError flags found at load
+  uint32_t n_tiles = 0;
+  uint32_t n_segs = 0;
+};
+
+struct Column {
+  FileMeta meta;
+  std::vector<HostChunk> chunks;
+  uint64_t compressed_bytes = 0;  // sum of chunk sizes in the file
+  uint64_t device_bytes = 0;      // bytes of the device body allocation
+  // device allocations
+  uint8_t *d_body = nullptr;
+  DevChunk *d_chunks = nullptr;
+  int64_t *d_tile_index = nullptr;
+  XorCkpt *d_ckpt = nullptr;
+  uint32_t *d_chunk_err = nullptr;
+  ColView *d_view = nullptr;  // slot used by single-column launches
+  // resident job lists
+  Job *d_tile_jobs = nullptr;
+  uint32_t n_tile_jobs = 0;
+  Job *d_serial_jobs = nullptr;  // Gorilla/Chimp/Elf/Chimp128, lane per chunk
+  uint32_t n_serial_jobs = 0;
+  Job *d_single_jobs = nullptr;  // one single-lane serial job per XOR chunk (comparator)
+  Job *d_seg_jobs = nullptr;     // Gorilla/Chimp/Elf checkpoint segments
+  uint32_t n_seg_jobs = 0;
+  std::vector<Job> h_tile_jobs, h_serial_jobs, h_seg_jobs;
+  std::vector<uint32_t> tile_job_first;  // per chunk: first tile job index (comparator)
+  uint64_t tile_index_entries = 0, ckpt_entries = 0;
+  bool xor_codec = false;
+  bool chimp128 = false;
+};
+
+}  // namespace ebi
+
+struct ebi_hip_col {
+  ebi::Column c;
+};
+
+struct ebi_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  std::string last_error;
+  std::recursive_mutex mu;
+  uint32_t *d_flags = nullptr;  // [0] error flags, [1] job counter, [2..] scratch counters
+  uint32_t *h_flags = nullptr;  // pinned mirror
+  ebi::ColView *h_view = nullptr;  // pinned staging for ColView uploads
+  size_t h_view_cap = 0;
+  ebi::ColView *d_views = nullptr;  // device ColView array for batches
+  size_t d_views_cap = 0;
+  ebi::DevBuf scratch_values;   // decoded f64 scratch for queries
+  ebi::DevBuf scratch_bits;     // bitsets
+  ebi::DevBuf scratch_misc;     // partials, scans
+  ebi::DevBuf scratch_out;      // compacted outputs
+  ebi::DevBuf batch_jobs;
+  int cu_count = 256;
+  int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks
+};

@@ -1,0 +1,123 @@
+// EBI1 on-disk layout helpers shared by the host side of libebi_hip.
+//
+// Byte layout (all little-endian, packed), following ebi/src/format.rs:143-258:
+//   FileHeader (30 B): magic "EBI1" | field_type u8 | scheme u8 |
+//                      chunk_option.kind u8 | reserved u8 | chunk_option.value u64 |
+//                      footer_offset u64 | version u16 x3
+//   chunks, contiguous from offset 30 (GeneralChunkHeader is zero-sized)
+//   footer @footer_offset: number_of_records u64 | number_of_chunks u64 |
+//                          C x {physical_offset u64, logical_offset u64} |
+//                          config_size u8 | config bytes |
+//                          FileFooter3 (u128 + 9 x u128 + u32 crc = 164 B)
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace ebi {
+
+constexpr size_t kFileHeaderSize = 30;
+constexpr size_t kFooter0Size = 16;
+constexpr size_t kChunkFooterSize = 16;
+constexpr size_t kFooter3Size = 16 + 9 * 16 + 4;
+constexpr uint64_t kDefaultCapacity = 8192;  // compressor.rs:465
+constexpr uint16_t kVersion[3] = {0, 1, 14};  // ebi/Cargo.toml:3
+constexpr uint64_t kNanBits = 0x7ff8000000000000ull;  // f64::NAN.to_bits()
+
+enum Scheme : int {
+  kUncompressed = 0,
+  kRLE = 1,
+  kBUFF = 2,
+  kGorilla = 3,
+  kChimp = 4,
+  kChimp128 = 5,
+  kElfOnChimp = 6,
+  kElf = 7,
+  kDeltaSprintz = 8,
+  kSchemeCount = 13,  // ids 9..12 are zstd, gzip, snappy, ALP
+};
+
+inline uint64_t rd_u64(const uint8_t *p) {
+  uint64_t v;
+  std::memcpy(&v, p, 8);
+  return v;
+}
+inline uint32_t rd_u32(const uint8_t *p) {
+  uint32_t v;
+  std::memcpy(&v, p, 4);
+  return v;
+}
+inline uint16_t rd_u16(const uint8_t *p) {
+  uint16_t v;
+  std::memcpy(&v, p, 2);
+  return v;
+}
+inline void put_u64(std::vector<uint8_t> &b, uint64_t v) {
+  uint8_t t[8];
+  std::memcpy(t, &v, 8);
+  b.insert(b.end(), t, t + 8);
+}
+inline void put_u32(std::vector<uint8_t> &b, uint32_t v) {
+  uint8_t t[4];
+  std::memcpy(t, &v, 4);
+  b.insert(b.end(), t, t + 4);
+}
+inline void put_u16(std::vector<uint8_t> &b, uint16_t v) {
+  uint8_t t[2];
+  std::memcpy(t, &v, 2);
+  b.insert(b.end(), t, t + 2);
+}
+inline uint64_t f64_bits(double d) {
+  uint64_t u;
+  std::memcpy(&u, &d, 8);
+  return u;
+}
+inline double bits_f64(uint64_t u) {
+  double d;
+  std::memcpy(&d, &u, 8);
+  return d;
+}
+
+// Serialized config blob size (without the leading size byte) per scheme,
+// compressor.rs:328-365: capacity u64 for the XOR/RLE/plain codecs, scale u64
+// for BUFF, capacity+scale for DeltaSprintz, i32/u32 for zstd/gzip, 0 else.
+inline int config_size_of(int scheme) {
+  switch (scheme) {
+    case kUncompressed: case kRLE: case kGorilla: case kChimp: case kChimp128:
+    case kElfOnChimp: case kElf: case kBUFF:
+      return 8;
+    case kDeltaSprintz:
+      return 16;
+    case 9: case 10:
+      return 4;
+    default:
+      return 0;
+  }
+}
+
+struct ChunkGeom {
+  uint64_t physical_offset;
+  uint64_t size;            // next physical offset (or footer_offset) - physical_offset
+  uint64_t logical_offset;
+  uint64_t number_of_records;
+};
+
+struct FileMeta {
+  int scheme = 0;
+  int chunk_kind = 0;
+  uint64_t chunk_value = 0;
+  uint64_t footer_offset = 0;
+  uint16_t version[3] = {0, 0, 0};
+  uint64_t number_of_records = 0;
+  uint64_t capacity = 0;
+  uint64_t scale = 0;
+  std::vector<ChunkGeom> chunks;
+};
+
+// Returns 0 on success or an ebi_status code; *err receives a message.
+int parse_file(const uint8_t *file, size_t len, FileMeta *meta, std::string *err);
+
+}  // namespace ebi

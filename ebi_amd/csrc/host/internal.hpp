@@ -1,0 +1,25 @@
+// Helpers shared by the host translation units of libebi_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "column.hpp"
+
+namespace ebi {
+
+int fail(ebi_hip_ctx *ctx, int code, const std::string &msg);
+int hip_fail(ebi_hip_ctx *ctx, hipError_t e, const char *what);
+int map_dev_err(uint32_t flags);
+bool is_xor_codec(int scheme);
+// Uploads the single-column ColView of `c` with output base `out` and waits.
+int upload_view(ebi_hip_ctx *ctx, Column &c, double *out);
+// Copies the device error word back (synchronising the stream) and maps it.
+int sync_and_check(ebi_hip_ctx *ctx, const char *what);
+// Fails if a selected chunk was rejected by the load-time index pass.
+int check_chunk_errors(ebi_hip_ctx *ctx, const Column &c, int64_t chunk_id);
+// Queues the kernels that decode every chunk of `c` into the view's output.
+int enqueue_decode(ebi_hip_ctx *ctx, Column &c);
+
+}  // namespace ebi

@@ -1,0 +1,473 @@
+// Per-codec device decoders for libebi_hip (gfx950).
+//
+// Two shapes of work:
+//  * tile codecs (Uncompressed, BUFF, DeltaSprintz, RLE): one wave decodes a
+//    tile of kTile = 512 records (fields for RLE), 8 consecutive records per
+//    lane, with wave scans for the prefix-dependent codecs; the prefix state
+//    at each tile start comes from the column's tile index;
+//  * XOR codecs (Gorilla, Chimp, Chimp128, Elf): the bit length of a record
+//    depends on state set by earlier records, so one lane walks one chunk
+//    (or one kSeg-record segment, restarting from a checkpoint written by the
+//    load-time index pass) serially.
+// All functions restate the reference decoders cited beside them
+// (paths relative to lemolatoon/ebi).
+#pragma once
+
+#include "common.hpp"
+
+namespace ebi {
+
+// Powers of ten as the reference obtains them (compression_common/elf.rs:95-109):
+// correctly rounded parses of "1.0E{i}" / "1.0E-{i}"; filled by the host.
+__constant__ double c_p10[400];
+__constant__ double c_n10[400];
+
+__device__ __forceinline__ int32_t sat_i32(double d) {
+  if (d != d) return 0;
+  if (d >= 2147483647.0) return INT32_MAX;
+  if (d <= -2147483648.0) return INT32_MIN;
+  return (int32_t)d;
+}
+
+// get_sp, compression_common/elf.rs:111-130: table walk for [1e-10, 1e9),
+// floor(log10) otherwise.
+__device__ __forceinline__ int32_t elf_get_sp(double v) {
+  if (v >= 1.0) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      if (v < c_p10[i + 1]) return i;
+  } else {
+#pragma unroll
+    for (int i = 1; i < 11; ++i)
+      if (v >= c_n10[i]) return -i;
+  }
+  return sat_i32(floor(log10(v)));
+}
+
+// round_up, compression_common/elf.rs:51-58 (no FMA contraction).
+__device__ __forceinline__ double elf_round_up(double v, uint32_t alpha) {
+  const double scale = alpha < 400 ? c_p10[alpha] : __longlong_as_double(0x7ff0000000000000ll);
+  const double t = v * scale;
+  return (v < 0.0 ? floor(t) : ceil(t)) / scale;
+}
+
+// ------------------------------------------------------------------ Gorilla
+// decoder/chunk_reader/gorilla.rs:77-157.  Decodes records [i0, i1) starting
+// at bit `bitpos` with state (v = record i0-1, L, T); when i1 == n also
+// checks that the next record is the NaN terminator.
+template <bool kWrite>
+__device__ uint32_t gorilla_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
+                                uint32_t i1, uint32_t bitpos, uint64_t v, uint32_t L, uint32_t T,
+                                double *out, XorCkpt *ck) {
+  LaneReader r;
+  r.init(data, bitpos);
+  const uint32_t last = i1 == n ? n + 1 : i1;
+  for (uint32_t i = i0; i < last; ++i) {
+    if (ck && i < n && (i % kSeg) == 0) ck[i / kSeg] = XorCkpt{v, r.pos, (uint8_t)L, (uint8_t)T, -1, 0};
+    if (i == 0) {
+      v = r.read64(64);
+    } else {
+      const uint32_t c2 = r.peek(2);
+      if (!(c2 & 2)) {
+        r.skip(1);
+      } else {
+        r.skip(2);
+        if (c2 & 1) {
+          const uint32_t h = r.read(11);
+          const uint32_t l = h >> 6, s = (h & 63) + 1;
+          if (l + s > 64) return kErrCorrupt;
+          L = l;
+          T = 64 - l - s;
+        }
+        v ^= r.read64(64 - L - T) << T;
+      }
+    }
+    if (r.pos > size_bits) return kErrOverrun;
+    if (i == n) return v == kNan ? kErrNone : kErrCount;
+    if (v == kNan) return kErrCount;
+    if (kWrite) out[i] = as_f64(v);
+  }
+  return kErrNone;
+}
+
+// -------------------------------------------------------------------- Chimp
+// decoder/chunk_reader/chimp.rs:59-125.  State: v, stored lead (0xffffffff
+// before the first 11/01 record).
+template <bool kWrite>
+__device__ uint32_t chimp_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
+                              uint32_t i1, uint32_t bitpos, uint64_t v, uint32_t lead, double *out,
+                              XorCkpt *ck) {
+  LaneReader r;
+  r.init(data, bitpos);
+  const uint32_t last = i1 == n ? n + 1 : i1;
+  for (uint32_t i = i0; i < last; ++i) {
+    if (ck && i < n && (i % kSeg) == 0)
+      ck[i / kSeg] = XorCkpt{v, r.pos, (uint8_t)(lead > 64 ? 255 : lead), 0, -1, 0};
+    if (i == 0) {
+      v = r.read64(64);
+    } else {
+      const uint32_t f = r.read(2);
+      if (f == 3) {
+        lead = lead_repr(r.read(3));
+        v ^= r.read64(64 - lead);
+      } else if (f == 2) {
+        if (lead > 64) return kErrCorrupt;
+        v ^= r.read64(64 - lead);
+      } else if (f == 1) {
+        const uint32_t h = r.read(9);
+        lead = lead_repr(h >> 6);
+        uint32_t sig = h & 63;
+        if (!sig) sig = 64;
+        if (sig + lead > 64) return kErrCorrupt;
+        v ^= r.read64(sig) << (64 - sig - lead);
+      }
+    }
+    if (r.pos > size_bits) return kErrOverrun;
+    if (i == n) return v == kNan ? kErrNone : kErrCount;
+    if (v == kNan) return kErrCount;
+    if (kWrite) out[i] = as_f64(v);
+  }
+  return kErrNone;
+}
+
+// ---------------------------------------------------------------------- Elf
+// ElfDecoderWrapper + ElfXorDecoder, decoder/chunk_reader/elf.rs:46-258.
+// State: stored v' bits, L, T (0xffffffff before the first 10/11 record),
+// last beta* (INT32_MAX before the first 11 prefix).
+template <bool kWrite>
+__device__ uint32_t elf_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
+                            uint32_t i1, uint32_t bitpos, uint64_t stored, uint32_t L, uint32_t T,
+                            int32_t beta, double *out, XorCkpt *ck) {
+  LaneReader r;
+  r.init(data, bitpos);
+  const uint32_t last = i1 == n ? n + 1 : i1;
+  for (uint32_t i = i0; i < last; ++i) {
+    if (ck && i < n && (i % kSeg) == 0)
+      ck[i / kSeg] = XorCkpt{stored, r.pos, (uint8_t)(L > 64 ? 255 : L), (uint8_t)(T > 64 ? 255 : T),
+                             (int8_t)(beta == INT32_MAX ? -1 : beta), 0};
+    // prefix: 0 -> erased, reuse beta*; 10 -> not erased; 11 + 4 bits -> erased, new beta*
+    const uint32_t p2 = r.peek(2);
+    bool recover;
+    if (!(p2 & 2)) {
+      r.skip(1);
+      recover = true;
+    } else if (p2 == 2) {
+      r.skip(2);
+      recover = false;
+    } else {
+      r.skip(2);
+      beta = (int32_t)r.read(4);
+      recover = true;
+    }
+    bool term = false;
+    if (i == 0) {
+      const uint32_t tz = r.read(7);
+      if (tz < 64) {
+        const uint64_t m = r.read64(63 - tz);
+        stored = ((m << 1) | 1) << tz;
+      } else {
+        stored = 0;
+      }
+      term = stored == kNan;
+    } else {
+      const uint32_t f = r.read(2);
+      if (f == 3) {
+        const uint32_t h = r.read(9);
+        L = lead_repr(h >> 6);
+        uint32_t center = h & 63;
+        if (!center) center = 64;
+        if (L + center > 64) return kErrCorrupt;
+        T = 64 - L - center;
+        const uint64_t x = ((r.read64(center - 1) << 1) | 1) << T;
+        const uint64_t val = x ^ stored;
+        if (val == kNan) term = true; else stored = val;
+      } else if (f == 2) {
+        const uint32_t h = r.read(7);
+        L = lead_repr(h >> 4);
+        uint32_t center = h & 15;
+        if (!center) center = 16;
+        if (L + center > 64) return kErrCorrupt;
+        T = 64 - L - center;
+        const uint64_t x = center == 1 ? (1ull << T) : ((r.read64(center - 1) << 1) | 1) << T;
+        const uint64_t val = x ^ stored;
+        if (val == kNan) term = true; else stored = val;
+      } else if (f == 0) {
+        if ((uint64_t)L + T > 64) return kErrCorrupt;
+        const uint32_t center = 64 - L - T;
+        const uint64_t val = (r.read64(center) << T) ^ stored;
+        if (val == kNan) term = true; else stored = val;
+      }
+    }
+    if (r.pos > size_bits) return kErrOverrun;
+    if (i == n) return term ? kErrNone : kErrCount;
+    if (term) return kErrCount;
+    if (kWrite) {
+      double v = as_f64(stored);
+      if (recover) {  // recover_v_by_beta_star, elf.rs:66-81
+        const int32_t sp = elf_get_sp(fabs(v));
+        if (beta == 0) {
+          const uint32_t k = 0u - (uint32_t)sp - 1u;  // (-sp - 1) as u32, wrapping
+          const double w = k < 400 ? c_n10[k] : 0.0;
+          v = v < 0.0 ? -w : w;
+        } else {
+          v = elf_round_up(v, (uint32_t)beta - (uint32_t)sp - 1u);
+        }
+      }
+      out[i] = v;
+    }
+  }
+  return kErrNone;
+}
+
+// ----------------------------------------------------------------- Chimp128
+// decoder/chunk_reader/chimp_n.rs:173-285.  `ring` is this lane's 128-slot
+// column in LDS, stride kWave (slot s of lane l at ring[s * 64]).
+template <bool kWrite>
+__device__ uint32_t chimp128_run(const uint8_t *data, uint32_t size_bits, uint32_t n, double *out,
+                                 uint64_t *ring) {
+  LaneReader r;
+  r.init(data, 0);
+  for (int s = 0; s < 128; ++s) ring[s * kWave] = 0;
+  uint32_t cur = 0, lead = 0xffffffffu;
+  uint64_t v = 0;
+  for (uint32_t i = 0; i <= n; ++i) {
+    bool term = false;
+    if (i == 0) {
+      v = r.read64(64);
+      ring[0] = v;
+      term = v == kNan;
+    } else {
+      const uint32_t f = r.read(2);
+      if (f == 0) {
+        v = ring[r.read(7) * kWave];
+        cur = (cur + 1) & 127;
+        ring[cur * kWave] = v;
+      } else {
+        if (f == 3) {
+          lead = lead_repr(r.read(3));
+          v ^= r.read64(64 - lead);
+        } else if (f == 2) {
+          if (lead > 64) return kErrCorrupt;
+          v ^= r.read64(64 - lead);
+        } else {
+          const uint32_t h = r.read(16);
+          const uint32_t idx = (h >> 9) & 127;
+          lead = lead_repr((h >> 6) & 7);
+          uint32_t sig = h & 63;
+          if (!sig) sig = 64;
+          if (sig + lead > 64) return kErrCorrupt;
+          v = ring[idx * kWave] ^ (r.read64(sig) << (64 - sig - lead));
+        }
+        if (v == kNan) {
+          term = true;
+        } else {
+          cur = (cur + 1) & 127;
+          ring[cur * kWave] = v;
+        }
+      }
+    }
+    if (r.pos > size_bits) return kErrOverrun;
+    if (i == n) return term ? kErrNone : kErrCount;
+    if (term) return kErrCount;
+    if (kWrite) out[i] = as_f64(v);
+  }
+  return kErrNone;
+}
+
+// -------------------------------------------------------------- tile codecs
+// Records of tile t handled by this lane: [t*kTile + lane*8, +8) clipped to n.
+
+// Uncompressed, chunk_reader/uncompressed.rs:56-71: a copy; lanes move
+// 16-byte pairs so each wave instruction covers 1 KiB contiguous.
+__device__ __forceinline__ void plain_tile(const DevChunk &c, uint32_t t, double *out) {
+  const double *src = reinterpret_cast<const double *>(c.data);
+  const uint32_t lane = lane_id();
+  const uint32_t b = t * kTile;
+  const bool pair_ok = ((c.rec_off + (uint64_t)reinterpret_cast<uintptr_t>(out) / 8) & 1) == 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = b + k * 128 + lane * 2;
+    if (pair_ok && i + 1 < c.n) {
+      const double2 x = *reinterpret_cast<const double2 *>(src + i);
+      *reinterpret_cast<double2 *>(out + c.rec_off + i) = x;
+    } else {
+      if (i < c.n) out[c.rec_off + i] = src[i];
+      if (i + 1 < c.n) out[c.rec_off + i + 1] = src[i + 1];
+    }
+  }
+}
+
+// BUFF deltas of this lane's 8 records: planes MSB-first (each byte XOR 0x80),
+// then the LSB-first tail of r bits (compression_common/buff.rs:85-287,
+// chunk_reader/buff/decode.rs:5-70).  Lanes whose records lie past n get
+// garbage-free zeros.
+__device__ __forceinline__ void buff_deltas(const DevChunk &c, uint32_t i0, uint64_t d[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) d[j] = 0;
+  if (i0 >= c.n) return;
+  const uint8_t *base = reinterpret_cast<const uint8_t *>(c.data);
+  for (uint32_t p = 0; p < c.n_planes; ++p) {
+    const uint64_t w = *reinterpret_cast<const uint64_t *>(base + (size_t)p * c.plane_stride + i0);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) d[j] = (d[j] << 8) | (((w >> (8 * j)) & 0xff) ^ 0x80);
+  }
+  if (c.r) {
+    // 8 records x r bits = r bytes starting at byte i0*r/8 (byte aligned).
+    const uint8_t *tail = base + (size_t)c.n_planes * c.plane_stride;
+    const uint32_t byte0 = (i0 / 8) * c.r;
+    const uint32_t a = byte0 & ~7u, sh = (byte0 & 7u) * 8;
+    const uint64_t lo = *reinterpret_cast<const uint64_t *>(tail + a);
+    const uint64_t hi = *reinterpret_cast<const uint64_t *>(tail + a + 8);
+    const uint64_t bits = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+    const uint64_t m = (1ull << c.r) - 1;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) d[j] = (d[j] << c.r) | ((bits >> (j * c.r)) & m);
+  }
+}
+
+__device__ __forceinline__ void store8(double *dst, const double v[8], uint32_t cnt, bool aligned) {
+  if (aligned && cnt == 8) {
+    double2 *p = reinterpret_cast<double2 *>(dst);
+    p[0] = make_double2(v[0], v[1]);
+    p[1] = make_double2(v[2], v[3]);
+    p[2] = make_double2(v[4], v[5]);
+    p[3] = make_double2(v[6], v[7]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if ((uint32_t)j < cnt) dst[j] = v[j];
+  }
+}
+
+// (base + delta) as f64 / 2^frac; ldexp is exact like the reference's divide.
+__device__ __forceinline__ double buff_value(const DevChunk &c, uint64_t d) {
+  return ldexp((double)(int64_t)((uint64_t)c.base + d), -(int)c.frac);
+}
+
+__device__ __forceinline__ void buff_tile(const DevChunk &c, uint32_t t, double *out) {
+  const uint32_t i0 = t * kTile + lane_id() * 8;
+  if (i0 >= c.n) return;
+  uint64_t d[8];
+  buff_deltas(c, i0, d);
+  double v[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = buff_value(c, d[j]);
+  const uint32_t cnt = min(8u, c.n - i0);
+  double *dst = out + c.rec_off + i0;
+  store8(dst, v, cnt, ((reinterpret_cast<uintptr_t>(dst)) & 15) == 0);
+}
+
+// DeltaSprintz zigzag deltas of this lane's 8 records (MSB-first W-bit
+// fields, chunk_reader/sprintz.rs:480-504), unzigzagged; 0 past n.
+__device__ __forceinline__ void sprintz_deltas(const DevChunk &c, uint32_t i0, int64_t d[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) d[j] = 0;
+  if (i0 >= c.n || c.bits == 0) return;
+  LaneReader r;
+  r.init(reinterpret_cast<const uint8_t *>(c.data), i0 * c.bits);
+  const uint32_t cnt = min(8u, c.n - i0);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if ((uint32_t)j < cnt) {
+      const uint64_t z = r.read64(c.bits);
+      d[j] = (int64_t)(z >> 1) ^ -(int64_t)(z & 1);
+    }
+  }
+}
+
+// Inclusive quantized values q_i of this lane's records given the tile's
+// starting prefix q_{t*kTile - 1} (or q0 - 0 for tile 0; the first delta is 0).
+__device__ __forceinline__ void sprintz_q(const DevChunk &c, uint32_t t, int64_t tile_base, int64_t q[8]) {
+  const uint32_t i0 = t * kTile + lane_id() * 8;
+  int64_t d[8];
+  sprintz_deltas(c, i0, d);
+  int64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    s = (int64_t)((uint64_t)s + (uint64_t)d[j]);
+    q[j] = s;
+  }
+  const int64_t incl = wave_incl_scan_i64(s);
+  const int64_t excl = (int64_t)((uint64_t)incl - (uint64_t)s) + tile_base;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) q[j] = (int64_t)((uint64_t)q[j] + (uint64_t)excl);
+}
+
+__device__ __forceinline__ void sprintz_tile(const DevChunk &c, uint32_t t, int64_t tile_base, double *out) {
+  int64_t q[8];
+  sprintz_q(c, t, tile_base, q);
+  const uint32_t i0 = t * kTile + lane_id() * 8;
+  if (i0 >= c.n) return;
+  double v[8];
+  const double sc = (double)c.scale;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = (double)q[j] / sc;  // quantized as f64 / scale as f64
+  const uint32_t cnt = min(8u, c.n - i0);
+  double *dst = out + c.rec_off + i0;
+  store8(dst, v, cnt, ((reinterpret_cast<uintptr_t>(dst)) & 15) == 0);
+}
+
+__device__ __forceinline__ int64_t sprintz_tile_sum(const DevChunk &c, uint32_t t) {
+  const uint32_t i0 = t * kTile + lane_id() * 8;
+  int64_t d[8];
+  sprintz_deltas(c, i0, d);
+  int64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s = (int64_t)((uint64_t)s + (uint64_t)d[j]);
+  return wave_sum_i64(s);
+}
+
+// RLE (chunk_reader/run_length.rs:68-91): runs[] u8 and values[] f64 (SoA).
+// Tile t covers fields [t*kTile, +kTile); its first output record is the
+// tile index entry.  Each lane owns 8 fields; the wave scans run lengths,
+// publishes field starts to LDS and expands by binary search so that output
+// stores are contiguous across lanes.
+__device__ __forceinline__ uint32_t rle_runs(const DevChunk &c, uint32_t f0, uint32_t run[8]) {
+  const uint8_t *runs = reinterpret_cast<const uint8_t *>(c.data);
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    run[j] = (f0 + j < c.fields) ? runs[f0 + j] : 0u;
+    s += run[j];
+  }
+  return s;
+}
+
+__device__ __forceinline__ uint64_t rle_tile_sum(const DevChunk &c, uint32_t t) {
+  uint32_t run[8];
+  const uint32_t s = rle_runs(c, t * kTile + lane_id() * 8, run);
+  return wave_sum_u64(s);
+}
+
+// starts: this wave's LDS scratch of kTile + 1 entries.
+__device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, int64_t tile_start, double *out,
+                                         uint32_t *starts) {
+  const uint32_t lane = lane_id();
+  const uint32_t f0 = t * kTile + lane * 8;
+  uint32_t run[8];
+  const uint32_t s = rle_runs(c, f0, run);
+  const uint32_t incl = wave_incl_scan_u32(s);
+  uint32_t pos = incl - s;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    starts[lane * 8 + j] = pos;
+    pos += run[j];
+  }
+  const uint32_t total = __shfl(incl, kWave - 1, kWave);
+  if (lane == 0) starts[kTile] = total;
+  wave_sync();
+  const uint32_t nf = (uint32_t)min<uint64_t>(kTile, c.fields > t * (uint64_t)kTile ? c.fields - t * (uint64_t)kTile : 0);
+  const double *vals = reinterpret_cast<const double *>(c.data + c.values_off);
+  double *dst = out + c.rec_off + tile_start;
+  for (uint32_t o = lane; o < total; o += kWave) {
+    // largest field f in [0, nf) with starts[f] <= o
+    uint32_t lo = 0, hi = nf;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (starts[mid] <= o) lo = mid; else hi = mid;
+    }
+    dst[o] = vals[t * kTile + lo];
+  }
+}
+
+}  // namespace ebi

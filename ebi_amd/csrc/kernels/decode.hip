@@ -1,0 +1,236 @@
+// Decode kernels of libebi_hip (gfx950) and their host launchers.
+//
+// Launch shapes:
+//  * k_tiles    -- 256 threads = 4 independent waves; each wave takes wave
+//                  jobs (one 512-record tile) from a device job list, first
+//                  statically then from an atomic counter, so one launch
+//                  covers every tile of every chunk and column in a batch;
+//  * k_serial   -- 64 threads; lane l walks chunk (job.chunk + l) serially
+//                  (Gorilla, Chimp, Elf); also the load-time index pass that
+//                  writes XOR checkpoints;
+//  * k_chimp128 -- 64 threads with a 64 KiB LDS ring (128 slots per lane);
+//  * k_segments -- 256 threads; lane l of a wave restarts from checkpoint
+//                  (job.tile + l) and decodes its kSeg records;
+//  * k_tile_sums / k_tile_scan -- the load-time tile index of DeltaSprintz
+//                  (prefix q per tile) and RLE (first output record per tile).
+#include "codecs.hpp"
+#include "launch.hpp"
+
+namespace ebi {
+
+__device__ __forceinline__ uint32_t next_job(uint32_t *ctr, uint32_t stride) {
+  uint32_t j = 0;
+  if (lane_id() == 0) j = atomicAdd(ctr, 1u) + stride;
+  return __builtin_amdgcn_readfirstlane(__shfl(j, 0, kWave));
+}
+
+__global__ void __launch_bounds__(256) k_tiles(const ColView *cols, const Job *jobs, uint32_t n_jobs,
+                                              uint32_t *ctr, uint32_t *err) {
+  __shared__ uint32_t lds_starts[4][kTile + 1];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t stride = gridDim.x * 4;
+  uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
+  while (j < n_jobs) {
+    const Job jb = jobs[j];
+    const ColView cv = cols[jb.col];
+    const DevChunk c = cv.chunks[jb.chunk];
+    if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
+      switch (c.codec) {
+        case kCodecPlain: plain_tile(c, jb.tile, cv.out); break;
+        case kCodecBUFF: buff_tile(c, jb.tile, cv.out); break;
+        case kCodecSprintz: sprintz_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out); break;
+        case kCodecRLE:
+          rle_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out, lds_starts[wave]);
+          break;
+        default:
+          if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
+          break;
+      }
+    }
+    j = next_job(ctr, stride);
+  }
+}
+
+// kWrite: decode to cv.out; kIndex: write checkpoints to c.xidx.
+template <bool kWrite, bool kIndex>
+__global__ void __launch_bounds__(64) k_serial(const ColView *cols, const Job *jobs, uint32_t *err) {
+  const Job jb = jobs[blockIdx.x];
+  const uint32_t lane = threadIdx.x;
+  if (lane >= jb.tile) return;
+  const ColView cv = cols[jb.col];
+  const uint32_t ci = jb.chunk + lane;
+  const DevChunk c = cv.chunks[ci];
+  if (!kIndex && cv.chunk_err && cv.chunk_err[ci]) return;
+  const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
+  double *out = cv.out + c.rec_off;
+  XorCkpt *ck = kIndex ? reinterpret_cast<XorCkpt *>(c.xidx) : nullptr;
+  const uint32_t bits = c.size * 8u;
+  uint32_t e;
+  switch (c.codec) {
+    case kCodecGorilla: e = gorilla_run<kWrite>(data, bits, c.n, 0, c.n, 0, 0, 0, 0, out, ck); break;
+    case kCodecChimp: e = chimp_run<kWrite>(data, bits, c.n, 0, c.n, 0, 0, 0xffffffffu, out, ck); break;
+    case kCodecElf:
+      e = elf_run<kWrite>(data, bits, c.n, 0, c.n, 0, 0, 0xffffffffu, 0xffffffffu, INT32_MAX, out, ck);
+      break;
+    default: e = kErrCorrupt; break;
+  }
+  if (e) {
+    atomicOr(err, e);
+    if (cv.chunk_err) atomicOr(cv.chunk_err + ci, e);
+  }
+}
+
+template <bool kWrite>
+__global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const Job *jobs, uint32_t *err) {
+  __shared__ uint64_t ring[128 * kWave];
+  const Job jb = jobs[blockIdx.x];
+  const uint32_t lane = threadIdx.x;
+  if (lane >= jb.tile) return;
+  const ColView cv = cols[jb.col];
+  const uint32_t ci = jb.chunk + lane;
+  const DevChunk c = cv.chunks[ci];
+  if (kWrite && cv.chunk_err && cv.chunk_err[ci]) return;
+  const uint32_t e = chimp128_run<kWrite>(reinterpret_cast<const uint8_t *>(c.data), c.size * 8u, c.n,
+                                          cv.out + c.rec_off, ring + lane);
+  if (e) {
+    atomicOr(err, e);
+    if (cv.chunk_err) atomicOr(cv.chunk_err + ci, e);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_segments(const ColView *cols, const Job *jobs, uint32_t n_jobs,
+                                                 uint32_t *err) {
+  const uint32_t j = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (j >= n_jobs) return;
+  const Job jb = jobs[j];
+  const ColView cv = cols[jb.col];
+  const DevChunk c = cv.chunks[jb.chunk];
+  if (cv.chunk_err && cv.chunk_err[jb.chunk]) return;
+  const uint32_t seg = jb.tile + lane_id();
+  const uint32_t i0 = seg * kSeg;
+  if (i0 >= c.n) return;
+  const uint32_t i1 = min(i0 + kSeg, c.n);
+  const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[seg];
+  const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
+  double *out = cv.out + c.rec_off;
+  const uint32_t bits = c.size * 8u;
+  uint32_t e;
+  switch (c.codec) {
+    case kCodecGorilla: e = gorilla_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a, k.b, out, nullptr); break;
+    case kCodecChimp:
+      e = chimp_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a == 255 ? 0xffffffffu : k.a, out, nullptr);
+      break;
+    case kCodecElf:
+      e = elf_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a == 255 ? 0xffffffffu : k.a,
+                        k.b == 255 ? 0xffffffffu : k.b, k.beta < 0 ? INT32_MAX : (int32_t)k.beta, out, nullptr);
+      break;
+    default: e = kErrCorrupt; break;
+  }
+  if (e) atomicOr(err, e);
+}
+
+// Tile sums for the load-time index: DeltaSprintz sum of deltas, RLE sum of runs.
+__global__ void __launch_bounds__(256) k_tile_sums(const ColView *cols, const Job *jobs, uint32_t n_jobs) {
+  const uint32_t j = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (j >= n_jobs) return;
+  const Job jb = jobs[j];
+  const ColView cv = cols[jb.col];
+  const DevChunk c = cv.chunks[jb.chunk];
+  int64_t s = 0;
+  if (c.codec == kCodecSprintz) s = sprintz_tile_sum(c, jb.tile);
+  else if (c.codec == kCodecRLE) s = (int64_t)rle_tile_sum(c, jb.tile);
+  else return;
+  if (lane_id() == 0) cv.tile_index[c.tile0 + jb.tile] = s;
+}
+
+// One thread per chunk: exclusive scan of its tile sums in place.
+__global__ void k_tile_scan(const ColView *cols, uint32_t col, uint32_t n_chunks, uint32_t *err) {
+  const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= n_chunks) return;
+  const ColView cv = cols[col];
+  const DevChunk c = cv.chunks[ci];
+  int64_t *idx = cv.tile_index + c.tile0;
+  if (c.codec == kCodecSprintz) {
+    const uint32_t nt = (c.n + kTile - 1) / kTile;
+    uint64_t acc = (uint64_t)c.base;
+    for (uint32_t t = 0; t < nt; ++t) {
+      const uint64_t s = (uint64_t)idx[t];
+      idx[t] = (int64_t)acc;
+      acc += s;
+    }
+  } else if (c.codec == kCodecRLE) {
+    const uint32_t nt = (uint32_t)((c.fields + kTile - 1) / kTile);
+    uint64_t acc = 0;
+    for (uint32_t t = 0; t < nt; ++t) {
+      const uint64_t s = (uint64_t)idx[t];
+      idx[t] = (int64_t)acc;
+      acc += s;
+    }
+    if (acc != c.n) {  // the run total must match the footer record count
+      atomicOr(err, (uint32_t)kErrCount);
+      if (cv.chunk_err) atomicOr(cv.chunk_err + ci, (uint32_t)kErrCount);
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------- launchers
+namespace launch {
+
+hipError_t set_pow10(const double *p10, const double *n10) {
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_p10), p10, sizeof(double) * 400);
+  if (e != hipSuccess) return e;
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_n10), n10, sizeof(double) * 400);
+}
+
+void zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
+  if (n) hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+void tiles(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *ctr,
+           uint32_t *err, uint32_t max_blocks) {
+  if (!n_jobs) return;
+  uint32_t blocks = (n_jobs + 3) / 4;
+  if (max_blocks && blocks > max_blocks) blocks = max_blocks;
+  zero_u32(s, ctr, 1);
+  hipLaunchKernelGGL(k_tiles, dim3(blocks), dim3(256), 0, s, cols, jobs, n_jobs, ctr, err);
+}
+
+void serial(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *err,
+            bool write, bool index) {
+  if (!n_jobs) return;
+  if (write && index)
+    hipLaunchKernelGGL((k_serial<true, true>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
+  else if (write)
+    hipLaunchKernelGGL((k_serial<true, false>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
+  else
+    hipLaunchKernelGGL((k_serial<false, true>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
+}
+
+void chimp128(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *err,
+              bool write) {
+  if (!n_jobs) return;
+  if (write)
+    hipLaunchKernelGGL((k_chimp128<true>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
+  else
+    hipLaunchKernelGGL((k_chimp128<false>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
+}
+
+void segments(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *err) {
+  if (!n_jobs) return;
+  hipLaunchKernelGGL(k_segments, dim3((n_jobs + 3) / 4), dim3(256), 0, s, cols, jobs, n_jobs, err);
+}
+
+void tile_index(hipStream_t s, const ColView *cols, uint32_t col, const Job *jobs, uint32_t n_jobs,
+                uint32_t n_chunks, uint32_t *err) {
+  if (n_jobs) hipLaunchKernelGGL(k_tile_sums, dim3((n_jobs + 3) / 4), dim3(256), 0, s, cols, jobs, n_jobs);
+  if (n_chunks)
+    hipLaunchKernelGGL(k_tile_scan, dim3((n_chunks + 255) / 256), dim3(256), 0, s, cols, col, n_chunks, err);
+}
+
+}  // namespace launch
+}  // namespace ebi

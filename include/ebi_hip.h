@@ -1,0 +1,191 @@
+/*
+ * ebi_hip.h -- C ABI of the MI355X (gfx950) chunked-decode and in-situ query
+ * library for EBI1 files (libebi_hip.so).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, throws no
+ * C++ exception across the boundary and returns an ebi_status.  Host buffers
+ * are owned by the caller and borrowed for the duration of one call (the
+ * shape of cuda-binding/src/lib.rs:59-140 in the reference); device memory
+ * is owned by the library through the ctx / col handles.  A ctx serialises
+ * its calls on one HIP stream, so calls from several threads on one ctx are
+ * safe but do not overlap; the serving entry points (ebi_hip_serve_*) are the
+ * concurrent path.
+ *
+ * Bitsets are dense little-endian u64 words over GLOBAL record ids: record r
+ * is bit (r % 64) of word (r / 64).  They replace the reference's
+ * RoaringBitmap of global ids (ebi/src/decoder/query.rs:3); the reference's
+ * own BUFF filter builds its result from lsb0 bytes the same way
+ * (decoder/chunk_reader/buff/filter_cmp.rs:532-541).
+ *
+ * Reference citations are relative to the lemolatoon/ebi repository.
+ */
+#ifndef EBI_HIP_H
+#define EBI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  They map one-to-one onto decoder::DecoderError
+ * (ebi/src/decoder/error.rs:6-21) plus the device-side conditions. */
+typedef enum ebi_status {
+    EBI_OK = 0,
+    EBI_UNEXPECTED_END = 1, /* DecoderError::UnexpectedEndOfChunk */
+    EBI_CONVERSION = 2,     /* DecoderError::ConversionError (bad scheme / field byte) */
+    EBI_PRECONDITION = 3,   /* DecoderError::PreconditionsNotMet */
+    EBI_HIP = 4,            /* a HIP runtime call failed (was CudaError) */
+    EBI_OOM = 5,            /* device or host allocation failed */
+    EBI_REJECTED = 6,       /* serving admission refused the request */
+    EBI_INVALID_ARGUMENT = 7,
+    EBI_UNSUPPORTED = 8     /* scheme outside the accelerated set */
+} ebi_status;
+
+/* Compression scheme ids, format.rs:185-199. */
+enum {
+    EBI_SCHEME_UNCOMPRESSED = 0,
+    EBI_SCHEME_RLE = 1,
+    EBI_SCHEME_BUFF = 2,
+    EBI_SCHEME_GORILLA = 3,
+    EBI_SCHEME_CHIMP = 4,
+    EBI_SCHEME_CHIMP128 = 5,
+    EBI_SCHEME_ELF_ON_CHIMP = 6,
+    EBI_SCHEME_ELF = 7,
+    EBI_SCHEME_DELTA_SPRINTZ = 8
+};
+
+/* ChunkOption kinds, format.rs:173-178 / encoder.rs:26-64. */
+enum { EBI_CHUNK_FULL = 0, EBI_CHUNK_RECORD_COUNT = 1, EBI_CHUNK_BYTE_SIZE = 2 };
+
+/* Predicate, decoder/query.rs:347-446.  kind: 0 Eq(value), 1 Ne(value),
+ * 2 Range(start, end); bound kinds: 0 None, 1 Inclusive, 2 Exclusive. */
+enum { EBI_PRED_EQ = 0, EBI_PRED_NE = 1, EBI_PRED_RANGE = 2 };
+enum { EBI_BOUND_NONE = 0, EBI_BOUND_INCLUSIVE = 1, EBI_BOUND_EXCLUSIVE = 2 };
+typedef struct ebi_pred {
+    int32_t kind;
+    int32_t start_kind;
+    int32_t end_kind;
+    int32_t reserved;
+    double value; /* Eq / Ne operand */
+    double start; /* Range start operand */
+    double end;   /* Range end operand */
+} ebi_pred;
+
+/* ------------------------------------------------------------------ */
+/* Write path (CPU): the reference's Encoder::encode                   */
+/* (ebi/src/api/encoder.rs:140-178, encoder.rs:169-405).               */
+/* ------------------------------------------------------------------ */
+typedef struct ebi_encoder_config {
+    int32_t scheme;       /* EBI_SCHEME_* (0..8) */
+    int32_t chunk_kind;   /* EBI_CHUNK_* */
+    uint64_t chunk_value; /* records (RECORD_COUNT) or bytes (BYTE_SIZE) */
+    uint64_t capacity;    /* Capacity config field; 0 -> 8192 (compressor.rs:465) */
+    uint64_t scale;       /* BUFF / DeltaSprintz scale; ignored otherwise */
+    int32_t n_threads;    /* chunks are encoded on this many host threads; 0 -> 1 */
+    int32_t reserved;
+} ebi_encoder_config;
+
+/* Encode n f64 values into one EBI1 file.  *out is allocated by the library
+ * (free it with ebi_buffer_free).  Bytes are identical for any n_threads. */
+int ebi_encode(const double *values, size_t n, const ebi_encoder_config *cfg,
+               uint8_t **out, size_t *out_len);
+void ebi_buffer_free(void *p);
+
+/* ------------------------------------------------------------------ */
+/* File metadata (CPU): FileReader::fetch_header / fetch_footer         */
+/* (ebi/src/decoder.rs:131-220) and GeneralChunkHandle geometry         */
+/* (decoder.rs:283-352).                                                */
+/* ------------------------------------------------------------------ */
+typedef struct ebi_file_info {
+    uint64_t number_of_records;
+    uint64_t number_of_chunks;
+    uint64_t footer_offset;
+    uint64_t chunk_option_value;
+    uint64_t config_capacity;
+    uint64_t config_scale;
+    int32_t scheme;
+    int32_t chunk_option_kind;
+    uint16_t version[3];
+    uint16_t reserved;
+} ebi_file_info;
+
+int ebi_file_parse(const uint8_t *file, size_t len, ebi_file_info *info);
+/* Per-chunk geometry; arrays hold info.number_of_chunks entries (nullable). */
+int ebi_file_chunks(const uint8_t *file, size_t len, uint64_t *physical_offset,
+                    uint64_t *chunk_size, uint64_t *logical_offset,
+                    uint64_t *number_of_records);
+
+/* ------------------------------------------------------------------ */
+/* Device context and resident columns                                 */
+/* ------------------------------------------------------------------ */
+typedef struct ebi_hip_ctx ebi_hip_ctx;
+typedef struct ebi_hip_col ebi_hip_col;
+
+/* Replaces cuda_binding::Context::new_at (cuda-binding/src/lib.rs:41-53). */
+int ebi_hip_ctx_create(int device, ebi_hip_ctx **out);
+int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx);
+/* Message of the last failing call on this ctx ("" if none). */
+const char *ebi_hip_last_error(const ebi_hip_ctx *ctx);
+
+/* Parse header + footer (Decoder::new, api/decoder.rs:136-156), build the
+ * chunk-descriptor table and upload chunk bytes; the column stays resident
+ * until ebi_hip_column_free. */
+int ebi_hip_column_load(ebi_hip_ctx *ctx, const uint8_t *file, size_t len,
+                        ebi_hip_col **out);
+int ebi_hip_column_free(ebi_hip_ctx *ctx, ebi_hip_col *col);
+int ebi_hip_column_info(const ebi_hip_col *col, ebi_file_info *info);
+/* Bytes of chunk data resident for the column (excluding padding). */
+uint64_t ebi_hip_column_compressed_bytes(const ebi_hip_col *col);
+
+/* Query entry points.  They mirror Decoder::{materialize, filter,
+ * filter_materialize, sum, min, max} (api/decoder.rs:202-439):
+ *   bitset   -- nullable input bitmask over global ids, ceil(N/64) words;
+ *   chunk_id -- -1 for every chunk, else ChunkId (api/decoder.rs:755-780);
+ * chunks whose record range has no bit set are pruned exactly as the
+ * reference does (api/decoder.rs:222-226). */
+int ebi_hip_materialize(ebi_hip_ctx *ctx, const ebi_hip_col *col,
+                        const uint64_t *bitset, int64_t chunk_id,
+                        double *out, size_t *n_out);
+int ebi_hip_filter(ebi_hip_ctx *ctx, const ebi_hip_col *col, const ebi_pred *pred,
+                   const uint64_t *bitset, int64_t chunk_id, uint64_t *out_bitset);
+int ebi_hip_filter_materialize(ebi_hip_ctx *ctx, const ebi_hip_col *col,
+                               const ebi_pred *pred, const uint64_t *bitset,
+                               int64_t chunk_id, double *out, size_t *n_out);
+int ebi_hip_sum(ebi_hip_ctx *ctx, const ebi_hip_col *col, const uint64_t *bitset,
+                int64_t chunk_id, double *out);
+int ebi_hip_min(ebi_hip_ctx *ctx, const ebi_hip_col *col, const uint64_t *bitset,
+                int64_t chunk_id, double *out);
+int ebi_hip_max(ebi_hip_ctx *ctx, const ebi_hip_col *col, const uint64_t *bitset,
+                int64_t chunk_id, double *out);
+
+/* Device-resident decode: writes all N records of the column to the device
+ * buffer d_out (N doubles) on the ctx stream and waits for completion.
+ * This is the whole-column decode the batched and DGEMM paths build on. */
+int ebi_hip_decode_device(ebi_hip_ctx *ctx, const ebi_hip_col *col, double *d_out);
+
+/* Batched decode of several resident columns in ONE persistent launch over
+ * a device-side chunk-descriptor queue (mixed codecs, uneven chunks).
+ * d_outs[i] receives column i. */
+int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
+                         size_t n_cols, double *const *d_outs);
+
+/* Comparator: the plain per-chunk-launch form -- one kernel launch per
+ * chunk and a host synchronisation after each. */
+int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col, double *d_out);
+
+/* Device allocation helpers for callers without their own allocator. */
+int ebi_hip_device_alloc(ebi_hip_ctx *ctx, size_t bytes, void **d_ptr);
+int ebi_hip_device_free(ebi_hip_ctx *ctx, void *d_ptr);
+int ebi_hip_copy_to_host(ebi_hip_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+
+/* Milliseconds of device time of the last kernel sequence issued by the
+ * previous call on this ctx (hipEvent pair on the ctx stream). */
+double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EBI_HIP_H */

@@ -1,0 +1,67 @@
+"""GPU decode parity: every codec's HIP decoder against the CPU oracle,
+bit-exact on the decoded f64 patterns (ebi/tests/round_trip.rs:249-336 is the
+reference's own round-trip strategy)."""
+import numpy as np
+import pytest
+
+import ebi_amd as E
+from ebi_amd import datagen as G
+from oracle.oracle import OracleDecoder
+
+pytestmark = pytest.mark.gpu
+
+CODECS = G.CODECS
+
+
+def _gpu_decode(ctx, data, per_chunk=False):
+    dec = E.Decoder(ctx, data)
+    buf = ctx.alloc(max(8 * dec.number_of_records, 8))
+    try:
+        (dec.decode_per_chunk if per_chunk else dec.decode_device)(buf)
+        return buf.to_host(np.float64, dec.number_of_records)
+    finally:
+        buf.free()
+        dec.close()
+
+
+def _check(ctx, values, codec, chunk, scale=100, per_chunk=False):
+    data = E.encode(values, E.CompressorConfig.by_name(codec, scale=scale), chunk)
+    want = OracleDecoder(data).materialize()
+    got = _gpu_decode(ctx, data, per_chunk)
+    assert got.shape == want.shape
+    bad = np.flatnonzero(got.view(np.uint64) != want.view(np.uint64))
+    assert bad.size == 0, f"{codec} {chunk}: {bad.size} mismatches, first at {bad[:5]}: {got[bad[:3]]} vs {want[bad[:3]]}"
+
+
+@pytest.mark.parametrize("codec", CODECS)
+@pytest.mark.parametrize("gen", list(G.GENERATORS))
+def test_decode_generators(ctx, codec, gen):
+    fn, dec = G.GENERATORS[gen]
+    vals = fn(20000)
+    _check(ctx, vals, codec, E.ChunkOption.record_count(4096), scale=10 ** dec)
+
+
+@pytest.mark.parametrize("codec", CODECS)
+@pytest.mark.parametrize("chunk", [E.ChunkOption.record_count(5), E.ChunkOption.record_count(1003),
+                                   E.ChunkOption.byte_size_best_effort(5274), E.ChunkOption.full()])
+def test_decode_ragged_chunks(ctx, codec, chunk):
+    vals = G.random_walk(10003, seed=7)
+    _check(ctx, vals, codec, chunk)
+
+
+@pytest.mark.parametrize("codec", [c for c in CODECS if c not in ("buff", "delta_sprintz")])
+def test_decode_xor_adversarial(ctx, codec):
+    _check(ctx, G.xor_values(), codec, E.ChunkOption.record_count(66))
+    _check(ctx, G.xor_values2(), codec, E.ChunkOption.record_count(3))
+
+
+@pytest.mark.parametrize("codec", CODECS)
+def test_decode_per_chunk_comparator(ctx, codec):
+    vals = G.random_walk(6000, seed=11)
+    _check(ctx, vals, codec, E.ChunkOption.record_count(1000), per_chunk=True)
+
+
+@pytest.mark.parametrize("codec", CODECS)
+def test_decode_single_record_and_constant(ctx, codec):
+    _check(ctx, np.array([42.5]), codec, E.ChunkOption.record_count(8))
+    _check(ctx, np.full(3000, 100.19), codec, E.ChunkOption.record_count(700))
