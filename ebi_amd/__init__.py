@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 from pathlib import Path
 from typing import Iterable, Optional
@@ -103,6 +104,8 @@ _P = C.c_void_p
 _SIGS = {
     "ebi_encode": (C.c_int, [_P, C.c_size_t, C.POINTER(_EncCfg), C.POINTER(_P), C.POINTER(C.c_size_t)]),
     "ebi_buffer_free": (None, [_P]),
+    "ebi_synth_column": (C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, _P]),
     "ebi_file_parse": (C.c_int, [_P, C.c_size_t, C.POINTER(_FileInfo)]),
     "ebi_file_chunks": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P]),
     "ebi_hip_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
@@ -221,8 +224,11 @@ class CompressorConfig:
         return CompressorConfig(SCHEMES[name], scale, 8192)
 
 
-def encode(values, config: CompressorConfig, chunk_option: ChunkOption, threads: int = 0) -> bytes:
-    """Encoder::encode (ebi/src/api/encoder.rs:140-178) on the host; returns the EBI1 file."""
+def encode(values, config: CompressorConfig, chunk_option: ChunkOption, threads: int = 0) -> np.ndarray:
+    """Encoder::encode (ebi/src/api/encoder.rs:140-178) on the host.
+
+    Returns the EBI1 file as a uint8 array that owns the library's buffer
+    (no copy); ``bytes(arr)`` gives a bytes object."""
     v = np.ascontiguousarray(values, dtype=np.float64)
     cfg = _EncCfg(config.scheme, chunk_option.kind, chunk_option.value, config.capacity,
                   config.scale, threads or min(16, os.cpu_count() or 1), 0)
@@ -231,10 +237,18 @@ def encode(values, config: CompressorConfig, chunk_option: ChunkOption, threads:
     L = lib()
     _check(L.ebi_encode(v.ctypes.data if v.size else None, v.size, C.byref(cfg), C.byref(out), C.byref(n)),
            None, "encode")
-    try:
-        return C.string_at(out, n.value)
-    finally:
-        L.ebi_buffer_free(out)
+    arr = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint8)), shape=(n.value,))
+    weakref.finalize(arr, L.ebi_buffer_free, out.value)
+    return arr
+
+
+def synth(kind: int, n: int, seed: int, decimals: int, p0: float = 0.0, p1: float = 0.0, p2: float = 0.0,
+          threads: int = 0) -> np.ndarray:
+    """Seeded synthetic column (ebi_synth_column)."""
+    out = np.empty(max(int(n), 1), dtype=np.float64)
+    _check(lib().ebi_synth_column(int(kind), int(n), int(seed), int(decimals), float(p0), float(p1), float(p2),
+                                  threads or min(16, os.cpu_count() or 1), out.ctypes.data), None, "synth")
+    return out[: int(n)]
 
 
 @dataclass

@@ -38,32 +38,42 @@ inline uint32_t sat_u32(double d) {
 
 // MSB-first bit writer with the semantics of BufferedBitWriter
 // (io/bit_write.rs:45-64, 239-286): the low `k` bits of a value are emitted
-// most significant first; the last byte is zero padded.
+// most significant first; the last byte is zero padded.  Bits accumulate in
+// a 64-bit register and leave as big-endian 8-byte words.
 struct MsbWriter {
   std::vector<uint8_t> out;
-  uint64_t acc = 0;  // pending bits (fewer than 8) in the low bits
-  unsigned n = 0;
-  void put32(uint32_t v, unsigned k) {
-    acc = (acc << k) | v;
-    n += k;
-    while (n >= 8) {
-      n -= 8;
-      out.push_back((uint8_t)(acc >> n));
-    }
-    acc &= (1ull << n) - 1;
+  uint64_t acc = 0;  // pending bits, right-aligned
+  unsigned n = 0;    // number of pending bits (< 64)
+  MsbWriter() { out.reserve(1 << 16); }
+  inline void flush64(uint64_t w) {
+    const size_t o = out.size();
+    out.resize(o + 8);
+    const uint64_t be = __builtin_bswap64(w);
+    std::memcpy(out.data() + o, &be, 8);
   }
-  void put(uint64_t v, unsigned k) {
+  inline void put(uint64_t v, unsigned k) {
     if (k == 0) return;
     if (k < 64) v &= (1ull << k) - 1;
-    if (k > 32) {
-      put32((uint32_t)(v >> 32), k - 32);
-      put32((uint32_t)v, 32);
-    } else {
-      put32((uint32_t)v, k);
+    if (n + k < 64) {
+      acc = (acc << k) | v;
+      n += k;
+      return;
     }
+    // fill the word: take the top (64 - n) bits of v
+    const unsigned take = 64 - n;
+    const unsigned rest = k - take;
+    const uint64_t word = (n ? (acc << take) : 0) | (take == 64 ? v : (v >> rest));
+    flush64(word);
+    n = rest;
+    acc = rest ? (v & ((1ull << rest) - 1)) : 0;
   }
-  void bit(bool b) { put32(b ? 1u : 0u, 1); }
+  inline void bit(bool b) { put(b ? 1u : 0u, 1); }
   std::vector<uint8_t> finish() {
+    // remaining n bits, MSB first, zero padded to a byte
+    while (n >= 8) {
+      out.push_back((uint8_t)(acc >> (n - 8)));
+      n -= 8;
+    }
     if (n) out.push_back((uint8_t)(acc << (8 - n)));
     n = 0;
     acc = 0;
@@ -691,29 +701,43 @@ extern "C" int ebi_encode(const double *values, size_t n, const ebi_encoder_conf
   for (auto &t : pool) t.join();
   if (failed) return failed;
 
-  size_t body = 0;
-  for (auto &b : bodies) body += b.size();
+  std::vector<uint64_t> phys(n_chunks + 1);
+  phys[0] = kFileHeaderSize;
+  for (size_t c = 0; c < n_chunks; ++c) phys[c + 1] = phys[c] + bodies[c].size();
+  const uint64_t body = phys[n_chunks] - kFileHeaderSize;
   int cfg_size = config_size_of(scheme);
   size_t total = kFileHeaderSize + body + kFooter0Size + n_chunks * kChunkFooterSize + 1 +
                  cfg_size + kFooter3Size;
-  std::vector<uint8_t> f;
-  f.reserve(total);
+  uint8_t *buf = (uint8_t *)std::malloc(total);
+  if (!buf) return EBI_OOM;
   // FileHeader, written with footer_offset already known (the reference
   // back-patches the same slot, encoder.rs:236-241).
-  f.insert(f.end(), {'E', 'B', 'I', '1'});
-  f.push_back(0);
-  f.push_back((uint8_t)scheme);
-  f.push_back((uint8_t)cfg->chunk_kind);
-  f.push_back(0);
-  put_u64(f, cfg->chunk_kind == EBI_CHUNK_FULL ? 0 : cfg->chunk_value);
-  put_u64(f, kFileHeaderSize + body);
-  for (int i = 0; i < 3; ++i) put_u16(f, kVersion[i]);
-  std::vector<uint64_t> phys(n_chunks);
-  for (size_t c = 0; c < n_chunks; ++c) {
-    phys[c] = f.size();
-    f.insert(f.end(), bodies[c].begin(), bodies[c].end());
-    std::vector<uint8_t>().swap(bodies[c]);
-  }
+  std::vector<uint8_t> h;
+  h.insert(h.end(), {'E', 'B', 'I', '1'});
+  h.push_back(0);
+  h.push_back((uint8_t)scheme);
+  h.push_back((uint8_t)cfg->chunk_kind);
+  h.push_back(0);
+  put_u64(h, cfg->chunk_kind == EBI_CHUNK_FULL ? 0 : cfg->chunk_value);
+  put_u64(h, kFileHeaderSize + body);
+  for (int i = 0; i < 3; ++i) put_u16(h, kVersion[i]);
+  std::memcpy(buf, h.data(), h.size());
+  // chunk bodies, copied in parallel
+  next = 0;
+  auto copy = [&]() {
+    for (;;) {
+      size_t c = next.fetch_add(1);
+      if (c >= n_chunks) return;
+      if (!bodies[c].empty()) std::memcpy(buf + phys[c], bodies[c].data(), bodies[c].size());
+      std::vector<uint8_t>().swap(bodies[c]);
+    }
+  };
+  pool.clear();
+  for (int t = 1; t < nt; ++t) pool.emplace_back(copy);
+  copy();
+  for (auto &t : pool) t.join();
+  std::vector<uint8_t> f;
+  f.reserve(kFooter0Size + n_chunks * kChunkFooterSize + 1 + cfg_size + kFooter3Size);
   put_u64(f, n);
   put_u64(f, n_chunks);
   for (size_t c = 0; c < n_chunks; ++c) {
@@ -733,11 +757,9 @@ extern "C" int ebi_encode(const double *values, size_t n, const ebi_encoder_conf
   // FileFooter3: compression time, 9 phase timers, crc -- all zero here so
   // that files are byte-reproducible (the reference stores wall-clock times).
   f.resize(f.size() + kFooter3Size, 0);
-  uint8_t *buf = (uint8_t *)std::malloc(f.size() ? f.size() : 1);
-  if (!buf) return EBI_OOM;
-  std::memcpy(buf, f.data(), f.size());
+  std::memcpy(buf + kFileHeaderSize + body, f.data(), f.size());
   *out = buf;
-  *out_len = f.size();
+  *out_len = total;
   return EBI_OK;
 }
 

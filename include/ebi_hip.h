@@ -93,6 +93,13 @@ int ebi_encode(const double *values, size_t n, const ebi_encoder_config *cfg,
                uint8_t **out, size_t *out_len);
 void ebi_buffer_free(void *p);
 
+/* Seeded synthetic column (benchmark and test workloads, not a reference
+ * interface).  kind 0: random walk (p0 = x0, p1 = sigma); 1: uniform
+ * [p0, p1); 2: step series with change probability p0; 3: 100 sin(2 pi i /
+ * p0) + p1 N(0,1).  Values are rounded half away from zero to `decimals`. */
+int ebi_synth_column(int kind, uint64_t n, uint64_t seed, int decimals, double p0, double p1,
+                     double p2, int threads, double *out);
+
 /* ------------------------------------------------------------------ */
 /* File metadata (CPU): FileReader::fetch_header / fetch_footer         */
 /* (ebi/src/decoder.rs:131-220) and GeneralChunkHandle geometry         */

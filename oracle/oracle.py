@@ -82,7 +82,8 @@ class OracleDecoder:
     """CPU restatement of Decoder over one EBI1 file held in memory."""
 
     def __init__(self, data: bytes):
-        self._buf = np.frombuffer(data, dtype=np.uint8).copy()
+        self._data = data  # keep the caller's buffer alive; it is only read
+        self._buf = np.frombuffer(data, dtype=np.uint8)
         self._p = self._buf.ctypes.data
         self._n = len(data)
         nr, nc = C.c_uint64(), C.c_uint64()
