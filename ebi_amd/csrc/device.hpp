@@ -59,8 +59,42 @@ struct alignas(16) DevChunk {
   uint64_t fields;    // RLE field count
   uint64_t values_off;    // RLE: byte offset of the f64 array from data
   uint64_t xidx;      // absolute address of this chunk's XOR checkpoints (0 = none)
+  uint32_t rtile0;    // first record tile (kTile records) of this chunk in the column
+  uint32_t pad0;
+  uint64_t pad1;
 };
-static_assert(sizeof(DevChunk) == 96, "DevChunk layout");
+static_assert(sizeof(DevChunk) == 112, "DevChunk layout");
+
+// Per-chunk query parameters computed on the host from the chunk header and
+// the predicate (BUFF fixed-point / DeltaSprintz quantized domain).
+enum SideMode : uint32_t {
+  kSideNone = 0,   // RangeValue::None (no bound on this side)
+  kSideAll = 1,    // early exit: every record of the chunk, bitmask ignored
+  kSideEmpty = 2,  // early exit: no record
+  kSideCmp = 3,    // compare per record (restricted to the bitmask)
+};
+struct alignas(16) ChunkPred {
+  uint32_t lmode, rmode;   // BUFF range sides / Eq (lmode) ; Sprintz: unused
+  uint32_t lincl, rincl;   // inclusive flags
+  uint64_t ldp, rdp;       // BUFF: predicate delta (fixed - base); Sprintz: quantized operands
+  uint32_t eq_u8;          // BUFF Eq with B < 8 and no bitmask: compare with (u8)dp
+  uint32_t pad;
+  uint64_t pad1;
+};
+static_assert(sizeof(ChunkPred) == 48, "ChunkPred layout");
+
+// Per-call query arguments (passed by value).
+struct QueryArgs {
+  const uint64_t *mask;      // input bitset over global ids (nullable)
+  uint64_t *out_bits;        // output bitset (filter / select)
+  const double *values;      // decoded records, global index (default codecs)
+  const uint32_t *chunk_sel; // per chunk: 1 if selected (chunk_id, pruning)
+  const ChunkPred *cpred;    // per chunk predicate parameters (BUFF, Sprintz)
+  uint64_t *acc;             // per chunk: sum, count, min key, max key
+  double *partials;          // per record tile: (hi, lo) double-double sum
+  int32_t kind, start_kind, end_kind, op;  // predicate (f64 domain) and aggregate op
+  double value, start, end;
+};
 
 // XOR-codec checkpoint (every kSeg records): enough state to restart the
 // serial decoder at record k*kSeg.
@@ -87,15 +121,24 @@ struct alignas(16) Job {
   uint32_t tile;   // tile / first segment index; lane count for kJobSerial
 };
 
-// Per-launch view of one column.
+// Per-launch view of one column.  A launch over several columns walks the
+// concatenation of their job lists; `prefix` arrays (one entry per column
+// plus one) map a launch-wide job id to (column, local job).
 struct alignas(16) ColView {
   const DevChunk *chunks;
-  int64_t *tile_index;   // per-tile prefix state (Sprintz q, RLE start record)
-  double *out;           // decode output base (record r -> out[r])
-  uint32_t *chunk_err;   // per-chunk DevError flags found by the index pass
+  int64_t *tile_index;     // per-tile prefix state (Sprintz q, RLE start record)
+  double *out;             // decode output base (record r -> out[r])
+  uint32_t *chunk_err;     // per-chunk DevError flags found by the index pass
+  const Job *tile_jobs;    // resident: kTile-record jobs of tile codecs
+  const Job *seg_jobs;     // resident: checkpoint-segment jobs of XOR codecs
+  const Job *serial_jobs;  // resident: lane-per-chunk jobs (Chimp128, cold XOR)
   uint64_t n_records;
-  uint64_t n_chunks;
+  uint32_t n_chunks;
+  uint32_t n_tile_jobs;
+  uint32_t n_seg_jobs;
+  uint32_t n_serial_jobs;
 };
+static_assert(sizeof(ColView) == 80, "ColView layout");
 
 // Device error flags (OR-ed by kernels).
 enum DevError : uint32_t {

@@ -71,9 +71,25 @@ bool is_xor_codec(int scheme) {
   return scheme == kGorilla || scheme == kChimp || scheme == kElf;
 }
 
+ColView view_of(const Column &c, double *out) {
+  ColView v{};
+  v.chunks = c.d_chunks;
+  v.tile_index = c.d_tile_index;
+  v.out = out;
+  v.chunk_err = c.d_chunk_err;
+  v.tile_jobs = c.d_tile_jobs;
+  v.seg_jobs = c.d_seg_jobs;
+  v.serial_jobs = c.d_serial_jobs;
+  v.n_records = c.meta.number_of_records;
+  v.n_chunks = (uint32_t)c.chunks.size();
+  v.n_tile_jobs = c.n_tile_jobs;
+  v.n_seg_jobs = c.n_seg_jobs;
+  v.n_serial_jobs = c.n_serial_jobs;
+  return v;
+}
+
 int upload_view(ebi_hip_ctx *ctx, Column &c, double *out) {
-  ColView v{c.d_chunks, c.d_tile_index, out, c.d_chunk_err, c.meta.number_of_records,
-            (uint64_t)c.chunks.size()};
+  ColView v = view_of(c, out);
   *ctx->h_view = v;
   hipError_t e = hipMemcpyAsync(c.d_view, ctx->h_view, sizeof(ColView), hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(ColView)");
@@ -111,16 +127,13 @@ int check_chunk_errors(ebi_hip_ctx *ctx, const Column &c, int64_t chunk_id) {
 // Queue the kernels that write every record of the selected chunks of `c`
 // to out (ColView already uploaded).  chunk_id < 0: all chunks.
 int enqueue_decode(ebi_hip_ctx *ctx, Column &c) {
-  launch::tiles(ctx->stream, c.d_view, c.d_tile_jobs, c.n_tile_jobs, ctx->d_flags + 1, ctx->d_flags,
-                (uint32_t)ctx->cu_count * 8);
-  if (c.chimp128) {
-    launch::chimp128(ctx->stream, c.d_view, c.d_serial_jobs, c.n_serial_jobs, ctx->d_flags, true);
-  } else if (c.xor_codec) {
-    if (ctx->xor_mode == 0)
-      launch::segments(ctx->stream, c.d_view, c.d_seg_jobs, c.n_seg_jobs, ctx->d_flags);
-    else
-      launch::serial(ctx->stream, c.d_view, c.d_serial_jobs, c.n_serial_jobs, ctx->d_flags, true, false);
-  }
+  const uint32_t jobs = c.n_tile_jobs + (ctx->xor_mode == 0 ? c.n_seg_jobs : 0);
+  launch::decode(ctx->stream, c.d_view, c.d_prefix, 1, jobs, ctx->d_flags + 1, ctx->d_flags,
+                 (uint32_t)ctx->cu_count * 8);
+  if (c.chimp128)
+    launch::chimp128(ctx->stream, c.d_view, c.d_prefix + 2, 1, c.n_serial_jobs, nullptr, ctx->d_flags, true);
+  else if (c.xor_codec && ctx->xor_mode != 0)
+    launch::serial(ctx->stream, c.d_view, c.d_prefix + 2, 1, c.n_serial_jobs, nullptr, ctx->d_flags, true, false);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(ctx, e, "decode launch");
   return EBI_OK;
@@ -133,10 +146,12 @@ static void free_column(Column &c) {
   hipFree(c.d_ckpt);
   hipFree(c.d_chunk_err);
   hipFree(c.d_view);
+  hipFree(c.d_prefix);
   hipFree(c.d_tile_jobs);
   hipFree(c.d_serial_jobs);
   hipFree(c.d_seg_jobs);
   hipFree(c.d_single_jobs);
+  hipFree(c.d_rec_jobs);
   c = Column();
 }
 
@@ -245,6 +260,8 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
       d.xidx = ckpts;  // relative
       ckpts += h.n_segs;
     }
+    d.rtile0 = (uint32_t)col.n_rtiles;
+    col.n_rtiles += (n + kTile - 1) / kTile;
     h.dev = d;
     dev_bytes += round_up(std::max<uint64_t>(body_len, 1), kAlign) + kPad;
   }
@@ -316,6 +333,11 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
       col.h_seg_jobs.push_back(Job{kJobSeg, 0, (uint32_t)k, s});
   }
   col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
+  std::vector<Job> rec_jobs;
+  for (size_t k = 0; k < nc; ++k)
+    for (uint32_t t = 0; t < (col.chunks[k].dev.n + kTile - 1) / kTile; ++t)
+      rec_jobs.push_back(Job{kJobTile, 0, (uint32_t)k, t});
+  col.n_rec_jobs = (uint32_t)rec_jobs.size();
   std::vector<Job> single;
   if (col.xor_codec || col.chimp128) {
     for (size_t k = 0; k < nc; k += kWaveLanes)
@@ -328,9 +350,14 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   if ((e = upload_vec(&col.d_tile_jobs, col.h_tile_jobs)) != hipSuccess ||
       (e = upload_vec(&col.d_serial_jobs, col.h_serial_jobs)) != hipSuccess ||
       (e = upload_vec(&col.d_seg_jobs, col.h_seg_jobs)) != hipSuccess ||
-      (e = upload_vec(&col.d_single_jobs, single)) != hipSuccess)
+      (e = upload_vec(&col.d_single_jobs, single)) != hipSuccess ||
+      (e = upload_vec(&col.d_rec_jobs, rec_jobs)) != hipSuccess)
     return hip_fail(ctx, e, "upload job lists");
 
+  {
+    const std::vector<uint32_t> pre = {0, col.n_tile_jobs + col.n_seg_jobs, 0, col.n_serial_jobs};
+    if ((e = upload_vec(&col.d_prefix, pre)) != hipSuccess) return hip_fail(ctx, e, "upload prefix");
+  }
   // Load-time index pass.
   launch::zero_u32(ctx->stream, col.d_chunk_err, (uint32_t)nc);
   launch::zero_u32(ctx->stream, ctx->d_flags, 2);
@@ -338,9 +365,10 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   if (scheme == kDeltaSprintz || scheme == kRLE)
     launch::tile_index(ctx->stream, col.d_view, 0, col.d_tile_jobs, col.n_tile_jobs, (uint32_t)nc, ctx->d_flags);
   if (col.xor_codec)
-    launch::serial(ctx->stream, col.d_view, col.d_serial_jobs, col.n_serial_jobs, ctx->d_flags, false, true);
+    launch::serial(ctx->stream, col.d_view, col.d_prefix + 2, 1, col.n_serial_jobs, nullptr, ctx->d_flags, false,
+                   true);
   if (col.chimp128)
-    launch::chimp128(ctx->stream, col.d_view, col.d_serial_jobs, col.n_serial_jobs, ctx->d_flags, false);
+    launch::chimp128(ctx->stream, col.d_view, col.d_prefix + 2, 1, col.n_serial_jobs, nullptr, ctx->d_flags, false);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "index launch");
   std::vector<uint32_t> errs(nc);
   if (nc) {
@@ -379,11 +407,13 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipMalloc((void **)&c->d_flags, 64 * sizeof(uint32_t)) != hipSuccess ||
       hipHostMalloc((void **)&c->h_flags, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_view, 64 * sizeof(ColView), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void **)&c->h_view, 64 * sizeof(ColView), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_batch, 1 << 16, hipHostMallocDefault) != hipSuccess) {
     ebi_hip_ctx_destroy(c);
     return EBI_HIP;
   }
   c->h_view_cap = 64;
+  c->h_batch_cap = 1 << 16;
   // Powers of ten for the Elf restore, parsed exactly as the reference does.
   static double p10[400], n10[400];
   char buf[32];
@@ -411,7 +441,8 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   ctx->scratch_misc.release();
   ctx->scratch_out.release();
   ctx->batch_jobs.release();
-  if (ctx->d_views) hipFree(ctx->d_views);
+  ctx->batch_args.release();
+  if (ctx->h_batch) hipHostFree(ctx->h_batch);
   if (ctx->d_flags) hipFree(ctx->d_flags);
   if (ctx->h_flags) hipHostFree(ctx->h_flags);
   if (ctx->h_view) hipHostFree(ctx->h_view);
@@ -500,6 +531,7 @@ extern "C" int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col
   int rc = check_chunk_errors(ctx, c, -1);
   if (rc) return rc;
   if ((rc = upload_view(ctx, c, d_out))) return rc;
+  if (ctx->batch_args.ensure(4096)) return fail(ctx, EBI_OOM, "batch arguments");
   launch::zero_u32(ctx->stream, ctx->d_flags, 2);
   hipEventRecord(ctx->ev0, ctx->stream);
   // The plain form: one launch per chunk and a host synchronisation after
@@ -507,13 +539,24 @@ extern "C" int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col
   for (size_t k = 0; k < c.chunks.size(); ++k) {
     if (c.xor_codec || c.chimp128) {
       if (c.chimp128)
-        launch::chimp128(ctx->stream, c.d_view, c.d_single_jobs + k, 1, ctx->d_flags, true);
+        launch::chimp128(ctx->stream, c.d_view, nullptr, 1, 1, c.d_single_jobs + k, ctx->d_flags, true);
       else
-        launch::serial(ctx->stream, c.d_view, c.d_single_jobs + k, 1, ctx->d_flags, true, false);
+        launch::serial(ctx->stream, c.d_view, nullptr, 1, 1, c.d_single_jobs + k, ctx->d_flags, true, false);
     } else {
+      // the chunk's tile jobs are a contiguous slice of the resident list
       const uint32_t a = c.tile_job_first[k], b = c.tile_job_first[k + 1];
-      launch::tiles(ctx->stream, c.d_view, c.d_tile_jobs + a, b - a, ctx->d_flags + 1, ctx->d_flags,
-                    (uint32_t)ctx->cu_count * 8);
+      ColView v = view_of(c, d_out);
+      v.tile_jobs = c.d_tile_jobs + a;
+      v.n_tile_jobs = b - a;
+      v.n_seg_jobs = 0;
+      uint32_t *pre = reinterpret_cast<uint32_t *>(ctx->h_batch);
+      pre[0] = 0;
+      pre[1] = b - a;
+      std::memcpy(ctx->h_batch + 256, &v, sizeof v);
+      hipMemcpyAsync(ctx->batch_args.p, ctx->h_batch, 256 + sizeof v, hipMemcpyHostToDevice, ctx->stream);
+      launch::decode(ctx->stream, (const ColView *)((uint8_t *)ctx->batch_args.p + 256),
+                     (const uint32_t *)ctx->batch_args.p, 1, b - a, ctx->d_flags + 1, ctx->d_flags,
+                     (uint32_t)ctx->cu_count * 8);
     }
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "per-chunk decode");
@@ -547,5 +590,64 @@ extern "C" int ebi_hip_copy_to_host(ebi_hip_ctx *ctx, void *h_dst, const void *d
   hipError_t e = hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "copy to host");
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols, size_t n_cols,
+                                    double *const *d_outs) {
+  EBI_GUARD(ctx);
+  if (!cols || !d_outs || n_cols == 0 || n_cols > (1u << 16)) return EBI_INVALID_ARGUMENT;
+  int rc;
+  for (size_t i = 0; i < n_cols; ++i) {
+    if (!cols[i] || (!d_outs[i] && cols[i]->c.meta.number_of_records)) return EBI_INVALID_ARGUMENT;
+    if ((rc = check_chunk_errors(ctx, cols[i]->c, -1))) return rc;
+  }
+  // arguments: prefix over tile+seg jobs, prefix over Chimp128 serial jobs,
+  // then the ColView array (256-byte aligned)
+  const size_t pre_bytes = ((2 * (n_cols + 1) * 4 + 255) / 256) * 256;
+  const size_t bytes = pre_bytes + n_cols * sizeof(ColView);
+  if (bytes > ctx->h_batch_cap) {
+    hipHostFree(ctx->h_batch);
+    ctx->h_batch = nullptr;
+    ctx->h_batch_cap = 0;
+    if (hipHostMalloc((void **)&ctx->h_batch, bytes, hipHostMallocDefault) != hipSuccess)
+      return fail(ctx, EBI_OOM, "batch staging");
+    ctx->h_batch_cap = bytes;
+  }
+  if (ctx->batch_args.ensure(bytes)) return fail(ctx, EBI_OOM, "batch arguments");
+  uint32_t *pre = reinterpret_cast<uint32_t *>(ctx->h_batch);
+  uint32_t *pre128 = pre + (n_cols + 1);
+  ColView *views = reinterpret_cast<ColView *>(ctx->h_batch + pre_bytes);
+  uint64_t jobs = 0, jobs128 = 0;
+  for (size_t i = 0; i < n_cols; ++i) {
+    const Column &c = cols[i]->c;
+    pre[i] = (uint32_t)jobs;
+    pre128[i] = (uint32_t)jobs128;
+    views[i] = view_of(c, d_outs[i]);
+    jobs += c.n_tile_jobs + c.n_seg_jobs;
+    if (c.chimp128) jobs128 += c.n_serial_jobs;
+  }
+  pre[n_cols] = (uint32_t)jobs;
+  pre128[n_cols] = (uint32_t)jobs128;
+  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32)) return fail(ctx, EBI_UNSUPPORTED, "batch too large");
+  // Chimp128 columns take part only through the second prefix
+  for (size_t i = 0; i < n_cols; ++i)
+    if (!cols[i]->c.chimp128) views[i].n_serial_jobs = 0;
+  uint8_t *d = (uint8_t *)ctx->batch_args.p;
+  hipError_t e = hipMemcpyAsync(d, ctx->h_batch, bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch upload");
+  launch::zero_u32(ctx->stream, ctx->d_flags, 2);
+  hipEventRecord(ctx->ev0, ctx->stream);
+  const ColView *dv = (const ColView *)(d + pre_bytes);
+  launch::decode(ctx->stream, dv, (const uint32_t *)d, (uint32_t)n_cols, (uint32_t)jobs, ctx->d_flags + 1,
+                 ctx->d_flags, (uint32_t)ctx->cu_count * 8);
+  launch::chimp128(ctx->stream, dv, (const uint32_t *)d + (n_cols + 1), (uint32_t)n_cols, (uint32_t)jobs128,
+                   nullptr, ctx->d_flags, true);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "batch launch");
+  hipEventRecord(ctx->ev1, ctx->stream);
+  if ((rc = sync_and_check(ctx, "batch decode"))) return rc;
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  ctx->last_ms = ms;
   return EBI_OK;
 }

@@ -44,6 +44,7 @@ struct Column {
   XorCkpt *d_ckpt = nullptr;
   uint32_t *d_chunk_err = nullptr;
   ColView *d_view = nullptr;  // slot used by single-column launches
+  uint32_t *d_prefix = nullptr;  // {0, tile+seg jobs, 0, serial jobs} for single-column launches
   // resident job lists
   Job *d_tile_jobs = nullptr;
   uint32_t n_tile_jobs = 0;
@@ -51,6 +52,9 @@ struct Column {
   uint32_t n_serial_jobs = 0;
   Job *d_single_jobs = nullptr;  // one single-lane serial job per XOR chunk (comparator)
   Job *d_seg_jobs = nullptr;     // Gorilla/Chimp/Elf checkpoint segments
+  Job *d_rec_jobs = nullptr;     // record tiles of every chunk (queries)
+  uint32_t n_rec_jobs = 0;
+  uint64_t n_rtiles = 0;
   uint32_t n_seg_jobs = 0;
   std::vector<Job> h_tile_jobs, h_serial_jobs, h_seg_jobs;
   std::vector<uint32_t> tile_job_first;  // per chunk: first tile job index (comparator)
@@ -76,8 +80,9 @@ struct ebi_hip_ctx {
   uint32_t *h_flags = nullptr;  // pinned mirror
   ebi::ColView *h_view = nullptr;  // pinned staging for ColView uploads
   size_t h_view_cap = 0;
-  ebi::ColView *d_views = nullptr;  // device ColView array for batches
-  size_t d_views_cap = 0;
+  ebi::DevBuf batch_args;       // device ColView array + prefixes for batches
+  uint8_t *h_batch = nullptr;     // pinned staging for batch arguments
+  size_t h_batch_cap = 0;
   ebi::DevBuf scratch_values;   // decoded f64 scratch for queries
   ebi::DevBuf scratch_bits;     // bitsets
   ebi::DevBuf scratch_misc;     // partials, scans

@@ -428,21 +428,6 @@ std::vector<uint8_t> enc_elf(const double *v, size_t n) {
 }
 
 // ------------------------------------------------------------------- BUFF
-// compression_common/buff.rs:10-52 fixed-point conversion.
-int64_t buff_fixed(double v, int32_t frac) {
-  uint64_t b = f64_bits(v);
-  int32_t exp = (int32_t)((b >> 52) & 0x7ff) - 1023;
-  if (exp < -frac) return 0;
-  int32_t sh = 63 - exp - frac;
-  uint64_t fixed;
-  if (sh >= 64)
-    fixed = 0;
-  else
-    fixed = ((b << 11) | (1ull << 63)) >> ((uint64_t)(int64_t)sh & 63);
-  if (b >> 63) fixed = ~(fixed - 1);
-  return (int64_t)fixed;
-}
-
 const uint64_t kPrecisionMap[13] = {0, 5, 8, 11, 15, 18, 21, 25, 28, 31, 35, 38, 50};
 
 int buff_frac_bits(uint64_t scale, uint32_t *frac) {

@@ -98,6 +98,20 @@ inline int config_size_of(int scheme) {
   }
 }
 
+// BUFF fixed-point conversion, compression_common/buff.rs:10-52: the value's
+// mantissa shifted to `frac` fractional bits (truncating), negated for
+// negative values; 0 below 2^-frac.  Shift amounts wrap as in the reference's
+// release build.
+inline int64_t buff_fixed(double v, int32_t frac) {
+  const uint64_t b = f64_bits(v);
+  const int32_t exp = (int32_t)((b >> 52) & 0x7ff) - 1023;
+  if (exp < -frac) return 0;
+  const int32_t sh = 63 - exp - frac;
+  uint64_t fixed = sh >= 64 ? 0 : ((b << 11) | (1ull << 63)) >> ((uint64_t)(int64_t)sh & 63);
+  if (b >> 63) fixed = ~(fixed - 1);
+  return (int64_t)fixed;
+}
+
 struct ChunkGeom {
   uint64_t physical_offset;
   uint64_t size;            // next physical offset (or footer_offset) - physical_offset

@@ -1,20 +1,20 @@
 // Decode kernels of libebi_hip (gfx950) and their host launchers.
 //
 // Launch shapes:
-//  * k_tiles    -- 256 threads = 4 independent waves; each wave takes wave
-//                  jobs (one 512-record tile) from a device job list, first
-//                  statically then from an atomic counter, so one launch
-//                  covers every tile of every chunk and column in a batch;
+//  * k_decode   -- persistent, 256 threads = 4 independent waves; each wave
+//                  takes wave jobs (a 512-record tile of a tile codec, or 64
+//                  checkpoint segments of an XOR chunk) from the job space of
+//                  every column in the launch, first by id then from an
+//                  atomic counter;
 //  * k_serial   -- 64 threads; lane l walks chunk (job.chunk + l) serially
 //                  (Gorilla, Chimp, Elf); also the load-time index pass that
 //                  writes XOR checkpoints;
 //  * k_chimp128 -- 64 threads with a 64 KiB LDS ring (128 slots per lane);
-//  * k_segments -- 256 threads; lane l of a wave restarts from checkpoint
-//                  (job.tile + l) and decodes its kSeg records;
 //  * k_tile_sums / k_tile_scan -- the load-time tile index of DeltaSprintz
 //                  (prefix q per tile) and RLE (first output record per tile).
 #include "codecs.hpp"
 #include "launch.hpp"
+#include "query.hpp"
 
 namespace ebi {
 
@@ -24,40 +24,99 @@ __device__ __forceinline__ uint32_t next_job(uint32_t *ctr, uint32_t stride) {
   return __builtin_amdgcn_readfirstlane(__shfl(j, 0, kWave));
 }
 
-__global__ void __launch_bounds__(256) k_tiles(const ColView *cols, const Job *jobs, uint32_t n_jobs,
-                                              uint32_t *ctr, uint32_t *err) {
+// Column c with prefix[c] <= j < prefix[c + 1] (wave-uniform binary search).
+__device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_cols, uint32_t j) {
+  uint32_t lo = 0, hi = n_cols;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= j) lo = mid; else hi = mid;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// One XOR checkpoint segment per lane: lane l restarts from checkpoint
+// (first + l) of chunk ci and decodes its kSeg records.
+__device__ __forceinline__ uint32_t segment_lane(const ColView &cv, uint32_t ci, uint32_t first) {
+  const DevChunk c = cv.chunks[ci];
+  const uint32_t seg = first + lane_id();
+  const uint32_t i0 = seg * kSeg;
+  if (i0 >= c.n) return 0;
+  const uint32_t i1 = min(i0 + kSeg, c.n);
+  const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[seg];
+  const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
+  double *out = cv.out + c.rec_off;
+  const uint32_t bits = c.size * 8u;
+  switch (c.codec) {
+    case kCodecGorilla:
+      return gorilla_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a, k.b, out, nullptr);
+    case kCodecChimp:
+      return chimp_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a == 255 ? 0xffffffffu : k.a, out,
+                             nullptr);
+    case kCodecElf:
+      return elf_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a == 255 ? 0xffffffffu : k.a,
+                           k.b == 255 ? 0xffffffffu : k.b, k.beta < 0 ? INT32_MAX : (int32_t)k.beta, out,
+                           nullptr);
+    default:
+      return kErrCorrupt;
+  }
+}
+
+// Persistent decode over the job space of n_cols columns: tile jobs of the
+// tile codecs and checkpoint-segment jobs of Gorilla / Chimp / Elf, mixed
+// codecs and uneven chunks in one launch.  Waves take their first job by id
+// and the rest from an atomic counter.
+__global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
+                                               uint32_t n_jobs, uint32_t *ctr, uint32_t *err) {
   __shared__ uint32_t lds_starts[4][kTile + 1];
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
   uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
   while (j < n_jobs) {
-    const Job jb = jobs[j];
-    const ColView cv = cols[jb.col];
-    const DevChunk c = cv.chunks[jb.chunk];
-    if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-      switch (c.codec) {
-        case kCodecPlain: plain_tile(c, jb.tile, cv.out); break;
-        case kCodecBUFF: buff_tile(c, jb.tile, cv.out); break;
-        case kCodecSprintz: sprintz_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out); break;
-        case kCodecRLE:
-          rle_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out, lds_starts[wave]);
-          break;
-        default:
-          if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
-          break;
+    const uint32_t ci = find_col(prefix, n_cols, j);
+    const ColView cv = cols[ci];
+    const uint32_t local = j - prefix[ci];
+    if (local < cv.n_tile_jobs) {
+      const Job jb = cv.tile_jobs[local];
+      const DevChunk c = cv.chunks[jb.chunk];
+      if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
+        switch (c.codec) {
+          case kCodecPlain: plain_tile(c, jb.tile, cv.out); break;
+          case kCodecBUFF: buff_tile(c, jb.tile, cv.out); break;
+          case kCodecSprintz: sprintz_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out); break;
+          case kCodecRLE: rle_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out, lds_starts[wave]); break;
+          default:
+            if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
+            break;
+        }
+      }
+    } else {
+      const Job jb = cv.seg_jobs[local - cv.n_tile_jobs];
+      if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
+        const uint32_t e = segment_lane(cv, jb.chunk, jb.tile);
+        if (e) atomicOr(err, e);
       }
     }
     j = next_job(ctr, stride);
   }
 }
 
-// kWrite: decode to cv.out; kIndex: write checkpoints to c.xidx.
+// kWrite: decode to cv.out; kIndex: write checkpoints to c.xidx.  Block b
+// runs serial job b of the launch-wide list (prefix over n_serial_jobs).
 template <bool kWrite, bool kIndex>
-__global__ void __launch_bounds__(64) k_serial(const ColView *cols, const Job *jobs, uint32_t *err) {
-  const Job jb = jobs[blockIdx.x];
+__global__ void __launch_bounds__(64) k_serial(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
+                                              const Job *single, uint32_t *err) {
+  Job jb;
+  ColView cv;
+  if (single) {  // one explicit job on column 0 (per-chunk comparator)
+    jb = *single;
+    cv = cols[0];
+  } else {
+    const uint32_t ci = find_col(prefix, n_cols, blockIdx.x);
+    cv = cols[ci];
+    jb = cv.serial_jobs[blockIdx.x - prefix[ci]];
+  }
   const uint32_t lane = threadIdx.x;
   if (lane >= jb.tile) return;
-  const ColView cv = cols[jb.col];
   const uint32_t ci = jb.chunk + lane;
   const DevChunk c = cv.chunks[ci];
   if (!kIndex && cv.chunk_err && cv.chunk_err[ci]) return;
@@ -81,12 +140,21 @@ __global__ void __launch_bounds__(64) k_serial(const ColView *cols, const Job *j
 }
 
 template <bool kWrite>
-__global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const Job *jobs, uint32_t *err) {
+__global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
+                                                const Job *single, uint32_t *err) {
   __shared__ uint64_t ring[128 * kWave];
-  const Job jb = jobs[blockIdx.x];
+  Job jb;
+  ColView cv;
+  if (single) {
+    jb = *single;
+    cv = cols[0];
+  } else {
+    const uint32_t ci = find_col(prefix, n_cols, blockIdx.x);
+    cv = cols[ci];
+    jb = cv.serial_jobs[blockIdx.x - prefix[ci]];
+  }
   const uint32_t lane = threadIdx.x;
   if (lane >= jb.tile) return;
-  const ColView cv = cols[jb.col];
   const uint32_t ci = jb.chunk + lane;
   const DevChunk c = cv.chunks[ci];
   if (kWrite && cv.chunk_err && cv.chunk_err[ci]) return;
@@ -96,37 +164,6 @@ __global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const Job 
     atomicOr(err, e);
     if (cv.chunk_err) atomicOr(cv.chunk_err + ci, e);
   }
-}
-
-__global__ void __launch_bounds__(256) k_segments(const ColView *cols, const Job *jobs, uint32_t n_jobs,
-                                                 uint32_t *err) {
-  const uint32_t j = blockIdx.x * 4 + threadIdx.x / kWave;
-  if (j >= n_jobs) return;
-  const Job jb = jobs[j];
-  const ColView cv = cols[jb.col];
-  const DevChunk c = cv.chunks[jb.chunk];
-  if (cv.chunk_err && cv.chunk_err[jb.chunk]) return;
-  const uint32_t seg = jb.tile + lane_id();
-  const uint32_t i0 = seg * kSeg;
-  if (i0 >= c.n) return;
-  const uint32_t i1 = min(i0 + kSeg, c.n);
-  const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[seg];
-  const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
-  double *out = cv.out + c.rec_off;
-  const uint32_t bits = c.size * 8u;
-  uint32_t e;
-  switch (c.codec) {
-    case kCodecGorilla: e = gorilla_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a, k.b, out, nullptr); break;
-    case kCodecChimp:
-      e = chimp_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a == 255 ? 0xffffffffu : k.a, out, nullptr);
-      break;
-    case kCodecElf:
-      e = elf_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a == 255 ? 0xffffffffu : k.a,
-                        k.b == 255 ? 0xffffffffu : k.b, k.beta < 0 ? INT32_MAX : (int32_t)k.beta, out, nullptr);
-      break;
-    default: e = kErrCorrupt; break;
-  }
-  if (e) atomicOr(err, e);
 }
 
 // Tile sums for the load-time index: DeltaSprintz sum of deltas, RLE sum of runs.
@@ -191,38 +228,33 @@ void zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
   if (n) hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
 }
 
-void tiles(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *ctr,
-           uint32_t *err, uint32_t max_blocks) {
+void decode(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
+            uint32_t *ctr, uint32_t *err, uint32_t max_blocks) {
   if (!n_jobs) return;
   uint32_t blocks = (n_jobs + 3) / 4;
   if (max_blocks && blocks > max_blocks) blocks = max_blocks;
   zero_u32(s, ctr, 1);
-  hipLaunchKernelGGL(k_tiles, dim3(blocks), dim3(256), 0, s, cols, jobs, n_jobs, ctr, err);
+  hipLaunchKernelGGL(k_decode, dim3(blocks), dim3(256), 0, s, cols, prefix, n_cols, n_jobs, ctr, err);
 }
 
-void serial(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *err,
-            bool write, bool index) {
+void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
+            const Job *single, uint32_t *err, bool write, bool index) {
   if (!n_jobs) return;
   if (write && index)
-    hipLaunchKernelGGL((k_serial<true, true>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
+    hipLaunchKernelGGL((k_serial<true, true>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
   else if (write)
-    hipLaunchKernelGGL((k_serial<true, false>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
+    hipLaunchKernelGGL((k_serial<true, false>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
   else
-    hipLaunchKernelGGL((k_serial<false, true>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
+    hipLaunchKernelGGL((k_serial<false, true>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
 }
 
-void chimp128(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *err,
-              bool write) {
+void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
+              const Job *single, uint32_t *err, bool write) {
   if (!n_jobs) return;
   if (write)
-    hipLaunchKernelGGL((k_chimp128<true>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
+    hipLaunchKernelGGL((k_chimp128<true>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
   else
-    hipLaunchKernelGGL((k_chimp128<false>), dim3(n_jobs), dim3(64), 0, s, cols, jobs, err);
-}
-
-void segments(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *err) {
-  if (!n_jobs) return;
-  hipLaunchKernelGGL(k_segments, dim3((n_jobs + 3) / 4), dim3(256), 0, s, cols, jobs, n_jobs, err);
+    hipLaunchKernelGGL((k_chimp128<false>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
 }
 
 void tile_index(hipStream_t s, const ColView *cols, uint32_t col, const Job *jobs, uint32_t n_jobs,

@@ -13,19 +13,32 @@ namespace launch {
 hipError_t set_pow10(const double *p10, const double *n10);
 void zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 
-// Decode: tile jobs (plain, BUFF, Sprintz, RLE) in one persistent-style launch.
-void tiles(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *ctr,
-           uint32_t *err, uint32_t max_blocks);
-// Lane-per-chunk serial XOR decode (Gorilla, Chimp, Elf); `index` also writes checkpoints.
-void serial(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *err,
-            bool write, bool index);
-void chimp128(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *err,
-              bool write);
-// Checkpoint-segment XOR decode.
-void segments(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, uint32_t *err);
+// Persistent decode over the tile + segment job space of n_cols columns
+// (prefix has n_cols + 1 entries).
+void decode(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
+            uint32_t *ctr, uint32_t *err, uint32_t max_blocks);
+// Lane-per-chunk serial XOR decode (Gorilla, Chimp, Elf); `index` writes
+// checkpoints.  prefix is over n_serial_jobs; `single` (device, nullable)
+// runs one explicit job on cols[0] instead.
+void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
+            const Job *single, uint32_t *err, bool write, bool index);
+void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
+              const Job *single, uint32_t *err, bool write);
 // Load-time tile index for DeltaSprintz and RLE chunks of column slot `col`.
 void tile_index(hipStream_t s, const ColView *cols, uint32_t col, const Job *jobs, uint32_t n_jobs,
                 uint32_t n_chunks, uint32_t *err);
+
+// Queries (record-tile jobs; single-column views at cols[0]).
+void chunk_select(hipStream_t s, const ColView *cols, const uint64_t *mask, int64_t chunk_id, uint32_t *sel,
+                  uint32_t n_chunks);
+void filter(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a, int mode);
+void aggregate(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a,
+               uint32_t n_chunks);
+void chunk_dd(hipStream_t s, const ColView *cols, const uint32_t *sel, const double *partials, double *chunk_sum,
+              uint32_t n_chunks);
+uint32_t compact_blocks(uint64_t n_words);
+void compact(hipStream_t s, const uint64_t *bits, uint64_t n_words, const double *values, double *out,
+             uint64_t *block_scratch, uint64_t *n_out);
 
 }  // namespace launch
 }  // namespace ebi

@@ -1,0 +1,430 @@
+// Query kernels of libebi_hip (gfx950): filters to a bitset, aggregates and
+// stream compaction.  Included once, by decode.hip (one device TU).
+//
+// Work unit: a record tile (kTile = 512 records of one chunk, 8 consecutive
+// records per lane).  BUFF and DeltaSprintz evaluate predicates and
+// aggregates in their code domain (plane deltas, quantized q); the other
+// codecs work on records decoded into HBM.  Semantics follow the reference
+// readers cited on each function (paths relative to lemolatoon/ebi).
+#pragma once
+
+#include "codecs.hpp"
+
+namespace ebi {
+
+// ------------------------------------------------------------ bit helpers
+__device__ __forceinline__ uint32_t mask_byte(const uint64_t *mask, uint64_t g, uint32_t cnt) {
+  // bits of records [g, g + cnt) of the global bitset, packed LSB-first
+  if (!mask) return (1u << cnt) - 1;
+  const uint64_t w = g >> 6;
+  const uint32_t sh = (uint32_t)(g & 63);
+  uint64_t v = mask[w] >> sh;
+  if (sh > 56) v |= mask[w + 1] << (64 - sh);
+  return (uint32_t)v & ((1u << cnt) - 1);
+}
+
+// Write this lane's 8 result bits (records [g, g + 8) of one chunk; bits of
+// records past the chunk end are 0).  A fully owned, byte-aligned group is
+// stored as a byte; groups that share a byte with another chunk use 32-bit
+// atomic ORs.  The output bitset is zeroed before the launch.
+__device__ __forceinline__ void put_bits(uint64_t *out, uint64_t g, uint32_t bits, bool owned) {
+  if (owned) {
+    reinterpret_cast<uint8_t *>(out)[g >> 3] = (uint8_t)bits;
+    return;
+  }
+  if (!bits) return;
+  uint32_t *w = reinterpret_cast<uint32_t *>(out);
+  const uint64_t wi = g >> 5;
+  const uint32_t sh = (uint32_t)(g & 31);
+  const uint64_t v = (uint64_t)bits << sh;
+  if ((uint32_t)v) atomicOr(w + wi, (uint32_t)v);
+  if ((uint32_t)(v >> 32)) atomicOr(w + wi + 1, (uint32_t)(v >> 32));
+}
+
+// Predicate::eval on f64 (decoder/query.rs:374-433).
+__device__ __forceinline__ bool pred_f64(const QueryArgs &a, double v) {
+  if (a.kind == 0) return v == a.value;
+  if (a.kind == 1) return v != a.value;
+  bool s = true, e = true;
+  if (a.start_kind == 1) s = v >= a.start;
+  else if (a.start_kind == 2) s = v > a.start;
+  if (a.end_kind == 1) e = v <= a.end;
+  else if (a.end_kind == 2) e = v < a.end;
+  return s && e;
+}
+
+// Sprintz predicate in the quantized domain (chunk_reader/sprintz.rs:116):
+// lmode/rmode carry the bound kinds, ldp/rdp the quantized operands.
+__device__ __forceinline__ bool pred_q(const QueryArgs &a, const ChunkPred &p, int64_t q) {
+  if (a.kind == 0) return q == (int64_t)p.ldp;
+  if (a.kind == 1) return q != (int64_t)p.ldp;
+  bool s = true, e = true;
+  if (p.lmode == 1) s = q >= (int64_t)p.ldp;
+  else if (p.lmode == 2) s = q > (int64_t)p.ldp;
+  if (p.rmode == 1) e = q <= (int64_t)p.rdp;
+  else if (p.rmode == 2) e = q < (int64_t)p.rdp;
+  return s && e;
+}
+
+// BUFF filter on one record (decoder/chunk_reader/buff.rs:128-228 with
+// buff/filter_cmp.rs:298-451 and buff/filter_eq.rs:73-201).  `m` is the
+// record's bitmask bit (1 if no bitmask), `has_mask` whether one was given.
+__device__ __forceinline__ bool buff_pred(const QueryArgs &a, const ChunkPred &p, uint64_t d, bool m) {
+  if (a.kind <= 1) {
+    bool eq;
+    if (p.lmode == kSideAll) eq = true;           // B == 0 and pred == base: all(), mask ignored
+    else if (p.lmode == kSideEmpty) eq = false;
+    else eq = m && (p.eq_u8 ? d == (p.ldp & 0xff) : d == p.ldp);
+    return a.kind == 0 ? eq : (m && !eq);
+  }
+  bool l = true, r = true;
+  const bool has_l = p.lmode != kSideNone, has_r = p.rmode != kSideNone;
+  if (has_l) {
+    if (p.lmode == kSideAll) l = true;
+    else if (p.lmode == kSideEmpty) l = false;
+    else l = m && (d > p.ldp || (p.lincl && d == p.ldp));
+  }
+  if (has_r) {
+    if (p.rmode == kSideAll) r = true;
+    else if (p.rmode == kSideEmpty) r = false;
+    else r = m && (d < p.rdp || (p.rincl && d == p.rdp));
+  }
+  if (!has_l && !has_r) return m;
+  return l && r;
+}
+
+// ---------------------------------------------------------------- filters
+// mode 0: select (bit = mask); 1: predicate, in the code domain for BUFF and
+// DeltaSprintz; 2: predicate on decoded f64 records.  One wave per record tile.
+__global__ void __launch_bounds__(256) k_filter(const ColView *cols, const Job *jobs, uint32_t n_jobs,
+                                               QueryArgs a, int mode) {
+  const uint32_t j = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (j >= n_jobs) return;
+  const Job jb = jobs[j];
+  const ColView cv = cols[jb.col];
+  if (!a.chunk_sel[jb.chunk]) return;
+  const DevChunk c = cv.chunks[jb.chunk];
+  const uint32_t lane = lane_id();
+  const uint32_t i0 = jb.tile * kTile + lane * 8;
+  // Sprintz needs every lane in the wave scan, so no early return before it.
+  int64_t q[8];
+  if (mode == 1 && c.codec == kCodecSprintz) sprintz_q(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], q);
+  if (i0 >= c.n) return;
+  const uint32_t cnt = min(8u, c.n - i0);
+  const uint64_t g = c.rec_off + i0;
+  const uint32_t m = mask_byte(a.mask, g, cnt);
+  uint32_t bits = 0;
+  if (mode == 0) {
+    bits = m;
+  } else if (mode == 1 && c.codec == kCodecBUFF) {
+    const ChunkPred p = a.cpred[jb.chunk];
+    uint64_t d[8];
+    buff_deltas(c, i0, d);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if ((uint32_t)k < cnt && buff_pred(a, p, d[k], (m >> k) & 1)) bits |= 1u << k;
+  } else if (mode == 1 && c.codec == kCodecSprintz) {
+    const ChunkPred p = a.cpred[jb.chunk];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if ((uint32_t)k < cnt && ((m >> k) & 1) && pred_q(a, p, q[k])) bits |= 1u << k;
+  } else {
+    const double *v = a.values + g;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if ((uint32_t)k < cnt && ((m >> k) & 1) && pred_f64(a, v[k])) bits |= 1u << k;
+  }
+  put_bits(a.out_bits, g, bits, (g & 7) == 0 && cnt == 8);
+}
+
+// ------------------------------------------------------------- aggregates
+// Order-preserving u64 keys.
+__device__ __forceinline__ uint64_t f64_key(double x) {
+  const uint64_t b = as_u64(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ uint64_t i64_key(int64_t x) { return (uint64_t)x ^ 0x8000000000000000ull; }
+
+// Double-double addition (Knuth two-sum), deterministic for a fixed order.
+__device__ __forceinline__ void dd_add(double &hi, double &lo, double bh, double bl) {
+  const double s = hi + bh;
+  if (!isfinite(s)) {  // infinities and NaN propagate as in a plain sum
+    hi = s;
+    lo = 0.0;
+    return;
+  }
+  const double v = s - hi;
+  const double e = (hi - (s - v)) + (bh - v);
+  const double t = e + lo + bl;
+  hi = s + t;
+  lo = t - (hi - s);
+}
+
+// op: 0 sum, 1 min, 2 max.  Per-chunk accumulators acc[4*chunk + {sum, count,
+// min key, max key}] hold exact integer sums for BUFF (plane deltas,
+// buff/sum.rs:60-119) and DeltaSprintz (quantized q, sprintz.rs:267-351) and
+// keys for min/max; f64 sums of the other codecs go to per-tile
+// double-double partials.
+__global__ void __launch_bounds__(256) k_aggregate(const ColView *cols, const Job *jobs, uint32_t n_jobs,
+                                                  QueryArgs a) {
+  const uint32_t j = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (j >= n_jobs) return;
+  const Job jb = jobs[j];
+  const ColView cv = cols[jb.col];
+  if (!a.chunk_sel[jb.chunk]) return;
+  const DevChunk c = cv.chunks[jb.chunk];
+  const uint32_t lane = lane_id();
+  const uint32_t i0 = jb.tile * kTile + lane * 8;
+  const uint32_t cnt = i0 < c.n ? min(8u, c.n - i0) : 0u;
+  const uint64_t g = c.rec_off + i0;
+  const uint32_t m = cnt ? mask_byte(a.mask, g, cnt) : 0u;
+  uint64_t *acc = a.acc + 4ull * jb.chunk;
+  if (c.codec == kCodecBUFF || c.codec == kCodecSprintz) {
+    uint64_t s = 0, n = 0, mn = ~0ull, mx = 0;
+    if (c.codec == kCodecBUFF) {
+      uint64_t d[8];
+      buff_deltas(c, i0, d);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if ((m >> k) & 1) {
+          s += d[k];
+          ++n;
+          mn = min(mn, d[k]);
+          mx = max(mx, d[k]);
+        }
+    } else {
+      int64_t q[8];
+      sprintz_q(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], q);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if ((m >> k) & 1) {
+          s += (uint64_t)q[k];
+          ++n;
+          mn = min(mn, i64_key(q[k]));
+          mx = max(mx, i64_key(q[k]));
+        }
+    }
+    if (a.op == 0) {
+      s = wave_sum_u64(s);
+      n = wave_sum_u64(n);
+      if (lane == 0 && n) {
+        atomicAdd((unsigned long long *)acc, (unsigned long long)s);
+        atomicAdd((unsigned long long *)acc + 1, (unsigned long long)n);
+      }
+    } else {
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        mn = min(mn, (uint64_t)__shfl_xor((unsigned long long)mn, d, kWave));
+        mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, d, kWave));
+      }
+      if (lane == 0) {
+        if (a.op == 1) atomicMin((unsigned long long *)acc + 2, (unsigned long long)mn);
+        else atomicMax((unsigned long long *)acc + 3, (unsigned long long)mx);
+      }
+    }
+    return;
+  }
+  // default codecs: decoded records
+  const double *v = a.values + g;
+  if (a.op == 0) {
+    double hi = 0.0, lo = 0.0;
+    for (int k = 0; k < 8; ++k)
+      if ((m >> k) & 1) dd_add(hi, lo, v[k], 0.0);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const double oh = __shfl_xor(hi, d, kWave), ol = __shfl_xor(lo, d, kWave);
+      // combine in a lane-independent order so every lane gets the same pair
+      if ((lane & d) == 0) dd_add(hi, lo, oh, ol);
+      else {
+        double h2 = oh, l2 = ol;
+        dd_add(h2, l2, hi, lo);
+        hi = h2;
+        lo = l2;
+      }
+    }
+    if (lane == 0) {
+      double *p = a.partials + 2ull * (c.rtile0 + jb.tile);
+      p[0] = hi;
+      p[1] = lo;
+    }
+  } else {
+    uint64_t mn = ~0ull, mx = 0;
+    for (int k = 0; k < 8; ++k)
+      if (((m >> k) & 1) && !(v[k] != v[k])) {  // Rust f64::min/max ignore NaN
+        mn = min(mn, f64_key(v[k]));
+        mx = max(mx, f64_key(v[k]));
+      }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      mn = min(mn, (uint64_t)__shfl_xor((unsigned long long)mn, d, kWave));
+      mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, d, kWave));
+    }
+    if (lane == 0) {
+      if (a.op == 1) atomicMin((unsigned long long *)acc + 2, (unsigned long long)mn);
+      else atomicMax((unsigned long long *)acc + 3, (unsigned long long)mx);
+    }
+  }
+}
+
+// Per-chunk accumulator init: sum 0, count 0, min key max, max key 0.
+__global__ void k_acc_init(uint64_t *acc, uint32_t n_chunks) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_chunks) return;
+  acc[4ull * i] = 0;
+  acc[4ull * i + 1] = 0;
+  acc[4ull * i + 2] = ~0ull;
+  acc[4ull * i + 3] = 0;
+}
+
+// One thread per chunk: fold its tile partials in tile order (double-double).
+__global__ void k_chunk_dd(const ColView *cols, const uint32_t *chunk_sel, const double *partials,
+                           double *chunk_sum, uint32_t n_chunks) {
+  const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= n_chunks) return;
+  double hi = 0.0, lo = 0.0;
+  if (chunk_sel[ci]) {
+    const DevChunk c = cols[0].chunks[ci];
+    const uint32_t nt = (c.n + kTile - 1) / kTile;
+    for (uint32_t t = 0; t < nt; ++t) dd_add(hi, lo, partials[2ull * (c.rtile0 + t)], partials[2ull * (c.rtile0 + t) + 1]);
+  }
+  chunk_sum[2ull * ci] = hi;
+  chunk_sum[2ull * ci + 1] = lo;
+}
+
+// Chunk selection: chunk_id filter and bitmask pruning (api/decoder.rs:216-226).
+// One wave per chunk ORs the bitmask words of its record range.
+__global__ void __launch_bounds__(256) k_chunk_select(const ColView *cols, const uint64_t *mask, int64_t chunk_id,
+                                                     uint32_t *sel, uint32_t n_chunks) {
+  const uint32_t ci = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (ci >= n_chunks) return;
+  const uint32_t lane = lane_id();
+  uint32_t s = (chunk_id < 0 || (uint64_t)chunk_id == ci) ? 1u : 0u;
+  if (s && mask) {
+    const DevChunk c = cols[0].chunks[ci];
+    const uint64_t a = c.rec_off, b = c.rec_off + c.n;  // [a, b)
+    uint64_t any = 0;
+    for (uint64_t w = (a >> 6) + lane; w <= ((b - 1) >> 6) && b > a; w += kWave) {
+      uint64_t x = mask[w];
+      const uint64_t lo = w << 6;
+      if (a > lo) x &= ~0ull << (a - lo);
+      if (b < lo + 64) x &= (1ull << (b - lo)) - 1;
+      any |= x;
+    }
+    s = __any(any != 0) ? 1u : 0u;
+  }
+  if (lane == 0) sel[ci] = s;
+}
+
+// --------------------------------------------------------------- compaction
+// Selected records of `values` (bit r of `bits` set) are written in record
+// order to `out`.  Block b covers words [b*4096, (b+1)*4096).
+constexpr uint32_t kScanWords = 16;  // words per thread
+constexpr uint32_t kScanBlock = 256;
+
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t x, uint64_t *lds, uint64_t *total) {
+  const uint32_t t = threadIdx.x, wv = t / kWave;
+  uint64_t incl = x;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint64_t y = __shfl_up((unsigned long long)incl, d, kWave);
+    if ((int)lane_id() >= d) incl += y;
+  }
+  if (lane_id() == kWave - 1) lds[wv] = incl;
+  __syncthreads();
+  uint64_t off = 0, tot = 0;
+  for (uint32_t k = 0; k < kScanBlock / kWave; ++k) {
+    if (k < wv) off += lds[k];
+    tot += lds[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - x;
+}
+
+__global__ void __launch_bounds__(256) k_popc_blocks(const uint64_t *bits, uint64_t n_words, uint64_t *block_cnt) {
+  __shared__ uint64_t lds[kScanBlock / kWave];
+  const uint64_t w0 = ((uint64_t)blockIdx.x * kScanBlock + threadIdx.x) * kScanWords;
+  uint64_t c = 0;
+  for (uint32_t k = 0; k < kScanWords; ++k)
+    if (w0 + k < n_words) c += __popcll(bits[w0 + k]);
+  uint64_t total;
+  block_excl_scan(c, lds, &total);
+  if (threadIdx.x == 0) block_cnt[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of block counts in place; total to *n_out.
+__global__ void __launch_bounds__(256) k_scan_counts(uint64_t *cnt, uint32_t n, uint64_t *n_out) {
+  __shared__ uint64_t lds[kScanBlock / kWave];
+  uint64_t carry = 0;
+  for (uint32_t base = 0; base < n; base += kScanBlock) {
+    const uint32_t i = base + threadIdx.x;
+    const uint64_t x = i < n ? cnt[i] : 0;
+    uint64_t total;
+    const uint64_t ex = block_excl_scan(x, lds, &total);
+    if (i < n) cnt[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) *n_out = carry;
+}
+
+__global__ void __launch_bounds__(256) k_scatter(const uint64_t *bits, uint64_t n_words, const uint64_t *block_off,
+                                                const double *values, double *out) {
+  __shared__ uint64_t lds[kScanBlock / kWave];
+  const uint64_t w0 = ((uint64_t)blockIdx.x * kScanBlock + threadIdx.x) * kScanWords;
+  uint64_t c = 0;
+  for (uint32_t k = 0; k < kScanWords; ++k)
+    if (w0 + k < n_words) c += __popcll(bits[w0 + k]);
+  uint64_t total;
+  uint64_t pos = block_off[blockIdx.x] + block_excl_scan(c, lds, &total);
+  for (uint32_t k = 0; k < kScanWords; ++k) {
+    if (w0 + k >= n_words) break;
+    uint64_t x = bits[w0 + k];
+    while (x) {
+      const uint32_t b = __builtin_ctzll(x);
+      out[pos++] = values[((w0 + k) << 6) + b];
+      x &= x - 1;
+    }
+  }
+}
+
+namespace launch {
+
+void chunk_select(hipStream_t s, const ColView *cols, const uint64_t *mask, int64_t chunk_id, uint32_t *sel,
+                  uint32_t n_chunks) {
+  if (n_chunks)
+    hipLaunchKernelGGL(k_chunk_select, dim3((n_chunks + 3) / 4), dim3(256), 0, s, cols, mask, chunk_id, sel, n_chunks);
+}
+
+void filter(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a, int mode) {
+  if (n_jobs) hipLaunchKernelGGL(k_filter, dim3((n_jobs + 3) / 4), dim3(256), 0, s, cols, jobs, n_jobs, a, mode);
+}
+
+void aggregate(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a,
+               uint32_t n_chunks) {
+  if (n_chunks) hipLaunchKernelGGL(k_acc_init, dim3((n_chunks + 255) / 256), dim3(256), 0, s, a.acc, n_chunks);
+  if (n_jobs) hipLaunchKernelGGL(k_aggregate, dim3((n_jobs + 3) / 4), dim3(256), 0, s, cols, jobs, n_jobs, a);
+}
+
+void chunk_dd(hipStream_t s, const ColView *cols, const uint32_t *sel, const double *partials, double *chunk_sum,
+              uint32_t n_chunks) {
+  if (n_chunks)
+    hipLaunchKernelGGL(k_chunk_dd, dim3((n_chunks + 255) / 256), dim3(256), 0, s, cols, sel, partials, chunk_sum,
+                       n_chunks);
+}
+
+uint32_t compact_blocks(uint64_t n_words) {
+  const uint64_t per = (uint64_t)kScanBlock * kScanWords;
+  return (uint32_t)((n_words + per - 1) / per);
+}
+
+void compact(hipStream_t s, const uint64_t *bits, uint64_t n_words, const double *values, double *out,
+             uint64_t *block_scratch, uint64_t *n_out) {
+  const uint32_t nb = compact_blocks(n_words);
+  if (!nb) return;
+  hipLaunchKernelGGL(k_popc_blocks, dim3(nb), dim3(kScanBlock), 0, s, bits, n_words, block_scratch);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanBlock), 0, s, block_scratch, nb, n_out);
+  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kScanBlock), 0, s, bits, n_words, block_scratch, values, out);
+}
+
+}  // namespace launch
+}  // namespace ebi

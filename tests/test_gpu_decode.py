@@ -65,3 +65,23 @@ def test_decode_per_chunk_comparator(ctx, codec):
 def test_decode_single_record_and_constant(ctx, codec):
     _check(ctx, np.array([42.5]), codec, E.ChunkOption.record_count(8))
     _check(ctx, np.full(3000, 100.19), codec, E.ChunkOption.record_count(700))
+
+
+def test_decode_batch_mixed_codecs(ctx):
+    """One persistent launch over columns of every codec, uneven chunk sizes."""
+    rng = np.random.default_rng(3)
+    decs, outs, wants = [], [], []
+    for i, codec in enumerate(CODECS * 2):
+        n = int(rng.integers(1000, 30000))
+        chunk = int(rng.integers(300, 9000))
+        vals = G.random_walk(n, seed=100 + i)
+        data = E.encode(vals, E.CompressorConfig.by_name(codec, scale=100), E.ChunkOption.record_count(chunk))
+        decs.append(E.Decoder(ctx, data))
+        outs.append(ctx.alloc(8 * n))
+        wants.append(OracleDecoder(data).materialize())
+    E.decode_batch(ctx, decs, outs)
+    for d, o, w in zip(decs, outs, wants):
+        got = o.to_host(np.float64, d.number_of_records)
+        assert np.array_equal(got.view(np.uint64), w.view(np.uint64)), E.SCHEME_NAMES[d.scheme]
+        o.free()
+        d.close()
