@@ -131,7 +131,7 @@ int enqueue_decode(ebi_hip_ctx *ctx, Column &c) {
   launch::decode(ctx->stream, c.d_view, c.d_prefix, 1, jobs, ctx->d_flags + 1, ctx->d_flags,
                  (uint32_t)ctx->cu_count * 8);
   if (c.chimp128)
-    launch::chimp128(ctx->stream, c.d_view, c.d_prefix + 2, 1, c.n_serial_jobs, nullptr, ctx->d_flags, true);
+    launch::chimp128_par(ctx->stream, c.d_view, c.d_prefix + 4, 1, (uint32_t)c.chunks.size(), ctx->d_flags);
   else if (c.xor_codec && ctx->xor_mode != 0)
     launch::serial(ctx->stream, c.d_view, c.d_prefix + 2, 1, c.n_serial_jobs, nullptr, ctx->d_flags, true, false);
   hipError_t e = hipGetLastError();
@@ -247,7 +247,8 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
         break;
       }
       default:  // XOR codecs: the bit stream as written
-        h.n_segs = col.chimp128 ? 0 : (n + kSeg - 1) / kSeg;
+        // Chimp128 keeps parse state for records 0..n (the terminator too)
+        h.n_segs = col.chimp128 ? n / kSeg + 1 : (n + kSeg - 1) / kSeg;
         break;
     }
     bodies[k] = {src, dev_bytes, body_len};
@@ -329,8 +330,9 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
     if (!col.xor_codec && !col.chimp128)
       for (uint32_t t = 0; t < col.chunks[k].n_tiles; ++t)
         col.h_tile_jobs.push_back(Job{kJobTile, 0, (uint32_t)k, t});
-    for (uint32_t s = 0; s < col.chunks[k].n_segs; s += kWaveLanes)
-      col.h_seg_jobs.push_back(Job{kJobSeg, 0, (uint32_t)k, s});
+    if (!col.chimp128)
+      for (uint32_t s = 0; s < col.chunks[k].n_segs; s += kWaveLanes)
+        col.h_seg_jobs.push_back(Job{kJobSeg, 0, (uint32_t)k, s});
   }
   col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
   std::vector<Job> rec_jobs;
@@ -355,7 +357,7 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
     return hip_fail(ctx, e, "upload job lists");
 
   {
-    const std::vector<uint32_t> pre = {0, col.n_tile_jobs + col.n_seg_jobs, 0, col.n_serial_jobs};
+    const std::vector<uint32_t> pre = {0, col.n_tile_jobs + col.n_seg_jobs, 0, col.n_serial_jobs, 0, (uint32_t)nc};
     if ((e = upload_vec(&col.d_prefix, pre)) != hipSuccess) return hip_fail(ctx, e, "upload prefix");
   }
   // Load-time index pass.
@@ -364,11 +366,9 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   if ((rc = upload_view(ctx, col, nullptr))) return rc;
   if (scheme == kDeltaSprintz || scheme == kRLE)
     launch::tile_index(ctx->stream, col.d_view, 0, col.d_tile_jobs, col.n_tile_jobs, (uint32_t)nc, ctx->d_flags);
-  if (col.xor_codec)
+  if (col.xor_codec || col.chimp128)
     launch::serial(ctx->stream, col.d_view, col.d_prefix + 2, 1, col.n_serial_jobs, nullptr, ctx->d_flags, false,
                    true);
-  if (col.chimp128)
-    launch::chimp128(ctx->stream, col.d_view, col.d_prefix + 2, 1, col.n_serial_jobs, nullptr, ctx->d_flags, false);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "index launch");
   std::vector<uint32_t> errs(nc);
   if (nc) {
@@ -602,7 +602,7 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
     if (!cols[i] || (!d_outs[i] && cols[i]->c.meta.number_of_records)) return EBI_INVALID_ARGUMENT;
     if ((rc = check_chunk_errors(ctx, cols[i]->c, -1))) return rc;
   }
-  // arguments: prefix over tile+seg jobs, prefix over Chimp128 serial jobs,
+  // arguments: prefix over tile+seg jobs, prefix over Chimp128 chunks,
   // then the ColView array (256-byte aligned)
   const size_t pre_bytes = ((2 * (n_cols + 1) * 4 + 255) / 256) * 256;
   const size_t bytes = pre_bytes + n_cols * sizeof(ColView);
@@ -625,14 +625,11 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
     pre128[i] = (uint32_t)jobs128;
     views[i] = view_of(c, d_outs[i]);
     jobs += c.n_tile_jobs + c.n_seg_jobs;
-    if (c.chimp128) jobs128 += c.n_serial_jobs;
+    if (c.chimp128) jobs128 += c.chunks.size();
   }
   pre[n_cols] = (uint32_t)jobs;
   pre128[n_cols] = (uint32_t)jobs128;
   if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32)) return fail(ctx, EBI_UNSUPPORTED, "batch too large");
-  // Chimp128 columns take part only through the second prefix
-  for (size_t i = 0; i < n_cols; ++i)
-    if (!cols[i]->c.chimp128) views[i].n_serial_jobs = 0;
   uint8_t *d = (uint8_t *)ctx->batch_args.p;
   hipError_t e = hipMemcpyAsync(d, ctx->h_batch, bytes, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "batch upload");
@@ -641,8 +638,8 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   const ColView *dv = (const ColView *)(d + pre_bytes);
   launch::decode(ctx->stream, dv, (const uint32_t *)d, (uint32_t)n_cols, (uint32_t)jobs, ctx->d_flags + 1,
                  ctx->d_flags, (uint32_t)ctx->cu_count * 8);
-  launch::chimp128(ctx->stream, dv, (const uint32_t *)d + (n_cols + 1), (uint32_t)n_cols, (uint32_t)jobs128,
-                   nullptr, ctx->d_flags, true);
+  launch::chimp128_par(ctx->stream, dv, (const uint32_t *)d + (n_cols + 1), (uint32_t)n_cols, (uint32_t)jobs128,
+                       ctx->d_flags);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "batch launch");
   hipEventRecord(ctx->ev1, ctx->stream);
   if ((rc = sync_and_check(ctx, "batch decode"))) return rc;

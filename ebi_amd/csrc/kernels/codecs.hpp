@@ -58,7 +58,7 @@ __device__ __forceinline__ double elf_round_up(double v, uint32_t alpha) {
 template <bool kWrite>
 __device__ uint32_t gorilla_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
                                 uint32_t i1, uint32_t bitpos, uint64_t v, uint32_t L, uint32_t T,
-                                double *out, XorCkpt *ck) {
+                                LaneSink out, XorCkpt *ck) {
   LaneReader r;
   r.init(data, bitpos);
   const uint32_t last = i1 == n ? n + 1 : i1;
@@ -85,7 +85,7 @@ __device__ uint32_t gorilla_run(const uint8_t *data, uint32_t size_bits, uint32_
     if (r.pos > size_bits) return kErrOverrun;
     if (i == n) return v == kNan ? kErrNone : kErrCount;
     if (v == kNan) return kErrCount;
-    if (kWrite) out[i] = as_f64(v);
+    if (kWrite) out.put(i, as_f64(v));
   }
   return kErrNone;
 }
@@ -95,7 +95,7 @@ __device__ uint32_t gorilla_run(const uint8_t *data, uint32_t size_bits, uint32_
 // before the first 11/01 record).
 template <bool kWrite>
 __device__ uint32_t chimp_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
-                              uint32_t i1, uint32_t bitpos, uint64_t v, uint32_t lead, double *out,
+                              uint32_t i1, uint32_t bitpos, uint64_t v, uint32_t lead, LaneSink out,
                               XorCkpt *ck) {
   LaneReader r;
   r.init(data, bitpos);
@@ -125,7 +125,7 @@ __device__ uint32_t chimp_run(const uint8_t *data, uint32_t size_bits, uint32_t 
     if (r.pos > size_bits) return kErrOverrun;
     if (i == n) return v == kNan ? kErrNone : kErrCount;
     if (v == kNan) return kErrCount;
-    if (kWrite) out[i] = as_f64(v);
+    if (kWrite) out.put(i, as_f64(v));
   }
   return kErrNone;
 }
@@ -137,7 +137,7 @@ __device__ uint32_t chimp_run(const uint8_t *data, uint32_t size_bits, uint32_t 
 template <bool kWrite>
 __device__ uint32_t elf_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
                             uint32_t i1, uint32_t bitpos, uint64_t stored, uint32_t L, uint32_t T,
-                            int32_t beta, double *out, XorCkpt *ck) {
+                            int32_t beta, LaneSink out, XorCkpt *ck) {
   LaneReader r;
   r.init(data, bitpos);
   const uint32_t last = i1 == n ? n + 1 : i1;
@@ -213,7 +213,7 @@ __device__ uint32_t elf_run(const uint8_t *data, uint32_t size_bits, uint32_t n,
           v = elf_round_up(v, (uint32_t)beta - (uint32_t)sp - 1u);
         }
       }
-      out[i] = v;
+      out.put(i, v);
     }
   }
   return kErrNone;
@@ -270,6 +270,105 @@ __device__ uint32_t chimp128_run(const uint8_t *data, uint32_t size_bits, uint32
     if (i == n) return term ? kErrNone : kErrCount;
     if (term) return kErrCount;
     if (kWrite) out[i] = as_f64(v);
+  }
+  return kErrNone;
+}
+
+// Chimp128 parse-only pass for the load-time index: record lengths depend
+// only on flags, lead codes and the stored lead, so one lane walks records
+// 0..n (n = the terminator) without the value ring and writes a checkpoint
+// {bit position, stored lead} every kSeg records (n / kSeg + 1 entries).
+__device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_bits, uint32_t n, XorCkpt *ck) {
+  LaneReader r;
+  r.init(data, 0);
+  uint32_t lead = 0xffffffffu;
+  for (uint32_t i = 0; i <= n; ++i) {
+    if ((i % kSeg) == 0) ck[i / kSeg] = XorCkpt{0, r.pos, (uint8_t)(lead > 64 ? 255 : lead), 0, -1, 0};
+    if (i == 0) {
+      r.read64(64);
+    } else {
+      const uint32_t f = r.read(2);
+      if (f == 0) {
+        r.read(7);
+      } else if (f == 3) {
+        lead = lead_repr(r.read(3));
+        r.read64(64 - lead);
+      } else if (f == 2) {
+        if (lead > 64) return kErrCorrupt;
+        r.read64(64 - lead);
+      } else {
+        const uint32_t h = r.read(16);
+        lead = lead_repr((h >> 6) & 7);
+        uint32_t sig = h & 63;
+        if (!sig) sig = 64;
+        if (sig + lead > 64) return kErrCorrupt;
+        r.read64(sig);
+      }
+    }
+    if (r.pos > size_bits) return kErrOverrun;
+  }
+  return kErrNone;
+}
+
+// Chimp128 record i as (source record, xor payload): value_i = value_src ^ x.
+// Flags 10 / 11 use the previous record; flags 00 / 01 name ring slot idx,
+// i.e. the latest record j <= i - 1 with j == idx (mod 128); j < 0 is an
+// unwritten slot (value 0).  Record 0 is the root (the first 64 bits).
+constexpr int32_t kRoot = INT32_MIN;
+
+// Parses records [i0, i1) from a checkpoint; writes src / x / is00 bits to
+// the wave's LDS block arrays at offset (i - base).  Sources inside the
+// lane's own segment are resolved on the way, so every stored source lies
+// before i0 (an earlier segment, the previous block, or the root).
+__device__ uint32_t chimp128_parse_seg(const uint8_t *data, uint32_t size_bits, uint32_t i0, uint32_t i1,
+                                        uint32_t bitpos, uint32_t lead, uint32_t base, int32_t *S, uint64_t *X,
+                                        uint32_t *F) {
+  LaneReader r;
+  r.init(data, bitpos);
+  for (uint32_t i = i0; i < i1; ++i) {
+    int32_t src;
+    uint64_t x = 0;
+    bool is00 = false;
+    if (i == 0) {
+      x = r.read64(64);
+      src = kRoot;
+    } else {
+      const uint32_t f = r.read(2);
+      if (f == 0) {
+        const uint32_t idx = r.read(7);
+        src = (int32_t)(i - 1) - (int32_t)(((i - 1) - idx) & 127u);
+        is00 = true;
+      } else if (f == 3) {
+        lead = lead_repr(r.read(3));
+        x = r.read64(64 - lead);
+        src = (int32_t)i - 1;
+      } else if (f == 2) {
+        if (lead > 64) return kErrCorrupt;
+        x = r.read64(64 - lead);
+        src = (int32_t)i - 1;
+      } else {
+        const uint32_t h = r.read(16);
+        const uint32_t idx = (h >> 9) & 127;
+        lead = lead_repr((h >> 6) & 7);
+        uint32_t sig = h & 63;
+        if (!sig) sig = 64;
+        if (sig + lead > 64) return kErrCorrupt;
+        x = r.read64(sig) << (64 - sig - lead);
+        src = (int32_t)(i - 1) - (int32_t)(((i - 1) - idx) & 127u);
+      }
+    }
+    if (r.pos > size_bits) return kErrOverrun;
+    const uint32_t k = i - base;
+    if (src >= (int32_t)i0) {
+      // source in this lane's own segment: already resolved to a source
+      // outside it, so fold it in now (value_i = value_ext ^ x_src ^ x)
+      const uint32_t q = (uint32_t)src - base;
+      x ^= X[q];
+      src = S[q];
+    }
+    S[k] = src;
+    X[k] = x;
+    if (is00) atomicOr(F + (k >> 5), 1u << (k & 31));
   }
   return kErrNone;
 }

@@ -23,17 +23,35 @@ __device__ __forceinline__ uint32_t lead_repr(uint32_t code) {
   return (uint32_t)(t >> (8 * code)) & 0xff;
 }
 
-// MSB-first reader over one lane's own bit stream.  `base` is 4-byte aligned
-// (chunk bodies are 256-byte aligned); reads may run up to 8 bytes past the
-// current position, which the per-chunk zero padding absorbs.
-struct LaneReader {
-  const uint32_t *w;
+// MSB-first reader over one lane's own bit stream.  `base` is 16-byte
+// aligned (chunk bodies are 256-byte aligned).  The stream is fetched in
+// blocks of kBlk x 16 bytes with one block of lookahead, so the next block
+// is in flight while the current one is consumed.  kBlk = 1 suits lanes
+// that read short, neighbouring pieces (a wave then covers one contiguous
+// range); kBlk = 4 suits lanes walking long private streams (XOR codecs):
+// each 64-byte block is fetched in one burst, so a line is not touched again
+// after the caches may have dropped it.  Reads run at most 2 blocks + 8
+// bytes past the current position, inside the per-chunk zero padding.
+template <int kBlk>
+struct LaneReaderT {
+  const uint4 *p;  // next block to fetch
+  uint4 cur[kBlk], nxt[kBlk];
+  uint32_t wi;     // next word of `cur` (0 .. 4 * kBlk)
   uint64_t buf;
   uint32_t nb;
   uint32_t pos;
 
+  __device__ __forceinline__ void load(uint4 (&dst)[kBlk]) {
+#pragma unroll
+    for (int k = 0; k < kBlk; ++k) dst[k] = p[k];
+    p += kBlk;
+  }
   __device__ __forceinline__ void init(const uint8_t *base, uint32_t bitpos) {
-    w = reinterpret_cast<const uint32_t *>(base) + (bitpos >> 5);
+    const uint32_t w = bitpos >> 5;
+    p = reinterpret_cast<const uint4 *>(base) + (w / (4 * kBlk)) * kBlk;
+    load(cur);
+    load(nxt);
+    wi = w % (4 * kBlk);
     buf = 0;
     nb = 0;
     pos = bitpos;
@@ -42,9 +60,25 @@ struct LaneReader {
     buf <<= s;
     nb -= s;
   }
+  __device__ __forceinline__ uint32_t next_word() {
+    if (wi == 4 * kBlk) {
+#pragma unroll
+      for (int k = 0; k < kBlk; ++k) cur[k] = nxt[k];
+      load(nxt);
+      wi = 0;
+    }
+    uint4 q = cur[0];
+#pragma unroll
+    for (int k = 1; k < kBlk; ++k)
+      if ((wi >> 2) == (uint32_t)k) q = cur[k];
+    const uint32_t c = wi & 3;
+    const uint32_t v = c == 0 ? q.x : c == 1 ? q.y : c == 2 ? q.z : q.w;
+    ++wi;
+    return v;
+  }
   __device__ __forceinline__ void fill() {
     while (nb <= 32) {
-      buf |= (uint64_t)__builtin_bswap32(*w++) << (32 - nb);
+      buf |= (uint64_t)__builtin_bswap32(next_word()) << (32 - nb);
       nb += 32;
     }
   }
@@ -75,6 +109,38 @@ struct LaneReader {
       return (hi << 32) | read(32);
     }
     return read(k);
+  }
+};
+using LaneReader = LaneReaderT<1>;     // short neighbouring reads (DeltaSprintz tiles)
+using StreamReader = LaneReaderT<4>;   // long private streams (XOR codecs)
+
+// Output of one lane's serial decode.  Without staging, records are stored
+// as they come.  With staging (`lds` = this lane's column of a [8][64] LDS
+// array), 8 consecutive records are collected and written as four 16-byte
+// stores, so each lane fills whole 64-byte pieces of a line in one burst
+// instead of touching the line 8 times over the decode.
+struct LaneSink {
+  double *out;
+  double *lds;  // nullptr: direct stores
+  uint32_t end; // records >= end are not stored
+  __device__ __forceinline__ void put(uint32_t i, double v) {
+    if (!lds) {
+      out[i] = v;
+      return;
+    }
+    lds[(i & 7) * 64] = v;
+    if ((i & 7) == 7 || i + 1 == end) flush(i);
+  }
+  __device__ __forceinline__ void flush(uint32_t i) {
+    const uint32_t first = i & ~7u, cnt = i - first + 1;
+    double *dst = out + first;
+    if (cnt == 8 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+      double2 *d2 = reinterpret_cast<double2 *>(dst);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d2[k] = make_double2(lds[(2 * k) * 64], lds[(2 * k + 1) * 64]);
+    } else {
+      for (uint32_t k = 0; k < cnt; ++k) dst[k] = lds[k * 64];
+    }
   }
 };
 

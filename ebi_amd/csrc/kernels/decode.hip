@@ -18,12 +18,6 @@
 
 namespace ebi {
 
-__device__ __forceinline__ uint32_t next_job(uint32_t *ctr, uint32_t stride) {
-  uint32_t j = 0;
-  if (lane_id() == 0) j = atomicAdd(ctr, 1u) + stride;
-  return __builtin_amdgcn_readfirstlane(__shfl(j, 0, kWave));
-}
-
 // Column c with prefix[c] <= j < prefix[c + 1] (wave-uniform binary search).
 __device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_cols, uint32_t j) {
   uint32_t lo = 0, hi = n_cols;
@@ -36,7 +30,7 @@ __device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_
 
 // One XOR checkpoint segment per lane: lane l restarts from checkpoint
 // (first + l) of chunk ci and decodes its kSeg records.
-__device__ __forceinline__ uint32_t segment_lane(const ColView &cv, uint32_t ci, uint32_t first) {
+__device__ __forceinline__ uint32_t segment_lane(const ColView &cv, uint32_t ci, uint32_t first, double *stage) {
   const DevChunk c = cv.chunks[ci];
   const uint32_t seg = first + lane_id();
   const uint32_t i0 = seg * kSeg;
@@ -44,7 +38,7 @@ __device__ __forceinline__ uint32_t segment_lane(const ColView &cv, uint32_t ci,
   const uint32_t i1 = min(i0 + kSeg, c.n);
   const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[seg];
   const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
-  double *out = cv.out + c.rec_off;
+  const LaneSink out{cv.out + c.rec_off, stage + lane_id(), i1};
   const uint32_t bits = c.size * 8u;
   switch (c.codec) {
     case kCodecGorilla:
@@ -63,11 +57,13 @@ __device__ __forceinline__ uint32_t segment_lane(const ColView &cv, uint32_t ci,
 
 // Persistent decode over the job space of n_cols columns: tile jobs of the
 // tile codecs and checkpoint-segment jobs of Gorilla / Chimp / Elf, mixed
-// codecs and uneven chunks in one launch.  Waves take their first job by id
-// and the rest from an atomic counter.
+// codecs and uneven chunks in one launch.  Wave w takes jobs w, w + W,
+// w + 2W, ... (W = waves in the grid): a static interleave, so neighbouring
+// jobs (similar cost) spread over all waves and no job counter is contended.
 __global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
                                                uint32_t n_jobs, uint32_t *ctr, uint32_t *err) {
   __shared__ uint32_t lds_starts[4][kTile + 1];
+  __shared__ double lds_stage[4][8 * kWave];
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
   uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
@@ -92,11 +88,11 @@ __global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint3
     } else {
       const Job jb = cv.seg_jobs[local - cv.n_tile_jobs];
       if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-        const uint32_t e = segment_lane(cv, jb.chunk, jb.tile);
+        const uint32_t e = segment_lane(cv, jb.chunk, jb.tile, lds_stage[wave]);
         if (e) atomicOr(err, e);
       }
     }
-    j = next_job(ctr, stride);
+    j += stride;
   }
 }
 
@@ -121,7 +117,7 @@ __global__ void __launch_bounds__(64) k_serial(const ColView *cols, const uint32
   const DevChunk c = cv.chunks[ci];
   if (!kIndex && cv.chunk_err && cv.chunk_err[ci]) return;
   const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
-  double *out = cv.out + c.rec_off;
+  const LaneSink out{cv.out + c.rec_off, nullptr, c.n};
   XorCkpt *ck = kIndex ? reinterpret_cast<XorCkpt *>(c.xidx) : nullptr;
   const uint32_t bits = c.size * 8u;
   uint32_t e;
@@ -130,6 +126,9 @@ __global__ void __launch_bounds__(64) k_serial(const ColView *cols, const uint32
     case kCodecChimp: e = chimp_run<kWrite>(data, bits, c.n, 0, c.n, 0, 0, 0xffffffffu, out, ck); break;
     case kCodecElf:
       e = elf_run<kWrite>(data, bits, c.n, 0, c.n, 0, 0, 0xffffffffu, 0xffffffffu, INT32_MAX, out, ck);
+      break;
+    case kCodecChimp128:
+      e = kIndex ? chimp128_index(data, bits, c.n, ck) : kErrCorrupt;
       break;
     default: e = kErrCorrupt; break;
   }
@@ -164,6 +163,81 @@ __global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const uint
     atomicOr(err, e);
     if (cv.chunk_err) atomicOr(cv.chunk_err + ci, e);
   }
+}
+
+// Wave-parallel Chimp128 decode (decoder/chunk_reader/chimp_n.rs:173-285).
+// One 64-thread block per chunk walks blocks of kC128Block records:
+//  1. lane l parses checkpoint segment l of the block into (src, x) pairs,
+//     folding sources inside its own segment, so every source ends up in an
+//     earlier segment, the previous block, or the chunk root;
+//  2. the wave resolves segment 0, 1, 2, ... in order, lane l taking record
+//     l of the segment: value = value[src] ^ x, where value[src] is final
+//     (earlier segment of this block, or ctx = the previous block's last 128
+//     values; zeros before the chunk: unwritten ring slots), and stores it
+//     (64 consecutive records per store instruction).
+// Record n is parsed too and must be the NaN terminator via flags 01/10/11;
+// an earlier NaN from those flags is a count error, as the reference would
+// stop there.
+constexpr uint32_t kC128Block = 4096;
+
+__global__ void __launch_bounds__(64) k_chimp128_par(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
+                                                    uint32_t *err) {
+  __shared__ uint64_t X[kC128Block];
+  __shared__ int32_t S[kC128Block];
+  __shared__ uint32_t F[kC128Block / 32];
+  __shared__ uint64_t Cx[128];
+  const uint32_t col = find_col(prefix, n_cols, blockIdx.x);
+  const ColView cv = cols[col];
+  const uint32_t ci = blockIdx.x - prefix[col];
+  if (cv.chunk_err && cv.chunk_err[ci]) return;
+  const DevChunk c = cv.chunks[ci];
+  const XorCkpt *ck = reinterpret_cast<const XorCkpt *>(c.xidx);
+  const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
+  double *out = cv.out + c.rec_off;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t bits = c.size * 8u;
+  const uint32_t total = c.n + 1;  // records 0..n, n = terminator
+  uint32_t e = 0;
+  for (uint32_t t = lane; t < 128; t += kWave) Cx[t] = 0;
+  for (uint32_t base = 0; base < total; base += kC128Block) {
+    const uint32_t bn = min(kC128Block, total - base);
+    for (uint32_t t = lane; t < kC128Block / 32; t += kWave) F[t] = 0;
+    wave_sync();
+    {
+      const uint32_t seg = base / kSeg + lane;
+      const uint32_t i0 = seg * kSeg;
+      if (i0 < base + bn) {
+        const uint32_t i1 = min(i0 + kSeg, base + bn);
+        const XorCkpt k = ck[seg];
+        e |= chimp128_parse_seg(data, bits, i0, i1, k.bitpos, k.a == 255 ? 0xffffffffu : k.a, base, S, X, F);
+      }
+    }
+    if (__any(e != 0)) break;
+    wave_sync();
+    const int32_t lo = (int32_t)base;
+    for (uint32_t kk = 0; kk < bn; kk += kWave) {
+      const uint32_t k = kk + lane;
+      if (k < bn) {
+        const int32_t sk = S[k];
+        const uint64_t src = sk == kRoot ? 0 : sk < lo ? Cx[sk - lo + 128] : X[sk - lo];
+        const uint64_t v = src ^ X[k];
+        X[k] = v;
+        const uint32_t i = base + k;
+        const bool is00 = (F[k >> 5] >> (k & 31)) & 1;
+        if (i < c.n) {
+          out[i] = as_f64(v);
+          if (!is00 && v == kNan) e |= kErrCount;
+        } else if (is00 || v != kNan) {
+          e |= kErrCount;  // record n must be the terminator
+        }
+      }
+      wave_sync();  // segment kk / 64 is final before the next one reads it
+    }
+    if (bn >= 128)
+      for (uint32_t t = lane; t < 128; t += kWave) Cx[t] = X[bn - 128 + t];
+    wave_sync();
+  }
+  if (e) atomicOr(err, e);
 }
 
 // Tile sums for the load-time index: DeltaSprintz sum of deltas, RLE sum of runs.
@@ -246,6 +320,11 @@ void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
     hipLaunchKernelGGL((k_serial<true, false>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
   else
     hipLaunchKernelGGL((k_serial<false, true>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
+}
+
+void chimp128_par(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
+                  uint32_t *err) {
+  if (n_chunks) hipLaunchKernelGGL(k_chimp128_par, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
 }
 
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,

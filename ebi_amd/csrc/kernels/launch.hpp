@@ -22,6 +22,9 @@ void decode(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
 // runs one explicit job on cols[0] instead.
 void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
             const Job *single, uint32_t *err, bool write, bool index);
+// Wave-parallel Chimp128 decode, one block per chunk; prefix over chunks.
+void chimp128_par(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
+                  uint32_t *err);
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
               const Job *single, uint32_t *err, bool write);
 // Load-time tile index for DeltaSprintz and RLE chunks of column slot `col`.
