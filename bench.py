@@ -13,10 +13,12 @@ decodes all 32 columns K times.
 value = algorithmic bytes (compressed chunk bytes + 8 bytes per decoded
 record, summed over all columns) / wall time of the step, in GB/s.
 
-Also reported: per-codec GB/s, the roofline of the dominant kernel (HIP
-event time on the library's stream), the plain per-chunk-launch comparator
-on Gorilla and Chimp, and the CPU oracle timed on a bounded sample of the
-same workload on one host core.
+Also reported: the roofline of the batched step's dominant kernel
+(algorithmic bytes of the columns it decodes / its mean launch time from HIP
+events on the library's stream, around each launch of the timed steps),
+per-codec GB/s from one launch per column, the plain per-chunk-launch
+comparator on Gorilla and Chimp, and the CPU oracle timed on a bounded sample
+of the same workload on one host core.
 """
 from __future__ import annotations
 
@@ -52,6 +54,9 @@ def main():
     ap.add_argument("--comparator-chunks", type=int, default=128,
                     help="chunks of the Gorilla/Chimp column used for the per-chunk-launch comparator")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-breakdown", action="store_true",
+                    help="skip the per-column breakdown and the comparator (after the timed region), so a "
+                         "profiler sees only the batched launches")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -95,64 +100,46 @@ def main():
     decs = [d for _, _, d, _ in cols]
 
     def step():
-        # one batched launch over every chunk of every column (mixed codecs)
+        # one batched decode over every chunk of every column (mixed codecs):
+        # k_decode (tile + segment jobs of all columns), then k_chimp128_par
         E.decode_batch(ctx, decs, outs)
-        return ctx.last_device_ms
+        return ctx.last_device_ms, ctx.last_phase_ms(0), ctx.last_phase_ms(1)
 
     for _ in range(args.warmup):
         step()
-    batch_dev_ms = []
+    batch_dev_ms, phase_ms = [], [[], []]
     t0 = time.perf_counter()
     for s in range(args.steps):
-        batch_dev_ms.append(step())
+        tot, p0, p1 = step()
+        batch_dev_ms.append(tot)
+        phase_ms[0].append(p0)
+        phase_ms[1].append(p1)
         log(f"[timed] step {s + 1}/{args.steps} done")
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = total_bytes / (ms_per_step / 1000.0) / 1e9
 
-    # per-column pass (outside the timed region) for the per-codec breakdown
-    per_col_ms = {i: [] for i in range(len(cols))}
-    for _ in range(2):
-        for i, d in enumerate(decs):
-            d.decode_device(outs[i])
-            per_col_ms[i].append(ctx.last_device_ms)
-
-    # per-codec and per-kernel aggregation (device time from HIP events)
+    # roofline of the batched step's kernels: algorithmic bytes of the columns
+    # each kernel decodes / its mean launch time (HIP events on the library's
+    # stream, around each launch of the timed steps)
     kernel_of = {c: ("k_chimp128_par" if c == "chimp128" else "k_decode") for c in codecs}
-    per_codec, per_kernel = {}, {}
-    for i, (g, codec, d, cb) in enumerate(cols):
-        ms = float(np.mean(per_col_ms[i]))
-        b = cb + 8 * n
-        pc = per_codec.setdefault(codec, {"bytes": 0, "ms": 0.0})
-        pc["bytes"] += b
-        pc["ms"] += ms
-        pk = per_kernel.setdefault(kernel_of[codec], {"bytes": 0, "ms": 0.0, "launches": 0})
-        pk["bytes"] += b
-        pk["ms"] += ms
-        pk["launches"] += 1
-    dominant = max(per_kernel, key=lambda k: per_kernel[k]["ms"])
-    dk = per_kernel[dominant]
-    achieved = dk["bytes"] / dk["launches"] / (dk["ms"] / dk["launches"] / 1000.0) / 1e9
+    batch_kernels = {}
+    for k, name in enumerate(("k_decode", "k_chimp128_par")):
+        b = sum(cb + 8 * n for _, codec, _, cb in cols if kernel_of[codec] == name)
+        if b:
+            ms = float(np.mean(phase_ms[k]))
+            batch_kernels[name] = {"bytes_per_launch": b, "mean_launch_ms": round(ms, 4),
+                                   "gbs": round(b / (ms / 1000.0) / 1e9, 2)}
+    dominant = max(batch_kernels, key=lambda k: batch_kernels[k]["mean_launch_ms"])
+    achieved = batch_kernels[dominant]["gbs"]
 
-    # plain per-chunk-launch comparator (Gorilla / Chimp on the random walk)
-    comparator = {}
-    for codec, data in comparator_files.items():
-        d = E.Decoder(ctx, data)
-        m = d.number_of_records
-        buf = ctx.alloc(8 * m)
-        d.decode_device(buf)
-        t = time.perf_counter()
-        for _ in range(3):
-            d.decode_device(buf)
-        batched = (time.perf_counter() - t) / 3
-        t = time.perf_counter()
-        d.decode_per_chunk(buf)
-        plain = time.perf_counter() - t
-        comparator[codec] = {"chunks": d.number_of_chunks, "batched_ms": batched * 1e3,
-                             "per_chunk_launch_ms": plain * 1e3, "speedup": plain / batched}
-        buf.free()
-        d.close()
+    per_codec, per_kernel, comparator = {}, {}, {}
+    if not args.no_breakdown:
+        per_codec, per_kernel = breakdown(ctx, cols, decs, outs, n, kernel_of)
+        comparator = run_comparator(E, ctx, comparator_files)
 
+    workload = (f"plain decode: 2^{args.log2n} f64 x {len(gens)} generators x {len(codecs)} codecs, "
+                f"{args.chunk} records/chunk, all columns per step")
     res = {
         "metric": "decode GB/s (% HBM peak) + in-situ query p50/p99 vs per-chunk-launch form",
         "value": round(value, 2),
@@ -167,8 +154,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded SplitMix64 generators, encoded on the host)",
         "config": {
-            "workload": f"plain decode: 2^{args.log2n} f64 x {len(gens)} generators x {len(codecs)} codecs, "
-                        f"{args.chunk} records/chunk, all columns per step",
+            "workload": workload,
             "columns": len(cols),
             "values_per_column": n,
             "records_per_chunk": args.chunk,
@@ -179,12 +165,15 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": dominant,
-            "achieved": round(achieved, 2),
+            "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_of(workload, dominant),
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE passes of this workload, calibrated on a copy "
+                              "(profiles/hbm_traffic.json); null when the workload differs",
         },
+        "batch_kernels": batch_kernels,
         "per_codec_gbs": {c: round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 2) for c, v in per_codec.items()},
         "per_kernel": {k: {"gbs": round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 2),
                            "ms_per_step": round(v["ms"], 3), "launches": v["launches"]}
@@ -207,6 +196,64 @@ def main():
         d.close()
     ctx.close()
     print(json.dumps(res), flush=True)
+
+
+def traffic_of(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload`, from the committed
+    rocprofv3 counter passes (None if they cover another workload)."""
+    try:
+        t = json.loads((ROOT / "profiles" / "hbm_traffic.json").read_text())
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    return (t.get("kernels", {}).get(kernel) or {}).get("traffic")
+
+
+def breakdown(ctx, cols, decs, outs, n, kernel_of):
+    """Per-codec and per-kernel GB/s from one decode_device per column (two
+    passes, HIP event time), outside the timed region."""
+    per_col_ms = {i: [] for i in range(len(cols))}
+    for _ in range(2):
+        for i, d in enumerate(decs):
+            d.decode_device(outs[i])
+            per_col_ms[i].append(ctx.last_device_ms)
+    per_codec, per_kernel = {}, {}
+    for i, (g, codec, d, cb) in enumerate(cols):
+        ms = float(np.mean(per_col_ms[i]))
+        b = cb + 8 * n
+        pc = per_codec.setdefault(codec, {"bytes": 0, "ms": 0.0})
+        pc["bytes"] += b
+        pc["ms"] += ms
+        pk = per_kernel.setdefault(kernel_of[codec], {"bytes": 0, "ms": 0.0, "launches": 0})
+        pk["bytes"] += b
+        pk["ms"] += ms
+        pk["launches"] += 1
+    return per_codec, per_kernel
+
+
+def run_comparator(E, ctx, comparator_files):
+    """Batched decode vs the plain form (one launch + host sync per chunk) on
+    the Gorilla / Chimp random-walk columns."""
+    comparator = {}
+    for codec, data in comparator_files.items():
+        d = E.Decoder(ctx, data)
+        m = d.number_of_records
+        buf = ctx.alloc(8 * m)
+        d.decode_device(buf)
+        t = time.perf_counter()
+        for _ in range(3):
+            d.decode_device(buf)
+        batched = (time.perf_counter() - t) / 3
+        t = time.perf_counter()
+        d.decode_per_chunk(buf)
+        plain = time.perf_counter() - t
+        comparator[codec] = {"chunks": d.number_of_chunks, "batched_ms": batched * 1e3,
+                             "per_chunk_launch_ms": plain * 1e3, "speedup": plain / batched}
+        buf.free()
+        d.close()
+
+    return comparator
 
 
 def cpu_sample(data, n, chunk, acc, budget_s):

@@ -112,6 +112,7 @@ _SIGS = {
     "ebi_hip_ctx_destroy": (C.c_int, [_P]),
     "ebi_hip_last_error": (C.c_char_p, [_P]),
     "ebi_hip_last_device_ms": (C.c_double, [_P]),
+    "ebi_hip_last_phase_ms": (C.c_double, [_P, C.c_int]),
     "ebi_hip_column_load": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(_P)]),
     "ebi_hip_column_free": (C.c_int, [_P, _P]),
     "ebi_hip_column_info": (C.c_int, [_P, C.POINTER(_FileInfo)]),
@@ -395,6 +396,11 @@ class Context:
     @property
     def last_device_ms(self) -> float:
         return float(lib().ebi_hip_last_device_ms(self._h))
+
+    def last_phase_ms(self, phase: int) -> float:
+        """Device ms of kernel `phase` of the last decode (0: tile/segment
+        kernel, 1: Chimp128 kernel)."""
+        return float(lib().ebi_hip_last_phase_ms(self._h, int(phase)))
 
     def alloc(self, nbytes: int) -> "DeviceBuffer":
         p = _P()

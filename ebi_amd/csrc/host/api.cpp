@@ -130,6 +130,7 @@ int enqueue_decode(ebi_hip_ctx *ctx, Column &c) {
   const uint32_t jobs = c.n_tile_jobs + (ctx->xor_mode == 0 ? c.n_seg_jobs : 0);
   launch::decode(ctx->stream, c.d_view, c.d_prefix, 1, jobs, ctx->d_flags + 1, ctx->d_flags,
                  (uint32_t)ctx->cu_count * 8);
+  hipEventRecord(ctx->ev_mid, ctx->stream);
   if (c.chimp128)
     launch::chimp128_par(ctx->stream, c.d_view, c.d_prefix + 4, 1, (uint32_t)c.chunks.size(), ctx->d_flags);
   else if (c.xor_codec && ctx->xor_mode != 0)
@@ -404,7 +405,7 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     c->cu_count = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess || hipEventCreate(&c->ev_mid) != hipSuccess ||
       hipMalloc((void **)&c->d_flags, 64 * sizeof(uint32_t)) != hipSuccess ||
       hipHostMalloc((void **)&c->h_flags, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&c->h_view, 64 * sizeof(ColView), hipHostMallocDefault) != hipSuccess ||
@@ -448,6 +449,7 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   if (ctx->h_view) hipHostFree(ctx->h_view);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->ev_mid) hipEventDestroy(ctx->ev_mid);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return EBI_OK;
@@ -458,6 +460,21 @@ extern "C" const char *ebi_hip_last_error(const ebi_hip_ctx *ctx) {
 }
 
 extern "C" double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx) { return ctx ? ctx->last_ms : 0.0; }
+
+extern "C" double ebi_hip_last_phase_ms(const ebi_hip_ctx *ctx, int phase) {
+  return ctx && phase >= 0 && phase < 2 ? ctx->phase_ms[phase] : 0.0;
+}
+
+// Total and per-kernel device time of a decode whose kernels were bracketed
+// by ev0, ev_mid, ev1.
+static void record_decode_times(ebi_hip_ctx *ctx) {
+  float a = 0, b = 0;
+  hipEventElapsedTime(&a, ctx->ev0, ctx->ev_mid);
+  hipEventElapsedTime(&b, ctx->ev_mid, ctx->ev1);
+  ctx->phase_ms[0] = a;
+  ctx->phase_ms[1] = b;
+  ctx->last_ms = (double)a + (double)b;
+}
 
 extern "C" int ebi_hip_column_load(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, ebi_hip_col **out) {
   EBI_GUARD(ctx);
@@ -518,9 +535,7 @@ extern "C" int ebi_hip_decode_device(ebi_hip_ctx *ctx, const ebi_hip_col *colc, 
   if ((rc = enqueue_decode(ctx, c))) return rc;
   hipEventRecord(ctx->ev1, ctx->stream);
   if ((rc = sync_and_check(ctx, "decode"))) return rc;
-  float ms = 0;
-  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-  ctx->last_ms = ms;
+  record_decode_times(ctx);
   return EBI_OK;
 }
 
@@ -638,13 +653,12 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   const ColView *dv = (const ColView *)(d + pre_bytes);
   launch::decode(ctx->stream, dv, (const uint32_t *)d, (uint32_t)n_cols, (uint32_t)jobs, ctx->d_flags + 1,
                  ctx->d_flags, (uint32_t)ctx->cu_count * 8);
+  hipEventRecord(ctx->ev_mid, ctx->stream);
   launch::chimp128_par(ctx->stream, dv, (const uint32_t *)d + (n_cols + 1), (uint32_t)n_cols, (uint32_t)jobs128,
                        ctx->d_flags);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "batch launch");
   hipEventRecord(ctx->ev1, ctx->stream);
   if ((rc = sync_and_check(ctx, "batch decode"))) return rc;
-  float ms = 0;
-  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-  ctx->last_ms = ms;
+  record_decode_times(ctx);
   return EBI_OK;
 }

@@ -73,7 +73,9 @@ struct ebi_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_mid = nullptr;  // between the tile/segment kernel and the Chimp128 kernel of a decode
   double last_ms = 0.0;
+  double phase_ms[2] = {0.0, 0.0};  // of the last decode_device / decode_batch
   std::string last_error;
   std::recursive_mutex mu;
   uint32_t *d_flags = nullptr;  // [0] error flags, [1] job counter, [2..] scratch counters

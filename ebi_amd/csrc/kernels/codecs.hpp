@@ -320,6 +320,11 @@ constexpr int32_t kRoot = INT32_MIN;
 // the wave's LDS block arrays at offset (i - base).  Sources inside the
 // lane's own segment are resolved on the way, so every stored source lies
 // before i0 (an earlier segment, the previous block, or the root).
+// LDS slot of block record k in the Chimp128 staging arrays: one pad element
+// per 64 records, so lanes walking their own 64-record segments in lockstep
+// hit distinct banks.
+__device__ __forceinline__ uint32_t c128_slot(uint32_t k) { return k + (k >> 6); }
+
 __device__ uint32_t chimp128_parse_seg(const uint8_t *data, uint32_t size_bits, uint32_t i0, uint32_t i1,
                                         uint32_t bitpos, uint32_t lead, uint32_t base, int32_t *S, uint64_t *X,
                                         uint32_t *F) {
@@ -362,12 +367,12 @@ __device__ uint32_t chimp128_parse_seg(const uint8_t *data, uint32_t size_bits, 
     if (src >= (int32_t)i0) {
       // source in this lane's own segment: already resolved to a source
       // outside it, so fold it in now (value_i = value_ext ^ x_src ^ x)
-      const uint32_t q = (uint32_t)src - base;
+      const uint32_t q = c128_slot((uint32_t)src - base);
       x ^= X[q];
       src = S[q];
     }
-    S[k] = src;
-    X[k] = x;
+    S[c128_slot(k)] = src;
+    X[c128_slot(k)] = x;
     if (is00) atomicOr(F + (k >> 5), 1u << (k & 31));
   }
   return kErrNone;

@@ -182,8 +182,8 @@ constexpr uint32_t kC128Block = 4096;
 
 __global__ void __launch_bounds__(64) k_chimp128_par(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
                                                     uint32_t *err) {
-  __shared__ uint64_t X[kC128Block];
-  __shared__ int32_t S[kC128Block];
+  __shared__ uint64_t X[kC128Block + kC128Block / 64];
+  __shared__ int32_t S[kC128Block + kC128Block / 64];
   __shared__ uint32_t F[kC128Block / 32];
   __shared__ uint64_t Cx[128];
   const uint32_t col = find_col(prefix, n_cols, blockIdx.x);
@@ -218,10 +218,11 @@ __global__ void __launch_bounds__(64) k_chimp128_par(const ColView *cols, const 
     for (uint32_t kk = 0; kk < bn; kk += kWave) {
       const uint32_t k = kk + lane;
       if (k < bn) {
-        const int32_t sk = S[k];
-        const uint64_t src = sk == kRoot ? 0 : sk < lo ? Cx[sk - lo + 128] : X[sk - lo];
-        const uint64_t v = src ^ X[k];
-        X[k] = v;
+        const uint32_t ks = c128_slot(k);
+        const int32_t sk = S[ks];
+        const uint64_t src = sk == kRoot ? 0 : sk < lo ? Cx[sk - lo + 128] : X[c128_slot(sk - lo)];
+        const uint64_t v = src ^ X[ks];
+        X[ks] = v;
         const uint32_t i = base + k;
         const bool is00 = (F[k >> 5] >> (k & 31)) & 1;
         if (i < c.n) {
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(64) k_chimp128_par(const ColView *cols, const 
       wave_sync();  // segment kk / 64 is final before the next one reads it
     }
     if (bn >= 128)
-      for (uint32_t t = lane; t < 128; t += kWave) Cx[t] = X[bn - 128 + t];
+      for (uint32_t t = lane; t < 128; t += kWave) Cx[t] = X[c128_slot(bn - 128 + t)];
     wave_sync();
   }
   if (e) atomicOr(err, e);

@@ -191,6 +191,12 @@ int ebi_hip_copy_to_host(ebi_hip_ctx *ctx, void *h_dst, const void *d_src, size_
  * previous call on this ctx (hipEvent pair on the ctx stream). */
 double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx);
 
+/* Device time of each kernel of the last ebi_hip_decode_device /
+ * ebi_hip_decode_batch call on this ctx: phase 0 = the persistent tile and
+ * segment kernel (every codec but Chimp128), phase 1 = the Chimp128 kernel.
+ * Instrumentation of this library; no counterpart in the reference. */
+double ebi_hip_last_phase_ms(const ebi_hip_ctx *ctx, int phase);
+
 #ifdef __cplusplus
 }
 #endif
