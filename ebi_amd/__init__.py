@@ -129,6 +129,9 @@ _SIGS = {
     "ebi_hip_sum": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
     "ebi_hip_min": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
     "ebi_hip_max": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
+    "ebi_hip_matmul": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _P]),
+    "ebi_hip_gemm_decoded": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t, _P, _P]),
+    "ebi_hip_dgemm_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t]),
 }
 
 
@@ -437,7 +440,7 @@ class Decoder:
 
     Bitmask arguments are iterables of global record ids (the reference's
     RoaringBitmap) or dense u64 word arrays; filter results are sorted u32 id
-    arrays.  ``chunk_id`` selects one chunk (ChunkId, api/decoder.rs:755-780).
+    arrays.  ``chunk_id`` selects one chunk (ChunkId, api/decoder.rs:97-106).
     """
 
     def __init__(self, ctx: Context, data: bytes):
@@ -536,6 +539,43 @@ class Decoder:
         if out.nbytes < 8 * self.number_of_records:
             raise EbiError(INVALID_ARGUMENT, "device buffer too small")
         _check(lib().ebi_hip_decode_per_chunk(self.ctx._h, self._h, out.ptr), self.ctx, "decode_per_chunk")
+
+
+    def matmul_cuda(self, target, target_matrix_shape, data_matrix_shape) -> np.ndarray:
+        """Decoder::matmul_cuda (api/decoder.rs:659-724): batch row-major
+        products A_b * target of the column's data matrices, flattened."""
+        k, n = (int(x) for x in target_matrix_shape)
+        m, kd = (int(x) for x in data_matrix_shape)
+        t = np.ascontiguousarray(target, dtype=np.float64).ravel()
+        if t.size != k * n:
+            raise EbiError(PRECONDITION, "target length does not match its shape")
+        size = m * kd
+        batch = self.number_of_records // size if size else 0
+        out = np.empty(max(batch * m * n, 1), dtype=np.float64)
+        _check(lib().ebi_hip_matmul(self.ctx._h, self._h, t.ctypes.data, k, n, m, kd, out.ctypes.data), self.ctx,
+               "matmul")
+        return out[: batch * m * n]
+
+
+def gemm_decoded(ctx: Context, a: Decoder, b: Decoder, m: int, n: int, k: int,
+                 out: Optional[DeviceBuffer] = None) -> Optional[np.ndarray]:
+    """C = A * B of two decoded row-major columns (A: m x k, B: k x n).
+    Returns C (m x n) on the host, or writes it into the device buffer `out`."""
+    if out is not None:
+        if out.nbytes < 8 * m * n:
+            raise EbiError(INVALID_ARGUMENT, "device buffer too small")
+        _check(lib().ebi_hip_gemm_decoded(ctx._h, a._h, b._h, m, n, k, None, out.ptr), ctx, "gemm_decoded")
+        return None
+    c = np.empty((m, n), dtype=np.float64)
+    _check(lib().ebi_hip_gemm_decoded(ctx._h, a._h, b._h, m, n, k, c.ctypes.data, None), ctx, "gemm_decoded")
+    return c
+
+
+def dgemm_device(ctx: Context, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, m: int, n: int, k: int) -> None:
+    """The DGEMM kernel alone on device buffers (row-major C = A * B)."""
+    if a.nbytes < 8 * m * k or b.nbytes < 8 * k * n or c.nbytes < 8 * m * n:
+        raise EbiError(INVALID_ARGUMENT, "device buffer too small")
+    _check(lib().ebi_hip_dgemm_device(ctx._h, a.ptr, b.ptr, c.ptr, m, n, k), ctx, "dgemm_device")
 
 
 def decode_batch(ctx: Context, decoders, outs) -> None:

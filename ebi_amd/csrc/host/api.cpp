@@ -385,10 +385,6 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
 
 using namespace ebi;
 
-#define EBI_GUARD(ctx)                                   \
-  if (!(ctx)) return EBI_INVALID_ARGUMENT;               \
-  std::lock_guard<std::recursive_mutex> _lk((ctx)->mu); \
-  (ctx)->last_error.clear();
 
 extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   if (!out) return EBI_INVALID_ARGUMENT;
@@ -406,6 +402,8 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
     c->cu_count = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess || hipEventCreate(&c->ev_mid) != hipSuccess ||
+      hipEventCreate(&c->ev_ph[0]) != hipSuccess || hipEventCreate(&c->ev_ph[1]) != hipSuccess ||
+      hipEventCreate(&c->ev_ph[2]) != hipSuccess || hipEventCreate(&c->ev_ph[3]) != hipSuccess ||
       hipMalloc((void **)&c->d_flags, 64 * sizeof(uint32_t)) != hipSuccess ||
       hipHostMalloc((void **)&c->h_flags, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&c->h_view, 64 * sizeof(ColView), hipHostMallocDefault) != hipSuccess ||
@@ -443,6 +441,9 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   ctx->scratch_out.release();
   ctx->batch_jobs.release();
   ctx->batch_args.release();
+  ctx->gemm_a.release();
+  ctx->gemm_b.release();
+  ctx->gemm_c.release();
   if (ctx->h_batch) hipHostFree(ctx->h_batch);
   if (ctx->d_flags) hipFree(ctx->d_flags);
   if (ctx->h_flags) hipHostFree(ctx->h_flags);
@@ -450,6 +451,8 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   if (ctx->ev_mid) hipEventDestroy(ctx->ev_mid);
+  for (hipEvent_t e : ctx->ev_ph)
+    if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return EBI_OK;
@@ -462,7 +465,7 @@ extern "C" const char *ebi_hip_last_error(const ebi_hip_ctx *ctx) {
 extern "C" double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx) { return ctx ? ctx->last_ms : 0.0; }
 
 extern "C" double ebi_hip_last_phase_ms(const ebi_hip_ctx *ctx, int phase) {
-  return ctx && phase >= 0 && phase < 2 ? ctx->phase_ms[phase] : 0.0;
+  return ctx && phase >= 0 && phase < 4 ? ctx->phase_ms[phase] : 0.0;
 }
 
 // Total and per-kernel device time of a decode whose kernels were bracketed
@@ -473,6 +476,7 @@ static void record_decode_times(ebi_hip_ctx *ctx) {
   hipEventElapsedTime(&b, ctx->ev_mid, ctx->ev1);
   ctx->phase_ms[0] = a;
   ctx->phase_ms[1] = b;
+  ctx->phase_ms[2] = ctx->phase_ms[3] = 0.0;
   ctx->last_ms = (double)a + (double)b;
 }
 

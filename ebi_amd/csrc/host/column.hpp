@@ -74,8 +74,9 @@ struct ebi_hip_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_mid = nullptr;  // between the tile/segment kernel and the Chimp128 kernel of a decode
+  hipEvent_t ev_ph[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries of matmul calls
   double last_ms = 0.0;
-  double phase_ms[2] = {0.0, 0.0};  // of the last decode_device / decode_batch
+  double phase_ms[4] = {0.0, 0.0, 0.0, 0.0};  // of the last decode_device / decode_batch / matmul call
   std::string last_error;
   std::recursive_mutex mu;
   uint32_t *d_flags = nullptr;  // [0] error flags, [1] job counter, [2..] scratch counters
@@ -90,6 +91,7 @@ struct ebi_hip_ctx {
   ebi::DevBuf scratch_misc;     // partials, scans
   ebi::DevBuf scratch_out;      // compacted outputs
   ebi::DevBuf batch_jobs;
+  ebi::DevBuf gemm_a, gemm_b, gemm_c;  // decoded operands and product of the matmul entry points
   int cu_count = 256;
   int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks
 };

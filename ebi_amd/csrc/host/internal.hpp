@@ -3,9 +3,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "column.hpp"
+
+// Entry-point prologue: null check, serialise on the ctx, clear the error.
+#define EBI_GUARD(ctx)                                   \
+  if (!(ctx)) return EBI_INVALID_ARGUMENT;               \
+  std::lock_guard<std::recursive_mutex> _lk((ctx)->mu); \
+  (ctx)->last_error.clear();
 
 namespace ebi {
 

@@ -309,10 +309,6 @@ int aggregate(ebi_hip_ctx *ctx, const ebi_hip_col *colc, const uint64_t *bitset,
 
 using namespace ebi;
 
-#define EBI_GUARD(ctx)                                   \
-  if (!(ctx)) return EBI_INVALID_ARGUMENT;               \
-  std::lock_guard<std::recursive_mutex> _lk((ctx)->mu); \
-  (ctx)->last_error.clear();
 
 extern "C" int ebi_hip_sum(ebi_hip_ctx *ctx, const ebi_hip_col *col, const uint64_t *bitset, int64_t chunk_id,
                            double *out) {

@@ -43,5 +43,10 @@ uint32_t compact_blocks(uint64_t n_words);
 void compact(hipStream_t s, const uint64_t *bits, uint64_t n_words, const double *values, double *out,
              uint64_t *block_scratch, uint64_t *n_out);
 
+// FP64 DGEMM on the matrix cores: C_b = A_b * B_b, row-major, b < batch,
+// element strides sA / sB / sC between batches (0 = shared operand).
+hipError_t dgemm(hipStream_t s, const double *A, const double *B, double *C, uint32_t m, uint32_t n, uint32_t k,
+                 uint64_t batch, uint64_t sA, uint64_t sB, uint64_t sC);
+
 }  // namespace launch
 }  // namespace ebi

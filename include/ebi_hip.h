@@ -149,7 +149,7 @@ uint64_t ebi_hip_column_compressed_bytes(const ebi_hip_col *col);
 /* Query entry points.  They mirror Decoder::{materialize, filter,
  * filter_materialize, sum, min, max} (api/decoder.rs:202-439):
  *   bitset   -- nullable input bitmask over global ids, ceil(N/64) words;
- *   chunk_id -- -1 for every chunk, else ChunkId (api/decoder.rs:755-780);
+ *   chunk_id -- -1 for every chunk, else ChunkId (api/decoder.rs:97-106, chunk_reader :726-737);
  * chunks whose record range has no bit set are pruned exactly as the
  * reference does (api/decoder.rs:222-226). */
 int ebi_hip_materialize(ebi_hip_ctx *ctx, const ebi_hip_col *col,
@@ -182,6 +182,34 @@ int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
  * chunk and a host synchronisation after each. */
 int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col, double *d_out);
 
+/* Decoder::matmul_cuda (api/decoder.rs:659-724, over
+ * cuda_binding::Context::gemm, cuda-binding/src/lib.rs:59-140): the column
+ * holds batch = N / (data_rows * data_cols) row-major data matrices A_b
+ * (data_rows x data_cols) back to back; `target` is the row-major host matrix
+ * B (target_rows x target_cols); `out` receives batch row-major products
+ * C_b = A_b * B (data_rows x target_cols each).  EBI_PRECONDITION when
+ * data_cols != target_rows, a dimension is 0, or N is not a multiple of the
+ * data matrix size.  A data matrix may span chunks (the reference requires
+ * each one inside a chunk, chunk_reader.rs:261-280).  Decode and DGEMM run
+ * on the device (FP64 matrix cores); phases for ebi_hip_last_phase_ms:
+ * 0 decode, 1 target upload, 2 DGEMM. */
+int ebi_hip_matmul(ebi_hip_ctx *ctx, const ebi_hip_col *col, const double *target,
+                   size_t target_rows, size_t target_cols, size_t data_rows,
+                   size_t data_cols, double *out);
+
+/* C = A * B with A (m x k) the whole decoded column col_a and B (k x n) the
+ * whole decoded column col_b, both row-major (e.g. matrices stored one row
+ * per chunk).  C (m x n, row-major) goes to c_device (a device buffer) if
+ * given, else to c_host.  EBI_PRECONDITION unless the columns hold exactly
+ * m*k and k*n records.  Phases: 0 decode A, 1 decode B, 2 DGEMM. */
+int ebi_hip_gemm_decoded(ebi_hip_ctx *ctx, const ebi_hip_col *col_a,
+                         const ebi_hip_col *col_b, size_t m, size_t n, size_t k,
+                         double *c_host, double *c_device);
+
+/* The DGEMM alone on device buffers (row-major, C = A * B); phase 0. */
+int ebi_hip_dgemm_device(ebi_hip_ctx *ctx, const double *d_a, const double *d_b,
+                         double *d_c, size_t m, size_t n, size_t k);
+
 /* Device allocation helpers for callers without their own allocator. */
 int ebi_hip_device_alloc(ebi_hip_ctx *ctx, size_t bytes, void **d_ptr);
 int ebi_hip_device_free(ebi_hip_ctx *ctx, void *d_ptr);
@@ -191,9 +219,10 @@ int ebi_hip_copy_to_host(ebi_hip_ctx *ctx, void *h_dst, const void *d_src, size_
  * previous call on this ctx (hipEvent pair on the ctx stream). */
 double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx);
 
-/* Device time of each kernel of the last ebi_hip_decode_device /
- * ebi_hip_decode_batch call on this ctx: phase 0 = the persistent tile and
- * segment kernel (every codec but Chimp128), phase 1 = the Chimp128 kernel.
+/* Device time of each phase of the last decode or matmul call on this ctx:
+ * for ebi_hip_decode_device / ebi_hip_decode_batch phase 0 = the persistent
+ * tile and segment kernel (every codec but Chimp128), phase 1 = the Chimp128
+ * kernel; for the matmul entry points the phases listed there.
  * Instrumentation of this library; no counterpart in the reference. */
 double ebi_hip_last_phase_ms(const ebi_hip_ctx *ctx, int phase);
 

@@ -5,6 +5,8 @@ Sources (lemolatoon/ebi):
   filter          ebi/tests/query.rs:285-760
   materialize     ebi/tests/query.rs:1116-1199
   sum / max / min ebi/tests/aggregation.rs:195-251, 303-316, 351-364
+  matmul_cuda     ebi/tests/ml_query.rs:455-576 (run on uncompressed, rle,
+                  gorilla, chimp, chimp128, elf: ml_query.rs:605-617)
 The reference runs the filter / materialize vectors on every codec, with BUFF
 at scale 100 (filters) or 10 (materialize) and DeltaSprintz at scale 10
 (query.rs:1280-1367); aggregations use scale 100 for the scaled codecs.
@@ -105,6 +107,34 @@ AGGREGATE = [
      "expected": -1.0},
 ]
 
+
+def mm(name, data, chunk, target, expected):
+    """data: batch x m x k, target: k x n, expected: batch x m x n."""
+    return {"name": name, "data": data, "chunk": RC(chunk), "target": target, "expected": expected}
+
+
+MATMUL = [
+    mm("Test 1: matmul m=2 k=3 n=4",
+       [[[-1.0, -2.0, 3.0], [-3.0, 3.0, 1.0]], [[-3.0, -1.0, 0.0], [2.0, 0.0, -3.0]]], 12,
+       [[1.0, -2.0, -1.0, -1.0], [2.0, -2.0, 1.0, 0.0], [0.0, 1.0, 1.0, -2.0]],
+       [[[-5.0, 9.0, 2.0, -5.0], [3.0, 1.0, 7.0, 1.0]], [[-5.0, 8.0, 2.0, 3.0], [2.0, -7.0, -5.0, 4.0]]]),
+    mm("Test 2: matmul m=3 k=2 n=2", [[[1.0, 1.0], [-3.0, -1.0], [0.0, -1.0]]], 6,
+       [[0.0, 2.0], [-2.0, 1.0]], [[[-2.0, 3.0], [2.0, -7.0], [2.0, -1.0]]]),
+    mm("Test 3: matmul m=1 k=2 n=3", [[[-1.0, -3.0]], [[3.0, -1.0]], [[2.0, 0.0]]], 6,
+       [[-2.0, 0.0, 1.0], [0.0, -2.0, -1.0]], [[[2.0, 6.0, 2.0]], [[-6.0, 2.0, 4.0]], [[-4.0, 0.0, 2.0]]]),
+    mm("Test 4: matmul m=2 k=2 n=1", [[[1.0, 3.0], [-1.0, -1.0]], [[2.0, -1.0], [-2.0, -2.0]]], 8,
+       [[0.0], [-1.0]], [[[-3.0], [1.0]], [[1.0], [2.0]]]),
+    mm("Test 5: matmul m=1 k=3 n=2", [[[1.0, -2.0, 0.0]], [[3.0, 1.0, -3.0]], [[-2.0, 3.0, 1.0]], [[-3.0, 1.0, 3.0]]],
+       12, [[-2.0, -1.0], [2.0, 0.0], [1.0, -2.0]], [[[-6.0, -1.0]], [[-7.0, 3.0]], [[11.0, 0.0]], [[11.0, -3.0]]]),
+    mm("Test 6: matmul m=2 k=2 n=5",
+       [[[1.0, -2.0], [1.0, 0.0]], [[1.0, -1.0], [-1.0, -1.0]], [[0.0, 2.0], [0.0, 2.0]]], 12,
+       [[1.0, 2.0, 0.0, 2.0, 0.0], [2.0, 0.0, 1.0, 1.0, 0.0]],
+       [[[-3.0, 2.0, -2.0, 0.0, 0.0], [1.0, 2.0, 0.0, 2.0, 0.0]],
+        [[-1.0, 2.0, -1.0, 1.0, 0.0], [-3.0, -2.0, -1.0, -3.0, 0.0]],
+        [[4.0, 0.0, 2.0, 2.0, 0.0], [4.0, 0.0, 2.0, 2.0, 0.0]]]),
+]
+MATMUL_CODECS = ["uncompressed", "rle", "gorilla", "chimp", "chimp128", "elf"]
+
 # scales the reference tests use per codec
 SCALES = {"filter": {"buff": 100, "delta_sprintz": 10}, "materialize": {"buff": 10, "delta_sprintz": 10},
           "aggregate": {"buff": 100, "delta_sprintz": 100}}
@@ -112,5 +142,5 @@ SCALES = {"filter": {"buff": 100, "delta_sprintz": 10}, "materialize": {"buff": 
 if __name__ == "__main__":
     out = Path(__file__).resolve().parent / "known_answers.json"
     out.write_text(json.dumps({"filter": FILTER, "materialize": MATERIALIZE, "aggregate": AGGREGATE,
-                               "scales": SCALES}, indent=1))
+                               "matmul": MATMUL, "matmul_codecs": MATMUL_CODECS, "scales": SCALES}, indent=1))
     print(out)
