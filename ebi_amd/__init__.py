@@ -27,7 +27,8 @@ import numpy as np
 _LIB_PATH = Path(__file__).resolve().parent / "libebi_hip.so"
 _lib = None
 
-OK, UNEXPECTED_END, CONVERSION, PRECONDITION, HIP, OOM, REJECTED, INVALID_ARGUMENT, UNSUPPORTED = range(9)
+OK, UNEXPECTED_END, CONVERSION, PRECONDITION, HIP, OOM, REJECTED, INVALID_ARGUMENT, UNSUPPORTED, PENDING = range(10)
+OP_MATERIALIZE, OP_FILTER, OP_FILTER_MATERIALIZE, OP_SUM, OP_MIN, OP_MAX = range(6)
 _STATUS = {
     1: "UnexpectedEndOfChunk",
     2: "ConversionError",
@@ -37,6 +38,7 @@ _STATUS = {
     6: "Rejected",
     7: "InvalidArgument",
     8: "Unsupported",
+    9: "Pending",
 }
 
 SCHEMES = {
@@ -71,6 +73,37 @@ class _Pred(C.Structure):
         ("start", C.c_double),
         ("end", C.c_double),
     ]
+
+
+class _Query(C.Structure):
+    _fields_ = [
+        ("op", C.c_int32),
+        ("reserved", C.c_int32),
+        ("col", C.c_void_p),
+        ("pred", _Pred),
+        ("bitset", C.c_void_p),
+        ("chunk_id", C.c_int64),
+        ("out_values", C.c_void_p),
+        ("out_bitset", C.c_void_p),
+        ("out_scalar", C.c_void_p),
+        ("n_out", C.c_void_p),
+    ]
+
+
+class _ServeOpts(C.Structure):
+    _fields_ = [
+        ("device_budget", C.c_uint64),
+        ("pinned_budget", C.c_uint64),
+        ("max_inflight", C.c_uint32),
+        ("n_streams", C.c_uint32),
+        ("max_batch", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class _ServeStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("submitted", "completed", "rejected", "batches", "max_inflight",
+                                           "device_bytes_peak")]
 
 
 class _EncCfg(C.Structure):
@@ -129,6 +162,13 @@ _SIGS = {
     "ebi_hip_sum": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
     "ebi_hip_min": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
     "ebi_hip_max": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
+    "ebi_hip_query": (C.c_int, [_P, C.POINTER(_Query)]),
+    "ebi_hip_serve_create": (C.c_int, [_P, C.POINTER(_ServeOpts), C.POINTER(_P)]),
+    "ebi_hip_serve_submit": (C.c_int, [_P, C.POINTER(_Query), C.c_size_t, C.POINTER(C.c_uint64)]),
+    "ebi_hip_serve_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int)]),
+    "ebi_hip_serve_wait": (C.c_int, [_P, C.c_uint64, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "ebi_hip_serve_stats": (C.c_int, [_P, C.POINTER(_ServeStats)]),
+    "ebi_hip_serve_destroy": (C.c_int, [_P]),
     "ebi_hip_matmul": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _P]),
     "ebi_hip_gemm_decoded": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t, _P, _P]),
     "ebi_hip_dgemm_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t]),
@@ -555,6 +595,117 @@ class Decoder:
         _check(lib().ebi_hip_matmul(self.ctx._h, self._h, t.ctypes.data, k, n, m, kd, out.ctypes.data), self.ctx,
                "matmul")
         return out[: batch * m * n]
+
+
+class Query:
+    """One query as data (ebi_query): op is one of OP_*; `decoder` names the
+    resident column; predicate for the filters; bitmask / chunk_id as in the
+    Decoder methods.  Holds its own output buffers until it completes."""
+
+    def __init__(self, op: int, decoder: "Decoder", predicate: Optional[Predicate] = None, bitmask=None,
+                 chunk_id=None):
+        self.op, self.decoder, self.predicate = int(op), decoder, predicate
+        self._mask, mp = decoder._mask(bitmask)
+        n = decoder.number_of_records
+        self._values = self._bits = None
+        self._scalar = C.c_double(0.0)
+        self._n = C.c_size_t(0)
+        q = _Query()
+        q.op = self.op
+        q.col = decoder._h
+        if predicate is not None:
+            q.pred = predicate.to_c()
+        q.bitset = mp
+        q.chunk_id = Decoder._cid(chunk_id)
+        if self.op in (OP_MATERIALIZE, OP_FILTER_MATERIALIZE):
+            self._values = np.empty(max(n, 1), dtype=np.float64)
+            q.out_values = self._values.ctypes.data
+            q.n_out = C.addressof(self._n)
+        elif self.op == OP_FILTER:
+            self._bits = np.zeros((n + 63) // 64 + 1, dtype=np.uint64)
+            q.out_bitset = self._bits.ctypes.data
+        else:
+            q.out_scalar = C.addressof(self._scalar)
+        self.c = q
+
+    def result(self):
+        if self.op in (OP_MATERIALIZE, OP_FILTER_MATERIALIZE):
+            return self._values[: self._n.value].copy()
+        if self.op == OP_FILTER:
+            return bitset_to_ids(self._bits, self.decoder.number_of_records)
+        return float(self._scalar.value)
+
+
+def run_query(ctx: "Context", q: Query):
+    """ebi_hip_query: one query synchronously; returns its result."""
+    _check(lib().ebi_hip_query(ctx._h, C.byref(q.c)), ctx, "query")
+    return q.result()
+
+
+class Server:
+    """ebi_hip_serve_*: continuous batching of concurrent queries."""
+
+    def __init__(self, ctx: "Context", device_budget: int = 0, pinned_budget: int = 0, max_inflight: int = 0,
+                 n_streams: int = 0, max_batch: int = 0):
+        self.ctx = ctx
+        o = _ServeOpts(device_budget, pinned_budget, max_inflight, n_streams, max_batch, 0)
+        h = _P()
+        _check(lib().ebi_hip_serve_create(ctx._h, C.byref(o), C.byref(h)), ctx, "serve_create")
+        self._h = h
+        self._live = {}
+
+    def submit(self, queries) -> list:
+        qs = list(queries)
+        arr = (_Query * len(qs))(*[q.c for q in qs])
+        t = (C.c_uint64 * len(qs))()
+        _check(lib().ebi_hip_serve_submit(self._h, arr, len(qs), t), None, "serve_submit")
+        ids = [int(x) for x in t]
+        for i, q in zip(ids, qs):
+            self._live[i] = q
+        return ids
+
+    def poll(self, ticket: int):
+        st = C.c_int()
+        rc = lib().ebi_hip_serve_poll(self._h, ticket, C.byref(st))
+        if rc == PENDING:
+            return None
+        _check(rc, None, "serve_poll")
+        return int(st.value)
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """(result, latency_ms); raises EbiError with the query's status."""
+        st, lat = C.c_int(), C.c_double()
+        rc = lib().ebi_hip_serve_wait(self._h, ticket, timeout_ms, C.byref(st), C.byref(lat))
+        if rc == PENDING:
+            raise TimeoutError(f"ticket {ticket} still pending")
+        _check(rc, None, "serve_wait")
+        q = self._live.pop(ticket)
+        if st.value != OK:
+            raise EbiError(st.value, f"query (ticket {ticket})")
+        return q.result(), float(lat.value)
+
+    def stats(self) -> dict:
+        s = _ServeStats()
+        _check(lib().ebi_hip_serve_stats(self._h, C.byref(s)), None, "serve_stats")
+        return {f: int(getattr(s, f)) for f, _ in _ServeStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ebi_hip_serve_destroy(self._h)
+            self._h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gemm_decoded(ctx: Context, a: Decoder, b: Decoder, m: int, n: int, k: int,

@@ -435,10 +435,8 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
 extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   if (!ctx) return EBI_INVALID_ARGUMENT;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  ctx->scratch_values.release();
-  ctx->scratch_bits.release();
-  ctx->scratch_misc.release();
-  ctx->scratch_out.release();
+  ctx->plan_dev.release();
+  if (ctx->plan_pin) hipHostFree(ctx->plan_pin);
   ctx->batch_jobs.release();
   ctx->batch_args.release();
   ctx->gemm_a.release();

@@ -86,10 +86,9 @@ struct ebi_hip_ctx {
   ebi::DevBuf batch_args;       // device ColView array + prefixes for batches
   uint8_t *h_batch = nullptr;     // pinned staging for batch arguments
   size_t h_batch_cap = 0;
-  ebi::DevBuf scratch_values;   // decoded f64 scratch for queries
-  ebi::DevBuf scratch_bits;     // bitsets
-  ebi::DevBuf scratch_misc;     // partials, scans
-  ebi::DevBuf scratch_out;      // compacted outputs
+  ebi::DevBuf plan_dev;         // device scratch of the synchronous query calls
+  uint8_t *plan_pin = nullptr;  // pinned staging of the synchronous query calls
+  size_t plan_pin_cap = 0;
   ebi::DevBuf batch_jobs;
   ebi::DevBuf gemm_a, gemm_b, gemm_c;  // decoded operands and product of the matmul entry points
   int cu_count = 256;

@@ -20,6 +20,8 @@ int fail(ebi_hip_ctx *ctx, int code, const std::string &msg);
 int hip_fail(ebi_hip_ctx *ctx, hipError_t e, const char *what);
 int map_dev_err(uint32_t flags);
 bool is_xor_codec(int scheme);
+// The ColView of `c` with output base `out`.
+ColView view_of(const Column &c, double *out);
 // Uploads the single-column ColView of `c` with output base `out` and waits.
 int upload_view(ebi_hip_ctx *ctx, Column &c, double *out);
 // Copies the device error word back (synchronising the stream) and maps it.

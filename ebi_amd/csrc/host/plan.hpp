@@ -1,0 +1,100 @@
+// Query plans: one Decoder query (materialize, filter, filter_materialize,
+// sum, min, max; ebi/src/api/decoder.rs:202-439) over a resident column,
+// split into an asynchronous device part and a host finish, so that the
+// synchronous C entry points and the serving scheduler run the same code.
+//
+// Life of a plan:
+//   plan_init   validate the request, derive what it needs (host only)
+//   plan_need   device and pinned bytes of its scratch
+//   plan_bind   carve that scratch out of caller-provided memory
+//   plan_stage  host staging into pinned memory (input bitmask copy, BUFF /
+//               DeltaSprintz per-chunk predicate parameters)
+//   group_enqueue (per group of plans sharing a stream): upload every plan's
+//               column view and inputs, chunk selection, ONE batched decode
+//               launch over every plan that needs decoded records, then each
+//               plan's query kernels and the copy of its small results into
+//               pinned memory
+//   plan_enqueue_values  after the device work: copy the materialized
+//               records (count known only then) into the caller's buffer
+//   plan_finish host fold of per-chunk partial results in chunk order and
+//               copies into the caller's buffers
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/ebi_hip.h"
+#include "column.hpp"
+
+namespace ebi {
+
+enum PlanOp : int {
+  kOpMaterialize = EBI_OP_MATERIALIZE,
+  kOpFilter = EBI_OP_FILTER,
+  kOpFilterMaterialize = EBI_OP_FILTER_MATERIALIZE,
+  kOpSum = EBI_OP_SUM,
+  kOpMin = EBI_OP_MIN,
+  kOpMax = EBI_OP_MAX,
+};
+
+struct PlanNeed {
+  uint64_t dev = 0;
+  uint64_t pin = 0;
+};
+
+struct Plan {
+  ebi_query q{};
+  Column *c = nullptr;
+  bool insitu = false;     // BUFF / DeltaSprintz operate on codes
+  bool decode = false;     // needs the column's records decoded into `values`
+  bool has_mask = false;
+  bool cpred = false;      // per-chunk predicate parameters are uploaded
+  int filter_mode = 0;     // k_filter mode (0 select, 1 code domain / f64, 2 f64)
+  uint64_t words = 0;      // ceil(N / 64)
+  uint32_t nc = 0;         // chunks
+  // device scratch
+  const ColView *d_view = nullptr;  // this plan's view (inside its group's block)
+  uint64_t *mask = nullptr, *bits = nullptr, *acc = nullptr, *block = nullptr, *d_n = nullptr;
+  uint32_t *sel = nullptr;
+  ChunkPred *d_cpred = nullptr;
+  double *partials = nullptr, *chunk_sum = nullptr, *values = nullptr, *out = nullptr;
+  // pinned staging
+  uint64_t *h_mask = nullptr, *h_bits = nullptr, *h_acc = nullptr, *h_n = nullptr;
+  uint32_t *h_sel = nullptr;
+  ChunkPred *h_cpred = nullptr;
+  double *h_csum = nullptr;
+  // result
+  int status = EBI_OK;
+  std::string error;
+};
+
+// Validates `q` (0 or an ebi_status; sets p.error).  Host only.
+int plan_init(Plan &p, const ebi_query &q);
+PlanNeed plan_need(const Plan &p);
+void plan_bind(Plan &p, uint8_t *dev, uint8_t *pin);
+void plan_stage(Plan &p);
+// Enqueues the D2H copy of materialized records once the device work has
+// completed (reads the pinned record count).  Returns false if nothing was
+// enqueued.
+bool plan_enqueue_values(Plan &p, hipStream_t s, hipError_t *err);
+// Host finish into the caller's buffers; returns the plan's status.
+int plan_finish(Plan &p);
+
+// A set of plans issued together on one stream: their views live in one
+// block (device + pinned mirror) of group_block_bytes(n) bytes each.
+struct PlanGroup {
+  std::vector<Plan *> plans;
+  uint8_t *dev = nullptr;
+  uint8_t *pin = nullptr;
+  uint32_t *d_flags = nullptr;  // [0] decode error flags, [1] unused counter
+  uint32_t *h_flags = nullptr;
+};
+uint64_t group_block_bytes(size_t n_plans);
+void group_bind(PlanGroup &g, uint8_t *dev, uint8_t *pin);
+// Issues the whole group; `cu_count` sizes the persistent decode grid.
+hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count);
+
+}  // namespace ebi

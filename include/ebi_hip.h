@@ -40,7 +40,8 @@ typedef enum ebi_status {
     EBI_OOM = 5,            /* device or host allocation failed */
     EBI_REJECTED = 6,       /* serving admission refused the request */
     EBI_INVALID_ARGUMENT = 7,
-    EBI_UNSUPPORTED = 8     /* scheme outside the accelerated set */
+    EBI_UNSUPPORTED = 8,    /* scheme outside the accelerated set */
+    EBI_PENDING = 9         /* serving: the ticket has not completed yet */
 } ebi_status;
 
 /* Compression scheme ids, format.rs:185-199. */
@@ -167,6 +168,33 @@ int ebi_hip_min(ebi_hip_ctx *ctx, const ebi_hip_col *col, const uint64_t *bitset
 int ebi_hip_max(ebi_hip_ctx *ctx, const ebi_hip_col *col, const uint64_t *bitset,
                 int64_t chunk_id, double *out);
 
+/* One query as data: the same six operations, for ebi_hip_query and the
+ * serving loop.  Output pointers are the caller's host buffers. */
+enum {
+    EBI_OP_MATERIALIZE = 0,        /* out_values (room for N), n_out */
+    EBI_OP_FILTER = 1,             /* out_bitset (ceil(N/64) words) */
+    EBI_OP_FILTER_MATERIALIZE = 2, /* out_values (room for N), n_out */
+    EBI_OP_SUM = 3,                /* out_scalar */
+    EBI_OP_MIN = 4,
+    EBI_OP_MAX = 5
+};
+typedef struct ebi_query {
+    int32_t op;              /* EBI_OP_* */
+    int32_t reserved;
+    const ebi_hip_col *col;
+    ebi_pred pred;           /* FILTER and FILTER_MATERIALIZE */
+    const uint64_t *bitset;  /* nullable input bitmask over global ids */
+    int64_t chunk_id;        /* -1 for every chunk */
+    double *out_values;
+    uint64_t *out_bitset;
+    double *out_scalar;
+    size_t *n_out;
+} ebi_query;
+
+/* Runs one query synchronously on the ctx stream (the ebi_hip_materialize
+ * ... ebi_hip_max entry points are this call with the matching op). */
+int ebi_hip_query(ebi_hip_ctx *ctx, const ebi_query *q);
+
 /* Device-resident decode: writes all N records of the column to the device
  * buffer d_out (N doubles) on the ctx stream and waits for completion.
  * This is the whole-column decode the batched and DGEMM paths build on. */
@@ -209,6 +237,52 @@ int ebi_hip_gemm_decoded(ebi_hip_ctx *ctx, const ebi_hip_col *col_a,
 /* The DGEMM alone on device buffers (row-major, C = A * B); phase 0. */
 int ebi_hip_dgemm_device(ebi_hip_ctx *ctx, const double *d_a, const double *d_b,
                          double *d_c, size_t m, size_t n, size_t k);
+
+/* ------------------------------------------------------------------ */
+/* Serving: continuous batching of concurrent queries over resident     */
+/* columns.  No counterpart in the reference (its Decoder is             */
+/* single-threaded and synchronous, api/decoder.rs:202-439); each query  */
+/* gives the same result as the synchronous call.                        */
+/* ------------------------------------------------------------------ */
+typedef struct ebi_hip_server ebi_hip_server;
+typedef struct ebi_serve_opts {
+    uint64_t device_budget; /* HBM bytes for in-flight query scratch (0: 8 GiB) */
+    uint64_t pinned_budget; /* pinned host bytes for staging (0: 256 MiB) */
+    uint32_t max_inflight;  /* queries on the device at once (0: 256) */
+    uint32_t n_streams;     /* HIP streams batches rotate over (0: 4) */
+    uint32_t max_batch;     /* queries admitted into one batch (0: 64) */
+    uint32_t reserved;
+} ebi_serve_opts;
+
+typedef struct ebi_serve_stats {
+    uint64_t submitted;
+    uint64_t completed;
+    uint64_t rejected;        /* needed more memory than the budgets hold */
+    uint64_t batches;         /* batched launches issued */
+    uint64_t max_inflight;    /* most queries on the device at once */
+    uint64_t device_bytes_peak;
+} ebi_serve_stats;
+
+/* Starts the scheduler thread of a server on ctx's device.  The columns
+ * queried must stay loaded while their queries are in flight. */
+int ebi_hip_serve_create(ebi_hip_ctx *ctx, const ebi_serve_opts *opts, ebi_hip_server **out);
+/* Queues n queries (thread-safe); tickets[i] names query i.  Each query's
+ * buffers are borrowed until its ticket completes.  Queries are admitted in
+ * arrival order as in-flight slots and memory budget free up; every query
+ * admitted in one scheduler pass shares one batched decode launch, and
+ * passes rotate over the streams so their work overlaps on the device. */
+int ebi_hip_serve_submit(ebi_hip_server *srv, const ebi_query *q, size_t n, uint64_t *tickets);
+/* Non-blocking: EBI_OK and *status = the query's ebi_status once it has
+ * completed, EBI_PENDING before. */
+int ebi_hip_serve_poll(ebi_hip_server *srv, uint64_t ticket, int *status);
+/* Waits up to timeout_ms (negative: forever).  EBI_OK once completed, with
+ * *status and *latency_ms (submit to completion, host clock); the ticket is
+ * then released.  EBI_PENDING on timeout. */
+int ebi_hip_serve_wait(ebi_hip_server *srv, uint64_t ticket, int64_t timeout_ms, int *status,
+                       double *latency_ms);
+int ebi_hip_serve_stats(const ebi_hip_server *srv, ebi_serve_stats *st);
+/* Drains every queued query, stops the scheduler and frees the server. */
+int ebi_hip_serve_destroy(ebi_hip_server *srv);
 
 /* Device allocation helpers for callers without their own allocator. */
 int ebi_hip_device_alloc(ebi_hip_ctx *ctx, size_t bytes, void **d_ptr);
