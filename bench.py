@@ -54,6 +54,13 @@ def main():
     ap.add_argument("--comparator-chunks", type=int, default=128,
                     help="chunks of the Gorilla/Chimp column used for the per-chunk-launch comparator")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-query", action="store_true", help="skip the in-situ query latency leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson stream) leg")
+    ap.add_argument("--no-matmul", action="store_true", help="skip the decompress-then-DGEMM leg")
+    ap.add_argument("--serve-rates", default="2000,5000,10000,20000,50000", help="Poisson arrival rates, queries/s")
+    ap.add_argument("--serve-queries", type=int, default=1000, help="queries per arrival rate")
+    ap.add_argument("--serve-log2max", type=int, default=26, help="largest serving column, log2 values")
+    ap.add_argument("--matmul-n", type=int, default=8192)
     ap.add_argument("--no-breakdown", action="store_true",
                     help="skip the per-column breakdown and the comparator (after the timed region), so a "
                          "profiler sees only the batched launches")
@@ -76,10 +83,12 @@ def main():
     cols = []  # (gen, codec, decoder, compressed_bytes)
     cpu = {"seconds": 0.0, "bytes": 0, "values": 0, "chunks": 0}
     comparator_files = {}
+    sample_q = {}  # generator -> every 64th value, for predicate quantiles
     t_setup = time.time()
     for g in gens:
         fn, dec = G.GENERATORS[g]
         vals = fn(n)
+        sample_q[g] = np.sort(vals[::64])
         for codec in codecs:
             cfg = E.CompressorConfig.by_name(codec, scale=10 ** dec)
             data = E.encode(vals, cfg, E.ChunkOption.record_count(args.chunk), threads=threads)
@@ -137,6 +146,16 @@ def main():
     if not args.no_breakdown:
         per_codec, per_kernel = breakdown(ctx, cols, decs, outs, n, kernel_of)
         comparator = run_comparator(E, ctx, comparator_files)
+    for o in outs:
+        o.free()
+    outs = []
+    query = serving = matmul = None
+    if not args.no_query:
+        query = query_leg(E, ctx, cols, sample_q, n)
+    if not args.no_serve:
+        serving = serve_leg(E, G, ctx, args, threads)
+    if not args.no_matmul:
+        matmul = matmul_leg(E, ctx, args.matmul_n, threads)
 
     workload = (f"plain decode: 2^{args.log2n} f64 x {len(gens)} generators x {len(codecs)} codecs, "
                 f"{args.chunk} records/chunk, all columns per step")
@@ -179,6 +198,9 @@ def main():
                            "ms_per_step": round(v["ms"], 3), "launches": v["launches"]}
                        for k, v in per_kernel.items()},
         "per_chunk_launch_comparator": comparator,
+        "query": query,
+        "serving": serving,
+        "matmul": matmul,
     }
     if not args.no_cpu and cpu["seconds"] > 0:
         res["cpu_baseline"] = {
@@ -190,12 +212,210 @@ def main():
                       f"({cpu['values']} values, {cpu['seconds']:.1f}s on one core)",
         }
         res["gpu_vs_cpu"] = round(value / res["cpu_baseline"]["value"], 1)
-    for o in outs:
-        o.free()
     for _, _, d, _ in cols:
         d.close()
     ctx.close()
     print(json.dumps(res), flush=True)
+
+
+def pct(xs, q):
+    return round(float(np.percentile(np.asarray(xs), q)), 4) if len(xs) else None
+
+
+def query_leg(E, ctx, cols, sample_q, n, reps=20):
+    """In-situ query latency on the resident 2^log2n random-walk columns:
+    range-filter + sum (greater-than at the (1 - s) quantile, s = 0.1, 1, 10,
+    50 %) and max, one synchronous query at a time; BUFF and DeltaSprintz run
+    on their codes, Gorilla decodes first.  Latency is host wall time of the
+    call; GB/s counts compressed bytes + 2 x ceil(N/8) (bitset written and
+    read) + 8."""
+    out = {}
+    sq = sample_q.get("walk2")
+    if sq is None:
+        return None
+    for g, codec, d, cb in cols:
+        if g != "walk2" or codec not in ("buff", "delta_sprintz", "gorilla"):
+            continue
+        res = {}
+        for s in (0.001, 0.01, 0.1, 0.5):
+            thr = float(sq[int((1.0 - s) * (len(sq) - 1))])
+            p = E.Predicate.range(E.RangeValue.exclusive(thr), E.RangeValue.none())
+            q = E.Query(E.OP_FILTER_SUM, d, predicate=p)
+            E.run_query(ctx, q)
+            lat = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                E.run_query(ctx, q)
+                lat.append((time.perf_counter() - t) * 1e3)
+            byt = cb + 2 * ((n + 7) // 8) + 8
+            res[f"filter_sum_sel{s * 100:g}%"] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                                                  "gbs_at_p50": round(byt / (pct(lat, 50) / 1e3) / 1e9, 1)}
+        q = E.Query(E.OP_MAX, d)
+        E.run_query(ctx, q)
+        lat = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            E.run_query(ctx, q)
+            lat.append((time.perf_counter() - t) * 1e3)
+        res["max"] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                      "gbs_at_p50": round((cb + 8) / (pct(lat, 50) / 1e3) / 1e9, 1)}
+        out[codec] = res
+        log(f"[query] {codec} done")
+    return {"workload": f"2^{int(np.log2(n))} f64 random walk (2 decimals) per column, one query at a time",
+            "codecs": out}
+
+
+def serve_leg(E, G, ctx, args, threads, n_cols=64):
+    """Serving: Poisson arrivals over 64 resident 4-decimal random walks
+    (sizes log-uniform on [2^20, 2^log2max], codecs uniform over the eight,
+    chunk sizes log-uniform on [8192, 262144]); mix 50 % range-filter + sum,
+    25 % min / max, 20 % range-filter + materialize, 5 % full-column decode
+    into HBM.  Each rate runs through the server and, on the same arrival
+    sequence, through the per-query-launch form (one synchronous query at a
+    time).  Latency = completion - arrival."""
+    rng = np.random.default_rng(7)
+    t0 = time.time()
+    cols = []
+    for i in range(n_cols):
+        nv = int(round(2 ** rng.uniform(20, args.serve_log2max)))
+        chunk = int(round(2 ** rng.uniform(13, 18)))
+        codec = G.CODECS[i % len(G.CODECS)]
+        vals = G.random_walk(nv, 100.0, 0.1, 4, seed=100 + i)
+        f = E.encode(vals, E.CompressorConfig.by_name(codec, scale=10000), E.ChunkOption.record_count(chunk),
+                     threads=threads)
+        cols.append((E.Decoder(ctx, f), np.sort(vals[::256]), nv))
+        del f, vals
+    log(f"[serve] {n_cols} columns resident in {time.time() - t0:.1f}s")
+    dec_bufs = [ctx.alloc(8 * max(nv for _, _, nv in cols)) for _ in range(8)]
+
+    def make(k):
+        d, sq, nv = cols[int(rng.integers(len(cols)))]
+        u = rng.random()
+        if u < 0.5 or 0.75 <= u < 0.95:  # range filter + sum, range filter + materialize
+            s = float(rng.choice([0.001, 0.01, 0.1]))
+            a = float(rng.uniform(0, 1 - s))
+            lo, hi = float(sq[int(a * (len(sq) - 1))]), float(sq[int((a + s) * (len(sq) - 1))])
+            p = E.Predicate.range(E.RangeValue.inclusive(lo), E.RangeValue.inclusive(hi))
+            return E.Query(E.OP_FILTER_SUM if u < 0.5 else E.OP_FILTER_MATERIALIZE, d, predicate=p)
+        if u < 0.75:  # min / max
+            return E.Query(E.OP_MIN if u < 0.625 else E.OP_MAX, d)
+        return E.Query(E.OP_DECODE, d, out=dec_bufs[k % len(dec_bufs)])  # full-column decode into HBM
+
+    rates = [float(x) for x in args.serve_rates.split(",") if x]
+    out = {}
+    for lam in rates:
+        nq = args.serve_queries
+        qs_served = [make(k) for k in range(nq)]
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, nq))
+        # served: submit each query at its arrival time (in batches of those due)
+        lat = np.zeros(nq)
+        with E.Server(ctx, device_budget=16 << 30, max_batch=64, n_streams=4) as srv:
+            tickets = [0] * nq
+            lag = np.zeros(nq)
+            start = time.perf_counter()
+            i = 0
+            while i < nq:
+                now = time.perf_counter() - start
+                if arrivals[i] > now:
+                    time.sleep(min(arrivals[i] - now, 0.001))
+                    continue
+                j = i
+                while j < nq and arrivals[j] <= now:
+                    j += 1
+                ts = srv.submit(qs_served[i:j])
+                sub = time.perf_counter() - start
+                tickets[i:j] = ts
+                lag[i:j] = sub - arrivals[i:j]
+                i = j
+            for k in range(nq):
+                _, ms = srv.wait(tickets[k])
+                lat[k] = lag[k] * 1e3 + ms
+            wall = time.perf_counter() - start
+            st = srv.stats()
+        # per-query-launch form on the same arrival sequence
+        qs_plain = [E.Query(q.op, q.decoder, q.predicate, out=getattr(q, "_out", None)) for q in qs_served]
+        lat_p = np.zeros(nq)
+        start = time.perf_counter()
+        for k, q in enumerate(qs_plain):
+            now = time.perf_counter() - start
+            if arrivals[k] > now:
+                time.sleep(arrivals[k] - now)
+            E.run_query(ctx, q)
+            lat_p[k] = (time.perf_counter() - start - arrivals[k]) * 1e3
+        wall_p = time.perf_counter() - start
+        same = all(qs_served[k].op not in (E.OP_FILTER_SUM, E.OP_MIN, E.OP_MAX) or
+                   np.float64(qs_served[k].result()).view(np.uint64) == np.float64(qs_plain[k].result()).view(np.uint64)
+                   for k in range(nq))
+        out[f"{int(lam)}qps"] = {
+            "served": {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "throughput_qps": round(nq / wall, 1),
+                       "batches": st["batches"], "max_inflight": st["max_inflight"]},
+            "per_query_launch": {"p50_ms": pct(lat_p, 50), "p99_ms": pct(lat_p, 99),
+                                 "throughput_qps": round(nq / wall_p, 1)},
+            "scalar_results_equal": bool(same),
+        }
+        log(f"[serve] {int(lam)} q/s: served p50 {pct(lat, 50)} p99 {pct(lat, 99)} ms; "
+            f"per-query p50 {pct(lat_p, 50)} p99 {pct(lat_p, 99)} ms")
+        del qs_served, qs_plain
+    for b in dec_bufs:
+        b.free()
+    for d, _, _ in cols:
+        d.close()
+    return {"workload": f"{n_cols} columns, 4-decimal random walks, 2^20..2^{args.serve_log2max} values, "
+                        f"mixed codecs, {args.serve_queries} Poisson arrivals per rate",
+            "rates": out}
+
+
+def matmul_leg(E, ctx, nmat, threads):
+    """Decompress-then-DGEMM: two nmat x nmat matrices, uniform[-10, 10] at 3
+    decimals, stored one row per chunk; decode both into HBM, then the FP64
+    MFMA DGEMM.  Decoded rows are checked bit-exact against the oracle on 16
+    sampled rows of A and of B; C is checked on those rows of A against a
+    float64 numpy product over the decoded data (relative Frobenius error)."""
+    from oracle.oracle import OracleDecoder
+
+    a = E.synth(1, nmat * nmat, 21, 3, -10.0, 10.0, threads=threads)
+    b = E.synth(1, nmat * nmat, 22, 3, -10.0, 10.0, threads=threads)
+    out = {}
+    cbuf = ctx.alloc(8 * nmat * nmat)
+    rows = np.random.default_rng(3).choice(nmat, 16, replace=False)
+    for codec, scale in (("chimp128", 1), ("elf", 1), ("buff", 1000)):
+        cfg = E.CompressorConfig.by_name(codec, scale=scale)
+        fa = E.encode(a, cfg, E.ChunkOption.record_count(nmat), threads=threads)
+        fb = E.encode(b, cfg, E.ChunkOption.record_count(nmat), threads=threads)
+        da, db = E.Decoder(ctx, fa), E.Decoder(ctx, fb)
+        E.gemm_decoded(ctx, da, db, nmat, nmat, nmat, out=cbuf)
+        ph = []
+        for _ in range(3):
+            t = time.perf_counter()
+            E.gemm_decoded(ctx, da, db, nmat, nmat, nmat, out=cbuf)
+            wall = (time.perf_counter() - t) * 1e3
+            ph.append((ctx.last_phase_ms(0), ctx.last_phase_ms(1), ctx.last_phase_ms(2), wall))
+        ph = np.median(np.array(ph), axis=0)
+        c = cbuf.to_host(count=nmat * nmat).reshape(nmat, nmat)
+        oa, ob = OracleDecoder(fa), OracleDecoder(fb)
+        arows = np.stack([oa.materialize(chunk_id=int(r)) for r in rows])
+        bdec = db.materialize().reshape(nmat, nmat)
+        exact = all(np.array_equal(da.materialize(chunk_id=int(r)).view(np.uint64), arows[i].view(np.uint64)) and
+                    np.array_equal(bdec[r].view(np.uint64), ob.materialize(chunk_id=int(r)).view(np.uint64))
+                    for i, r in enumerate(rows))
+        ref = arows @ bdec
+        err = float(np.linalg.norm(c[rows] - ref) / np.linalg.norm(ref))
+        flops = 2.0 * nmat ** 3
+        out[codec] = {"decode_a_ms": round(ph[0], 3), "decode_b_ms": round(ph[1], 3), "dgemm_ms": round(ph[2], 3),
+                      "dgemm_tflops": round(flops / (ph[2] / 1e3) / 1e12, 2),
+                      "dgemm_frac_fp64_peak": round(flops / (ph[2] / 1e3) / 78.6e12, 4),
+                      "end_to_end_ms": round(ph[3], 3),
+                      "decode_gbs": round((da.compressed_bytes + db.compressed_bytes + 16 * nmat * nmat)
+                                          / ((ph[0] + ph[1]) / 1e3) / 1e9, 1),
+                      "sampled_rows_decoded_bit_exact": bool(exact),
+                      "sampled_rel_fro_err": err, "within_1e-12": err <= 1e-12}
+        log(f"[matmul] {codec}: dgemm {ph[2]:.2f} ms, decode {ph[0] + ph[1]:.2f} ms, err {err:.2e}")
+        da.close()
+        db.close()
+        del fa, fb, oa, ob, bdec
+    cbuf.free()
+    return {"workload": f"{nmat}x{nmat} f64 x2, uniform[-10,10] 3 decimals, one row per chunk",
+            "codecs": out}
 
 
 def traffic_of(workload, kernel):

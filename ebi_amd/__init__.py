@@ -28,7 +28,7 @@ _LIB_PATH = Path(__file__).resolve().parent / "libebi_hip.so"
 _lib = None
 
 OK, UNEXPECTED_END, CONVERSION, PRECONDITION, HIP, OOM, REJECTED, INVALID_ARGUMENT, UNSUPPORTED, PENDING = range(10)
-OP_MATERIALIZE, OP_FILTER, OP_FILTER_MATERIALIZE, OP_SUM, OP_MIN, OP_MAX = range(6)
+OP_MATERIALIZE, OP_FILTER, OP_FILTER_MATERIALIZE, OP_SUM, OP_MIN, OP_MAX, OP_FILTER_SUM, OP_DECODE = range(8)
 _STATUS = {
     1: "UnexpectedEndOfChunk",
     2: "ConversionError",
@@ -603,7 +603,7 @@ class Query:
     Decoder methods.  Holds its own output buffers until it completes."""
 
     def __init__(self, op: int, decoder: "Decoder", predicate: Optional[Predicate] = None, bitmask=None,
-                 chunk_id=None):
+                 chunk_id=None, out: Optional["DeviceBuffer"] = None):
         self.op, self.decoder, self.predicate = int(op), decoder, predicate
         self._mask, mp = decoder._mask(bitmask)
         n = decoder.number_of_records
@@ -624,6 +624,11 @@ class Query:
         elif self.op == OP_FILTER:
             self._bits = np.zeros((n + 63) // 64 + 1, dtype=np.uint64)
             q.out_bitset = self._bits.ctypes.data
+        elif self.op == OP_DECODE:
+            if out is None or out.nbytes < 8 * n:
+                raise EbiError(INVALID_ARGUMENT, "OP_DECODE needs a device buffer of N doubles")
+            self._out = out
+            q.out_values = out.ptr
         else:
             q.out_scalar = C.addressof(self._scalar)
         self.c = q
@@ -633,6 +638,8 @@ class Query:
             return self._values[: self._n.value].copy()
         if self.op == OP_FILTER:
             return bitset_to_ids(self._bits, self.decoder.number_of_records)
+        if self.op == OP_DECODE:
+            return None
         return float(self._scalar.value)
 
 

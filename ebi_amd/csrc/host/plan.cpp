@@ -97,8 +97,13 @@ ChunkPred sprintz_chunk_pred(const DevChunk &d, const ebi_pred &p) {
   return cp;
 }
 
-bool is_agg(int op) { return op == kOpSum || op == kOpMin || op == kOpMax; }
-bool has_pred(int op) { return op == kOpFilter || op == kOpFilterMaterialize; }
+bool is_agg(int op) { return op == kOpSum || op == kOpMin || op == kOpMax || op == kOpFilterSum; }
+bool has_pred(int op) { return op == kOpFilter || op == kOpFilterMaterialize || op == kOpFilterSum; }
+// ops whose filter writes the bitset `bits`
+bool writes_bits(const Plan &p) {
+  return p.q.op == kOpFilter || p.q.op == kOpFilterMaterialize || p.q.op == kOpFilterSum ||
+         (p.q.op == kOpMaterialize && p.has_mask);
+}
 
 // Offsets of the device scratch of a plan (all 256-byte aligned).
 struct Layout {
@@ -124,7 +129,7 @@ Layout layout_of(const Plan &p) {
   L.part = take((uint64_t)p.c->n_rtiles * 16);
   L.csum = take((uint64_t)p.nc * 16);
   L.block = take(nb * 8);
-  L.values = p.decode ? take(N * 8) : ~0ull;
+  L.values = p.decode && p.q.op != kOpDecode ? take(N * 8) : ~0ull;  // DECODE writes the caller's buffer
   L.out = (p.q.op == kOpFilterMaterialize || (p.q.op == kOpMaterialize && p.has_mask)) ? take(N * 8) : ~0ull;
   L.total = off;
   return L;
@@ -163,7 +168,7 @@ QueryArgs args_of(const Plan &p) {
   a.cpred = p.d_cpred;
   a.acc = p.acc;
   a.partials = p.partials;
-  a.op = p.q.op == kOpSum ? 0 : p.q.op == kOpMin ? 1 : p.q.op == kOpMax ? 2 : 0;
+  a.op = p.q.op == kOpMin ? 1 : p.q.op == kOpMax ? 2 : 0;
   if (has_pred(p.q.op)) {
     a.kind = p.q.pred.kind;
     a.start_kind = p.q.pred.start_kind;
@@ -191,8 +196,7 @@ hipError_t plan_enqueue_pre(Plan &p, hipStream_t s) {
     hipError_t e = hipMemcpyAsync(p.d_cpred, p.h_cpred, (size_t)p.nc * sizeof(ChunkPred), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
   }
-  if (p.q.op == kOpFilter || p.q.op == kOpFilterMaterialize || (p.q.op == kOpMaterialize && p.has_mask))
-    launch::zero_u32(s, (uint32_t *)p.bits, (uint32_t)((p.words + 2) * 2));
+  if (writes_bits(p)) launch::zero_u32(s, (uint32_t *)p.bits, (uint32_t)((p.words + 2) * 2));
   launch::chunk_select(s, p.d_view, p.has_mask ? p.mask : nullptr, p.q.chunk_id, p.sel, p.nc);
   return hipGetLastError();
 }
@@ -202,11 +206,20 @@ hipError_t plan_enqueue_post(Plan &p, hipStream_t s) {
   Column &c = *p.c;
   const int op = p.q.op;
   if (is_agg(op)) {
-    launch::aggregate(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, args_of(p), p.nc);
-    if (!p.insitu && op == kOpSum) launch::chunk_dd(s, p.d_view, p.sel, p.partials, p.chunk_sum, p.nc);
+    QueryArgs a = args_of(p);
+    if (op == kOpFilterSum) {
+      // filter into `bits`, then sum over them exactly as sum(Some(bits))
+      // does: chunks re-selected (pruned) by the filter's bitset
+      launch::filter(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, a, p.filter_mode);
+      launch::chunk_select(s, p.d_view, p.bits, p.q.chunk_id, p.sel, p.nc);
+      a.mask = p.bits;
+    }
+    launch::aggregate(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, a, p.nc);
+    const bool f64_sum = !p.insitu && (op == kOpSum || op == kOpFilterSum);
+    if (f64_sum) launch::chunk_dd(s, p.d_view, p.sel, p.partials, p.chunk_sum, p.nc);
     hipMemcpyAsync(p.h_sel, p.sel, (size_t)p.nc * 4, hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(p.h_acc, p.acc, (size_t)p.nc * 32, hipMemcpyDeviceToHost, s);
-    if (!p.insitu && op == kOpSum) hipMemcpyAsync(p.h_csum, p.chunk_sum, (size_t)p.nc * 16, hipMemcpyDeviceToHost, s);
+    if (f64_sum) hipMemcpyAsync(p.h_csum, p.chunk_sum, (size_t)p.nc * 16, hipMemcpyDeviceToHost, s);
   } else if (op == kOpFilter) {
     launch::filter(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, args_of(p), p.filter_mode);
     if (p.words) hipMemcpyAsync(p.h_bits, p.bits, p.words * 8, hipMemcpyDeviceToHost, s);
@@ -234,8 +247,11 @@ int plan_init(Plan &p, const ebi_query &q) {
     case kOpFilter:
       if (!q.out_bitset && N) return EBI_INVALID_ARGUMENT;
       break;
-    case kOpSum: case kOpMin: case kOpMax:
+    case kOpSum: case kOpMin: case kOpMax: case kOpFilterSum:
       if (!q.out_scalar) return EBI_INVALID_ARGUMENT;
+      break;
+    case kOpDecode:
+      if (!q.out_values && N) return EBI_INVALID_ARGUMENT;
       break;
     default:
       return EBI_INVALID_ARGUMENT;
@@ -275,6 +291,14 @@ int plan_init(Plan &p, const ebi_query &q) {
       p.cpred = scheme == kBUFF;
       p.filter_mode = scheme == kBUFF ? 1 : 2;
       break;
+    case kOpFilterSum:
+      p.decode = !p.insitu;
+      p.cpred = p.insitu;
+      p.filter_mode = 1;  // as kOpFilter
+      break;
+    case kOpDecode:
+      p.decode = true;
+      break;
     default:
       p.decode = !p.insitu;
       break;
@@ -300,7 +324,7 @@ void plan_bind(Plan &p, uint8_t *dev, uint8_t *pin) {
   p.chunk_sum = at<double>(dev, L.csum);
   p.block = at<uint64_t>(dev, L.block);
   p.d_n = p.block + launch::compact_blocks(p.words);
-  p.values = at<double>(dev, L.values);
+  p.values = p.q.op == kOpDecode ? p.q.out_values : at<double>(dev, L.values);
   p.out = at<double>(dev, L.out);
   const PinLayout P = pin_layout_of(p);
   p.h_mask = at<uint64_t>(pin, P.mask);
@@ -365,9 +389,10 @@ int plan_finish(Plan &p) {
     if (p.words) std::memcpy(p.q.out_bitset, p.h_bits, p.words * 8);
     return EBI_OK;
   }
+  if (op == kOpDecode) return EBI_OK;
   // Decoder fold over the selected chunks, api/decoder.rs:323-439.
-  const int agg = op == kOpSum ? 0 : op == kOpMin ? 1 : 2;
-  const bool mask = p.has_mask;
+  const int agg = op == kOpMin ? 1 : op == kOpMax ? 2 : 0;
+  const bool mask = p.has_mask || op == kOpFilterSum;
   double result = agg == 0 ? 0.0 : agg == 1 ? INFINITY : -INFINITY;
   double hi = 0.0, lo = 0.0;  // double-double total for the default codecs
   for (uint32_t k = 0; k < p.nc; ++k) {

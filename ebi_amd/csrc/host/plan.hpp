@@ -38,6 +38,8 @@ enum PlanOp : int {
   kOpSum = EBI_OP_SUM,
   kOpMin = EBI_OP_MIN,
   kOpMax = EBI_OP_MAX,
+  kOpFilterSum = EBI_OP_FILTER_SUM,
+  kOpDecode = EBI_OP_DECODE,
 };
 
 struct PlanNeed {

@@ -176,13 +176,19 @@ enum {
     EBI_OP_FILTER_MATERIALIZE = 2, /* out_values (room for N), n_out */
     EBI_OP_SUM = 3,                /* out_scalar */
     EBI_OP_MIN = 4,
-    EBI_OP_MAX = 5
+    EBI_OP_MAX = 5,
+    /* sum over the records the predicate selects: the same result as FILTER
+     * followed by SUM with that bitset (bit for bit), without the round trip */
+    EBI_OP_FILTER_SUM = 6,         /* out_scalar */
+    /* whole-column decode into the DEVICE buffer out_values (N doubles),
+     * as ebi_hip_decode_device */
+    EBI_OP_DECODE = 7
 };
 typedef struct ebi_query {
     int32_t op;              /* EBI_OP_* */
     int32_t reserved;
     const ebi_hip_col *col;
-    ebi_pred pred;           /* FILTER and FILTER_MATERIALIZE */
+    ebi_pred pred;           /* FILTER, FILTER_MATERIALIZE, FILTER_SUM */
     const uint64_t *bitset;  /* nullable input bitmask over global ids */
     int64_t chunk_id;        /* -1 for every chunk */
     double *out_values;

@@ -187,3 +187,35 @@ def test_query_entry_point_matches_methods(ctx, columns):
     p = E.Predicate.range(E.RangeValue.inclusive(99.0), E.RangeValue.inclusive(101.0))
     assert np.array_equal(E.run_query(ctx, E.Query(E.OP_FILTER, d, predicate=p)), d.filter(p))
     assert same(E.run_query(ctx, E.Query(E.OP_SUM, d)), d.sum())
+
+
+def test_filter_sum_equals_filter_then_sum(ctx, columns):
+    for d, o, vals in columns:
+        lo, hi = np.quantile(vals, [0.2, 0.7])
+        p = E.Predicate.range(E.RangeValue.inclusive(float(lo)), E.RangeValue.inclusive(float(hi)))
+        ids = d.filter(p)
+        want = d.sum(bitmask=ids) if ids.size else None
+        got = E.run_query(ctx, E.Query(E.OP_FILTER_SUM, d, predicate=p))
+        if want is not None:
+            assert same(got, want), d.scheme
+        # with an input bitmask and a chunk restriction too
+        m = np.arange(0, d.number_of_records, 3)
+        ids = d.filter(p, bitmask=m, chunk_id=0)
+        got = E.run_query(ctx, E.Query(E.OP_FILTER_SUM, d, predicate=p, bitmask=m, chunk_id=0))
+        if ids.size:
+            assert same(got, d.sum(bitmask=ids, chunk_id=0)), d.scheme
+
+
+def test_decode_op_writes_device_buffer(ctx, columns):
+    with E.Server(ctx, device_budget=1 << 28) as srv:
+        bufs, qs = [], []
+        for d, o, vals in columns:
+            b = ctx.alloc(8 * d.number_of_records)
+            bufs.append(b)
+            qs.append(E.Query(E.OP_DECODE, d, out=b))
+        ts = srv.submit(qs)
+        for t, b, (d, o, vals) in zip(ts, bufs, columns):
+            srv.wait(t)
+            assert same(b.to_host(count=d.number_of_records), o.materialize())
+    for b in bufs:
+        b.free()
