@@ -106,6 +106,11 @@ class _ServeStats(C.Structure):
                                            "device_bytes_peak")]
 
 
+class _BatchCfg(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("c128_block", C.c_uint32), ("overlap", C.c_uint32),
+                ("reserved", C.c_uint32), ("tuned_ms", C.c_double)]
+
+
 class _EncCfg(C.Structure):
     _fields_ = [
         ("scheme", C.c_int32),
@@ -156,6 +161,7 @@ _SIGS = {
     "ebi_hip_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "ebi_hip_device_free": (C.c_int, [_P, _P]),
     "ebi_hip_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "ebi_hip_device_zero": (C.c_int, [_P, _P, C.c_size_t]),
     "ebi_hip_materialize": (C.c_int, [_P, _P, _P, C.c_int64, _P, C.POINTER(C.c_size_t)]),
     "ebi_hip_filter": (C.c_int, [_P, _P, C.POINTER(_Pred), _P, C.c_int64, _P]),
     "ebi_hip_filter_materialize": (C.c_int, [_P, _P, C.POINTER(_Pred), _P, C.c_int64, _P, C.POINTER(C.c_size_t)]),
@@ -163,6 +169,10 @@ _SIGS = {
     "ebi_hip_min": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
     "ebi_hip_max": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
     "ebi_hip_query": (C.c_int, [_P, C.POINTER(_Query)]),
+    "ebi_hip_batch_create": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_int, C.POINTER(_P)]),
+    "ebi_hip_batch_run": (C.c_int, [_P, _P, C.c_int]),
+    "ebi_hip_batch_config_get": (C.c_int, [_P, C.POINTER(_BatchCfg)]),
+    "ebi_hip_batch_destroy": (C.c_int, [_P, _P]),
     "ebi_hip_serve_create": (C.c_int, [_P, C.POINTER(_ServeOpts), C.POINTER(_P)]),
     "ebi_hip_serve_submit": (C.c_int, [_P, C.POINTER(_Query), C.c_size_t, C.POINTER(C.c_uint64)]),
     "ebi_hip_serve_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int)]),
@@ -462,6 +472,9 @@ class DeviceBuffer:
         _check(lib().ebi_hip_copy_to_host(self.ctx._h, out.ctypes.data, self.ptr, n * it), self.ctx, "copy_to_host")
         return out
 
+    def zero(self):
+        _check(lib().ebi_hip_device_zero(self.ctx._h, self.ptr, self.nbytes & ~3), self.ctx, "device_zero")
+
     def free(self):
         if self.ptr:
             lib().ebi_hip_device_free(self.ctx._h, self.ptr)
@@ -742,6 +755,45 @@ def decode_batch(ctx: Context, decoders, outs) -> None:
     cols = (_P * n)(*[d._h for d in decoders])
     ptrs = (_P * n)(*[o.ptr for o in outs])
     _check(lib().ebi_hip_decode_batch(ctx._h, cols, n, ptrs), ctx, "decode_batch")
+
+
+class Batch:
+    """ebi_hip_batch_*: a prepared (optionally autotuned) batched decode of
+    several resident columns into device buffers, replayed as a hipGraph."""
+
+    def __init__(self, ctx: Context, decoders, outs, autotune: bool = True):
+        self.ctx = ctx
+        self._keep = (list(decoders), list(outs))
+        n = len(decoders)
+        cols = (_P * n)(*[d._h for d in decoders])
+        ptrs = (_P * n)(*[o.ptr for o in outs])
+        h = _P()
+        _check(lib().ebi_hip_batch_create(ctx._h, cols, n, ptrs, 1 if autotune else 0, C.byref(h)), ctx,
+               "batch_create")
+        self._h = h
+
+    def run(self, eager: bool = False) -> float:
+        """One replay; returns its device ms."""
+        _check(lib().ebi_hip_batch_run(self.ctx._h, self._h, 1 if eager else 0), self.ctx, "batch_run")
+        return self.ctx.last_device_ms
+
+    @property
+    def config(self) -> dict:
+        c = _BatchCfg()
+        _check(lib().ebi_hip_batch_config_get(self._h, C.byref(c)), None, "batch_config")
+        return {"blocks_per_cu": c.blocks_per_cu, "c128_block": c.c128_block, "overlap": bool(c.overlap),
+                "tuned_ms": c.tuned_ms}
+
+    def close(self):
+        if getattr(self, "_h", None) and self.ctx._h:
+            lib().ebi_hip_batch_destroy(self.ctx._h, self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gpu_available() -> bool:

@@ -129,10 +129,11 @@ int check_chunk_errors(ebi_hip_ctx *ctx, const Column &c, int64_t chunk_id) {
 int enqueue_decode(ebi_hip_ctx *ctx, Column &c) {
   const uint32_t jobs = c.n_tile_jobs + (ctx->xor_mode == 0 ? c.n_seg_jobs : 0);
   launch::decode(ctx->stream, c.d_view, c.d_prefix, 1, jobs, ctx->d_flags + 1, ctx->d_flags,
-                 (uint32_t)ctx->cu_count * 8);
+                 (uint32_t)ctx->cu_count * ctx->blocks_per_cu);
   hipEventRecord(ctx->ev_mid, ctx->stream);
   if (c.chimp128)
-    launch::chimp128_par(ctx->stream, c.d_view, c.d_prefix + 4, 1, (uint32_t)c.chunks.size(), ctx->d_flags);
+    launch::chimp128_par(ctx->stream, c.d_view, c.d_prefix + 4, 1, (uint32_t)c.chunks.size(), ctx->d_flags,
+                         ctx->c128_block);
   else if (c.xor_codec && ctx->xor_mode != 0)
     launch::serial(ctx->stream, c.d_view, c.d_prefix + 2, 1, c.n_serial_jobs, nullptr, ctx->d_flags, true, false);
   hipError_t e = hipGetLastError();
@@ -401,6 +402,8 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     c->cu_count = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev_c0) != hipSuccess || hipEventCreate(&c->ev_c1) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess || hipEventCreate(&c->ev_mid) != hipSuccess ||
       hipEventCreate(&c->ev_ph[0]) != hipSuccess || hipEventCreate(&c->ev_ph[1]) != hipSuccess ||
       hipEventCreate(&c->ev_ph[2]) != hipSuccess || hipEventCreate(&c->ev_ph[3]) != hipSuccess ||
@@ -428,6 +431,13 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   }
   const char *mode = std::getenv("EBI_HIP_XOR_MODE");
   if (mode && std::strcmp(mode, "serial") == 0) c->xor_mode = 1;
+  // launch-shape overrides (the batch tuner picks its own per batch)
+  if (const char *v = std::getenv("EBI_HIP_BLOCKS_PER_CU")) c->blocks_per_cu = std::max(1, std::min(64, std::atoi(v)));
+  if (const char *v = std::getenv("EBI_HIP_C128_BLOCK")) {
+    const int b = std::atoi(v);
+    if (b == 1024 || b == 2048 || b == 4096) c->c128_block = (uint32_t)b;
+  }
+  if (const char *v = std::getenv("EBI_HIP_OVERLAP")) c->overlap = std::atoi(v) != 0;
   *out = c;
   return EBI_OK;
 }
@@ -449,6 +459,12 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   if (ctx->ev_mid) hipEventDestroy(ctx->ev_mid);
+  if (ctx->ev_c0) hipEventDestroy(ctx->ev_c0);
+  if (ctx->ev_c1) hipEventDestroy(ctx->ev_c1);
+  if (ctx->aux) {
+    hipStreamSynchronize(ctx->aux);
+    hipStreamDestroy(ctx->aux);
+  }
   for (hipEvent_t e : ctx->ev_ph)
     if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -601,6 +617,19 @@ extern "C" int ebi_hip_device_free(ebi_hip_ctx *ctx, void *d_ptr) {
   return EBI_OK;
 }
 
+extern "C" int ebi_hip_device_zero(ebi_hip_ctx *ctx, void *d_ptr, size_t bytes) {
+  EBI_GUARD(ctx);
+  if ((!d_ptr && bytes) || (bytes & 3)) return EBI_INVALID_ARGUMENT;
+  const size_t step = (size_t)1 << 33;  // 2^31 words per launch
+  for (size_t off = 0; off < bytes; off += step) {
+    const size_t n = std::min(bytes - off, step) / 4;
+    launch::zero_u32(ctx->stream, reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(d_ptr) + off), (uint32_t)n);
+  }
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "device zero");
+  return EBI_OK;
+}
+
 extern "C" int ebi_hip_copy_to_host(ebi_hip_ctx *ctx, void *h_dst, const void *d_src, size_t bytes) {
   EBI_GUARD(ctx);
   if (!bytes) return EBI_OK;
@@ -653,14 +682,29 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   launch::zero_u32(ctx->stream, ctx->d_flags, 2);
   hipEventRecord(ctx->ev0, ctx->stream);
   const ColView *dv = (const ColView *)(d + pre_bytes);
+  const uint32_t *pre128_d = (const uint32_t *)d + (n_cols + 1);
+  // The Chimp128 columns go to the aux stream so that their latency-bound
+  // kernel fills the CUs beside the persistent k_decode.
+  const bool side = ctx->overlap && jobs128 && jobs;
+  hipStream_t s128 = side ? ctx->aux : ctx->stream;
+  if (side) hipStreamWaitEvent(ctx->aux, ctx->ev0, 0);
   launch::decode(ctx->stream, dv, (const uint32_t *)d, (uint32_t)n_cols, (uint32_t)jobs, ctx->d_flags + 1,
-                 ctx->d_flags, (uint32_t)ctx->cu_count * 8);
+                 ctx->d_flags, (uint32_t)ctx->cu_count * ctx->blocks_per_cu);
   hipEventRecord(ctx->ev_mid, ctx->stream);
-  launch::chimp128_par(ctx->stream, dv, (const uint32_t *)d + (n_cols + 1), (uint32_t)n_cols, (uint32_t)jobs128,
-                       ctx->d_flags);
+  hipEventRecord(ctx->ev_c0, s128);
+  launch::chimp128_par(s128, dv, pre128_d, (uint32_t)n_cols, (uint32_t)jobs128, ctx->d_flags, ctx->c128_block);
+  hipEventRecord(ctx->ev_c1, s128);
+  if (side) hipStreamWaitEvent(ctx->stream, ctx->ev_c1, 0);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "batch launch");
   hipEventRecord(ctx->ev1, ctx->stream);
   if ((rc = sync_and_check(ctx, "batch decode"))) return rc;
-  record_decode_times(ctx);
+  float a = 0, b = 0, t = 0;
+  hipEventElapsedTime(&a, ctx->ev0, ctx->ev_mid);
+  hipEventElapsedTime(&b, ctx->ev_c0, ctx->ev_c1);
+  hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+  ctx->phase_ms[0] = a;
+  ctx->phase_ms[1] = b;
+  ctx->phase_ms[2] = ctx->phase_ms[3] = 0.0;
+  ctx->last_ms = t;
   return EBI_OK;
 }

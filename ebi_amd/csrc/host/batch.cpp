@@ -1,0 +1,239 @@
+// Prepared batched decode: a fixed set of resident columns decoded into fixed
+// device outputs, replayed many times.  The job-space arguments are uploaded
+// once; an autotuner times candidate launch shapes on this very batch
+// (persistent grid size, Chimp128 LDS block, Chimp128 kernel beside k_decode
+// or after it) and keeps the fastest; the launch sequence is captured in a
+// hipGraph and replayed with one launch.  No counterpart in lemolatoon/ebi,
+// whose Decoder decodes chunk after chunk on the CPU (api/decoder.rs:202-241).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../../include/ebi_hip.h"
+#include "../kernels/launch.hpp"
+#include "column.hpp"
+#include "internal.hpp"
+
+using namespace ebi;
+
+struct ebi_hip_batch {
+  std::vector<Column *> cols;
+  std::vector<double *> outs;
+  DevBuf args;  // prefix[n + 1] | prefix128[n + 1] | pad | ColView[n]
+  uint64_t pre_bytes = 0;
+  uint32_t n_cols = 0, jobs = 0, jobs128 = 0;
+  ebi_batch_config cfg{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;  // fork / join of the aux branch (no timing)
+};
+
+namespace {
+
+const ColView *views_of(const ebi_hip_batch *b) {
+  return reinterpret_cast<const ColView *>(static_cast<uint8_t *>(b->args.p) + b->pre_bytes);
+}
+const uint32_t *prefix_of(const ebi_hip_batch *b) { return static_cast<const uint32_t *>(b->args.p); }
+
+// Issues the batch on ctx->stream with configuration c.  timed: record the
+// phase events (eager runs only; a captured sequence carries no timing).
+void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool timed) {
+  const bool side = c.overlap && b->jobs && b->jobs128;
+  hipStream_t s = ctx->stream, s128 = side ? ctx->aux : ctx->stream;
+  launch::zero_u32(s, ctx->d_flags, 2);
+  if (timed) hipEventRecord(ctx->ev0, s);
+  if (side) {
+    hipEventRecord(b->fork, s);
+    hipStreamWaitEvent(ctx->aux, b->fork, 0);
+  }
+  launch::decode(s, views_of(b), prefix_of(b), b->n_cols, b->jobs, ctx->d_flags + 1, ctx->d_flags,
+                 (uint32_t)ctx->cu_count * c.blocks_per_cu);
+  if (timed) hipEventRecord(ctx->ev_mid, s);
+  if (timed) hipEventRecord(ctx->ev_c0, s128);
+  launch::chimp128_par(s128, views_of(b), prefix_of(b) + (b->n_cols + 1), b->n_cols, b->jobs128, ctx->d_flags,
+                       c.c128_block);
+  if (timed) hipEventRecord(ctx->ev_c1, s128);
+  if (side) {
+    hipEventRecord(b->join, ctx->aux);
+    hipStreamWaitEvent(s, b->join, 0);
+  }
+  if (timed) hipEventRecord(ctx->ev1, s);
+}
+
+void read_times(ebi_hip_ctx *ctx) {
+  float a = 0, c = 0, t = 0;
+  hipEventElapsedTime(&a, ctx->ev0, ctx->ev_mid);
+  hipEventElapsedTime(&c, ctx->ev_c0, ctx->ev_c1);
+  hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+  ctx->phase_ms[0] = a;
+  ctx->phase_ms[1] = c;
+  ctx->phase_ms[2] = ctx->phase_ms[3] = 0.0;
+  ctx->last_ms = t;
+}
+
+// Median device time of `reps` eager runs of configuration c (after one
+// warm-up); < 0 on error.
+double time_config(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, int reps) {
+  std::vector<double> ms;
+  for (int r = 0; r <= reps; ++r) {
+    enqueue(ctx, b, c, true);
+    if (sync_and_check(ctx, "batch tuning")) return -1.0;
+    read_times(ctx);
+    if (r) ms.push_back(ctx->last_ms);
+  }
+  std::sort(ms.begin(), ms.end());
+  return ms[ms.size() / 2];
+}
+
+int capture(ebi_hip_ctx *ctx, ebi_hip_batch *b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch capture begin");
+  enqueue(ctx, b, b->cfg, false);
+  e = hipStreamEndCapture(ctx->stream, &b->graph);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch capture end");
+  e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch graph instantiate");
+  return EBI_OK;
+}
+
+void destroy(ebi_hip_batch *b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->join) hipEventDestroy(b->join);
+  b->args.release();
+  delete b;
+}
+
+}  // namespace
+
+extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols, size_t n_cols,
+                                    double *const *d_outs, int autotune, ebi_hip_batch **out) {
+  EBI_GUARD(ctx);
+  if (!cols || !d_outs || !out || n_cols == 0 || n_cols > (1u << 16)) return EBI_INVALID_ARGUMENT;
+  *out = nullptr;
+  int rc;
+  for (size_t i = 0; i < n_cols; ++i) {
+    if (!cols[i] || (!d_outs[i] && cols[i]->c.meta.number_of_records)) return EBI_INVALID_ARGUMENT;
+    if ((rc = check_chunk_errors(ctx, cols[i]->c, -1))) return rc;
+  }
+  ebi_hip_batch *b = new (std::nothrow) ebi_hip_batch();
+  if (!b) return EBI_OOM;
+  b->n_cols = (uint32_t)n_cols;
+  b->pre_bytes = ((2 * (n_cols + 1) * 4 + 255) / 256) * 256;
+  std::vector<uint8_t> host(b->pre_bytes + n_cols * sizeof(ColView));
+  uint32_t *pre = reinterpret_cast<uint32_t *>(host.data()), *pre128 = pre + (n_cols + 1);
+  ColView *views = reinterpret_cast<ColView *>(host.data() + b->pre_bytes);
+  uint64_t jobs = 0, jobs128 = 0;
+  for (size_t i = 0; i < n_cols; ++i) {
+    Column &c = const_cast<ebi_hip_col *>(cols[i])->c;
+    b->cols.push_back(&c);
+    b->outs.push_back(d_outs[i]);
+    pre[i] = (uint32_t)jobs;
+    pre128[i] = (uint32_t)jobs128;
+    views[i] = view_of(c, d_outs[i]);
+    jobs += c.n_tile_jobs + c.n_seg_jobs;
+    if (c.chimp128) jobs128 += c.chunks.size();
+  }
+  pre[n_cols] = (uint32_t)jobs;
+  pre128[n_cols] = (uint32_t)jobs128;
+  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32)) {
+    destroy(b);
+    return fail(ctx, EBI_UNSUPPORTED, "batch too large");
+  }
+  b->jobs = (uint32_t)jobs;
+  b->jobs128 = (uint32_t)jobs128;
+  hipError_t e;
+  if (b->args.ensure(host.size()) ||
+      (e = hipMemcpy(b->args.p, host.data(), host.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+    destroy(b);
+    return fail(ctx, EBI_OOM, "batch arguments");
+  }
+  b->cfg.blocks_per_cu = ctx->blocks_per_cu;
+  b->cfg.c128_block = ctx->c128_block;
+  b->cfg.overlap = ctx->overlap ? 1 : 0;
+  if (autotune) {
+    // coordinate search: grid size, then the Chimp128 block, then overlap
+    double best = time_config(ctx, b, b->cfg, 2);
+    if (best < 0) {
+      destroy(b);
+      return EBI_HIP;
+    }
+    auto consider = [&](ebi_batch_config c) {
+      const double t = time_config(ctx, b, c, 2);
+      if (t >= 0 && t < best) {
+        best = t;
+        b->cfg = c;
+      }
+    };
+    for (uint32_t bpc : {4u, 8u, 16u})
+      if (bpc != b->cfg.blocks_per_cu) {
+        ebi_batch_config c = b->cfg;
+        c.blocks_per_cu = bpc;
+        consider(c);
+      }
+    if (b->jobs128)
+      for (uint32_t blk : {1024u, 2048u, 4096u})
+        if (blk != b->cfg.c128_block) {
+          ebi_batch_config c = b->cfg;
+          c.c128_block = blk;
+          consider(c);
+        }
+    if (b->jobs && b->jobs128) {
+      ebi_batch_config c = b->cfg;
+      c.overlap = !c.overlap;
+      consider(c);
+    }
+    b->cfg.tuned_ms = best;
+  }
+  if ((rc = capture(ctx, b))) {
+    destroy(b);
+    return rc;
+  }
+  *out = b;
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_batch_run(ebi_hip_ctx *ctx, ebi_hip_batch *b, int eager) {
+  EBI_GUARD(ctx);
+  if (!b) return EBI_INVALID_ARGUMENT;
+  if (eager) {
+    enqueue(ctx, b, b->cfg, true);
+    int rc = sync_and_check(ctx, "batch run");
+    if (rc) return rc;
+    read_times(ctx);
+    return EBI_OK;
+  }
+  hipEventRecord(ctx->ev0, ctx->stream);
+  hipError_t e = hipGraphLaunch(b->exec, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "batch graph launch");
+  hipEventRecord(ctx->ev1, ctx->stream);
+  int rc = sync_and_check(ctx, "batch run");
+  if (rc) return rc;
+  float t = 0;
+  hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+  ctx->last_ms = t;
+  ctx->phase_ms[0] = ctx->phase_ms[1] = ctx->phase_ms[2] = ctx->phase_ms[3] = 0.0;
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_batch_config_get(const ebi_hip_batch *b, ebi_batch_config *cfg) {
+  if (!b || !cfg) return EBI_INVALID_ARGUMENT;
+  *cfg = b->cfg;
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_batch_destroy(ebi_hip_ctx *ctx, ebi_hip_batch *b) {
+  EBI_GUARD(ctx);
+  if (!b) return EBI_INVALID_ARGUMENT;
+  hipStreamSynchronize(ctx->stream);
+  hipStreamSynchronize(ctx->aux);
+  destroy(b);
+  return EBI_OK;
+}

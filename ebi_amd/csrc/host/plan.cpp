@@ -469,7 +469,7 @@ void group_bind(PlanGroup &g, uint8_t *dev, uint8_t *pin) {
   for (size_t i = 0; i < n; ++i) g.plans[i]->d_view = reinterpret_cast<const ColView *>(dev) + i;
 }
 
-hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count) {
+hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blocks_per_cu, uint32_t c128_block) {
   const size_t n = g.plans.size();
   if (!n) return hipSuccess;
   const uint64_t vbytes = round_up(n * sizeof(ColView), kAlign);
@@ -498,8 +498,8 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count) {
     if ((e = plan_enqueue_pre(*p, s)) != hipSuccess) return e;
   const ColView *dv = reinterpret_cast<const ColView *>(g.dev);
   const uint32_t *dpre = reinterpret_cast<const uint32_t *>(g.dev + vbytes);
-  launch::decode(s, dv, dpre, (uint32_t)n, (uint32_t)jobs, g.d_flags + 1, g.d_flags, (uint32_t)cu_count * 8);
-  launch::chimp128_par(s, dv, dpre + (pbytes / 4), (uint32_t)n, (uint32_t)jobs128, g.d_flags);
+  launch::decode(s, dv, dpre, (uint32_t)n, (uint32_t)jobs, g.d_flags + 1, g.d_flags, (uint32_t)cu_count * blocks_per_cu);
+  launch::chimp128_par(s, dv, dpre + (pbytes / 4), (uint32_t)n, (uint32_t)jobs128, g.d_flags, c128_block);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   for (Plan *p : g.plans)
     if ((e = plan_enqueue_post(*p, s)) != hipSuccess) return e;

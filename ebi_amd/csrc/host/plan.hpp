@@ -96,7 +96,9 @@ struct PlanGroup {
 };
 uint64_t group_block_bytes(size_t n_plans);
 void group_bind(PlanGroup &g, uint8_t *dev, uint8_t *pin);
-// Issues the whole group; `cu_count` sizes the persistent decode grid.
-hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count);
+// Issues the whole group; the persistent decode grid is cu_count *
+// blocks_per_cu blocks, c128_block the Chimp128 kernel's LDS block.
+hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blocks_per_cu = 8,
+                         uint32_t c128_block = 2048);
 
 }  // namespace ebi

@@ -326,7 +326,7 @@ constexpr int32_t kRoot = INT32_MIN;
 __device__ __forceinline__ uint32_t c128_slot(uint32_t k) { return k + (k >> 6); }
 
 __device__ uint32_t chimp128_parse_seg(const uint8_t *data, uint32_t size_bits, uint32_t i0, uint32_t i1,
-                                        uint32_t bitpos, uint32_t lead, uint32_t base, int32_t *S, uint64_t *X,
+                                        uint32_t bitpos, uint32_t lead, uint32_t base, uint8_t *S, uint64_t *X,
                                         uint32_t *F) {
   LaneReader r;
   r.init(data, bitpos);
@@ -367,11 +367,14 @@ __device__ uint32_t chimp128_parse_seg(const uint8_t *data, uint32_t size_bits, 
     if (src >= (int32_t)i0) {
       // source in this lane's own segment: already resolved to a source
       // outside it, so fold it in now (value_i = value_ext ^ x_src ^ x)
-      const uint32_t q = c128_slot((uint32_t)src - base);
+      const uint32_t sq = (uint32_t)src - base;
+      const uint32_t q = c128_slot(sq);
       x ^= X[q];
-      src = S[q];
+      src = S[q] ? (int32_t)(base + sq) - (int32_t)S[q] : kRoot;
     }
-    S[c128_slot(k)] = src;
+    // Every source is now the root or in [i0 - 128, i0) (induction over the
+    // segment), so i - src is in [1, 191]: one byte, 0 for the root.
+    S[c128_slot(k)] = src == kRoot ? 0 : (uint8_t)((int32_t)i - src);
     X[c128_slot(k)] = x;
     if (is00) atomicOr(F + (k >> 5), 1u << (k & 31));
   }
@@ -393,10 +396,10 @@ __device__ __forceinline__ void plain_tile(const DevChunk &c, uint32_t t, double
     const uint32_t i = b + k * 128 + lane * 2;
     if (pair_ok && i + 1 < c.n) {
       const double2 x = *reinterpret_cast<const double2 *>(src + i);
-      *reinterpret_cast<double2 *>(out + c.rec_off + i) = x;
+      st_pair(out + c.rec_off + i, x.x, x.y);
     } else {
-      if (i < c.n) out[c.rec_off + i] = src[i];
-      if (i + 1 < c.n) out[c.rec_off + i + 1] = src[i + 1];
+      if (i < c.n) st_one(out + c.rec_off + i, src[i]);
+      if (i + 1 < c.n) st_one(out + c.rec_off + i + 1, src[i + 1]);
     }
   }
 }
@@ -448,17 +451,45 @@ __device__ __forceinline__ double buff_value(const DevChunk &c, uint64_t d) {
   return ldexp((double)(int64_t)((uint64_t)c.base + d), -(int)c.frac);
 }
 
-__device__ __forceinline__ void buff_tile(const DevChunk &c, uint32_t t, double *out) {
+// Stores a tile whose lane l holds records 8l .. 8l+7 (v[]) as coalesced
+// 16-byte stores: the wave transposes through `stage` (kStageStride doubles
+// per lane), then store k of lane l covers records 128k + 2l and 128k + 2l + 1,
+// so each store instruction writes 1 KiB contiguous.  The 80-byte lane stride
+// keeps both the 16-byte LDS writes and reads free of bank conflicts.  Every
+// lane of the wave must call it (it synchronises the wave).  nv = records of
+// the tile to write (<= kTile).
+constexpr uint32_t kStageStride = 10;
+__device__ __forceinline__ void tile_store(double *dst, const double v[8], uint32_t nv, double *stage) {
+  const uint32_t lane = lane_id();
+  double2 *mine = reinterpret_cast<double2 *>(stage + lane * kStageStride);
+  mine[0] = make_double2(v[0], v[1]);
+  mine[1] = make_double2(v[2], v[3]);
+  mine[2] = make_double2(v[4], v[5]);
+  mine[3] = make_double2(v[6], v[7]);
+  wave_sync();
+  const bool aligned = (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t r = 128 * k + 2 * lane;
+    const double2 x = *reinterpret_cast<const double2 *>(stage + (r >> 3) * kStageStride + (r & 7));
+    if (aligned && r + 1 < nv) {
+      st_pair(dst + r, x.x, x.y);
+    } else {
+      if (r < nv) st_one(dst + r, x.x);
+      if (r + 1 < nv) st_one(dst + r + 1, x.y);
+    }
+  }
+  wave_sync();  // the stage is reused by the wave's next job
+}
+
+__device__ __forceinline__ void buff_tile(const DevChunk &c, uint32_t t, double *out, double *stage) {
   const uint32_t i0 = t * kTile + lane_id() * 8;
-  if (i0 >= c.n) return;
   uint64_t d[8];
-  buff_deltas(c, i0, d);
+  buff_deltas(c, i0, d);  // zeros past the chunk's end
   double v[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) v[j] = buff_value(c, d[j]);
-  const uint32_t cnt = min(8u, c.n - i0);
-  double *dst = out + c.rec_off + i0;
-  store8(dst, v, cnt, ((reinterpret_cast<uintptr_t>(dst)) & 15) == 0);
+  tile_store(out + c.rec_off + (uint64_t)t * kTile, v, min(kTile, c.n - t * kTile), stage);
 }
 
 // DeltaSprintz zigzag deltas of this lane's 8 records (MSB-first W-bit
@@ -497,18 +528,15 @@ __device__ __forceinline__ void sprintz_q(const DevChunk &c, uint32_t t, int64_t
   for (int j = 0; j < 8; ++j) q[j] = (int64_t)((uint64_t)q[j] + (uint64_t)excl);
 }
 
-__device__ __forceinline__ void sprintz_tile(const DevChunk &c, uint32_t t, int64_t tile_base, double *out) {
+__device__ __forceinline__ void sprintz_tile(const DevChunk &c, uint32_t t, int64_t tile_base, double *out,
+                                             double *stage) {
   int64_t q[8];
   sprintz_q(c, t, tile_base, q);
-  const uint32_t i0 = t * kTile + lane_id() * 8;
-  if (i0 >= c.n) return;
   double v[8];
   const double sc = (double)c.scale;
 #pragma unroll
   for (int j = 0; j < 8; ++j) v[j] = (double)q[j] / sc;  // quantized as f64 / scale as f64
-  const uint32_t cnt = min(8u, c.n - i0);
-  double *dst = out + c.rec_off + i0;
-  store8(dst, v, cnt, ((reinterpret_cast<uintptr_t>(dst)) & 15) == 0);
+  tile_store(out + c.rec_off + (uint64_t)t * kTile, v, min(kTile, c.n - t * kTile), stage);
 }
 
 __device__ __forceinline__ int64_t sprintz_tile_sum(const DevChunk &c, uint32_t t) {
@@ -544,6 +572,23 @@ __device__ __forceinline__ uint64_t rle_tile_sum(const DevChunk &c, uint32_t t) 
 }
 
 // starts: this wave's LDS scratch of kTile + 1 entries.
+// Largest field g in [f, nf) with starts[g] <= o, given starts[f] <= o
+// (starts is non-decreasing): first the next field, then a galloping search.
+__device__ __forceinline__ uint32_t rle_find(const uint32_t *starts, uint32_t nf, uint32_t f, uint32_t o) {
+  if (f + 1 >= nf || starts[f + 1] > o) return f;
+  uint32_t lo = f + 1, step = 1;
+  while (lo + step < nf && starts[lo + step] <= o) {
+    lo += step;
+    step <<= 1;
+  }
+  uint32_t hi = min(nf, lo + step);
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (starts[mid] <= o) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, int64_t tile_start, double *out,
                                          uint32_t *starts) {
   const uint32_t lane = lane_id();
@@ -561,16 +606,26 @@ __device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, int64_t 
   if (lane == 0) starts[kTile] = total;
   wave_sync();
   const uint32_t nf = (uint32_t)min<uint64_t>(kTile, c.fields > t * (uint64_t)kTile ? c.fields - t * (uint64_t)kTile : 0);
-  const double *vals = reinterpret_cast<const double *>(c.data + c.values_off);
+  const double *vals = reinterpret_cast<const double *>(c.data + c.values_off) + (uint64_t)t * kTile;
   double *dst = out + c.rec_off + tile_start;
-  for (uint32_t o = lane; o < total; o += kWave) {
-    // largest field f in [0, nf) with starts[f] <= o
-    uint32_t lo = 0, hi = nf;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (starts[mid] <= o) lo = mid; else hi = mid;
+  const bool aligned = (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
+  // lane l writes records 2l, 2l + 1 of every 128: one 1 KiB store per step;
+  // its field index only moves forward
+  uint32_t f = 0;
+  for (uint32_t o = 2 * lane; o < total; o += 2 * kWave) {
+    f = rle_find(starts, nf, f, o);
+    const double a = vals[f];
+    if (o + 1 < total) {
+      const double b = vals[rle_find(starts, nf, f, o + 1)];
+      if (aligned) {
+        st_pair(dst + o, a, b);
+      } else {
+        st_one(dst + o, a);
+        st_one(dst + o + 1, b);
+      }
+    } else {
+      st_one(dst + o, a);
     }
-    dst[o] = vals[t * kTile + lo];
   }
 }
 

@@ -16,6 +16,16 @@ __device__ __forceinline__ double as_f64(uint64_t u) { return __longlong_as_doub
 __device__ __forceinline__ uint64_t as_u64(double d) { return (uint64_t)__double_as_longlong(d); }
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Decoded records are written once and not read back by the kernel: the
+// tile decoders, whose every store instruction covers whole cache lines,
+// stream them past the caches (non-temporal stores).  Partial-line writers
+// (the XOR lane sinks, Chimp128) use ordinary stores, which L2 combines.
+typedef double v2f64 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_pair(double *p, double a, double b) {
+  __builtin_nontemporal_store(v2f64{a, b}, reinterpret_cast<v2f64 *>(p));
+}
+__device__ __forceinline__ void st_one(double *p, double a) { __builtin_nontemporal_store(a, p); }
+
 // Chimp / Chimp128 / ElfXor 3-bit leading-zero codes (chunk_reader/chimp.rs:40).
 __device__ __forceinline__ uint32_t lead_repr(uint32_t code) {
   // 0, 8, 12, 16, 18, 20, 22, 24 packed as bytes.

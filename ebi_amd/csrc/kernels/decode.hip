@@ -62,8 +62,9 @@ __device__ __forceinline__ uint32_t segment_lane(const ColView &cv, uint32_t ci,
 // jobs (similar cost) spread over all waves and no job counter is contended.
 __global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
                                                uint32_t n_jobs, uint32_t *ctr, uint32_t *err) {
-  __shared__ uint32_t lds_starts[4][kTile + 1];
-  __shared__ double lds_stage[4][8 * kWave];
+  // per wave: RLE field starts or the store stage (a wave uses one at a time)
+  __shared__ __align__(16) double lds[4][kStageStride * kWave];
+  static_assert(sizeof(lds[0]) >= (kTile + 1) * sizeof(uint32_t), "RLE starts fit the stage");
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
   uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
@@ -77,9 +78,11 @@ __global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint3
       if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
         switch (c.codec) {
           case kCodecPlain: plain_tile(c, jb.tile, cv.out); break;
-          case kCodecBUFF: buff_tile(c, jb.tile, cv.out); break;
-          case kCodecSprintz: sprintz_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out); break;
-          case kCodecRLE: rle_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out, lds_starts[wave]); break;
+          case kCodecBUFF: buff_tile(c, jb.tile, cv.out, lds[wave]); break;
+          case kCodecSprintz:
+            sprintz_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out, lds[wave]);
+            break;
+          case kCodecRLE: rle_tile(c, jb.tile, cv.tile_index[c.tile0 + jb.tile], cv.out, reinterpret_cast<uint32_t *>(lds[wave])); break;
           default:
             if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
             break;
@@ -88,7 +91,7 @@ __global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint3
     } else {
       const Job jb = cv.seg_jobs[local - cv.n_tile_jobs];
       if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-        const uint32_t e = segment_lane(cv, jb.chunk, jb.tile, lds_stage[wave]);
+        const uint32_t e = segment_lane(cv, jb.chunk, jb.tile, lds[wave]);
         if (e) atomicOr(err, e);
       }
     }
@@ -166,7 +169,8 @@ __global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const uint
 }
 
 // Wave-parallel Chimp128 decode (decoder/chunk_reader/chimp_n.rs:173-285).
-// One 64-thread block per chunk walks blocks of kC128Block records:
+// One 64-thread block per chunk walks blocks of kBlk records (LDS: 9 bytes a
+// record, so smaller blocks raise occupancy; the tuner picks kBlk):
 //  1. lane l parses checkpoint segment l of the block into (src, x) pairs,
 //     folding sources inside its own segment, so every source ends up in an
 //     earlier segment, the previous block, or the chunk root;
@@ -178,13 +182,13 @@ __global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const uint
 // Record n is parsed too and must be the NaN terminator via flags 01/10/11;
 // an earlier NaN from those flags is a count error, as the reference would
 // stop there.
-constexpr uint32_t kC128Block = 4096;
-
+template <uint32_t kBlk>
 __global__ void __launch_bounds__(64) k_chimp128_par(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
                                                     uint32_t *err) {
-  __shared__ uint64_t X[kC128Block + kC128Block / 64];
-  __shared__ int32_t S[kC128Block + kC128Block / 64];
-  __shared__ uint32_t F[kC128Block / 32];
+  static_assert(kBlk % kSeg == 0 && kBlk / kSeg <= kWave && kBlk >= 128, "block of at most 64 segments");
+  __shared__ uint64_t X[kBlk + kBlk / 64];
+  __shared__ uint8_t S[kBlk + kBlk / 64];
+  __shared__ uint32_t F[kBlk / 32];
   __shared__ uint64_t Cx[128];
   const uint32_t col = find_col(prefix, n_cols, blockIdx.x);
   const ColView cv = cols[col];
@@ -199,11 +203,11 @@ __global__ void __launch_bounds__(64) k_chimp128_par(const ColView *cols, const 
   const uint32_t total = c.n + 1;  // records 0..n, n = terminator
   uint32_t e = 0;
   for (uint32_t t = lane; t < 128; t += kWave) Cx[t] = 0;
-  for (uint32_t base = 0; base < total; base += kC128Block) {
-    const uint32_t bn = min(kC128Block, total - base);
-    for (uint32_t t = lane; t < kC128Block / 32; t += kWave) F[t] = 0;
+  for (uint32_t base = 0; base < total; base += kBlk) {
+    const uint32_t bn = min(kBlk, total - base);
+    for (uint32_t t = lane; t < kBlk / 32; t += kWave) F[t] = 0;
     wave_sync();
-    {
+    if (lane < kBlk / kSeg) {
       const uint32_t seg = base / kSeg + lane;
       const uint32_t i0 = seg * kSeg;
       if (i0 < base + bn) {
@@ -219,8 +223,9 @@ __global__ void __launch_bounds__(64) k_chimp128_par(const ColView *cols, const 
       const uint32_t k = kk + lane;
       if (k < bn) {
         const uint32_t ks = c128_slot(k);
-        const int32_t sk = S[ks];
-        const uint64_t src = sk == kRoot ? 0 : sk < lo ? Cx[sk - lo + 128] : X[c128_slot(sk - lo)];
+        const uint32_t off = S[ks];
+        const int32_t sk = (int32_t)(base + k) - (int32_t)off;
+        const uint64_t src = off == 0 ? 0 : sk < lo ? Cx[sk - lo + 128] : X[c128_slot(sk - lo)];
         const uint64_t v = src ^ X[ks];
         X[ks] = v;
         const uint32_t i = base + k;
@@ -324,8 +329,19 @@ void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
 }
 
 void chimp128_par(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
-                  uint32_t *err) {
-  if (n_chunks) hipLaunchKernelGGL(k_chimp128_par, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
+                  uint32_t *err, uint32_t block) {
+  if (!n_chunks) return;
+  switch (block) {
+    case 1024:
+      hipLaunchKernelGGL(k_chimp128_par<1024>, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
+      break;
+    case 4096:
+      hipLaunchKernelGGL(k_chimp128_par<4096>, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
+      break;
+    default:
+      hipLaunchKernelGGL(k_chimp128_par<2048>, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
+      break;
+  }
 }
 
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,

@@ -24,7 +24,7 @@ void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
             const Job *single, uint32_t *err, bool write, bool index);
 // Wave-parallel Chimp128 decode, one block per chunk; prefix over chunks.
 void chimp128_par(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
-                  uint32_t *err);
+                  uint32_t *err, uint32_t block = 2048);
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
               const Job *single, uint32_t *err, bool write);
 // Load-time tile index for DeltaSprintz and RLE chunks of column slot `col`.

@@ -212,6 +212,30 @@ int ebi_hip_decode_device(ebi_hip_ctx *ctx, const ebi_hip_col *col, double *d_ou
 int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
                          size_t n_cols, double *const *d_outs);
 
+/* Prepared batch: the same decode for a fixed set of columns and outputs,
+ * replayed many times.  The job-space arguments are uploaded once; with
+ * autotune != 0 the launch shape (persistent grid size, Chimp128 LDS block,
+ * Chimp128 kernel beside k_decode or after it) is timed on this batch and the
+ * fastest kept; the launch sequence is captured in a hipGraph.  Columns and
+ * outputs must outlive the batch.  No counterpart in the reference. */
+typedef struct ebi_hip_batch ebi_hip_batch;
+typedef struct ebi_batch_config {
+    uint32_t blocks_per_cu; /* persistent k_decode grid = CUs x blocks_per_cu */
+    uint32_t c128_block;    /* records per LDS block of the Chimp128 kernel */
+    uint32_t overlap;       /* 1: Chimp128 kernel on a second stream beside k_decode */
+    uint32_t reserved;
+    double tuned_ms;        /* device ms of the chosen shape (0 if not tuned) */
+} ebi_batch_config;
+int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols, size_t n_cols,
+                         double *const *d_outs, int autotune, ebi_hip_batch **out);
+/* Replays the batch and waits.  eager = 0: one hipGraph launch
+ * (ebi_hip_last_device_ms only); eager != 0: the same kernels launched one by
+ * one with per-kernel events (ebi_hip_last_phase_ms as for
+ * ebi_hip_decode_batch). */
+int ebi_hip_batch_run(ebi_hip_ctx *ctx, ebi_hip_batch *b, int eager);
+int ebi_hip_batch_config_get(const ebi_hip_batch *b, ebi_batch_config *cfg);
+int ebi_hip_batch_destroy(ebi_hip_ctx *ctx, ebi_hip_batch *b);
+
 /* Comparator: the plain per-chunk-launch form -- one kernel launch per
  * chunk and a host synchronisation after each. */
 int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col, double *d_out);
@@ -294,6 +318,8 @@ int ebi_hip_serve_destroy(ebi_hip_server *srv);
 int ebi_hip_device_alloc(ebi_hip_ctx *ctx, size_t bytes, void **d_ptr);
 int ebi_hip_device_free(ebi_hip_ctx *ctx, void *d_ptr);
 int ebi_hip_copy_to_host(ebi_hip_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+/* Zeroes `bytes` (a multiple of 4) of device memory with a fill kernel. */
+int ebi_hip_device_zero(ebi_hip_ctx *ctx, void *d_ptr, size_t bytes);
 
 /* Milliseconds of device time of the last kernel sequence issued by the
  * previous call on this ctx (hipEvent pair on the ctx stream). */

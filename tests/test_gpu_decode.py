@@ -85,3 +85,36 @@ def test_decode_batch_mixed_codecs(ctx):
         assert np.array_equal(got.view(np.uint64), w.view(np.uint64)), E.SCHEME_NAMES[d.scheme]
         o.free()
         d.close()
+
+
+def test_prepared_batch_graph_replays_match_eager(ctx):
+    """The tuned, graph-captured batch replays bit-exact, several times, with
+    eager decodes of the same columns (and syncs) between the replays."""
+    specs = [("walk2", "gorilla", 70001, 8192), ("unif3", "chimp128", 50000, 4096), ("step2", "rle", 90000, 16384),
+             ("sin4", "buff", 60000, 65536), ("walk2", "delta_sprintz", 40000, 2048), ("unif3", "elf", 30000, 1000),
+             ("sin4", "uncompressed", 20000, 333), ("walk2", "chimp", 45000, 5000)]
+    decs, wants, outs = [], [], []
+    for gen, codec, n, chunk in specs:
+        fn, dec = G.GENERATORS[gen]
+        f = E.encode(fn(n), E.CompressorConfig.by_name(codec, scale=10 ** dec), E.ChunkOption.record_count(chunk))
+        decs.append(E.Decoder(ctx, f))
+        wants.append(OracleDecoder(f).materialize())
+        outs.append(ctx.alloc(8 * n))
+    b = E.Batch(ctx, decs, outs, autotune=True)
+    cfg = b.config
+    assert cfg["blocks_per_cu"] in (4, 8, 16) and cfg["c128_block"] in (1024, 2048, 4096) and cfg["tuned_ms"] > 0
+    eager_out = ctx.alloc(8 * max(s[2] for s in specs))
+    for rep in range(3):
+        for o in outs:  # cleared between replays: every replay must write every record
+            o.zero()
+        b.run(eager=(rep == 1))
+        for d, w, o in zip(decs, wants, outs):
+            got = o.to_host(count=d.number_of_records)
+            assert np.array_equal(got.view(np.uint64), w.view(np.uint64))
+        decs[rep].decode_device(eager_out)  # the program's own eager work between replays
+    b.close()
+    eager_out.free()
+    for o in outs:
+        o.free()
+    for d in decs:
+        d.close()
