@@ -63,6 +63,25 @@ def _compile(src: Path, cc: str) -> tuple[Path, str]:
     return obj, log
 
 
+NATIVE_TESTS = PKG.parent / "tests" / "native"
+
+
+def build_native_tests(cc: str | None = None) -> list[Path]:
+    """Host-run checks of device helpers (tests/native/*.hip), built into
+    ebi_amd/_build/."""
+    cc = cc or hipcc()
+    BUILD.mkdir(exist_ok=True)
+    out = []
+    for src in sorted(NATIVE_TESTS.glob("*.hip")):
+        exe = BUILD / src.stem
+        cmd = [cc, *COMMON, *DEVICE, "-I", str(PKG.parent / "include"), str(src), "-o", str(exe)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {src}\n{r.stdout}{r.stderr}")
+        out.append(exe)
+    return out
+
+
 def build(verbose: bool = False) -> Path:
     cc = hipcc()
     BUILD.mkdir(exist_ok=True)
@@ -81,6 +100,7 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}{r.stderr}")
     os.replace(tmp, LIB)
+    build_native_tests(cc)
     return LIB
 
 

@@ -435,7 +435,7 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   if (const char *v = std::getenv("EBI_HIP_BLOCKS_PER_CU")) c->blocks_per_cu = std::max(1, std::min(64, std::atoi(v)));
   if (const char *v = std::getenv("EBI_HIP_C128_BLOCK")) {
     const int b = std::atoi(v);
-    if (b == 1024 || b == 2048 || b == 4096) c->c128_block = (uint32_t)b;
+    if (b == 512 || b == 1024 || b == 2048 || b == 4096) c->c128_block = (uint32_t)b;
   }
   if (const char *v = std::getenv("EBI_HIP_OVERLAP")) c->overlap = std::atoi(v) != 0;
   *out = c;

@@ -179,7 +179,7 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
         consider(c);
       }
     if (b->jobs128)
-      for (uint32_t blk : {1024u, 2048u, 4096u})
+      for (uint32_t blk : {512u, 1024u, 2048u, 4096u})
         if (blk != b->cfg.c128_block) {
           ebi_batch_config c = b->cfg;
           c.c128_block = blk;

@@ -59,29 +59,39 @@ template <bool kWrite>
 __device__ uint32_t gorilla_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
                                 uint32_t i1, uint32_t bitpos, uint64_t v, uint32_t L, uint32_t T,
                                 LaneSink out, XorCkpt *ck) {
-  LaneReader r;
+  BitWindow r;
   r.init(data, bitpos);
   const uint32_t last = i1 == n ? n + 1 : i1;
-  for (uint32_t i = i0; i < last; ++i) {
+  uint32_t i = i0;
+  if (i == 0 && i < last) {  // first value: 64 raw bits
+    if (ck && n) ck[0] = XorCkpt{v, r.pos, (uint8_t)L, (uint8_t)T, -1, 0};
+    v = r.field(0, 64);
+    r.consume(64);
+    r.refill();
+    if (r.pos > size_bits) return kErrOverrun;
+    if (n == 0) return v == kNan ? kErrNone : kErrCount;
+    if (v == kNan) return kErrCount;
+    if (kWrite) out.put(0, as_f64(v));
+    i = 1;
+  }
+  for (; i < last; ++i) {
     if (ck && i < n && (i % kSeg) == 0) ck[i / kSeg] = XorCkpt{v, r.pos, (uint8_t)L, (uint8_t)T, -1, 0};
-    if (i == 0) {
-      v = r.read64(64);
-    } else {
-      const uint32_t c2 = r.peek(2);
-      if (!(c2 & 2)) {
-        r.skip(1);
-      } else {
-        r.skip(2);
-        if (c2 & 1) {
-          const uint32_t h = r.read(11);
-          const uint32_t l = h >> 6, s = (h & 63) + 1;
-          if (l + s > 64) return kErrCorrupt;
-          L = l;
-          T = 64 - l - s;
-        }
-        v ^= r.read64(64 - L - T) << T;
-      }
+    // control bit 0: repeat; 10: reuse (L, T); 11: 5-bit L, 6-bit sig - 1, then sig bits
+    const uint32_t c2 = (uint32_t)(r.hi >> 62);
+    const bool nz = c2 >= 2, fresh = c2 == 3;
+    const uint32_t h = (uint32_t)(r.hi >> 51) & 0x7ff;
+    const uint32_t l = fresh ? h >> 6 : L;
+    const uint32_t sig = fresh ? (h & 63) + 1 : 64 - L - T;
+    if (fresh && l + sig > 64) return kErrCorrupt;
+    const uint32_t o = fresh ? 13 : 2;
+    if (nz) {
+      const uint32_t t = 64 - l - sig;
+      v ^= r.field(o, sig) << t;
+      L = l;
+      T = t;
     }
+    r.consume(nz ? o + sig : 1);
+    r.refill();
     if (r.pos > size_bits) return kErrOverrun;
     if (i == n) return v == kNan ? kErrNone : kErrCount;
     if (v == kNan) return kErrCount;
@@ -97,31 +107,43 @@ template <bool kWrite>
 __device__ uint32_t chimp_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
                               uint32_t i1, uint32_t bitpos, uint64_t v, uint32_t lead, LaneSink out,
                               XorCkpt *ck) {
-  LaneReader r;
+  BitWindow r;
   r.init(data, bitpos);
   const uint32_t last = i1 == n ? n + 1 : i1;
-  for (uint32_t i = i0; i < last; ++i) {
+  uint32_t i = i0;
+  if (i == 0 && i < last) {  // first value: 64 raw bits
+    if (ck && n) ck[0] = XorCkpt{v, r.pos, (uint8_t)(lead > 64 ? 255 : lead), 0, -1, 0};
+    v = r.field(0, 64);
+    r.consume(64);
+    r.refill();
+    if (r.pos > size_bits) return kErrOverrun;
+    if (n == 0) return v == kNan ? kErrNone : kErrCount;
+    if (v == kNan) return kErrCount;
+    if (kWrite) out.put(0, as_f64(v));
+    i = 1;
+  }
+  for (; i < last; ++i) {
     if (ck && i < n && (i % kSeg) == 0)
       ck[i / kSeg] = XorCkpt{v, r.pos, (uint8_t)(lead > 64 ? 255 : lead), 0, -1, 0};
-    if (i == 0) {
-      v = r.read64(64);
-    } else {
-      const uint32_t f = r.read(2);
-      if (f == 3) {
-        lead = lead_repr(r.read(3));
-        v ^= r.read64(64 - lead);
-      } else if (f == 2) {
-        if (lead > 64) return kErrCorrupt;
-        v ^= r.read64(64 - lead);
-      } else if (f == 1) {
-        const uint32_t h = r.read(9);
-        lead = lead_repr(h >> 6);
-        uint32_t sig = h & 63;
-        if (!sig) sig = 64;
-        if (sig + lead > 64) return kErrCorrupt;
-        v ^= r.read64(sig) << (64 - sig - lead);
-      }
+    // flag 00: same value; 11: 3-bit lead code, 64 - lead bits; 10: 64 - lead
+    // bits with the stored lead; 01: 9-bit header (lead code, sig), sig bits
+    const uint32_t f = (uint32_t)(r.hi >> 62);
+    const uint32_t h9 = (uint32_t)(r.hi >> 53) & 0x1ff;
+    const uint32_t c3 = h9 >> 6;  // bits 2..4: the lead code of flags 11 and 01
+    const uint32_t nl = (f == 3 || f == 1) ? lead_repr(c3) : lead;
+    if (f == 2 && lead > 64) return kErrCorrupt;
+    uint32_t sig = 64 - nl;
+    if (f == 1) {
+      sig = (h9 & 63) ? (h9 & 63) : 64;
+      if (sig + nl > 64) return kErrCorrupt;
     }
+    const uint32_t o = f == 3 ? 5 : f == 1 ? 11 : 2;
+    if (f) {
+      v ^= r.field(o, sig) << (f == 1 ? 64 - sig - nl : 0);
+      lead = nl;
+    }
+    r.consume(f ? o + sig : 2);
+    r.refill();
     if (r.pos > size_bits) return kErrOverrun;
     if (i == n) return v == kNan ? kErrNone : kErrCount;
     if (v == kNan) return kErrCount;
@@ -138,7 +160,7 @@ template <bool kWrite>
 __device__ uint32_t elf_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
                             uint32_t i1, uint32_t bitpos, uint64_t stored, uint32_t L, uint32_t T,
                             int32_t beta, LaneSink out, XorCkpt *ck) {
-  LaneReader r;
+  BitWindow r;
   r.init(data, bitpos);
   const uint32_t last = i1 == n ? n + 1 : i1;
   for (uint32_t i = i0; i < last; ++i) {
@@ -146,58 +168,61 @@ __device__ uint32_t elf_run(const uint8_t *data, uint32_t size_bits, uint32_t n,
       ck[i / kSeg] = XorCkpt{stored, r.pos, (uint8_t)(L > 64 ? 255 : L), (uint8_t)(T > 64 ? 255 : T),
                              (int8_t)(beta == INT32_MAX ? -1 : beta), 0};
     // prefix: 0 -> erased, reuse beta*; 10 -> not erased; 11 + 4 bits -> erased, new beta*
-    const uint32_t p2 = r.peek(2);
-    bool recover;
-    if (!(p2 & 2)) {
-      r.skip(1);
-      recover = true;
-    } else if (p2 == 2) {
-      r.skip(2);
-      recover = false;
-    } else {
-      r.skip(2);
-      beta = (int32_t)r.read(4);
-      recover = true;
-    }
+    const uint32_t p2 = (uint32_t)(r.hi >> 62);
+    const uint32_t pre = (p2 & 2) ? (p2 == 2 ? 2u : 6u) : 1u;
+    const bool recover = p2 != 2;
+    if (p2 == 3) beta = (int32_t)((r.hi >> 58) & 15);
     bool term = false;
-    if (i == 0) {
-      const uint32_t tz = r.read(7);
+    uint32_t used;
+    if (i == 0) {  // ElfXor first value: 7-bit trailing zeros, then 63 - tz bits
+      const uint32_t tz = (uint32_t)r.field(pre, 7);
       if (tz < 64) {
-        const uint64_t m = r.read64(63 - tz);
+        const uint32_t len = 63 - tz;
+        const uint64_t m = len ? r.field(pre + 7, len) : 0;
         stored = ((m << 1) | 1) << tz;
+        used = pre + 7 + len;
       } else {
         stored = 0;
+        used = pre + 7;
       }
       term = stored == kNan;
     } else {
-      const uint32_t f = r.read(2);
+      // 11: 9-bit header (lead code, center), center - 1 bits; 10: 7-bit
+      // header, center - 1 bits; 00: 64 - L - T bits with the stored L, T;
+      // 01: same value
+      const uint32_t f = (uint32_t)r.field(pre, 2);
+      const uint32_t h9 = (uint32_t)r.field(pre + 2, 9), h7 = h9 >> 2;
+      uint32_t nL = L, nT = T, len = 0, off = pre + 2;
       if (f == 3) {
-        const uint32_t h = r.read(9);
-        L = lead_repr(h >> 6);
-        uint32_t center = h & 63;
-        if (!center) center = 64;
-        if (L + center > 64) return kErrCorrupt;
-        T = 64 - L - center;
-        const uint64_t x = ((r.read64(center - 1) << 1) | 1) << T;
-        const uint64_t val = x ^ stored;
-        if (val == kNan) term = true; else stored = val;
+        nL = lead_repr(h9 >> 6);
+        const uint32_t c = (h9 & 63) ? (h9 & 63) : 64;
+        if (nL + c > 64) return kErrCorrupt;
+        nT = 64 - nL - c;
+        len = c - 1;
+        off = pre + 11;
       } else if (f == 2) {
-        const uint32_t h = r.read(7);
-        L = lead_repr(h >> 4);
-        uint32_t center = h & 15;
-        if (!center) center = 16;
-        if (L + center > 64) return kErrCorrupt;
-        T = 64 - L - center;
-        const uint64_t x = center == 1 ? (1ull << T) : ((r.read64(center - 1) << 1) | 1) << T;
-        const uint64_t val = x ^ stored;
-        if (val == kNan) term = true; else stored = val;
+        nL = lead_repr(h7 >> 4);
+        const uint32_t c = (h7 & 15) ? (h7 & 15) : 16;
+        if (nL + c > 64) return kErrCorrupt;
+        nT = 64 - nL - c;
+        len = c - 1;
+        off = pre + 9;
       } else if (f == 0) {
         if ((uint64_t)L + T > 64) return kErrCorrupt;
-        const uint32_t center = 64 - L - T;
-        const uint64_t val = (r.read64(center) << T) ^ stored;
+        len = 64 - L - T;
+      }
+      const uint64_t m = len ? r.field(off, len) : 0;
+      used = f == 1 ? pre + 2 : off + len;
+      if (f != 1) {
+        const uint64_t x = f == 0 ? m << T : ((m << 1) | 1) << nT;
+        const uint64_t val = x ^ stored;
         if (val == kNan) term = true; else stored = val;
       }
+      L = nL;
+      T = nT;
     }
+    r.consume(used);
+    r.refill();
     if (r.pos > size_bits) return kErrOverrun;
     if (i == n) return term ? kErrNone : kErrCount;
     if (term) return kErrCount;
@@ -225,7 +250,7 @@ __device__ uint32_t elf_run(const uint8_t *data, uint32_t size_bits, uint32_t n,
 template <bool kWrite>
 __device__ uint32_t chimp128_run(const uint8_t *data, uint32_t size_bits, uint32_t n, double *out,
                                  uint64_t *ring) {
-  LaneReader r;
+  StreamReader r;
   r.init(data, 0);
   for (int s = 0; s < 128; ++s) ring[s * kWave] = 0;
   uint32_t cur = 0, lead = 0xffffffffu;
@@ -278,90 +303,100 @@ __device__ uint32_t chimp128_run(const uint8_t *data, uint32_t size_bits, uint32
 // only on flags, lead codes and the stored lead, so one lane walks records
 // 0..n (n = the terminator) without the value ring and writes a checkpoint
 // {bit position, stored lead} every kSeg records (n / kSeg + 1 entries).
+// One Chimp128 record (i >= 1) at the start of window r
+// (decoder/chunk_reader/chimp_n.rs:189-285): flag 00 = 7-bit ring index;
+// 11 = 3-bit lead code, 64 - lead bits; 10 = 64 - lead bits with the stored
+// lead; 01 = 16-bit header (index, lead code, sig), sig bits.  Sets the
+// flag, ring index, XOR payload (aligned) and bits used; false on a header
+// the reference would reject.
+__device__ __forceinline__ bool chimp128_rec(const BitWindow &r, uint32_t &lead, uint32_t &f, uint32_t &idx,
+                                             uint64_t &x, uint32_t &used) {
+  f = (uint32_t)(r.hi >> 62);
+  const uint32_t h = (uint32_t)(r.hi >> 46) & 0xffff;  // bits 2..17
+  idx = h >> 9;
+  if (f == 0) {
+    x = 0;
+    used = 9;
+    return true;
+  }
+  uint32_t nl = lead, len, off;
+  if (f == 3) {
+    nl = lead_repr(h >> 13);
+    len = 64 - nl;
+    off = 5;
+  } else if (f == 2) {
+    if (lead > 64) return false;
+    len = 64 - nl;
+    off = 2;
+  } else {
+    idx &= 127;
+    nl = lead_repr((h >> 6) & 7);
+    len = (h & 63) ? (h & 63) : 64;
+    if (len + nl > 64) return false;
+    off = 18;
+  }
+  x = r.field(off, len) << (f == 1 ? 64 - len - nl : 0);
+  lead = nl;
+  used = off + len;
+  return true;
+}
+
 __device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_bits, uint32_t n, XorCkpt *ck) {
-  LaneReader r;
+  BitWindow r;
   r.init(data, 0);
   uint32_t lead = 0xffffffffu;
   for (uint32_t i = 0; i <= n; ++i) {
     if ((i % kSeg) == 0) ck[i / kSeg] = XorCkpt{0, r.pos, (uint8_t)(lead > 64 ? 255 : lead), 0, -1, 0};
-    if (i == 0) {
-      r.read64(64);
-    } else {
-      const uint32_t f = r.read(2);
-      if (f == 0) {
-        r.read(7);
-      } else if (f == 3) {
-        lead = lead_repr(r.read(3));
-        r.read64(64 - lead);
-      } else if (f == 2) {
-        if (lead > 64) return kErrCorrupt;
-        r.read64(64 - lead);
-      } else {
-        const uint32_t h = r.read(16);
-        lead = lead_repr((h >> 6) & 7);
-        uint32_t sig = h & 63;
-        if (!sig) sig = 64;
-        if (sig + lead > 64) return kErrCorrupt;
-        r.read64(sig);
-      }
+    uint32_t used = 64;
+    if (i) {
+      uint32_t f, idx;
+      uint64_t x;
+      if (!chimp128_rec(r, lead, f, idx, x, used)) return kErrCorrupt;
     }
+    r.consume(used);
+    r.refill();
     if (r.pos > size_bits) return kErrOverrun;
   }
   return kErrNone;
 }
-
 // Chimp128 record i as (source record, xor payload): value_i = value_src ^ x.
 // Flags 10 / 11 use the previous record; flags 00 / 01 name ring slot idx,
 // i.e. the latest record j <= i - 1 with j == idx (mod 128); j < 0 is an
 // unwritten slot (value 0).  Record 0 is the root (the first 64 bits).
 constexpr int32_t kRoot = INT32_MIN;
 
-// Parses records [i0, i1) from a checkpoint; writes src / x / is00 bits to
-// the wave's LDS block arrays at offset (i - base).  Sources inside the
-// lane's own segment are resolved on the way, so every stored source lies
-// before i0 (an earlier segment, the previous block, or the root).
 // LDS slot of block record k in the Chimp128 staging arrays: one pad element
 // per 64 records, so lanes walking their own 64-record segments in lockstep
 // hit distinct banks.
 __device__ __forceinline__ uint32_t c128_slot(uint32_t k) { return k + (k >> 6); }
 
-__device__ uint32_t chimp128_parse_seg(const uint8_t *data, uint32_t size_bits, uint32_t i0, uint32_t i1,
+// Parses records [i0, i1) from a checkpoint; writes src / x / is00 bits to
+// the wave's LDS block arrays at offset (i - base).  Sources inside the
+// lane's own segment are resolved on the way, so every stored source lies
+// before i0 (an earlier segment, the previous block, or the root).
+__device__ __forceinline__ uint32_t chimp128_parse_seg(const uint8_t *data, uint32_t size_bits, uint32_t i0, uint32_t i1,
                                         uint32_t bitpos, uint32_t lead, uint32_t base, uint8_t *S, uint64_t *X,
                                         uint32_t *F) {
-  LaneReader r;
+  BitWindow r;
   r.init(data, bitpos);
   for (uint32_t i = i0; i < i1; ++i) {
     int32_t src;
-    uint64_t x = 0;
+    uint64_t x;
     bool is00 = false;
     if (i == 0) {
-      x = r.read64(64);
+      x = r.field(0, 64);
       src = kRoot;
+      r.consume(64);
     } else {
-      const uint32_t f = r.read(2);
-      if (f == 0) {
-        const uint32_t idx = r.read(7);
-        src = (int32_t)(i - 1) - (int32_t)(((i - 1) - idx) & 127u);
-        is00 = true;
-      } else if (f == 3) {
-        lead = lead_repr(r.read(3));
-        x = r.read64(64 - lead);
-        src = (int32_t)i - 1;
-      } else if (f == 2) {
-        if (lead > 64) return kErrCorrupt;
-        x = r.read64(64 - lead);
-        src = (int32_t)i - 1;
-      } else {
-        const uint32_t h = r.read(16);
-        const uint32_t idx = (h >> 9) & 127;
-        lead = lead_repr((h >> 6) & 7);
-        uint32_t sig = h & 63;
-        if (!sig) sig = 64;
-        if (sig + lead > 64) return kErrCorrupt;
-        x = r.read64(sig) << (64 - sig - lead);
-        src = (int32_t)(i - 1) - (int32_t)(((i - 1) - idx) & 127u);
-      }
+      uint32_t f, idx, used;
+      if (!chimp128_rec(r, lead, f, idx, x, used)) return kErrCorrupt;
+      // flags 00 / 01 name a ring slot, 10 / 11 the previous value
+      const int32_t ring = (int32_t)(i - 1) - (int32_t)(((i - 1) - idx) & 127u);
+      src = (f == 0 || f == 1) ? ring : (int32_t)i - 1;
+      is00 = f == 0;
+      r.consume(used);
     }
+    r.refill();
     if (r.pos > size_bits) return kErrOverrun;
     const uint32_t k = i - base;
     if (src >= (int32_t)i0) {

@@ -122,7 +122,158 @@ struct LaneReaderT {
   }
 };
 using LaneReader = LaneReaderT<1>;     // short neighbouring reads (DeltaSprintz tiles)
-using StreamReader = LaneReaderT<4>;   // long private streams (XOR codecs)
+
+// MSB-first reader of a long private stream (one lane's XOR segment): the
+// current 16-byte block plus kDepth - 1 blocks already requested, rotated by
+// static register moves, so each lane keeps kDepth loads in flight and a
+// refill waits on memory only when the stream outruns the whole queue.
+// Reads at most kDepth * 16 bytes past the current position (chunk bodies
+// carry 256 zero bytes of padding).
+template <int kDepth>
+struct PrefetchReader {
+  const uint4 *p;        // next block to request
+  uint4 cur;
+  uint4 q[kDepth - 1];   // requested blocks, oldest first
+  uint32_t wi;           // next word of `cur` (0 .. 4)
+  uint64_t buf;
+  uint32_t nb;
+  uint32_t pos;
+
+  __device__ __forceinline__ void init(const uint8_t *base, uint32_t bitpos) {
+    const uint32_t w = bitpos >> 5;
+    p = reinterpret_cast<const uint4 *>(base) + (w >> 2);
+    cur = *p++;
+#pragma unroll
+    for (int k = 0; k < kDepth - 1; ++k) q[k] = *p++;
+    wi = w & 3;
+    buf = 0;
+    nb = 0;
+    pos = bitpos;
+    fill();
+    const uint32_t s = bitpos & 31;
+    buf <<= s;
+    nb -= s;
+  }
+  __device__ __forceinline__ uint32_t next_word() {
+    if (wi == 4) {
+      cur = q[0];
+#pragma unroll
+      for (int k = 0; k < kDepth - 2; ++k) q[k] = q[k + 1];
+      q[kDepth - 2] = *p++;
+      wi = 0;
+    }
+    const uint32_t v = wi == 0 ? cur.x : wi == 1 ? cur.y : wi == 2 ? cur.z : cur.w;
+    ++wi;
+    return v;
+  }
+  __device__ __forceinline__ void fill() {
+    while (nb <= 32) {
+      buf |= (uint64_t)__builtin_bswap32(next_word()) << (32 - nb);
+      nb += 32;
+    }
+  }
+  __device__ __forceinline__ uint32_t peek(uint32_t k) {
+    fill();
+    return k ? (uint32_t)(buf >> (64 - k)) : 0u;
+  }
+  __device__ __forceinline__ void skip(uint32_t k) {
+    buf <<= k;
+    nb -= k;
+    pos += k;
+  }
+  __device__ __forceinline__ uint32_t read(uint32_t k) {
+    fill();
+    if (!k) return 0;
+    const uint32_t v = (uint32_t)(buf >> (64 - k));
+    buf <<= k;
+    nb -= k;
+    pos += k;
+    return v;
+  }
+  __device__ __forceinline__ uint64_t read64(uint32_t k) {
+    if (k > 32) {
+      const uint64_t hi = read(k - 32);
+      return (hi << 32) | read(32);
+    }
+    return read(k);
+  }
+};
+// 128-bit window over an MSB-first stream for branch-free field extraction:
+// bit 0 of the window is the MSB of `hi`.  refill() tops the window up to at
+// least 96 valid bits with 32-bit words from 16-byte block loads (one block
+// requested ahead), so a record of up to 96 bits decodes from registers with
+// shifts and selects, without per-field refills.  Reads at most 48 bytes past
+// the current position.
+struct BitWindow {
+  const uint4 *p;
+  uint4 cur, nxt;
+  uint32_t wi;   // next word of cur
+  uint64_t hi, lo;
+  uint32_t nb;   // valid bits
+  uint32_t pos;  // stream bit position of the window's first bit
+
+  __host__ __device__ __forceinline__ uint32_t next_word() {
+    if (wi == 4) {
+      cur = nxt;
+      nxt = *p++;
+      wi = 0;
+    }
+    const uint32_t v = wi == 0 ? cur.x : wi == 1 ? cur.y : wi == 2 ? cur.z : cur.w;
+    ++wi;
+    return __builtin_bswap32(v);
+  }
+  // append 32 bits at window bit nb (nb <= 96); every shift count stays in [0, 63]
+  __host__ __device__ __forceinline__ void push(uint32_t w) {
+    const uint64_t x = w;
+    if (nb <= 32) {
+      hi |= x << (32 - nb);
+    } else if (nb < 64) {
+      hi |= x >> (nb - 32);
+      lo |= x << (96 - nb);
+    } else {
+      lo |= x << (96 - nb);
+    }
+    nb += 32;
+  }
+  __host__ __device__ __forceinline__ void refill() {
+    while (nb < 96) push(next_word());
+  }
+  __host__ __device__ __forceinline__ void init(const uint8_t *base, uint32_t bitpos) {
+    const uint32_t w = bitpos >> 5;
+    p = reinterpret_cast<const uint4 *>(base) + (w >> 2);
+    cur = *p++;
+    nxt = *p++;
+    wi = w & 3;
+    hi = lo = 0;
+    nb = 0;
+    pos = bitpos & ~31u;
+    refill();
+    consume(bitpos & 31);
+    refill();
+  }
+  // the `len` bits at window offset o (o < 64, 1 <= len <= 64)
+  __host__ __device__ __forceinline__ uint64_t field(uint32_t o, uint32_t len) const {
+    const uint64_t top = o ? (hi << o) | (lo >> (64 - o)) : hi;
+    return top >> (64 - len);
+  }
+  // drop k bits (k <= 96, k <= nb)
+  __host__ __device__ __forceinline__ void consume(uint32_t k) {
+    if (k >= 64) {
+      hi = lo << (k - 64);
+      lo = 0;
+    } else if (k) {
+      hi = (hi << k) | (lo >> (64 - k));
+      lo <<= k;
+    }
+    nb -= k;
+    pos += k;
+  }
+};
+
+#ifndef EBI_XOR_DEPTH
+#define EBI_XOR_DEPTH 2
+#endif
+using StreamReader = PrefetchReader<EBI_XOR_DEPTH>;  // long private streams (XOR codecs)
 
 // Output of one lane's serial decode.  Without staging, records are stored
 // as they come.  With staging (`lds` = this lane's column of a [8][64] LDS

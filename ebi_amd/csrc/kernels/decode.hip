@@ -332,6 +332,9 @@ void chimp128_par(hipStream_t s, const ColView *cols, const uint32_t *prefix, ui
                   uint32_t *err, uint32_t block) {
   if (!n_chunks) return;
   switch (block) {
+    case 512:
+      hipLaunchKernelGGL(k_chimp128_par<512>, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
+      break;
     case 1024:
       hipLaunchKernelGGL(k_chimp128_par<1024>, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
       break;

@@ -221,7 +221,7 @@ int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
 typedef struct ebi_hip_batch ebi_hip_batch;
 typedef struct ebi_batch_config {
     uint32_t blocks_per_cu; /* persistent k_decode grid = CUs x blocks_per_cu */
-    uint32_t c128_block;    /* records per LDS block of the Chimp128 kernel */
+    uint32_t c128_block;    /* records per LDS block of the Chimp128 kernel (512..4096) */
     uint32_t overlap;       /* 1: Chimp128 kernel on a second stream beside k_decode */
     uint32_t reserved;
     double tuned_ms;        /* device ms of the chosen shape (0 if not tuned) */
