@@ -26,6 +26,7 @@ constexpr uint32_t kAlign = 256;
 constexpr uint32_t kPad = 256;
 constexpr uint32_t kTile = 512;  // records (fields for RLE) per wave job: 64 lanes x 8
 constexpr uint32_t kSeg = 64;    // records per checkpoint of the XOR index
+constexpr uint32_t kTilesPerJob = 4;  // consecutive tiles of one chunk per decode job
 constexpr uint32_t kWaveLanes = 64;  // gfx950 wavefront size
 
 enum Codec : uint32_t {
@@ -109,7 +110,8 @@ static_assert(sizeof(XorCkpt) == 16, "XorCkpt layout");
 
 // One wave's unit of work in a decode launch.
 enum JobKind : uint32_t {
-  kJobTile = 0,    // kTile records of a tile codec (plain, BUFF, Sprintz, RLE)
+  kJobTile = 0,    // tile(s) of a tile codec (plain, BUFF, Sprintz, RLE); a decode
+                   // job's `kind` field holds its tile count instead
   kJobSerial = 1,  // up to 64 XOR chunks, one lane per chunk, serial decode
   kJobSeg = 2,     // 64 checkpoint segments of one XOR chunk, one lane each
 };

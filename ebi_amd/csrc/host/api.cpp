@@ -325,13 +325,20 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   }
   if ((e = upload_vec(&col.d_chunks, table)) != hipSuccess) return hip_fail(ctx, e, "upload chunk table");
 
-  // Resident job lists.
+  // Resident job lists.  Decode tile jobs cover up to kTilesPerJob
+  // consecutive tiles of one chunk (Job::kind = tile count), so a wave pays
+  // the descriptor lookups once per up to 16 KiB of output; the load-time
+  // index pass walks single tiles.
   col.tile_job_first.assign(nc + 1, 0);
+  std::vector<Job> index_jobs;
   for (size_t k = 0; k < nc; ++k) {
     col.tile_job_first[k] = (uint32_t)col.h_tile_jobs.size();
-    if (!col.xor_codec && !col.chimp128)
-      for (uint32_t t = 0; t < col.chunks[k].n_tiles; ++t)
-        col.h_tile_jobs.push_back(Job{kJobTile, 0, (uint32_t)k, t});
+    if (!col.xor_codec && !col.chimp128) {
+      const uint32_t nt = col.chunks[k].n_tiles;
+      for (uint32_t t = 0; t < nt; ++t) index_jobs.push_back(Job{1, 0, (uint32_t)k, t});
+      for (uint32_t t = 0; t < nt; t += kTilesPerJob)
+        col.h_tile_jobs.push_back(Job{std::min(kTilesPerJob, nt - t), 0, (uint32_t)k, t});
+    }
     if (!col.chimp128)
       for (uint32_t s = 0; s < col.chunks[k].n_segs; s += kWaveLanes)
         col.h_seg_jobs.push_back(Job{kJobSeg, 0, (uint32_t)k, s});
@@ -366,18 +373,22 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   launch::zero_u32(ctx->stream, col.d_chunk_err, (uint32_t)nc);
   launch::zero_u32(ctx->stream, ctx->d_flags, 2);
   if ((rc = upload_view(ctx, col, nullptr))) return rc;
-  if (scheme == kDeltaSprintz || scheme == kRLE)
-    launch::tile_index(ctx->stream, col.d_view, 0, col.d_tile_jobs, col.n_tile_jobs, (uint32_t)nc, ctx->d_flags);
+  Job *d_index_jobs = nullptr;
+  if (scheme == kDeltaSprintz || scheme == kRLE) {
+    if ((e = upload_vec(&d_index_jobs, index_jobs)) != hipSuccess) return hip_fail(ctx, e, "upload index jobs");
+    launch::tile_index(ctx->stream, col.d_view, 0, d_index_jobs, (uint32_t)index_jobs.size(), (uint32_t)nc,
+                       ctx->d_flags);
+  }
   if (col.xor_codec || col.chimp128)
     launch::serial(ctx->stream, col.d_view, col.d_prefix + 2, 1, col.n_serial_jobs, nullptr, ctx->d_flags, false,
                    true);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "index launch");
   std::vector<uint32_t> errs(nc);
-  if (nc) {
-    e = hipMemcpyAsync(errs.data(), col.d_chunk_err, nc * 4, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "index pass");
-  }
+  if (nc) e = hipMemcpyAsync(errs.data(), col.d_chunk_err, nc * 4, hipMemcpyDeviceToHost, ctx->stream);
+  const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  if (d_index_jobs) hipFree(d_index_jobs);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) return hip_fail(ctx, e, "index pass");
   for (size_t k = 0; k < nc; ++k) col.chunks[k].err = errs[k];
   return EBI_OK;
 }
