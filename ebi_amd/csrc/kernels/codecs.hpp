@@ -607,25 +607,9 @@ __device__ __forceinline__ uint64_t rle_tile_sum(const DevChunk &c, uint32_t t) 
 }
 
 // starts: this wave's LDS scratch of kTile + 1 entries.
-// Largest field g in [f, nf) with starts[g] <= o, given starts[f] <= o
-// (starts is non-decreasing): first the next field, then a galloping search.
-__device__ __forceinline__ uint32_t rle_find(const uint32_t *starts, uint32_t nf, uint32_t f, uint32_t o) {
-  if (f + 1 >= nf || starts[f + 1] > o) return f;
-  uint32_t lo = f + 1, step = 1;
-  while (lo + step < nf && starts[lo + step] <= o) {
-    lo += step;
-    step <<= 1;
-  }
-  uint32_t hi = min(nf, lo + step);
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (starts[mid] <= o) lo = mid; else hi = mid;
-  }
-  return lo;
-}
 
 __device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, int64_t tile_start, double *out,
-                                         uint32_t *starts) {
+                                         uint32_t *starts, double *stage) {
   const uint32_t lane = lane_id();
   const uint32_t f0 = t * kTile + lane * 8;
   uint32_t run[8];
@@ -643,24 +627,24 @@ __device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, int64_t 
   const uint32_t nf = (uint32_t)min<uint64_t>(kTile, c.fields > t * (uint64_t)kTile ? c.fields - t * (uint64_t)kTile : 0);
   const double *vals = reinterpret_cast<const double *>(c.data + c.values_off) + (uint64_t)t * kTile;
   double *dst = out + c.rec_off + tile_start;
-  const bool aligned = (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
-  // lane l writes records 2l, 2l + 1 of every 128: one 1 KiB store per step;
-  // its field index only moves forward
-  uint32_t f = 0;
-  for (uint32_t o = 2 * lane; o < total; o += 2 * kWave) {
-    f = rle_find(starts, nf, f, o);
-    const double a = vals[f];
-    if (o + 1 < total) {
-      const double b = vals[rle_find(starts, nf, f, o + 1)];
-      if (aligned) {
-        st_pair(dst + o, a, b);
-      } else {
-        st_one(dst + o, a);
-        st_one(dst + o + 1, b);
-      }
-    } else {
-      st_one(dst + o, a);
+  // each step expands 512 records: lane l takes records 8l .. 8l + 7 (one
+  // binary search for the first one's field, then a forward walk) and the
+  // wave writes them with coalesced stores
+  for (uint32_t base = 0; base < total; base += kTile) {
+    const uint32_t o0 = base + lane * 8;
+    double v[8];
+    uint32_t lo = 0, hi = nf;  // largest f with starts[f] <= o0
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (starts[mid] <= o0) lo = mid; else hi = mid;
     }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t o = o0 + j;
+      while (lo + 1 < nf && starts[lo + 1] <= o) ++lo;
+      v[j] = o < total ? vals[lo] : 0.0;
+    }
+    tile_store(dst + base, v, min(kTile, total - base), stage);
   }
 }
 

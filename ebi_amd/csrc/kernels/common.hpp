@@ -314,21 +314,41 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Wave-wide inclusive scans (64 lanes).
-__device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t x) {
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const int64_t y = __shfl_up(x, d, kWave);
-    if ((int)lane_id() >= d) x += y;
-  }
+// Inclusive wave prefix sums with DPP (register moves, no LDS): row shifts
+// by 1, 2, 4, 8 scan each 16-lane row (lanes without a source read 0), then
+// row_bcast:15 adds lane 15 of rows 0 / 2 into rows 1 / 3 and row_bcast:31
+// adds lane 31 into rows 2 and 3.
+template <int kCtrl, int kRowMask, bool kBound>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kCtrl, kRowMask, 0xf, kBound);
+}
+template <int kCtrl, int kRowMask, bool kBound>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t x) {
+  const uint32_t lo = dpp_u32<kCtrl, kRowMask, kBound>((uint32_t)x);
+  const uint32_t hi = dpp_u32<kCtrl, kRowMask, kBound>((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x) {
+  x += dpp_u32<kRowShr1, 0xf, true>(x);
+  x += dpp_u32<kRowShr2, 0xf, true>(x);
+  x += dpp_u32<kRowShr4, 0xf, true>(x);
+  x += dpp_u32<kRowShr8, 0xf, true>(x);
+  x += dpp_u32<kRowBcast15, 0xa, false>(x);
+  x += dpp_u32<kRowBcast31, 0xc, false>(x);
   return x;
 }
-__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x) {
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const uint32_t y = __shfl_up(x, d, kWave);
-    if ((int)lane_id() >= d) x += y;
-  }
-  return x;
+__device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v) {
+  uint64_t x = (uint64_t)v;  // wrapping, as the reference's release build
+  x += dpp_u64<kRowShr1, 0xf, true>(x);
+  x += dpp_u64<kRowShr2, 0xf, true>(x);
+  x += dpp_u64<kRowShr4, 0xf, true>(x);
+  x += dpp_u64<kRowShr8, 0xf, true>(x);
+  x += dpp_u64<kRowBcast15, 0xa, false>(x);
+  x += dpp_u64<kRowBcast31, 0xc, false>(x);
+  return (int64_t)x;
 }
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t x) {
 #pragma unroll

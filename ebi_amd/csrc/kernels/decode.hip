@@ -62,9 +62,9 @@ __device__ __forceinline__ uint32_t segment_lane(const ColView &cv, uint32_t ci,
 // jobs (similar cost) spread over all waves and no job counter is contended.
 __global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
                                                uint32_t n_jobs, uint32_t *ctr, uint32_t *err) {
-  // per wave: RLE field starts or the store stage (a wave uses one at a time)
+  // per wave: the store stage, and the field starts of an RLE tile
   __shared__ __align__(16) double lds[4][kStageStride * kWave];
-  static_assert(sizeof(lds[0]) >= (kTile + 1) * sizeof(uint32_t), "RLE starts fit the stage");
+  __shared__ uint32_t lds_starts[4][kTile + 1];
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
   uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint3
             case kCodecBUFF: buff_tile(c, t, cv.out, lds[wave]); break;
             case kCodecSprintz: sprintz_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, lds[wave]); break;
             case kCodecRLE:
-              rle_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, reinterpret_cast<uint32_t *>(lds[wave]));
+              rle_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, lds_starts[wave], lds[wave]);
               break;
             default:
               if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
