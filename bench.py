@@ -16,7 +16,7 @@ record, summed over all columns) / wall time of the step, in GB/s.
 Also reported: the roofline of the batched step's dominant kernel
 (algorithmic bytes of the columns it decodes / its mean launch time from HIP
 events on the library's stream, around each launch of the timed steps),
-per-codec GB/s from one launch per column, the plain per-chunk-launch
+per-codec GB/s from one prepared batch per codec, the plain per-chunk-launch
 comparator on Gorilla and Chimp, and the CPU oracle timed on a bounded sample
 of the same workload on one host core.
 """
@@ -159,14 +159,15 @@ def main():
 
     per_codec, per_kernel, comparator = {}, {}, {}
     if not args.no_breakdown:
-        per_codec, per_kernel = breakdown(ctx, cols, decs, outs, n, kernel_of)
+        per_codec, per_kernel = breakdown(E, ctx, cols, decs, outs, n, kernel_of)
         comparator = run_comparator(E, ctx, comparator_files)
     for o in outs:
         o.free()
     outs = []
-    query = serving = matmul = None
+    query = serving = matmul = c1 = None
     if not args.no_query:
         query = query_leg(E, ctx, cols, sample_q, n)
+        c1 = c1_leg(E, G, ctx)
     if not args.no_serve:
         serving = serve_leg(E, G, ctx, args, threads)
     if not args.no_matmul:
@@ -217,6 +218,7 @@ def main():
                        for k, v in per_kernel.items()},
         "per_chunk_launch_comparator": comparator,
         "query": query,
+        "cpu_plain_case": c1,
         "serving": serving,
         "matmul": matmul,
     }
@@ -283,6 +285,42 @@ def query_leg(E, ctx, cols, sample_q, n, reps=20):
             "codecs": out}
 
 
+def c1_leg(E, G, ctx, reps=5):
+    """The reference's CPU-plain case: 1,048,576-value 2-decimal random walk,
+    Gorilla and BUFF (scale 100) at 131,072 records per chunk; full decode,
+    range filter [99.5, 100.5], sum over the filtered rows.  Median of 5 on
+    the CPU oracle (one core) and on the GPU (synchronous calls, results to
+    the host)."""
+    from oracle.oracle import OracleDecoder
+
+    vals = G.random_walk(1 << 20, 100.0, 0.1, 2, seed=1)
+    pred = E.Predicate.range(E.RangeValue.inclusive(99.5), E.RangeValue.inclusive(100.5))
+    out = {}
+    for name, cfg in (("gorilla", E.CompressorConfig.gorilla()), ("buff", E.CompressorConfig.buff(scale=100))):
+        f = E.encode(vals, cfg, E.ChunkOption.record_count(131072))
+        res = {}
+        for side, dec in (("cpu", OracleDecoder(f)), ("gpu", E.Decoder(ctx, f))):
+            t_dec, t_fil, t_sum = [], [], []
+            for _ in range(reps):
+                t = time.perf_counter()
+                dec.materialize()
+                t_dec.append(time.perf_counter() - t)
+                t = time.perf_counter()
+                ids = dec.filter(pred)
+                t_fil.append(time.perf_counter() - t)
+                t = time.perf_counter()
+                s_ = dec.sum(ids)
+                t_sum.append(time.perf_counter() - t)
+            res[side] = {"decode_ms": round(float(np.median(t_dec)) * 1e3, 3),
+                         "filter_ms": round(float(np.median(t_fil)) * 1e3, 3),
+                         "sum_ms": round(float(np.median(t_sum)) * 1e3, 3), "selected": int(ids.size), "sum": s_}
+            if side == "gpu":
+                dec.close()
+        res["cores_cpu"] = 1
+        out[name] = res
+    return {"workload": "2^20 f64 random walk (2 decimals), 131072 records/chunk", "codecs": out}
+
+
 def serve_leg(E, G, ctx, args, threads, n_cols=64):
     """Serving: Poisson arrivals over 64 resident 4-decimal random walks
     (sizes log-uniform on [2^20, 2^log2max], codecs uniform over the eight,
@@ -304,6 +342,7 @@ def serve_leg(E, G, ctx, args, threads, n_cols=64):
         cols.append((E.Decoder(ctx, f), np.sort(vals[::256]), nv))
         del f, vals
     log(f"[serve] {n_cols} columns resident in {time.time() - t0:.1f}s")
+    c3 = batched_queries(E, ctx, cols)
     dec_bufs = [ctx.alloc(8 * max(nv for _, _, nv in cols)) for _ in range(8)]
 
     def make(k):
@@ -379,8 +418,52 @@ def serve_leg(E, G, ctx, args, threads, n_cols=64):
     for d, _, _ in cols:
         d.close()
     return {"workload": f"{n_cols} columns, 4-decimal random walks, 2^20..2^{args.serve_log2max} values, "
-                        f"mixed codecs, {args.serve_queries} Poisson arrivals per rate",
-            "rates": out}
+                        f"mixed codecs, chunks of 2^13..2^18 records, {args.serve_queries} Poisson arrivals per rate",
+            "rates": out, "batched": c3}
+
+
+def batched_queries(E, ctx, cols):
+    """The batched workload over the same 64 resident columns: (1) every column
+    decoded by one tuned prepared batch replayed as a hipGraph; (2) per
+    selectivity s (0.1, 1, 10, 50 %), a greater-than filter + sum at the
+    (1 - s) quantile and a max on every column, all 64 submitted together to
+    the server (in-situ on BUFF / DeltaSprintz codes).  GB/s counts compressed
+    bytes (+ 8 per decoded record for the decode)."""
+    decs = [d for d, _, _ in cols]
+    nrec = sum(nv for _, _, nv in cols)
+    comp = sum(d.compressed_bytes for d in decs)
+    outs = [ctx.alloc(8 * nv) for _, _, nv in cols]
+    b = E.Batch(ctx, decs, outs, autotune=True)
+    b.run()
+    ms = []
+    for _ in range(5):
+        t = time.perf_counter()
+        b.run()
+        ms.append((time.perf_counter() - t) * 1e3)
+    cfg = b.config
+    b.close()
+    for o in outs:
+        o.free()
+    res = {"decode": {"graph_ms": round(float(np.median(ms)), 3),
+                      "gbs": round((comp + 8 * nrec) / (float(np.median(ms)) / 1e3) / 1e9, 1),
+                      "launch_shape": cfg}}
+    with E.Server(ctx, device_budget=16 << 30, max_batch=64, n_streams=4) as srv:
+        for sel in (0.001, 0.01, 0.1, 0.5):
+            qs = []
+            for d, sq, nv in cols:
+                thr = float(sq[int((1.0 - sel) * (len(sq) - 1))])
+                p = E.Predicate.range(E.RangeValue.exclusive(thr), E.RangeValue.none())
+                qs += [E.Query(E.OP_FILTER_SUM, d, predicate=p), E.Query(E.OP_MAX, d)]
+            for x in srv.submit(qs[:2]):  # warm
+                srv.wait(x)
+            t = time.perf_counter()
+            ts = srv.submit(qs)
+            lat = [srv.wait(x)[1] for x in ts]
+            wall = time.perf_counter() - t
+            res[f"filter_sum+max_sel{sel * 100:g}%"] = {
+                "queries": len(qs), "wall_ms": round(wall * 1e3, 3), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                "gbs": round(2 * comp / wall / 1e9, 1)}
+    return res
 
 
 def matmul_leg(E, ctx, nmat, threads):
@@ -448,24 +531,25 @@ def traffic_of(workload, kernel):
     return (t.get("kernels", {}).get(kernel) or {}).get("traffic")
 
 
-def breakdown(ctx, cols, decs, outs, n, kernel_of):
-    """Per-codec and per-kernel GB/s from one decode_device per column (two
-    passes, HIP event time), outside the timed region."""
-    per_col_ms = {i: [] for i in range(len(cols))}
-    for _ in range(2):
-        for i, d in enumerate(decs):
-            d.decode_device(outs[i])
-            per_col_ms[i].append(ctx.last_device_ms)
+def breakdown(E, ctx, cols, decs, outs, n, kernel_of):
+    """Per-codec GB/s: one tuned prepared batch per codec over its columns
+    (every generator), median of three eager replays (HIP event time);
+    per-kernel totals from the same runs.  Outside the timed region."""
     per_codec, per_kernel = {}, {}
-    for i, (g, codec, d, cb) in enumerate(cols):
-        ms = float(np.mean(per_col_ms[i]))
-        b = cb + 8 * n
-        pc = per_codec.setdefault(codec, {"bytes": 0, "ms": 0.0})
-        pc["bytes"] += b
-        pc["ms"] += ms
+    for codec in dict.fromkeys(c for _, c, _, _ in cols):
+        idx = [i for i, (_, c, _, _) in enumerate(cols) if c == codec]
+        b = E.Batch(ctx, [decs[i] for i in idx], [outs[i] for i in idx], autotune=True)
+        ms = []
+        for _ in range(3):
+            b.run(eager=True)
+            ms.append(ctx.last_device_ms)
+        b.close()
+        t = float(np.median(ms))
+        byt = sum(cols[i][3] + 8 * n for i in idx)
+        per_codec[codec] = {"bytes": byt, "ms": t}
         pk = per_kernel.setdefault(kernel_of[codec], {"bytes": 0, "ms": 0.0, "launches": 0})
-        pk["bytes"] += b
-        pk["ms"] += ms
+        pk["bytes"] += byt
+        pk["ms"] += t
         pk["launches"] += 1
     return per_codec, per_kernel
 

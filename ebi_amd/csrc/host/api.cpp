@@ -390,6 +390,37 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return hip_fail(ctx, e, "index pass");
   for (size_t k = 0; k < nc; ++k) col.chunks[k].err = errs[k];
+  if (scheme == kRLE && nc) {
+    // Balance RLE decode jobs by output: a job expands up to kTilesPerJob
+    // steps of kTile output records of one field tile (runs of up to 255
+    // make a field tile's output 1 to 255 steps).
+    uint64_t n_tiles = 0;
+    for (size_t k = 0; k < nc; ++k) n_tiles += col.chunks[k].n_tiles;
+    std::vector<int64_t> starts(n_tiles);
+    if (n_tiles && (e = hipMemcpy(starts.data(), col.d_tile_index, n_tiles * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+      return hip_fail(ctx, e, "RLE tile index");
+    col.h_tile_jobs.clear();
+    for (size_t k = 0; k < nc; ++k) {
+      col.tile_job_first[k] = (uint32_t)col.h_tile_jobs.size();
+      const HostChunk &h = col.chunks[k];
+      if (h.err) continue;  // never decoded
+      for (uint32_t t = 0; t < h.n_tiles; ++t) {
+        const int64_t a = starts[h.dev.tile0 + t];
+        const int64_t b = t + 1 < h.n_tiles ? starts[h.dev.tile0 + t + 1] : (int64_t)h.dev.n;
+        const uint32_t steps = b > a ? (uint32_t)((b - a + kTile - 1) / kTile) : 0;
+        for (uint32_t s0 = 0; s0 < steps; s0 += kTilesPerJob)
+          col.h_tile_jobs.push_back(Job{std::min(kTilesPerJob, steps - s0), 0, (uint32_t)k, t | (s0 << 24)});
+      }
+    }
+    col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
+    col.n_tile_jobs = (uint32_t)col.h_tile_jobs.size();
+    hipFree(col.d_tile_jobs);
+    col.d_tile_jobs = nullptr;
+    const uint32_t jobs = col.n_tile_jobs + col.n_seg_jobs;
+    if ((e = upload_vec(&col.d_tile_jobs, col.h_tile_jobs)) != hipSuccess ||
+        (e = hipMemcpy(col.d_prefix + 1, &jobs, 4, hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(ctx, e, "RLE job lists");
+  }
   return EBI_OK;
 }
 

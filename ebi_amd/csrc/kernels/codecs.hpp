@@ -608,8 +608,9 @@ __device__ __forceinline__ uint64_t rle_tile_sum(const DevChunk &c, uint32_t t) 
 
 // starts: this wave's LDS scratch of kTile + 1 entries.
 
-__device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, int64_t tile_start, double *out,
-                                         uint32_t *starts, double *stage) {
+// Expands output steps [s0, s0 + ns) (kTile records each) of field tile t.
+__device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, uint32_t s0, uint32_t ns, int64_t tile_start,
+                                         double *out, uint32_t *starts, double *stage) {
   const uint32_t lane = lane_id();
   const uint32_t f0 = t * kTile + lane * 8;
   uint32_t run[8];
@@ -630,7 +631,8 @@ __device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, int64_t 
   // each step expands 512 records: lane l takes records 8l .. 8l + 7 (one
   // binary search for the first one's field, then a forward walk) and the
   // wave writes them with coalesced stores
-  for (uint32_t base = 0; base < total; base += kTile) {
+  const uint32_t end = min(total, (s0 + ns) * kTile);
+  for (uint32_t base = s0 * kTile; base < end; base += kTile) {
     const uint32_t o0 = base + lane * 8;
     double v[8];
     uint32_t lo = 0, hi = nf;  // largest f with starts[f] <= o0

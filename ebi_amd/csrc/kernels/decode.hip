@@ -73,17 +73,21 @@ __global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint3
     const ColView cv = cols[ci];
     const uint32_t local = j - prefix[ci];
     if (local < cv.n_tile_jobs) {
-      const Job jb = cv.tile_jobs[local];  // jb.kind consecutive tiles from jb.tile
+      // jb.kind consecutive tiles from jb.tile; RLE: jb.kind output steps
+      // from step jb.tile >> 24 of field tile jb.tile & 0xffffff
+      const Job jb = cv.tile_jobs[local];
       const DevChunk c = cv.chunks[jb.chunk];
-      if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
+      if (c.codec == kCodecRLE) {
+        if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
+          const uint32_t t = jb.tile & 0xffffffu;
+          rle_tile(c, t, jb.tile >> 24, jb.kind, cv.tile_index[c.tile0 + t], cv.out, lds_starts[wave], lds[wave]);
+        }
+      } else if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
         for (uint32_t t = jb.tile; t < jb.tile + jb.kind; ++t) {
           switch (c.codec) {
             case kCodecPlain: plain_tile(c, t, cv.out); break;
             case kCodecBUFF: buff_tile(c, t, cv.out, lds[wave]); break;
             case kCodecSprintz: sprintz_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, lds[wave]); break;
-            case kCodecRLE:
-              rle_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, lds_starts[wave], lds[wave]);
-              break;
             default:
               if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
               break;
