@@ -533,13 +533,45 @@ __device__ __forceinline__ void sprintz_deltas(const DevChunk &c, uint32_t i0, i
 #pragma unroll
   for (int j = 0; j < 8; ++j) d[j] = 0;
   if (i0 >= c.n || c.bits == 0) return;
-  LaneReader r;
-  r.init(reinterpret_cast<const uint8_t *>(c.data), i0 * c.bits);
+  const uint32_t W = c.bits;
   const uint32_t cnt = min(8u, c.n - i0);
+  if (W <= 16) {
+    // 8 records x W bits = W bytes from byte i0 * W / 8: two aligned 16-byte
+    // loads, a byte-aligned 128-bit funnel shift, then fixed-position fields
+    const uint32_t start = (i0 / 8) * W;
+    const uint4 *q = reinterpret_cast<const uint4 *>(c.data) + (start >> 4);
+    const uint4 a = q[0], b = q[1];
+    const uint32_t sb = start & 15, wi = sb >> 2, sh = (sb & 3) * 8;
+    // big-endian word wi + k of the 32 loaded bytes
+    auto word = [&](uint32_t k) {
+      const uint32_t i = wi + k;
+      const uint32_t x = i == 0 ? a.x : i == 1 ? a.y : i == 2 ? a.z : i == 3 ? a.w
+                       : i == 4 ? b.x : i == 5 ? b.y : i == 6 ? b.z : b.w;
+      return __builtin_bswap32(x);
+    };
+    uint32_t u0 = word(0), u1 = word(1), u2 = word(2), u3 = word(3), u4 = word(4);
+    if (sh) {
+      u0 = (u0 << sh) | (u1 >> (32 - sh));
+      u1 = (u1 << sh) | (u2 >> (32 - sh));
+      u2 = (u2 << sh) | (u3 >> (32 - sh));
+      u3 = (u3 << sh) | (u4 >> (32 - sh));
+    }
+    uint64_t hi = ((uint64_t)u0 << 32) | u1, lo = ((uint64_t)u2 << 32) | u3;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {  // field j = the top W bits, then shift the pair by W
+      const uint64_t z = hi >> (64 - W);
+      if ((uint32_t)j < cnt) d[j] = (int64_t)(z >> 1) ^ -(int64_t)(z & 1);
+      hi = (hi << W) | (lo >> (64 - W));
+      lo <<= W;
+    }
+    return;
+  }
+  LaneReader r;
+  r.init(reinterpret_cast<const uint8_t *>(c.data), i0 * W);
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     if ((uint32_t)j < cnt) {
-      const uint64_t z = r.read64(c.bits);
+      const uint64_t z = r.read64(W);
       d[j] = (int64_t)(z >> 1) ^ -(int64_t)(z & 1);
     }
   }

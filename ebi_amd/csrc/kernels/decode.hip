@@ -60,7 +60,10 @@ __device__ __forceinline__ uint32_t segment_lane(const ColView &cv, uint32_t ci,
 // codecs and uneven chunks in one launch.  Wave w takes jobs w, w + W,
 // w + 2W, ... (W = waves in the grid): a static interleave, so neighbouring
 // jobs (similar cost) spread over all waves and no job counter is contended.
-__global__ void __launch_bounds__(256) k_decode(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
+// amdgpu_waves_per_eu(5): the register budget of 5 waves per SIMD (a few
+// spilled values in cold paths cost less than a lower occupancy).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_decode(
+    const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
                                                uint32_t n_jobs, uint32_t *ctr, uint32_t *err) {
   // per wave: the store stage, and the field starts of an RLE tile
   __shared__ __align__(16) double lds[4][kStageStride * kWave];
