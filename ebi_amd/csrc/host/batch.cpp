@@ -1,8 +1,7 @@
 // Prepared batched decode: a fixed set of resident columns decoded into fixed
 // device outputs, replayed many times.  The job-space arguments are uploaded
 // once; an autotuner times candidate launch shapes on this very batch
-// (persistent grid size, Chimp128 LDS block, Chimp128 kernel beside k_decode
-// or after it) and keeps the fastest; the launch sequence is captured in a
+// (persistent grid size, Chimp128 LDS block) and keeps the fastest; the launch sequence is captured in a
 // hipGraph and replayed with one launch.  No counterpart in lemolatoon/ebi,
 // whose Decoder decodes chunk after chunk on the CPU (api/decoder.rs:202-241).
 #include <algorithm>
@@ -159,7 +158,8 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   b->cfg.c128_block = ctx->c128_block;
   b->cfg.overlap = ctx->overlap ? 1 : 0;
   if (autotune) {
-    // coordinate search: grid size, then the Chimp128 block, then overlap
+    // coordinate search: grid size, then the Chimp128 block (the overlap
+    // setting is the context's: kernels back to back unless asked)
     double best = time_config(ctx, b, b->cfg, 2);
     if (best < 0) {
       destroy(b);
@@ -185,11 +185,6 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
           c.c128_block = blk;
           consider(c);
         }
-    if (b->jobs && b->jobs128) {
-      ebi_batch_config c = b->cfg;
-      c.overlap = !c.overlap;
-      consider(c);
-    }
     b->cfg.tuned_ms = best;
   }
   if ((rc = capture(ctx, b))) {

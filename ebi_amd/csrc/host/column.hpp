@@ -97,6 +97,8 @@ struct ebi_hip_ctx {
   // launch shape of the decode kernels (defaults; a prepared batch tunes its own)
   uint32_t blocks_per_cu = 8;   // persistent k_decode grid = cu_count * blocks_per_cu
   uint32_t c128_block = 2048;   // records per LDS block of the Chimp128 kernel
-  bool overlap = true;          // Chimp128 kernel concurrent with k_decode in batches
+  // Chimp128 kernel concurrent with k_decode in batches (EBI_HIP_OVERLAP=1):
+  // a few percent faster, but neither kernel's duration is then its own
+  bool overlap = false;
   int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks
 };

@@ -206,65 +206,66 @@ struct PrefetchReader {
 // the current position.
 struct BitWindow {
   const uint4 *p;
-  uint4 cur, nxt;
-  uint32_t wi;   // next word of cur
+  uint64_t c0, c1;  // current 16-byte block as two little-endian halves
+  uint4 nxt;
+  uint32_t wi;      // next word of the current block (0 .. 4)
   uint64_t hi, lo;
-  uint32_t nb;   // valid bits
-  uint32_t pos;  // stream bit position of the window's first bit
+  uint32_t nb;      // valid bits
+  uint32_t pos;     // stream bit position of the window's first bit
 
-  __host__ __device__ __forceinline__ uint32_t next_word() {
-    if (wi == 4) {
-      cur = nxt;
-      nxt = *p++;
-      wi = 0;
-    }
-    const uint32_t v = wi == 0 ? cur.x : wi == 1 ? cur.y : wi == 2 ? cur.z : cur.w;
-    ++wi;
-    return __builtin_bswap32(v);
+  // Every step below is branch-free (selects and shifts with counts kept in
+  // [0, 63]) except the block reload once per four words.
+  __host__ __device__ __forceinline__ void load_next() {
+    c0 = (uint64_t)nxt.x | ((uint64_t)nxt.y << 32);
+    c1 = (uint64_t)nxt.z | ((uint64_t)nxt.w << 32);
+    nxt = *p++;
+    wi = 0;
   }
-  // append 32 bits at window bit nb (nb <= 96); every shift count stays in [0, 63]
-  __host__ __device__ __forceinline__ void push(uint32_t w) {
-    const uint64_t x = w;
-    if (nb <= 32) {
-      hi |= x << (32 - nb);
-    } else if (nb < 64) {
-      hi |= x >> (nb - 32);
-      lo |= x << (96 - nb);
-    } else {
-      lo |= x << (96 - nb);
+  // append the next stream word when `need` (window bit nb, nb < 96)
+  __host__ __device__ __forceinline__ void push_if(bool need) {
+    const uint64_t half = (wi & 2) ? c1 : c0;
+    const uint64_t w = __builtin_bswap32((uint32_t)(half >> ((wi & 1) * 32)));
+    const uint64_t top = w << 32;  // the word in the top 32 bits
+    const uint64_t h = nb < 64 ? top >> (nb & 63) : 0;             // bits landing in hi
+    const uint64_t l = nb > 32 ? w << ((96 - nb) & 63) : 0;        // bits landing in lo
+    if (need) {
+      hi |= h;
+      lo |= l;
+      nb += 32;
+      ++wi;
+      if (wi == 4) load_next();
     }
-    nb += 32;
   }
-  __host__ __device__ __forceinline__ void refill() {
-    while (nb < 96) push(next_word());
+  __host__ __device__ __forceinline__ void refill() {  // to >= 96 bits from >= 14
+    push_if(nb < 96);
+    push_if(nb < 96);
+    push_if(nb < 96);
   }
   __host__ __device__ __forceinline__ void init(const uint8_t *base, uint32_t bitpos) {
     const uint32_t w = bitpos >> 5;
     p = reinterpret_cast<const uint4 *>(base) + (w >> 2);
-    cur = *p++;
     nxt = *p++;
+    load_next();
     wi = w & 3;
     hi = lo = 0;
     nb = 0;
     pos = bitpos & ~31u;
+    refill();
     refill();
     consume(bitpos & 31);
     refill();
   }
   // the `len` bits at window offset o (o < 64, 1 <= len <= 64)
   __host__ __device__ __forceinline__ uint64_t field(uint32_t o, uint32_t len) const {
-    const uint64_t top = o ? (hi << o) | (lo >> (64 - o)) : hi;
-    return top >> (64 - len);
+    const uint64_t top = o ? (hi << (o & 63)) | (lo >> ((64 - o) & 63)) : hi;
+    return top >> ((64 - len) & 63);
   }
   // drop k bits (k <= 96, k <= nb)
   __host__ __device__ __forceinline__ void consume(uint32_t k) {
-    if (k >= 64) {
-      hi = lo << (k - 64);
-      lo = 0;
-    } else if (k) {
-      hi = (hi << k) | (lo >> (64 - k));
-      lo <<= k;
-    }
+    const uint64_t h_small = k ? (hi << (k & 63)) | (lo >> ((64 - k) & 63)) : hi;
+    const uint64_t h_big = lo << ((k - 64) & 63);
+    hi = k >= 64 ? h_big : h_small;
+    lo = k >= 64 ? 0 : lo << (k & 63);
     nb -= k;
     pos += k;
   }

@@ -214,9 +214,9 @@ int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
 
 /* Prepared batch: the same decode for a fixed set of columns and outputs,
  * replayed many times.  The job-space arguments are uploaded once; with
- * autotune != 0 the launch shape (persistent grid size, Chimp128 LDS block,
- * Chimp128 kernel beside k_decode or after it) is timed on this batch and the
- * fastest kept; the launch sequence is captured in a hipGraph.  Columns and
+ * autotune != 0 the launch shape (persistent grid size, Chimp128 LDS block)
+ * is timed on this batch and the fastest kept; the launch sequence is
+ * captured in a hipGraph.  Columns and
  * outputs must outlive the batch.  No counterpart in the reference. */
 typedef struct ebi_hip_batch ebi_hip_batch;
 typedef struct ebi_batch_config {
