@@ -206,7 +206,7 @@ struct PrefetchReader {
 // the current position.
 struct BitWindow {
   const uint4 *p;
-  uint64_t c0, c1;  // current 16-byte block as two little-endian halves
+  uint64_t c0, c1;  // current 16-byte block as two big-endian halves
   uint4 nxt;
   uint32_t wi;      // next word of the current block (0 .. 4)
   uint64_t hi, lo;
@@ -214,17 +214,20 @@ struct BitWindow {
   uint32_t pos;     // stream bit position of the window's first bit
 
   // Every step below is branch-free (selects and shifts with counts kept in
-  // [0, 63]) except the block reload once per four words.
+  // [0, 63]) except the block reload once per four words.  The reload
+  // byte-swaps the prefetched block into c0, c1 (a computation, not a copy,
+  // so the prefetch load can reuse nxt's registers and is waited for only at
+  // the next reload).
   __host__ __device__ __forceinline__ void load_next() {
-    c0 = (uint64_t)nxt.x | ((uint64_t)nxt.y << 32);
-    c1 = (uint64_t)nxt.z | ((uint64_t)nxt.w << 32);
+    c0 = __builtin_bswap64((uint64_t)nxt.x | ((uint64_t)nxt.y << 32));
+    c1 = __builtin_bswap64((uint64_t)nxt.z | ((uint64_t)nxt.w << 32));
     nxt = *p++;
     wi = 0;
   }
   // append the next stream word when `need` (window bit nb, nb < 96)
   __host__ __device__ __forceinline__ void push_if(bool need) {
     const uint64_t half = (wi & 2) ? c1 : c0;
-    const uint64_t w = __builtin_bswap32((uint32_t)(half >> ((wi & 1) * 32)));
+    const uint64_t w = (uint32_t)(half >> ((~wi & 1) * 32));
     const uint64_t top = w << 32;  // the word in the top 32 bits
     const uint64_t h = nb < 64 ? top >> (nb & 63) : 0;             // bits landing in hi
     const uint64_t l = nb > 32 ? w << ((96 - nb) & 63) : 0;        // bits landing in lo
