@@ -35,6 +35,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# kernel that decodes each codec in a batch (the rest: k_decode)
+KERNEL_OF = {"chimp128": "k_chimp128_par", "gorilla": "k_decode_seg", "chimp": "k_decode_seg", "elf": "k_decode_seg"}
 
 
 def log(*a):
@@ -121,20 +123,22 @@ def main():
 
     def step():
         # one batched decode over every chunk of every column (mixed codecs):
-        # k_decode (tile + segment jobs of all columns) and k_chimp128_par,
-        # launched eagerly so each kernel is bracketed by HIP events
+        # k_decode (tile jobs of all columns), k_decode_seg (XOR checkpoint
+        # segments) and k_chimp128_par, launched eagerly so each kernel is
+        # bracketed by HIP events
         batch.run(eager=True)
-        return ctx.last_device_ms, ctx.last_phase_ms(0), ctx.last_phase_ms(1)
+        return ctx.last_device_ms, ctx.last_phase_ms(0), ctx.last_phase_ms(1), ctx.last_phase_ms(2)
 
     for _ in range(args.warmup):
         step()
-    batch_dev_ms, phase_ms = [], [[], []]
+    batch_dev_ms, phase_ms = [], [[], [], []]
     t0 = time.perf_counter()
     for s in range(args.steps):
-        tot, p0, p1 = step()
+        tot, p0, p1, p2 = step()
         batch_dev_ms.append(tot)
         phase_ms[0].append(p0)
         phase_ms[1].append(p1)
+        phase_ms[2].append(p2)
         log(f"[timed] step {s + 1}/{args.steps} done")
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
@@ -150,9 +154,9 @@ def main():
     # roofline of the batched step's kernels: algorithmic bytes of the columns
     # each kernel decodes / its mean launch time (HIP events on the library's
     # stream, around each launch of the timed steps)
-    kernel_of = {c: ("k_chimp128_par" if c == "chimp128" else "k_decode") for c in codecs}
+    kernel_of = {c: KERNEL_OF.get(c, "k_decode") for c in codecs}
     batch_kernels = {}
-    for k, name in enumerate(("k_decode", "k_chimp128_par")):
+    for k, name in enumerate(("k_decode", "k_chimp128_par", "k_decode_seg")):
         b = sum(cb + 8 * n for _, codec, _, cb in cols if kernel_of[codec] == name)
         if b:
             ms = float(np.mean(phase_ms[k]))

@@ -451,8 +451,8 @@ class Context:
         return float(lib().ebi_hip_last_device_ms(self._h))
 
     def last_phase_ms(self, phase: int) -> float:
-        """Device ms of kernel `phase` of the last decode (0: tile/segment
-        kernel, 1: Chimp128 kernel)."""
+        """Device ms of kernel `phase` of the last decode (0: tile kernel,
+        1: Chimp128 kernel, 2: XOR segment kernel)."""
         return float(lib().ebi_hip_last_phase_ms(self._h, int(phase)))
 
     def alloc(self, nbytes: int) -> "DeviceBuffer":

@@ -127,9 +127,9 @@ int check_chunk_errors(ebi_hip_ctx *ctx, const Column &c, int64_t chunk_id) {
 // Queue the kernels that write every record of the selected chunks of `c`
 // to out (ColView already uploaded).  chunk_id < 0: all chunks.
 int enqueue_decode(ebi_hip_ctx *ctx, Column &c) {
-  const uint32_t jobs = c.n_tile_jobs + (ctx->xor_mode == 0 ? c.n_seg_jobs : 0);
-  launch::decode(ctx->stream, c.d_view, c.d_prefix, 1, jobs, ctx->d_flags + 1, ctx->d_flags,
-                 (uint32_t)ctx->cu_count * ctx->blocks_per_cu);
+  const uint32_t segs = ctx->xor_mode == 0 ? c.n_seg_jobs : 0;
+  launch::decode(ctx->stream, c.d_view, 1, c.d_prefix, c.n_tile_jobs, c.d_prefix + 6, segs, ctx->d_flags,
+                 (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg);
   hipEventRecord(ctx->ev_mid, ctx->stream);
   if (c.chimp128)
     launch::chimp128_par(ctx->stream, c.d_view, c.d_prefix + 4, 1, (uint32_t)c.chunks.size(), ctx->d_flags,
@@ -366,7 +366,7 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
     return hip_fail(ctx, e, "upload job lists");
 
   {
-    const std::vector<uint32_t> pre = {0, col.n_tile_jobs + col.n_seg_jobs, 0, col.n_serial_jobs, 0, (uint32_t)nc};
+    const std::vector<uint32_t> pre = {0, col.n_tile_jobs, 0, col.n_serial_jobs, 0, (uint32_t)nc, 0, col.n_seg_jobs};
     if ((e = upload_vec(&col.d_prefix, pre)) != hipSuccess) return hip_fail(ctx, e, "upload prefix");
   }
   // Load-time index pass.
@@ -416,7 +416,7 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
     col.n_tile_jobs = (uint32_t)col.h_tile_jobs.size();
     hipFree(col.d_tile_jobs);
     col.d_tile_jobs = nullptr;
-    const uint32_t jobs = col.n_tile_jobs + col.n_seg_jobs;
+    const uint32_t jobs = col.n_tile_jobs;
     if ((e = upload_vec(&col.d_tile_jobs, col.h_tile_jobs)) != hipSuccess ||
         (e = hipMemcpy(col.d_prefix + 1, &jobs, 4, hipMemcpyHostToDevice)) != hipSuccess)
       return hip_fail(ctx, e, "RLE job lists");
@@ -447,6 +447,7 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
       hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev_c0) != hipSuccess || hipEventCreate(&c->ev_c1) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess || hipEventCreate(&c->ev_mid) != hipSuccess ||
+      hipEventCreate(&c->ev_seg) != hipSuccess ||
       hipEventCreate(&c->ev_ph[0]) != hipSuccess || hipEventCreate(&c->ev_ph[1]) != hipSuccess ||
       hipEventCreate(&c->ev_ph[2]) != hipSuccess || hipEventCreate(&c->ev_ph[3]) != hipSuccess ||
       hipMalloc((void **)&c->d_flags, 64 * sizeof(uint32_t)) != hipSuccess ||
@@ -501,6 +502,7 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   if (ctx->ev_mid) hipEventDestroy(ctx->ev_mid);
+  if (ctx->ev_seg) hipEventDestroy(ctx->ev_seg);
   if (ctx->ev_c0) hipEventDestroy(ctx->ev_c0);
   if (ctx->ev_c1) hipEventDestroy(ctx->ev_c1);
   if (ctx->aux) {
@@ -525,15 +527,18 @@ extern "C" double ebi_hip_last_phase_ms(const ebi_hip_ctx *ctx, int phase) {
 }
 
 // Total and per-kernel device time of a decode whose kernels were bracketed
-// by ev0, ev_mid, ev1.
+// by ev0, ev_seg, ev_mid, ev1 (phase 0 k_decode, 2 k_decode_seg, 1 the
+// Chimp128 / serial kernel).
 static void record_decode_times(ebi_hip_ctx *ctx) {
-  float a = 0, b = 0;
-  hipEventElapsedTime(&a, ctx->ev0, ctx->ev_mid);
+  float a = 0, b = 0, c = 0;
+  hipEventElapsedTime(&a, ctx->ev0, ctx->ev_seg);
+  hipEventElapsedTime(&c, ctx->ev_seg, ctx->ev_mid);
   hipEventElapsedTime(&b, ctx->ev_mid, ctx->ev1);
   ctx->phase_ms[0] = a;
   ctx->phase_ms[1] = b;
-  ctx->phase_ms[2] = ctx->phase_ms[3] = 0.0;
-  ctx->last_ms = (double)a + (double)b;
+  ctx->phase_ms[2] = c;
+  ctx->phase_ms[3] = 0.0;
+  ctx->last_ms = (double)a + (double)b + (double)c;
 }
 
 extern "C" int ebi_hip_column_load(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, ebi_hip_col **out) {
@@ -629,8 +634,8 @@ extern "C" int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col
       pre[1] = b - a;
       std::memcpy(ctx->h_batch + 256, &v, sizeof v);
       hipMemcpyAsync(ctx->batch_args.p, ctx->h_batch, 256 + sizeof v, hipMemcpyHostToDevice, ctx->stream);
-      launch::decode(ctx->stream, (const ColView *)((uint8_t *)ctx->batch_args.p + 256),
-                     (const uint32_t *)ctx->batch_args.p, 1, b - a, ctx->d_flags + 1, ctx->d_flags,
+      launch::decode(ctx->stream, (const ColView *)((uint8_t *)ctx->batch_args.p + 256), 1,
+                     (const uint32_t *)ctx->batch_args.p, b - a, nullptr, 0, ctx->d_flags,
                      (uint32_t)ctx->cu_count * 8);
     }
     hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -690,9 +695,9 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
     if (!cols[i] || (!d_outs[i] && cols[i]->c.meta.number_of_records)) return EBI_INVALID_ARGUMENT;
     if ((rc = check_chunk_errors(ctx, cols[i]->c, -1))) return rc;
   }
-  // arguments: prefix over tile+seg jobs, prefix over Chimp128 chunks,
-  // then the ColView array (256-byte aligned)
-  const size_t pre_bytes = ((2 * (n_cols + 1) * 4 + 255) / 256) * 256;
+  // arguments: prefix over tile jobs, prefix over Chimp128 chunks, prefix
+  // over segment jobs, then the ColView array (256-byte aligned)
+  const size_t pre_bytes = ((3 * (n_cols + 1) * 4 + 255) / 256) * 256;
   const size_t bytes = pre_bytes + n_cols * sizeof(ColView);
   if (bytes > ctx->h_batch_cap) {
     hipHostFree(ctx->h_batch);
@@ -704,20 +709,24 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   }
   if (ctx->batch_args.ensure(bytes)) return fail(ctx, EBI_OOM, "batch arguments");
   uint32_t *pre = reinterpret_cast<uint32_t *>(ctx->h_batch);
-  uint32_t *pre128 = pre + (n_cols + 1);
+  uint32_t *pre128 = pre + (n_cols + 1), *preseg = pre128 + (n_cols + 1);
   ColView *views = reinterpret_cast<ColView *>(ctx->h_batch + pre_bytes);
-  uint64_t jobs = 0, jobs128 = 0;
+  uint64_t jobs = 0, jobs128 = 0, segs = 0;
   for (size_t i = 0; i < n_cols; ++i) {
     const Column &c = cols[i]->c;
     pre[i] = (uint32_t)jobs;
     pre128[i] = (uint32_t)jobs128;
+    preseg[i] = (uint32_t)segs;
     views[i] = view_of(c, d_outs[i]);
-    jobs += c.n_tile_jobs + c.n_seg_jobs;
+    jobs += c.n_tile_jobs;
+    segs += c.n_seg_jobs;
     if (c.chimp128) jobs128 += c.chunks.size();
   }
   pre[n_cols] = (uint32_t)jobs;
   pre128[n_cols] = (uint32_t)jobs128;
-  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32)) return fail(ctx, EBI_UNSUPPORTED, "batch too large");
+  preseg[n_cols] = (uint32_t)segs;
+  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32) || segs >= (1ull << 32))
+    return fail(ctx, EBI_UNSUPPORTED, "batch too large");
   uint8_t *d = (uint8_t *)ctx->batch_args.p;
   hipError_t e = hipMemcpyAsync(d, ctx->h_batch, bytes, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "batch upload");
@@ -727,11 +736,12 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   const uint32_t *pre128_d = (const uint32_t *)d + (n_cols + 1);
   // The Chimp128 columns go to the aux stream so that their latency-bound
   // kernel fills the CUs beside the persistent k_decode.
-  const bool side = ctx->overlap && jobs128 && jobs;
+  const bool side = ctx->overlap && jobs128 && (jobs || segs);
   hipStream_t s128 = side ? ctx->aux : ctx->stream;
   if (side) hipStreamWaitEvent(ctx->aux, ctx->ev0, 0);
-  launch::decode(ctx->stream, dv, (const uint32_t *)d, (uint32_t)n_cols, (uint32_t)jobs, ctx->d_flags + 1,
-                 ctx->d_flags, (uint32_t)ctx->cu_count * ctx->blocks_per_cu);
+  launch::decode(ctx->stream, dv, (uint32_t)n_cols, (const uint32_t *)d, (uint32_t)jobs,
+                 (const uint32_t *)d + 2 * (n_cols + 1), (uint32_t)segs, ctx->d_flags,
+                 (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg);
   hipEventRecord(ctx->ev_mid, ctx->stream);
   hipEventRecord(ctx->ev_c0, s128);
   launch::chimp128_par(s128, dv, pre128_d, (uint32_t)n_cols, (uint32_t)jobs128, ctx->d_flags, ctx->c128_block);
@@ -740,13 +750,15 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "batch launch");
   hipEventRecord(ctx->ev1, ctx->stream);
   if ((rc = sync_and_check(ctx, "batch decode"))) return rc;
-  float a = 0, b = 0, t = 0;
-  hipEventElapsedTime(&a, ctx->ev0, ctx->ev_mid);
+  float a = 0, b = 0, c = 0, t = 0;
+  hipEventElapsedTime(&a, ctx->ev0, ctx->ev_seg);
+  hipEventElapsedTime(&c, ctx->ev_seg, ctx->ev_mid);
   hipEventElapsedTime(&b, ctx->ev_c0, ctx->ev_c1);
   hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
   ctx->phase_ms[0] = a;
   ctx->phase_ms[1] = b;
-  ctx->phase_ms[2] = ctx->phase_ms[3] = 0.0;
+  ctx->phase_ms[2] = c;
+  ctx->phase_ms[3] = 0.0;
   ctx->last_ms = t;
   return EBI_OK;
 }

@@ -18,9 +18,9 @@ using namespace ebi;
 struct ebi_hip_batch {
   std::vector<Column *> cols;
   std::vector<double *> outs;
-  DevBuf args;  // prefix[n + 1] | prefix128[n + 1] | pad | ColView[n]
+  DevBuf args;  // prefix[n + 1] | prefix128[n + 1] | prefix_seg[n + 1] | pad | ColView[n]
   uint64_t pre_bytes = 0;
-  uint32_t n_cols = 0, jobs = 0, jobs128 = 0;
+  uint32_t n_cols = 0, jobs = 0, jobs128 = 0, segs = 0;
   ebi_batch_config cfg{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -37,7 +37,7 @@ const uint32_t *prefix_of(const ebi_hip_batch *b) { return static_cast<const uin
 // Issues the batch on ctx->stream with configuration c.  timed: record the
 // phase events (eager runs only; a captured sequence carries no timing).
 void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool timed) {
-  const bool side = c.overlap && b->jobs && b->jobs128;
+  const bool side = c.overlap && (b->jobs || b->segs) && b->jobs128;
   hipStream_t s = ctx->stream, s128 = side ? ctx->aux : ctx->stream;
   launch::zero_u32(s, ctx->d_flags, 2);
   if (timed) hipEventRecord(ctx->ev0, s);
@@ -45,8 +45,8 @@ void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool
     hipEventRecord(b->fork, s);
     hipStreamWaitEvent(ctx->aux, b->fork, 0);
   }
-  launch::decode(s, views_of(b), prefix_of(b), b->n_cols, b->jobs, ctx->d_flags + 1, ctx->d_flags,
-                 (uint32_t)ctx->cu_count * c.blocks_per_cu);
+  launch::decode(s, views_of(b), b->n_cols, prefix_of(b), b->jobs, prefix_of(b) + 2 * (b->n_cols + 1), b->segs,
+                 ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr);
   if (timed) hipEventRecord(ctx->ev_mid, s);
   if (timed) hipEventRecord(ctx->ev_c0, s128);
   launch::chimp128_par(s128, views_of(b), prefix_of(b) + (b->n_cols + 1), b->n_cols, b->jobs128, ctx->d_flags,
@@ -59,14 +59,17 @@ void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool
   if (timed) hipEventRecord(ctx->ev1, s);
 }
 
+// phase 0: k_decode, 1: the Chimp128 kernel, 2: k_decode_seg
 void read_times(ebi_hip_ctx *ctx) {
-  float a = 0, c = 0, t = 0;
-  hipEventElapsedTime(&a, ctx->ev0, ctx->ev_mid);
+  float a = 0, c = 0, g = 0, t = 0;
+  hipEventElapsedTime(&a, ctx->ev0, ctx->ev_seg);
+  hipEventElapsedTime(&g, ctx->ev_seg, ctx->ev_mid);
   hipEventElapsedTime(&c, ctx->ev_c0, ctx->ev_c1);
   hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
   ctx->phase_ms[0] = a;
   ctx->phase_ms[1] = c;
-  ctx->phase_ms[2] = ctx->phase_ms[3] = 0.0;
+  ctx->phase_ms[2] = g;
+  ctx->phase_ms[3] = 0.0;
   ctx->last_ms = t;
 }
 
@@ -123,29 +126,34 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   ebi_hip_batch *b = new (std::nothrow) ebi_hip_batch();
   if (!b) return EBI_OOM;
   b->n_cols = (uint32_t)n_cols;
-  b->pre_bytes = ((2 * (n_cols + 1) * 4 + 255) / 256) * 256;
+  b->pre_bytes = ((3 * (n_cols + 1) * 4 + 255) / 256) * 256;
   std::vector<uint8_t> host(b->pre_bytes + n_cols * sizeof(ColView));
-  uint32_t *pre = reinterpret_cast<uint32_t *>(host.data()), *pre128 = pre + (n_cols + 1);
+  uint32_t *pre = reinterpret_cast<uint32_t *>(host.data()), *pre128 = pre + (n_cols + 1),
+           *preseg = pre128 + (n_cols + 1);
   ColView *views = reinterpret_cast<ColView *>(host.data() + b->pre_bytes);
-  uint64_t jobs = 0, jobs128 = 0;
+  uint64_t jobs = 0, jobs128 = 0, segs = 0;
   for (size_t i = 0; i < n_cols; ++i) {
     Column &c = const_cast<ebi_hip_col *>(cols[i])->c;
     b->cols.push_back(&c);
     b->outs.push_back(d_outs[i]);
     pre[i] = (uint32_t)jobs;
     pre128[i] = (uint32_t)jobs128;
+    preseg[i] = (uint32_t)segs;
     views[i] = view_of(c, d_outs[i]);
-    jobs += c.n_tile_jobs + c.n_seg_jobs;
+    jobs += c.n_tile_jobs;
+    segs += c.n_seg_jobs;
     if (c.chimp128) jobs128 += c.chunks.size();
   }
   pre[n_cols] = (uint32_t)jobs;
   pre128[n_cols] = (uint32_t)jobs128;
-  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32)) {
+  preseg[n_cols] = (uint32_t)segs;
+  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32) || segs >= (1ull << 32)) {
     destroy(b);
     return fail(ctx, EBI_UNSUPPORTED, "batch too large");
   }
   b->jobs = (uint32_t)jobs;
   b->jobs128 = (uint32_t)jobs128;
+  b->segs = (uint32_t)segs;
   hipError_t e;
   if (b->args.ensure(host.size()) ||
       (e = hipMemcpy(b->args.p, host.data(), host.size(), hipMemcpyHostToDevice)) != hipSuccess ||

@@ -44,7 +44,7 @@ struct Column {
   XorCkpt *d_ckpt = nullptr;
   uint32_t *d_chunk_err = nullptr;
   ColView *d_view = nullptr;  // slot used by single-column launches
-  uint32_t *d_prefix = nullptr;  // {0, tile+seg jobs, 0, serial jobs} for single-column launches
+  uint32_t *d_prefix = nullptr;  // {0, tile jobs, 0, serial jobs, 0, chunks, 0, seg jobs}: single-column launches
   // resident job lists
   Job *d_tile_jobs = nullptr;
   uint32_t n_tile_jobs = 0;
@@ -73,7 +73,8 @@ struct ebi_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t ev_mid = nullptr;  // between the tile/segment kernel and the Chimp128 kernel of a decode
+  hipEvent_t ev_seg = nullptr;  // between the tile kernel and the segment kernel of a decode
+  hipEvent_t ev_mid = nullptr;  // between the segment kernel and the Chimp128 kernel of a decode
   hipStream_t aux = nullptr;    // second stream: the Chimp128 kernel of a batch runs beside k_decode
   hipEvent_t ev_c0 = nullptr, ev_c1 = nullptr;  // around the Chimp128 kernel on `aux`
   hipEvent_t ev_ph[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries of matmul calls

@@ -453,17 +453,17 @@ int plan_finish(Plan &p) {
   return EBI_OK;
 }
 
-// Group block: views[n] | prefix[n + 1] | prefix128[n + 1] | flags[4],
+// Group block: views[n] | prefix[n + 1] | prefix128[n + 1] | prefix_seg[n + 1] | flags[4],
 // each part 256-byte aligned.
 uint64_t group_block_bytes(size_t n) {
-  return round_up(n * sizeof(ColView), kAlign) + 2 * round_up((n + 1) * 4, kAlign) + kAlign;
+  return round_up(n * sizeof(ColView), kAlign) + 3 * round_up((n + 1) * 4, kAlign) + kAlign;
 }
 
 void group_bind(PlanGroup &g, uint8_t *dev, uint8_t *pin) {
   g.dev = dev;
   g.pin = pin;
   const size_t n = g.plans.size();
-  const uint64_t flags = round_up(n * sizeof(ColView), kAlign) + 2 * round_up((n + 1) * 4, kAlign);
+  const uint64_t flags = round_up(n * sizeof(ColView), kAlign) + 3 * round_up((n + 1) * 4, kAlign);
   g.d_flags = reinterpret_cast<uint32_t *>(dev + flags);
   g.h_flags = reinterpret_cast<uint32_t *>(pin + flags);
   for (size_t i = 0; i < n; ++i) g.plans[i]->d_view = reinterpret_cast<const ColView *>(dev) + i;
@@ -477,28 +477,33 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   ColView *views = reinterpret_cast<ColView *>(g.pin);
   uint32_t *pre = reinterpret_cast<uint32_t *>(g.pin + vbytes);
   uint32_t *pre128 = reinterpret_cast<uint32_t *>(g.pin + vbytes + pbytes);
-  uint64_t jobs = 0, jobs128 = 0;
+  uint32_t *preseg = reinterpret_cast<uint32_t *>(g.pin + vbytes + 2 * pbytes);
+  uint64_t jobs = 0, jobs128 = 0, segs = 0;
   for (size_t i = 0; i < n; ++i) {
     Plan &p = *g.plans[i];
     views[i] = view_of(*p.c, p.values);
     pre[i] = (uint32_t)jobs;
     pre128[i] = (uint32_t)jobs128;
+    preseg[i] = (uint32_t)segs;
     if (p.decode) {
-      jobs += p.c->n_tile_jobs + p.c->n_seg_jobs;
+      jobs += p.c->n_tile_jobs;
+      segs += p.c->n_seg_jobs;
       if (p.c->chimp128) jobs128 += p.c->chunks.size();
     }
   }
   pre[n] = (uint32_t)jobs;
   pre128[n] = (uint32_t)jobs128;
-  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32)) return hipErrorInvalidValue;
+  preseg[n] = (uint32_t)segs;
+  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32) || segs >= (1ull << 32)) return hipErrorInvalidValue;
   g.h_flags[0] = g.h_flags[1] = 0;
-  hipError_t e = hipMemcpyAsync(g.dev, g.pin, vbytes + 2 * pbytes + 8, hipMemcpyHostToDevice, s);
+  hipError_t e = hipMemcpyAsync(g.dev, g.pin, vbytes + 3 * pbytes + 8, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return e;
   for (Plan *p : g.plans)
     if ((e = plan_enqueue_pre(*p, s)) != hipSuccess) return e;
   const ColView *dv = reinterpret_cast<const ColView *>(g.dev);
   const uint32_t *dpre = reinterpret_cast<const uint32_t *>(g.dev + vbytes);
-  launch::decode(s, dv, dpre, (uint32_t)n, (uint32_t)jobs, g.d_flags + 1, g.d_flags, (uint32_t)cu_count * blocks_per_cu);
+  launch::decode(s, dv, (uint32_t)n, dpre, (uint32_t)jobs, dpre + 2 * (pbytes / 4), (uint32_t)segs, g.d_flags,
+                 (uint32_t)cu_count * blocks_per_cu);
   launch::chimp128_par(s, dv, dpre + (pbytes / 4), (uint32_t)n, (uint32_t)jobs128, g.d_flags, c128_block);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   for (Plan *p : g.plans)

@@ -51,16 +51,57 @@ __device__ __forceinline__ double elf_round_up(double v, uint32_t alpha) {
   return (v < 0.0 ? floor(t) : ceil(t)) / scale;
 }
 
-// ------------------------------------------------------------------ Gorilla
-// decoder/chunk_reader/gorilla.rs:77-157.  Decodes records [i0, i1) starting
-// at bit `bitpos` with state (v = record i0-1, L, T); when i1 == n also
-// checks that the next record is the NaN terminator.
-template <bool kWrite>
-__device__ uint32_t gorilla_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
-                                uint32_t i1, uint32_t bitpos, uint64_t v, uint32_t L, uint32_t T,
-                                LaneSink out, XorCkpt *ck) {
+// State of one XOR stream between calls of its decoder: the bit window and
+// the values the next record depends on (Gorilla: v = previous value, a = L,
+// b = T; Chimp: v, a = stored lead; Elf: v = stored bits, a = L, b = T,
+// beta = last beta*).  A decoder called on records [i0, i1) leaves the
+// state at record i1, so a segment can be decoded in several runs.
+struct XorState {
   BitWindow r;
-  r.init(data, bitpos);
+  uint64_t v;
+  uint32_t a, b;
+  int32_t beta;
+};
+
+// State at a checkpoint written by the index pass (255 / -1 encode "unset").
+__device__ __forceinline__ void xor_state_at(XorState &s, uint32_t codec, const uint8_t *data, const XorCkpt &k) {
+  s.r.init(data, k.bitpos);
+  s.v = k.value;
+  if (codec == kCodecGorilla) {
+    s.a = k.a;
+    s.b = k.b;
+  } else {
+    s.a = k.a == 255 ? 0xffffffffu : k.a;
+    s.b = k.b == 255 ? 0xffffffffu : k.b;
+  }
+  s.beta = k.beta < 0 ? INT32_MAX : (int32_t)k.beta;
+}
+
+// State before record 0 of a chunk.
+__device__ __forceinline__ void xor_state_start(XorState &s, uint32_t codec, const uint8_t *data) {
+  s.r.init(data, 0);
+  s.v = 0;
+  s.a = codec == kCodecGorilla ? 0u : 0xffffffffu;
+  s.b = codec == kCodecGorilla ? 0u : 0xffffffffu;
+  s.beta = INT32_MAX;
+}
+
+// Record sinks of the XOR decoders: put(i, v) receives record i of the chunk.
+struct DirectSink {  // one lane per chunk: records stored as they come
+  double *out;
+  __device__ __forceinline__ void put(uint32_t i, double v) const { out[i] = v; }
+};
+
+// ------------------------------------------------------------------ Gorilla
+// decoder/chunk_reader/gorilla.rs:77-157.  Decodes records [i0, i1) from
+// state s (v = record i0-1, L, T); when i1 == n also checks that the next
+// record is the NaN terminator.  ck (index pass): checkpoints every kSeg.
+template <bool kWrite, class Sink>
+__device__ __forceinline__ uint32_t gorilla_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+                                                uint32_t i1, const Sink &out, XorCkpt *ck) {
+  BitWindow &r = s.r;
+  uint64_t v = s.v;
+  uint32_t L = s.a, T = s.b;
   const uint32_t last = i1 == n ? n + 1 : i1;
   uint32_t i = i0;
   if (i == 0 && i < last) {  // first value: 64 raw bits
@@ -97,18 +138,21 @@ __device__ uint32_t gorilla_run(const uint8_t *data, uint32_t size_bits, uint32_
     if (v == kNan) return kErrCount;
     if (kWrite) out.put(i, as_f64(v));
   }
+  s.v = v;
+  s.a = L;
+  s.b = T;
   return kErrNone;
 }
 
 // -------------------------------------------------------------------- Chimp
 // decoder/chunk_reader/chimp.rs:59-125.  State: v, stored lead (0xffffffff
 // before the first 11/01 record).
-template <bool kWrite>
-__device__ uint32_t chimp_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
-                              uint32_t i1, uint32_t bitpos, uint64_t v, uint32_t lead, LaneSink out,
-                              XorCkpt *ck) {
-  BitWindow r;
-  r.init(data, bitpos);
+template <bool kWrite, class Sink>
+__device__ __forceinline__ uint32_t chimp_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+                                              uint32_t i1, const Sink &out, XorCkpt *ck) {
+  BitWindow &r = s.r;
+  uint64_t v = s.v;
+  uint32_t lead = s.a;
   const uint32_t last = i1 == n ? n + 1 : i1;
   uint32_t i = i0;
   if (i == 0 && i < last) {  // first value: 64 raw bits
@@ -149,6 +193,8 @@ __device__ uint32_t chimp_run(const uint8_t *data, uint32_t size_bits, uint32_t 
     if (v == kNan) return kErrCount;
     if (kWrite) out.put(i, as_f64(v));
   }
+  s.v = v;
+  s.a = lead;
   return kErrNone;
 }
 
@@ -156,12 +202,13 @@ __device__ uint32_t chimp_run(const uint8_t *data, uint32_t size_bits, uint32_t 
 // ElfDecoderWrapper + ElfXorDecoder, decoder/chunk_reader/elf.rs:46-258.
 // State: stored v' bits, L, T (0xffffffff before the first 10/11 record),
 // last beta* (INT32_MAX before the first 11 prefix).
-template <bool kWrite>
-__device__ uint32_t elf_run(const uint8_t *data, uint32_t size_bits, uint32_t n, uint32_t i0,
-                            uint32_t i1, uint32_t bitpos, uint64_t stored, uint32_t L, uint32_t T,
-                            int32_t beta, LaneSink out, XorCkpt *ck) {
-  BitWindow r;
-  r.init(data, bitpos);
+template <bool kWrite, class Sink>
+__device__ __forceinline__ uint32_t elf_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+                                            uint32_t i1, const Sink &out, XorCkpt *ck) {
+  BitWindow &r = s.r;
+  uint64_t stored = s.v;
+  uint32_t L = s.a, T = s.b;
+  int32_t beta = s.beta;
   const uint32_t last = i1 == n ? n + 1 : i1;
   for (uint32_t i = i0; i < last; ++i) {
     if (ck && i < n && (i % kSeg) == 0)
@@ -241,7 +288,71 @@ __device__ uint32_t elf_run(const uint8_t *data, uint32_t size_bits, uint32_t n,
       out.put(i, v);
     }
   }
+  s.v = stored;
+  s.a = L;
+  s.b = T;
+  s.beta = beta;
   return kErrNone;
+}
+
+// Dispatch on the codec (wave-uniform).
+template <bool kWrite, class Sink>
+__device__ __forceinline__ uint32_t xor_run(uint32_t codec, XorState &s, uint32_t size_bits, uint32_t n,
+                                            uint32_t i0, uint32_t i1, const Sink &out, XorCkpt *ck) {
+  switch (codec) {
+    case kCodecGorilla: return gorilla_run<kWrite>(s, size_bits, n, i0, i1, out, ck);
+    case kCodecChimp: return chimp_run<kWrite>(s, size_bits, n, i0, i1, out, ck);
+    case kCodecElf: return elf_run<kWrite>(s, size_bits, n, i0, i1, out, ck);
+    default: return kErrCorrupt;
+  }
+}
+
+// Output staging of a wave decoding 64 checkpoint segments (segment
+// first + l on lane l) in runs of kRun = 16 records: record i of lane l
+// goes to slot (i mod 16, l ^ stage_swz(i mod 16)) of the wave's [16][64]
+// array; the XOR swizzle keeps both the per-record writes (one row, all
+// lanes) and the flush reads (pairs of rows, 8 lanes) free of LDS bank
+// conflicts beyond the two passes a 64-bit access takes.
+constexpr uint32_t kRun = 16;
+__device__ __forceinline__ uint32_t stage_swz(uint32_t r) { return (r >> 1) << 2; }
+
+struct StageSink {
+  double *stage;
+  __device__ __forceinline__ void put(uint32_t i, double v) const {
+    const uint32_t r = i & (kRun - 1);
+    stage[r * kWave + (lane_id() ^ stage_swz(r))] = v;
+  }
+};
+
+// Writes the staged records k0 .. k0 + 15 of the wave's 64 segments to dst
+// (record 0 of segment `first`), called by every lane of the wave: store k
+// covers segments 8k .. 8k + 7, lane j the records k0 + 2(j % 8) and
+// k0 + 2(j % 8) + 1 of segment 8k + j / 8, so each store instruction writes
+// eight whole 128-byte lines.  Records at or past n_job (the chunk's end)
+// are not stored.
+__device__ __forceinline__ void stage_flush(const double *stage, double *dst, uint32_t k0, uint32_t n_job) {
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t j = lane_id();
+  const uint32_t r = 2 * (j & 7);
+  const bool aligned = (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t p = k * 8 + (j >> 3);
+    const uint32_t g = p * kSeg + k0 + r;
+    const uint32_t x = p ^ stage_swz(r);
+    const double a = stage[r * kWave + x], b = stage[(r + 1) * kWave + x];
+    if (g + 1 < n_job) {
+      if (aligned) {
+        st_pair(dst + g, a, b);
+      } else {
+        st_one(dst + g, a);
+        st_one(dst + g + 1, b);
+      }
+    } else if (g < n_job) {
+      st_one(dst + g, a);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
 }
 
 // ----------------------------------------------------------------- Chimp128

@@ -279,36 +279,6 @@ struct BitWindow {
 #endif
 using StreamReader = PrefetchReader<EBI_XOR_DEPTH>;  // long private streams (XOR codecs)
 
-// Output of one lane's serial decode.  Without staging, records are stored
-// as they come.  With staging (`lds` = this lane's column of a [8][64] LDS
-// array), 8 consecutive records are collected and written as four 16-byte
-// stores, so each lane fills whole 64-byte pieces of a line in one burst
-// instead of touching the line 8 times over the decode.
-struct LaneSink {
-  double *out;
-  double *lds;  // nullptr: direct stores
-  uint32_t end; // records >= end are not stored
-  __device__ __forceinline__ void put(uint32_t i, double v) {
-    if (!lds) {
-      out[i] = v;
-      return;
-    }
-    lds[(i & 7) * 64] = v;
-    if ((i & 7) == 7 || i + 1 == end) flush(i);
-  }
-  __device__ __forceinline__ void flush(uint32_t i) {
-    const uint32_t first = i & ~7u, cnt = i - first + 1;
-    double *dst = out + first;
-    if (cnt == 8 && (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
-      double2 *d2 = reinterpret_cast<double2 *>(dst);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) d2[k] = make_double2(lds[(2 * k) * 64], lds[(2 * k + 1) * 64]);
-    } else {
-      for (uint32_t k = 0; k < cnt; ++k) dst[k] = lds[k * 64];
-    }
-  }
-};
-
 // Orders this wave's LDS writes before its later LDS reads by other lanes
 // (waves of one block run independent jobs, so no block barrier is used).
 __device__ __forceinline__ void wave_sync() {

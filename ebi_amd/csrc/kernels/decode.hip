@@ -2,10 +2,10 @@
 //
 // Launch shapes:
 //  * k_decode   -- persistent, 256 threads = 4 independent waves; each wave
-//                  takes wave jobs (a 512-record tile of a tile codec, or 64
-//                  checkpoint segments of an XOR chunk) from the job space of
-//                  every column in the launch, first by id then from an
-//                  atomic counter;
+//                  takes tile jobs (up to four 512-record tiles of a tile
+//                  codec) from the job space of every column in the launch;
+//  * k_decode_seg -- the same over checkpoint-segment jobs (64 segments of
+//                  an XOR chunk, one per lane);
 //  * k_serial   -- 64 threads; lane l walks chunk (job.chunk + l) serially
 //                  (Gorilla, Chimp, Elf); also the load-time index pass that
 //                  writes XOR checkpoints;
@@ -28,43 +28,51 @@ __device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-// One XOR checkpoint segment per lane: lane l restarts from checkpoint
-// (first + l) of chunk ci and decodes its kSeg records.
-__device__ __forceinline__ uint32_t segment_lane(const ColView &cv, uint32_t ci, uint32_t first, double *stage) {
-  const DevChunk c = cv.chunks[ci];
-  const uint32_t seg = first + lane_id();
-  const uint32_t i0 = seg * kSeg;
-  if (i0 >= c.n) return 0;
-  const uint32_t i1 = min(i0 + kSeg, c.n);
-  const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[seg];
+// Checkpoint segments first .. first + 63 of chunk ci, segment first + l on
+// lane l, decoded in runs of kRun records from the checkpoints; after each
+// run the whole wave writes the staged records out (stage_flush).  The loop
+// over runs is wave-uniform: lanes without a segment (past the chunk's end)
+// or stopped by an error keep taking part in the stores.
+template <uint32_t kCodec>
+__device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChunk &c, uint32_t first,
+                                                 double *stage) {
+  const uint32_t i0 = (first + lane_id()) * kSeg;
+  const bool live = i0 < c.n;
+  const uint32_t i1 = live ? min(i0 + kSeg, c.n) : 0;
   const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
-  const LaneSink out{cv.out + c.rec_off, stage + lane_id(), i1};
+  XorState s;
+  if (live) xor_state_at(s, kCodec, data, reinterpret_cast<const XorCkpt *>(c.xidx)[first + lane_id()]);
+  const StageSink sink{stage};
+  double *dst = cv.out + c.rec_off + (size_t)first * kSeg;
+  const uint32_t n_job = first * kSeg < c.n ? c.n - first * kSeg : 0;  // records from segment `first` on
   const uint32_t bits = c.size * 8u;
+  uint32_t e = 0;
+  for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
+    const uint32_t a = i0 + k0, b = min(a + kRun, i1);
+    if (live && e == 0 && a < b) e = xor_run<true>(kCodec, s, bits, c.n, a, b, sink, nullptr);
+    stage_flush(stage, dst, k0, n_job);
+  }
+  return e;
+}
+
+__device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, uint32_t first, double *stage) {
+  const DevChunk c = cv.chunks[ci];
   switch (c.codec) {
-    case kCodecGorilla:
-      return gorilla_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a, k.b, out, nullptr);
-    case kCodecChimp:
-      return chimp_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a == 255 ? 0xffffffffu : k.a, out,
-                             nullptr);
-    case kCodecElf:
-      return elf_run<true>(data, bits, c.n, i0, i1, k.bitpos, k.value, k.a == 255 ? 0xffffffffu : k.a,
-                           k.b == 255 ? 0xffffffffu : k.b, k.beta < 0 ? INT32_MAX : (int32_t)k.beta, out,
-                           nullptr);
-    default:
-      return kErrCorrupt;
+    case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, stage);
+    case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, stage);
+    case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, stage);
+    default: return kErrCorrupt;
   }
 }
 
-// Persistent decode over the job space of n_cols columns: tile jobs of the
-// tile codecs and checkpoint-segment jobs of Gorilla / Chimp / Elf, mixed
-// codecs and uneven chunks in one launch.  Wave w takes jobs w, w + W,
-// w + 2W, ... (W = waves in the grid): a static interleave, so neighbouring
-// jobs (similar cost) spread over all waves and no job counter is contended.
-// amdgpu_waves_per_eu(5): the register budget of 5 waves per SIMD (a few
-// spilled values in cold paths cost less than a lower occupancy).
+// Persistent decode over the tile jobs of n_cols columns (Uncompressed,
+// BUFF, DeltaSprintz, RLE; mixed codecs and uneven chunks in one launch).
+// Wave w takes jobs w, w + W, w + 2W, ... (W = waves in the grid): a static
+// interleave, so neighbouring jobs (similar cost) spread over all waves and
+// no job counter is contended.  amdgpu_waves_per_eu(5): the register budget
+// of 5 waves per SIMD.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_decode(
-    const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
-                                               uint32_t n_jobs, uint32_t *ctr, uint32_t *err) {
+    const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
   // per wave: the store stage, and the field starts of an RLE tile
   __shared__ __align__(16) double lds[4][kStageStride * kWave];
   __shared__ uint32_t lds_starts[4][kTile + 1];
@@ -74,35 +82,49 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
   while (j < n_jobs) {
     const uint32_t ci = find_col(prefix, n_cols, j);
     const ColView cv = cols[ci];
-    const uint32_t local = j - prefix[ci];
-    if (local < cv.n_tile_jobs) {
-      // jb.kind consecutive tiles from jb.tile; RLE: jb.kind output steps
-      // from step jb.tile >> 24 of field tile jb.tile & 0xffffff
-      const Job jb = cv.tile_jobs[local];
-      const DevChunk c = cv.chunks[jb.chunk];
-      if (c.codec == kCodecRLE) {
-        if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-          const uint32_t t = jb.tile & 0xffffffu;
-          rle_tile(c, t, jb.tile >> 24, jb.kind, cv.tile_index[c.tile0 + t], cv.out, lds_starts[wave], lds[wave]);
-        }
-      } else if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-        for (uint32_t t = jb.tile; t < jb.tile + jb.kind; ++t) {
-          switch (c.codec) {
-            case kCodecPlain: plain_tile(c, t, cv.out); break;
-            case kCodecBUFF: buff_tile(c, t, cv.out, lds[wave]); break;
-            case kCodecSprintz: sprintz_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, lds[wave]); break;
-            default:
-              if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
-              break;
-          }
-        }
-      }
+    // jb.kind consecutive tiles from jb.tile; RLE: jb.kind output steps
+    // from step jb.tile >> 24 of field tile jb.tile & 0xffffff
+    const Job jb = cv.tile_jobs[j - prefix[ci]];
+    const DevChunk c = cv.chunks[jb.chunk];
+    if (cv.chunk_err && cv.chunk_err[jb.chunk]) {
+      // chunk rejected at load: nothing to decode
+    } else if (c.codec == kCodecRLE) {
+      const uint32_t t = jb.tile & 0xffffffu;
+      rle_tile(c, t, jb.tile >> 24, jb.kind, cv.tile_index[c.tile0 + t], cv.out, lds_starts[wave], lds[wave]);
     } else {
-      const Job jb = cv.seg_jobs[local - cv.n_tile_jobs];
-      if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-        const uint32_t e = segment_lane(cv, jb.chunk, jb.tile, lds[wave]);
-        if (e) atomicOr(err, e);
+      for (uint32_t t = jb.tile; t < jb.tile + jb.kind; ++t) {
+        switch (c.codec) {
+          case kCodecPlain: plain_tile(c, t, cv.out); break;
+          case kCodecBUFF: buff_tile(c, t, cv.out, lds[wave]); break;
+          case kCodecSprintz: sprintz_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, lds[wave]); break;
+          default:
+            if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
+            break;
+        }
       }
+    }
+    j += stride;
+  }
+}
+
+// Persistent decode over the checkpoint-segment jobs (64 segments of a
+// Gorilla / Chimp / Elf chunk per job) of n_cols columns, with the same
+// static interleave as k_decode.  A kernel of its own, so that the serial
+// bit-stream loops get their own register allocation and LDS (one [16][64]
+// output stage per wave).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_decode_seg(
+    const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
+  __shared__ __align__(16) double lds[4][kRun * kWave];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t stride = gridDim.x * 4;
+  uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
+  while (j < n_jobs) {
+    const uint32_t ci = find_col(prefix, n_cols, j);
+    const ColView cv = cols[ci];
+    const Job jb = cv.seg_jobs[j - prefix[ci]];
+    if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
+      const uint32_t e = segment_job(cv, jb.chunk, jb.tile, lds[wave]);
+      if (e) atomicOr(err, e);
     }
     j += stride;
   }
@@ -129,20 +151,16 @@ __global__ void __launch_bounds__(64) k_serial(const ColView *cols, const uint32
   const DevChunk c = cv.chunks[ci];
   if (!kIndex && cv.chunk_err && cv.chunk_err[ci]) return;
   const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
-  const LaneSink out{cv.out + c.rec_off, nullptr, c.n};
+  const DirectSink out{cv.out + c.rec_off};
   XorCkpt *ck = kIndex ? reinterpret_cast<XorCkpt *>(c.xidx) : nullptr;
   const uint32_t bits = c.size * 8u;
   uint32_t e;
-  switch (c.codec) {
-    case kCodecGorilla: e = gorilla_run<kWrite>(data, bits, c.n, 0, c.n, 0, 0, 0, 0, out, ck); break;
-    case kCodecChimp: e = chimp_run<kWrite>(data, bits, c.n, 0, c.n, 0, 0, 0xffffffffu, out, ck); break;
-    case kCodecElf:
-      e = elf_run<kWrite>(data, bits, c.n, 0, c.n, 0, 0, 0xffffffffu, 0xffffffffu, INT32_MAX, out, ck);
-      break;
-    case kCodecChimp128:
-      e = kIndex ? chimp128_index(data, bits, c.n, ck) : kErrCorrupt;
-      break;
-    default: e = kErrCorrupt; break;
+  if (c.codec == kCodecChimp128) {
+    e = kIndex ? chimp128_index(data, bits, c.n, ck) : kErrCorrupt;
+  } else {
+    XorState s;
+    xor_state_start(s, c.codec, data);
+    e = xor_run<kWrite>(c.codec, s, bits, c.n, 0, c.n, out, ck);
   }
   if (e) {
     atomicOr(err, e);
@@ -317,13 +335,19 @@ void zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
   if (n) hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
 }
 
-void decode(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
-            uint32_t *ctr, uint32_t *err, uint32_t max_blocks) {
-  if (!n_jobs) return;
-  uint32_t blocks = (n_jobs + 3) / 4;
-  if (max_blocks && blocks > max_blocks) blocks = max_blocks;
-  zero_u32(s, ctr, 1);
-  hipLaunchKernelGGL(k_decode, dim3(blocks), dim3(256), 0, s, cols, prefix, n_cols, n_jobs, ctr, err);
+void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t *tile_prefix, uint32_t n_tile_jobs,
+            const uint32_t *seg_prefix, uint32_t n_seg_jobs, uint32_t *err, uint32_t max_blocks, hipEvent_t mid) {
+  auto grid = [&](uint32_t jobs) {
+    const uint32_t blocks = (jobs + 3) / 4;
+    return max_blocks && blocks > max_blocks ? max_blocks : blocks;
+  };
+  if (n_tile_jobs)
+    hipLaunchKernelGGL(k_decode, dim3(grid(n_tile_jobs)), dim3(256), 0, s, cols, tile_prefix, n_cols, n_tile_jobs,
+                       err);
+  if (mid) hipEventRecord(mid, s);
+  if (n_seg_jobs)
+    hipLaunchKernelGGL(k_decode_seg, dim3(grid(n_seg_jobs)), dim3(256), 0, s, cols, seg_prefix, n_cols,
+                       n_seg_jobs, err);
 }
 
 void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,

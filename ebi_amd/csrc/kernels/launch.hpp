@@ -13,10 +13,14 @@ namespace launch {
 hipError_t set_pow10(const double *p10, const double *n10);
 void zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 
-// Persistent decode over the tile + segment job space of n_cols columns
-// (prefix has n_cols + 1 entries).
-void decode(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
-            uint32_t *ctr, uint32_t *err, uint32_t max_blocks);
+// Persistent decode of n_cols columns: k_decode over their tile jobs, then
+// k_decode_seg over their checkpoint-segment jobs (tile_prefix / seg_prefix:
+// n_cols + 1 entries each, the job prefix of each list; either may be null
+// when its job count is 0).  Grids of at most max_blocks blocks; `mid`
+// (nullable) is recorded between the two kernels.
+void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t *tile_prefix, uint32_t n_tile_jobs,
+            const uint32_t *seg_prefix, uint32_t n_seg_jobs, uint32_t *err, uint32_t max_blocks,
+            hipEvent_t mid = nullptr);
 // Lane-per-chunk serial XOR decode (Gorilla, Chimp, Elf); `index` writes
 // checkpoints.  prefix is over n_serial_jobs; `single` (device, nullable)
 // runs one explicit job on cols[0] instead.

@@ -327,7 +327,8 @@ double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx);
 
 /* Device time of each phase of the last decode or matmul call on this ctx:
  * for ebi_hip_decode_device / ebi_hip_decode_batch phase 0 = the persistent
- * tile and segment kernel (every codec but Chimp128), phase 1 = the Chimp128
+ * tile kernel (Uncompressed, RLE, BUFF, DeltaSprintz), phase 2 = the
+ * persistent segment kernel (Gorilla, Chimp, Elf), phase 1 = the Chimp128
  * kernel; for the matmul entry points the phases listed there.
  * Instrumentation of this library; no counterpart in the reference. */
 double ebi_hip_last_phase_ms(const ebi_hip_ctx *ctx, int phase);
