@@ -22,31 +22,50 @@ namespace ebi {
 __constant__ double c_p10[400];
 __constant__ double c_n10[400];
 
-__device__ __forceinline__ int32_t sat_i32(double d) {
+__host__ __device__ __forceinline__ int32_t sat_i32(double d) {
   if (d != d) return 0;
   if (d >= 2147483647.0) return INT32_MAX;
   if (d <= -2147483648.0) return INT32_MIN;
   return (int32_t)d;
 }
 
-// get_sp, compression_common/elf.rs:111-130: table walk for [1e-10, 1e9),
-// floor(log10) otherwise.
-__device__ __forceinline__ int32_t elf_get_sp(double v) {
+// get_sp, compression_common/elf.rs:111-130.  The reference walks a table
+// for v in [1e-10, 1e9) and takes floor(log10(v)) otherwise; the walk is
+// restated as counts of threshold compares (no divergent loop): for v >= 1
+// it returns the first i in 0..8 with v < 1e{i+1}, i.e. #{j in 1..9 : v >=
+// 1e{j}}; for v < 1 the first i in 1..10 with v >= 1e-{i}, i.e. 11 - #{j in
+// 1..10 : v >= 1e-{j}} (a count of 0 -- v below 1e-10, zero, NaN -- falls
+// through).  The literals are the correctly rounded values the reference
+// parses from "1.0E{i}" / "1.0E-{i}".
+__host__ __device__ __forceinline__ int32_t elf_get_sp(double v) {
   if (v >= 1.0) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i)
-      if (v < c_p10[i + 1]) return i;
+    const int32_t c = (v >= 1e1) + (v >= 1e2) + (v >= 1e3) + (v >= 1e4) + (v >= 1e5) + (v >= 1e6) + (v >= 1e7) +
+                      (v >= 1e8) + (v >= 1e9);
+    if (c < 9) return c;
   } else {
-#pragma unroll
-    for (int i = 1; i < 11; ++i)
-      if (v >= c_n10[i]) return -i;
+    const int32_t d = (v >= 1e-1) + (v >= 1e-2) + (v >= 1e-3) + (v >= 1e-4) + (v >= 1e-5) + (v >= 1e-6) +
+                      (v >= 1e-7) + (v >= 1e-8) + (v >= 1e-9) + (v >= 1e-10);
+    if (d > 0) return d - 11;
   }
   return sat_i32(floor(log10(v)));
 }
 
+// 10^a for a <= 22, exactly representable: each partial product below is an
+// exact power of ten, so the result equals the reference's parse of "1.0E{a}".
+__host__ __device__ __forceinline__ double p10_exact(uint32_t a) {
+  double s = (a & 1) ? 10.0 : 1.0;
+  s *= (a & 2) ? 100.0 : 1.0;
+  s *= (a & 4) ? 1e4 : 1.0;
+  s *= (a & 8) ? 1e8 : 1.0;
+  s *= (a & 16) ? 1e16 : 1.0;
+  return s;
+}
+
 // round_up, compression_common/elf.rs:51-58 (no FMA contraction).
 __device__ __forceinline__ double elf_round_up(double v, uint32_t alpha) {
-  const double scale = alpha < 400 ? c_p10[alpha] : __longlong_as_double(0x7ff0000000000000ll);
+  const double scale = alpha <= 22 ? p10_exact(alpha)
+                       : alpha < 400 ? c_p10[alpha]
+                                     : __longlong_as_double(0x7ff0000000000000ll);
   const double t = v * scale;
   return (v < 0.0 ? floor(t) : ceil(t)) / scale;
 }

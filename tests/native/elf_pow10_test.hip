@@ -1,0 +1,65 @@
+// Host check of the Elf helpers: elf_get_sp (threshold counts) against the
+// reference's table walk of compression_common/elf.rs:111-130 over tables
+// parsed with strtod exactly as the reference builds them (elf.rs:95-109),
+// on every power of ten and its two neighbours, special values and random
+// magnitudes; p10_exact(a) against the parsed "1.0E{a}" for a <= 22.
+// Exit code 0 on success.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <limits>
+#include <random>
+#include <vector>
+
+#include "../../ebi_amd/csrc/kernels/codecs.hpp"
+
+static double p10[400], n10[400];
+
+static int32_t walk_sp(double v) {  // the reference's loop, over the parsed tables
+  if (v >= 1.0) {
+    for (int i = 0; i < 9; ++i)
+      if (v < p10[i + 1]) return i;
+  } else {
+    for (int i = 1; i < 11; ++i)
+      if (v >= n10[i]) return -i;
+  }
+  return ebi::sat_i32(std::floor(std::log10(v)));
+}
+
+int main() {
+  char buf[32];
+  for (int i = 0; i < 400; ++i) {
+    std::snprintf(buf, sizeof buf, "1.0E%d", i);
+    p10[i] = std::strtod(buf, nullptr);
+    std::snprintf(buf, sizeof buf, "1.0E-%d", i);
+    n10[i] = std::strtod(buf, nullptr);
+  }
+  long fails = 0, checks = 0;
+  for (uint32_t a = 0; a <= 22; ++a, ++checks)
+    if (ebi::p10_exact(a) != p10[a]) {
+      std::printf("p10_exact(%u) = %.17g, parsed %.17g\n", a, ebi::p10_exact(a), p10[a]);
+      ++fails;
+    }
+  std::vector<double> vs = {0.0, 1.0, std::numeric_limits<double>::denorm_min(), std::numeric_limits<double>::min(),
+                            std::numeric_limits<double>::max(), std::numeric_limits<double>::infinity(),
+                            std::numeric_limits<double>::quiet_NaN()};
+  for (int i = 0; i < 40; ++i)
+    for (double x : {p10[i], n10[i]}) {
+      vs.push_back(x);
+      vs.push_back(std::nextafter(x, 0.0));
+      vs.push_back(std::nextafter(x, std::numeric_limits<double>::infinity()));
+    }
+  std::mt19937_64 rng(2024);
+  std::uniform_real_distribution<double> e(-14.0, 14.0);
+  for (int i = 0; i < 1000000; ++i) vs.push_back(std::pow(10.0, e(rng)));
+  for (double v : vs) {
+    ++checks;
+    const int32_t a = ebi::elf_get_sp(v), b = walk_sp(v);
+    if (a != b) {
+      if (fails < 10) std::printf("get_sp(%.17g) = %d, table walk %d\n", v, a, b);
+      ++fails;
+    }
+  }
+  std::printf("%s: %ld checks, %ld failures\n", fails ? "FAIL" : "OK", checks, fails);
+  return fails ? 1 : 0;
+}
