@@ -107,8 +107,8 @@ class _ServeStats(C.Structure):
 
 
 class _BatchCfg(C.Structure):
-    _fields_ = [("blocks_per_cu", C.c_uint32), ("c128_block", C.c_uint32), ("overlap", C.c_uint32),
-                ("reserved", C.c_uint32), ("tuned_ms", C.c_double)]
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("reserved0", C.c_uint32), ("reserved1", C.c_uint32),
+                ("reserved2", C.c_uint32), ("tuned_ms", C.c_double)]
 
 
 class _EncCfg(C.Structure):
@@ -452,7 +452,7 @@ class Context:
 
     def last_phase_ms(self, phase: int) -> float:
         """Device ms of kernel `phase` of the last decode (0: tile kernel,
-        1: Chimp128 kernel, 2: XOR segment kernel)."""
+        1: Chimp128 resolve kernel, 2: XOR segment kernel)."""
         return float(lib().ebi_hip_last_phase_ms(self._h, int(phase)))
 
     def alloc(self, nbytes: int) -> "DeviceBuffer":
@@ -781,8 +781,7 @@ class Batch:
     def config(self) -> dict:
         c = _BatchCfg()
         _check(lib().ebi_hip_batch_config_get(self._h, C.byref(c)), None, "batch_config")
-        return {"blocks_per_cu": c.blocks_per_cu, "c128_block": c.c128_block, "overlap": bool(c.overlap),
-                "tuned_ms": c.tuned_ms}
+        return {"blocks_per_cu": c.blocks_per_cu, "tuned_ms": c.tuned_ms}
 
     def close(self):
         if getattr(self, "_h", None) and self.ctx._h:

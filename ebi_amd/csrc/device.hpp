@@ -123,6 +123,15 @@ struct alignas(16) Job {
   uint32_t tile;   // tile / first segment index; lane count for kJobSerial
 };
 
+// Parse of a Chimp128 chunk's terminator record n: XOR payload and source
+// distance (its value must come out as the NaN end marker).
+struct alignas(16) C128Term {
+  uint64_t x;
+  uint32_t dist;
+  uint32_t pad;
+};
+static_assert(sizeof(C128Term) == 16, "C128Term layout");
+
 // Per-launch view of one column.  A launch over several columns walks the
 // concatenation of their job lists; `prefix` arrays (one entry per column
 // plus one) map a launch-wide job id to (column, local job).
@@ -139,8 +148,19 @@ struct alignas(16) ColView {
   uint32_t n_tile_jobs;
   uint32_t n_seg_jobs;
   uint32_t n_serial_jobs;
+  // Chimp128 decode scratch of this launch (null for other codecs): source
+  // distance of every record (byte rtile0 * kTile + i for record i of a
+  // chunk), and per chunk the terminator record's parse.
+  uint8_t *c128_dist;
+  C128Term *c128_term;
 };
-static_assert(sizeof(ColView) == 80, "ColView layout");
+static_assert(sizeof(ColView) == 96, "ColView layout");
+
+// Device bytes of the Chimp128 decode scratch of a column of n_rtiles record
+// tiles and n_chunks chunks: distances, then the terminator slots.
+inline uint64_t c128_scratch_bytes(uint64_t n_rtiles, uint64_t n_chunks) {
+  return n_rtiles * kTile + n_chunks * sizeof(C128Term);
+}
 
 // Device error flags (OR-ed by kernels).
 enum DevError : uint32_t {

@@ -88,8 +88,25 @@ ColView view_of(const Column &c, double *out) {
   return v;
 }
 
+uint64_t c128_scratch_need(const Column &c) {
+  return c.chimp128 ? round_up(c128_scratch_bytes(c.n_rtiles, c.chunks.size()), kAlign) : 0;
+}
+
+void bind_c128_scratch(ColView &v, const Column &c, uint8_t *scratch) {
+  if (!c.chimp128) return;
+  v.c128_dist = scratch;
+  v.c128_term = reinterpret_cast<C128Term *>(scratch + (uint64_t)c.n_rtiles * kTile);
+}
+
 int upload_view(ebi_hip_ctx *ctx, Column &c, double *out) {
   ColView v = view_of(c, out);
+  if (c.chimp128) {
+    // earlier work on the stream may still use the scratch: it must finish
+    // before a regrow frees it
+    hipStreamSynchronize(ctx->stream);
+    if (ctx->c128_scratch.ensure(c128_scratch_need(c))) return fail(ctx, EBI_OOM, "Chimp128 decode scratch");
+    bind_c128_scratch(v, c, static_cast<uint8_t *>(ctx->c128_scratch.p));
+  }
   *ctx->h_view = v;
   hipError_t e = hipMemcpyAsync(c.d_view, ctx->h_view, sizeof(ColView), hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(ColView)");
@@ -127,13 +144,13 @@ int check_chunk_errors(ebi_hip_ctx *ctx, const Column &c, int64_t chunk_id) {
 // Queue the kernels that write every record of the selected chunks of `c`
 // to out (ColView already uploaded).  chunk_id < 0: all chunks.
 int enqueue_decode(ebi_hip_ctx *ctx, Column &c) {
-  const uint32_t segs = ctx->xor_mode == 0 ? c.n_seg_jobs : 0;
+  // Chimp128 always parses in segments (the resolve kernel needs them)
+  const uint32_t segs = ctx->xor_mode == 0 || c.chimp128 ? c.n_seg_jobs : 0;
   launch::decode(ctx->stream, c.d_view, 1, c.d_prefix, c.n_tile_jobs, c.d_prefix + 6, segs, ctx->d_flags,
                  (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg);
   hipEventRecord(ctx->ev_mid, ctx->stream);
   if (c.chimp128)
-    launch::chimp128_par(ctx->stream, c.d_view, c.d_prefix + 4, 1, (uint32_t)c.chunks.size(), ctx->d_flags,
-                         ctx->c128_block);
+    launch::chimp128_resolve(ctx->stream, c.d_view, c.d_prefix + 4, 1, (uint32_t)c.chunks.size(), ctx->d_flags);
   else if (c.xor_codec && ctx->xor_mode != 0)
     launch::serial(ctx->stream, c.d_view, c.d_prefix + 2, 1, c.n_serial_jobs, nullptr, ctx->d_flags, true, false);
   hipError_t e = hipGetLastError();
@@ -339,9 +356,8 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
       for (uint32_t t = 0; t < nt; t += kTilesPerJob)
         col.h_tile_jobs.push_back(Job{std::min(kTilesPerJob, nt - t), 0, (uint32_t)k, t});
     }
-    if (!col.chimp128)
-      for (uint32_t s = 0; s < col.chunks[k].n_segs; s += kWaveLanes)
-        col.h_seg_jobs.push_back(Job{kJobSeg, 0, (uint32_t)k, s});
+    for (uint32_t s = 0; s < col.chunks[k].n_segs; s += kWaveLanes)
+      col.h_seg_jobs.push_back(Job{kJobSeg, 0, (uint32_t)k, s});
   }
   col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
   std::vector<Job> rec_jobs;
@@ -444,8 +460,6 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     c->cu_count = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev_c0) != hipSuccess || hipEventCreate(&c->ev_c1) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess || hipEventCreate(&c->ev_mid) != hipSuccess ||
       hipEventCreate(&c->ev_seg) != hipSuccess ||
       hipEventCreate(&c->ev_ph[0]) != hipSuccess || hipEventCreate(&c->ev_ph[1]) != hipSuccess ||
@@ -476,11 +490,6 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   if (mode && std::strcmp(mode, "serial") == 0) c->xor_mode = 1;
   // launch-shape overrides (the batch tuner picks its own per batch)
   if (const char *v = std::getenv("EBI_HIP_BLOCKS_PER_CU")) c->blocks_per_cu = std::max(1, std::min(64, std::atoi(v)));
-  if (const char *v = std::getenv("EBI_HIP_C128_BLOCK")) {
-    const int b = std::atoi(v);
-    if (b == 512 || b == 1024 || b == 2048 || b == 4096) c->c128_block = (uint32_t)b;
-  }
-  if (const char *v = std::getenv("EBI_HIP_OVERLAP")) c->overlap = std::atoi(v) != 0;
   *out = c;
   return EBI_OK;
 }
@@ -495,6 +504,7 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   ctx->gemm_a.release();
   ctx->gemm_b.release();
   ctx->gemm_c.release();
+  ctx->c128_scratch.release();
   if (ctx->h_batch) hipHostFree(ctx->h_batch);
   if (ctx->d_flags) hipFree(ctx->d_flags);
   if (ctx->h_flags) hipHostFree(ctx->h_flags);
@@ -503,12 +513,6 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   if (ctx->ev_mid) hipEventDestroy(ctx->ev_mid);
   if (ctx->ev_seg) hipEventDestroy(ctx->ev_seg);
-  if (ctx->ev_c0) hipEventDestroy(ctx->ev_c0);
-  if (ctx->ev_c1) hipEventDestroy(ctx->ev_c1);
-  if (ctx->aux) {
-    hipStreamSynchronize(ctx->aux);
-    hipStreamDestroy(ctx->aux);
-  }
   for (hipEvent_t e : ctx->ev_ph)
     if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -711,13 +715,19 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   uint32_t *pre = reinterpret_cast<uint32_t *>(ctx->h_batch);
   uint32_t *pre128 = pre + (n_cols + 1), *preseg = pre128 + (n_cols + 1);
   ColView *views = reinterpret_cast<ColView *>(ctx->h_batch + pre_bytes);
-  uint64_t jobs = 0, jobs128 = 0, segs = 0;
+  uint64_t jobs = 0, jobs128 = 0, segs = 0, scratch = 0;
+  for (size_t i = 0; i < n_cols; ++i) scratch += c128_scratch_need(cols[i]->c);
+  if (scratch && (hipStreamSynchronize(ctx->stream), ctx->c128_scratch.ensure(scratch)))
+    return fail(ctx, EBI_OOM, "Chimp128 decode scratch");
+  scratch = 0;
   for (size_t i = 0; i < n_cols; ++i) {
     const Column &c = cols[i]->c;
     pre[i] = (uint32_t)jobs;
     pre128[i] = (uint32_t)jobs128;
     preseg[i] = (uint32_t)segs;
     views[i] = view_of(c, d_outs[i]);
+    bind_c128_scratch(views[i], c, static_cast<uint8_t *>(ctx->c128_scratch.p) + scratch);
+    scratch += c128_scratch_need(c);
     jobs += c.n_tile_jobs;
     segs += c.n_seg_jobs;
     if (c.chimp128) jobs128 += c.chunks.size();
@@ -734,26 +744,18 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   hipEventRecord(ctx->ev0, ctx->stream);
   const ColView *dv = (const ColView *)(d + pre_bytes);
   const uint32_t *pre128_d = (const uint32_t *)d + (n_cols + 1);
-  // The Chimp128 columns go to the aux stream so that their latency-bound
-  // kernel fills the CUs beside the persistent k_decode.
-  const bool side = ctx->overlap && jobs128 && (jobs || segs);
-  hipStream_t s128 = side ? ctx->aux : ctx->stream;
-  if (side) hipStreamWaitEvent(ctx->aux, ctx->ev0, 0);
   launch::decode(ctx->stream, dv, (uint32_t)n_cols, (const uint32_t *)d, (uint32_t)jobs,
                  (const uint32_t *)d + 2 * (n_cols + 1), (uint32_t)segs, ctx->d_flags,
                  (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg);
   hipEventRecord(ctx->ev_mid, ctx->stream);
-  hipEventRecord(ctx->ev_c0, s128);
-  launch::chimp128_par(s128, dv, pre128_d, (uint32_t)n_cols, (uint32_t)jobs128, ctx->d_flags, ctx->c128_block);
-  hipEventRecord(ctx->ev_c1, s128);
-  if (side) hipStreamWaitEvent(ctx->stream, ctx->ev_c1, 0);
+  launch::chimp128_resolve(ctx->stream, dv, pre128_d, (uint32_t)n_cols, (uint32_t)jobs128, ctx->d_flags);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "batch launch");
   hipEventRecord(ctx->ev1, ctx->stream);
   if ((rc = sync_and_check(ctx, "batch decode"))) return rc;
   float a = 0, b = 0, c = 0, t = 0;
   hipEventElapsedTime(&a, ctx->ev0, ctx->ev_seg);
   hipEventElapsedTime(&c, ctx->ev_seg, ctx->ev_mid);
-  hipEventElapsedTime(&b, ctx->ev_c0, ctx->ev_c1);
+  hipEventElapsedTime(&b, ctx->ev_mid, ctx->ev1);
   hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
   ctx->phase_ms[0] = a;
   ctx->phase_ms[1] = b;

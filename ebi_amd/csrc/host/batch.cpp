@@ -1,8 +1,8 @@
 // Prepared batched decode: a fixed set of resident columns decoded into fixed
 // device outputs, replayed many times.  The job-space arguments are uploaded
-// once; an autotuner times candidate launch shapes on this very batch
-// (persistent grid size, Chimp128 LDS block) and keeps the fastest; the launch sequence is captured in a
-// hipGraph and replayed with one launch.  No counterpart in lemolatoon/ebi,
+// once; an autotuner times candidate persistent grid sizes on this very
+// batch and keeps the fastest; the launch sequence is captured in a hipGraph
+// and replayed with one launch.  The batch owns its Chimp128 decode scratch.  No counterpart in lemolatoon/ebi,
 // whose Decoder decodes chunk after chunk on the CPU (api/decoder.rs:202-241).
 #include <algorithm>
 #include <cstring>
@@ -18,13 +18,13 @@ using namespace ebi;
 struct ebi_hip_batch {
   std::vector<Column *> cols;
   std::vector<double *> outs;
-  DevBuf args;  // prefix[n + 1] | prefix128[n + 1] | prefix_seg[n + 1] | pad | ColView[n]
+  DevBuf args;     // prefix[n + 1] | prefix128[n + 1] | prefix_seg[n + 1] | pad | ColView[n]
+  DevBuf scratch;  // Chimp128 decode scratch of the batch's columns
   uint64_t pre_bytes = 0;
   uint32_t n_cols = 0, jobs = 0, jobs128 = 0, segs = 0;
   ebi_batch_config cfg{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;  // fork / join of the aux branch (no timing)
 };
 
 namespace {
@@ -37,34 +37,22 @@ const uint32_t *prefix_of(const ebi_hip_batch *b) { return static_cast<const uin
 // Issues the batch on ctx->stream with configuration c.  timed: record the
 // phase events (eager runs only; a captured sequence carries no timing).
 void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool timed) {
-  const bool side = c.overlap && (b->jobs || b->segs) && b->jobs128;
-  hipStream_t s = ctx->stream, s128 = side ? ctx->aux : ctx->stream;
+  hipStream_t s = ctx->stream;
   launch::zero_u32(s, ctx->d_flags, 2);
   if (timed) hipEventRecord(ctx->ev0, s);
-  if (side) {
-    hipEventRecord(b->fork, s);
-    hipStreamWaitEvent(ctx->aux, b->fork, 0);
-  }
   launch::decode(s, views_of(b), b->n_cols, prefix_of(b), b->jobs, prefix_of(b) + 2 * (b->n_cols + 1), b->segs,
                  ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr);
   if (timed) hipEventRecord(ctx->ev_mid, s);
-  if (timed) hipEventRecord(ctx->ev_c0, s128);
-  launch::chimp128_par(s128, views_of(b), prefix_of(b) + (b->n_cols + 1), b->n_cols, b->jobs128, ctx->d_flags,
-                       c.c128_block);
-  if (timed) hipEventRecord(ctx->ev_c1, s128);
-  if (side) {
-    hipEventRecord(b->join, ctx->aux);
-    hipStreamWaitEvent(s, b->join, 0);
-  }
+  launch::chimp128_resolve(s, views_of(b), prefix_of(b) + (b->n_cols + 1), b->n_cols, b->jobs128, ctx->d_flags);
   if (timed) hipEventRecord(ctx->ev1, s);
 }
 
-// phase 0: k_decode, 1: the Chimp128 kernel, 2: k_decode_seg
+// phase 0: k_decode, 1: k_chimp128_resolve, 2: k_decode_seg
 void read_times(ebi_hip_ctx *ctx) {
   float a = 0, c = 0, g = 0, t = 0;
   hipEventElapsedTime(&a, ctx->ev0, ctx->ev_seg);
   hipEventElapsedTime(&g, ctx->ev_seg, ctx->ev_mid);
-  hipEventElapsedTime(&c, ctx->ev_c0, ctx->ev_c1);
+  hipEventElapsedTime(&c, ctx->ev_mid, ctx->ev1);
   hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
   ctx->phase_ms[0] = a;
   ctx->phase_ms[1] = c;
@@ -105,9 +93,8 @@ int capture(ebi_hip_ctx *ctx, ebi_hip_batch *b) {
 void destroy(ebi_hip_batch *b) {
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
-  if (b->fork) hipEventDestroy(b->fork);
-  if (b->join) hipEventDestroy(b->join);
   b->args.release();
+  b->scratch.release();
   delete b;
 }
 
@@ -131,7 +118,13 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   uint32_t *pre = reinterpret_cast<uint32_t *>(host.data()), *pre128 = pre + (n_cols + 1),
            *preseg = pre128 + (n_cols + 1);
   ColView *views = reinterpret_cast<ColView *>(host.data() + b->pre_bytes);
-  uint64_t jobs = 0, jobs128 = 0, segs = 0;
+  uint64_t jobs = 0, jobs128 = 0, segs = 0, scratch = 0;
+  for (size_t i = 0; i < n_cols; ++i) scratch += c128_scratch_need(cols[i]->c);
+  if (scratch && b->scratch.ensure(scratch)) {
+    destroy(b);
+    return fail(ctx, EBI_OOM, "batch Chimp128 scratch");
+  }
+  scratch = 0;
   for (size_t i = 0; i < n_cols; ++i) {
     Column &c = const_cast<ebi_hip_col *>(cols[i])->c;
     b->cols.push_back(&c);
@@ -140,6 +133,8 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
     pre128[i] = (uint32_t)jobs128;
     preseg[i] = (uint32_t)segs;
     views[i] = view_of(c, d_outs[i]);
+    bind_c128_scratch(views[i], c, static_cast<uint8_t *>(b->scratch.p) + scratch);
+    scratch += c128_scratch_need(c);
     jobs += c.n_tile_jobs;
     segs += c.n_seg_jobs;
     if (c.chimp128) jobs128 += c.chunks.size();
@@ -156,18 +151,13 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   b->segs = (uint32_t)segs;
   hipError_t e;
   if (b->args.ensure(host.size()) ||
-      (e = hipMemcpy(b->args.p, host.data(), host.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+      (e = hipMemcpy(b->args.p, host.data(), host.size(), hipMemcpyHostToDevice)) != hipSuccess) {
     destroy(b);
     return fail(ctx, EBI_OOM, "batch arguments");
   }
   b->cfg.blocks_per_cu = ctx->blocks_per_cu;
-  b->cfg.c128_block = ctx->c128_block;
-  b->cfg.overlap = ctx->overlap ? 1 : 0;
   if (autotune) {
-    // coordinate search: grid size, then the Chimp128 block (the overlap
-    // setting is the context's: kernels back to back unless asked)
+    // persistent grid size of the decode kernels
     double best = time_config(ctx, b, b->cfg, 2);
     if (best < 0) {
       destroy(b);
@@ -186,13 +176,6 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
         c.blocks_per_cu = bpc;
         consider(c);
       }
-    if (b->jobs128)
-      for (uint32_t blk : {512u, 1024u, 2048u, 4096u})
-        if (blk != b->cfg.c128_block) {
-          ebi_batch_config c = b->cfg;
-          c.c128_block = blk;
-          consider(c);
-        }
     b->cfg.tuned_ms = best;
   }
   if ((rc = capture(ctx, b))) {
@@ -236,7 +219,6 @@ extern "C" int ebi_hip_batch_destroy(ebi_hip_ctx *ctx, ebi_hip_batch *b) {
   EBI_GUARD(ctx);
   if (!b) return EBI_INVALID_ARGUMENT;
   hipStreamSynchronize(ctx->stream);
-  hipStreamSynchronize(ctx->aux);
   destroy(b);
   return EBI_OK;
 }

@@ -75,8 +75,6 @@ struct ebi_hip_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_seg = nullptr;  // between the tile kernel and the segment kernel of a decode
   hipEvent_t ev_mid = nullptr;  // between the segment kernel and the Chimp128 kernel of a decode
-  hipStream_t aux = nullptr;    // second stream: the Chimp128 kernel of a batch runs beside k_decode
-  hipEvent_t ev_c0 = nullptr, ev_c1 = nullptr;  // around the Chimp128 kernel on `aux`
   hipEvent_t ev_ph[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries of matmul calls
   double last_ms = 0.0;
   double phase_ms[4] = {0.0, 0.0, 0.0, 0.0};  // of the last decode_device / decode_batch / matmul call
@@ -97,9 +95,6 @@ struct ebi_hip_ctx {
   int cu_count = 256;
   // launch shape of the decode kernels (defaults; a prepared batch tunes its own)
   uint32_t blocks_per_cu = 8;   // persistent k_decode grid = cu_count * blocks_per_cu
-  uint32_t c128_block = 2048;   // records per LDS block of the Chimp128 kernel
-  // Chimp128 kernel concurrent with k_decode in batches (EBI_HIP_OVERLAP=1):
-  // a few percent faster, but neither kernel's duration is then its own
-  bool overlap = false;
-  int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks
+  int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks (Gorilla, Chimp, Elf)
+  ebi::DevBuf c128_scratch;     // Chimp128 decode scratch of the synchronous calls
 };

@@ -107,7 +107,7 @@ bool writes_bits(const Plan &p) {
 
 // Offsets of the device scratch of a plan (all 256-byte aligned).
 struct Layout {
-  uint64_t mask, bits, sel, cpred, acc, part, csum, block, values, out, total;
+  uint64_t mask, bits, sel, cpred, acc, part, csum, block, values, out, c128, total;
 };
 
 Layout layout_of(const Plan &p) {
@@ -131,6 +131,7 @@ Layout layout_of(const Plan &p) {
   L.block = take(nb * 8);
   L.values = p.decode && p.q.op != kOpDecode ? take(N * 8) : ~0ull;  // DECODE writes the caller's buffer
   L.out = (p.q.op == kOpFilterMaterialize || (p.q.op == kOpMaterialize && p.has_mask)) ? take(N * 8) : ~0ull;
+  L.c128 = p.decode && p.c->chimp128 ? take(c128_scratch_need(*p.c)) : ~0ull;  // Chimp128 decode scratch
   L.total = off;
   return L;
 }
@@ -326,6 +327,7 @@ void plan_bind(Plan &p, uint8_t *dev, uint8_t *pin) {
   p.d_n = p.block + launch::compact_blocks(p.words);
   p.values = p.q.op == kOpDecode ? p.q.out_values : at<double>(dev, L.values);
   p.out = at<double>(dev, L.out);
+  p.c128 = at<uint8_t>(dev, L.c128);
   const PinLayout P = pin_layout_of(p);
   p.h_mask = at<uint64_t>(pin, P.mask);
   p.h_bits = at<uint64_t>(pin, P.bits);
@@ -469,7 +471,7 @@ void group_bind(PlanGroup &g, uint8_t *dev, uint8_t *pin) {
   for (size_t i = 0; i < n; ++i) g.plans[i]->d_view = reinterpret_cast<const ColView *>(dev) + i;
 }
 
-hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blocks_per_cu, uint32_t c128_block) {
+hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blocks_per_cu) {
   const size_t n = g.plans.size();
   if (!n) return hipSuccess;
   const uint64_t vbytes = round_up(n * sizeof(ColView), kAlign);
@@ -482,6 +484,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   for (size_t i = 0; i < n; ++i) {
     Plan &p = *g.plans[i];
     views[i] = view_of(*p.c, p.values);
+    if (p.c128) bind_c128_scratch(views[i], *p.c, p.c128);
     pre[i] = (uint32_t)jobs;
     pre128[i] = (uint32_t)jobs128;
     preseg[i] = (uint32_t)segs;
@@ -504,7 +507,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   const uint32_t *dpre = reinterpret_cast<const uint32_t *>(g.dev + vbytes);
   launch::decode(s, dv, (uint32_t)n, dpre, (uint32_t)jobs, dpre + 2 * (pbytes / 4), (uint32_t)segs, g.d_flags,
                  (uint32_t)cu_count * blocks_per_cu);
-  launch::chimp128_par(s, dv, dpre + (pbytes / 4), (uint32_t)n, (uint32_t)jobs128, g.d_flags, c128_block);
+  launch::chimp128_resolve(s, dv, dpre + (pbytes / 4), (uint32_t)n, (uint32_t)jobs128, g.d_flags);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   for (Plan *p : g.plans)
     if ((e = plan_enqueue_post(*p, s)) != hipSuccess) return e;

@@ -63,6 +63,7 @@ struct Plan {
   uint32_t *sel = nullptr;
   ChunkPred *d_cpred = nullptr;
   double *partials = nullptr, *chunk_sum = nullptr, *values = nullptr, *out = nullptr;
+  uint8_t *c128 = nullptr;  // Chimp128 decode scratch (decode plans of Chimp128 columns)
   // pinned staging
   uint64_t *h_mask = nullptr, *h_bits = nullptr, *h_acc = nullptr, *h_n = nullptr;
   uint32_t *h_sel = nullptr;
@@ -96,9 +97,8 @@ struct PlanGroup {
 };
 uint64_t group_block_bytes(size_t n_plans);
 void group_bind(PlanGroup &g, uint8_t *dev, uint8_t *pin);
-// Issues the whole group; the persistent decode grid is cu_count *
-// blocks_per_cu blocks, c128_block the Chimp128 kernel's LDS block.
-hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blocks_per_cu = 8,
-                         uint32_t c128_block = 2048);
+// Issues the whole group; the persistent decode grids are cu_count *
+// blocks_per_cu blocks.
+hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blocks_per_cu = 8);
 
 }  // namespace ebi

@@ -45,7 +45,7 @@ int run_query(ebi_hip_ctx *ctx, const ebi_query &q) {
   plan_bind(p, dev + gb, pin + gb);
   plan_stage(p);
   hipEventRecord(ctx->ev0, ctx->stream);
-  hipError_t e = group_enqueue(g, ctx->stream, ctx->cu_count, ctx->blocks_per_cu, ctx->c128_block);
+  hipError_t e = group_enqueue(g, ctx->stream, ctx->cu_count, ctx->blocks_per_cu);
   if (e != hipSuccess) return hip_fail(ctx, e, "query launch");
   hipEventRecord(ctx->ev1, ctx->stream);
   if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "query");

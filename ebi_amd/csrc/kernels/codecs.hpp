@@ -489,59 +489,49 @@ __device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_bits, uint
   }
   return kErrNone;
 }
-// Chimp128 record i as (source record, xor payload): value_i = value_src ^ x.
-// Flags 10 / 11 use the previous record; flags 00 / 01 name ring slot idx,
-// i.e. the latest record j <= i - 1 with j == idx (mod 128); j < 0 is an
-// unwritten slot (value 0).  Record 0 is the root (the first 64 bits).
-constexpr int32_t kRoot = INT32_MIN;
-
-// LDS slot of block record k in the Chimp128 staging arrays: one pad element
-// per 64 records, so lanes walking their own 64-record segments in lockstep
-// hit distinct banks.
-__device__ __forceinline__ uint32_t c128_slot(uint32_t k) { return k + (k >> 6); }
-
-// Parses records [i0, i1) from a checkpoint; writes src / x / is00 bits to
-// the wave's LDS block arrays at offset (i - base).  Sources inside the
-// lane's own segment are resolved on the way, so every stored source lies
-// before i0 (an earlier segment, the previous block, or the root).
-__device__ __forceinline__ uint32_t chimp128_parse_seg(const uint8_t *data, uint32_t size_bits, uint32_t i0, uint32_t i1,
-                                        uint32_t bitpos, uint32_t lead, uint32_t base, uint8_t *S, uint64_t *X,
-                                        uint32_t *F) {
-  BitWindow r;
-  r.init(data, bitpos);
+// Chimp128 record i as (source distance, XOR payload): value_i =
+// value_{i - dist} ^ x.  Flags 10 / 11 use the previous record (dist 1);
+// flags 00 / 01 name ring slot idx, i.e. the latest record j <= i - 1 with
+// j == idx (mod 128), so dist = i - j is in [1, 128].  dist 0: no source --
+// record 0 (x = its 64 raw bits) or an unwritten ring slot (value 0) -- so
+// value_i = x.
+//
+// Parses records [i0, i1) from window r (stored lead `lead`): record i < n
+// goes to out.put(i, x) and byte i % 16 of dist16 (a run of at most 16
+// records that starts at a multiple of 16); the terminator record n goes to
+// *term.  No values are formed here (that needs the ring): the resolve pass
+// does it.
+template <class Sink>
+__device__ __forceinline__ uint32_t chimp128_parse_run(BitWindow &r, uint32_t &lead, uint32_t size_bits, uint32_t n,
+                                                       uint32_t i0, uint32_t i1, const Sink &out,
+                                                       uint32_t (&dist16)[4], C128Term *term) {
   for (uint32_t i = i0; i < i1; ++i) {
-    int32_t src;
     uint64_t x;
-    bool is00 = false;
+    uint32_t dist;
     if (i == 0) {
       x = r.field(0, 64);
-      src = kRoot;
+      dist = 0;
       r.consume(64);
     } else {
       uint32_t f, idx, used;
       if (!chimp128_rec(r, lead, f, idx, x, used)) return kErrCorrupt;
-      // flags 00 / 01 name a ring slot, 10 / 11 the previous value
-      const int32_t ring = (int32_t)(i - 1) - (int32_t)(((i - 1) - idx) & 127u);
-      src = (f == 0 || f == 1) ? ring : (int32_t)i - 1;
-      is00 = f == 0;
+      const uint32_t back = f <= 1 ? (((i - 1) - idx) & 127u) + 1 : 1;  // i - source
+      dist = back > i ? 0 : back;
       r.consume(used);
     }
     r.refill();
     if (r.pos > size_bits) return kErrOverrun;
-    const uint32_t k = i - base;
-    if (src >= (int32_t)i0) {
-      // source in this lane's own segment: already resolved to a source
-      // outside it, so fold it in now (value_i = value_ext ^ x_src ^ x)
-      const uint32_t sq = (uint32_t)src - base;
-      const uint32_t q = c128_slot(sq);
-      x ^= X[q];
-      src = S[q] ? (int32_t)(base + sq) - (int32_t)S[q] : kRoot;
+    if (i < n) {
+      out.put(i, as_f64(x));
+      const uint32_t k = i & 15, sh = (k & 3) * 8, w = k >> 2;
+      const uint32_t keep = ~(0xffu << sh), put = dist << sh;
+      dist16[0] = w == 0 ? (dist16[0] & keep) | put : dist16[0];
+      dist16[1] = w == 1 ? (dist16[1] & keep) | put : dist16[1];
+      dist16[2] = w == 2 ? (dist16[2] & keep) | put : dist16[2];
+      dist16[3] = w == 3 ? (dist16[3] & keep) | put : dist16[3];
+    } else {
+      *term = C128Term{x, dist, 0};
     }
-    // Every source is now the root or in [i0 - 128, i0) (induction over the
-    // segment), so i - src is in [1, 191]: one byte, 0 for the root.
-    S[c128_slot(k)] = src == kRoot ? 0 : (uint8_t)((int32_t)i - src);
-    X[c128_slot(k)] = x;
-    if (is00) atomicOr(F + (k >> 5), 1u << (k & 31));
   }
   return kErrNone;
 }

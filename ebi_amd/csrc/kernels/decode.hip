@@ -55,12 +55,50 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
   return e;
 }
 
+// Chimp128 parse of checkpoint segments first .. first + 63 of chunk ci
+// (records 0..n, n = the terminator), segment first + l on lane l, in runs
+// of kRun records: the XOR payloads go through the output stage into the
+// records' output slots, the source distances (16 bytes a lane a run)
+// straight to the scratch, the terminator's parse to its slot.
+// k_chimp128_resolve then forms the values.
+__device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_t ci, const DevChunk &c, uint32_t first,
+                                                      double *stage) {
+  const uint32_t total = c.n + 1;
+  const uint32_t seg = first + lane_id();
+  const uint32_t i0 = seg * kSeg;
+  const bool live = i0 < total;
+  const uint32_t i1 = live ? min(i0 + kSeg, total) : 0;
+  BitWindow r;
+  uint32_t lead = 0xffffffffu;
+  if (live) {
+    const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[seg];
+    r.init(reinterpret_cast<const uint8_t *>(c.data), k.bitpos);
+    lead = k.a == 255 ? 0xffffffffu : k.a;
+  }
+  const StageSink sink{stage};
+  double *dst = cv.out + c.rec_off + (size_t)first * kSeg;
+  const uint32_t n_job = first * kSeg < c.n ? c.n - first * kSeg : 0;
+  uint8_t *dist = cv.c128_dist + (size_t)c.rtile0 * kTile;  // 16-byte aligned, ceil(n / kTile) * kTile bytes
+  const uint32_t bits = c.size * 8u;
+  uint32_t e = 0;
+  for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
+    const uint32_t a = i0 + k0, b = min(a + kRun, i1);
+    uint32_t d16[4] = {0, 0, 0, 0};
+    if (live && e == 0 && a < b) e = chimp128_parse_run(r, lead, bits, c.n, a, b, sink, d16, cv.c128_term + ci);
+    stage_flush(stage, dst, k0, n_job);
+    // a < n: the 16 bytes from a lie inside the chunk's distance region
+    if (live && a < c.n) *reinterpret_cast<uint4 *>(dist + a) = make_uint4(d16[0], d16[1], d16[2], d16[3]);
+  }
+  return e;
+}
+
 __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, uint32_t first, double *stage) {
   const DevChunk c = cv.chunks[ci];
   switch (c.codec) {
     case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, stage);
     case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, stage);
     case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, stage);
+    case kCodecChimp128: return segment_wave_c128(cv, ci, c, first, stage);
     default: return kErrCorrupt;
   }
 }
@@ -195,80 +233,76 @@ __global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const uint
   }
 }
 
-// Wave-parallel Chimp128 decode (decoder/chunk_reader/chimp_n.rs:173-285).
-// One 64-thread block per chunk walks blocks of kBlk records (LDS: 9 bytes a
-// record, so smaller blocks raise occupancy; the tuner picks kBlk):
-//  1. lane l parses checkpoint segment l of the block into (src, x) pairs,
-//     folding sources inside its own segment, so every source ends up in an
-//     earlier segment, the previous block, or the chunk root;
-//  2. the wave resolves segment 0, 1, 2, ... in order, lane l taking record
-//     l of the segment: value = value[src] ^ x, where value[src] is final
-//     (earlier segment of this block, or ctx = the previous block's last 128
-//     values; zeros before the chunk: unwritten ring slots), and stores it
-//     (64 consecutive records per store instruction).
-// Record n is parsed too and must be the NaN terminator via flags 01/10/11;
-// an earlier NaN from those flags is a count error, as the reference would
-// stop there.
-template <uint32_t kBlk>
-__global__ void __launch_bounds__(64) k_chimp128_par(const ColView *cols, const uint32_t *prefix, uint32_t n_cols,
-                                                    uint32_t *err) {
-  static_assert(kBlk % kSeg == 0 && kBlk / kSeg <= kWave && kBlk >= 128, "block of at most 64 segments");
-  __shared__ uint64_t X[kBlk + kBlk / 64];
-  __shared__ uint8_t S[kBlk + kBlk / 64];
-  __shared__ uint32_t F[kBlk / 32];
-  __shared__ uint64_t Cx[128];
-  const uint32_t col = find_col(prefix, n_cols, blockIdx.x);
+// Chimp128 resolve (decoder/chunk_reader/chimp_n.rs:173-285, the value
+// side), after k_decode_seg left each record's XOR payload in its output
+// slot and its source distance in the scratch: one wave per chunk walks
+// records 0..n in steps of 64 (lane l = record base + l).  Sources inside
+// the step are followed by pointer jumping over lane shuffles (x ^= x[src],
+// src = src[src] while the source lies in the step; sources only point
+// back, so this ends, after at most six rounds), then value = ring[src] ^ x
+// with the ring the last 256 values in LDS (a source is at most 128 records
+// back).  Loads and stores go in batches of kC128Batch steps, so a batch's
+// memory traffic is waited for once.
+// Record n must be the NaN end marker and an earlier NaN is a count error:
+// a NaN can only come from flags 01 / 10 / 11 (the ring never holds one,
+// as the reference stops at the first).
+constexpr uint32_t kC128Batch = 8;
+
+__global__ void __launch_bounds__(256) k_chimp128_resolve(const ColView *cols, const uint32_t *prefix,
+                                                         uint32_t n_cols, uint32_t n_chunks, uint32_t *err) {
+  __shared__ uint64_t ring[4][256];
+  const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
+  const uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
+  if (j >= n_chunks) return;
+  const uint32_t col = find_col(prefix, n_cols, j);
   const ColView cv = cols[col];
-  const uint32_t ci = blockIdx.x - prefix[col];
+  const uint32_t ci = j - prefix[col];
   if (cv.chunk_err && cv.chunk_err[ci]) return;
   const DevChunk c = cv.chunks[ci];
-  const XorCkpt *ck = reinterpret_cast<const XorCkpt *>(c.xidx);
-  const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
   double *out = cv.out + c.rec_off;
-  const uint32_t lane = threadIdx.x;
-  const uint32_t bits = c.size * 8u;
-  const uint32_t total = c.n + 1;  // records 0..n, n = terminator
+  const uint8_t *dist = cv.c128_dist + (size_t)c.rtile0 * kTile;
+  const C128Term tm = cv.c128_term[ci];
+  uint64_t *R = ring[wave];
+  const uint32_t n = c.n, total = n + 1;
   uint32_t e = 0;
-  for (uint32_t t = lane; t < 128; t += kWave) Cx[t] = 0;
-  for (uint32_t base = 0; base < total; base += kBlk) {
-    const uint32_t bn = min(kBlk, total - base);
-    for (uint32_t t = lane; t < kBlk / 32; t += kWave) F[t] = 0;
-    wave_sync();
-    if (lane < kBlk / kSeg) {
-      const uint32_t seg = base / kSeg + lane;
-      const uint32_t i0 = seg * kSeg;
-      if (i0 < base + bn) {
-        const uint32_t i1 = min(i0 + kSeg, base + bn);
-        const XorCkpt k = ck[seg];
-        e |= chimp128_parse_seg(data, bits, i0, i1, k.bitpos, k.a == 255 ? 0xffffffffu : k.a, base, S, X, F);
-      }
+  for (uint32_t b0 = 0; b0 < total; b0 += kC128Batch * kWave) {
+    uint64_t x[kC128Batch];
+    uint32_t d[kC128Batch];
+#pragma unroll
+    for (uint32_t g = 0; g < kC128Batch; ++g) {
+      const uint32_t i = b0 + g * kWave + lane;
+      x[g] = i < n ? as_u64(out[i]) : i == n ? tm.x : 0;
+      d[g] = i < n ? dist[i] : i == n ? tm.dist : 0;
     }
-    if (__any(e != 0)) break;
-    wave_sync();
-    const int32_t lo = (int32_t)base;
-    for (uint32_t kk = 0; kk < bn; kk += kWave) {
-      const uint32_t k = kk + lane;
-      if (k < bn) {
-        const uint32_t ks = c128_slot(k);
-        const uint32_t off = S[ks];
-        const int32_t sk = (int32_t)(base + k) - (int32_t)off;
-        const uint64_t src = off == 0 ? 0 : sk < lo ? Cx[sk - lo + 128] : X[c128_slot(sk - lo)];
-        const uint64_t v = src ^ X[ks];
-        X[ks] = v;
-        const uint32_t i = base + k;
-        const bool is00 = (F[k >> 5] >> (k & 31)) & 1;
-        if (i < c.n) {
-          out[i] = as_f64(v);
-          if (!is00 && v == kNan) e |= kErrCount;
-        } else if (is00 || v != kNan) {
-          e |= kErrCount;  // record n must be the terminator
-        }
+#pragma unroll
+    for (uint32_t g = 0; g < kC128Batch; ++g) {
+      const int32_t base = (int32_t)(b0 + g * kWave);
+      const uint32_t i = (uint32_t)base + lane;
+      int32_t p = d[g] ? (int32_t)(i - d[g]) : -1;  // source record, -1: none (value 0)
+      uint64_t v = x[g];
+      while (__any(p >= base)) {
+        const int32_t q = (p - base) & (kWave - 1);  // source lane (used only when in the step)
+        const uint64_t vq = __shfl(v, q);
+        const int32_t pq = __shfl(p, q);
+        const bool in = p >= base;
+        v = in ? v ^ vq : v;
+        p = in ? pq : p;
       }
-      wave_sync();  // segment kk / 64 is final before the next one reads it
+      v ^= p < 0 ? 0 : R[(uint32_t)p & 255];
+      if (i < total) R[i & 255] = v;
+      wave_sync();  // this step's values are in the ring before the next step reads them
+      if (i < n) {
+        if (v == kNan) e |= kErrCount;
+      } else if (i == n && v != kNan) {
+        e |= kErrCount;
+      }
+      x[g] = v;
     }
-    if (bn >= 128)
-      for (uint32_t t = lane; t < 128; t += kWave) Cx[t] = X[c128_slot(bn - 128 + t)];
-    wave_sync();
+#pragma unroll
+    for (uint32_t g = 0; g < kC128Batch; ++g) {
+      const uint32_t i = b0 + g * kWave + lane;
+      if (i < n) st_one(out + i, as_f64(x[g]));
+    }
   }
   if (e) atomicOr(err, e);
 }
@@ -361,23 +395,11 @@ void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
     hipLaunchKernelGGL((k_serial<false, true>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
 }
 
-void chimp128_par(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
-                  uint32_t *err, uint32_t block) {
+void chimp128_resolve(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
+                      uint32_t *err) {
   if (!n_chunks) return;
-  switch (block) {
-    case 512:
-      hipLaunchKernelGGL(k_chimp128_par<512>, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
-      break;
-    case 1024:
-      hipLaunchKernelGGL(k_chimp128_par<1024>, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
-      break;
-    case 4096:
-      hipLaunchKernelGGL(k_chimp128_par<4096>, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
-      break;
-    default:
-      hipLaunchKernelGGL(k_chimp128_par<2048>, dim3(n_chunks), dim3(64), 0, s, cols, prefix, n_cols, err);
-      break;
-  }
+  hipLaunchKernelGGL(k_chimp128_resolve, dim3((n_chunks + 3) / 4), dim3(256), 0, s, cols, prefix, n_cols, n_chunks,
+                     err);
 }
 
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,

@@ -26,9 +26,10 @@ void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t 
 // runs one explicit job on cols[0] instead.
 void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
             const Job *single, uint32_t *err, bool write, bool index);
-// Wave-parallel Chimp128 decode, one block per chunk; prefix over chunks.
-void chimp128_par(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
-                  uint32_t *err, uint32_t block = 2048);
+// Chimp128 value resolve, one wave per chunk (prefix over chunks), after
+// decode() parsed the chunks' segments into the views' Chimp128 scratch.
+void chimp128_resolve(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
+                      uint32_t *err);
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
               const Job *single, uint32_t *err, bool write);
 // Load-time tile index for DeltaSprintz and RLE chunks of column slot `col`.

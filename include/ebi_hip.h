@@ -214,16 +214,17 @@ int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
 
 /* Prepared batch: the same decode for a fixed set of columns and outputs,
  * replayed many times.  The job-space arguments are uploaded once; with
- * autotune != 0 the launch shape (persistent grid size, Chimp128 LDS block)
- * is timed on this batch and the fastest kept; the launch sequence is
- * captured in a hipGraph.  Columns and
- * outputs must outlive the batch.  No counterpart in the reference. */
+ * autotune != 0 the persistent grid size is timed on this batch and the
+ * fastest kept; the launch sequence is captured in a hipGraph.  The batch
+ * holds the Chimp128 decode scratch of its columns (1 byte per record and
+ * 16 per chunk).  Columns and outputs must outlive the batch.  No
+ * counterpart in the reference. */
 typedef struct ebi_hip_batch ebi_hip_batch;
 typedef struct ebi_batch_config {
-    uint32_t blocks_per_cu; /* persistent k_decode grid = CUs x blocks_per_cu */
-    uint32_t c128_block;    /* records per LDS block of the Chimp128 kernel (512..4096) */
-    uint32_t overlap;       /* 1: Chimp128 kernel on a second stream beside k_decode */
-    uint32_t reserved;
+    uint32_t blocks_per_cu; /* persistent decode grids = CUs x blocks_per_cu */
+    uint32_t reserved0;     /* 0 */
+    uint32_t reserved1;     /* 0 */
+    uint32_t reserved2;     /* 0 */
     double tuned_ms;        /* device ms of the chosen shape (0 if not tuned) */
 } ebi_batch_config;
 int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols, size_t n_cols,
@@ -328,8 +329,9 @@ double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx);
 /* Device time of each phase of the last decode or matmul call on this ctx:
  * for ebi_hip_decode_device / ebi_hip_decode_batch phase 0 = the persistent
  * tile kernel (Uncompressed, RLE, BUFF, DeltaSprintz), phase 2 = the
- * persistent segment kernel (Gorilla, Chimp, Elf), phase 1 = the Chimp128
- * kernel; for the matmul entry points the phases listed there.
+ * persistent segment kernel (Gorilla, Chimp, Elf, and the Chimp128 parse),
+ * phase 1 = the Chimp128 resolve kernel; for the matmul entry points the
+ * phases listed there.
  * Instrumentation of this library; no counterpart in the reference. */
 double ebi_hip_last_phase_ms(const ebi_hip_ctx *ctx, int phase);
 

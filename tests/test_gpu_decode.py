@@ -102,7 +102,7 @@ def test_prepared_batch_graph_replays_match_eager(ctx):
         outs.append(ctx.alloc(8 * n))
     b = E.Batch(ctx, decs, outs, autotune=True)
     cfg = b.config
-    assert cfg["blocks_per_cu"] in (4, 8, 16) and cfg["c128_block"] in (512, 1024, 2048, 4096) and cfg["tuned_ms"] > 0
+    assert cfg["blocks_per_cu"] in (4, 8, 16) and cfg["tuned_ms"] > 0
     eager_out = ctx.alloc(8 * max(s[2] for s in specs))
     for rep in range(3):
         for o in outs:  # cleared between replays: every replay must write every record
