@@ -74,7 +74,9 @@ def build_native_tests(cc: str | None = None) -> list[Path]:
     out = []
     for src in sorted(NATIVE_TESTS.glob("*.hip")):
         exe = BUILD / src.stem
-        cmd = [cc, *COMMON, *DEVICE, "-I", str(PKG.parent / "include"), str(src), "-o", str(exe)]
+        # linked against the library (its host encoder makes the test inputs)
+        cmd = [cc, *COMMON, *DEVICE, "-I", str(PKG.parent / "include"), str(src), "-o", str(exe),
+               "-L", str(LIB.parent), "-lebi_hip", "-Wl,-rpath,$ORIGIN/.."]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"compile failed: {src}\n{r.stdout}{r.stderr}")

@@ -62,10 +62,10 @@ __host__ __device__ __forceinline__ double p10_exact(uint32_t a) {
 }
 
 // round_up, compression_common/elf.rs:51-58 (no FMA contraction).
-__device__ __forceinline__ double elf_round_up(double v, uint32_t alpha) {
+__host__ __device__ __forceinline__ double elf_round_up(double v, uint32_t alpha) {
   const double scale = alpha <= 22 ? p10_exact(alpha)
                        : alpha < 400 ? c_p10[alpha]
-                                     : __longlong_as_double(0x7ff0000000000000ll);
+                                     : as_f64(0x7ff0000000000000ull);
   const double t = v * scale;
   return (v < 0.0 ? floor(t) : ceil(t)) / scale;
 }
@@ -83,7 +83,7 @@ struct XorState {
 };
 
 // State at a checkpoint written by the index pass (255 / -1 encode "unset").
-__device__ __forceinline__ void xor_state_at(XorState &s, uint32_t codec, const uint8_t *data, const XorCkpt &k) {
+__host__ __device__ __forceinline__ void xor_state_at(XorState &s, uint32_t codec, const uint8_t *data, const XorCkpt &k) {
   s.r.init(data, k.bitpos);
   s.v = k.value;
   if (codec == kCodecGorilla) {
@@ -97,7 +97,7 @@ __device__ __forceinline__ void xor_state_at(XorState &s, uint32_t codec, const 
 }
 
 // State before record 0 of a chunk.
-__device__ __forceinline__ void xor_state_start(XorState &s, uint32_t codec, const uint8_t *data) {
+__host__ __device__ __forceinline__ void xor_state_start(XorState &s, uint32_t codec, const uint8_t *data) {
   s.r.init(data, 0);
   s.v = 0;
   s.a = codec == kCodecGorilla ? 0u : 0xffffffffu;
@@ -108,7 +108,7 @@ __device__ __forceinline__ void xor_state_start(XorState &s, uint32_t codec, con
 // Record sinks of the XOR decoders: put(i, v) receives record i of the chunk.
 struct DirectSink {  // one lane per chunk: records stored as they come
   double *out;
-  __device__ __forceinline__ void put(uint32_t i, double v) const { out[i] = v; }
+  __host__ __device__ __forceinline__ void put(uint32_t i, double v) const { out[i] = v; }
 };
 
 // ------------------------------------------------------------------ Gorilla
@@ -116,7 +116,7 @@ struct DirectSink {  // one lane per chunk: records stored as they come
 // state s (v = record i0-1, L, T); when i1 == n also checks that the next
 // record is the NaN terminator.  ck (index pass): checkpoints every kSeg.
 template <bool kWrite, class Sink>
-__device__ __forceinline__ uint32_t gorilla_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+__host__ __device__ __forceinline__ uint32_t gorilla_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
                                                 uint32_t i1, const Sink &out, XorCkpt *ck) {
   BitWindow &r = s.r;
   uint64_t v = s.v;
@@ -167,7 +167,7 @@ __device__ __forceinline__ uint32_t gorilla_run(XorState &s, uint32_t size_bits,
 // decoder/chunk_reader/chimp.rs:59-125.  State: v, stored lead (0xffffffff
 // before the first 11/01 record).
 template <bool kWrite, class Sink>
-__device__ __forceinline__ uint32_t chimp_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+__host__ __device__ __forceinline__ uint32_t chimp_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
                                               uint32_t i1, const Sink &out, XorCkpt *ck) {
   BitWindow &r = s.r;
   uint64_t v = s.v;
@@ -222,7 +222,7 @@ __device__ __forceinline__ uint32_t chimp_run(XorState &s, uint32_t size_bits, u
 // State: stored v' bits, L, T (0xffffffff before the first 10/11 record),
 // last beta* (INT32_MAX before the first 11 prefix).
 template <bool kWrite, class Sink>
-__device__ __forceinline__ uint32_t elf_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+__host__ __device__ __forceinline__ uint32_t elf_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
                                             uint32_t i1, const Sink &out, XorCkpt *ck) {
   BitWindow &r = s.r;
   uint64_t stored = s.v;
@@ -316,7 +316,7 @@ __device__ __forceinline__ uint32_t elf_run(XorState &s, uint32_t size_bits, uin
 
 // Dispatch on the codec (wave-uniform).
 template <bool kWrite, class Sink>
-__device__ __forceinline__ uint32_t xor_run(uint32_t codec, XorState &s, uint32_t size_bits, uint32_t n,
+__host__ __device__ __forceinline__ uint32_t xor_run(uint32_t codec, XorState &s, uint32_t size_bits, uint32_t n,
                                             uint32_t i0, uint32_t i1, const Sink &out, XorCkpt *ck) {
   switch (codec) {
     case kCodecGorilla: return gorilla_run<kWrite>(s, size_bits, n, i0, i1, out, ck);
@@ -439,7 +439,7 @@ __device__ uint32_t chimp128_run(const uint8_t *data, uint32_t size_bits, uint32
 // lead; 01 = 16-bit header (index, lead code, sig), sig bits.  Sets the
 // flag, ring index, XOR payload (aligned) and bits used; false on a header
 // the reference would reject.
-__device__ __forceinline__ bool chimp128_rec(const BitWindow &r, uint32_t &lead, uint32_t &f, uint32_t &idx,
+__host__ __device__ __forceinline__ bool chimp128_rec(const BitWindow &r, uint32_t &lead, uint32_t &f, uint32_t &idx,
                                              uint64_t &x, uint32_t &used) {
   f = (uint32_t)(r.hi >> 62);
   const uint32_t h = (uint32_t)(r.hi >> 46) & 0xffff;  // bits 2..17
@@ -471,7 +471,7 @@ __device__ __forceinline__ bool chimp128_rec(const BitWindow &r, uint32_t &lead,
   return true;
 }
 
-__device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_bits, uint32_t n, XorCkpt *ck) {
+__host__ __device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_bits, uint32_t n, XorCkpt *ck) {
   BitWindow r;
   r.init(data, 0);
   uint32_t lead = 0xffffffffu;
@@ -502,7 +502,7 @@ __device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_bits, uint
 // *term.  No values are formed here (that needs the ring): the resolve pass
 // does it.
 template <class Sink>
-__device__ __forceinline__ uint32_t chimp128_parse_run(BitWindow &r, uint32_t &lead, uint32_t size_bits, uint32_t n,
+__host__ __device__ __forceinline__ uint32_t chimp128_parse_run(BitWindow &r, uint32_t &lead, uint32_t size_bits, uint32_t n,
                                                        uint32_t i0, uint32_t i1, const Sink &out,
                                                        uint32_t (&dist16)[4], C128Term *term) {
   for (uint32_t i = i0; i < i1; ++i) {

@@ -12,22 +12,37 @@ namespace ebi {
 constexpr uint64_t kNan = 0x7ff8000000000000ull;
 constexpr int kWave = 64;
 
-__device__ __forceinline__ double as_f64(uint64_t u) { return __longlong_as_double((long long)u); }
-__device__ __forceinline__ uint64_t as_u64(double d) { return (uint64_t)__double_as_longlong(d); }
+__host__ __device__ __forceinline__ double as_f64(uint64_t u) { return __builtin_bit_cast(double, u); }
+__host__ __device__ __forceinline__ uint64_t as_u64(double d) { return __builtin_bit_cast(uint64_t, d); }
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Global-memory pointers for hot loops: pointers that arrive through
+// descriptors are generic, which makes the compiler emit FLAT accesses (each
+// waited for on both the memory and the LDS counter); casting them to the
+// global address space gives global loads and stores.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define EBI_GLOBAL __attribute__((address_space(1)))
+#else
+#define EBI_GLOBAL
+#endif
+template <class T>
+__host__ __device__ __forceinline__ EBI_GLOBAL T *as_global(T *p) {
+  return (EBI_GLOBAL T *)p;
+}
 
 // Decoded records are written once and not read back by the kernel: the
 // tile decoders, whose every store instruction covers whole cache lines,
 // stream them past the caches (non-temporal stores).  Partial-line writers
 // (the XOR lane sinks, Chimp128) use ordinary stores, which L2 combines.
 typedef double v2f64 __attribute__((ext_vector_type(2)));
+// p: decoded-record output in HBM (stored through the global address space)
 __device__ __forceinline__ void st_pair(double *p, double a, double b) {
-  __builtin_nontemporal_store(v2f64{a, b}, reinterpret_cast<v2f64 *>(p));
+  __builtin_nontemporal_store(v2f64{a, b}, reinterpret_cast<EBI_GLOBAL v2f64 *>(as_global(p)));
 }
-__device__ __forceinline__ void st_one(double *p, double a) { __builtin_nontemporal_store(a, p); }
+__device__ __forceinline__ void st_one(double *p, double a) { __builtin_nontemporal_store(a, as_global(p)); }
 
 // Chimp / Chimp128 / ElfXor 3-bit leading-zero codes (chunk_reader/chimp.rs:40).
-__device__ __forceinline__ uint32_t lead_repr(uint32_t code) {
+__host__ __device__ __forceinline__ uint32_t lead_repr(uint32_t code) {
   // 0, 8, 12, 16, 18, 20, 22, 24 packed as bytes.
   const uint64_t t = 0x18161412100C0800ull;
   return (uint32_t)(t >> (8 * code)) & 0xff;
@@ -199,77 +214,69 @@ struct PrefetchReader {
   }
 };
 // 128-bit window over an MSB-first stream for branch-free field extraction:
-// bit 0 of the window is the MSB of `hi`.  refill() tops the window up to at
-// least 96 valid bits with 32-bit words from 16-byte block loads (one block
-// requested ahead), so a record of up to 96 bits decodes from registers with
-// shifts and selects, without per-field refills.  Reads at most 48 bytes past
-// the current position.
+// bit 0 of the window is the MSB of `hi`; (hi, lo) always hold the 128
+// stream bits from `pos` on.  The stream is held as the current and the next
+// 16-byte block (big-endian 64-bit halves c0 c1 | n0 n1) and the bit offset
+// b of pos in the current block; refill() rebuilds the window from them with
+// two funnel shifts, and once b passes the current block the next becomes
+// current and the block after it -- requested one block earlier -- is
+// byte-swapped in while the following one is requested.  So a record of up
+// to 128 bits decodes from registers with shifts and selects, and a lane
+// waits on memory only for a load issued a whole block earlier.  Reads at
+// most 48 bytes past the current position.
 struct BitWindow {
-  const uint4 *p;
-  uint64_t c0, c1;  // current 16-byte block as two big-endian halves
-  uint4 nxt;
-  uint32_t wi;      // next word of the current block (0 .. 4)
+  const EBI_GLOBAL uint4 *p;  // next block to request
+  uint64_t c0, c1, n0, n1; // current and next block
+  uint4 nxt;               // requested block (after the next)
+  uint32_t b;              // bit offset of pos in the current block, [0, 128)
   uint64_t hi, lo;
-  uint32_t nb;      // valid bits
-  uint32_t pos;     // stream bit position of the window's first bit
+  uint32_t pos;            // stream bit position of the window's first bit
+  static constexpr uint32_t nb = 128;  // valid bits of the window
 
-  // Every step below is branch-free (selects and shifts with counts kept in
-  // [0, 63]) except the block reload once per four words.  The reload
-  // byte-swaps the prefetched block into c0, c1 (a computation, not a copy,
-  // so the prefetch load can reuse nxt's registers and is waited for only at
-  // the next reload).
-  __host__ __device__ __forceinline__ void load_next() {
-    c0 = __builtin_bswap64((uint64_t)nxt.x | ((uint64_t)nxt.y << 32));
-    c1 = __builtin_bswap64((uint64_t)nxt.z | ((uint64_t)nxt.w << 32));
+  __host__ __device__ static __forceinline__ uint64_t be64(uint32_t x, uint32_t y) {
+    return __builtin_bswap64((uint64_t)x | ((uint64_t)y << 32));
+  }
+  // bits s .. s + 63 of the 128-bit value (x, y), s in [0, 63]
+  __host__ __device__ static __forceinline__ uint64_t funnel(uint64_t x, uint64_t y, uint32_t s) {
+    return s ? (x << s) | (y >> ((64 - s) & 63)) : x;
+  }
+  __host__ __device__ __forceinline__ void advance() {
+    c0 = n0;
+    c1 = n1;
+    n0 = be64(nxt.x, nxt.y);
+    n1 = be64(nxt.z, nxt.w);
     nxt = *p++;
-    wi = 0;
+    b -= 128;
   }
-  // append the next stream word when `need` (window bit nb, nb < 96)
-  __host__ __device__ __forceinline__ void push_if(bool need) {
-    const uint64_t half = (wi & 2) ? c1 : c0;
-    const uint64_t w = (uint32_t)(half >> ((~wi & 1) * 32));
-    const uint64_t top = w << 32;  // the word in the top 32 bits
-    const uint64_t h = nb < 64 ? top >> (nb & 63) : 0;             // bits landing in hi
-    const uint64_t l = nb > 32 ? w << ((96 - nb) & 63) : 0;        // bits landing in lo
-    if (need) {
-      hi |= h;
-      lo |= l;
-      nb += 32;
-      ++wi;
-      if (wi == 4) load_next();
-    }
-  }
-  __host__ __device__ __forceinline__ void refill() {  // to >= 96 bits from >= 14
-    push_if(nb < 96);
-    push_if(nb < 96);
-    push_if(nb < 96);
+  // after consume(): the window at the new position
+  __host__ __device__ __forceinline__ void refill() {
+    if (b >= 128) advance();
+    const bool first = b < 64;
+    const uint32_t s = b & 63;
+    const uint64_t x0 = first ? c0 : c1, x1 = first ? c1 : n0, x2 = first ? n0 : n1;
+    hi = funnel(x0, x1, s);
+    lo = funnel(x1, x2, s);
   }
   __host__ __device__ __forceinline__ void init(const uint8_t *base, uint32_t bitpos) {
-    const uint32_t w = bitpos >> 5;
-    p = reinterpret_cast<const uint4 *>(base) + (w >> 2);
-    nxt = *p++;
-    load_next();
-    wi = w & 3;
-    hi = lo = 0;
-    nb = 0;
-    pos = bitpos & ~31u;
-    refill();
-    refill();
-    consume(bitpos & 31);
+    p = as_global(reinterpret_cast<const uint4 *>(base)) + (bitpos >> 7);
+    const uint4 c = p[0], n = p[1];
+    nxt = p[2];
+    p += 3;
+    c0 = be64(c.x, c.y);
+    c1 = be64(c.z, c.w);
+    n0 = be64(n.x, n.y);
+    n1 = be64(n.z, n.w);
+    b = bitpos & 127;
+    pos = bitpos;
     refill();
   }
-  // the `len` bits at window offset o (o < 64, 1 <= len <= 64)
+  // the `len` bits at window offset o (o < 64, 1 <= len <= 64, o + len <= 128)
   __host__ __device__ __forceinline__ uint64_t field(uint32_t o, uint32_t len) const {
-    const uint64_t top = o ? (hi << (o & 63)) | (lo >> ((64 - o) & 63)) : hi;
-    return top >> ((64 - len) & 63);
+    return funnel(hi, lo, o) >> ((64 - len) & 63);
   }
-  // drop k bits (k <= 96, k <= nb)
+  // drop k bits (k <= 128); refill() before the next field
   __host__ __device__ __forceinline__ void consume(uint32_t k) {
-    const uint64_t h_small = k ? (hi << (k & 63)) | (lo >> ((64 - k) & 63)) : hi;
-    const uint64_t h_big = lo << ((k - 64) & 63);
-    hi = k >= 64 ? h_big : h_small;
-    lo = k >= 64 ? 0 : lo << (k & 63);
-    nb -= k;
+    b += k;
     pos += k;
   }
 };

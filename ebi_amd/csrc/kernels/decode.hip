@@ -47,6 +47,7 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
   const uint32_t n_job = first * kSeg < c.n ? c.n - first * kSeg : 0;  // records from segment `first` on
   const uint32_t bits = c.size * 8u;
   uint32_t e = 0;
+#pragma unroll 1  // one copy of the codec loop per codec (instruction cache)
   for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
     const uint32_t a = i0 + k0, b = min(a + kRun, i1);
     if (live && e == 0 && a < b) e = xor_run<true>(kCodec, s, bits, c.n, a, b, sink, nullptr);
@@ -81,6 +82,7 @@ __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_
   uint8_t *dist = cv.c128_dist + (size_t)c.rtile0 * kTile;  // 16-byte aligned, ceil(n / kTile) * kTile bytes
   const uint32_t bits = c.size * 8u;
   uint32_t e = 0;
+#pragma unroll 1
   for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
     const uint32_t a = i0 + k0, b = min(a + kRun, i1);
     uint32_t d16[4] = {0, 0, 0, 0};
@@ -146,11 +148,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 }
 
 // Persistent decode over the checkpoint-segment jobs (64 segments of a
-// Gorilla / Chimp / Elf chunk per job) of n_cols columns, with the same
-// static interleave as k_decode.  A kernel of its own, so that the serial
-// bit-stream loops get their own register allocation and LDS (one [16][64]
-// output stage per wave).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_decode_seg(
+// Gorilla / Chimp / Elf / Chimp128 chunk per job) of n_cols columns, with
+// the same static interleave as k_decode.  A kernel of its own, so that the
+// serial bit-stream loops get their own register allocation and LDS (one
+// [16][64] output stage per wave).  amdgpu_waves_per_eu(4): the register
+// budget at which the kernel needs no scratch; built to fit 5 waves per SIMD
+// it spilled, and then decoded Elf wrongly in workgroups that shared a CU
+// with an earlier one (the decode kernels are kept spill-free, which a test
+// checks).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_decode_seg(
     const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
   __shared__ __align__(16) double lds[4][kRun * kWave];
   const uint32_t wave = threadIdx.x / kWave;

@@ -49,6 +49,23 @@ def test_decode_ragged_chunks(ctx, codec, chunk):
     _check(ctx, vals, codec, chunk)
 
 
+@pytest.mark.parametrize("blocks_per_cu", [1, 2, 4, 8])
+def test_decode_grid_shapes(blocks_per_cu, monkeypatch):
+    """Every persistent grid size gives the same records, with small chunks so
+    that the segment kernel's jobs spread over workgroups that share CUs (a
+    spilling build of that kernel decoded Elf wrongly in every workgroup but
+    the first on a CU)."""
+    monkeypatch.setenv("EBI_HIP_BLOCKS_PER_CU", str(blocks_per_cu))  # read when a context is created
+    c = E.Context(0)
+    try:
+        vals = G.random_walk(40000, seed=7)
+        for codec in ("elf", "gorilla", "chimp", "chimp128"):
+            for chunk in (3, 5, 64):
+                _check(c, vals, codec, E.ChunkOption.record_count(chunk))
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("codec", [c for c in CODECS if c not in ("buff", "delta_sprintz")])
 def test_decode_xor_adversarial(ctx, codec):
     _check(ctx, G.xor_values(), codec, E.ChunkOption.record_count(66))
