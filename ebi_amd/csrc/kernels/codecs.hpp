@@ -542,7 +542,7 @@ __host__ __device__ __forceinline__ uint32_t chimp128_parse_run(BitWindow &r, ui
 // Uncompressed, chunk_reader/uncompressed.rs:56-71: a copy; lanes move
 // 16-byte pairs so each wave instruction covers 1 KiB contiguous.
 __device__ __forceinline__ void plain_tile(const DevChunk &c, uint32_t t, double *out) {
-  const double *src = reinterpret_cast<const double *>(c.data);
+  const EBI_GLOBAL double *src = as_global(reinterpret_cast<const double *>(c.data));
   const uint32_t lane = lane_id();
   const uint32_t b = t * kTile;
   const bool pair_ok = ((c.rec_off + (uint64_t)reinterpret_cast<uintptr_t>(out) / 8) & 1) == 0;
@@ -550,7 +550,7 @@ __device__ __forceinline__ void plain_tile(const DevChunk &c, uint32_t t, double
   for (int k = 0; k < 4; ++k) {
     const uint32_t i = b + k * 128 + lane * 2;
     if (pair_ok && i + 1 < c.n) {
-      const double2 x = *reinterpret_cast<const double2 *>(src + i);
+      const double2 x = *reinterpret_cast<const EBI_GLOBAL double2 *>(src + i);
       st_pair(out + c.rec_off + i, x.x, x.y);
     } else {
       if (i < c.n) st_one(out + c.rec_off + i, src[i]);
@@ -567,19 +567,19 @@ __device__ __forceinline__ void buff_deltas(const DevChunk &c, uint32_t i0, uint
 #pragma unroll
   for (int j = 0; j < 8; ++j) d[j] = 0;
   if (i0 >= c.n) return;
-  const uint8_t *base = reinterpret_cast<const uint8_t *>(c.data);
+  const EBI_GLOBAL uint8_t *base = as_global(reinterpret_cast<const uint8_t *>(c.data));
   for (uint32_t p = 0; p < c.n_planes; ++p) {
-    const uint64_t w = *reinterpret_cast<const uint64_t *>(base + (size_t)p * c.plane_stride + i0);
+    const uint64_t w = *reinterpret_cast<const EBI_GLOBAL uint64_t *>(base + (size_t)p * c.plane_stride + i0);
 #pragma unroll
     for (int j = 0; j < 8; ++j) d[j] = (d[j] << 8) | (((w >> (8 * j)) & 0xff) ^ 0x80);
   }
   if (c.r) {
     // 8 records x r bits = r bytes starting at byte i0*r/8 (byte aligned).
-    const uint8_t *tail = base + (size_t)c.n_planes * c.plane_stride;
+    const EBI_GLOBAL uint8_t *tail = base + (size_t)c.n_planes * c.plane_stride;
     const uint32_t byte0 = (i0 / 8) * c.r;
     const uint32_t a = byte0 & ~7u, sh = (byte0 & 7u) * 8;
-    const uint64_t lo = *reinterpret_cast<const uint64_t *>(tail + a);
-    const uint64_t hi = *reinterpret_cast<const uint64_t *>(tail + a + 8);
+    const uint64_t lo = *reinterpret_cast<const EBI_GLOBAL uint64_t *>(tail + a);
+    const uint64_t hi = *reinterpret_cast<const EBI_GLOBAL uint64_t *>(tail + a + 8);
     const uint64_t bits = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
     const uint64_t m = (1ull << c.r) - 1;
 #pragma unroll
@@ -659,7 +659,7 @@ __device__ __forceinline__ void sprintz_deltas(const DevChunk &c, uint32_t i0, i
     // 8 records x W bits = W bytes from byte i0 * W / 8: two aligned 16-byte
     // loads, a byte-aligned 128-bit funnel shift, then fixed-position fields
     const uint32_t start = (i0 / 8) * W;
-    const uint4 *q = reinterpret_cast<const uint4 *>(c.data) + (start >> 4);
+    const EBI_GLOBAL uint4 *q = as_global(reinterpret_cast<const uint4 *>(c.data)) + (start >> 4);
     const uint4 a = q[0], b = q[1];
     const uint32_t sb = start & 15, wi = sb >> 2, sh = (sb & 3) * 8;
     // big-endian word wi + k of the 32 loaded bytes
@@ -742,7 +742,7 @@ __device__ __forceinline__ int64_t sprintz_tile_sum(const DevChunk &c, uint32_t 
 // publishes field starts to LDS and expands by binary search so that output
 // stores are contiguous across lanes.
 __device__ __forceinline__ uint32_t rle_runs(const DevChunk &c, uint32_t f0, uint32_t run[8]) {
-  const uint8_t *runs = reinterpret_cast<const uint8_t *>(c.data);
+  const EBI_GLOBAL uint8_t *runs = as_global(reinterpret_cast<const uint8_t *>(c.data));
   uint32_t s = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -778,7 +778,7 @@ __device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, uint32_t
   if (lane == 0) starts[kTile] = total;
   wave_sync();
   const uint32_t nf = (uint32_t)min<uint64_t>(kTile, c.fields > t * (uint64_t)kTile ? c.fields - t * (uint64_t)kTile : 0);
-  const double *vals = reinterpret_cast<const double *>(c.data + c.values_off) + (uint64_t)t * kTile;
+  const EBI_GLOBAL double *vals = as_global(reinterpret_cast<const double *>(c.data + c.values_off)) + (uint64_t)t * kTile;
   double *dst = out + c.rec_off + tile_start;
   // each step expands 512 records: lane l takes records 8l .. 8l + 7 (one
   // binary search for the first one's field, then a forward walk) and the

@@ -59,7 +59,7 @@ __host__ __device__ __forceinline__ uint32_t lead_repr(uint32_t code) {
 // bytes past the current position, inside the per-chunk zero padding.
 template <int kBlk>
 struct LaneReaderT {
-  const uint4 *p;  // next block to fetch
+  const EBI_GLOBAL uint4 *p;  // next block to fetch
   uint4 cur[kBlk], nxt[kBlk];
   uint32_t wi;     // next word of `cur` (0 .. 4 * kBlk)
   uint64_t buf;
@@ -73,7 +73,7 @@ struct LaneReaderT {
   }
   __device__ __forceinline__ void init(const uint8_t *base, uint32_t bitpos) {
     const uint32_t w = bitpos >> 5;
-    p = reinterpret_cast<const uint4 *>(base) + (w / (4 * kBlk)) * kBlk;
+    p = as_global(reinterpret_cast<const uint4 *>(base)) + (w / (4 * kBlk)) * kBlk;
     load(cur);
     load(nxt);
     wi = w % (4 * kBlk);

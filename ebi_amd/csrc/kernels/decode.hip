@@ -266,7 +266,8 @@ __global__ void __launch_bounds__(256) k_chimp128_resolve(const ColView *cols, c
   if (cv.chunk_err && cv.chunk_err[ci]) return;
   const DevChunk c = cv.chunks[ci];
   double *out = cv.out + c.rec_off;
-  const uint8_t *dist = cv.c128_dist + (size_t)c.rtile0 * kTile;
+  const EBI_GLOBAL double *gout = as_global(out);
+  const EBI_GLOBAL uint8_t *dist = as_global(cv.c128_dist + (size_t)c.rtile0 * kTile);
   const C128Term tm = cv.c128_term[ci];
   uint64_t *R = ring[wave];
   const uint32_t n = c.n, total = n + 1;
@@ -277,7 +278,7 @@ __global__ void __launch_bounds__(256) k_chimp128_resolve(const ColView *cols, c
 #pragma unroll
     for (uint32_t g = 0; g < kC128Batch; ++g) {
       const uint32_t i = b0 + g * kWave + lane;
-      x[g] = i < n ? as_u64(out[i]) : i == n ? tm.x : 0;
+      x[g] = i < n ? as_u64(gout[i]) : i == n ? tm.x : 0;
       d[g] = i < n ? dist[i] : i == n ? tm.dist : 0;
     }
 #pragma unroll
