@@ -148,18 +148,21 @@ struct alignas(16) ColView {
   uint32_t n_tile_jobs;
   uint32_t n_seg_jobs;
   uint32_t n_serial_jobs;
-  // Chimp128 decode scratch of this launch (null for other codecs): source
-  // distance of every record (byte rtile0 * kTile + i for record i of a
-  // chunk), and per chunk the terminator record's parse.
+  // Chimp128 decode scratch of this launch (null for other codecs): the XOR
+  // payload and the source distance of every record (element rtile0 * kTile
+  // + i for record i of a chunk), and per chunk the terminator record's
+  // parse.  The output is only written, once, by the resolve pass.
+  double *c128_x;
   uint8_t *c128_dist;
   C128Term *c128_term;
+  uint64_t pad;
 };
-static_assert(sizeof(ColView) == 96, "ColView layout");
+static_assert(sizeof(ColView) == 112, "ColView layout");
 
 // Device bytes of the Chimp128 decode scratch of a column of n_rtiles record
-// tiles and n_chunks chunks: distances, then the terminator slots.
+// tiles and n_chunks chunks: payloads, distances, then the terminator slots.
 inline uint64_t c128_scratch_bytes(uint64_t n_rtiles, uint64_t n_chunks) {
-  return n_rtiles * kTile + n_chunks * sizeof(C128Term);
+  return n_rtiles * kTile * 9 + n_chunks * sizeof(C128Term);
 }
 
 // Device error flags (OR-ed by kernels).

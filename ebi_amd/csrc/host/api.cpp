@@ -94,8 +94,10 @@ uint64_t c128_scratch_need(const Column &c) {
 
 void bind_c128_scratch(ColView &v, const Column &c, uint8_t *scratch) {
   if (!c.chimp128) return;
-  v.c128_dist = scratch;
-  v.c128_term = reinterpret_cast<C128Term *>(scratch + (uint64_t)c.n_rtiles * kTile);
+  const uint64_t recs = (uint64_t)c.n_rtiles * kTile;
+  v.c128_x = reinterpret_cast<double *>(scratch);
+  v.c128_dist = scratch + recs * 8;
+  v.c128_term = reinterpret_cast<C128Term *>(scratch + recs * 9);
 }
 
 int upload_view(ebi_hip_ctx *ctx, Column &c, double *out) {

@@ -59,8 +59,8 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
 // Chimp128 parse of checkpoint segments first .. first + 63 of chunk ci
 // (records 0..n, n = the terminator), segment first + l on lane l, in runs
 // of kRun records: the XOR payloads go through the output stage into the
-// records' output slots, the source distances (16 bytes a lane a run)
-// straight to the scratch, the terminator's parse to its slot.
+// scratch, the source distances (16 bytes a lane a run) straight to the
+// scratch, the terminator's parse to its slot.
 // k_chimp128_resolve then forms the values.
 __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_t ci, const DevChunk &c, uint32_t first,
                                                       double *stage) {
@@ -77,7 +77,7 @@ __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_
     lead = k.a == 255 ? 0xffffffffu : k.a;
   }
   const StageSink sink{stage};
-  double *dst = cv.out + c.rec_off + (size_t)first * kSeg;
+  double *dst = cv.c128_x + (size_t)c.rtile0 * kTile + (size_t)first * kSeg;
   const uint32_t n_job = first * kSeg < c.n ? c.n - first * kSeg : 0;
   uint8_t *dist = cv.c128_dist + (size_t)c.rtile0 * kTile;  // 16-byte aligned, ceil(n / kTile) * kTile bytes
   const uint32_t bits = c.size * 8u;
@@ -240,8 +240,8 @@ __global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const uint
 }
 
 // Chimp128 resolve (decoder/chunk_reader/chimp_n.rs:173-285, the value
-// side), after k_decode_seg left each record's XOR payload in its output
-// slot and its source distance in the scratch: one wave per chunk walks
+// side), after k_decode_seg left each record's XOR payload and source
+// distance in the launch's scratch: one wave per chunk walks
 // records 0..n in steps of 64 (lane l = record base + l).  Sources inside
 // the step are followed by pointer jumping over lane shuffles (x ^= x[src],
 // src = src[src] while the source lies in the step; sources only point
@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(256) k_chimp128_resolve(const ColView *cols, c
   if (cv.chunk_err && cv.chunk_err[ci]) return;
   const DevChunk c = cv.chunks[ci];
   double *out = cv.out + c.rec_off;
-  const EBI_GLOBAL double *gout = as_global(out);
+  const EBI_GLOBAL double *px = as_global(cv.c128_x + (size_t)c.rtile0 * kTile);
   const EBI_GLOBAL uint8_t *dist = as_global(cv.c128_dist + (size_t)c.rtile0 * kTile);
   const C128Term tm = cv.c128_term[ci];
   uint64_t *R = ring[wave];
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(256) k_chimp128_resolve(const ColView *cols, c
 #pragma unroll
     for (uint32_t g = 0; g < kC128Batch; ++g) {
       const uint32_t i = b0 + g * kWave + lane;
-      x[g] = i < n ? as_u64(gout[i]) : i == n ? tm.x : 0;
+      x[g] = i < n ? as_u64(px[i]) : i == n ? tm.x : 0;
       d[g] = i < n ? dist[i] : i == n ? tm.dist : 0;
     }
 #pragma unroll

@@ -216,7 +216,7 @@ int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
  * replayed many times.  The job-space arguments are uploaded once; with
  * autotune != 0 the persistent grid size is timed on this batch and the
  * fastest kept; the launch sequence is captured in a hipGraph.  The batch
- * holds the Chimp128 decode scratch of its columns (1 byte per record and
+ * holds the Chimp128 decode scratch of its columns (9 bytes per record and
  * 16 per chunk).  Columns and outputs must outlive the batch.  No
  * counterpart in the reference. */
 typedef struct ebi_hip_batch ebi_hip_batch;

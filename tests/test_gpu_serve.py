@@ -219,3 +219,21 @@ def test_decode_op_writes_device_buffer(ctx, columns):
             assert same(b.to_host(count=d.number_of_records), o.materialize())
     for b in bufs:
         b.free()
+
+
+def test_decodes_sharing_one_output_buffer(ctx, columns):
+    """Many in-flight DECODE queries of one column into one shared device
+    buffer on several streams all complete without error, and the buffer ends
+    up holding the column: a decode only writes its output (for Chimp128 the
+    parse's payloads live in the query's own scratch, not in the output)."""
+    for d, o, vals in columns:
+        if d.scheme not in (E.CompressorConfig.by_name("chimp128").scheme, E.CompressorConfig.by_name("elf").scheme,
+                            E.CompressorConfig.by_name("gorilla").scheme):
+            continue
+        b = ctx.alloc(8 * d.number_of_records)
+        with E.Server(ctx, device_budget=1 << 30, max_batch=4, n_streams=4) as srv:
+            ts = srv.submit([E.Query(E.OP_DECODE, d, out=b) for _ in range(24)])
+            for t in ts:
+                srv.wait(t)  # raises on a device error
+        assert same(b.to_host(count=d.number_of_records), o.materialize()), d.scheme
+        b.free()
