@@ -15,8 +15,12 @@
 //   * Uncompressed, Gorilla, Chimp, Chimp128 and Elf bodies are the chunk
 //     bytes unchanged.
 // Next to the bodies the column keeps a small tile index computed once at
-// load time by the GPU (prefix state every kTile records or fields).
+// load time by the GPU (prefix state every kTile records or fields), and
+// XOR columns a second copy of their streams, lane-interleaved for the
+// segment decode (ilv, below; its own allocation).
 #pragma once
+
+#include <hip/hip_runtime.h>
 
 #include <cstdint>
 
@@ -118,10 +122,26 @@ enum JobKind : uint32_t {
 
 struct alignas(16) Job {
   uint32_t kind;
-  uint32_t col;    // column slot in the launch (output pointer / table)
+  uint32_t col;    // column slot in the launch (output pointer / table);
+                   // kJobSeg: first row of the job in the column's ilv
   uint32_t chunk;  // chunk index within the column (first chunk for kJobSerial)
   uint32_t tile;   // tile / first segment index; lane count for kJobSerial
 };
+
+// Lane-interleaved segment streams (`ilv`) of an XOR column: the 64
+// segments of a segment job are read by the 64 lanes of a wave at once,
+// each sequentially in 16-byte blocks, so the job's streams are stored
+// interleaved: row r of the job (1 KiB) holds block r of every lane's
+// stream, lane l's at block r * 64 + l.  Lane l's stream starts with the
+// chunk block holding its segment's first bit and runs to two blocks past
+// the block holding its last bit (the window holds two blocks and requests
+// the next); the job has as many rows as its longest lane stream (shorter
+// ones are zero-filled).  A wave's loads then cover whole 128-byte lines of
+// few rows instead of one line per lane.
+constexpr uint32_t kIlvRowBlocks = 64;
+__host__ __device__ inline uint32_t ilv_lane_blocks(uint32_t first_bit, uint32_t end_bit) {
+  return (end_bit >> 7) + 3 - (first_bit >> 7);
+}
 
 // Parse of a Chimp128 chunk's terminator record n: XOR payload and source
 // distance (its value must come out as the NaN end marker).
@@ -155,7 +175,7 @@ struct alignas(16) ColView {
   double *c128_x;
   uint8_t *c128_dist;
   C128Term *c128_term;
-  uint64_t pad;
+  const uint4 *ilv;  // lane-interleaved segment streams (XOR codecs; null before they are built)
 };
 static_assert(sizeof(ColView) == 112, "ColView layout");
 

@@ -85,6 +85,7 @@ ColView view_of(const Column &c, double *out) {
   v.n_tile_jobs = c.n_tile_jobs;
   v.n_seg_jobs = c.n_seg_jobs;
   v.n_serial_jobs = c.n_serial_jobs;
+  v.ilv = c.d_ilv;
   return v;
 }
 
@@ -165,6 +166,7 @@ static void free_column(Column &c) {
   hipFree(c.d_chunks);
   hipFree(c.d_tile_index);
   hipFree(c.d_ckpt);
+  hipFree(c.d_ilv);
   hipFree(c.d_chunk_err);
   hipFree(c.d_view);
   hipFree(c.d_prefix);
@@ -183,6 +185,44 @@ static hipError_t upload_vec(T **dst, const std::vector<T> &v) {
   hipError_t e = hipMalloc((void **)dst, v.size() * sizeof(T));
   if (e != hipSuccess) return e;
   return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// Lane-interleaved copy of an XOR column's segment streams (after the index
+// pass, whose checkpoints give each segment's bits): the row count of every
+// segment job, their prefix as each job's first row (Job::col), then the
+// copy.  Chunks rejected by the index pass get no rows.
+static int build_ilv(ebi_hip_ctx *ctx, Column &col) {
+  const uint32_t nj = col.n_seg_jobs;
+  uint32_t *d_rows = nullptr;
+  hipError_t e = hipMalloc((void **)&d_rows, (size_t)nj * 4);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(ilv row counts)");
+  std::vector<uint32_t> rows(nj);
+  launch::ilv_rows(ctx->stream, col.d_view, nj, d_rows);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(rows.data(), d_rows, (size_t)nj * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  uint64_t total = 0;
+  for (uint32_t j = 0; j < nj && e == hipSuccess; ++j) {
+    col.h_seg_jobs[j].col = (uint32_t)total;
+    total += rows[j];
+  }
+  if (e == hipSuccess && total > UINT32_MAX) {
+    hipFree(d_rows);
+    return fail(ctx, EBI_OOM, "interleaved segment streams exceed 2^32 rows");
+  }
+  if (e == hipSuccess) e = hipMalloc((void **)&col.d_ilv, std::max<uint64_t>(total, 1) * kIlvRowBlocks * 16);
+  if (e == hipSuccess) e = hipMemcpy(col.d_seg_jobs, col.h_seg_jobs.data(), (size_t)nj * sizeof(Job), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch::ilv_copy(ctx->stream, col.d_view, nj, d_rows, col.d_ilv);
+    e = hipGetLastError();
+  }
+  const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  hipFree(d_rows);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) return hip_fail(ctx, e, "interleaved segment streams");
+  col.ilv_rows = total;
+  col.device_bytes += total * kIlvRowBlocks * 16;
+  return EBI_OK;
 }
 
 // Parse, validate codec headers, repack into the device layout, upload and
@@ -408,6 +448,7 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return hip_fail(ctx, e, "index pass");
   for (size_t k = 0; k < nc; ++k) col.chunks[k].err = errs[k];
+  if ((col.xor_codec || col.chimp128) && col.n_seg_jobs && (rc = build_ilv(ctx, col))) return rc;
   if (scheme == kRLE && nc) {
     // Balance RLE decode jobs by output: a job expands up to kTilesPerJob
     // steps of kTile output records of one field tile (runs of up to 255

@@ -42,6 +42,8 @@ struct Column {
   DevChunk *d_chunks = nullptr;
   int64_t *d_tile_index = nullptr;
   XorCkpt *d_ckpt = nullptr;
+  uint4 *d_ilv = nullptr;    // lane-interleaved segment streams (XOR codecs)
+  uint64_t ilv_rows = 0;     // 1 KiB rows of d_ilv
   uint32_t *d_chunk_err = nullptr;
   ColView *d_view = nullptr;  // slot used by single-column launches
   uint32_t *d_prefix = nullptr;  // {0, tile jobs, 0, serial jobs, 0, chunks, 0, seg jobs}: single-column launches

@@ -82,9 +82,9 @@ struct XorState {
   int32_t beta;
 };
 
-// State at a checkpoint written by the index pass (255 / -1 encode "unset").
-__host__ __device__ __forceinline__ void xor_state_at(XorState &s, uint32_t codec, const uint8_t *data, const XorCkpt &k) {
-  s.r.init(data, k.bitpos);
+// Decoder state (all but the window) at a checkpoint written by the index
+// pass (255 / -1 encode "unset").
+__host__ __device__ __forceinline__ void xor_state_set(XorState &s, uint32_t codec, const XorCkpt &k) {
   s.v = k.value;
   if (codec == kCodecGorilla) {
     s.a = k.a;
@@ -94,6 +94,12 @@ __host__ __device__ __forceinline__ void xor_state_at(XorState &s, uint32_t code
     s.b = k.b == 255 ? 0xffffffffu : k.b;
   }
   s.beta = k.beta < 0 ? INT32_MAX : (int32_t)k.beta;
+}
+
+// State at a checkpoint, reading the chunk's stream in place.
+__host__ __device__ __forceinline__ void xor_state_at(XorState &s, uint32_t codec, const uint8_t *data, const XorCkpt &k) {
+  s.r.init(data, k.bitpos);
+  xor_state_set(s, codec, k);
 }
 
 // State before record 0 of a chunk.
@@ -326,6 +332,110 @@ __host__ __device__ __forceinline__ uint32_t xor_run(uint32_t codec, XorState &s
   }
 }
 
+// ------------------------------------------------ lockstep segment runs
+// The segment kernel's form of the decoders above: the 64 lanes of a wave
+// decode records a + k (k < cnt) of their own segments in one loop whose
+// trip count cnt is the same on every lane, so the body is straight-line
+// code (selects, no per-lane exits).  A lane takes part in record i while
+// from <= i < lim and it has no error; lim = min(a + kRun, the segment's
+// end), or n + 1 when the segment ends the chunk and its terminator n falls
+// in [a, a + kRun] (the terminator is parsed and checked; cnt = kRun + 1
+// when it is record a + kRun of some lane).  Record 0 of a chunk (64 raw
+// bits) is decoded before, by xor_seg_first, which sets from = 1 on its
+// lane.  A lane that meets an error keeps it,
+// consumes no more bits and so reads no further than the forms above; the
+// records it leaves in the sink are then meaningless, as are the sink
+// entries of records it does not take part in (sink.row is called for every
+// k).  Returns the lane's error.
+
+// Record 0 of a Gorilla or Chimp chunk (the stream's first 64 bits).
+template <class Sink>
+__host__ __device__ __forceinline__ uint32_t xor_seg_first(XorState &s, uint32_t size_bits, const Sink &out) {
+  BitWindow &r = s.r;
+  s.v = r.field(0, 64);
+  r.consume(64);
+  r.refill();
+  out.row(0, 0, as_f64(s.v));
+  if (r.pos > size_bits) return kErrOverrun;
+  return s.v == kNan ? kErrCount : kErrNone;  // n >= 1: record 0 is not the terminator
+}
+
+// Gorilla, decoder/chunk_reader/gorilla.rs:77-157 (the record step of
+// gorilla_run).
+template <class Sink>
+__host__ __device__ __forceinline__ uint32_t gorilla_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
+                                                uint32_t cnt, uint32_t from, uint32_t lim, const Sink &out,
+                                                uint32_t e) {
+  BitWindow &r = s.r;
+  uint64_t v = s.v;
+  uint32_t L = s.a, T = s.b;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t i = a + k;
+    const bool act = e == 0 && i >= from && i < lim;
+    const uint32_t c2 = (uint32_t)(r.hi >> 62);
+    const bool nz = c2 >= 2, fresh = c2 == 3;
+    const uint32_t h = (uint32_t)(r.hi >> 51) & 0x7ff;
+    const uint32_t l = fresh ? h >> 6 : L;
+    const uint32_t sig = fresh ? (h & 63) + 1 : 64 - L - T;
+    const bool bad = fresh && l + sig > 64;
+    const uint32_t o = fresh ? 13 : 2;
+    const uint32_t t = (64 - l - sig) & 63;
+    const uint64_t x = r.field(o, sig) << t;
+    const bool upd = act && nz && !bad;
+    v ^= upd ? x : 0;
+    L = upd ? l : L;
+    T = upd ? t : T;
+    r.consume(act && !bad ? (nz ? o + sig : 1u) : 0u);
+    r.refill();
+    const uint32_t code = bad                      ? (uint32_t)kErrCorrupt
+                          : r.pos > size_bits      ? (uint32_t)kErrOverrun
+                          : (i == n) != (v == kNan) ? (uint32_t)kErrCount
+                                                   : 0u;
+    e = act ? code : e;
+    out.row(i, k, as_f64(v));
+  }
+  s.v = v;
+  s.a = L;
+  s.b = T;
+  return e;
+}
+
+// Chimp, decoder/chunk_reader/chimp.rs:59-125 (the record step of chimp_run).
+template <class Sink>
+__host__ __device__ __forceinline__ uint32_t chimp_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
+                                              uint32_t cnt, uint32_t from, uint32_t lim, const Sink &out,
+                                              uint32_t e) {
+  BitWindow &r = s.r;
+  uint64_t v = s.v;
+  uint32_t lead = s.a;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t i = a + k;
+    const bool act = e == 0 && i >= from && i < lim;
+    const uint32_t f = (uint32_t)(r.hi >> 62);
+    const uint32_t h9 = (uint32_t)(r.hi >> 53) & 0x1ff;
+    const uint32_t nl = (f & 1) ? lead_repr(h9 >> 6) : lead;  // flags 11 and 01 carry a lead code
+    const uint32_t s6 = h9 & 63;
+    const uint32_t sig = f == 1 ? (s6 ? s6 : 64u) : 64 - nl;
+    const bool bad = (f == 2 && lead > 64) || (f == 1 && sig + nl > 64);
+    const uint32_t o = f == 3 ? 5u : f == 1 ? 11u : 2u;
+    const uint64_t x = r.field(o, sig) << (f == 1 ? (64 - sig - nl) & 63 : 0u);
+    const bool upd = act && f != 0 && !bad;
+    v ^= upd ? x : 0;
+    lead = upd ? nl : lead;
+    r.consume(act && !bad ? (f ? o + sig : 2u) : 0u);
+    r.refill();
+    const uint32_t code = bad                      ? (uint32_t)kErrCorrupt
+                          : r.pos > size_bits      ? (uint32_t)kErrOverrun
+                          : (i == n) != (v == kNan) ? (uint32_t)kErrCount
+                                                   : 0u;
+    e = act ? code : e;
+    out.row(i, k, as_f64(v));
+  }
+  s.v = v;
+  s.a = lead;
+  return e;
+}
+
 // Output staging of a wave decoding 64 checkpoint segments (segment
 // first + l on lane l) in runs of kRun = 16 records: record i of lane l
 // goes to slot (i mod 16, l ^ stage_swz(i mod 16)) of the wave's [16][64]
@@ -340,6 +450,11 @@ struct StageSink {
   __device__ __forceinline__ void put(uint32_t i, double v) const {
     const uint32_t r = i & (kRun - 1);
     stage[r * kWave + (lane_id() ^ stage_swz(r))] = v;
+  }
+  // lockstep runs: record i is row k of the run (k <= kRun; row kRun holds
+  // only a terminator, never flushed)
+  __device__ __forceinline__ void row(uint32_t, uint32_t k, double v) const {
+    stage[k * kWave + (lane_id() ^ stage_swz(k))] = v;
   }
 };
 

@@ -229,6 +229,7 @@ struct BitWindow {
   uint64_t c0, c1, n0, n1; // current and next block
   uint4 nxt;               // requested block (after the next)
   uint32_t b;              // bit offset of pos in the current block, [0, 128)
+  uint32_t stride;         // blocks from one stream block to the next
   uint64_t hi, lo;
   uint32_t pos;            // stream bit position of the window's first bit
   static constexpr uint32_t nb = 128;  // valid bits of the window
@@ -245,7 +246,8 @@ struct BitWindow {
     c1 = n1;
     n0 = be64(nxt.x, nxt.y);
     n1 = be64(nxt.z, nxt.w);
-    nxt = *p++;
+    nxt = *p;
+    p += stride;
     b -= 128;
   }
   // after consume(): the window at the new position
@@ -258,10 +260,16 @@ struct BitWindow {
     lo = funnel(x1, x2, s);
   }
   __host__ __device__ __forceinline__ void init(const uint8_t *base, uint32_t bitpos) {
-    p = as_global(reinterpret_cast<const uint4 *>(base)) + (bitpos >> 7);
-    const uint4 c = p[0], n = p[1];
-    nxt = p[2];
-    p += 3;
+    init_at(reinterpret_cast<const uint4 *>(base) + (bitpos >> 7), bitpos, 1);
+  }
+  // blk: the stream's block holding bit `bitpos`; consecutive blocks of the
+  // stream lie `step` blocks apart (1, or kIlvRowBlocks in an ilv)
+  __host__ __device__ __forceinline__ void init_at(const uint4 *blk, uint32_t bitpos, uint32_t step) {
+    stride = step;
+    p = as_global(blk);
+    const uint4 c = p[0], n = p[step];
+    nxt = p[2 * step];
+    p += 3 * step;
     c0 = be64(c.x, c.y);
     c1 = be64(c.z, c.w);
     n0 = be64(n.x, n.y);

@@ -34,23 +34,58 @@ __device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_
 // over runs is wave-uniform: lanes without a segment (past the chunk's end)
 // or stopped by an error keep taking part in the stores.
 template <uint32_t kCodec>
-__device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChunk &c, uint32_t first,
-                                                 double *stage) {
+__device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChunk &c, uint32_t first, uint32_t row,
+                                                double *stage) {
   const uint32_t i0 = (first + lane_id()) * kSeg;
   const bool live = i0 < c.n;
   const uint32_t i1 = live ? min(i0 + kSeg, c.n) : 0;
-  const uint8_t *data = reinterpret_cast<const uint8_t *>(c.data);
   XorState s;
-  if (live) xor_state_at(s, kCodec, data, reinterpret_cast<const XorCkpt *>(c.xidx)[first + lane_id()]);
+  if (live) {
+    const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[first + lane_id()];
+    s.r.init_at(cv.ilv + (size_t)row * kIlvRowBlocks + lane_id(), k.bitpos, kIlvRowBlocks);
+    xor_state_set(s, kCodec, k);
+  }
   const StageSink sink{stage};
   double *dst = cv.out + c.rec_off + (size_t)first * kSeg;
   const uint32_t n_job = first * kSeg < c.n ? c.n - first * kSeg : 0;  // records from segment `first` on
   const uint32_t bits = c.size * 8u;
   uint32_t e = 0;
+  if (kCodec == kCodecElf) {
 #pragma unroll 1  // one copy of the codec loop per codec (instruction cache)
+    for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
+      const uint32_t a = i0 + k0, b = min(a + kRun, i1);
+      if (live && e == 0 && a < b) e = xor_run<true>(kCodec, s, bits, c.n, a, b, sink, nullptr);
+      stage_flush(stage, dst, k0, n_job);
+    }
+    return e;
+  }
+  // lockstep runs: a lane without a segment has an empty window that it
+  // never advances (it takes part in no record)
+  if (!live) {
+    s.r.b = 0;
+    s.r.pos = 0;
+    s.r.hi = s.r.lo = 0;
+    s.v = 0;
+    s.a = s.b = 0;
+  }
+  const uint32_t last = live && i1 == c.n ? c.n + 1 : i1;  // the chunk's last segment parses the terminator
+  uint32_t from = i0;
+  if (live && i0 == 0) {
+    e = xor_seg_first(s, bits, sink);
+    from = 1;
+  }
+#pragma unroll 1
   for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
-    const uint32_t a = i0 + k0, b = min(a + kRun, i1);
-    if (live && e == 0 && a < b) e = xor_run<true>(kCodec, s, bits, c.n, a, b, sink, nullptr);
+    const uint32_t a = i0 + k0;
+    // kRun records, and one more when a lane's terminator follows them
+    const bool term_next = last == a + kRun + 1;
+    const uint32_t cnt = kRun + (__ballot(term_next) != 0);
+    const uint32_t lim = term_next ? last : min(last, a + kRun);
+    if (kCodec == kCodecGorilla)
+      e = gorilla_seg(s, bits, c.n, a, cnt, from, lim, sink, e);
+    else
+      e = chimp_seg(s, bits, c.n, a, cnt, from, lim, sink, e);
+    from = max(from, lim);  // a terminator taken as record a + kRun is not parsed again
     stage_flush(stage, dst, k0, n_job);
   }
   return e;
@@ -63,7 +98,7 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
 // scratch, the terminator's parse to its slot.
 // k_chimp128_resolve then forms the values.
 __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_t ci, const DevChunk &c, uint32_t first,
-                                                      double *stage) {
+                                                      uint32_t row, double *stage) {
   const uint32_t total = c.n + 1;
   const uint32_t seg = first + lane_id();
   const uint32_t i0 = seg * kSeg;
@@ -73,7 +108,7 @@ __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_
   uint32_t lead = 0xffffffffu;
   if (live) {
     const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[seg];
-    r.init(reinterpret_cast<const uint8_t *>(c.data), k.bitpos);
+    r.init_at(cv.ilv + (size_t)row * kIlvRowBlocks + lane_id(), k.bitpos, kIlvRowBlocks);
     lead = k.a == 255 ? 0xffffffffu : k.a;
   }
   const StageSink sink{stage};
@@ -94,13 +129,14 @@ __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_
   return e;
 }
 
-__device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, uint32_t first, double *stage) {
+__device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, uint32_t first, uint32_t row,
+                                               double *stage) {
   const DevChunk c = cv.chunks[ci];
   switch (c.codec) {
-    case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, stage);
-    case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, stage);
-    case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, stage);
-    case kCodecChimp128: return segment_wave_c128(cv, ci, c, first, stage);
+    case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, row, stage);
+    case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, row, stage);
+    case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, row, stage);
+    case kCodecChimp128: return segment_wave_c128(cv, ci, c, first, row, stage);
     default: return kErrCorrupt;
   }
 }
@@ -158,7 +194,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 // checks).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_decode_seg(
     const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
-  __shared__ __align__(16) double lds[4][kRun * kWave];
+  __shared__ __align__(16) double lds[4][(kRun + 1) * kWave];
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
   uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
@@ -167,7 +203,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const ColView cv = cols[ci];
     const Job jb = cv.seg_jobs[j - prefix[ci]];
     if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-      const uint32_t e = segment_job(cv, jb.chunk, jb.tile, lds[wave]);
+      const uint32_t e = segment_job(cv, jb.chunk, jb.tile, jb.col, lds[wave]);
       if (e) atomicOr(err, e);
     }
     j += stride;
@@ -358,6 +394,61 @@ __global__ void k_tile_scan(const ColView *cols, uint32_t col, uint32_t n_chunks
   }
 }
 
+// Lane l of the wave for segment job jb of view cv: the stream of segment
+// jb.tile + l as an ilv lane -- its first chunk block (first_blk) and its
+// block count (0 for a lane past the chunk's segments or a chunk the index
+// pass rejected).  The segment's bits run from its checkpoint to the next
+// (a Chimp128 chunk's last segment, the one with the terminator, and the
+// last segment of the other codecs to the chunk's end), clamped to the
+// chunk.
+__device__ __forceinline__ uint32_t ilv_lane(const ColView &cv, const Job &jb, uint32_t &first_blk) {
+  const DevChunk c = cv.chunks[jb.chunk];
+  const uint32_t n_segs = c.codec == kCodecChimp128 ? c.n / kSeg + 1 : (c.n + kSeg - 1) / kSeg;
+  const uint32_t seg = jb.tile + lane_id();
+  const uint32_t bits = c.size * 8u;
+  first_blk = 0;
+  if (seg >= n_segs || (cv.chunk_err && cv.chunk_err[jb.chunk])) return 0;
+  const XorCkpt *ck = reinterpret_cast<const XorCkpt *>(c.xidx);
+  const uint32_t a = min(ck[seg].bitpos, bits);
+  const uint32_t b = seg + 1 < n_segs ? min(max(ck[seg + 1].bitpos, a), bits) : bits;
+  first_blk = a >> 7;
+  return ilv_lane_blocks(a, b);
+}
+
+// Rows of each segment job of cols[0] (one wave per job): its longest lane.
+__global__ void __launch_bounds__(256) k_ilv_rows(const ColView *cols, uint32_t n_jobs, uint32_t *rows) {
+  const uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / kWave);
+  if (j >= n_jobs) return;
+  const ColView cv = cols[0];
+  const Job jb = cv.seg_jobs[j];
+  uint32_t fb;
+  uint32_t m = ilv_lane(cv, jb, fb);
+  for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+  if (lane_id() == 0) rows[j] = m;
+}
+
+// Fills the rows of each segment job of cols[0] (jb.col = its first row,
+// rows[j] of them): block r of lane l is the lane's chunk block first + r
+// while r is below its block count, zero after.  Each store covers a whole
+// 1 KiB row.
+__global__ void __launch_bounds__(256) k_ilv_copy(const ColView *cols, uint32_t n_jobs, const uint32_t *rows,
+                                                  uint4 *ilv) {
+  const uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / kWave);
+  if (j >= n_jobs) return;
+  const ColView cv = cols[0];
+  const Job jb = cv.seg_jobs[j];
+  uint32_t fb;
+  const uint32_t nb = ilv_lane(cv, jb, fb);
+  const EBI_GLOBAL uint4 *src = as_global(reinterpret_cast<const uint4 *>(cv.chunks[jb.chunk].data)) + fb;
+  uint4 *dst = ilv + (size_t)jb.col * kIlvRowBlocks + lane_id();
+  const uint32_t n_rows = rows[j];
+  for (uint32_t r = 0; r < n_rows; ++r) {
+    uint4 x = make_uint4(0, 0, 0, 0);
+    if (r < nb) x = src[r];
+    dst[(size_t)r * kIlvRowBlocks] = x;
+  }
+}
+
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -416,6 +507,14 @@ void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32
     hipLaunchKernelGGL((k_chimp128<true>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
   else
     hipLaunchKernelGGL((k_chimp128<false>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
+}
+
+void ilv_rows(hipStream_t s, const ColView *cols, uint32_t n_jobs, uint32_t *rows) {
+  if (n_jobs) hipLaunchKernelGGL(k_ilv_rows, dim3((n_jobs + 3) / 4), dim3(256), 0, s, cols, n_jobs, rows);
+}
+
+void ilv_copy(hipStream_t s, const ColView *cols, uint32_t n_jobs, const uint32_t *rows, uint4 *ilv) {
+  if (n_jobs) hipLaunchKernelGGL(k_ilv_copy, dim3((n_jobs + 3) / 4), dim3(256), 0, s, cols, n_jobs, rows, ilv);
 }
 
 void tile_index(hipStream_t s, const ColView *cols, uint32_t col, const Job *jobs, uint32_t n_jobs,

@@ -32,6 +32,11 @@ void chimp128_resolve(hipStream_t s, const ColView *cols, const uint32_t *prefix
                       uint32_t *err);
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
               const Job *single, uint32_t *err, bool write);
+// Load-time lane-interleaved segment streams of the XOR column cols[0]:
+// ilv_rows writes the row count of each segment job; ilv_copy fills the
+// rows (each job's first row in its Job::col).
+void ilv_rows(hipStream_t s, const ColView *cols, uint32_t n_jobs, uint32_t *rows);
+void ilv_copy(hipStream_t s, const ColView *cols, uint32_t n_jobs, const uint32_t *rows, uint4 *ilv);
 // Load-time tile index for DeltaSprintz and RLE chunks of column slot `col`.
 void tile_index(hipStream_t s, const ColView *cols, uint32_t col, const Job *jobs, uint32_t n_jobs,
                 uint32_t n_chunks, uint32_t *err);

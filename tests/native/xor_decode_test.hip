@@ -6,6 +6,11 @@
 // carried between runs.  Both must give back the encoded values bit for bit
 // (the codecs are lossless); Chimp128's (payload, source distance) parse is
 // resolved on the host the way k_chimp128_resolve defines the values.
+// Gorilla and Chimp also in the kernel's lockstep form (every lane of a
+// wave through the same record loop), on intact and on damaged streams; the
+// segment forms read the streams both in place and through the column's
+// lane-interleaved copy, as the segment kernel does, with a check that no
+// lane reads past the blocks the copy gives it.
 // Synthetic columns of the four generators, ragged chunkings down to one
 // record per chunk.  Exit code 0 on success.
 #include <cmath>
@@ -28,6 +33,124 @@ static void expect(bool ok, const char *what, int codec, int gen, uint64_t chunk
                 (unsigned long long)chunk, (unsigned long long)k, (unsigned long long)i);
 }
 
+// The column's lane-interleaved segment streams as the load-time kernels
+// build them (same block counts, ilv_lane_blocks), and the check that a
+// lane's window never requested a block past its own: after a decode the
+// window's next request is at p, so (p - lane_base) / 64 blocks were taken.
+struct Ilv {
+  std::vector<uint4> blocks;
+  std::vector<uint32_t> job_row, nb;  // per job: first row; per segment: block count
+  const uint4 *lane(uint32_t seg) const {
+    return blocks.data() + (size_t)job_row[seg / kWave] * kIlvRowBlocks + seg % kWave;
+  }
+};
+
+static Ilv build_ilv(const uint8_t *data, uint32_t bits, const XorCkpt *ck, uint32_t nseg) {
+  Ilv v;
+  const uint4 *src = reinterpret_cast<const uint4 *>(data);
+  v.nb.resize(nseg);
+  for (uint32_t first = 0; first < nseg; first += kWave) {
+    uint32_t rows = 0;
+    for (uint32_t seg = first; seg < std::min(nseg, first + kWave); ++seg) {
+      const uint32_t a = std::min(ck[seg].bitpos, bits);
+      const uint32_t b = seg + 1 < nseg ? std::min(std::max(ck[seg + 1].bitpos, a), bits) : bits;
+      v.nb[seg] = ilv_lane_blocks(a, b);
+      rows = std::max(rows, v.nb[seg]);
+    }
+    const uint32_t row0 = (uint32_t)(v.blocks.size() / kIlvRowBlocks);
+    v.job_row.push_back(row0);
+    v.blocks.resize(v.blocks.size() + (size_t)rows * kIlvRowBlocks, make_uint4(0, 0, 0, 0));
+    for (uint32_t seg = first; seg < std::min(nseg, first + kWave); ++seg)
+      for (uint32_t r = 0; r < v.nb[seg]; ++r)
+        v.blocks[((size_t)row0 + r) * kIlvRowBlocks + seg % kWave] = src[(ck[seg].bitpos >> 7) + r];
+  }
+  return v;
+}
+
+static bool within_lane(const Ilv &v, uint32_t seg, const BitWindow &r) {
+  return (size_t)(r.p - v.lane(seg)) <= (size_t)v.nb[seg] * kIlvRowBlocks;
+}
+
+// The segment kernel's lockstep form, wave by wave: 64 segments per wave
+// job, lanes without a segment included (their window has no stream, so a
+// read through it would crash this host run), runs of kRun records with the
+// wave's shared trip count.
+struct RowSink {
+  double *out;
+  uint32_t lim;  // records the buffer holds
+  void row(uint32_t i, uint32_t k, double v) const {  // row kRun: the stage row that is never flushed
+    if (k < kRun && i < lim) out[i] = v;
+  }
+};
+
+static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t bits, uint32_t cn, const XorCkpt *ck,
+                                      uint32_t nseg, double *out, const Ilv *ilv = nullptr) {
+  const RowSink sink{out, cn};
+  std::vector<uint32_t> err(nseg, 0);
+  for (uint32_t first = 0; first < nseg; first += kWave) {
+    XorState st[kWave];
+    uint32_t e[kWave], from[kWave], last[kWave], i0[kWave];
+    for (uint32_t l = 0; l < kWave; ++l) {
+      i0[l] = (first + l) * kSeg;
+      const bool live = i0[l] < cn;
+      const uint32_t i1 = live ? std::min(i0[l] + kSeg, cn) : 0;
+      if (live && ilv) {
+        st[l].r.init_at(ilv->lane(first + l), ck[first + l].bitpos, kIlvRowBlocks);
+        xor_state_set(st[l], codec, ck[first + l]);
+      } else if (live) {
+        xor_state_at(st[l], codec, data, ck[first + l]);
+      } else {
+        st[l] = XorState{};
+        st[l].r.p = nullptr;
+      }
+      last[l] = live && i1 == cn ? cn + 1 : i1;
+      from[l] = i0[l];
+      e[l] = 0;
+      if (live && i0[l] == 0) {
+        e[l] = xor_seg_first(st[l], bits, sink);
+        from[l] = 1;
+      }
+    }
+    for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
+      bool extra = false;
+      for (uint32_t l = 0; l < kWave; ++l) extra |= last[l] == i0[l] + k0 + kRun + 1;
+      const uint32_t cnt = kRun + (extra ? 1 : 0);
+      for (uint32_t l = 0; l < kWave; ++l) {
+        const uint32_t a = i0[l] + k0;
+        const uint32_t lim = last[l] == a + kRun + 1 ? last[l] : std::min(last[l], a + kRun);
+        e[l] = codec == kCodecGorilla ? gorilla_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l])
+                                      : chimp_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l]);
+        from[l] = std::max(from[l], lim);
+      }
+    }
+    for (uint32_t l = 0; l < kWave && first + l < nseg; ++l) {
+      err[first + l] = e[l];
+      if (ilv && !within_lane(*ilv, first + l, st[l].r)) err[first + l] |= 0x100;  // read past the lane's blocks
+    }
+  }
+  return err;
+}
+
+// The per-record form over the same segments (the reference's order of
+// checks); per-segment errors.
+static std::vector<uint32_t> segform(int codec, const uint8_t *data, uint32_t bits, uint32_t cn, const XorCkpt *ck,
+                                     uint32_t nseg, double *out) {
+  const DirectSink sink{out};
+  std::vector<uint32_t> err(nseg, 0);
+  XorState s;
+  for (uint32_t seg = 0; seg < nseg; ++seg) {
+    const uint32_t i0 = seg * kSeg, i1 = std::min(i0 + kSeg, cn);
+    xor_state_at(s, codec, data, ck[seg]);
+    uint32_t e = 0;
+    for (uint32_t k0 = 0; k0 < kSeg && e == kErrNone; k0 += kRun) {
+      const uint32_t a = i0 + k0, b = std::min(a + kRun, i1);
+      if (a < b) e = xor_run<true>(codec, s, bits, cn, a, b, sink, nullptr);
+    }
+    err[seg] = e;
+  }
+  return err;
+}
+
 int main() {
   char buf[32];
   for (int i = 0; i < 400; ++i) {  // the host copies of the tables, filled as the library fills the device's
@@ -41,7 +164,7 @@ int main() {
     double p0, p1;
   } gens[] = {{0, 2, 100.0, 0.1}, {1, 3, -1000.0, 1000.0}, {2, 2, 0.01, 0.0}, {3, 4, 1000.0, 0.01}};
   const int codecs[] = {kCodecGorilla, kCodecChimp, kCodecElf, kCodecChimp128};
-  const uint64_t chunks[] = {1, 3, 5, 64, 65, 127, 1003, 4096};
+  const uint64_t chunks[] = {1, 3, 5, 64, 65, 80, 96, 112, 127, 1003, 4096};  // 80 / 96 / 112: terminator after a full run
   const uint64_t n = 10003;
   std::vector<double> v(n);
   for (int codec : codecs)
@@ -94,15 +217,65 @@ int main() {
             }
             for (uint32_t i = 0; i < cn; ++i)
               expect(as_u64(out[i]) == as_u64(want[i]), "segment value", codec, g, chunk, k, i);
+            const Ilv ilv = build_ilv(data, bits, ck.data(), nseg);
+            if (codec == kCodecElf) {  // the per-record form through the ilv
+              std::fill(out.begin(), out.end(), 0.0);
+              for (uint32_t seg = 0; seg < nseg; ++seg) {
+                const uint32_t i0 = seg * kSeg, i1 = std::min(i0 + kSeg, cn);
+                s.r.init_at(ilv.lane(seg), ck[seg].bitpos, kIlvRowBlocks);
+                xor_state_set(s, codec, ck[seg]);
+                for (uint32_t k0 = 0; k0 < kSeg && e == kErrNone; k0 += kRun) {
+                  const uint32_t a = i0 + k0, b = std::min(a + kRun, i1);
+                  if (a < b) e = xor_run<true>(codec, s, bits, cn, a, b, sink, nullptr);
+                }
+                expect(e == kErrNone, "ilv segment decode error", codec, g, chunk, k, i0);
+                expect(within_lane(ilv, seg, s.r), "ilv lane overread", codec, g, chunk, k, i0);
+              }
+              for (uint32_t i = 0; i < cn; ++i)
+                expect(as_u64(out[i]) == as_u64(want[i]), "ilv segment value", codec, g, chunk, k, i);
+            } else {
+              std::fill(out.begin(), out.end(), 0.0);
+              const std::vector<uint32_t> le = lockstep(codec, data, bits, cn, ck.data(), nseg, out.data(), &ilv);
+              for (uint32_t sg = 0; sg < nseg; ++sg)
+                expect(le[sg] == kErrNone, "lockstep (ilv) error", codec, g, chunk, k, sg * kSeg);
+              for (uint32_t i = 0; i < cn; ++i)
+                expect(as_u64(out[i]) == as_u64(want[i]), "lockstep (ilv) value", codec, g, chunk, k, i);
+              // damaged streams (checkpoints of the intact one): a flipped
+              // bit, then a stream cut short; both forms must stop each
+              // segment with the same error, and agree on segments that
+              // decode
+              std::vector<uint4> bad(body);
+              uint8_t *bd = reinterpret_cast<uint8_t *>(bad.data());
+              std::vector<double> o1(cn + 1), o2(cn + 1);
+              for (int trial = 0; trial < 2; ++trial) {
+                uint32_t bb = bits;
+                if (trial == 0) {
+                  const uint32_t at = (uint32_t)((k * 2654435761u + chunk) % bits);
+                  bd[at / 8] ^= (uint8_t)(0x80u >> (at % 8));
+                } else {
+                  bd = const_cast<uint8_t *>(data);
+                  bb = bits > 40 ? bits - 40 : 0;
+                }
+                const std::vector<uint32_t> e1 = segform(codec, bd, bb, cn, ck.data(), nseg, o1.data());
+                const std::vector<uint32_t> e2 = lockstep(codec, bd, bb, cn, ck.data(), nseg, o2.data());
+                for (uint32_t sg = 0; sg < nseg; ++sg) {
+                  expect(e1[sg] == e2[sg], "damaged stream: error", codec, g, chunk, k, sg * kSeg);
+                  if (e1[sg] == kErrNone)
+                    for (uint32_t i = sg * kSeg; i < std::min(cn, sg * kSeg + kSeg); ++i)
+                      expect(as_u64(o1[i]) == as_u64(o2[i]), "damaged stream: value", codec, g, chunk, k, i);
+                }
+              }
+            }
           } else {
             uint32_t e = chimp128_index(data, bits, cn, ck.data());
             expect(e == kErrNone, "Chimp128 index error", codec, g, chunk, k, 0);
             std::vector<uint8_t> dist((cn + kTile - 1) / kTile * kTile + 16, 0);
             C128Term term{};
+            const Ilv ilv = build_ilv(data, bits, ck.data(), nseg);  // the kernel parses through the ilv
             for (uint32_t seg = 0; seg < nseg; ++seg) {
               const uint32_t i0 = seg * kSeg, i1 = std::min(i0 + kSeg, cn + 1);
               BitWindow r;
-              r.init(data, ck[seg].bitpos);
+              r.init_at(ilv.lane(seg), ck[seg].bitpos, kIlvRowBlocks);
               uint32_t lead = ck[seg].a == 255 ? 0xffffffffu : ck[seg].a;
               for (uint32_t k0 = 0; k0 < kSeg && e == kErrNone; k0 += kRun) {
                 const uint32_t a = i0 + k0, b = std::min(a + kRun, i1);
@@ -111,6 +284,7 @@ int main() {
                 if (a < cn) std::memcpy(dist.data() + a, d16, 16);
               }
               expect(e == kErrNone, "Chimp128 parse error", codec, g, chunk, k, i0);
+              expect(within_lane(ilv, seg, r), "Chimp128 ilv lane overread", codec, g, chunk, k, i0);
             }
             std::vector<uint64_t> val(cn + 1, 0);
             for (uint32_t i = 0; i <= cn; ++i) {
