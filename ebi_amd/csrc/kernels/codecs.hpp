@@ -70,6 +70,18 @@ __host__ __device__ __forceinline__ double elf_round_up(double v, uint32_t alpha
   return (v < 0.0 ? floor(t) : ceil(t)) / scale;
 }
 
+// recover_v_by_beta_star, decoder/chunk_reader/elf.rs:66-81: the value of
+// an erased record from its stored bits v' and beta*.
+__host__ __device__ __forceinline__ double elf_recover(double v, int32_t beta) {
+  const int32_t sp = elf_get_sp(fabs(v));
+  if (beta == 0) {
+    const uint32_t k = 0u - (uint32_t)sp - 1u;  // (-sp - 1) as u32, wrapping
+    const double w = k < 400 ? c_n10[k] : 0.0;
+    return v < 0.0 ? -w : w;
+  }
+  return elf_round_up(v, (uint32_t)beta - (uint32_t)sp - 1u);
+}
+
 // State of one XOR stream between calls of its decoder: the bit window and
 // the values the next record depends on (Gorilla: v = previous value, a = L,
 // b = T; Chimp: v, a = stored lead; Elf: v = stored bits, a = L, b = T,
@@ -299,18 +311,8 @@ __host__ __device__ __forceinline__ uint32_t elf_run(XorState &s, uint32_t size_
     if (i == n) return term ? kErrNone : kErrCount;
     if (term) return kErrCount;
     if (kWrite) {
-      double v = as_f64(stored);
-      if (recover) {  // recover_v_by_beta_star, elf.rs:66-81
-        const int32_t sp = elf_get_sp(fabs(v));
-        if (beta == 0) {
-          const uint32_t k = 0u - (uint32_t)sp - 1u;  // (-sp - 1) as u32, wrapping
-          const double w = k < 400 ? c_n10[k] : 0.0;
-          v = v < 0.0 ? -w : w;
-        } else {
-          v = elf_round_up(v, (uint32_t)beta - (uint32_t)sp - 1u);
-        }
-      }
-      out.put(i, v);
+      const double v = as_f64(stored);
+      out.put(i, recover ? elf_recover(v, beta) : v);
     }
   }
   s.v = stored;
@@ -333,6 +335,8 @@ __host__ __device__ __forceinline__ uint32_t xor_run(uint32_t codec, XorState &s
 }
 
 // ------------------------------------------------ lockstep segment runs
+constexpr uint32_t kRun = 16;  // records of a run (between two flushes of the output stage)
+
 // The segment kernel's form of the decoders above: the 64 lanes of a wave
 // decode records a + k (k < cnt) of their own segments in one loop whose
 // trip count cnt is the same on every lane, so the body is straight-line
@@ -436,13 +440,159 @@ __host__ __device__ __forceinline__ uint32_t chimp_seg(XorState &s, uint32_t siz
   return e;
 }
 
+// Record 0 of an Elf chunk: prefix, then the ElfXor first value (7-bit
+// trailing zeros, 63 - tz bits), decoder/chunk_reader/elf.rs:105-140.
+template <class Sink>
+__host__ __device__ __forceinline__ uint32_t elf_seg_first(XorState &s, uint32_t size_bits, const Sink &out) {
+  BitWindow &r = s.r;
+  const uint32_t p2 = (uint32_t)(r.hi >> 62);
+  const uint32_t pre = (p2 & 2) ? (p2 == 2 ? 2u : 6u) : 1u;
+  if (p2 == 3) s.beta = (int32_t)((r.hi >> 58) & 15);
+  const uint32_t tz = (uint32_t)r.field(pre, 7);
+  uint32_t used = pre + 7;
+  s.v = 0;
+  if (tz < 64) {
+    const uint32_t len = 63 - tz;
+    const uint64_t m = len ? r.field(pre + 7, len) : 0;
+    s.v = ((m << 1) | 1) << tz;
+    used += len;
+  }
+  r.consume(used);
+  r.refill();
+  const double v = as_f64(s.v);
+  out.row(0, 0, p2 != 2 ? elf_recover(v, s.beta) : v);
+  if (r.pos > size_bits) return kErrOverrun;
+  return s.v == kNan ? kErrCount : kErrNone;  // n >= 1: record 0 is not the terminator
+}
+
+// Elf, decoder/chunk_reader/elf.rs:46-258 (the record step of elf_run for
+// records i >= 1).  Only records the lane takes part in reach the sink.
+template <class Sink>
+__host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
+                                            uint32_t cnt, uint32_t from, uint32_t lim, const Sink &out,
+                                            uint32_t e) {
+  BitWindow &r = s.r;
+  uint64_t stored = s.v;
+  uint32_t L = s.a, T = s.b;
+  int32_t beta = s.beta;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t i = a + k;
+    const bool act = e == 0 && i >= from && i < lim;
+    // prefix (0: erased, reuse beta*; 10: not erased; 11 + 4 bits: erased,
+    // new beta*), then the 2-bit XOR flag and a 9-bit header, all within
+    // the window's first 17 bits
+    const uint32_t h32 = (uint32_t)(r.hi >> 32);
+    const uint32_t p2 = h32 >> 30;
+    const uint32_t pre = (p2 & 2) ? (p2 == 2 ? 2u : 6u) : 1u;
+    const int32_t nbeta = p2 == 3 ? (int32_t)((h32 >> 26) & 15) : beta;
+    const uint32_t f = (h32 >> (30 - pre)) & 3;
+    const uint32_t h9 = (h32 >> (21 - pre)) & 0x1ff, h7 = h9 >> 2;
+    // 11: lead code + 6-bit center; 10: lead code + 4-bit center; 00: the
+    // stored L, T; 01: same value
+    const bool hdr = f >= 2;
+    const uint32_t nL = f == 3 ? lead_repr(h9 >> 6) : f == 2 ? lead_repr(h7 >> 4) : L;
+    const uint32_t c = f == 3 ? ((h9 & 63) ? (h9 & 63) : 64u) : ((h7 & 15) ? (h7 & 15) : 16u);
+    const bool bad = hdr ? nL + c > 64 : f == 0 && (uint64_t)L + T > 64;
+    const uint32_t nT = hdr ? (64 - nL - c) & 63 : T;
+    const uint32_t len = hdr ? c - 1 : f == 0 ? 64 - L - T : 0u;
+    const uint32_t off = pre + (f == 3 ? 11u : f == 2 ? 9u : 2u);
+    const uint64_t m = len ? r.field(off, len) : 0;
+    const uint64_t x = f == 0 ? m << (T & 63) : ((m << 1) | 1) << nT;
+    const uint64_t val = x ^ stored;
+    const bool term = f != 1 && val == kNan;
+    const bool upd = act && !bad;
+    stored = upd && f != 1 && !term ? val : stored;
+    L = upd ? nL : L;
+    T = upd ? nT : T;
+    beta = upd ? nbeta : beta;
+    r.consume(upd ? (f == 1 ? pre + 2 : off + len) : 0u);
+    r.refill();
+    const uint32_t code = bad                 ? (uint32_t)kErrCorrupt
+                          : r.pos > size_bits ? (uint32_t)kErrOverrun
+                          : (i == n) != term  ? (uint32_t)kErrCount
+                                              : 0u;
+    e = act ? code : e;
+    const double v = as_f64(stored);
+    const double w = p2 != 2 ? elf_recover(v, beta) : v;
+    if (act) out.row(i, k, w);
+  }
+  s.v = stored;
+  s.a = L;
+  s.b = T;
+  s.beta = beta;
+  return e;
+}
+
+// Chimp128 parse, decoder/chunk_reader/chimp_n.rs:173-285, in the lockstep
+// form.  Record i is parsed as (source distance, XOR payload): value_i =
+// value_{i - dist} ^ x.  Flags 10 / 11 use the previous record (dist 1);
+// flags 00 / 01 name ring slot idx, i.e. the latest record j <= i - 1 with
+// j == idx (mod 128), so dist = i - j is in [1, 128].  dist 0: no source --
+// record 0 (x = its 64 raw bits) or an unwritten ring slot (value 0) -- so
+// value_i = x.  No values are formed here (that needs the ring): the
+// resolve pass does it.  Record i < n goes to the sink as its payload and
+// to byte k of dist16 (k = i - a < kRun) as its distance; the terminator n
+// goes to *term.  Record 0 is parsed before, by chimp128_seg_first.
+template <class Sink>
+__host__ __device__ __forceinline__ uint32_t chimp128_seg_first(BitWindow &r, uint32_t size_bits, uint32_t n,
+                                                       const Sink &out, uint32_t (&dist16)[4], C128Term *term) {
+  const uint64_t x = r.field(0, 64);
+  r.consume(64);
+  r.refill();
+  if (n) {
+    out.row(0, 0, as_f64(x));
+    dist16[0] &= ~0xffu;
+  } else {
+    *term = C128Term{x, 0, 0};
+  }
+  return r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
+}
+
+template <class Sink>
+__host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, uint32_t &lead, uint32_t size_bits,
+                                                       uint32_t n, uint32_t a, uint32_t cnt, uint32_t from,
+                                                       uint32_t lim, const Sink &out, uint32_t (&dist16)[4],
+                                                       C128Term *term, uint32_t e) {
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t i = a + k;
+    const bool act = e == 0 && i >= from && i < lim;
+    // 00: 7-bit ring index; 11: lead code, 64 - lead bits; 10: 64 - lead
+    // bits with the stored lead; 01: 16-bit header (index, lead code, sig)
+    const uint32_t f = (uint32_t)(r.hi >> 62);
+    const uint32_t h = (uint32_t)(r.hi >> 46) & 0xffff;  // bits 2..17
+    const uint32_t idx = h >> 9;
+    const uint32_t nl = f == 3 ? lead_repr(h >> 13) : f == 1 ? lead_repr((h >> 6) & 7) : lead;
+    const uint32_t len = f == 1 ? ((h & 63) ? (h & 63) : 64u) : 64 - nl;
+    const bool bad = (f == 2 && lead > 64) || (f == 1 && len + nl > 64);
+    const uint32_t off = f == 3 ? 5u : f == 1 ? 18u : 2u;
+    const uint64_t x = f == 0 ? 0 : r.field(off, len) << (f == 1 ? (64 - len - nl) & 63 : 0u);
+    const uint32_t back = f <= 1 ? (((i - 1) - idx) & 127u) + 1 : 1u;  // i - source
+    const uint32_t dist = back > i ? 0u : back;
+    const bool upd = act && !bad;
+    lead = upd && f != 0 ? nl : lead;
+    r.consume(upd ? (f == 0 ? 9u : off + len) : 0u);
+    r.refill();
+    const uint32_t code = bad ? (uint32_t)kErrCorrupt : r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
+    e = act ? code : e;
+    if (act && i < n) out.row(i, k, as_f64(x));
+    const uint32_t sh = (k & 3) * 8, w = k >> 2;
+    const bool put = act && i < n && k < kRun;
+    const uint32_t keep = ~(0xffu << sh), byte = dist << sh;
+    dist16[0] = put && w == 0 ? (dist16[0] & keep) | byte : dist16[0];
+    dist16[1] = put && w == 1 ? (dist16[1] & keep) | byte : dist16[1];
+    dist16[2] = put && w == 2 ? (dist16[2] & keep) | byte : dist16[2];
+    dist16[3] = put && w == 3 ? (dist16[3] & keep) | byte : dist16[3];
+    if (act && i == n) *term = C128Term{x, dist, 0};
+  }
+  return e;
+}
+
 // Output staging of a wave decoding 64 checkpoint segments (segment
 // first + l on lane l) in runs of kRun = 16 records: record i of lane l
 // goes to slot (i mod 16, l ^ stage_swz(i mod 16)) of the wave's [16][64]
 // array; the XOR swizzle keeps both the per-record writes (one row, all
 // lanes) and the flush reads (pairs of rows, 8 lanes) free of LDS bank
 // conflicts beyond the two passes a 64-bit access takes.
-constexpr uint32_t kRun = 16;
 __device__ __forceinline__ uint32_t stage_swz(uint32_t r) { return (r >> 1) << 2; }
 
 struct StageSink {
@@ -604,53 +754,6 @@ __host__ __device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_b
   }
   return kErrNone;
 }
-// Chimp128 record i as (source distance, XOR payload): value_i =
-// value_{i - dist} ^ x.  Flags 10 / 11 use the previous record (dist 1);
-// flags 00 / 01 name ring slot idx, i.e. the latest record j <= i - 1 with
-// j == idx (mod 128), so dist = i - j is in [1, 128].  dist 0: no source --
-// record 0 (x = its 64 raw bits) or an unwritten ring slot (value 0) -- so
-// value_i = x.
-//
-// Parses records [i0, i1) from window r (stored lead `lead`): record i < n
-// goes to out.put(i, x) and byte i % 16 of dist16 (a run of at most 16
-// records that starts at a multiple of 16); the terminator record n goes to
-// *term.  No values are formed here (that needs the ring): the resolve pass
-// does it.
-template <class Sink>
-__host__ __device__ __forceinline__ uint32_t chimp128_parse_run(BitWindow &r, uint32_t &lead, uint32_t size_bits, uint32_t n,
-                                                       uint32_t i0, uint32_t i1, const Sink &out,
-                                                       uint32_t (&dist16)[4], C128Term *term) {
-  for (uint32_t i = i0; i < i1; ++i) {
-    uint64_t x;
-    uint32_t dist;
-    if (i == 0) {
-      x = r.field(0, 64);
-      dist = 0;
-      r.consume(64);
-    } else {
-      uint32_t f, idx, used;
-      if (!chimp128_rec(r, lead, f, idx, x, used)) return kErrCorrupt;
-      const uint32_t back = f <= 1 ? (((i - 1) - idx) & 127u) + 1 : 1;  // i - source
-      dist = back > i ? 0 : back;
-      r.consume(used);
-    }
-    r.refill();
-    if (r.pos > size_bits) return kErrOverrun;
-    if (i < n) {
-      out.put(i, as_f64(x));
-      const uint32_t k = i & 15, sh = (k & 3) * 8, w = k >> 2;
-      const uint32_t keep = ~(0xffu << sh), put = dist << sh;
-      dist16[0] = w == 0 ? (dist16[0] & keep) | put : dist16[0];
-      dist16[1] = w == 1 ? (dist16[1] & keep) | put : dist16[1];
-      dist16[2] = w == 2 ? (dist16[2] & keep) | put : dist16[2];
-      dist16[3] = w == 3 ? (dist16[3] & keep) | put : dist16[3];
-    } else {
-      *term = C128Term{x, dist, 0};
-    }
-  }
-  return kErrNone;
-}
-
 // -------------------------------------------------------------- tile codecs
 // Records of tile t handled by this lane: [t*kTile + lane*8, +8) clipped to n.
 

@@ -49,32 +49,23 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
   double *dst = cv.out + c.rec_off + (size_t)first * kSeg;
   const uint32_t n_job = first * kSeg < c.n ? c.n - first * kSeg : 0;  // records from segment `first` on
   const uint32_t bits = c.size * 8u;
-  uint32_t e = 0;
-  if (kCodec == kCodecElf) {
-#pragma unroll 1  // one copy of the codec loop per codec (instruction cache)
-    for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
-      const uint32_t a = i0 + k0, b = min(a + kRun, i1);
-      if (live && e == 0 && a < b) e = xor_run<true>(kCodec, s, bits, c.n, a, b, sink, nullptr);
-      stage_flush(stage, dst, k0, n_job);
-    }
-    return e;
-  }
-  // lockstep runs: a lane without a segment has an empty window that it
-  // never advances (it takes part in no record)
+  // a lane without a segment has an empty window that it never advances
+  // (it takes part in no record)
   if (!live) {
     s.r.b = 0;
     s.r.pos = 0;
     s.r.hi = s.r.lo = 0;
     s.v = 0;
     s.a = s.b = 0;
+    s.beta = 0;
   }
   const uint32_t last = live && i1 == c.n ? c.n + 1 : i1;  // the chunk's last segment parses the terminator
-  uint32_t from = i0;
+  uint32_t from = i0, e = 0;
   if (live && i0 == 0) {
-    e = xor_seg_first(s, bits, sink);
+    e = kCodec == kCodecElf ? elf_seg_first(s, bits, sink) : xor_seg_first(s, bits, sink);
     from = 1;
   }
-#pragma unroll 1
+#pragma unroll 1  // one copy of the codec loop per codec (instruction cache)
   for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
     const uint32_t a = i0 + k0;
     // kRun records, and one more when a lane's terminator follows them
@@ -83,8 +74,10 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
     const uint32_t lim = term_next ? last : min(last, a + kRun);
     if (kCodec == kCodecGorilla)
       e = gorilla_seg(s, bits, c.n, a, cnt, from, lim, sink, e);
-    else
+    else if (kCodec == kCodecChimp)
       e = chimp_seg(s, bits, c.n, a, cnt, from, lim, sink, e);
+    else
+      e = elf_seg(s, bits, c.n, a, cnt, from, lim, sink, e);
     from = max(from, lim);  // a terminator taken as record a + kRun is not parsed again
     stage_flush(stage, dst, k0, n_job);
   }
@@ -103,25 +96,37 @@ __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_
   const uint32_t seg = first + lane_id();
   const uint32_t i0 = seg * kSeg;
   const bool live = i0 < total;
-  const uint32_t i1 = live ? min(i0 + kSeg, total) : 0;
+  const uint32_t lim_seg = live ? min(i0 + kSeg, total) : 0;  // the chunk's last segment holds the terminator
   BitWindow r;
   uint32_t lead = 0xffffffffu;
   if (live) {
     const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[seg];
     r.init_at(cv.ilv + (size_t)row * kIlvRowBlocks + lane_id(), k.bitpos, kIlvRowBlocks);
     lead = k.a == 255 ? 0xffffffffu : k.a;
+  } else {  // an empty window that is never advanced
+    r.b = 0;
+    r.pos = 0;
+    r.hi = r.lo = 0;
   }
   const StageSink sink{stage};
   double *dst = cv.c128_x + (size_t)c.rtile0 * kTile + (size_t)first * kSeg;
   const uint32_t n_job = first * kSeg < c.n ? c.n - first * kSeg : 0;
   uint8_t *dist = cv.c128_dist + (size_t)c.rtile0 * kTile;  // 16-byte aligned, ceil(n / kTile) * kTile bytes
+  C128Term *term = cv.c128_term + ci;
   const uint32_t bits = c.size * 8u;
-  uint32_t e = 0;
+  uint32_t from = i0, e = 0;
+  uint32_t d16[4] = {0, 0, 0, 0};
+  if (live && i0 == 0) {
+    e = chimp128_seg_first(r, bits, c.n, sink, d16, term);
+    from = 1;
+  }
 #pragma unroll 1
   for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
-    const uint32_t a = i0 + k0, b = min(a + kRun, i1);
-    uint32_t d16[4] = {0, 0, 0, 0};
-    if (live && e == 0 && a < b) e = chimp128_parse_run(r, lead, bits, c.n, a, b, sink, d16, cv.c128_term + ci);
+    const uint32_t a = i0 + k0;
+    // a segment ends at a multiple of kSeg or at the terminator, so a run
+    // never needs a record past a + kRun - 1
+    const uint32_t lim = min(lim_seg, a + kRun);
+    e = chimp128_parse_seg(r, lead, bits, c.n, a, kRun, from, lim, sink, d16, term, e);
     stage_flush(stage, dst, k0, n_job);
     // a < n: the 16 bytes from a lie inside the chunk's distance region
     if (live && a < c.n) *reinterpret_cast<uint4 *>(dist + a) = make_uint4(d16[0], d16[1], d16[2], d16[3]);

@@ -107,7 +107,7 @@ static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t b
       from[l] = i0[l];
       e[l] = 0;
       if (live && i0[l] == 0) {
-        e[l] = xor_seg_first(st[l], bits, sink);
+        e[l] = codec == kCodecElf ? elf_seg_first(st[l], bits, sink) : xor_seg_first(st[l], bits, sink);
         from[l] = 1;
       }
     }
@@ -119,7 +119,8 @@ static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t b
         const uint32_t a = i0[l] + k0;
         const uint32_t lim = last[l] == a + kRun + 1 ? last[l] : std::min(last[l], a + kRun);
         e[l] = codec == kCodecGorilla ? gorilla_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l])
-                                      : chimp_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l]);
+               : codec == kCodecChimp ? chimp_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l])
+                                      : elf_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l]);
         from[l] = std::max(from[l], lim);
       }
     }
@@ -218,7 +219,7 @@ int main() {
             for (uint32_t i = 0; i < cn; ++i)
               expect(as_u64(out[i]) == as_u64(want[i]), "segment value", codec, g, chunk, k, i);
             const Ilv ilv = build_ilv(data, bits, ck.data(), nseg);
-            if (codec == kCodecElf) {  // the per-record form through the ilv
+            if (codec == kCodecElf) {  // the per-record form through the ilv (the lockstep form follows)
               std::fill(out.begin(), out.end(), 0.0);
               for (uint32_t seg = 0; seg < nseg; ++seg) {
                 const uint32_t i0 = seg * kSeg, i1 = std::min(i0 + kSeg, cn);
@@ -233,7 +234,8 @@ int main() {
               }
               for (uint32_t i = 0; i < cn; ++i)
                 expect(as_u64(out[i]) == as_u64(want[i]), "ilv segment value", codec, g, chunk, k, i);
-            } else {
+            }
+            {
               std::fill(out.begin(), out.end(), 0.0);
               const std::vector<uint32_t> le = lockstep(codec, data, bits, cn, ck.data(), nseg, out.data(), &ilv);
               for (uint32_t sg = 0; sg < nseg; ++sg)
@@ -271,20 +273,40 @@ int main() {
             expect(e == kErrNone, "Chimp128 index error", codec, g, chunk, k, 0);
             std::vector<uint8_t> dist((cn + kTile - 1) / kTile * kTile + 16, 0);
             C128Term term{};
-            const Ilv ilv = build_ilv(data, bits, ck.data(), nseg);  // the kernel parses through the ilv
-            for (uint32_t seg = 0; seg < nseg; ++seg) {
-              const uint32_t i0 = seg * kSeg, i1 = std::min(i0 + kSeg, cn + 1);
-              BitWindow r;
-              r.init_at(ilv.lane(seg), ck[seg].bitpos, kIlvRowBlocks);
-              uint32_t lead = ck[seg].a == 255 ? 0xffffffffu : ck[seg].a;
-              for (uint32_t k0 = 0; k0 < kSeg && e == kErrNone; k0 += kRun) {
-                const uint32_t a = i0 + k0, b = std::min(a + kRun, i1);
-                uint32_t d16[4] = {0, 0, 0, 0};
-                if (a < b) e = chimp128_parse_run(r, lead, bits, cn, a, b, sink, d16, &term);
-                if (a < cn) std::memcpy(dist.data() + a, d16, 16);
+            // the kernel's lockstep parse through the ilv, wave by wave
+            const Ilv ilv = build_ilv(data, bits, ck.data(), nseg);
+            const RowSink xsink{out.data(), cn};
+            for (uint32_t first = 0; first < nseg; first += kWave) {
+              BitWindow r[kWave];
+              uint32_t lead[kWave], from[kWave], lim_seg[kWave], d16[kWave][4], ee[kWave];
+              for (uint32_t l = 0; l < kWave; ++l) {
+                const uint32_t seg = first + l, i0 = seg * kSeg;
+                const bool live = seg < nseg;
+                lim_seg[l] = live ? std::min(i0 + kSeg, cn + 1) : 0;
+                lead[l] = live && ck[seg].a != 255 ? ck[seg].a : 0xffffffffu;
+                r[l] = BitWindow{};
+                r[l].p = nullptr;
+                if (live) r[l].init_at(ilv.lane(seg), ck[seg].bitpos, kIlvRowBlocks);
+                std::memset(d16[l], 0, 16);
+                from[l] = i0;
+                ee[l] = 0;
+                if (live && i0 == 0) {
+                  ee[l] = chimp128_seg_first(r[l], bits, cn, xsink, d16[l], &term);
+                  from[l] = 1;
+                }
               }
-              expect(e == kErrNone, "Chimp128 parse error", codec, g, chunk, k, i0);
-              expect(within_lane(ilv, seg, r), "Chimp128 ilv lane overread", codec, g, chunk, k, i0);
+              for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun)
+                for (uint32_t l = 0; l < kWave; ++l) {
+                  const uint32_t a = (first + l) * kSeg + k0;
+                  ee[l] = chimp128_parse_seg(r[l], lead[l], bits, cn, a, kRun, from[l], std::min(lim_seg[l], a + kRun),
+                                             xsink, d16[l], &term, ee[l]);
+                  if (lim_seg[l] && a < cn) std::memcpy(dist.data() + a, d16[l], 16);
+                }
+              for (uint32_t l = 0; l < kWave && first + l < nseg; ++l) {
+                expect(ee[l] == kErrNone, "Chimp128 parse error", codec, g, chunk, k, (first + l) * kSeg);
+                expect(within_lane(ilv, first + l, r[l]), "Chimp128 ilv lane overread", codec, g, chunk, k,
+                       (first + l) * kSeg);
+              }
             }
             std::vector<uint64_t> val(cn + 1, 0);
             for (uint32_t i = 0; i <= cn; ++i) {
