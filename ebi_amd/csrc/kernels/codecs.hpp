@@ -82,6 +82,68 @@ __host__ __device__ __forceinline__ double elf_recover(double v, int32_t beta) {
   return elf_round_up(v, (uint32_t)beta - (uint32_t)sp - 1u);
 }
 
+// elf_recover with the powers of ten in a small table, for the segment
+// kernel (which keeps it in LDS).  tab[k + 10] = 1e{k} for k in -10..9
+// (get_sp's thresholds), tab[20 + a] = 1e{a} and tab[43 + a] = 1e-{a} for
+// a in 0..22, all the reference's parses of "1.0E{k}" (elf_tab_fill).
+// Results equal elf_recover's for every input:
+//  * get_sp: for x = |v| in [1e-10, 1e9) the table walk returns the largest
+//    k with x >= 1e{k}.  x in [2^e, 2^{e+1}) puts log10(x) in
+//    [e log10(2), (e + 1) log10(2)), an interval shorter than 1, so that k
+//    is k0 = floor(e log10(2)) or k0 + 1: one compare with 1e{k0 + 1}.
+//    floor(e log10(2)) = (e * 1233) >> 12 for the e of that range;
+//  * round_up's division N / D, D = 10^alpha exact (alpha <= 22), N a
+//    nonzero integer-valued double below 1e31 in magnitude: with y = 1e-alpha
+//    = RN(1/D), q0 = RN(N y) is within 1.5 ulp of N / D, one Newton
+//    correction q1 = RN(q0 + RN(N - q0 D) y) within one ulp, and the second,
+//    q2 = RN(q1 + r1 y) with r1 = N - q1 D exact by FMA, is RN(N / D) by
+//    Markstein's theorem (P. Markstein, "Computation of elementary functions
+//    on the IBM RISC System/6000 processor", IBM J. Res. Dev. 34(1), 1990).
+// Other inputs (x outside [1e-10, 1e9), alpha > 22, beta* = 0 with a power
+// past 1e-22) take elf_recover's path.
+constexpr uint32_t kElfTab = 66;
+
+// RN(N / D) for D = 10^a exact (a <= 22), y = RN(1 / D), N a nonzero
+// integer-valued double with |N| < 1e31 (the two corrections above).
+__host__ __device__ __forceinline__ double div_pow10(double N, double D, double y) {
+  const double q0 = N * y;
+  const double q1 = fma(fma(-q0, D, N), y, q0);
+  return fma(fma(-q1, D, N), y, q1);
+}
+
+__host__ __device__ __forceinline__ void elf_tab_fill(double *tab, uint32_t i, const double *p10, const double *n10) {
+  if (i < 20) tab[i] = i >= 10 ? p10[i - 10] : n10[10 - i];
+  else if (i < 43) tab[i] = p10[i - 20];
+  else if (i < kElfTab) tab[i] = n10[i - 43];
+}
+
+__host__ __device__ __forceinline__ double elf_recover_tab(double v, int32_t beta, const double *tab) {
+  const double x = fabs(v);
+  int32_t sp;
+  if (x >= tab[0] && x < 1e9) {
+    const int32_t e = (int32_t)((as_u64(x) >> 52) & 0x7ff) - 1023;
+    const int32_t k0 = (e * 1233) >> 12;  // floor(e log10(2)); k0 + 1 in [-10, 9]
+    sp = k0 + (x >= tab[k0 + 11] ? 1 : 0);
+  } else {
+    sp = elf_get_sp(x);
+  }
+  if (beta == 0) {
+    const uint32_t k = 0u - (uint32_t)sp - 1u;
+    double w = 0.0;
+    if (k <= 22) w = tab[43 + k];
+    else if (k < 400) w = c_n10[k];
+    return v < 0.0 ? -w : w;
+  }
+  const uint32_t alpha = (uint32_t)beta - (uint32_t)sp - 1u;
+  if (alpha <= 22 && x >= tab[0] && x < 1e9) {
+    const double D = tab[20 + alpha], y = tab[43 + alpha];
+    const double t = v * D;
+    const double N = v < 0.0 ? floor(t) : ceil(t);
+    return div_pow10(N, D, y);
+  }
+  return elf_round_up(v, alpha);
+}
+
 // State of one XOR stream between calls of its decoder: the bit window and
 // the values the next record depends on (Gorilla: v = previous value, a = L,
 // b = T; Chimp: v, a = stored lead; Elf: v = stored bits, a = L, b = T,
@@ -443,7 +505,8 @@ __host__ __device__ __forceinline__ uint32_t chimp_seg(XorState &s, uint32_t siz
 // Record 0 of an Elf chunk: prefix, then the ElfXor first value (7-bit
 // trailing zeros, 63 - tz bits), decoder/chunk_reader/elf.rs:105-140.
 template <class Sink>
-__host__ __device__ __forceinline__ uint32_t elf_seg_first(XorState &s, uint32_t size_bits, const Sink &out) {
+__host__ __device__ __forceinline__ uint32_t elf_seg_first(XorState &s, uint32_t size_bits, const Sink &out,
+                                                  const double *tab) {
   BitWindow &r = s.r;
   const uint32_t p2 = (uint32_t)(r.hi >> 62);
   const uint32_t pre = (p2 & 2) ? (p2 == 2 ? 2u : 6u) : 1u;
@@ -460,17 +523,18 @@ __host__ __device__ __forceinline__ uint32_t elf_seg_first(XorState &s, uint32_t
   r.consume(used);
   r.refill();
   const double v = as_f64(s.v);
-  out.row(0, 0, p2 != 2 ? elf_recover(v, s.beta) : v);
+  out.row(0, 0, p2 != 2 ? elf_recover_tab(v, s.beta, tab) : v);
   if (r.pos > size_bits) return kErrOverrun;
   return s.v == kNan ? kErrCount : kErrNone;  // n >= 1: record 0 is not the terminator
 }
 
 // Elf, decoder/chunk_reader/elf.rs:46-258 (the record step of elf_run for
 // records i >= 1).  Only records the lane takes part in reach the sink.
+// tab: the powers of ten of elf_recover_tab.
 template <class Sink>
 __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
                                             uint32_t cnt, uint32_t from, uint32_t lim, const Sink &out,
-                                            uint32_t e) {
+                                            uint32_t e, const double *tab) {
   BitWindow &r = s.r;
   uint64_t stored = s.v;
   uint32_t L = s.a, T = s.b;
@@ -513,7 +577,7 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
                                               : 0u;
     e = act ? code : e;
     const double v = as_f64(stored);
-    const double w = p2 != 2 ? elf_recover(v, beta) : v;
+    const double w = p2 != 2 ? elf_recover_tab(v, beta, tab) : v;
     if (act) out.row(i, k, w);
   }
   s.v = stored;

@@ -35,7 +35,7 @@ __device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_
 // or stopped by an error keep taking part in the stores.
 template <uint32_t kCodec>
 __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChunk &c, uint32_t first, uint32_t row,
-                                                double *stage) {
+                                                double *stage, const double *elf_tab) {
   const uint32_t i0 = (first + lane_id()) * kSeg;
   const bool live = i0 < c.n;
   const uint32_t i1 = live ? min(i0 + kSeg, c.n) : 0;
@@ -62,7 +62,7 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
   const uint32_t last = live && i1 == c.n ? c.n + 1 : i1;  // the chunk's last segment parses the terminator
   uint32_t from = i0, e = 0;
   if (live && i0 == 0) {
-    e = kCodec == kCodecElf ? elf_seg_first(s, bits, sink) : xor_seg_first(s, bits, sink);
+    e = kCodec == kCodecElf ? elf_seg_first(s, bits, sink, elf_tab) : xor_seg_first(s, bits, sink);
     from = 1;
   }
 #pragma unroll 1  // one copy of the codec loop per codec (instruction cache)
@@ -77,7 +77,7 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
     else if (kCodec == kCodecChimp)
       e = chimp_seg(s, bits, c.n, a, cnt, from, lim, sink, e);
     else
-      e = elf_seg(s, bits, c.n, a, cnt, from, lim, sink, e);
+      e = elf_seg(s, bits, c.n, a, cnt, from, lim, sink, e, elf_tab);
     from = max(from, lim);  // a terminator taken as record a + kRun is not parsed again
     stage_flush(stage, dst, k0, n_job);
   }
@@ -135,12 +135,12 @@ __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_
 }
 
 __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, uint32_t first, uint32_t row,
-                                               double *stage) {
+                                               double *stage, const double *elf_tab) {
   const DevChunk c = cv.chunks[ci];
   switch (c.codec) {
-    case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, row, stage);
-    case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, row, stage);
-    case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, row, stage);
+    case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, row, stage, elf_tab);
+    case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, row, stage, elf_tab);
+    case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, row, stage, elf_tab);
     case kCodecChimp128: return segment_wave_c128(cv, ci, c, first, row, stage);
     default: return kErrCorrupt;
   }
@@ -200,6 +200,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_decode_seg(
     const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
   __shared__ __align__(16) double lds[4][(kRun + 1) * kWave];
+  __shared__ double elf_tab[kElfTab];
+  if (threadIdx.x < kElfTab) elf_tab_fill(elf_tab, threadIdx.x, c_p10, c_n10);
+  __syncthreads();
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
   uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
@@ -208,7 +211,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const ColView cv = cols[ci];
     const Job jb = cv.seg_jobs[j - prefix[ci]];
     if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-      const uint32_t e = segment_job(cv, jb.chunk, jb.tile, jb.col, lds[wave]);
+      const uint32_t e = segment_job(cv, jb.chunk, jb.tile, jb.col, lds[wave], elf_tab);
       if (e) atomicOr(err, e);
     }
     j += stride;

@@ -2,7 +2,9 @@
 // reference's table walk of compression_common/elf.rs:111-130 over tables
 // parsed with strtod exactly as the reference builds them (elf.rs:95-109),
 // on every power of ten and its two neighbours, special values and random
-// magnitudes; p10_exact(a) against the parsed "1.0E{a}" for a <= 22.
+// magnitudes; p10_exact(a) against the parsed "1.0E{a}" for a <= 22; the
+// table form of the Elf value recovery (elf_recover_tab) against
+// elf_recover, and its division (div_pow10) against IEEE division.
 // Exit code 0 on success.
 #include <cmath>
 #include <cstdio>
@@ -58,6 +60,47 @@ int main() {
     if (a != b) {
       if (fails < 10) std::printf("get_sp(%.17g) = %d, table walk %d\n", v, a, b);
       ++fails;
+    }
+  }
+  // the table form of the value recovery against elf_recover, on the same
+  // magnitudes with both signs and every beta*, over the library's tables
+  for (int i = 0; i < 400; ++i) {
+    ebi::c_p10[i] = p10[i];
+    ebi::c_n10[i] = n10[i];
+  }
+  double tab[ebi::kElfTab];
+  for (uint32_t i = 0; i < ebi::kElfTab; ++i) ebi::elf_tab_fill(tab, i, p10, n10);
+  for (double x : vs)
+    for (double v : {x, -x})
+      for (int32_t beta = 0; beta < 16; ++beta) {
+        ++checks;
+        const double a = ebi::elf_recover(v, beta), b = ebi::elf_recover_tab(v, beta, tab);
+        if (ebi::as_u64(a) != ebi::as_u64(b) && !(a != a && b != b)) {
+          if (fails < 10) std::printf("recover(%.17g, %d) = %.17g, table form %.17g\n", v, beta, a, b);
+          ++fails;
+        }
+      }
+  // div_pow10 against IEEE division: every integer up to 2e5, random
+  // integers below 2^53, random integer-valued doubles up to 1e31
+  std::uniform_int_distribution<uint64_t> i53(1, (1ull << 53) - 1);
+  std::uniform_real_distribution<double> big(0.0, 31.0 * 3.321928094887362);
+  for (uint32_t a = 0; a <= 22; ++a) {
+    const double D = p10[a], y = n10[a];
+    auto one = [&](double N) {
+      for (double M : {N, -N}) {
+        ++checks;
+        const double q = ebi::div_pow10(M, D, y), w = M / D;
+        if (ebi::as_u64(q) != ebi::as_u64(w)) {
+          if (fails < 10) std::printf("div_pow10(%.17g, 1e%u) = %.17g, IEEE %.17g\n", M, a, q, w);
+          ++fails;
+        }
+      }
+    };
+    for (uint32_t N = 1; N <= 200000; ++N) one((double)N);
+    for (int i = 0; i < 200000; ++i) one((double)i53(rng));
+    for (int i = 0; i < 200000; ++i) {
+      const double N = std::floor(std::exp2(big(rng)));
+      if (N >= 1.0) one(N);
     }
   }
   std::printf("%s: %ld checks, %ld failures\n", fails ? "FAIL" : "OK", checks, fails);

@@ -25,6 +25,7 @@
 using namespace ebi;
 
 static long fails = 0, checks = 0;
+static double g_elf_tab[kElfTab];  // the segment kernel's LDS table of powers of ten
 
 static void expect(bool ok, const char *what, int codec, int gen, uint64_t chunk, uint64_t k, uint64_t i) {
   ++checks;
@@ -107,7 +108,7 @@ static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t b
       from[l] = i0[l];
       e[l] = 0;
       if (live && i0[l] == 0) {
-        e[l] = codec == kCodecElf ? elf_seg_first(st[l], bits, sink) : xor_seg_first(st[l], bits, sink);
+        e[l] = codec == kCodecElf ? elf_seg_first(st[l], bits, sink, g_elf_tab) : xor_seg_first(st[l], bits, sink);
         from[l] = 1;
       }
     }
@@ -120,7 +121,7 @@ static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t b
         const uint32_t lim = last[l] == a + kRun + 1 ? last[l] : std::min(last[l], a + kRun);
         e[l] = codec == kCodecGorilla ? gorilla_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l])
                : codec == kCodecChimp ? chimp_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l])
-                                      : elf_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l]);
+                                      : elf_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l], g_elf_tab);
         from[l] = std::max(from[l], lim);
       }
     }
@@ -160,6 +161,7 @@ int main() {
     std::snprintf(buf, sizeof buf, "1.0E-%d", i);
     c_n10[i] = std::strtod(buf, nullptr);
   }
+  for (uint32_t i = 0; i < kElfTab; ++i) elf_tab_fill(g_elf_tab, i, c_p10, c_n10);
   struct Gen {
     int kind, dec;
     double p0, p1;
