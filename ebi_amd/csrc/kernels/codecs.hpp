@@ -547,20 +547,22 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
     // the window's first 17 bits
     const uint32_t h32 = (uint32_t)(r.hi >> 32);
     const uint32_t p2 = h32 >> 30;
-    const uint32_t pre = (p2 & 2) ? (p2 == 2 ? 2u : 6u) : 1u;
+    const uint32_t pre = p2 == 3 ? 6u : p2 == 2 ? 2u : 1u;
     const int32_t nbeta = p2 == 3 ? (int32_t)((h32 >> 26) & 15) : beta;
     const uint32_t f = (h32 >> (30 - pre)) & 3;
-    const uint32_t h9 = (h32 >> (21 - pre)) & 0x1ff, h7 = h9 >> 2;
-    // 11: lead code + 6-bit center; 10: lead code + 4-bit center; 00: the
-    // stored L, T; 01: same value
-    const bool hdr = f >= 2;
-    const uint32_t nL = f == 3 ? lead_repr(h9 >> 6) : f == 2 ? lead_repr(h7 >> 4) : L;
-    const uint32_t c = f == 3 ? ((h9 & 63) ? (h9 & 63) : 64u) : ((h7 & 15) ? (h7 & 15) : 16u);
-    const bool bad = hdr ? nL + c > 64 : f == 0 && (uint64_t)L + T > 64;
+    const uint32_t h9 = (h32 >> (21 - pre)) & 0x1ff;
+    // 11: 3-bit lead code + 6-bit center; 10: 3-bit lead code + 4-bit
+    // center; 00: the stored L, T; 01: same value.  Plain selects (the
+    // compiler keeps them branch-free).
+    const bool f3 = f == 3, hdr = f >= 2, f0 = f == 0;
+    const uint32_t nL = hdr ? lead_repr(h9 >> 6) : L;
+    const uint32_t cb = f3 ? h9 & 63 : (h9 >> 2) & 15;
+    const uint32_t c = cb ? cb : (f3 ? 64u : 16u);
+    const bool bad = (hdr & (nL + c > 64)) | (f0 & ((uint64_t)L + T > 64));
     const uint32_t nT = hdr ? (64 - nL - c) & 63 : T;
-    const uint32_t len = hdr ? c - 1 : f == 0 ? 64 - L - T : 0u;
-    const uint32_t off = pre + (f == 3 ? 11u : f == 2 ? 9u : 2u);
-    const uint64_t m = len ? r.field(off, len) : 0;
+    const uint32_t len = hdr ? c - 1 : (f0 ? 64 - L - T : 0u);
+    const uint32_t off = pre + (f3 ? 11u : hdr ? 9u : 2u);
+    const uint64_t m = r.field(off, len ? len : 1u) & (len ? ~0ull : 0ull);
     const uint64_t x = f == 0 ? m << (T & 63) : ((m << 1) | 1) << nT;
     const uint64_t val = x ^ stored;
     const bool term = f != 1 && val == kNan;
@@ -625,11 +627,13 @@ __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, ui
     const uint32_t f = (uint32_t)(r.hi >> 62);
     const uint32_t h = (uint32_t)(r.hi >> 46) & 0xffff;  // bits 2..17
     const uint32_t idx = h >> 9;
-    const uint32_t nl = f == 3 ? lead_repr(h >> 13) : f == 1 ? lead_repr((h >> 6) & 7) : lead;
-    const uint32_t len = f == 1 ? ((h & 63) ? (h & 63) : 64u) : 64 - nl;
-    const bool bad = (f == 2 && lead > 64) || (f == 1 && len + nl > 64);
+    const uint32_t lc = lead_repr(f == 3 ? h >> 13 : (h >> 6) & 7);  // flags 11 and 01 carry a lead code
+    const uint32_t nl = (f & 1) ? lc : lead;
+    const uint32_t s6 = h & 63;
+    const uint32_t len = f == 1 ? (s6 ? s6 : 64u) : 64 - nl;
+    const bool bad = ((f == 2) & (lead > 64)) | ((f == 1) & (len + nl > 64));
     const uint32_t off = f == 3 ? 5u : f == 1 ? 18u : 2u;
-    const uint64_t x = f == 0 ? 0 : r.field(off, len) << (f == 1 ? (64 - len - nl) & 63 : 0u);
+    const uint64_t x = r.field(off, len) << (f == 1 ? (64 - len - nl) & 63 : 0u) & (f ? ~0ull : 0ull);
     const uint32_t back = f <= 1 ? (((i - 1) - idx) & 127u) + 1 : 1u;  // i - source
     const uint32_t dist = back > i ? 0u : back;
     const bool upd = act && !bad;
