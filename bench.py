@@ -260,15 +260,16 @@ def query_leg(E, ctx, cols, sample_q, n, reps=20):
     """In-situ query latency on the resident 2^log2n random-walk columns:
     range-filter + sum (greater-than at the (1 - s) quantile, s = 0.1, 1, 10,
     50 %) and max, one synchronous query at a time; BUFF and DeltaSprintz run
-    on their codes, Gorilla decodes first.  Latency is host wall time of the
-    call; GB/s counts compressed bytes + 2 x ceil(N/8) (bitset written and
-    read) + 8."""
+    on their codes, Gorilla, Chimp and Elf decode and query in one kernel
+    (records stay in registers), Chimp128 decodes into HBM first.  Latency is
+    host wall time of the call; GB/s counts compressed bytes + 2 x ceil(N/8)
+    (bitset written and read) + 8."""
     out = {}
     sq = sample_q.get("walk2")
     if sq is None:
         return None
     for g, codec, d, cb in cols:
-        if g != "walk2" or codec not in ("buff", "delta_sprintz", "gorilla"):
+        if g != "walk2" or codec not in ("buff", "delta_sprintz", "gorilla", "chimp", "elf", "chimp128"):
             continue
         res = {}
         for s in (0.001, 0.01, 0.1, 0.5):

@@ -203,9 +203,27 @@ hipError_t plan_enqueue_pre(Plan &p, hipStream_t s) {
 }
 
 // Async part after the decode: query kernels and small results to pinned.
-hipError_t plan_enqueue_post(Plan &p, hipStream_t s) {
+// err: the group's device error flags; max_blocks: the persistent grid.
+hipError_t plan_enqueue_post(Plan &p, hipStream_t s, uint32_t *err, uint32_t max_blocks) {
   Column &c = *p.c;
   const int op = p.q.op;
+  if (p.fused) {
+    // decode + filter / aggregate in one pass; afterwards the same chunk
+    // selection, per-chunk fold and copies as below
+    QueryArgs a = args_of(p);
+    const int agg = op == kOpSum || op == kOpFilterSum ? 0 : op == kOpMin ? 1 : op == kOpMax ? 2 : -1;
+    launch::query_seg(s, p.d_view, c.n_seg_jobs, a, has_pred(op), writes_bits(p), agg, p.nc, err, max_blocks);
+    if (op == kOpFilter) {
+      if (p.words) hipMemcpyAsync(p.h_bits, p.bits, p.words * 8, hipMemcpyDeviceToHost, s);
+      return hipGetLastError();
+    }
+    if (op == kOpFilterSum) launch::chunk_select(s, p.d_view, p.bits, p.q.chunk_id, p.sel, p.nc);
+    if (agg == 0) launch::chunk_dd(s, p.d_view, p.sel, p.partials, p.chunk_sum, p.nc);
+    hipMemcpyAsync(p.h_sel, p.sel, (size_t)p.nc * 4, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(p.h_acc, p.acc, (size_t)p.nc * 32, hipMemcpyDeviceToHost, s);
+    if (agg == 0) hipMemcpyAsync(p.h_csum, p.chunk_sum, (size_t)p.nc * 16, hipMemcpyDeviceToHost, s);
+    return hipGetLastError();
+  }
   if (is_agg(op)) {
     QueryArgs a = args_of(p);
     if (op == kOpFilterSum) {
@@ -303,6 +321,14 @@ int plan_init(Plan &p, const ebi_query &q) {
     default:
       p.decode = !p.insitu;
       break;
+  }
+  // Gorilla, Chimp and Elf filters and aggregates run fused: the segment
+  // decode feeds the query in registers (Chimp128 forms its values in a
+  // second pass, so it decodes into HBM first)
+  const bool seg_codec = scheme == kGorilla || scheme == kChimp || scheme == kElf;
+  if (seg_codec && p.c->n_seg_jobs && (q.op == kOpFilter || q.op == kOpFilterSum || is_agg(q.op))) {
+    p.fused = true;
+    p.decode = false;
   }
   return EBI_OK;
 }
@@ -510,7 +536,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   launch::chimp128_resolve(s, dv, dpre + (pbytes / 4), (uint32_t)n, (uint32_t)jobs128, g.d_flags);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   for (Plan *p : g.plans)
-    if ((e = plan_enqueue_post(*p, s)) != hipSuccess) return e;
+    if ((e = plan_enqueue_post(*p, s, g.d_flags, (uint32_t)cu_count * blocks_per_cu)) != hipSuccess) return e;
   return hipMemcpyAsync(g.h_flags, g.d_flags, 4, hipMemcpyDeviceToHost, s);
 }
 

@@ -12,7 +12,8 @@
 //   group_enqueue (per group of plans sharing a stream): upload every plan's
 //               column view and inputs, chunk selection, ONE batched decode
 //               launch over every plan that needs decoded records, then each
-//               plan's query kernels and the copy of its small results into
+//               plan's query kernels (for a fused plan the decode and the
+//               query in one kernel) and the copy of its small results into
 //               pinned memory
 //   plan_enqueue_values  after the device work: copy the materialized
 //               records (count known only then) into the caller's buffer
@@ -54,6 +55,8 @@ struct Plan {
   bool decode = false;     // needs the column's records decoded into `values`
   bool has_mask = false;
   bool cpred = false;      // per-chunk predicate parameters are uploaded
+  bool fused = false;      // Gorilla / Chimp / Elf filter or aggregate: decoded and queried in one
+                           // kernel (k_query_seg), the records never written to HBM
   int filter_mode = 0;     // k_filter mode (0 select, 1 code domain / f64, 2 f64)
   uint64_t words = 0;      // ceil(N / 64)
   uint32_t nc = 0;         // chunks

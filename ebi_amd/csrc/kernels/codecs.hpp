@@ -410,18 +410,19 @@ constexpr uint32_t kRun = 16;  // records of a run (between two flushes of the o
 // bits) is decoded before, by xor_seg_first, which sets from = 1 on its
 // lane.  A lane that meets an error keeps it,
 // consumes no more bits and so reads no further than the forms above; the
-// records it leaves in the sink are then meaningless, as are the sink
-// entries of records it does not take part in (sink.row is called for every
-// k).  Returns the lane's error.
+// records it leaves in the sink are then meaningless.  sink.row(i, k, v,
+// act) receives record i as row k of the run, act = whether the lane takes
+// part in it (a sink may ignore the rows with act false).  Returns the
+// lane's error.
 
 // Record 0 of a Gorilla or Chimp chunk (the stream's first 64 bits).
 template <class Sink>
-__host__ __device__ __forceinline__ uint32_t xor_seg_first(XorState &s, uint32_t size_bits, const Sink &out) {
+__host__ __device__ __forceinline__ uint32_t xor_seg_first(XorState &s, uint32_t size_bits, Sink &out) {
   BitWindow &r = s.r;
   s.v = r.field(0, 64);
   r.consume(64);
   r.refill();
-  out.row(0, 0, as_f64(s.v));
+  out.row(0, 0, as_f64(s.v), true);
   if (r.pos > size_bits) return kErrOverrun;
   return s.v == kNan ? kErrCount : kErrNone;  // n >= 1: record 0 is not the terminator
 }
@@ -430,7 +431,7 @@ __host__ __device__ __forceinline__ uint32_t xor_seg_first(XorState &s, uint32_t
 // gorilla_run).
 template <class Sink>
 __host__ __device__ __forceinline__ uint32_t gorilla_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
-                                                uint32_t cnt, uint32_t from, uint32_t lim, const Sink &out,
+                                                uint32_t cnt, uint32_t from, uint32_t lim, Sink &out,
                                                 uint32_t e) {
   BitWindow &r = s.r;
   uint64_t v = s.v;
@@ -458,7 +459,7 @@ __host__ __device__ __forceinline__ uint32_t gorilla_seg(XorState &s, uint32_t s
                           : (i == n) != (v == kNan) ? (uint32_t)kErrCount
                                                    : 0u;
     e = act ? code : e;
-    out.row(i, k, as_f64(v));
+    out.row(i, k, as_f64(v), act);
   }
   s.v = v;
   s.a = L;
@@ -469,7 +470,7 @@ __host__ __device__ __forceinline__ uint32_t gorilla_seg(XorState &s, uint32_t s
 // Chimp, decoder/chunk_reader/chimp.rs:59-125 (the record step of chimp_run).
 template <class Sink>
 __host__ __device__ __forceinline__ uint32_t chimp_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
-                                              uint32_t cnt, uint32_t from, uint32_t lim, const Sink &out,
+                                              uint32_t cnt, uint32_t from, uint32_t lim, Sink &out,
                                               uint32_t e) {
   BitWindow &r = s.r;
   uint64_t v = s.v;
@@ -495,7 +496,7 @@ __host__ __device__ __forceinline__ uint32_t chimp_seg(XorState &s, uint32_t siz
                           : (i == n) != (v == kNan) ? (uint32_t)kErrCount
                                                    : 0u;
     e = act ? code : e;
-    out.row(i, k, as_f64(v));
+    out.row(i, k, as_f64(v), act);
   }
   s.v = v;
   s.a = lead;
@@ -505,7 +506,7 @@ __host__ __device__ __forceinline__ uint32_t chimp_seg(XorState &s, uint32_t siz
 // Record 0 of an Elf chunk: prefix, then the ElfXor first value (7-bit
 // trailing zeros, 63 - tz bits), decoder/chunk_reader/elf.rs:105-140.
 template <class Sink>
-__host__ __device__ __forceinline__ uint32_t elf_seg_first(XorState &s, uint32_t size_bits, const Sink &out,
+__host__ __device__ __forceinline__ uint32_t elf_seg_first(XorState &s, uint32_t size_bits, Sink &out,
                                                   const double *tab) {
   BitWindow &r = s.r;
   const uint32_t p2 = (uint32_t)(r.hi >> 62);
@@ -523,7 +524,7 @@ __host__ __device__ __forceinline__ uint32_t elf_seg_first(XorState &s, uint32_t
   r.consume(used);
   r.refill();
   const double v = as_f64(s.v);
-  out.row(0, 0, p2 != 2 ? elf_recover_tab(v, s.beta, tab) : v);
+  out.row(0, 0, p2 != 2 ? elf_recover_tab(v, s.beta, tab) : v, true);
   if (r.pos > size_bits) return kErrOverrun;
   return s.v == kNan ? kErrCount : kErrNone;  // n >= 1: record 0 is not the terminator
 }
@@ -533,7 +534,7 @@ __host__ __device__ __forceinline__ uint32_t elf_seg_first(XorState &s, uint32_t
 // tab: the powers of ten of elf_recover_tab.
 template <class Sink>
 __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
-                                            uint32_t cnt, uint32_t from, uint32_t lim, const Sink &out,
+                                            uint32_t cnt, uint32_t from, uint32_t lim, Sink &out,
                                             uint32_t e, const double *tab) {
   BitWindow &r = s.r;
   uint64_t stored = s.v;
@@ -580,7 +581,7 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
     e = act ? code : e;
     const double v = as_f64(stored);
     const double w = p2 != 2 ? elf_recover_tab(v, beta, tab) : v;
-    if (act) out.row(i, k, w);
+    out.row(i, k, w, act);
   }
   s.v = stored;
   s.a = L;
@@ -670,9 +671,10 @@ struct StageSink {
     stage[r * kWave + (lane_id() ^ stage_swz(r))] = v;
   }
   // lockstep runs: record i is row k of the run (k <= kRun; row kRun holds
-  // only a terminator, never flushed)
-  __device__ __forceinline__ void row(uint32_t, uint32_t k, double v) const {
-    stage[k * kWave + (lane_id() ^ stage_swz(k))] = v;
+  // only a terminator, never flushed); rows the lane takes no part in keep
+  // what they held
+  __device__ __forceinline__ void row(uint32_t, uint32_t k, double v, bool act = true) const {
+    if (act) stage[k * kWave + (lane_id() ^ stage_swz(k))] = v;
   }
 };
 

@@ -28,14 +28,15 @@ __device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-// Checkpoint segments first .. first + 63 of chunk ci, segment first + l on
-// lane l, decoded in runs of kRun records from the checkpoints; after each
-// run the whole wave writes the staged records out (stage_flush).  The loop
-// over runs is wave-uniform: lanes without a segment (past the chunk's end)
-// or stopped by an error keep taking part in the stores.
-template <uint32_t kCodec>
+// Checkpoint segments first .. first + 63 of chunk c (Gorilla, Chimp or
+// Elf), segment first + l on lane l, decoded in lockstep runs of kRun
+// records from the checkpoints into `out`: out.row(i, k, v, act) takes each
+// record, out.run_end(k0) follows each run.  The loop over runs is
+// wave-uniform: lanes without a segment (past the chunk's end) or stopped by
+// an error stay in it.
+template <uint32_t kCodec, class Out>
 __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChunk &c, uint32_t first, uint32_t row,
-                                                double *stage, const double *elf_tab) {
+                                                const double *elf_tab, Out &out) {
   const uint32_t i0 = (first + lane_id()) * kSeg;
   const bool live = i0 < c.n;
   const uint32_t i1 = live ? min(i0 + kSeg, c.n) : 0;
@@ -45,9 +46,6 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
     s.r.init_at(cv.ilv + (size_t)row * kIlvRowBlocks + lane_id(), k.bitpos, kIlvRowBlocks);
     xor_state_set(s, kCodec, k);
   }
-  const StageSink sink{stage};
-  double *dst = cv.out + c.rec_off + (size_t)first * kSeg;
-  const uint32_t n_job = first * kSeg < c.n ? c.n - first * kSeg : 0;  // records from segment `first` on
   const uint32_t bits = c.size * 8u;
   // a lane without a segment has an empty window that it never advances
   // (it takes part in no record)
@@ -62,7 +60,7 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
   const uint32_t last = live && i1 == c.n ? c.n + 1 : i1;  // the chunk's last segment parses the terminator
   uint32_t from = i0, e = 0;
   if (live && i0 == 0) {
-    e = kCodec == kCodecElf ? elf_seg_first(s, bits, sink, elf_tab) : xor_seg_first(s, bits, sink);
+    e = kCodec == kCodecElf ? elf_seg_first(s, bits, out, elf_tab) : xor_seg_first(s, bits, out);
     from = 1;
   }
 #pragma unroll 1  // one copy of the codec loop per codec (instruction cache)
@@ -73,16 +71,26 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
     const uint32_t cnt = kRun + (__ballot(term_next) != 0);
     const uint32_t lim = term_next ? last : min(last, a + kRun);
     if (kCodec == kCodecGorilla)
-      e = gorilla_seg(s, bits, c.n, a, cnt, from, lim, sink, e);
+      e = gorilla_seg(s, bits, c.n, a, cnt, from, lim, out, e);
     else if (kCodec == kCodecChimp)
-      e = chimp_seg(s, bits, c.n, a, cnt, from, lim, sink, e);
+      e = chimp_seg(s, bits, c.n, a, cnt, from, lim, out, e);
     else
-      e = elf_seg(s, bits, c.n, a, cnt, from, lim, sink, e, elf_tab);
+      e = elf_seg(s, bits, c.n, a, cnt, from, lim, out, e, elf_tab);
     from = max(from, lim);  // a terminator taken as record a + kRun is not parsed again
-    stage_flush(stage, dst, k0, n_job);
+    out.run_end(k0);
   }
   return e;
 }
+
+// Decode output of segment_wave: the wave's LDS stage, written out after
+// each run (stage_flush) to the job's records in cv.out.
+struct StageOut {
+  StageSink sink;
+  double *dst;      // record 0 of segment `first`
+  uint32_t n_job;   // records of the chunk from segment `first` on
+  __device__ __forceinline__ void row(uint32_t i, uint32_t k, double v, bool act) { sink.row(i, k, v, act); }
+  __device__ __forceinline__ void run_end(uint32_t k0) { stage_flush(sink.stage, dst, k0, n_job); }
+};
 
 // Chimp128 parse of checkpoint segments first .. first + 63 of chunk ci
 // (records 0..n, n = the terminator), segment first + l on lane l, in runs
@@ -137,10 +145,12 @@ __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_
 __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, uint32_t first, uint32_t row,
                                                double *stage, const double *elf_tab) {
   const DevChunk c = cv.chunks[ci];
+  StageOut out{StageSink{stage}, cv.out + c.rec_off + (size_t)first * kSeg,
+               first * kSeg < c.n ? c.n - first * kSeg : 0u};
   switch (c.codec) {
-    case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, row, stage, elf_tab);
-    case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, row, stage, elf_tab);
-    case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, row, stage, elf_tab);
+    case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, row, elf_tab, out);
+    case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, row, elf_tab, out);
+    case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, row, elf_tab, out);
     case kCodecChimp128: return segment_wave_c128(cv, ci, c, first, row, stage);
     default: return kErrCorrupt;
   }
@@ -215,6 +225,119 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       if (e) atomicOr(err, e);
     }
     j += stride;
+  }
+}
+
+// Query output of segment_wave (the fused queries of k_query_seg): each
+// lane keeps, for its own segment, the bitmask bits of its records (all set
+// without a bitmask), the records it selects (bitmask and, with `pred`, the
+// f64 predicate; bit p = record i0 + p), and their double-double sum (agg 0)
+// or min / max order keys (agg 1 / 2, NaN skipped as Rust's f64::min / max
+// do).  No record leaves the registers.
+struct QueryOut {
+  const QueryArgs *a;
+  uint32_t i0, n;   // the lane's first record, the chunk's record count
+  bool pred;
+  int agg;          // -1 none, 0 sum, 1 min, 2 max
+  uint64_t mbits, obits;
+  double hi, lo;
+  uint64_t kmin, kmax;
+  __device__ __forceinline__ void row(uint32_t i, uint32_t, double v, bool act) {
+    const uint32_t p = (i - i0) & 63;  // the terminator (i == n) may be record i0 + 64: never selected
+    const bool sel = act && i < n && ((mbits >> p) & 1) && (!pred || pred_f64(*a, v));
+    obits |= (uint64_t)sel << p;
+    if (agg == 0) {
+      if (sel) dd_add(hi, lo, v, 0.0);
+    } else if (agg > 0 && sel && v == v) {
+      kmin = min(kmin, f64_key(v));
+      kmax = max(kmax, f64_key(v));
+    }
+  }
+  __device__ __forceinline__ void run_end(uint32_t) {}
+};
+
+// Fused queries over the checkpoint-segment jobs of one Gorilla / Chimp / Elf
+// column (cols[0]): the records are decoded as in k_decode_seg and consumed
+// in registers.  flags: bit 0 apply the f64 predicate, bit 1 write the
+// selected records' bits to a.out_bits (zeroed before), bits 2-3 aggregate
+// (0 none, 1 sum, 2 min, 3 max).  Sums go to the per-record-tile
+// double-double partials (k_chunk_dd folds them per chunk), each the
+// butterfly sum of its 8 segments' lane sums; min / max to the chunk
+// accumulators (k_acc_init before).  Chunks with a.chunk_sel 0 are skipped.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_query_seg(
+    const ColView *cols, uint32_t n_jobs, QueryArgs a, uint32_t flags, uint32_t *err) {
+  __shared__ double elf_tab[kElfTab];
+  if (threadIdx.x < kElfTab) elf_tab_fill(elf_tab, threadIdx.x, c_p10, c_n10);
+  __syncthreads();
+  const ColView cv = cols[0];
+  const uint32_t lane = lane_id();
+  const bool want_bits = (flags >> 1) & 1;
+  const int agg = (int)((flags >> 2) & 3) - 1;
+  const uint32_t stride = gridDim.x * 4;
+  for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / kWave); j < n_jobs; j += stride) {
+    const Job jb = cv.seg_jobs[j];
+    if (!a.chunk_sel[jb.chunk]) continue;
+    const DevChunk c = cv.chunks[jb.chunk];
+    const uint32_t i0 = (jb.tile + lane) * kSeg;
+    const uint64_t g = c.rec_off + i0;
+    uint64_t m = ~0ull;
+    if (a.mask && i0 < c.n) {  // the 64 bitmask bits from record g (the bitmask has a spare word)
+      const uint32_t sh = (uint32_t)(g & 63);
+      m = a.mask[g >> 6] >> sh;
+      if (sh) m |= a.mask[(g >> 6) + 1] << (64 - sh);
+    }
+    QueryOut q{&a, i0, c.n, (flags & 1) != 0, agg, m, 0, 0.0, 0.0, ~0ull, 0};
+    uint32_t e;
+    switch (c.codec) {
+      case kCodecGorilla: e = segment_wave<kCodecGorilla>(cv, c, jb.tile, jb.col, elf_tab, q); break;
+      case kCodecChimp: e = segment_wave<kCodecChimp>(cv, c, jb.tile, jb.col, elf_tab, q); break;
+      case kCodecElf: e = segment_wave<kCodecElf>(cv, c, jb.tile, jb.col, elf_tab, q); break;
+      default: e = kErrCorrupt; break;
+    }
+    if (e) atomicOr(err, e);
+    if (want_bits && i0 < c.n) {
+      // a word wholly inside this segment is the lane's own; one shared
+      // with another segment or chunk takes atomic ORs
+      unsigned long long *w = reinterpret_cast<unsigned long long *>(a.out_bits) + (g >> 6);
+      const uint32_t sh = (uint32_t)(g & 63);
+      if (sh == 0 && i0 + kSeg <= c.n) {
+        *w = q.obits;
+      } else if (q.obits) {
+        atomicOr(w, (unsigned long long)(q.obits << sh));
+        if (sh) atomicOr(w + 1, (unsigned long long)(q.obits >> (64 - sh)));
+      }
+    }
+    if (agg == 0) {
+      // the 8 segments of a record tile: butterfly in a lane-independent order
+#pragma unroll
+      for (int d = 1; d < 8; d <<= 1) {
+        const double oh = __shfl_xor(q.hi, d, kWave), ol = __shfl_xor(q.lo, d, kWave);
+        if ((lane & d) == 0) {
+          dd_add(q.hi, q.lo, oh, ol);
+        } else {
+          double h2 = oh, l2 = ol;
+          dd_add(h2, l2, q.hi, q.lo);
+          q.hi = h2;
+          q.lo = l2;
+        }
+      }
+      if ((lane & 7) == 0 && i0 < c.n) {
+        double *p = a.partials + 2ull * (c.rtile0 + (jb.tile + lane) / 8);
+        p[0] = q.hi;
+        p[1] = q.lo;
+      }
+    } else if (agg > 0) {
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        q.kmin = min(q.kmin, (uint64_t)__shfl_xor((unsigned long long)q.kmin, d, kWave));
+        q.kmax = max(q.kmax, (uint64_t)__shfl_xor((unsigned long long)q.kmax, d, kWave));
+      }
+      unsigned long long *acc = reinterpret_cast<unsigned long long *>(a.acc) + 4ull * jb.chunk;
+      if (lane == 0) {
+        if (agg == 1) atomicMin(acc + 2, (unsigned long long)q.kmin);
+        else atomicMax(acc + 3, (unsigned long long)q.kmax);
+      }
+    }
   }
 }
 
@@ -488,6 +611,17 @@ void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t 
   if (n_seg_jobs)
     hipLaunchKernelGGL(k_decode_seg, dim3(grid(n_seg_jobs)), dim3(256), 0, s, cols, seg_prefix, n_cols,
                        n_seg_jobs, err);
+}
+
+void query_seg(hipStream_t s, const ColView *cols, uint32_t n_seg_jobs, const QueryArgs &a, bool pred, bool bits,
+               int agg, uint32_t n_chunks, uint32_t *err, uint32_t max_blocks) {
+  if (agg > 0 && n_chunks)
+    hipLaunchKernelGGL(k_acc_init, dim3((n_chunks + 255) / 256), dim3(256), 0, s, a.acc, n_chunks);
+  if (!n_seg_jobs) return;
+  const uint32_t flags = (pred ? 1u : 0u) | (bits ? 2u : 0u) | ((uint32_t)(agg + 1) << 2);
+  uint32_t blocks = (n_seg_jobs + 3) / 4;
+  if (max_blocks && blocks > max_blocks) blocks = max_blocks;
+  hipLaunchKernelGGL(k_query_seg, dim3(blocks), dim3(256), 0, s, cols, n_seg_jobs, a, flags, err);
 }
 
 void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,

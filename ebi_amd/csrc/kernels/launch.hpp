@@ -47,6 +47,14 @@ void chunk_select(hipStream_t s, const ColView *cols, const uint64_t *mask, int6
 void filter(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a, int mode);
 void aggregate(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a,
                uint32_t n_chunks);
+// Fused decode + query over the segment jobs of one Gorilla / Chimp / Elf
+// column (cols[0]; records never reach HBM): with `pred` the f64 predicate
+// of `a` on top of a.mask, with `bits` the selection to a.out_bits (zeroed
+// before), agg 0 sum (per-tile partials for chunk_dd), 1 min, 2 max (chunk
+// accumulators, initialized here), -1 none.  Chunks with a.chunk_sel 0 are
+// skipped.
+void query_seg(hipStream_t s, const ColView *cols, uint32_t n_seg_jobs, const QueryArgs &a, bool pred, bool bits,
+               int agg, uint32_t n_chunks, uint32_t *err, uint32_t max_blocks);
 void chunk_dd(hipStream_t s, const ColView *cols, const uint32_t *sel, const double *partials, double *chunk_sum,
               uint32_t n_chunks);
 uint32_t compact_blocks(uint64_t n_words);

@@ -79,14 +79,14 @@ static bool within_lane(const Ilv &v, uint32_t seg, const BitWindow &r) {
 struct RowSink {
   double *out;
   uint32_t lim;  // records the buffer holds
-  void row(uint32_t i, uint32_t k, double v) const {  // row kRun: the stage row that is never flushed
-    if (k < kRun && i < lim) out[i] = v;
+  void row(uint32_t i, uint32_t k, double v, bool act = true) const {  // row kRun: the stage row never flushed
+    if (act && k < kRun && i < lim) out[i] = v;
   }
 };
 
 static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t bits, uint32_t cn, const XorCkpt *ck,
                                       uint32_t nseg, double *out, const Ilv *ilv = nullptr) {
-  const RowSink sink{out, cn};
+  RowSink sink{out, cn};
   std::vector<uint32_t> err(nseg, 0);
   for (uint32_t first = 0; first < nseg; first += kWave) {
     XorState st[kWave];
@@ -277,7 +277,7 @@ int main() {
             C128Term term{};
             // the kernel's lockstep parse through the ilv, wave by wave
             const Ilv ilv = build_ilv(data, bits, ck.data(), nseg);
-            const RowSink xsink{out.data(), cn};
+            RowSink xsink{out.data(), cn};
             for (uint32_t first = 0; first < nseg; first += kWave) {
               BitWindow r[kWave];
               uint32_t lead[kWave], from[kWave], lim_seg[kWave], d16[kWave][4], ee[kWave];
