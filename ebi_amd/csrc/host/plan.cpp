@@ -129,7 +129,8 @@ Layout layout_of(const Plan &p) {
   L.part = take((uint64_t)p.c->n_rtiles * 16);
   L.csum = take((uint64_t)p.nc * 16);
   L.block = take(nb * 8);
-  L.values = p.decode && p.q.op != kOpDecode ? take(N * 8) : ~0ull;  // DECODE writes the caller's buffer
+  // DECODE writes the caller's buffer; fused plans keep the records in registers
+  L.values = p.decode && !p.fused && p.q.op != kOpDecode ? take(N * 8) : ~0ull;
   L.out = (p.q.op == kOpFilterMaterialize || (p.q.op == kOpMaterialize && p.has_mask)) ? take(N * 8) : ~0ull;
   L.c128 = p.decode && p.c->chimp128 ? take(c128_scratch_need(*p.c)) : ~0ull;  // Chimp128 decode scratch
   L.total = off;
@@ -212,7 +213,10 @@ hipError_t plan_enqueue_post(Plan &p, hipStream_t s, uint32_t *err, uint32_t max
     // selection, per-chunk fold and copies as below
     QueryArgs a = args_of(p);
     const int agg = op == kOpSum || op == kOpFilterSum ? 0 : op == kOpMin ? 1 : op == kOpMax ? 2 : -1;
-    launch::query_seg(s, p.d_view, c.n_seg_jobs, a, has_pred(op), writes_bits(p), agg, p.nc, err, max_blocks);
+    if (c.chimp128)  // the parse ran in the group's decode launch; the values form here
+      launch::chimp128_query(s, p.d_view, p.nc, a, has_pred(op), writes_bits(p), agg, err);
+    else
+      launch::query_seg(s, p.d_view, c.n_seg_jobs, a, has_pred(op), writes_bits(p), agg, p.nc, err, max_blocks);
     if (op == kOpFilter) {
       if (p.words) hipMemcpyAsync(p.h_bits, p.bits, p.words * 8, hipMemcpyDeviceToHost, s);
       return hipGetLastError();
@@ -322,13 +326,13 @@ int plan_init(Plan &p, const ebi_query &q) {
       p.decode = !p.insitu;
       break;
   }
-  // Gorilla, Chimp and Elf filters and aggregates run fused: the segment
-  // decode feeds the query in registers (Chimp128 forms its values in a
-  // second pass, so it decodes into HBM first)
-  const bool seg_codec = scheme == kGorilla || scheme == kChimp || scheme == kElf;
+  // Gorilla, Chimp, Elf and Chimp128 filters and aggregates run fused: the
+  // segment decode (Chimp128: its value-forming pass, after the segment
+  // parse) feeds the query in registers
+  const bool seg_codec = scheme == kGorilla || scheme == kChimp || scheme == kElf || scheme == kChimp128;
   if (seg_codec && p.c->n_seg_jobs && (q.op == kOpFilter || q.op == kOpFilterSum || is_agg(q.op))) {
     p.fused = true;
-    p.decode = false;
+    p.decode = scheme == kChimp128;  // the parse runs in the group's decode launch
   }
   return EBI_OK;
 }
@@ -517,7 +521,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
     if (p.decode) {
       jobs += p.c->n_tile_jobs;
       segs += p.c->n_seg_jobs;
-      if (p.c->chimp128) jobs128 += p.c->chunks.size();
+      if (p.c->chimp128 && !p.fused) jobs128 += p.c->chunks.size();
     }
   }
   pre[n] = (uint32_t)jobs;

@@ -55,8 +55,8 @@ struct Plan {
   bool decode = false;     // needs the column's records decoded into `values`
   bool has_mask = false;
   bool cpred = false;      // per-chunk predicate parameters are uploaded
-  bool fused = false;      // Gorilla / Chimp / Elf filter or aggregate: decoded and queried in one
-                           // kernel (k_query_seg), the records never written to HBM
+  bool fused = false;      // filter or aggregate of an XOR codec: decoded and queried in one kernel
+                           // (k_query_seg; Chimp128: its resolve pass), the records never written to HBM
   int filter_mode = 0;     // k_filter mode (0 select, 1 code domain / f64, 2 f64)
   uint64_t words = 0;      // ceil(N / 64)
   uint32_t nc = 0;         // chunks

@@ -419,18 +419,31 @@ __global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const uint
 // Record n must be the NaN end marker and an earlier NaN is a count error:
 // a NaN can only come from flags 01 / 10 / 11 (the ring never holds one,
 // as the reference stops at the first).
+//
+// kQuery: the fused queries of one Chimp128 column (cols[0], chunk j): the
+// values go to a query sink instead of cv.out, as in k_query_seg (flags
+// alike): a step's 64 consecutive records give one ballot of selection bits,
+// a batch of 8 steps is one record tile (its double-double partial the
+// butterfly sum of the lanes' sums), min / max keys go to the chunk
+// accumulators.  Chunks with a.chunk_sel 0 are skipped.
 constexpr uint32_t kC128Batch = 8;
-
+template <bool kQuery>
 __global__ void __launch_bounds__(256) k_chimp128_resolve(const ColView *cols, const uint32_t *prefix,
-                                                         uint32_t n_cols, uint32_t n_chunks, uint32_t *err) {
+                                                         uint32_t n_cols, uint32_t n_chunks, uint32_t *err,
+                                                         QueryArgs a, uint32_t flags) {
   __shared__ uint64_t ring[4][256];
   const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
   const uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
   if (j >= n_chunks) return;
-  const uint32_t col = find_col(prefix, n_cols, j);
+  const uint32_t col = kQuery ? 0u : find_col(prefix, n_cols, j);
   const ColView cv = cols[col];
-  const uint32_t ci = j - prefix[col];
+  const uint32_t ci = kQuery ? j : j - prefix[col];
   if (cv.chunk_err && cv.chunk_err[ci]) return;
+  if (kQuery && !a.chunk_sel[ci]) return;
+  const bool pred = flags & 1, want_bits = (flags >> 1) & 1;
+  const int agg = (int)((flags >> 2) & 3) - 1;
+  double hi = 0.0, lo = 0.0;
+  uint64_t kmin = ~0ull, kmax = 0;
   const DevChunk c = cv.chunks[ci];
   double *out = cv.out + c.rec_off;
   const EBI_GLOBAL double *px = as_global(cv.c128_x + (size_t)c.rtile0 * kTile);
@@ -471,11 +484,73 @@ __global__ void __launch_bounds__(256) k_chimp128_resolve(const ColView *cols, c
         e |= kErrCount;
       }
       x[g] = v;
+      if (kQuery) {
+        const uint64_t gi = c.rec_off + i;
+        const double fv = as_f64(v);
+        const bool m = i < n && (!a.mask || ((a.mask[gi >> 6] >> (gi & 63)) & 1));
+        const bool sel = m && (!pred || pred_f64(a, fv));
+        if (want_bits) {
+          const uint64_t bal = __ballot(sel);
+          if (lane == 0 && (uint32_t)base < n) {
+            // the step's word: the wave's own when aligned and inside the
+            // chunk, atomic ORs when it is shared
+            const uint64_t g0 = c.rec_off + (uint32_t)base;
+            unsigned long long *w = reinterpret_cast<unsigned long long *>(a.out_bits) + (g0 >> 6);
+            const uint32_t sh = (uint32_t)(g0 & 63);
+            if (sh == 0 && (uint32_t)base + kWave <= n) {
+              *w = bal;
+            } else if (bal) {
+              atomicOr(w, (unsigned long long)(bal << sh));
+              if (sh) atomicOr(w + 1, (unsigned long long)(bal >> (64 - sh)));
+            }
+          }
+        }
+        if (agg == 0) {
+          if (sel) dd_add(hi, lo, fv, 0.0);
+        } else if (agg > 0 && sel && fv == fv) {
+          kmin = min(kmin, f64_key(fv));
+          kmax = max(kmax, f64_key(fv));
+        }
+      }
     }
+    if (!kQuery) {
 #pragma unroll
-    for (uint32_t g = 0; g < kC128Batch; ++g) {
-      const uint32_t i = b0 + g * kWave + lane;
-      if (i < n) st_one(out + i, as_f64(x[g]));
+      for (uint32_t g = 0; g < kC128Batch; ++g) {
+        const uint32_t i = b0 + g * kWave + lane;
+        if (i < n) st_one(out + i, as_f64(x[g]));
+      }
+    } else if (agg == 0) {
+      // this batch is record tile b0 / kTile: butterfly in a lane-independent order
+#pragma unroll
+      for (int d2 = 32; d2 >= 1; d2 >>= 1) {
+        const double oh = __shfl_xor(hi, d2, kWave), ol = __shfl_xor(lo, d2, kWave);
+        if ((lane & d2) == 0) {
+          dd_add(hi, lo, oh, ol);
+        } else {
+          double h2 = oh, l2 = ol;
+          dd_add(h2, l2, hi, lo);
+          hi = h2;
+          lo = l2;
+        }
+      }
+      if (lane == 0 && b0 < n) {
+        double *pp = a.partials + 2ull * (c.rtile0 + b0 / kTile);
+        pp[0] = hi;
+        pp[1] = lo;
+      }
+      hi = lo = 0.0;
+    }
+  }
+  if (kQuery && agg > 0) {
+#pragma unroll
+    for (int d2 = 32; d2 >= 1; d2 >>= 1) {
+      kmin = min(kmin, (uint64_t)__shfl_xor((unsigned long long)kmin, d2, kWave));
+      kmax = max(kmax, (uint64_t)__shfl_xor((unsigned long long)kmax, d2, kWave));
+    }
+    unsigned long long *acc = reinterpret_cast<unsigned long long *>(a.acc) + 4ull * ci;
+    if (lane == 0) {
+      if (agg == 1) atomicMin(acc + 2, (unsigned long long)kmin);
+      else atomicMax(acc + 3, (unsigned long long)kmax);
     }
   }
   if (e) atomicOr(err, e);
@@ -638,8 +713,18 @@ void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
 void chimp128_resolve(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
                       uint32_t *err) {
   if (!n_chunks) return;
-  hipLaunchKernelGGL(k_chimp128_resolve, dim3((n_chunks + 3) / 4), dim3(256), 0, s, cols, prefix, n_cols, n_chunks,
-                     err);
+  hipLaunchKernelGGL(k_chimp128_resolve<false>, dim3((n_chunks + 3) / 4), dim3(256), 0, s, cols, prefix, n_cols,
+                     n_chunks, err, QueryArgs{}, 0u);
+}
+
+void chimp128_query(hipStream_t s, const ColView *cols, uint32_t n_chunks, const QueryArgs &a, bool pred, bool bits,
+                    int agg, uint32_t *err) {
+  if (agg > 0 && n_chunks)
+    hipLaunchKernelGGL(k_acc_init, dim3((n_chunks + 255) / 256), dim3(256), 0, s, a.acc, n_chunks);
+  if (!n_chunks) return;
+  const uint32_t flags = (pred ? 1u : 0u) | (bits ? 2u : 0u) | ((uint32_t)(agg + 1) << 2);
+  hipLaunchKernelGGL(k_chimp128_resolve<true>, dim3((n_chunks + 3) / 4), dim3(256), 0, s, cols, nullptr, 1u,
+                     n_chunks, err, a, flags);
 }
 
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,

@@ -30,6 +30,10 @@ void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
 // decode() parsed the chunks' segments into the views' Chimp128 scratch.
 void chimp128_resolve(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
                       uint32_t *err);
+// The fused queries of one Chimp128 column (cols[0]) in the resolve pass,
+// after decode() parsed its segments (arguments as query_seg).
+void chimp128_query(hipStream_t s, const ColView *cols, uint32_t n_chunks, const QueryArgs &a, bool pred, bool bits,
+                    int agg, uint32_t *err);
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
               const Job *single, uint32_t *err, bool write);
 // Load-time lane-interleaved segment streams of the XOR column cols[0]:
