@@ -36,9 +36,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # kernels of each codec in a batch: (reads the compressed stream, writes the
-# decoded records); Chimp128 parses in k_decode_seg and forms its values in
-# k_chimp128_resolve; the rest: k_decode
-KERNELS_OF = {"chimp128": ("k_decode_seg", "k_chimp128_resolve"), "gorilla": ("k_decode_seg", "k_decode_seg"),
+# decoded records); the XOR codecs: k_decode_seg; the rest: k_decode
+KERNELS_OF = {"chimp128": ("k_decode_seg", "k_decode_seg"), "gorilla": ("k_decode_seg", "k_decode_seg"),
               "chimp": ("k_decode_seg", "k_decode_seg"), "elf": ("k_decode_seg", "k_decode_seg")}
 
 
@@ -126,9 +125,9 @@ def main():
 
     def step():
         # one batched decode over every chunk of every column (mixed codecs):
-        # k_decode (tile jobs of all columns), k_decode_seg (XOR checkpoint
-        # segments, Chimp128 parse) and k_chimp128_resolve, launched eagerly
-        # so each kernel is bracketed by HIP events
+        # k_decode (tile jobs of all columns) and k_decode_seg (XOR
+        # checkpoint segments), launched eagerly so each kernel is bracketed
+        # by HIP events
         batch.run(eager=True)
         return ctx.last_device_ms, ctx.last_phase_ms(0), ctx.last_phase_ms(1), ctx.last_phase_ms(2)
 
@@ -160,7 +159,7 @@ def main():
     kernels_of = {c: KERNELS_OF.get(c, ("k_decode", "k_decode")) for c in codecs}
     kernel_of = {c: "+".join(dict.fromkeys(k)) for c, k in kernels_of.items()}  # per-codec grouping
     batch_kernels = {}
-    for k, name in enumerate(("k_decode", "k_chimp128_resolve", "k_decode_seg")):
+    for k, name in enumerate(("k_decode", "", "k_decode_seg")):  # phase 1 is unused
         # algorithmic bytes: the compressed stream to its reader, 8 B a record to its writer
         b = sum((cb if kernels_of[codec][0] == name else 0) + (8 * n if kernels_of[codec][1] == name else 0)
                 for _, codec, _, cb in cols)

@@ -452,7 +452,7 @@ class Context:
 
     def last_phase_ms(self, phase: int) -> float:
         """Device ms of kernel `phase` of the last decode (0: tile kernel,
-        1: Chimp128 resolve kernel, 2: XOR segment kernel)."""
+        2: XOR segment kernel; 1 is 0)."""
         return float(lib().ebi_hip_last_phase_ms(self._h, int(phase)))
 
     def alloc(self, nbytes: int) -> "DeviceBuffer":

@@ -30,6 +30,8 @@ constexpr uint32_t kAlign = 256;
 constexpr uint32_t kPad = 256;
 constexpr uint32_t kTile = 512;  // records (fields for RLE) per wave job: 64 lanes x 8
 constexpr uint32_t kSeg = 64;    // records per checkpoint of the XOR index
+constexpr uint32_t kSeg128 = 16; // records per checkpoint of the Chimp128 index
+constexpr uint32_t kJob128 = kSeg128 * 64;  // records of a Chimp128 segment job (64 segments)
 constexpr uint32_t kTilesPerJob = 4;  // consecutive tiles of one chunk per decode job
 constexpr uint32_t kWaveLanes = 64;  // gfx950 wavefront size
 
@@ -124,6 +126,7 @@ struct alignas(16) Job {
   uint32_t kind;
   uint32_t col;    // column slot in the launch (output pointer / table);
                    // kJobSeg: first row of the job in the column's ilv
+                   // (Chimp128: the job's index in its column, for c128_ring)
   uint32_t chunk;  // chunk index within the column (first chunk for kJobSerial)
   uint32_t tile;   // tile / first segment index; lane count for kJobSerial
 };
@@ -143,14 +146,6 @@ __host__ __device__ inline uint32_t ilv_lane_blocks(uint32_t first_bit, uint32_t
   return (end_bit >> 7) + 3 - (first_bit >> 7);
 }
 
-// Parse of a Chimp128 chunk's terminator record n: XOR payload and source
-// distance (its value must come out as the NaN end marker).
-struct alignas(16) C128Term {
-  uint64_t x;
-  uint32_t dist;
-  uint32_t pad;
-};
-static_assert(sizeof(C128Term) == 16, "C128Term layout");
 
 // Per-launch view of one column.  A launch over several columns walks the
 // concatenation of their job lists; `prefix` arrays (one entry per column
@@ -168,22 +163,14 @@ struct alignas(16) ColView {
   uint32_t n_tile_jobs;
   uint32_t n_seg_jobs;
   uint32_t n_serial_jobs;
-  // Chimp128 decode scratch of this launch (null for other codecs): the XOR
-  // payload and the source distance of every record (element rtile0 * kTile
-  // + i for record i of a chunk), and per chunk the terminator record's
-  // parse.  The output is only written, once, by the resolve pass.
-  double *c128_x;
-  uint8_t *c128_dist;
-  C128Term *c128_term;
-  const uint4 *ilv;  // lane-interleaved segment streams (XOR codecs; null before they are built)
+  // Chimp128 (null for other codecs): per segment job, the values of the
+  // 128 records before its first record (the ring a job starts from; zeros
+  // before record 0), at c128_ring[128 * Job::col]
+  const uint64_t *c128_ring;
+  uint64_t pad0, pad1;
+  const uint4 *ilv;  // lane-interleaved segment streams (Gorilla, Chimp, Elf; null before they are built)
 };
 static_assert(sizeof(ColView) == 112, "ColView layout");
-
-// Device bytes of the Chimp128 decode scratch of a column of n_rtiles record
-// tiles and n_chunks chunks: payloads, distances, then the terminator slots.
-inline uint64_t c128_scratch_bytes(uint64_t n_rtiles, uint64_t n_chunks) {
-  return n_rtiles * kTile * 9 + n_chunks * sizeof(C128Term);
-}
 
 // Device error flags (OR-ed by kernels).
 enum DevError : uint32_t {

@@ -86,30 +86,12 @@ ColView view_of(const Column &c, double *out) {
   v.n_seg_jobs = c.n_seg_jobs;
   v.n_serial_jobs = c.n_serial_jobs;
   v.ilv = c.d_ilv;
+  v.c128_ring = c.d_c128_ring;
   return v;
-}
-
-uint64_t c128_scratch_need(const Column &c) {
-  return c.chimp128 ? round_up(c128_scratch_bytes(c.n_rtiles, c.chunks.size()), kAlign) : 0;
-}
-
-void bind_c128_scratch(ColView &v, const Column &c, uint8_t *scratch) {
-  if (!c.chimp128) return;
-  const uint64_t recs = (uint64_t)c.n_rtiles * kTile;
-  v.c128_x = reinterpret_cast<double *>(scratch);
-  v.c128_dist = scratch + recs * 8;
-  v.c128_term = reinterpret_cast<C128Term *>(scratch + recs * 9);
 }
 
 int upload_view(ebi_hip_ctx *ctx, Column &c, double *out) {
   ColView v = view_of(c, out);
-  if (c.chimp128) {
-    // earlier work on the stream may still use the scratch: it must finish
-    // before a regrow frees it
-    hipStreamSynchronize(ctx->stream);
-    if (ctx->c128_scratch.ensure(c128_scratch_need(c))) return fail(ctx, EBI_OOM, "Chimp128 decode scratch");
-    bind_c128_scratch(v, c, static_cast<uint8_t *>(ctx->c128_scratch.p));
-  }
   *ctx->h_view = v;
   hipError_t e = hipMemcpyAsync(c.d_view, ctx->h_view, sizeof(ColView), hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(ColView)");
@@ -147,14 +129,12 @@ int check_chunk_errors(ebi_hip_ctx *ctx, const Column &c, int64_t chunk_id) {
 // Queue the kernels that write every record of the selected chunks of `c`
 // to out (ColView already uploaded).  chunk_id < 0: all chunks.
 int enqueue_decode(ebi_hip_ctx *ctx, Column &c) {
-  // Chimp128 always parses in segments (the resolve kernel needs them)
+  // Chimp128 always decodes in segment jobs (parse and values in one pass)
   const uint32_t segs = ctx->xor_mode == 0 || c.chimp128 ? c.n_seg_jobs : 0;
   launch::decode(ctx->stream, c.d_view, 1, c.d_prefix, c.n_tile_jobs, c.d_prefix + 6, segs, ctx->d_flags,
                  (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg);
   hipEventRecord(ctx->ev_mid, ctx->stream);
-  if (c.chimp128)
-    launch::chimp128_resolve(ctx->stream, c.d_view, c.d_prefix + 4, 1, (uint32_t)c.chunks.size(), ctx->d_flags);
-  else if (c.xor_codec && ctx->xor_mode != 0)
+  if (c.xor_codec && ctx->xor_mode != 0)
     launch::serial(ctx->stream, c.d_view, c.d_prefix + 2, 1, c.n_serial_jobs, nullptr, ctx->d_flags, true, false);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(ctx, e, "decode launch");
@@ -167,6 +147,7 @@ static void free_column(Column &c) {
   hipFree(c.d_tile_index);
   hipFree(c.d_ckpt);
   hipFree(c.d_ilv);
+  hipFree(c.d_c128_ring);
   hipFree(c.d_chunk_err);
   hipFree(c.d_view);
   hipFree(c.d_prefix);
@@ -223,6 +204,33 @@ static int build_ilv(ebi_hip_ctx *ctx, Column &col) {
   col.ilv_rows = total;
   col.device_bytes += total * kIlvRowBlocks * 16;
   return EBI_OK;
+}
+
+// Ring checkpoints of a Chimp128 column (after the index pass): the column
+// is decoded once by the lane-per-chunk decoder into a temporary buffer, and
+// each segment job keeps the values of the 128 records before it
+// (128 * 8 bytes per 1024 records).
+static int build_c128_ring(ebi_hip_ctx *ctx, Column &col) {
+  const uint64_t N = col.meta.number_of_records;
+  double *tmp = nullptr;
+  hipError_t e = hipMalloc((void **)&col.d_c128_ring, (size_t)col.n_seg_jobs * 128 * 8);
+  if (e == hipSuccess) e = hipMalloc((void **)&tmp, std::max<uint64_t>(N, 1) * 8);
+  if (e != hipSuccess) {
+    hipFree(tmp);
+    return hip_fail(ctx, e, "hipMalloc(Chimp128 ring checkpoints)");
+  }
+  int rc = upload_view(ctx, col, tmp);
+  if (!rc) {
+    launch::chimp128(ctx->stream, col.d_view, col.d_prefix + 2, 1, col.n_serial_jobs, nullptr, ctx->d_flags, true);
+    launch::c128_ring(ctx->stream, col.d_view, col.n_seg_jobs, col.d_c128_ring);
+    e = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "Chimp128 ring checkpoints");
+  }
+  hipFree(tmp);
+  if (!rc) col.device_bytes += (uint64_t)col.n_seg_jobs * 128 * 8;
+  return rc;
 }
 
 // Parse, validate codec headers, repack into the device layout, upload and
@@ -309,7 +317,7 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
       }
       default:  // XOR codecs: the bit stream as written
         // Chimp128 keeps parse state for records 0..n (the terminator too)
-        h.n_segs = col.chimp128 ? n / kSeg + 1 : (n + kSeg - 1) / kSeg;
+        h.n_segs = col.chimp128 ? n / kSeg128 + 1 : (n + kSeg - 1) / kSeg;
         break;
     }
     bodies[k] = {src, dev_bytes, body_len};
@@ -398,8 +406,10 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
       for (uint32_t t = 0; t < nt; t += kTilesPerJob)
         col.h_tile_jobs.push_back(Job{std::min(kTilesPerJob, nt - t), 0, (uint32_t)k, t});
     }
+    // Job::col: the job's index (Chimp128's ring checkpoints); the other
+    // codecs' first ilv row, set once the ilv is built
     for (uint32_t s = 0; s < col.chunks[k].n_segs; s += kWaveLanes)
-      col.h_seg_jobs.push_back(Job{kJobSeg, 0, (uint32_t)k, s});
+      col.h_seg_jobs.push_back(Job{kJobSeg, (uint32_t)col.h_seg_jobs.size(), (uint32_t)k, s});
   }
   col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
   std::vector<Job> rec_jobs;
@@ -448,7 +458,8 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return hip_fail(ctx, e, "index pass");
   for (size_t k = 0; k < nc; ++k) col.chunks[k].err = errs[k];
-  if ((col.xor_codec || col.chimp128) && col.n_seg_jobs && (rc = build_ilv(ctx, col))) return rc;
+  if (col.xor_codec && col.n_seg_jobs && (rc = build_ilv(ctx, col))) return rc;
+  if (col.chimp128 && col.n_seg_jobs && (rc = build_c128_ring(ctx, col))) return rc;
   if (scheme == kRLE && nc) {
     // Balance RLE decode jobs by output: a job expands up to kTilesPerJob
     // steps of kTile output records of one field tile (runs of up to 255
@@ -547,7 +558,6 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   ctx->gemm_a.release();
   ctx->gemm_b.release();
   ctx->gemm_c.release();
-  ctx->c128_scratch.release();
   if (ctx->h_batch) hipHostFree(ctx->h_batch);
   if (ctx->d_flags) hipFree(ctx->d_flags);
   if (ctx->h_flags) hipHostFree(ctx->h_flags);
@@ -758,22 +768,15 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   uint32_t *pre = reinterpret_cast<uint32_t *>(ctx->h_batch);
   uint32_t *pre128 = pre + (n_cols + 1), *preseg = pre128 + (n_cols + 1);
   ColView *views = reinterpret_cast<ColView *>(ctx->h_batch + pre_bytes);
-  uint64_t jobs = 0, jobs128 = 0, segs = 0, scratch = 0;
-  for (size_t i = 0; i < n_cols; ++i) scratch += c128_scratch_need(cols[i]->c);
-  if (scratch && (hipStreamSynchronize(ctx->stream), ctx->c128_scratch.ensure(scratch)))
-    return fail(ctx, EBI_OOM, "Chimp128 decode scratch");
-  scratch = 0;
+  uint64_t jobs = 0, jobs128 = 0, segs = 0;
   for (size_t i = 0; i < n_cols; ++i) {
     const Column &c = cols[i]->c;
     pre[i] = (uint32_t)jobs;
     pre128[i] = (uint32_t)jobs128;
     preseg[i] = (uint32_t)segs;
     views[i] = view_of(c, d_outs[i]);
-    bind_c128_scratch(views[i], c, static_cast<uint8_t *>(ctx->c128_scratch.p) + scratch);
-    scratch += c128_scratch_need(c);
     jobs += c.n_tile_jobs;
     segs += c.n_seg_jobs;
-    if (c.chimp128) jobs128 += c.chunks.size();
   }
   pre[n_cols] = (uint32_t)jobs;
   pre128[n_cols] = (uint32_t)jobs128;
@@ -786,12 +789,10 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   launch::zero_u32(ctx->stream, ctx->d_flags, 2);
   hipEventRecord(ctx->ev0, ctx->stream);
   const ColView *dv = (const ColView *)(d + pre_bytes);
-  const uint32_t *pre128_d = (const uint32_t *)d + (n_cols + 1);
   launch::decode(ctx->stream, dv, (uint32_t)n_cols, (const uint32_t *)d, (uint32_t)jobs,
                  (const uint32_t *)d + 2 * (n_cols + 1), (uint32_t)segs, ctx->d_flags,
                  (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg);
   hipEventRecord(ctx->ev_mid, ctx->stream);
-  launch::chimp128_resolve(ctx->stream, dv, pre128_d, (uint32_t)n_cols, (uint32_t)jobs128, ctx->d_flags);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "batch launch");
   hipEventRecord(ctx->ev1, ctx->stream);
   if ((rc = sync_and_check(ctx, "batch decode"))) return rc;

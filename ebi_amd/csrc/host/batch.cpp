@@ -2,7 +2,7 @@
 // device outputs, replayed many times.  The job-space arguments are uploaded
 // once; an autotuner times candidate persistent grid sizes on this very
 // batch and keeps the fastest; the launch sequence is captured in a hipGraph
-// and replayed with one launch.  The batch owns its Chimp128 decode scratch.  No counterpart in lemolatoon/ebi,
+// and replayed with one launch.  No counterpart in lemolatoon/ebi,
 // whose Decoder decodes chunk after chunk on the CPU (api/decoder.rs:202-241).
 #include <algorithm>
 #include <cstring>
@@ -19,7 +19,6 @@ struct ebi_hip_batch {
   std::vector<Column *> cols;
   std::vector<double *> outs;
   DevBuf args;     // prefix[n + 1] | prefix128[n + 1] | prefix_seg[n + 1] | pad | ColView[n]
-  DevBuf scratch;  // Chimp128 decode scratch of the batch's columns
   uint64_t pre_bytes = 0;
   uint32_t n_cols = 0, jobs = 0, jobs128 = 0, segs = 0;
   ebi_batch_config cfg{};
@@ -43,11 +42,10 @@ void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool
   launch::decode(s, views_of(b), b->n_cols, prefix_of(b), b->jobs, prefix_of(b) + 2 * (b->n_cols + 1), b->segs,
                  ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr);
   if (timed) hipEventRecord(ctx->ev_mid, s);
-  launch::chimp128_resolve(s, views_of(b), prefix_of(b) + (b->n_cols + 1), b->n_cols, b->jobs128, ctx->d_flags);
   if (timed) hipEventRecord(ctx->ev1, s);
 }
 
-// phase 0: k_decode, 1: k_chimp128_resolve, 2: k_decode_seg
+// phase 0: k_decode, 1: none (kept 0), 2: k_decode_seg
 void read_times(ebi_hip_ctx *ctx) {
   float a = 0, c = 0, g = 0, t = 0;
   hipEventElapsedTime(&a, ctx->ev0, ctx->ev_seg);
@@ -94,7 +92,6 @@ void destroy(ebi_hip_batch *b) {
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   b->args.release();
-  b->scratch.release();
   delete b;
 }
 
@@ -118,13 +115,7 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   uint32_t *pre = reinterpret_cast<uint32_t *>(host.data()), *pre128 = pre + (n_cols + 1),
            *preseg = pre128 + (n_cols + 1);
   ColView *views = reinterpret_cast<ColView *>(host.data() + b->pre_bytes);
-  uint64_t jobs = 0, jobs128 = 0, segs = 0, scratch = 0;
-  for (size_t i = 0; i < n_cols; ++i) scratch += c128_scratch_need(cols[i]->c);
-  if (scratch && b->scratch.ensure(scratch)) {
-    destroy(b);
-    return fail(ctx, EBI_OOM, "batch Chimp128 scratch");
-  }
-  scratch = 0;
+  uint64_t jobs = 0, jobs128 = 0, segs = 0;
   for (size_t i = 0; i < n_cols; ++i) {
     Column &c = const_cast<ebi_hip_col *>(cols[i])->c;
     b->cols.push_back(&c);
@@ -133,11 +124,8 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
     pre128[i] = (uint32_t)jobs128;
     preseg[i] = (uint32_t)segs;
     views[i] = view_of(c, d_outs[i]);
-    bind_c128_scratch(views[i], c, static_cast<uint8_t *>(b->scratch.p) + scratch);
-    scratch += c128_scratch_need(c);
     jobs += c.n_tile_jobs;
     segs += c.n_seg_jobs;
-    if (c.chimp128) jobs128 += c.chunks.size();
   }
   pre[n_cols] = (uint32_t)jobs;
   pre128[n_cols] = (uint32_t)jobs128;

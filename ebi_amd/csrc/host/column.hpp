@@ -42,7 +42,8 @@ struct Column {
   DevChunk *d_chunks = nullptr;
   int64_t *d_tile_index = nullptr;
   XorCkpt *d_ckpt = nullptr;
-  uint4 *d_ilv = nullptr;    // lane-interleaved segment streams (XOR codecs)
+  uint4 *d_ilv = nullptr;    // lane-interleaved segment streams (Gorilla, Chimp, Elf)
+  uint64_t *d_c128_ring = nullptr;  // Chimp128: per segment job the 128 values before it
   uint64_t ilv_rows = 0;     // 1 KiB rows of d_ilv
   uint32_t *d_chunk_err = nullptr;
   ColView *d_view = nullptr;  // slot used by single-column launches
@@ -98,5 +99,4 @@ struct ebi_hip_ctx {
   // launch shape of the decode kernels (defaults; a prepared batch tunes its own)
   uint32_t blocks_per_cu = 8;   // persistent k_decode grid = cu_count * blocks_per_cu
   int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks (Gorilla, Chimp, Elf)
-  ebi::DevBuf c128_scratch;     // Chimp128 decode scratch of the synchronous calls
 };

@@ -22,14 +22,7 @@ int map_dev_err(uint32_t flags);
 bool is_xor_codec(int scheme);
 // The ColView of `c` with output base `out`.
 ColView view_of(const Column &c, double *out);
-// Device bytes of the Chimp128 decode scratch one decode of `c` needs
-// (0 for other codecs; a multiple of 256).
-uint64_t c128_scratch_need(const Column &c);
-// Points v's Chimp128 scratch fields into `scratch` (c128_scratch_need(c)
-// bytes); no-op for other codecs.
-void bind_c128_scratch(ColView &v, const Column &c, uint8_t *scratch);
-// Uploads the single-column ColView of `c` with output base `out` and waits
-// (a Chimp128 column's decode scratch is the context's).
+// Uploads the single-column ColView of `c` with output base `out` and waits.
 int upload_view(ebi_hip_ctx *ctx, Column &c, double *out);
 // Copies the device error word back (synchronising the stream) and maps it.
 int sync_and_check(ebi_hip_ctx *ctx, const char *what);

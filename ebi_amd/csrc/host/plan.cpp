@@ -107,7 +107,7 @@ bool writes_bits(const Plan &p) {
 
 // Offsets of the device scratch of a plan (all 256-byte aligned).
 struct Layout {
-  uint64_t mask, bits, sel, cpred, acc, part, csum, block, values, out, c128, total;
+  uint64_t mask, bits, sel, cpred, acc, part, csum, block, values, out, total;
 };
 
 Layout layout_of(const Plan &p) {
@@ -132,7 +132,6 @@ Layout layout_of(const Plan &p) {
   // DECODE writes the caller's buffer; fused plans keep the records in registers
   L.values = p.decode && !p.fused && p.q.op != kOpDecode ? take(N * 8) : ~0ull;
   L.out = (p.q.op == kOpFilterMaterialize || (p.q.op == kOpMaterialize && p.has_mask)) ? take(N * 8) : ~0ull;
-  L.c128 = p.decode && p.c->chimp128 ? take(c128_scratch_need(*p.c)) : ~0ull;  // Chimp128 decode scratch
   L.total = off;
   return L;
 }
@@ -213,10 +212,7 @@ hipError_t plan_enqueue_post(Plan &p, hipStream_t s, uint32_t *err, uint32_t max
     // selection, per-chunk fold and copies as below
     QueryArgs a = args_of(p);
     const int agg = op == kOpSum || op == kOpFilterSum ? 0 : op == kOpMin ? 1 : op == kOpMax ? 2 : -1;
-    if (c.chimp128)  // the parse ran in the group's decode launch; the values form here
-      launch::chimp128_query(s, p.d_view, p.nc, a, has_pred(op), writes_bits(p), agg, err);
-    else
-      launch::query_seg(s, p.d_view, c.n_seg_jobs, a, has_pred(op), writes_bits(p), agg, p.nc, err, max_blocks);
+    launch::query_seg(s, p.d_view, c.n_seg_jobs, a, has_pred(op), writes_bits(p), agg, p.nc, err, max_blocks);
     if (op == kOpFilter) {
       if (p.words) hipMemcpyAsync(p.h_bits, p.bits, p.words * 8, hipMemcpyDeviceToHost, s);
       return hipGetLastError();
@@ -327,12 +323,11 @@ int plan_init(Plan &p, const ebi_query &q) {
       break;
   }
   // Gorilla, Chimp, Elf and Chimp128 filters and aggregates run fused: the
-  // segment decode (Chimp128: its value-forming pass, after the segment
-  // parse) feeds the query in registers
+  // segment decode feeds the query in registers
   const bool seg_codec = scheme == kGorilla || scheme == kChimp || scheme == kElf || scheme == kChimp128;
   if (seg_codec && p.c->n_seg_jobs && (q.op == kOpFilter || q.op == kOpFilterSum || is_agg(q.op))) {
     p.fused = true;
-    p.decode = scheme == kChimp128;  // the parse runs in the group's decode launch
+    p.decode = false;
   }
   return EBI_OK;
 }
@@ -357,7 +352,6 @@ void plan_bind(Plan &p, uint8_t *dev, uint8_t *pin) {
   p.d_n = p.block + launch::compact_blocks(p.words);
   p.values = p.q.op == kOpDecode ? p.q.out_values : at<double>(dev, L.values);
   p.out = at<double>(dev, L.out);
-  p.c128 = at<uint8_t>(dev, L.c128);
   const PinLayout P = pin_layout_of(p);
   p.h_mask = at<uint64_t>(pin, P.mask);
   p.h_bits = at<uint64_t>(pin, P.bits);
@@ -514,14 +508,12 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   for (size_t i = 0; i < n; ++i) {
     Plan &p = *g.plans[i];
     views[i] = view_of(*p.c, p.values);
-    if (p.c128) bind_c128_scratch(views[i], *p.c, p.c128);
     pre[i] = (uint32_t)jobs;
     pre128[i] = (uint32_t)jobs128;
     preseg[i] = (uint32_t)segs;
     if (p.decode) {
       jobs += p.c->n_tile_jobs;
       segs += p.c->n_seg_jobs;
-      if (p.c->chimp128 && !p.fused) jobs128 += p.c->chunks.size();
     }
   }
   pre[n] = (uint32_t)jobs;
@@ -537,7 +529,6 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   const uint32_t *dpre = reinterpret_cast<const uint32_t *>(g.dev + vbytes);
   launch::decode(s, dv, (uint32_t)n, dpre, (uint32_t)jobs, dpre + 2 * (pbytes / 4), (uint32_t)segs, g.d_flags,
                  (uint32_t)cu_count * blocks_per_cu);
-  launch::chimp128_resolve(s, dv, dpre + (pbytes / 4), (uint32_t)n, (uint32_t)jobs128, g.d_flags);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   for (Plan *p : g.plans)
     if ((e = plan_enqueue_post(*p, s, g.d_flags, (uint32_t)cu_count * blocks_per_cu)) != hipSuccess) return e;

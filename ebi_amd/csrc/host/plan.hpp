@@ -56,7 +56,7 @@ struct Plan {
   bool has_mask = false;
   bool cpred = false;      // per-chunk predicate parameters are uploaded
   bool fused = false;      // filter or aggregate of an XOR codec: decoded and queried in one kernel
-                           // (k_query_seg; Chimp128: its resolve pass), the records never written to HBM
+                           // (k_query_seg), the records never written to HBM
   int filter_mode = 0;     // k_filter mode (0 select, 1 code domain / f64, 2 f64)
   uint64_t words = 0;      // ceil(N / 64)
   uint32_t nc = 0;         // chunks
@@ -66,7 +66,6 @@ struct Plan {
   uint32_t *sel = nullptr;
   ChunkPred *d_cpred = nullptr;
   double *partials = nullptr, *chunk_sum = nullptr, *values = nullptr, *out = nullptr;
-  uint8_t *c128 = nullptr;  // Chimp128 decode scratch (decode plans of Chimp128 columns)
   // pinned staging
   uint64_t *h_mask = nullptr, *h_bits = nullptr, *h_acc = nullptr, *h_n = nullptr;
   uint32_t *h_sel = nullptr;

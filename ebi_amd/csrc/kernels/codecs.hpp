@@ -591,35 +591,31 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
 }
 
 // Chimp128 parse, decoder/chunk_reader/chimp_n.rs:173-285, in the lockstep
-// form.  Record i is parsed as (source distance, XOR payload): value_i =
-// value_{i - dist} ^ x.  Flags 10 / 11 use the previous record (dist 1);
-// flags 00 / 01 name ring slot idx, i.e. the latest record j <= i - 1 with
-// j == idx (mod 128), so dist = i - j is in [1, 128].  dist 0: no source --
-// record 0 (x = its 64 raw bits) or an unwritten ring slot (value 0) -- so
-// value_i = x.  No values are formed here (that needs the ring): the
-// resolve pass does it.  Record i < n goes to the sink as its payload and
-// to byte k of dist16 (k = i - a < kRun) as its distance; the terminator n
-// goes to *term.  Record 0 is parsed before, by chimp128_seg_first.
+// form, over one segment of kSeg128 records per lane.  Record i is parsed
+// as (source distance, XOR payload): value_i = value_{i - dist} ^ x.  Flags
+// 10 / 11 use the previous record (dist 1); flags 00 / 01 name ring slot
+// idx, i.e. the latest record j <= i - 1 with j == idx (mod 128), so dist =
+// i - j is in [1, 128].  dist 0: no source -- record 0 (x = its 64 raw bits)
+// or an unwritten ring slot (value 0) -- so value_i = x.  No values are
+// formed here (that needs the ring): the segment kernel resolves them after
+// the parse.  Record i (i <= n, the terminator n included) goes to the sink
+// as its payload (row k = i - a) and to byte k of dist16 as its distance.
+// Record 0 is parsed before, by chimp128_seg_first.
 template <class Sink>
-__host__ __device__ __forceinline__ uint32_t chimp128_seg_first(BitWindow &r, uint32_t size_bits, uint32_t n,
-                                                       const Sink &out, uint32_t (&dist16)[4], C128Term *term) {
+__host__ __device__ __forceinline__ uint32_t chimp128_seg_first(BitWindow &r, uint32_t size_bits, Sink &out,
+                                                       uint32_t (&dist16)[4]) {
   const uint64_t x = r.field(0, 64);
   r.consume(64);
   r.refill();
-  if (n) {
-    out.row(0, 0, as_f64(x));
-    dist16[0] &= ~0xffu;
-  } else {
-    *term = C128Term{x, 0, 0};
-  }
+  out.row(0, 0, as_f64(x), true);
+  dist16[0] &= ~0xffu;
   return r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
 }
 
 template <class Sink>
 __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, uint32_t &lead, uint32_t size_bits,
-                                                       uint32_t n, uint32_t a, uint32_t cnt, uint32_t from,
-                                                       uint32_t lim, const Sink &out, uint32_t (&dist16)[4],
-                                                       C128Term *term, uint32_t e) {
+                                                       uint32_t a, uint32_t cnt, uint32_t from, uint32_t lim,
+                                                       Sink &out, uint32_t (&dist16)[4], uint32_t e) {
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t i = a + k;
     const bool act = e == 0 && i >= from && i < lim;
@@ -643,15 +639,14 @@ __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, ui
     r.refill();
     const uint32_t code = bad ? (uint32_t)kErrCorrupt : r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
     e = act ? code : e;
-    if (act && i < n) out.row(i, k, as_f64(x));
+    out.row(i, k, as_f64(x), act);
     const uint32_t sh = (k & 3) * 8, w = k >> 2;
-    const bool put = act && i < n && k < kRun;
+    const bool put = act && k < kSeg128;
     const uint32_t keep = ~(0xffu << sh), byte = dist << sh;
     dist16[0] = put && w == 0 ? (dist16[0] & keep) | byte : dist16[0];
     dist16[1] = put && w == 1 ? (dist16[1] & keep) | byte : dist16[1];
     dist16[2] = put && w == 2 ? (dist16[2] & keep) | byte : dist16[2];
     dist16[3] = put && w == 3 ? (dist16[3] & keep) | byte : dist16[3];
-    if (act && i == n) *term = C128Term{x, dist, 0};
   }
   return e;
 }
@@ -767,7 +762,8 @@ __device__ uint32_t chimp128_run(const uint8_t *data, uint32_t size_bits, uint32
 // Chimp128 parse-only pass for the load-time index: record lengths depend
 // only on flags, lead codes and the stored lead, so one lane walks records
 // 0..n (n = the terminator) without the value ring and writes a checkpoint
-// {bit position, stored lead} every kSeg records (n / kSeg + 1 entries).
+// {bit position, stored lead} every kSeg128 records (n / kSeg128 + 1
+// entries).
 // One Chimp128 record (i >= 1) at the start of window r
 // (decoder/chunk_reader/chimp_n.rs:189-285): flag 00 = 7-bit ring index;
 // 11 = 3-bit lead code, 64 - lead bits; 10 = 64 - lead bits with the stored
@@ -811,7 +807,7 @@ __host__ __device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_b
   r.init(data, 0);
   uint32_t lead = 0xffffffffu;
   for (uint32_t i = 0; i <= n; ++i) {
-    if ((i % kSeg) == 0) ck[i / kSeg] = XorCkpt{0, r.pos, (uint8_t)(lead > 64 ? 255 : lead), 0, -1, 0};
+    if ((i % kSeg128) == 0) ck[i / kSeg128] = XorCkpt{0, r.pos, (uint8_t)(lead > 64 ? 255 : lead), 0, -1, 0};
     uint32_t used = 64;
     if (i) {
       uint32_t f, idx;

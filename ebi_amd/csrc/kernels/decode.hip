@@ -92,24 +92,33 @@ struct StageOut {
   __device__ __forceinline__ void run_end(uint32_t k0) { stage_flush(sink.stage, dst, k0, n_job); }
 };
 
-// Chimp128 parse of checkpoint segments first .. first + 63 of chunk ci
-// (records 0..n, n = the terminator), segment first + l on lane l, in runs
-// of kRun records: the XOR payloads go through the output stage into the
-// scratch, the source distances (16 bytes a lane a run) straight to the
-// scratch, the terminator's parse to its slot.
-// k_chimp128_resolve then forms the values.
-__device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_t ci, const DevChunk &c, uint32_t first,
-                                                      uint32_t row, double *stage) {
+// Chimp128 segment job: segments first .. first + 63 of chunk c (kSeg128
+// records each; records 0..n, n the terminator), segment first + l on lane
+// l, one job = kJob128 records.  The wave parses every segment in one
+// lockstep run (payloads into the stage, row k = record i0 + k; source
+// distances in each lane's dist16), then resolves the job's records in
+// record order in steps of 64 (lane = record): sources inside the step by
+// pointer jumping over lane shuffles (x ^= x[src], src = src[src] while the
+// source lies in the step; sources only point back, so this ends after at
+// most six rounds), sources in earlier steps from the stage (each step's
+// values replace its payloads), sources before the job from its ring
+// checkpoint (cv.c128_ring + 128 ring_idx).  out.put(i, v, base) takes
+// record i's value (base: the step's first record; every lane calls it).
+// Record n must be the NaN end marker and an earlier NaN is a count error:
+// a NaN can only come from flags 01 / 10 / 11 (the ring never holds one, as
+// the reference stops at the first).
+template <class Out>
+__device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &c, uint32_t first, uint32_t ring_idx,
+                                            double *stage, Out &out) {
+  const uint32_t lane = lane_id();
   const uint32_t total = c.n + 1;
-  const uint32_t seg = first + lane_id();
-  const uint32_t i0 = seg * kSeg;
+  const uint32_t i0 = (first + lane) * kSeg128;
   const bool live = i0 < total;
-  const uint32_t lim_seg = live ? min(i0 + kSeg, total) : 0;  // the chunk's last segment holds the terminator
   BitWindow r;
   uint32_t lead = 0xffffffffu;
   if (live) {
-    const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[seg];
-    r.init_at(cv.ilv + (size_t)row * kIlvRowBlocks + lane_id(), k.bitpos, kIlvRowBlocks);
+    const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[first + lane];
+    r.init(reinterpret_cast<const uint8_t *>(c.data), k.bitpos);
     lead = k.a == 255 ? 0xffffffffu : k.a;
   } else {  // an empty window that is never advanced
     r.b = 0;
@@ -117,30 +126,66 @@ __device__ __forceinline__ uint32_t segment_wave_c128(const ColView &cv, uint32_
     r.hi = r.lo = 0;
   }
   const StageSink sink{stage};
-  double *dst = cv.c128_x + (size_t)c.rtile0 * kTile + (size_t)first * kSeg;
-  const uint32_t n_job = first * kSeg < c.n ? c.n - first * kSeg : 0;
-  uint8_t *dist = cv.c128_dist + (size_t)c.rtile0 * kTile;  // 16-byte aligned, ceil(n / kTile) * kTile bytes
-  C128Term *term = cv.c128_term + ci;
   const uint32_t bits = c.size * 8u;
   uint32_t from = i0, e = 0;
   uint32_t d16[4] = {0, 0, 0, 0};
   if (live && i0 == 0) {
-    e = chimp128_seg_first(r, bits, c.n, sink, d16, term);
+    e = chimp128_seg_first(r, bits, sink, d16);
     from = 1;
   }
+  e = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, live ? min(i0 + kSeg128, total) : 0u, sink, d16, e);
+  wave_sync();  // the stage holds every payload of the job
+  const uint32_t j0 = first * kSeg128;  // the job's first record
+  const EBI_GLOBAL uint64_t *ring = as_global(cv.c128_ring + 128ull * ring_idx);
 #pragma unroll 1
-  for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun) {
-    const uint32_t a = i0 + k0;
-    // a segment ends at a multiple of kSeg or at the terminator, so a run
-    // never needs a record past a + kRun - 1
-    const uint32_t lim = min(lim_seg, a + kRun);
-    e = chimp128_parse_seg(r, lead, bits, c.n, a, kRun, from, lim, sink, d16, term, e);
-    stage_flush(stage, dst, k0, n_job);
-    // a < n: the 16 bytes from a lie inside the chunk's distance region
-    if (live && a < c.n) *reinterpret_cast<uint4 *>(dist + a) = make_uint4(d16[0], d16[1], d16[2], d16[3]);
+  for (uint32_t s = 0; s < kJob128 / kWave; ++s) {
+    const uint32_t base = j0 + s * kWave;
+    if (base >= total) break;
+    const uint32_t t = s * kWave + lane;  // record j0 + t: row t % 16 of lane t / 16's segment
+    const uint32_t i = j0 + t;
+    const uint32_t sg = t / kSeg128, kk = t % kSeg128;
+    double *slot = stage + kk * kWave + (sg ^ stage_swz(kk));
+    uint64_t v = as_u64(*slot);
+    const uint32_t w0 = __shfl(d16[0], sg), w1 = __shfl(d16[1], sg), w2 = __shfl(d16[2], sg), w3 = __shfl(d16[3], sg);
+    const uint32_t wsel = kk >> 2;
+    const uint32_t word = wsel == 0 ? w0 : wsel == 1 ? w1 : wsel == 2 ? w2 : w3;
+    const uint32_t d = i < total ? (word >> ((kk & 3) * 8)) & 0xffu : 0u;
+    int32_t p = d ? (int32_t)(i - d) : -1;  // source record, -1: none (value 0)
+    while (__any(p >= (int32_t)base)) {
+      const int32_t q = (p - (int32_t)base) & (kWave - 1);  // source lane (used only when in the step)
+      const uint64_t vq = __shfl(v, q);
+      const int32_t pq = __shfl(p, q);
+      const bool in = p >= (int32_t)base;
+      v = in ? v ^ vq : v;
+      p = in ? pq : p;
+    }
+    if (p >= (int32_t)j0) {  // an earlier step of this job: its value replaced the payload
+      const uint32_t tp = (uint32_t)p - j0;
+      v ^= as_u64(stage[(tp % kSeg128) * kWave + ((tp / kSeg128) ^ stage_swz(tp % kSeg128))]);
+    } else if (p >= 0) {  // before the job: its ring checkpoint
+      v ^= ring[(uint32_t)p + 128 - j0];
+    }
+    if (i < total) *slot = as_f64(v);
+    wave_sync();  // this step's values are in the stage before the next step reads them
+    if (i < c.n) {
+      if (v == kNan) e |= kErrCount;
+    } else if (i == c.n && v != kNan) {
+      e |= kErrCount;
+    }
+    out.put(i, v, base);
   }
   return e;
 }
+
+// Decode output of c128_job: record i < n to cv.out (a step's 64 records are
+// one 512-byte store).
+struct C128Out {
+  double *dst;  // record 0 of the chunk
+  uint32_t n;
+  __device__ __forceinline__ void put(uint32_t i, uint64_t v, uint32_t) {
+    if (i < n) st_one(dst + i, as_f64(v));
+  }
+};
 
 __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, uint32_t first, uint32_t row,
                                                double *stage, const double *elf_tab) {
@@ -151,7 +196,10 @@ __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, 
     case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, row, elf_tab, out);
     case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, row, elf_tab, out);
     case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, row, elf_tab, out);
-    case kCodecChimp128: return segment_wave_c128(cv, ci, c, first, row, stage);
+    case kCodecChimp128: {
+      C128Out o{cv.out + c.rec_off, c.n};
+      return c128_job(cv, c, first, row, stage, o);
+    }
     default: return kErrCorrupt;
   }
 }
@@ -256,9 +304,87 @@ struct QueryOut {
   __device__ __forceinline__ void run_end(uint32_t) {}
 };
 
-// Fused queries over the checkpoint-segment jobs of one Gorilla / Chimp / Elf
-// column (cols[0]): the records are decoded as in k_decode_seg and consumed
-// in registers.  flags: bit 0 apply the f64 predicate, bit 1 write the
+// Query output of c128_job (the fused queries of a Chimp128 column), flags
+// as k_query_seg's: per step one ballot of selection bits (the step's 64
+// consecutive records), per record tile (8 steps; a job holds two) one
+// double-double partial, the butterfly sum of the lanes' sums; min / max
+// keys per lane.
+struct C128Query {
+  const QueryArgs *a;
+  const DevChunk *c;
+  bool pred, want_bits;
+  int agg;
+  double hi, lo;
+  uint64_t kmin, kmax;
+  __device__ __forceinline__ void put(uint32_t i, uint64_t vb, uint32_t base) {
+    const uint32_t n = c->n;
+    const uint64_t gi = c->rec_off + i;
+    const double v = as_f64(vb);
+    const bool m = i < n && (!a->mask || ((a->mask[gi >> 6] >> (gi & 63)) & 1));
+    const bool sel = m && (!pred || pred_f64(*a, v));
+    if (want_bits) {
+      const uint64_t bal = __ballot(sel);
+      if (lane_id() == 0 && base < n) {
+        // the step's word: the wave's own when aligned and inside the
+        // chunk, atomic ORs when it is shared
+        const uint64_t g0 = c->rec_off + base;
+        unsigned long long *w = reinterpret_cast<unsigned long long *>(a->out_bits) + (g0 >> 6);
+        const uint32_t sh = (uint32_t)(g0 & 63);
+        if (sh == 0 && base + kWave <= n) {
+          *w = bal;
+        } else if (bal) {
+          atomicOr(w, (unsigned long long)(bal << sh));
+          if (sh) atomicOr(w + 1, (unsigned long long)(bal >> (64 - sh)));
+        }
+      }
+    }
+    if (agg == 0) {
+      if (sel) dd_add(hi, lo, v, 0.0);
+      // the last step with records of its tile: the tile's partial
+      if (base < n && ((base % kTile) == kTile - kWave || base + kWave >= n)) {
+        const uint32_t lane = lane_id();
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+          const double oh = __shfl_xor(hi, d, kWave), ol = __shfl_xor(lo, d, kWave);
+          if ((lane & d) == 0) {
+            dd_add(hi, lo, oh, ol);
+          } else {
+            double h2 = oh, l2 = ol;
+            dd_add(h2, l2, hi, lo);
+            hi = h2;
+            lo = l2;
+          }
+        }
+        if (lane == 0) {
+          double *pp = a->partials + 2ull * (c->rtile0 + base / kTile);
+          pp[0] = hi;
+          pp[1] = lo;
+        }
+        hi = lo = 0.0;
+      }
+    } else if (agg > 0 && sel && v == v) {
+      kmin = min(kmin, f64_key(v));
+      kmax = max(kmax, f64_key(v));
+    }
+  }
+};
+
+// Wave min / max of the lanes' keys into chunk accumulators acc[2] / acc[3].
+__device__ __forceinline__ void keys_to_acc(int agg, uint64_t kmin, uint64_t kmax, uint64_t *acc) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    kmin = min(kmin, (uint64_t)__shfl_xor((unsigned long long)kmin, d, kWave));
+    kmax = max(kmax, (uint64_t)__shfl_xor((unsigned long long)kmax, d, kWave));
+  }
+  if (lane_id() == 0) {
+    if (agg == 1) atomicMin(reinterpret_cast<unsigned long long *>(acc) + 2, (unsigned long long)kmin);
+    else atomicMax(reinterpret_cast<unsigned long long *>(acc) + 3, (unsigned long long)kmax);
+  }
+}
+
+// Fused queries over the checkpoint-segment jobs of one Gorilla / Chimp /
+// Elf / Chimp128 column (cols[0]): the records are decoded as in
+// k_decode_seg and consumed in registers.  flags: bit 0 apply the f64 predicate, bit 1 write the
 // selected records' bits to a.out_bits (zeroed before), bits 2-3 aggregate
 // (0 none, 1 sum, 2 min, 3 max).  Sums go to the per-record-tile
 // double-double partials (k_chunk_dd folds them per chunk), each the
@@ -266,6 +392,7 @@ struct QueryOut {
 // accumulators (k_acc_init before).  Chunks with a.chunk_sel 0 are skipped.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_query_seg(
     const ColView *cols, uint32_t n_jobs, QueryArgs a, uint32_t flags, uint32_t *err) {
+  __shared__ __align__(16) double lds[4][(kRun + 1) * kWave];  // Chimp128 jobs: the payload / value stage
   __shared__ double elf_tab[kElfTab];
   if (threadIdx.x < kElfTab) elf_tab_fill(elf_tab, threadIdx.x, c_p10, c_n10);
   __syncthreads();
@@ -278,6 +405,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const Job jb = cv.seg_jobs[j];
     if (!a.chunk_sel[jb.chunk]) continue;
     const DevChunk c = cv.chunks[jb.chunk];
+    if (c.codec == kCodecChimp128) {
+      C128Query q{&a, &c, (flags & 1) != 0, want_bits, agg, 0.0, 0.0, ~0ull, 0};
+      const uint32_t e = c128_job(cv, c, jb.tile, jb.col, lds[threadIdx.x / kWave], q);
+      if (e) atomicOr(err, e);
+      if (agg > 0) keys_to_acc(agg, q.kmin, q.kmax, a.acc + 4ull * jb.chunk);
+      continue;
+    }
     const uint32_t i0 = (jb.tile + lane) * kSeg;
     const uint64_t g = c.rec_off + i0;
     uint64_t m = ~0ull;
@@ -327,16 +461,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         p[1] = q.lo;
       }
     } else if (agg > 0) {
-#pragma unroll
-      for (int d = 32; d >= 1; d >>= 1) {
-        q.kmin = min(q.kmin, (uint64_t)__shfl_xor((unsigned long long)q.kmin, d, kWave));
-        q.kmax = max(q.kmax, (uint64_t)__shfl_xor((unsigned long long)q.kmax, d, kWave));
-      }
-      unsigned long long *acc = reinterpret_cast<unsigned long long *>(a.acc) + 4ull * jb.chunk;
-      if (lane == 0) {
-        if (agg == 1) atomicMin(acc + 2, (unsigned long long)q.kmin);
-        else atomicMax(acc + 3, (unsigned long long)q.kmax);
-      }
+      keys_to_acc(agg, q.kmin, q.kmax, a.acc + 4ull * jb.chunk);
     }
   }
 }
@@ -406,154 +531,22 @@ __global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const uint
   }
 }
 
-// Chimp128 resolve (decoder/chunk_reader/chimp_n.rs:173-285, the value
-// side), after k_decode_seg left each record's XOR payload and source
-// distance in the launch's scratch: one wave per chunk walks
-// records 0..n in steps of 64 (lane l = record base + l).  Sources inside
-// the step are followed by pointer jumping over lane shuffles (x ^= x[src],
-// src = src[src] while the source lies in the step; sources only point
-// back, so this ends, after at most six rounds), then value = ring[src] ^ x
-// with the ring the last 256 values in LDS (a source is at most 128 records
-// back).  Loads and stores go in batches of kC128Batch steps, so a batch's
-// memory traffic is waited for once.
-// Record n must be the NaN end marker and an earlier NaN is a count error:
-// a NaN can only come from flags 01 / 10 / 11 (the ring never holds one,
-// as the reference stops at the first).
-//
-// kQuery: the fused queries of one Chimp128 column (cols[0], chunk j): the
-// values go to a query sink instead of cv.out, as in k_query_seg (flags
-// alike): a step's 64 consecutive records give one ballot of selection bits,
-// a batch of 8 steps is one record tile (its double-double partial the
-// butterfly sum of the lanes' sums), min / max keys go to the chunk
-// accumulators.  Chunks with a.chunk_sel 0 are skipped.
-constexpr uint32_t kC128Batch = 8;
-template <bool kQuery>
-__global__ void __launch_bounds__(256) k_chimp128_resolve(const ColView *cols, const uint32_t *prefix,
-                                                         uint32_t n_cols, uint32_t n_chunks, uint32_t *err,
-                                                         QueryArgs a, uint32_t flags) {
-  __shared__ uint64_t ring[4][256];
-  const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
-  const uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
-  if (j >= n_chunks) return;
-  const uint32_t col = kQuery ? 0u : find_col(prefix, n_cols, j);
-  const ColView cv = cols[col];
-  const uint32_t ci = kQuery ? j : j - prefix[col];
-  if (cv.chunk_err && cv.chunk_err[ci]) return;
-  if (kQuery && !a.chunk_sel[ci]) return;
-  const bool pred = flags & 1, want_bits = (flags >> 1) & 1;
-  const int agg = (int)((flags >> 2) & 3) - 1;
-  double hi = 0.0, lo = 0.0;
-  uint64_t kmin = ~0ull, kmax = 0;
-  const DevChunk c = cv.chunks[ci];
-  double *out = cv.out + c.rec_off;
-  const EBI_GLOBAL double *px = as_global(cv.c128_x + (size_t)c.rtile0 * kTile);
-  const EBI_GLOBAL uint8_t *dist = as_global(cv.c128_dist + (size_t)c.rtile0 * kTile);
-  const C128Term tm = cv.c128_term[ci];
-  uint64_t *R = ring[wave];
-  const uint32_t n = c.n, total = n + 1;
-  uint32_t e = 0;
-  for (uint32_t b0 = 0; b0 < total; b0 += kC128Batch * kWave) {
-    uint64_t x[kC128Batch];
-    uint32_t d[kC128Batch];
-#pragma unroll
-    for (uint32_t g = 0; g < kC128Batch; ++g) {
-      const uint32_t i = b0 + g * kWave + lane;
-      x[g] = i < n ? as_u64(px[i]) : i == n ? tm.x : 0;
-      d[g] = i < n ? dist[i] : i == n ? tm.dist : 0;
-    }
-#pragma unroll
-    for (uint32_t g = 0; g < kC128Batch; ++g) {
-      const int32_t base = (int32_t)(b0 + g * kWave);
-      const uint32_t i = (uint32_t)base + lane;
-      int32_t p = d[g] ? (int32_t)(i - d[g]) : -1;  // source record, -1: none (value 0)
-      uint64_t v = x[g];
-      while (__any(p >= base)) {
-        const int32_t q = (p - base) & (kWave - 1);  // source lane (used only when in the step)
-        const uint64_t vq = __shfl(v, q);
-        const int32_t pq = __shfl(p, q);
-        const bool in = p >= base;
-        v = in ? v ^ vq : v;
-        p = in ? pq : p;
-      }
-      v ^= p < 0 ? 0 : R[(uint32_t)p & 255];
-      if (i < total) R[i & 255] = v;
-      wave_sync();  // this step's values are in the ring before the next step reads them
-      if (i < n) {
-        if (v == kNan) e |= kErrCount;
-      } else if (i == n && v != kNan) {
-        e |= kErrCount;
-      }
-      x[g] = v;
-      if (kQuery) {
-        const uint64_t gi = c.rec_off + i;
-        const double fv = as_f64(v);
-        const bool m = i < n && (!a.mask || ((a.mask[gi >> 6] >> (gi & 63)) & 1));
-        const bool sel = m && (!pred || pred_f64(a, fv));
-        if (want_bits) {
-          const uint64_t bal = __ballot(sel);
-          if (lane == 0 && (uint32_t)base < n) {
-            // the step's word: the wave's own when aligned and inside the
-            // chunk, atomic ORs when it is shared
-            const uint64_t g0 = c.rec_off + (uint32_t)base;
-            unsigned long long *w = reinterpret_cast<unsigned long long *>(a.out_bits) + (g0 >> 6);
-            const uint32_t sh = (uint32_t)(g0 & 63);
-            if (sh == 0 && (uint32_t)base + kWave <= n) {
-              *w = bal;
-            } else if (bal) {
-              atomicOr(w, (unsigned long long)(bal << sh));
-              if (sh) atomicOr(w + 1, (unsigned long long)(bal >> (64 - sh)));
-            }
-          }
-        }
-        if (agg == 0) {
-          if (sel) dd_add(hi, lo, fv, 0.0);
-        } else if (agg > 0 && sel && fv == fv) {
-          kmin = min(kmin, f64_key(fv));
-          kmax = max(kmax, f64_key(fv));
-        }
-      }
-    }
-    if (!kQuery) {
-#pragma unroll
-      for (uint32_t g = 0; g < kC128Batch; ++g) {
-        const uint32_t i = b0 + g * kWave + lane;
-        if (i < n) st_one(out + i, as_f64(x[g]));
-      }
-    } else if (agg == 0) {
-      // this batch is record tile b0 / kTile: butterfly in a lane-independent order
-#pragma unroll
-      for (int d2 = 32; d2 >= 1; d2 >>= 1) {
-        const double oh = __shfl_xor(hi, d2, kWave), ol = __shfl_xor(lo, d2, kWave);
-        if ((lane & d2) == 0) {
-          dd_add(hi, lo, oh, ol);
-        } else {
-          double h2 = oh, l2 = ol;
-          dd_add(h2, l2, hi, lo);
-          hi = h2;
-          lo = l2;
-        }
-      }
-      if (lane == 0 && b0 < n) {
-        double *pp = a.partials + 2ull * (c.rtile0 + b0 / kTile);
-        pp[0] = hi;
-        pp[1] = lo;
-      }
-      hi = lo = 0.0;
-    }
+// Load-time ring checkpoints of the Chimp128 column cols[0], decoded into
+// cols[0].out (one wave per segment job): the values of the 128 records
+// before the job's first record, zeros before record 0 (and for chunks the
+// index pass rejected).
+__global__ void __launch_bounds__(256) k_c128_ring(const ColView *cols, uint32_t n_jobs, uint64_t *ring) {
+  const uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / kWave);
+  if (j >= n_jobs) return;
+  const ColView cv = cols[0];
+  const Job jb = cv.seg_jobs[j];
+  const DevChunk c = cv.chunks[jb.chunk];
+  const bool ok = !(cv.chunk_err && cv.chunk_err[jb.chunk]);
+  const int64_t j0 = (int64_t)jb.tile * kSeg128;
+  for (uint32_t u = lane_id(); u < 128; u += kWave) {
+    const int64_t r = j0 - 128 + u;
+    ring[128ull * jb.col + u] = ok && r >= 0 ? as_u64(cv.out[c.rec_off + r]) : 0;
   }
-  if (kQuery && agg > 0) {
-#pragma unroll
-    for (int d2 = 32; d2 >= 1; d2 >>= 1) {
-      kmin = min(kmin, (uint64_t)__shfl_xor((unsigned long long)kmin, d2, kWave));
-      kmax = max(kmax, (uint64_t)__shfl_xor((unsigned long long)kmax, d2, kWave));
-    }
-    unsigned long long *acc = reinterpret_cast<unsigned long long *>(a.acc) + 4ull * ci;
-    if (lane == 0) {
-      if (agg == 1) atomicMin(acc + 2, (unsigned long long)kmin);
-      else atomicMax(acc + 3, (unsigned long long)kmax);
-    }
-  }
-  if (e) atomicOr(err, e);
 }
 
 // Tile sums for the load-time index: DeltaSprintz sum of deltas, RLE sum of runs.
@@ -600,16 +593,15 @@ __global__ void k_tile_scan(const ColView *cols, uint32_t col, uint32_t n_chunks
   }
 }
 
-// Lane l of the wave for segment job jb of view cv: the stream of segment
-// jb.tile + l as an ilv lane -- its first chunk block (first_blk) and its
-// block count (0 for a lane past the chunk's segments or a chunk the index
-// pass rejected).  The segment's bits run from its checkpoint to the next
-// (a Chimp128 chunk's last segment, the one with the terminator, and the
-// last segment of the other codecs to the chunk's end), clamped to the
-// chunk.
+// Lane l of the wave for segment job jb of view cv (Gorilla, Chimp, Elf):
+// the stream of segment jb.tile + l as an ilv lane -- its first chunk block
+// (first_blk) and its block count (0 for a lane past the chunk's segments or
+// a chunk the index pass rejected).  The segment's bits run from its
+// checkpoint to the next (the last segment's to the chunk's end), clamped to
+// the chunk.
 __device__ __forceinline__ uint32_t ilv_lane(const ColView &cv, const Job &jb, uint32_t &first_blk) {
   const DevChunk c = cv.chunks[jb.chunk];
-  const uint32_t n_segs = c.codec == kCodecChimp128 ? c.n / kSeg + 1 : (c.n + kSeg - 1) / kSeg;
+  const uint32_t n_segs = (c.n + kSeg - 1) / kSeg;
   const uint32_t seg = jb.tile + lane_id();
   const uint32_t bits = c.size * 8u;
   first_blk = 0;
@@ -710,21 +702,8 @@ void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
     hipLaunchKernelGGL((k_serial<false, true>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
 }
 
-void chimp128_resolve(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
-                      uint32_t *err) {
-  if (!n_chunks) return;
-  hipLaunchKernelGGL(k_chimp128_resolve<false>, dim3((n_chunks + 3) / 4), dim3(256), 0, s, cols, prefix, n_cols,
-                     n_chunks, err, QueryArgs{}, 0u);
-}
-
-void chimp128_query(hipStream_t s, const ColView *cols, uint32_t n_chunks, const QueryArgs &a, bool pred, bool bits,
-                    int agg, uint32_t *err) {
-  if (agg > 0 && n_chunks)
-    hipLaunchKernelGGL(k_acc_init, dim3((n_chunks + 255) / 256), dim3(256), 0, s, a.acc, n_chunks);
-  if (!n_chunks) return;
-  const uint32_t flags = (pred ? 1u : 0u) | (bits ? 2u : 0u) | ((uint32_t)(agg + 1) << 2);
-  hipLaunchKernelGGL(k_chimp128_resolve<true>, dim3((n_chunks + 3) / 4), dim3(256), 0, s, cols, nullptr, 1u,
-                     n_chunks, err, a, flags);
+void c128_ring(hipStream_t s, const ColView *cols, uint32_t n_jobs, uint64_t *ring) {
+  if (n_jobs) hipLaunchKernelGGL(k_c128_ring, dim3((n_jobs + 3) / 4), dim3(256), 0, s, cols, n_jobs, ring);
 }
 
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,

@@ -14,7 +14,8 @@ hipError_t set_pow10(const double *p10, const double *n10);
 void zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 
 // Persistent decode of n_cols columns: k_decode over their tile jobs, then
-// k_decode_seg over their checkpoint-segment jobs (tile_prefix / seg_prefix:
+// k_decode_seg over their checkpoint-segment jobs (Chimp128: parse and
+// values in one pass) (tile_prefix / seg_prefix:
 // n_cols + 1 entries each, the job prefix of each list; either may be null
 // when its job count is 0).  Grids of at most max_blocks blocks; `mid`
 // (nullable) is recorded between the two kernels.
@@ -26,14 +27,9 @@ void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t 
 // runs one explicit job on cols[0] instead.
 void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
             const Job *single, uint32_t *err, bool write, bool index);
-// Chimp128 value resolve, one wave per chunk (prefix over chunks), after
-// decode() parsed the chunks' segments into the views' Chimp128 scratch.
-void chimp128_resolve(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_chunks,
-                      uint32_t *err);
-// The fused queries of one Chimp128 column (cols[0]) in the resolve pass,
-// after decode() parsed its segments (arguments as query_seg).
-void chimp128_query(hipStream_t s, const ColView *cols, uint32_t n_chunks, const QueryArgs &a, bool pred, bool bits,
-                    int agg, uint32_t *err);
+// Load-time ring checkpoints of the Chimp128 column cols[0] (decoded into
+// cols[0].out): per segment job the values of the 128 records before it.
+void c128_ring(hipStream_t s, const ColView *cols, uint32_t n_jobs, uint64_t *ring);
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
               const Job *single, uint32_t *err, bool write);
 // Load-time lane-interleaved segment streams of the XOR column cols[0]:
@@ -51,8 +47,8 @@ void chunk_select(hipStream_t s, const ColView *cols, const uint64_t *mask, int6
 void filter(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a, int mode);
 void aggregate(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a,
                uint32_t n_chunks);
-// Fused decode + query over the segment jobs of one Gorilla / Chimp / Elf
-// column (cols[0]; records never reach HBM): with `pred` the f64 predicate
+// Fused decode + query over the segment jobs of one Gorilla / Chimp / Elf /
+// Chimp128 column (cols[0]; records never reach HBM): with `pred` the f64 predicate
 // of `a` on top of a.mask, with `bits` the selection to a.out_bits (zeroed
 // before), agg 0 sum (per-tile partials for chunk_dd), 1 min, 2 max (chunk
 // accumulators, initialized here), -1 none.  Chunks with a.chunk_sel 0 are

@@ -215,10 +215,8 @@ int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
 /* Prepared batch: the same decode for a fixed set of columns and outputs,
  * replayed many times.  The job-space arguments are uploaded once; with
  * autotune != 0 the persistent grid size is timed on this batch and the
- * fastest kept; the launch sequence is captured in a hipGraph.  The batch
- * holds the Chimp128 decode scratch of its columns (9 bytes per record and
- * 16 per chunk).  Columns and outputs must outlive the batch.  No
- * counterpart in the reference. */
+ * fastest kept; the launch sequence is captured in a hipGraph.  Columns and
+ * outputs must outlive the batch.  No counterpart in the reference. */
 typedef struct ebi_hip_batch ebi_hip_batch;
 typedef struct ebi_batch_config {
     uint32_t blocks_per_cu; /* persistent decode grids = CUs x blocks_per_cu */
@@ -329,9 +327,8 @@ double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx);
 /* Device time of each phase of the last decode or matmul call on this ctx:
  * for ebi_hip_decode_device / ebi_hip_decode_batch phase 0 = the persistent
  * tile kernel (Uncompressed, RLE, BUFF, DeltaSprintz), phase 2 = the
- * persistent segment kernel (Gorilla, Chimp, Elf, and the Chimp128 parse),
- * phase 1 = the Chimp128 resolve kernel; for the matmul entry points the
- * phases listed there.
+ * persistent segment kernel (Gorilla, Chimp, Elf, Chimp128), phase 1 = 0 (no
+ * kernel); for the matmul entry points the phases listed there.
  * Instrumentation of this library; no counterpart in the reference. */
 double ebi_hip_last_phase_ms(const ebi_hip_ctx *ctx, int phase);
 

@@ -1,18 +1,18 @@
-// Host run of the XOR decoders (Gorilla, Chimp, Elf, and the Chimp128
-// parse) on files from this library's encoder.  Every chunk is decoded
-// whole from the stream start, and in the segment kernel's form: the index
+// Host run of the XOR decoders (Gorilla, Chimp, Elf, Chimp128) on files
+// from this library's encoder.  Every Gorilla / Chimp / Elf chunk is decoded
+// whole from the stream start, and in the segment kernel's forms: the index
 // pass writes a checkpoint every kSeg records, then each segment restarts
 // from its checkpoint and decodes in runs of kRun records with the state
-// carried between runs.  Both must give back the encoded values bit for bit
-// (the codecs are lossless); Chimp128's (payload, source distance) parse is
-// resolved on the host the way k_chimp128_resolve defines the values.
-// Gorilla and Chimp also in the kernel's lockstep form (every lane of a
-// wave through the same record loop), on intact and on damaged streams; the
-// segment forms read the streams both in place and through the column's
-// lane-interleaved copy, as the segment kernel does, with a check that no
-// lane reads past the blocks the copy gives it.
-// Synthetic columns of the four generators, ragged chunkings down to one
-// record per chunk.  Exit code 0 on success.
+// carried between runs -- per record, and in the kernel's lockstep form
+// (every lane of a wave through the same record loop) through the column's
+// lane-interleaved copy, with a check that no lane reads past the blocks the
+// copy gives it; Gorilla and Chimp also on damaged streams (both forms stop
+// each segment with the same error).  Chimp128 as the segment kernel's job
+// does it: a checkpoint every kSeg128 records, the lockstep parse of 64
+// segments, then the values in record order from the job's ring checkpoint.
+// All must give back the encoded values bit for bit (the codecs are
+// lossless).  Synthetic columns of the four generators, ragged chunkings
+// down to one record per chunk.  Exit code 0 on success.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -196,7 +196,7 @@ int main() {
           const double *want = v.data() + loff[k];
           std::vector<double> out(cn + 1, 0.0);
           const DirectSink sink{out.data()};
-          const uint32_t nseg = codec == kCodecChimp128 ? cn / kSeg + 1 : (cn + kSeg - 1) / kSeg;
+          const uint32_t nseg = codec == kCodecChimp128 ? cn / kSeg128 + 1 : (cn + kSeg - 1) / kSeg;
           std::vector<XorCkpt> ck(nseg + 1);
           XorState s;
           if (codec != kCodecChimp128) {
@@ -273,52 +273,46 @@ int main() {
           } else {
             uint32_t e = chimp128_index(data, bits, cn, ck.data());
             expect(e == kErrNone, "Chimp128 index error", codec, g, chunk, k, 0);
-            std::vector<uint8_t> dist((cn + kTile - 1) / kTile * kTile + 16, 0);
-            C128Term term{};
-            // the kernel's lockstep parse through the ilv, wave by wave
-            const Ilv ilv = build_ilv(data, bits, ck.data(), nseg);
-            RowSink xsink{out.data(), cn};
+            // the segment kernel's Chimp128 job, wave by wave: the lockstep
+            // parse of 64 segments of kSeg128 records (in place), then the
+            // values in record order from the job's ring checkpoint (the
+            // 128 values before it)
             for (uint32_t first = 0; first < nseg; first += kWave) {
-              BitWindow r[kWave];
-              uint32_t lead[kWave], from[kWave], lim_seg[kWave], d16[kWave][4], ee[kWave];
+              const uint32_t j0 = first * kSeg128;
+              std::vector<double> xs(kJob128, 0.0);
+              std::vector<uint32_t> ds(kJob128, 0);
               for (uint32_t l = 0; l < kWave; ++l) {
-                const uint32_t seg = first + l, i0 = seg * kSeg;
-                const bool live = seg < nseg;
-                lim_seg[l] = live ? std::min(i0 + kSeg, cn + 1) : 0;
-                lead[l] = live && ck[seg].a != 255 ? ck[seg].a : 0xffffffffu;
-                r[l] = BitWindow{};
-                r[l].p = nullptr;
-                if (live) r[l].init_at(ilv.lane(seg), ck[seg].bitpos, kIlvRowBlocks);
-                std::memset(d16[l], 0, 16);
-                from[l] = i0;
-                ee[l] = 0;
-                if (live && i0 == 0) {
-                  ee[l] = chimp128_seg_first(r[l], bits, cn, xsink, d16[l], &term);
-                  from[l] = 1;
+                const uint32_t seg = first + l, i0 = seg * kSeg128;
+                if (seg >= nseg) break;
+                struct JobSink {
+                  double *x;
+                  void row(uint32_t, uint32_t kk, double v, bool act) {
+                    if (act) x[kk] = v;
+                  }
+                } js{xs.data() + l * kSeg128};
+                BitWindow r;
+                r.init(data, ck[seg].bitpos);
+                uint32_t lead = ck[seg].a != 255 ? ck[seg].a : 0xffffffffu, from = i0, ee = 0, d16[4] = {0, 0, 0, 0};
+                if (i0 == 0) {
+                  ee = chimp128_seg_first(r, bits, js, d16);
+                  from = 1;
                 }
+                ee = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, std::min(i0 + kSeg128, cn + 1), js, d16, ee);
+                expect(ee == kErrNone, "Chimp128 parse error", codec, g, chunk, k, i0);
+                for (uint32_t kk = 0; kk < kSeg128; ++kk) ds[l * kSeg128 + kk] = (d16[kk / 4] >> ((kk % 4) * 8)) & 0xff;
               }
-              for (uint32_t k0 = 0; k0 < kSeg; k0 += kRun)
-                for (uint32_t l = 0; l < kWave; ++l) {
-                  const uint32_t a = (first + l) * kSeg + k0;
-                  ee[l] = chimp128_parse_seg(r[l], lead[l], bits, cn, a, kRun, from[l], std::min(lim_seg[l], a + kRun),
-                                             xsink, d16[l], &term, ee[l]);
-                  if (lim_seg[l] && a < cn) std::memcpy(dist.data() + a, d16[l], 16);
-                }
-              for (uint32_t l = 0; l < kWave && first + l < nseg; ++l) {
-                expect(ee[l] == kErrNone, "Chimp128 parse error", codec, g, chunk, k, (first + l) * kSeg);
-                expect(within_lane(ilv, first + l, r[l]), "Chimp128 ilv lane overread", codec, g, chunk, k,
-                       (first + l) * kSeg);
+              std::vector<uint64_t> res(kJob128, 0);
+              for (uint32_t t = 0; t < kJob128 && j0 + t <= cn; ++t) {
+                const uint32_t i = j0 + t, d = ds[t];
+                uint64_t src = 0;
+                if (d) src = i - d >= j0 ? res[i - d - j0] : as_u64(want[i - d]);  // before the job: the ring checkpoint
+                res[t] = as_u64(xs[t]) ^ src;
+                if (i < cn)
+                  expect(res[t] == as_u64(want[i]), "Chimp128 value", codec, g, chunk, k, i);
+                else
+                  expect(res[t] == kNan, "Chimp128 terminator", codec, g, chunk, k, cn);
               }
             }
-            std::vector<uint64_t> val(cn + 1, 0);
-            for (uint32_t i = 0; i <= cn; ++i) {
-              const uint64_t x = i < cn ? as_u64(out[i]) : term.x;
-              const uint32_t d = i < cn ? dist[i] : term.dist;
-              val[i] = (d ? val[i - d] : 0) ^ x;
-            }
-            for (uint32_t i = 0; i < cn; ++i)
-              expect(val[i] == as_u64(want[i]), "Chimp128 value", codec, g, chunk, k, i);
-            expect(val[cn] == kNan, "Chimp128 terminator", codec, g, chunk, k, cn);
           }
         }
         ebi_buffer_free(file);

@@ -13,8 +13,6 @@ from ebi_amd import build as B
 KERNELS = {  # mangled-name prefix -> minimum occupancy (waves per SIMD)
     "_ZN3ebi8k_decodeE": 5,
     "_ZN3ebi12k_decode_segE": 4,
-    "_ZN3ebi18k_chimp128_resolveILb0E": 8,
-    "_ZN3ebi18k_chimp128_resolveILb1E": 6,  # the resolve with the fused queries
     "_ZN3ebi11k_query_segE": 4,  # the segment decode with the fused queries
 }
 
