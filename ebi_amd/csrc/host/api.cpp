@@ -220,14 +220,23 @@ static int build_c128_ring(ebi_hip_ctx *ctx, Column &col) {
     return hip_fail(ctx, e, "hipMalloc(Chimp128 ring checkpoints)");
   }
   int rc = upload_view(ctx, col, tmp);
+  const size_t nc = col.chunks.size();
+  std::vector<uint32_t> errs(nc);
   if (!rc) {
+    // the decode validates the values too (the parse-only index cannot see
+    // a wrong terminator): the chunks it rejects are marked like the index
+    // pass's, and the ring gather skips them
     launch::chimp128(ctx->stream, col.d_view, col.d_prefix + 2, 1, col.n_serial_jobs, nullptr, ctx->d_flags, true);
     launch::c128_ring(ctx->stream, col.d_view, col.n_seg_jobs, col.d_c128_ring);
     e = hipGetLastError();
+    if (e == hipSuccess && nc)
+      e = hipMemcpyAsync(errs.data(), col.d_chunk_err, nc * 4, hipMemcpyDeviceToHost, ctx->stream);
     const hipError_t e2 = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = e2;
     if (e != hipSuccess) rc = hip_fail(ctx, e, "Chimp128 ring checkpoints");
   }
+  if (!rc)
+    for (size_t k = 0; k < nc; ++k) col.chunks[k].err |= errs[k];
   hipFree(tmp);
   if (!rc) col.device_bytes += (uint64_t)col.n_seg_jobs * 128 * 8;
   return rc;

@@ -168,3 +168,40 @@ def test_decode_random_shapes(ctx, seed):
             o.free()
         for d in decs:
             d.close()
+
+
+@pytest.mark.parametrize("codec", ["gorilla", "chimp", "chimp128", "elf"])
+@pytest.mark.parametrize("how", ["zeroed", "tail_zeroed"])
+def test_damaged_chunk(ctx, codec, how):
+    """A damaged XOR chunk (its body, or the second half of it, zeroed): the
+    oracle reports an error for the whole column, and so does every decode
+    and query on the GPU that selects that chunk -- UnexpectedEndOfChunk
+    where the oracle says so; the intact chunks still decode and query
+    bit-exactly (chunk_id 0)."""
+    from oracle.oracle import OracleError
+    vals = G.random_walk(5000, seed=3)
+    data = bytearray(E.encode(vals, E.CompressorConfig.by_name(codec, scale=100), E.ChunkOption.record_count(1000)))
+    off, size, _, _ = E.file_chunks(bytes(data))
+    o, sz = int(off[2]), int(size[2])
+    lo = o if how == "zeroed" else o + sz // 2
+    data[lo:o + sz] = bytes(o + sz - lo)
+    data = bytes(data)
+    orc = OracleDecoder(data)
+    with pytest.raises(OracleError) as oe:
+        orc.materialize()
+    d = E.Decoder(ctx, data)
+    buf = ctx.alloc(8 * d.number_of_records)
+    try:
+        for run in (lambda: d.decode_device(buf), lambda: d.decode_per_chunk(buf), d.materialize, d.sum, d.max,
+                    lambda: d.filter(E.Predicate.eq(100.0))):
+            with pytest.raises(E.EbiError) as ge:
+                run()
+            if oe.value.status == 1:
+                assert ge.value.status == E.UNEXPECTED_END
+        p = E.Predicate.range(E.RangeValue.inclusive(99.0), E.RangeValue.inclusive(101.0))
+        assert np.array_equal(d.materialize(None, 0).view(np.uint64), orc.materialize(None, 0).view(np.uint64))
+        assert np.array_equal(d.filter(p, None, 0), orc.filter(p, None, 0))
+        assert np.float64(d.max(None, 0)).view(np.uint64) == np.float64(orc.max(None, 0)).view(np.uint64)
+    finally:
+        buf.free()
+        d.close()
