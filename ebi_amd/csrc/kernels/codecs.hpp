@@ -598,24 +598,35 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
 // i - j is in [1, 128].  dist 0: no source -- record 0 (x = its 64 raw bits)
 // or an unwritten ring slot (value 0) -- so value_i = x.  No values are
 // formed here (that needs the ring): the segment kernel resolves them after
-// the parse.  Record i (i <= n, the terminator n included) goes to the sink
-// as its payload (row k = i - a) and to byte k of dist16 as its distance.
-// Record 0 is parsed before, by chimp128_seg_first.
+// the parse.  Within the lane's segment a record with dist 1 is folded into
+// its predecessor (value_i = value_{i-1} ^ x_i = value_{src(i-1)} ^
+// (x'_{i-1} ^ x_i)): it is passed on as x' = x'_{i-1} ^ x_i with the
+// predecessor's source (dist' = dist'_{i-1} + 1, or 0 when the predecessor
+// has none; at most 128 + 15), so the resolve follows a few
+// segment-to-segment links instead of record chains.  Record i (i <= n, the
+// terminator n included) goes to the sink as x' (row k = i - a) and to byte
+// k of dist16 as dist'.  Record 0 is parsed before, by chimp128_seg_first,
+// which starts the fold state (px, pd).
 template <class Sink>
 __host__ __device__ __forceinline__ uint32_t chimp128_seg_first(BitWindow &r, uint32_t size_bits, Sink &out,
-                                                       uint32_t (&dist16)[4]) {
+                                                       uint32_t (&dist16)[4], uint64_t &px, uint32_t &pd) {
   const uint64_t x = r.field(0, 64);
   r.consume(64);
   r.refill();
   out.row(0, 0, as_f64(x), true);
   dist16[0] &= ~0xffu;
+  px = x;
+  pd = 0;
   return r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
 }
 
+// px / pd: x' and dist' of the record before `a` when it is in this segment
+// (have_prev), then of each record parsed.
 template <class Sink>
 __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, uint32_t &lead, uint32_t size_bits,
                                                        uint32_t a, uint32_t cnt, uint32_t from, uint32_t lim,
-                                                       Sink &out, uint32_t (&dist16)[4], uint32_t e) {
+                                                       Sink &out, uint32_t (&dist16)[4], uint32_t e, uint64_t px,
+                                                       uint32_t pd, bool have_prev) {
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t i = a + k;
     const bool act = e == 0 && i >= from && i < lim;
@@ -632,14 +643,20 @@ __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, ui
     const uint32_t off = f == 3 ? 5u : f == 1 ? 18u : 2u;
     const uint64_t x = r.field(off, len) << (f == 1 ? (64 - len - nl) & 63 : 0u) & (f ? ~0ull : 0ull);
     const uint32_t back = f <= 1 ? (((i - 1) - idx) & 127u) + 1 : 1u;  // i - source
-    const uint32_t dist = back > i ? 0u : back;
+    const uint32_t dist0 = back > i ? 0u : back;
+    const bool fold = have_prev && dist0 == 1;  // the source is this segment's previous record
+    const uint64_t xf = fold ? x ^ px : x;
+    const uint32_t dist = fold ? (pd ? pd + 1 : 0u) : dist0;
+    px = act ? xf : px;
+    pd = act ? dist : pd;
+    have_prev = have_prev || act;
     const bool upd = act && !bad;
     lead = upd && f != 0 ? nl : lead;
     r.consume(upd ? (f == 0 ? 9u : off + len) : 0u);
     r.refill();
     const uint32_t code = bad ? (uint32_t)kErrCorrupt : r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
     e = act ? code : e;
-    out.row(i, k, as_f64(x), act);
+    out.row(i, k, as_f64(xf), act);
     const uint32_t sh = (k & 3) * 8, w = k >> 2;
     const bool put = act && k < kSeg128;
     const uint32_t keep = ~(0xffu << sh), byte = dist << sh;

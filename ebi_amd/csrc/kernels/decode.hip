@@ -129,11 +129,14 @@ __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &
   const uint32_t bits = c.size * 8u;
   uint32_t from = i0, e = 0;
   uint32_t d16[4] = {0, 0, 0, 0};
+  uint64_t px = 0;
+  uint32_t pd = 0;
   if (live && i0 == 0) {
-    e = chimp128_seg_first(r, bits, sink, d16);
+    e = chimp128_seg_first(r, bits, sink, d16, px, pd);
     from = 1;
   }
-  e = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, live ? min(i0 + kSeg128, total) : 0u, sink, d16, e);
+  e = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, live ? min(i0 + kSeg128, total) : 0u, sink, d16, e, px,
+                         pd, live && i0 == 0);
   wave_sync();  // the stage holds every payload of the job
   const uint32_t j0 = first * kSeg128;  // the job's first record
   const EBI_GLOBAL uint64_t *ring = as_global(cv.c128_ring + 128ull * ring_idx);

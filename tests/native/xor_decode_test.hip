@@ -293,11 +293,14 @@ int main() {
                 BitWindow r;
                 r.init(data, ck[seg].bitpos);
                 uint32_t lead = ck[seg].a != 255 ? ck[seg].a : 0xffffffffu, from = i0, ee = 0, d16[4] = {0, 0, 0, 0};
+                uint64_t px = 0;
+                uint32_t pd = 0;
                 if (i0 == 0) {
-                  ee = chimp128_seg_first(r, bits, js, d16);
+                  ee = chimp128_seg_first(r, bits, js, d16, px, pd);
                   from = 1;
                 }
-                ee = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, std::min(i0 + kSeg128, cn + 1), js, d16, ee);
+                ee = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, std::min(i0 + kSeg128, cn + 1), js, d16, ee,
+                                        px, pd, i0 == 0);
                 expect(ee == kErrNone, "Chimp128 parse error", codec, g, chunk, k, i0);
                 for (uint32_t kk = 0; kk < kSeg128; ++kk) ds[l * kSeg128 + kk] = (d16[kk / 4] >> ((kk % 4) * 8)) & 0xff;
               }
@@ -305,7 +308,9 @@ int main() {
               for (uint32_t t = 0; t < kJob128 && j0 + t <= cn; ++t) {
                 const uint32_t i = j0 + t, d = ds[t];
                 uint64_t src = 0;
-                if (d) src = i - d >= j0 ? res[i - d - j0] : as_u64(want[i - d]);  // before the job: the ring checkpoint
+                if (d && i - d < j0)  // before the job: the ring checkpoint holds only the 128 records before it
+                  expect(i - d + 128 >= j0, "Chimp128 source before the ring checkpoint", codec, g, chunk, k, i);
+                if (d) src = i - d >= j0 ? res[i - d - j0] : as_u64(want[i - d]);
                 res[t] = as_u64(xs[t]) ^ src;
                 if (i < cn)
                   expect(res[t] == as_u64(want[i]), "Chimp128 value", codec, g, chunk, k, i);
