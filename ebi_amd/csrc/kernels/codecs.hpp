@@ -604,17 +604,17 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
 // predecessor's source (dist' = dist'_{i-1} + 1, or 0 when the predecessor
 // has none; at most 128 + 15), so the resolve follows a few
 // segment-to-segment links instead of record chains.  Record i (i <= n, the
-// terminator n included) goes to the sink as x' (row k = i - a) and to byte
-// k of dist16 as dist'.  Record 0 is parsed before, by chimp128_seg_first,
-// which starts the fold state (px, pd).
+// terminator n included) goes to the sink: out.row(i, k, x', act) and
+// out.dist(k, dist', act) (k = i - a).  Record 0 is parsed before, by
+// chimp128_seg_first, which starts the fold state (px, pd).
 template <class Sink>
 __host__ __device__ __forceinline__ uint32_t chimp128_seg_first(BitWindow &r, uint32_t size_bits, Sink &out,
-                                                       uint32_t (&dist16)[4], uint64_t &px, uint32_t &pd) {
+                                                       uint64_t &px, uint32_t &pd) {
   const uint64_t x = r.field(0, 64);
   r.consume(64);
   r.refill();
   out.row(0, 0, as_f64(x), true);
-  dist16[0] &= ~0xffu;
+  out.dist(0, 0, true);
   px = x;
   pd = 0;
   return r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
@@ -625,8 +625,8 @@ __host__ __device__ __forceinline__ uint32_t chimp128_seg_first(BitWindow &r, ui
 template <class Sink>
 __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, uint32_t &lead, uint32_t size_bits,
                                                        uint32_t a, uint32_t cnt, uint32_t from, uint32_t lim,
-                                                       Sink &out, uint32_t (&dist16)[4], uint32_t e, uint64_t px,
-                                                       uint32_t pd, bool have_prev) {
+                                                       Sink &out, uint32_t e, uint64_t px, uint32_t pd,
+                                                       bool have_prev) {
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t i = a + k;
     const bool act = e == 0 && i >= from && i < lim;
@@ -657,13 +657,7 @@ __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, ui
     const uint32_t code = bad ? (uint32_t)kErrCorrupt : r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
     e = act ? code : e;
     out.row(i, k, as_f64(xf), act);
-    const uint32_t sh = (k & 3) * 8, w = k >> 2;
-    const bool put = act && k < kSeg128;
-    const uint32_t keep = ~(0xffu << sh), byte = dist << sh;
-    dist16[0] = put && w == 0 ? (dist16[0] & keep) | byte : dist16[0];
-    dist16[1] = put && w == 1 ? (dist16[1] & keep) | byte : dist16[1];
-    dist16[2] = put && w == 2 ? (dist16[2] & keep) | byte : dist16[2];
-    dist16[3] = put && w == 3 ? (dist16[3] & keep) | byte : dist16[3];
+    out.dist(k, dist, act);
   }
   return e;
 }
@@ -678,6 +672,7 @@ __device__ __forceinline__ uint32_t stage_swz(uint32_t r) { return (r >> 1) << 2
 
 struct StageSink {
   double *stage;
+  uint8_t *dists;  // Chimp128 jobs: the source distances, job record t at dists[t]
   __device__ __forceinline__ void put(uint32_t i, double v) const {
     const uint32_t r = i & (kRun - 1);
     stage[r * kWave + (lane_id() ^ stage_swz(r))] = v;
@@ -687,6 +682,10 @@ struct StageSink {
   // what they held
   __device__ __forceinline__ void row(uint32_t, uint32_t k, double v, bool act = true) const {
     if (act) stage[k * kWave + (lane_id() ^ stage_swz(k))] = v;
+  }
+  // a Chimp128 lane's record k: job record lane * kSeg128 + k
+  __device__ __forceinline__ void dist(uint32_t k, uint32_t d, bool act) const {
+    if (act) dists[lane_id() * kSeg128 + k] = (uint8_t)d;
   }
 };
 

@@ -96,7 +96,7 @@ struct StageOut {
 // records each; records 0..n, n the terminator), segment first + l on lane
 // l, one job = kJob128 records.  The wave parses every segment in one
 // lockstep run (payloads into the stage, row k = record i0 + k; source
-// distances in each lane's dist16), then resolves the job's records in
+// distances into `dists`, job record t at byte t), then resolves the job's records in
 // record order in steps of 64 (lane = record): sources inside the step by
 // pointer jumping over lane shuffles (x ^= x[src], src = src[src] while the
 // source lies in the step; sources only point back, so this ends after at
@@ -109,7 +109,7 @@ struct StageOut {
 // the reference stops at the first).
 template <class Out>
 __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &c, uint32_t first, uint32_t ring_idx,
-                                            double *stage, Out &out) {
+                                            double *stage, uint8_t *dists, Out &out) {
   const uint32_t lane = lane_id();
   const uint32_t total = c.n + 1;
   const uint32_t i0 = (first + lane) * kSeg128;
@@ -125,18 +125,17 @@ __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &
     r.pos = 0;
     r.hi = r.lo = 0;
   }
-  const StageSink sink{stage};
+  const StageSink sink{stage, dists};
   const uint32_t bits = c.size * 8u;
   uint32_t from = i0, e = 0;
-  uint32_t d16[4] = {0, 0, 0, 0};
   uint64_t px = 0;
   uint32_t pd = 0;
   if (live && i0 == 0) {
-    e = chimp128_seg_first(r, bits, sink, d16, px, pd);
+    e = chimp128_seg_first(r, bits, sink, px, pd);
     from = 1;
   }
-  e = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, live ? min(i0 + kSeg128, total) : 0u, sink, d16, e, px,
-                         pd, live && i0 == 0);
+  e = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, live ? min(i0 + kSeg128, total) : 0u, sink, e, px, pd,
+                         live && i0 == 0);
   wave_sync();  // the stage holds every payload of the job
   const uint32_t j0 = first * kSeg128;  // the job's first record
   const EBI_GLOBAL uint64_t *ring = as_global(cv.c128_ring + 128ull * ring_idx);
@@ -149,10 +148,7 @@ __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &
     const uint32_t sg = t / kSeg128, kk = t % kSeg128;
     double *slot = stage + kk * kWave + (sg ^ stage_swz(kk));
     uint64_t v = as_u64(*slot);
-    const uint32_t w0 = __shfl(d16[0], sg), w1 = __shfl(d16[1], sg), w2 = __shfl(d16[2], sg), w3 = __shfl(d16[3], sg);
-    const uint32_t wsel = kk >> 2;
-    const uint32_t word = wsel == 0 ? w0 : wsel == 1 ? w1 : wsel == 2 ? w2 : w3;
-    const uint32_t d = i < total ? (word >> ((kk & 3) * 8)) & 0xffu : 0u;
+    const uint32_t d = i < total ? dists[t] : 0u;
     int32_t p = d ? (int32_t)(i - d) : -1;  // source record, -1: none (value 0)
     while (__any(p >= (int32_t)base)) {
       const int32_t q = (p - (int32_t)base) & (kWave - 1);  // source lane (used only when in the step)
@@ -191,9 +187,9 @@ struct C128Out {
 };
 
 __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, uint32_t first, uint32_t row,
-                                               double *stage, const double *elf_tab) {
+                                               double *stage, uint8_t *dists, const double *elf_tab) {
   const DevChunk c = cv.chunks[ci];
-  StageOut out{StageSink{stage}, cv.out + c.rec_off + (size_t)first * kSeg,
+  StageOut out{StageSink{stage, nullptr}, cv.out + c.rec_off + (size_t)first * kSeg,
                first * kSeg < c.n ? c.n - first * kSeg : 0u};
   switch (c.codec) {
     case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, row, elf_tab, out);
@@ -201,7 +197,7 @@ __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, 
     case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, row, elf_tab, out);
     case kCodecChimp128: {
       C128Out o{cv.out + c.rec_off, c.n};
-      return c128_job(cv, c, first, row, stage, o);
+      return c128_job(cv, c, first, row, stage, dists, o);
     }
     default: return kErrCorrupt;
   }
@@ -261,6 +257,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_decode_seg(
     const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
   __shared__ __align__(16) double lds[4][(kRun + 1) * kWave];
+  __shared__ uint8_t dists[4][kJob128];  // Chimp128 jobs: source distances
   __shared__ double elf_tab[kElfTab];
   if (threadIdx.x < kElfTab) elf_tab_fill(elf_tab, threadIdx.x, c_p10, c_n10);
   __syncthreads();
@@ -272,7 +269,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const ColView cv = cols[ci];
     const Job jb = cv.seg_jobs[j - prefix[ci]];
     if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-      const uint32_t e = segment_job(cv, jb.chunk, jb.tile, jb.col, lds[wave], elf_tab);
+      const uint32_t e = segment_job(cv, jb.chunk, jb.tile, jb.col, lds[wave], dists[wave], elf_tab);
       if (e) atomicOr(err, e);
     }
     j += stride;
@@ -396,6 +393,7 @@ __device__ __forceinline__ void keys_to_acc(int agg, uint64_t kmin, uint64_t kma
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_query_seg(
     const ColView *cols, uint32_t n_jobs, QueryArgs a, uint32_t flags, uint32_t *err) {
   __shared__ __align__(16) double lds[4][(kRun + 1) * kWave];  // Chimp128 jobs: the payload / value stage
+  __shared__ uint8_t dists[4][kJob128];                         // and the source distances
   __shared__ double elf_tab[kElfTab];
   if (threadIdx.x < kElfTab) elf_tab_fill(elf_tab, threadIdx.x, c_p10, c_n10);
   __syncthreads();
@@ -410,7 +408,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const DevChunk c = cv.chunks[jb.chunk];
     if (c.codec == kCodecChimp128) {
       C128Query q{&a, &c, (flags & 1) != 0, want_bits, agg, 0.0, 0.0, ~0ull, 0};
-      const uint32_t e = c128_job(cv, c, jb.tile, jb.col, lds[threadIdx.x / kWave], q);
+      const uint32_t e = c128_job(cv, c, jb.tile, jb.col, lds[threadIdx.x / kWave], dists[threadIdx.x / kWave], q);
       if (e) atomicOr(err, e);
       if (agg > 0) keys_to_acc(agg, q.kmin, q.kmax, a.acc + 4ull * jb.chunk);
       continue;

@@ -286,23 +286,26 @@ int main() {
                 if (seg >= nseg) break;
                 struct JobSink {
                   double *x;
+                  uint32_t *d;
                   void row(uint32_t, uint32_t kk, double v, bool act) {
                     if (act) x[kk] = v;
                   }
-                } js{xs.data() + l * kSeg128};
+                  void dist(uint32_t kk, uint32_t v, bool act) {
+                    if (act) d[kk] = v;
+                  }
+                } js{xs.data() + l * kSeg128, ds.data() + l * kSeg128};
                 BitWindow r;
                 r.init(data, ck[seg].bitpos);
-                uint32_t lead = ck[seg].a != 255 ? ck[seg].a : 0xffffffffu, from = i0, ee = 0, d16[4] = {0, 0, 0, 0};
+                uint32_t lead = ck[seg].a != 255 ? ck[seg].a : 0xffffffffu, from = i0, ee = 0;
                 uint64_t px = 0;
                 uint32_t pd = 0;
                 if (i0 == 0) {
-                  ee = chimp128_seg_first(r, bits, js, d16, px, pd);
+                  ee = chimp128_seg_first(r, bits, js, px, pd);
                   from = 1;
                 }
-                ee = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, std::min(i0 + kSeg128, cn + 1), js, d16, ee,
-                                        px, pd, i0 == 0);
+                ee = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, std::min(i0 + kSeg128, cn + 1), js, ee, px, pd,
+                                        i0 == 0);
                 expect(ee == kErrNone, "Chimp128 parse error", codec, g, chunk, k, i0);
-                for (uint32_t kk = 0; kk < kSeg128; ++kk) ds[l * kSeg128 + kk] = (d16[kk / 4] >> ((kk % 4) * 8)) & 0xff;
               }
               std::vector<uint64_t> res(kJob128, 0);
               for (uint32_t t = 0; t < kJob128 && j0 + t <= cn; ++t) {
