@@ -1062,42 +1062,52 @@ __device__ __forceinline__ uint64_t rle_tile_sum(const DevChunk &c, uint32_t t) 
 
 // Expands output steps [s0, s0 + ns) (kTile records each) of field tile t.
 __device__ __forceinline__ void rle_tile(const DevChunk &c, uint32_t t, uint32_t s0, uint32_t ns, int64_t tile_start,
-                                         double *out, uint32_t *starts, double *stage) {
+                                         double *out, double *stage) {
   const uint32_t lane = lane_id();
   const uint32_t f0 = t * kTile + lane * 8;
-  uint32_t run[8];
+  uint32_t run[8], st[8];
   const uint32_t s = rle_runs(c, f0, run);
   const uint32_t incl = wave_incl_scan_u32(s);
-  uint32_t pos = incl - s;
+  uint32_t pos = incl - s;  // the tile's output position of this lane's first field
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    starts[lane * 8 + j] = pos;
+    st[j] = pos;
     pos += run[j];
   }
   const uint32_t total = __shfl(incl, kWave - 1, kWave);
-  if (lane == 0) starts[kTile] = total;
-  wave_sync();
-  const uint32_t nf = (uint32_t)min<uint64_t>(kTile, c.fields > t * (uint64_t)kTile ? c.fields - t * (uint64_t)kTile : 0);
   const EBI_GLOBAL double *vals = as_global(reinterpret_cast<const double *>(c.data + c.values_off)) + (uint64_t)t * kTile;
   double *dst = out + c.rec_off + tile_start;
-  // each step expands 512 records: lane l takes records 8l .. 8l + 7 (one
-  // binary search for the first one's field, then a forward walk) and the
-  // wave writes them with coalesced stores
+  uint32_t *mark = reinterpret_cast<uint32_t *>(stage);  // the step's field marks (512 u32), before the stage is used
+  // each step expands 512 records: every field marks the first of its
+  // records in the step with its index, then record o's field is the
+  // largest mark at or before o (fields grow with position): a prefix max
+  // over the lane's 8 records and a wave max-scan over the lanes
   const uint32_t end = min(total, (s0 + ns) * kTile);
   for (uint32_t base = s0 * kTile; base < end; base += kTile) {
-    const uint32_t o0 = base + lane * 8;
-    double v[8];
-    uint32_t lo = 0, hi = nf;  // largest f with starts[f] <= o0
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (starts[mid] <= o0) lo = mid; else hi = mid;
-    }
+    uint4 *m4 = reinterpret_cast<uint4 *>(mark) + lane * 2;
+    m4[0] = make_uint4(0, 0, 0, 0);
+    m4[1] = make_uint4(0, 0, 0, 0);
+    wave_sync();
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint32_t o = o0 + j;
-      while (lo + 1 < nf && starts[lo + 1] <= o) ++lo;
-      v[j] = o < total ? vals[lo] : 0.0;
+    for (int j = 0; j < 8; ++j)
+      if (run[j] && st[j] < base + kTile && st[j] + run[j] > base) mark[max(st[j], base) - base] = lane * 8 + j;
+    wave_sync();
+    const uint4 a = m4[0], b = m4[1];
+    uint32_t f[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int j = 1; j < 8; ++j) f[j] = max(f[j], f[j - 1]);
+    uint32_t x = f[7];  // inclusive max-scan of the lanes' last marks
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, kWave);
+      if (lane >= (uint32_t)d) x = max(x, y);
     }
+    const uint32_t up = __shfl_up(x, 1, kWave);  // every lane takes part in the shuffle
+    const uint32_t carry = lane ? up : 0u;
+    wave_sync();  // every mark is read before tile_store reuses the stage
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = base + lane * 8 + j < total ? vals[max(f[j], carry)] : 0.0;
     tile_store(dst + base, v, min(kTile, total - base), stage);
   }
 }

@@ -211,9 +211,8 @@ __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, 
 // of 5 waves per SIMD.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_decode(
     const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
-  // per wave: the store stage, and the field starts of an RLE tile
+  // per wave: the store stage (an RLE step's field marks before that)
   __shared__ __align__(16) double lds[4][kStageStride * kWave];
-  __shared__ uint32_t lds_starts[4][kTile + 1];
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
   uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
@@ -228,7 +227,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
       // chunk rejected at load: nothing to decode
     } else if (c.codec == kCodecRLE) {
       const uint32_t t = jb.tile & 0xffffffu;
-      rle_tile(c, t, jb.tile >> 24, jb.kind, cv.tile_index[c.tile0 + t], cv.out, lds_starts[wave], lds[wave]);
+      rle_tile(c, t, jb.tile >> 24, jb.kind, cv.tile_index[c.tile0 + t], cv.out, lds[wave]);
     } else {
       for (uint32_t t = jb.tile; t < jb.tile + jb.kind; ++t) {
         switch (c.codec) {
