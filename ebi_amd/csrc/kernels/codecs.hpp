@@ -103,9 +103,11 @@ __host__ __device__ __forceinline__ double elf_recover(double v, int32_t beta) {
 // past 1e-22) take elf_recover's path.
 constexpr uint32_t kElfTab = 66;
 
-// RN(N / D) for D = 10^a exact (a <= 22), y = RN(1 / D), N a nonzero
-// integer-valued double with |N| < 1e31 (the two corrections above).
-__host__ __device__ __forceinline__ double div_pow10(double N, double D, double y) {
+// RN(N / D) for D >= 1 and N integer-valued doubles (N = 0 or |N| >= 1, so
+// the quotient is zero or normal) and y = RN(1 / D): the product N y and two
+// FMA corrections (Markstein; see above).  Used for the Elf powers of ten
+// and the DeltaSprintz scale.
+__host__ __device__ __forceinline__ double div_rn(double N, double D, double y) {
   const double q0 = N * y;
   const double q1 = fma(fma(-q0, D, N), y, q0);
   return fma(fma(-q1, D, N), y, q1);
@@ -139,7 +141,7 @@ __host__ __device__ __forceinline__ double elf_recover_tab(double v, int32_t bet
     const double D = tab[20 + alpha], y = tab[43 + alpha];
     const double t = v * D;
     const double N = v < 0.0 ? floor(t) : ceil(t);
-    return div_pow10(N, D, y);
+    return div_rn(N, D, y);
   }
   return elf_round_up(v, alpha);
 }
@@ -1021,8 +1023,14 @@ __device__ __forceinline__ void sprintz_tile(const DevChunk &c, uint32_t t, int6
   sprintz_q(c, t, tile_base, q);
   double v[8];
   const double sc = (double)c.scale;
+  if (c.scale) {  // quantized as f64 / scale as f64, correctly rounded via RN(1 / scale)
+    const double y = 1.0 / sc;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) v[j] = (double)q[j] / sc;  // quantized as f64 / scale as f64
+    for (int j = 0; j < 8; ++j) v[j] = div_rn((double)q[j], sc, y);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (double)q[j] / sc;
+  }
   tile_store(out + c.rec_off + (uint64_t)t * kTile, v, min(kTile, c.n - t * kTile), stage);
 }
 
@@ -1038,9 +1046,9 @@ __device__ __forceinline__ int64_t sprintz_tile_sum(const DevChunk &c, uint32_t 
 
 // RLE (chunk_reader/run_length.rs:68-91): runs[] u8 and values[] f64 (SoA).
 // Tile t covers fields [t*kTile, +kTile); its first output record is the
-// tile index entry.  Each lane owns 8 fields; the wave scans run lengths,
-// publishes field starts to LDS and expands by binary search so that output
-// stores are contiguous across lanes.
+// tile index entry.  Each lane owns 8 fields; the wave scans run lengths and
+// expands step by step (field marks and a max-scan, rle_tile) so that
+// output stores are contiguous across lanes.
 __device__ __forceinline__ uint32_t rle_runs(const DevChunk &c, uint32_t f0, uint32_t run[8]) {
   const EBI_GLOBAL uint8_t *runs = as_global(reinterpret_cast<const uint8_t *>(c.data));
   uint32_t s = 0;

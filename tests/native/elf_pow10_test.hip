@@ -4,7 +4,8 @@
 // on every power of ten and its two neighbours, special values and random
 // magnitudes; p10_exact(a) against the parsed "1.0E{a}" for a <= 22; the
 // table form of the Elf value recovery (elf_recover_tab) against
-// elf_recover, and its division (div_pow10) against IEEE division.
+// elf_recover, and its division (div_rn) against IEEE division, for the
+// powers of ten and for DeltaSprintz scales.
 // Exit code 0 on success.
 #include <cmath>
 #include <cstdio>
@@ -80,7 +81,7 @@ int main() {
           ++fails;
         }
       }
-  // div_pow10 against IEEE division: every integer up to 2e5, random
+  // div_rn against IEEE division: every integer up to 2e5, random
   // integers below 2^53, random integer-valued doubles up to 1e31
   std::uniform_int_distribution<uint64_t> i53(1, (1ull << 53) - 1);
   std::uniform_real_distribution<double> big(0.0, 31.0 * 3.321928094887362);
@@ -89,9 +90,9 @@ int main() {
     auto one = [&](double N) {
       for (double M : {N, -N}) {
         ++checks;
-        const double q = ebi::div_pow10(M, D, y), w = M / D;
+        const double q = ebi::div_rn(M, D, y), w = M / D;
         if (ebi::as_u64(q) != ebi::as_u64(w)) {
-          if (fails < 10) std::printf("div_pow10(%.17g, 1e%u) = %.17g, IEEE %.17g\n", M, a, q, w);
+          if (fails < 10) std::printf("div_rn(%.17g, 1e%u) = %.17g, IEEE %.17g\n", M, a, q, w);
           ++fails;
         }
       }
@@ -101,6 +102,28 @@ int main() {
     for (int i = 0; i < 200000; ++i) {
       const double N = std::floor(std::exp2(big(rng)));
       if (N >= 1.0) one(N);
+    }
+  }
+  // div_rn for the DeltaSprintz scale: random u32 scales (and the small and
+  // power-of-ten ones) against IEEE division of random integers of every
+  // magnitude up to 2^63, both signs
+  std::uniform_int_distribution<uint32_t> u32(1, 0xffffffffu);
+  std::uniform_int_distribution<int> mag(0, 63);
+  std::vector<double> scales = {1, 2, 3, 7, 10, 100, 1000, 10000, 65535, 4294967295.0};
+  for (int i = 0; i < 200; ++i) scales.push_back((double)u32(rng));
+  for (double D : scales) {
+    const double y = 1.0 / D;
+    for (int i = 0; i < 20000; ++i) {
+      const int m = mag(rng);
+      const int64_t iN = (int64_t)(rng() >> (63 - m));
+      for (double N : {(double)iN, (double)(-iN)}) {  // int64 -> f64: never -0
+        ++checks;
+        const double q = ebi::div_rn(N, D, y), w = N / D;
+        if (ebi::as_u64(q) != ebi::as_u64(w)) {
+          if (fails < 10) std::printf("div_rn(%.17g, %.17g) = %.17g, IEEE %.17g\n", N, D, q, w);
+          ++fails;
+        }
+      }
     }
   }
   std::printf("%s: %ld checks, %ld failures\n", fails ? "FAIL" : "OK", checks, fails);
