@@ -125,9 +125,10 @@ def main():
 
     def step():
         # one batched decode over every chunk of every column (mixed codecs):
-        # k_decode (tile jobs of all columns) and k_decode_seg (XOR
-        # checkpoint segments), launched eagerly so each kernel is bracketed
-        # by HIP events
+        # k_decode_mix over the tile jobs of all columns and the XOR
+        # checkpoint segments (or, when the tuner finds it faster, k_decode
+        # then k_decode_seg), launched eagerly so each kernel is bracketed by
+        # HIP events
         batch.run(eager=True)
         return ctx.last_device_ms, ctx.last_phase_ms(0), ctx.last_phase_ms(1), ctx.last_phase_ms(2)
 
@@ -158,8 +159,12 @@ def main():
     # stream, around each launch of the timed steps)
     kernels_of = {c: KERNELS_OF.get(c, ("k_decode", "k_decode")) for c in codecs}
     kernel_of = {c: "+".join(dict.fromkeys(k)) for c, k in kernels_of.items()}  # per-codec grouping
+    phases = ("k_decode", "", "k_decode_seg")  # phase 1 is unused
+    if batch_cfg.get("mixed"):  # both job lists in one k_decode_mix launch (phase 0)
+        kernels_of = {c: ("k_decode_mix", "k_decode_mix") for c in codecs}
+        phases = ("k_decode_mix", "", "")
     batch_kernels = {}
-    for k, name in enumerate(("k_decode", "", "k_decode_seg")):  # phase 1 is unused
+    for k, name in enumerate(phases):
         # algorithmic bytes: the compressed stream to its reader, 8 B a record to its writer
         b = sum((cb if kernels_of[codec][0] == name else 0) + (8 * n if kernels_of[codec][1] == name else 0)
                 for _, codec, _, cb in cols)

@@ -107,7 +107,7 @@ class _ServeStats(C.Structure):
 
 
 class _BatchCfg(C.Structure):
-    _fields_ = [("blocks_per_cu", C.c_uint32), ("reserved0", C.c_uint32), ("reserved1", C.c_uint32),
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("mixed", C.c_uint32), ("reserved1", C.c_uint32),
                 ("reserved2", C.c_uint32), ("tuned_ms", C.c_double)]
 
 
@@ -781,7 +781,7 @@ class Batch:
     def config(self) -> dict:
         c = _BatchCfg()
         _check(lib().ebi_hip_batch_config_get(self._h, C.byref(c)), None, "batch_config")
-        return {"blocks_per_cu": c.blocks_per_cu, "tuned_ms": c.tuned_ms}
+        return {"blocks_per_cu": c.blocks_per_cu, "mixed": bool(c.mixed), "tuned_ms": c.tuned_ms}
 
     def close(self):
         if getattr(self, "_h", None) and self.ctx._h:

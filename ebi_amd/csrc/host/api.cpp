@@ -553,6 +553,7 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   if (mode && std::strcmp(mode, "serial") == 0) c->xor_mode = 1;
   // launch-shape overrides (the batch tuner picks its own per batch)
   if (const char *v = std::getenv("EBI_HIP_BLOCKS_PER_CU")) c->blocks_per_cu = std::max(1, std::min(64, std::atoi(v)));
+  if (const char *v = std::getenv("EBI_HIP_BATCH_MIXED")) c->batch_mixed = std::atoi(v) != 0;
   *out = c;
   return EBI_OK;
 }

@@ -40,12 +40,13 @@ void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool
   launch::zero_u32(s, ctx->d_flags, 2);
   if (timed) hipEventRecord(ctx->ev0, s);
   launch::decode(s, views_of(b), b->n_cols, prefix_of(b), b->jobs, prefix_of(b) + 2 * (b->n_cols + 1), b->segs,
-                 ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr);
+                 ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr,
+                 c.mixed != 0);
   if (timed) hipEventRecord(ctx->ev_mid, s);
   if (timed) hipEventRecord(ctx->ev1, s);
 }
 
-// phase 0: k_decode, 1: none (kept 0), 2: k_decode_seg
+// phase 0: k_decode (or k_decode_mix), 1: none (kept 0), 2: k_decode_seg (0 when mixed)
 void read_times(ebi_hip_ctx *ctx) {
   float a = 0, c = 0, g = 0, t = 0;
   hipEventElapsedTime(&a, ctx->ev0, ctx->ev_seg);
@@ -144,8 +145,10 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
     return fail(ctx, EBI_OOM, "batch arguments");
   }
   b->cfg.blocks_per_cu = ctx->blocks_per_cu;
+  b->cfg.mixed = ctx->batch_mixed && b->jobs && b->segs ? 1u : 0u;
   if (autotune) {
-    // persistent grid size of the decode kernels
+    // persistent grid size of the decode kernels, and (with both job kinds)
+    // one mixed launch or the two kernels in turn
     double best = time_config(ctx, b, b->cfg, 2);
     if (best < 0) {
       destroy(b);
@@ -158,11 +161,14 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
         b->cfg = c;
       }
     };
-    for (uint32_t bpc : {4u, 8u, 16u})
-      if (bpc != b->cfg.blocks_per_cu) {
-        ebi_batch_config c = b->cfg;
+    const ebi_batch_config first = b->cfg;
+    const uint32_t n_mixed = b->jobs && b->segs ? 2u : 1u;
+    for (uint32_t mixed = 0; mixed < n_mixed; ++mixed)
+      for (uint32_t bpc : {4u, 8u, 16u}) {
+        ebi_batch_config c = first;
         c.blocks_per_cu = bpc;
-        consider(c);
+        c.mixed = mixed;
+        if (bpc != first.blocks_per_cu || mixed != first.mixed) consider(c);
       }
     b->cfg.tuned_ms = best;
   }

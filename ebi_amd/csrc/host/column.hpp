@@ -98,5 +98,6 @@ struct ebi_hip_ctx {
   int cu_count = 256;
   // launch shape of the decode kernels (defaults; a prepared batch tunes its own)
   uint32_t blocks_per_cu = 8;   // persistent k_decode grid = cu_count * blocks_per_cu
+  uint32_t batch_mixed = 0;     // untuned prepared batches: 1 = one k_decode_mix launch
   int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks (Gorilla, Chimp, Elf)
 };

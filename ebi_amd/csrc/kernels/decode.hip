@@ -203,52 +203,67 @@ __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, 
   }
 }
 
-// Persistent decode over the tile jobs of n_cols columns (Uncompressed,
-// BUFF, DeltaSprintz, RLE; mixed codecs and uneven chunks in one launch).
-// Wave w takes jobs w, w + W, w + 2W, ... (W = waves in the grid): a static
-// interleave, so neighbouring jobs (similar cost) spread over all waves and
-// no job counter is contended.  amdgpu_waves_per_eu(5): the register budget
-// of 5 waves per SIMD.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_decode(
-    const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
-  // per wave: the store stage (an RLE step's field marks before that)
-  __shared__ __align__(16) double lds[4][kStageStride * kWave];
-  const uint32_t wave = threadIdx.x / kWave;
-  const uint32_t stride = gridDim.x * 4;
-  uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
-  while (j < n_jobs) {
-    const uint32_t ci = find_col(prefix, n_cols, j);
-    const ColView cv = cols[ci];
-    // jb.kind consecutive tiles from jb.tile; RLE: jb.kind output steps
-    // from step jb.tile >> 24 of field tile jb.tile & 0xffffff
-    const Job jb = cv.tile_jobs[j - prefix[ci]];
-    const DevChunk c = cv.chunks[jb.chunk];
-    if (cv.chunk_err && cv.chunk_err[jb.chunk]) {
-      // chunk rejected at load: nothing to decode
-    } else if (c.codec == kCodecRLE) {
-      const uint32_t t = jb.tile & 0xffffffu;
-      rle_tile(c, t, jb.tile >> 24, jb.kind, cv.tile_index[c.tile0 + t], cv.out, lds[wave]);
-    } else {
-      for (uint32_t t = jb.tile; t < jb.tile + jb.kind; ++t) {
-        switch (c.codec) {
-          case kCodecPlain: plain_tile(c, t, cv.out); break;
-          case kCodecBUFF: buff_tile(c, t, cv.out, lds[wave]); break;
-          case kCodecSprintz: sprintz_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, lds[wave]); break;
-          default:
-            if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
-            break;
-        }
+// Tile job j of the launch (Uncompressed, BUFF, DeltaSprintz, RLE; the
+// columns' tile_jobs lists concatenated).  stage: the wave's store stage
+// (kStageStride * kWave doubles; an RLE step's field marks before that).
+__device__ __forceinline__ void tile_job(const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t j,
+                                         double *stage, uint32_t *err) {
+  const uint32_t ci = find_col(prefix, n_cols, j);
+  const ColView cv = cols[ci];
+  // jb.kind consecutive tiles from jb.tile; RLE: jb.kind output steps
+  // from step jb.tile >> 24 of field tile jb.tile & 0xffffff
+  const Job jb = cv.tile_jobs[j - prefix[ci]];
+  const DevChunk c = cv.chunks[jb.chunk];
+  if (cv.chunk_err && cv.chunk_err[jb.chunk]) {
+    // chunk rejected at load: nothing to decode
+  } else if (c.codec == kCodecRLE) {
+    const uint32_t t = jb.tile & 0xffffffu;
+    rle_tile(c, t, jb.tile >> 24, jb.kind, cv.tile_index[c.tile0 + t], cv.out, stage);
+  } else {
+    for (uint32_t t = jb.tile; t < jb.tile + jb.kind; ++t) {
+      switch (c.codec) {
+        case kCodecPlain: plain_tile(c, t, cv.out); break;
+        case kCodecBUFF: buff_tile(c, t, cv.out, stage); break;
+        case kCodecSprintz: sprintz_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, stage); break;
+        default:
+          if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
+          break;
       }
     }
-    j += stride;
   }
 }
 
-// Persistent decode over the checkpoint-segment jobs (64 segments of a
-// Gorilla / Chimp / Elf / Chimp128 chunk per job) of n_cols columns, with
+// Segment job j of the launch (64 checkpoint segments of a Gorilla / Chimp /
+// Elf chunk, or a Chimp128 job; the columns' seg_jobs lists concatenated).
+__device__ __forceinline__ void seg_job(const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t j,
+                                        double *stage, uint8_t *dists, const double *elf_tab, uint32_t *err) {
+  const uint32_t ci = find_col(prefix, n_cols, j);
+  const ColView cv = cols[ci];
+  const Job jb = cv.seg_jobs[j - prefix[ci]];
+  if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
+    const uint32_t e = segment_job(cv, jb.chunk, jb.tile, jb.col, stage, dists, elf_tab);
+    if (e) atomicOr(err, e);
+  }
+}
+
+// Persistent decode over the tile jobs of n_cols columns (mixed codecs and
+// uneven chunks in one launch).  Wave w takes jobs w, w + W, w + 2W, ...
+// (W = waves in the grid): a static interleave, so neighbouring jobs
+// (similar cost) spread over all waves and no job counter is contended.
+// amdgpu_waves_per_eu(5): the register budget of 5 waves per SIMD.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_decode(
+    const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
+  __shared__ __align__(16) double lds[4][kStageStride * kWave];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t stride = gridDim.x * 4;
+  for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); j < n_jobs; j += stride)
+    tile_job(cols, prefix, n_cols, j, lds[wave], err);
+}
+
+// Persistent decode over the checkpoint-segment jobs of n_cols columns, with
 // the same static interleave as k_decode.  A kernel of its own, so that the
 // serial bit-stream loops get their own register allocation and LDS (one
-// [16][64] output stage per wave).  amdgpu_waves_per_eu(4): the register
+// [17][64] output stage per wave).  amdgpu_waves_per_eu(4): the register
 // budget at which the kernel needs no scratch; built to fit 5 waves per SIMD
 // it spilled, and then decoded Elf wrongly in workgroups that shared a CU
 // with an earlier one (the decode kernels are kept spill-free, which a test
@@ -262,16 +277,40 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   __syncthreads();
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
-  uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
-  while (j < n_jobs) {
-    const uint32_t ci = find_col(prefix, n_cols, j);
-    const ColView cv = cols[ci];
-    const Job jb = cv.seg_jobs[j - prefix[ci]];
-    if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
-      const uint32_t e = segment_job(cv, jb.chunk, jb.tile, jb.col, lds[wave], dists[wave], elf_tab);
-      if (e) atomicOr(err, e);
-    }
-    j += stride;
+  for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); j < n_jobs; j += stride)
+    seg_job(cols, prefix, n_cols, j, lds[wave], dists[wave], elf_tab, err);
+}
+
+// Both job lists in one persistent launch: job j of the n_seg + n_tile
+// combined jobs is segment job floor(j * n_seg / n) when that floor steps
+// at j + 1, else tile job j - floor(j * n_seg / n) -- the two kinds spread
+// evenly over the combined range, so every wave alternates between the
+// HBM-bound tile jobs and the bit-stream loops of the segment jobs.
+// Register budget and LDS of k_decode_seg (the tile stage is the segment
+// stage's first kStageStride rows).  A candidate of the batch tuner: on a
+// batch of 16 tile and 16 XOR columns of 2^28 records it took 31.6-32.3 ms
+// at 4-32 blocks per CU against 30.6 ms for k_decode then k_decode_seg (the
+// segment jobs share the CU's memory pipeline with the tile jobs rather
+// than leaving it idle), so the two kernels stay the untuned default.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_decode_mix(
+    const ColView *cols, const uint32_t *tile_prefix, uint32_t n_tile_jobs, const uint32_t *seg_prefix,
+    uint32_t n_seg_jobs, uint32_t n_cols, uint32_t *err) {
+  static_assert(kStageStride <= kRun + 1, "tile stage inside the segment stage");
+  __shared__ __align__(16) double lds[4][(kRun + 1) * kWave];
+  __shared__ uint8_t dists[4][kJob128];
+  __shared__ double elf_tab[kElfTab];
+  if (threadIdx.x < kElfTab) elf_tab_fill(elf_tab, threadIdx.x, c_p10, c_n10);
+  __syncthreads();
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t stride = gridDim.x * 4;
+  const uint32_t n = n_tile_jobs + n_seg_jobs;
+  for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); j < n; j += stride) {
+    const uint32_t a = (uint32_t)((uint64_t)j * n_seg_jobs / n);
+    const uint32_t b = (uint32_t)((uint64_t)(j + 1) * n_seg_jobs / n);
+    if (b > a)
+      seg_job(cols, seg_prefix, n_cols, a, lds[wave], dists[wave], elf_tab, err);
+    else
+      tile_job(cols, tile_prefix, n_cols, j - a, lds[wave], err);
   }
 }
 
@@ -666,11 +705,19 @@ void zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
 }
 
 void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t *tile_prefix, uint32_t n_tile_jobs,
-            const uint32_t *seg_prefix, uint32_t n_seg_jobs, uint32_t *err, uint32_t max_blocks, hipEvent_t mid) {
+            const uint32_t *seg_prefix, uint32_t n_seg_jobs, uint32_t *err, uint32_t max_blocks, hipEvent_t mid,
+            bool mixed) {
   auto grid = [&](uint32_t jobs) {
     const uint32_t blocks = (jobs + 3) / 4;
     return max_blocks && blocks > max_blocks ? max_blocks : blocks;
   };
+  // the combined job index and its grid stride stay below 2^32: n < 2^31 - 2
+  if (mixed && n_tile_jobs && n_seg_jobs && (uint64_t)n_tile_jobs + n_seg_jobs < (1ull << 31) - 2) {
+    hipLaunchKernelGGL(k_decode_mix, dim3(grid(n_tile_jobs + n_seg_jobs)), dim3(256), 0, s, cols, tile_prefix,
+                       n_tile_jobs, seg_prefix, n_seg_jobs, n_cols, err);
+    if (mid) hipEventRecord(mid, s);
+    return;
+  }
   if (n_tile_jobs)
     hipLaunchKernelGGL(k_decode, dim3(grid(n_tile_jobs)), dim3(256), 0, s, cols, tile_prefix, n_cols, n_tile_jobs,
                        err);

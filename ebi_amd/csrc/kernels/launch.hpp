@@ -18,10 +18,12 @@ void zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 // values in one pass) (tile_prefix / seg_prefix:
 // n_cols + 1 entries each, the job prefix of each list; either may be null
 // when its job count is 0).  Grids of at most max_blocks blocks; `mid`
-// (nullable) is recorded between the two kernels.
+// (nullable) is recorded between the two kernels.  mixed: when both lists
+// are non-empty, one k_decode_mix launch over both instead (`mid` recorded
+// after it).
 void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t *tile_prefix, uint32_t n_tile_jobs,
             const uint32_t *seg_prefix, uint32_t n_seg_jobs, uint32_t *err, uint32_t max_blocks,
-            hipEvent_t mid = nullptr);
+            hipEvent_t mid = nullptr, bool mixed = false);
 // Lane-per-chunk serial XOR decode (Gorilla, Chimp, Elf); `index` writes
 // checkpoints.  prefix is over n_serial_jobs; `single` (device, nullable)
 // runs one explicit job on cols[0] instead.

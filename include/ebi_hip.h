@@ -214,13 +214,17 @@ int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
 
 /* Prepared batch: the same decode for a fixed set of columns and outputs,
  * replayed many times.  The job-space arguments are uploaded once; with
- * autotune != 0 the persistent grid size is timed on this batch and the
- * fastest kept; the launch sequence is captured in a hipGraph.  Columns and
- * outputs must outlive the batch.  No counterpart in the reference. */
+ * autotune != 0 the persistent grid size and (when the batch holds both tile
+ * and segment jobs) the mixed or two-kernel launch are timed on this batch
+ * and the fastest kept (without: the context's defaults, the two kernels
+ * unless EBI_HIP_BATCH_MIXED=1 was set when it was created); the launch sequence is
+ * captured in a hipGraph.  Columns and outputs must outlive the batch.  No
+ * counterpart in the reference. */
 typedef struct ebi_hip_batch ebi_hip_batch;
 typedef struct ebi_batch_config {
     uint32_t blocks_per_cu; /* persistent decode grids = CUs x blocks_per_cu */
-    uint32_t reserved0;     /* 0 */
+    uint32_t mixed;         /* 1: tile and segment jobs in one launch (k_decode_mix),
+                               0: k_decode then k_decode_seg */
     uint32_t reserved1;     /* 0 */
     uint32_t reserved2;     /* 0 */
     double tuned_ms;        /* device ms of the chosen shape (0 if not tuned) */
@@ -328,7 +332,9 @@ double ebi_hip_last_device_ms(const ebi_hip_ctx *ctx);
  * for ebi_hip_decode_device / ebi_hip_decode_batch phase 0 = the persistent
  * tile kernel (Uncompressed, RLE, BUFF, DeltaSprintz), phase 2 = the
  * persistent segment kernel (Gorilla, Chimp, Elf, Chimp128), phase 1 = 0 (no
- * kernel); for the matmul entry points the phases listed there.
+ * kernel); for an eager ebi_hip_batch_run of a mixed batch phase 0 = the one
+ * mixed kernel and phase 2 = 0; for the matmul entry points the phases listed
+ * there.
  * Instrumentation of this library; no counterpart in the reference. */
 double ebi_hip_last_phase_ms(const ebi_hip_ctx *ctx, int phase);
 

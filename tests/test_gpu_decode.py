@@ -137,6 +137,48 @@ def test_prepared_batch_graph_replays_match_eager(ctx):
         d.close()
 
 
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_prepared_batch_mixed_and_two_kernel(mixed, monkeypatch):
+    """Untuned prepared batches of tile and segment codecs: one k_decode_mix
+    launch over both job lists, or k_decode then k_decode_seg, at small and
+    large grids (small chunks, so that one wave takes jobs of both kinds and
+    of several columns); bit-exact against the oracle in both forms."""
+    monkeypatch.setenv("EBI_HIP_BATCH_MIXED", str(mixed))  # read when a context is created
+    for bpc in (1, 16):
+        monkeypatch.setenv("EBI_HIP_BLOCKS_PER_CU", str(bpc))
+        c = E.Context(0)
+        try:
+            specs = [("walk2", "elf", 30011, 97), ("unif3", "chimp128", 20000, 300), ("step2", "rle", 25000, 640),
+                     ("sin4", "buff", 21000, 1500), ("walk2", "delta_sprintz", 17000, 700),
+                     ("walk2", "gorilla", 26000, 129), ("sin4", "uncompressed", 9000, 333), ("unif3", "chimp", 15000, 64)]
+            decs, wants, outs = [], [], []
+            for gen, codec, n, chunk in specs:
+                fn, dec = G.GENERATORS[gen]
+                f = E.encode(fn(n), E.CompressorConfig.by_name(codec, scale=10 ** dec),
+                             E.ChunkOption.record_count(chunk))
+                decs.append(E.Decoder(c, f))
+                wants.append(OracleDecoder(f).materialize())
+                outs.append(c.alloc(8 * n))
+            b = E.Batch(c, decs, outs, autotune=False)
+            assert b.config["mixed"] == bool(mixed) and b.config["blocks_per_cu"] == bpc
+            for eager in (True, False, False):
+                for o in outs:
+                    o.zero()
+                b.run(eager=eager)
+                if eager and mixed:  # one kernel: the segment phase is empty
+                    assert c.last_phase_ms(0) > 0 and c.last_phase_ms(2) < 0.05
+                for d, w, o in zip(decs, wants, outs):
+                    got = o.to_host(count=d.number_of_records)
+                    assert np.array_equal(got.view(np.uint64), w.view(np.uint64)), (E.SCHEME_NAMES[d.scheme], bpc)
+            b.close()
+            for o in outs:
+                o.free()
+            for d in decs:
+                d.close()
+        finally:
+            c.close()
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_decode_random_shapes(ctx, seed):
     """Random record counts, chunk sizes and generators for the XOR codecs

@@ -13,6 +13,7 @@ from ebi_amd import build as B
 KERNELS = {  # mangled-name prefix -> minimum occupancy (waves per SIMD)
     "_ZN3ebi8k_decodeE": 5,
     "_ZN3ebi12k_decode_segE": 4,
+    "_ZN3ebi12k_decode_mixE": 4,  # both job lists in one launch
     "_ZN3ebi11k_query_segE": 4,  # the segment decode with the fused queries
 }
 
