@@ -276,19 +276,29 @@ __global__ void k_acc_init(uint64_t *acc, uint32_t n_chunks) {
   acc[4ull * i + 3] = 0;
 }
 
-// One thread per chunk: fold its tile partials in tile order (double-double).
-__global__ void k_chunk_dd(const ColView *cols, const uint32_t *chunk_sel, const double *partials,
-                           double *chunk_sum, uint32_t n_chunks) {
-  const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ci >= n_chunks) return;
+// One wave per chunk: lane l folds the chunk's tile partials l, l + 64,
+// l + 128, ... (double-double), then a butterfly over the lanes; lane 0
+// writes.  The order is fixed, so every call gives the same sum.
+__global__ void __launch_bounds__(256) k_chunk_dd(const ColView *cols, const uint32_t *chunk_sel,
+                                                 const double *partials, double *chunk_sum, uint32_t n_chunks) {
+  const uint32_t ci = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (ci >= n_chunks) return;  // wave-uniform
+  const uint32_t lane = lane_id();
   double hi = 0.0, lo = 0.0;
-  if (chunk_sel[ci]) {
+  if (chunk_sel[ci]) {  // wave-uniform
     const DevChunk c = cols[0].chunks[ci];
     const uint32_t nt = (c.n + kTile - 1) / kTile;
-    for (uint32_t t = 0; t < nt; ++t) dd_add(hi, lo, partials[2ull * (c.rtile0 + t)], partials[2ull * (c.rtile0 + t) + 1]);
+    for (uint32_t t = lane; t < nt; t += kWave)
+      dd_add(hi, lo, partials[2ull * (c.rtile0 + t)], partials[2ull * (c.rtile0 + t) + 1]);
+    for (int d = kWave / 2; d >= 1; d >>= 1) {
+      const double oh = __shfl_xor(hi, d), ol = __shfl_xor(lo, d);
+      dd_add(hi, lo, oh, ol);
+    }
   }
-  chunk_sum[2ull * ci] = hi;
-  chunk_sum[2ull * ci + 1] = lo;
+  if (lane == 0) {
+    chunk_sum[2ull * ci] = hi;
+    chunk_sum[2ull * ci + 1] = lo;
+  }
 }
 
 // Chunk selection: chunk_id filter and bitmask pruning (api/decoder.rs:216-226).
@@ -318,7 +328,7 @@ __global__ void __launch_bounds__(256) k_chunk_select(const ColView *cols, const
 // --------------------------------------------------------------- compaction
 // Selected records of `values` (bit r of `bits` set) are written in record
 // order to `out`.  Block b covers words [b*4096, (b+1)*4096).
-constexpr uint32_t kScanWords = 16;  // words per thread
+constexpr uint32_t kScanWords = 2;  // words per thread (a block: 512 words = 32,768 records)
 constexpr uint32_t kScanBlock = 256;
 
 __device__ __forceinline__ uint64_t block_excl_scan(uint64_t x, uint64_t *lds, uint64_t *total) {
@@ -367,23 +377,56 @@ __global__ void __launch_bounds__(256) k_scan_counts(uint64_t *cnt, uint32_t n, 
   if (threadIdx.x == 0) *n_out = carry;
 }
 
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t src) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Writes the selected records of the block's words to out[block_off + ...]
+// in record order.  Each thread holds kScanWords bitset words; the wave then
+// walks its threads' words one 64-record word at a time, lane l taking
+// record l (its output slot = the word's first slot + the set bits below l),
+// so the value loads and the output stores are contiguous across the wave;
+// eight threads' words per round, loads issued before the stores.
 __global__ void __launch_bounds__(256) k_scatter(const uint64_t *bits, uint64_t n_words, const uint64_t *block_off,
-                                                const double *values, double *out) {
+                                                const double *__restrict__ values, double *__restrict__ out) {
   __shared__ uint64_t lds[kScanBlock / kWave];
   const uint64_t w0 = ((uint64_t)blockIdx.x * kScanBlock + threadIdx.x) * kScanWords;
+  uint64_t x[kScanWords];
   uint64_t c = 0;
-  for (uint32_t k = 0; k < kScanWords; ++k)
-    if (w0 + k < n_words) c += __popcll(bits[w0 + k]);
-  uint64_t total;
-  uint64_t pos = block_off[blockIdx.x] + block_excl_scan(c, lds, &total);
+#pragma unroll
   for (uint32_t k = 0; k < kScanWords; ++k) {
-    if (w0 + k >= n_words) break;
-    uint64_t x = bits[w0 + k];
-    while (x) {
-      const uint32_t b = __builtin_ctzll(x);
-      out[pos++] = values[((w0 + k) << 6) + b];
-      x &= x - 1;
+    x[k] = w0 + k < n_words ? bits[w0 + k] : 0;
+    c += __popcll(x[k]);
+  }
+  uint64_t total;
+  const uint64_t pos = block_off[blockIdx.x] + block_excl_scan(c, lds, &total);
+  const uint32_t lane = lane_id();
+  const uint64_t wave_w0 = w0 - (uint64_t)lane * kScanWords;  // the wave's first word
+  constexpr uint32_t kRound = 8;
+  for (uint32_t s0 = 0; s0 < kWave; s0 += kRound) {
+    double v[kRound][kScanWords];
+    uint64_t dst[kRound][kScanWords];
+    bool sel[kRound][kScanWords];
+#pragma unroll
+    for (uint32_t u = 0; u < kRound; ++u) {
+      uint64_t p = readlane64(pos, s0 + u);
+#pragma unroll
+      for (uint32_t k = 0; k < kScanWords; ++k) {
+        const uint64_t word = readlane64(x[k], s0 + u);
+        const uint64_t w = wave_w0 + (uint64_t)(s0 + u) * kScanWords + k;
+        sel[u][k] = (word >> lane) & 1;
+        dst[u][k] = p + __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0));
+        v[u][k] = sel[u][k] ? values[(w << 6) + lane] : 0.0;
+        p += __popcll(word);
+      }
     }
+#pragma unroll
+    for (uint32_t u = 0; u < kRound; ++u)
+#pragma unroll
+      for (uint32_t k = 0; k < kScanWords; ++k)
+        if (sel[u][k]) out[dst[u][k]] = v[u][k];
   }
 }
 
@@ -408,7 +451,7 @@ void aggregate(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_j
 void chunk_dd(hipStream_t s, const ColView *cols, const uint32_t *sel, const double *partials, double *chunk_sum,
               uint32_t n_chunks) {
   if (n_chunks)
-    hipLaunchKernelGGL(k_chunk_dd, dim3((n_chunks + 255) / 256), dim3(256), 0, s, cols, sel, partials, chunk_sum,
+    hipLaunchKernelGGL(k_chunk_dd, dim3((n_chunks + 3) / 4), dim3(256), 0, s, cols, sel, partials, chunk_sum,
                        n_chunks);
 }
 
