@@ -89,7 +89,8 @@ def main():
 
     ctx = E.Context(0)
     cols = []  # (gen, codec, decoder, compressed_bytes)
-    cpu = {"seconds": 0.0, "bytes": 0, "values": 0, "chunks": 0}
+    cpu = {"seconds": 0.0, "bytes": 0, "values": 0, "chunks": 0,
+           "par_seconds": 0.0, "par_bytes": 0, "par_chunks": 0, "par_threads": cpu_threads(), "core": None}
     comparator_files = {}
     sample_q = {}  # generator -> every 64th value, for predicate quantiles
     t_setup = time.time()
@@ -248,7 +249,16 @@ def main():
             "kind": "port",
             "sample": f"oracle decode of the first {cpu['chunks']} chunks across the {len(cols)} columns "
                       f"({cpu['values']} values, {cpu['seconds']:.1f}s on one core)",
+            "pinned_core": cpu["core"],
+            "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count(),
         }
+        if cpu["par_seconds"] > 0:  # secondary: chunk-parallel over the host cores this process may use
+            res["cpu_baseline"]["chunk_parallel"] = {
+                "value": round(cpu["par_bytes"] / cpu["par_seconds"] / 1e9, 4), "unit": "GB/s",
+                "cores": cpu["par_threads"],
+                "sample": f"oracle decode of {cpu['par_chunks']} chunks across the {len(cols)} columns, "
+                          f"one chunk per task on {cpu['par_threads']} threads ({cpu['par_seconds']:.1f}s wall)"}
         res["gpu_vs_cpu"] = round(value / res["cpu_baseline"]["value"], 1)
     for _, _, d, _ in cols:
         d.close()
@@ -597,24 +607,60 @@ def run_comparator(E, ctx, comparator_files):
     return comparator
 
 
+def cpu_threads():
+    """Host cores this process may use (at most 16, the box's CPU share)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_sample(data, n, chunk, acc, budget_s):
-    """Time the CPU oracle (one core) on the first chunks of this column until
-    roughly `budget_s` seconds have been spent."""
+    """Time the CPU oracle on the first chunks of this column: one core (the
+    calling thread pinned to the last core it may use) until roughly
+    `budget_s` seconds have been spent, then the chunk-parallel form (one
+    chunk per task on acc["par_threads"] threads; the oracle's ctypes calls
+    release the GIL) over as many chunks as the single core would decode in
+    about half that time on every thread."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from oracle.oracle import OracleDecoder
     import ebi_amd as E
 
     phys, size, logi, nrec = E.file_chunks(data)
     od = OracleDecoder(data)
-    t_end = time.perf_counter() + budget_s
-    k = 0
-    while k < len(size) and time.perf_counter() < t_end:
+    mask = os.sched_getaffinity(0)
+    acc["core"] = max(mask)
+    os.sched_setaffinity(0, {acc["core"]})  # this thread only
+    try:
+        t_end = time.perf_counter() + budget_s
+        k = 0
+        while k < len(size) and time.perf_counter() < t_end:
+            t = time.perf_counter()
+            v = od.materialize(chunk_id=k)
+            acc["seconds"] += time.perf_counter() - t
+            acc["bytes"] += int(size[k]) + 8 * v.size
+            acc["values"] += v.size
+            acc["chunks"] += 1
+            k += 1
+    finally:
+        os.sched_setaffinity(0, mask)
+    T = acc["par_threads"]
+    if T > 1 and k:
+        m = min(len(size), max(T, k * T // 2))
         t = time.perf_counter()
-        v = od.materialize(chunk_id=k)
-        acc["seconds"] += time.perf_counter() - t
-        acc["bytes"] += int(size[k]) + 8 * v.size
-        acc["values"] += v.size
-        acc["chunks"] += 1
-        k += 1
+        with ThreadPoolExecutor(T) as pool:
+            got = list(pool.map(lambda c: od.materialize(chunk_id=c).size, range(m)))
+        acc["par_seconds"] += time.perf_counter() - t
+        acc["par_bytes"] += int(np.sum(size[:m])) + 8 * sum(got)
+        acc["par_chunks"] += m
 
 
 if __name__ == "__main__":
