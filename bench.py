@@ -69,6 +69,7 @@ def main():
                     help="use the context's launch shape (EBI_HIP_* overrides) instead of autotuning the batch, "
                          "so a profiler sees only the timed launches")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replays after the timed region")
+    ap.add_argument("--serve-budget-gib", type=int, default=16, help="server device arena (query scratch), GiB")
     ap.add_argument("--no-breakdown", action="store_true",
                     help="skip the per-column breakdown and the comparator (after the timed region), so a "
                          "profiler sees only the batched launches")
@@ -395,7 +396,7 @@ def serve_leg(E, G, ctx, args, threads, n_cols=64):
         arrivals = np.cumsum(rng.exponential(1.0 / lam, nq))
         # served: submit each query at its arrival time (in batches of those due)
         lat = np.zeros(nq)
-        with E.Server(ctx, device_budget=16 << 30, max_batch=64, n_streams=4) as srv:
+        with E.Server(ctx, device_budget=args.serve_budget_gib << 30, max_batch=64, n_streams=4) as srv:
             tickets = [0] * nq
             lag = np.zeros(nq)
             start = time.perf_counter()
