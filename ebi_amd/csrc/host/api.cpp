@@ -101,9 +101,16 @@ int upload_view(ebi_hip_ctx *ctx, Column &c, double *out) {
   return EBI_OK;
 }
 
+hipError_t stream_wait(hipStream_t s) {
+  hipError_t e;
+  while ((e = hipStreamQuery(s)) == hipErrorNotReady) {
+  }
+  return e;
+}
+
 int sync_and_check(ebi_hip_ctx *ctx, const char *what) {
   hipError_t e = hipMemcpyAsync(ctx->h_flags, ctx->d_flags, 4, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = stream_wait(ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, what);
   uint32_t f = ctx->h_flags[0];
   if (f) {
@@ -682,13 +689,28 @@ extern "C" int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col
   launch::zero_u32(ctx->stream, ctx->d_flags, 2);
   hipEventRecord(ctx->ev0, ctx->stream);
   // The plain form: one launch per chunk and a host synchronisation after
-  // each (lane-serial for the XOR codecs, the chunk's tiles otherwise).
+  // each, of the same kernels as the batched form over the chunk's own jobs
+  // (its segment jobs for the XOR codecs, its tile jobs otherwise; both are
+  // contiguous slices of the resident lists, and a segment job's Job::col
+  // is absolute, so a slice decodes as in the whole list).
+  size_t sj = 0;  // the chunk's first segment job
   for (size_t k = 0; k < c.chunks.size(); ++k) {
     if (c.xor_codec || c.chimp128) {
-      if (c.chimp128)
-        launch::chimp128(ctx->stream, c.d_view, nullptr, 1, 1, c.d_single_jobs + k, ctx->d_flags, true);
-      else
-        launch::serial(ctx->stream, c.d_view, nullptr, 1, 1, c.d_single_jobs + k, ctx->d_flags, true, false);
+      size_t sj1 = sj;
+      while (sj1 < c.h_seg_jobs.size() && c.h_seg_jobs[sj1].chunk == k) ++sj1;
+      ColView v = view_of(c, d_out);
+      v.seg_jobs = c.d_seg_jobs + sj;
+      v.n_seg_jobs = (uint32_t)(sj1 - sj);
+      v.n_tile_jobs = 0;
+      uint32_t *pre = reinterpret_cast<uint32_t *>(ctx->h_batch);
+      pre[0] = 0;
+      pre[1] = v.n_seg_jobs;
+      std::memcpy(ctx->h_batch + 256, &v, sizeof v);
+      hipMemcpyAsync(ctx->batch_args.p, ctx->h_batch, 256 + sizeof v, hipMemcpyHostToDevice, ctx->stream);
+      launch::decode(ctx->stream, (const ColView *)((uint8_t *)ctx->batch_args.p + 256), 1, nullptr, 0,
+                     (const uint32_t *)ctx->batch_args.p, v.n_seg_jobs, ctx->d_flags,
+                     (uint32_t)ctx->cu_count * 8);
+      sj = sj1;
     } else {
       // the chunk's tile jobs are a contiguous slice of the resident list
       const uint32_t a = c.tile_job_first[k], b = c.tile_job_first[k + 1];
@@ -705,7 +727,7 @@ extern "C" int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col
                      (const uint32_t *)ctx->batch_args.p, b - a, nullptr, 0, ctx->d_flags,
                      (uint32_t)ctx->cu_count * 8);
     }
-    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipError_t e = stream_wait(ctx->stream);  // the same wait as the batched forms
     if (e != hipSuccess) return hip_fail(ctx, e, "per-chunk decode");
   }
   hipEventRecord(ctx->ev1, ctx->stream);

@@ -20,6 +20,10 @@ int fail(ebi_hip_ctx *ctx, int code, const std::string &msg);
 int hip_fail(ebi_hip_ctx *ctx, hipError_t e, const char *what);
 int map_dev_err(uint32_t flags);
 bool is_xor_codec(int scheme);
+// Waits for the work queued on `s` by polling it (the synchronous entry
+// points are latency-bound: a blocking wait adds the host thread's wake-up
+// time to every call); returns the first error the stream reports.
+hipError_t stream_wait(hipStream_t s);
 // The ColView of `c` with output base `out`.
 ColView view_of(const Column &c, double *out);
 // Uploads the single-column ColView of `c` with output base `out` and waits.

@@ -48,7 +48,7 @@ int run_query(ebi_hip_ctx *ctx, const ebi_query &q) {
   hipError_t e = group_enqueue(g, ctx->stream, ctx->cu_count, ctx->blocks_per_cu);
   if (e != hipSuccess) return hip_fail(ctx, e, "query launch");
   hipEventRecord(ctx->ev1, ctx->stream);
-  if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "query");
+  if ((e = stream_wait(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "query");
   float ms = 0;
   hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
   ctx->last_ms = ms;
@@ -58,7 +58,7 @@ int run_query(ebi_hip_ctx *ctx, const ebi_query &q) {
     return fail(ctx, map_dev_err(g.h_flags[0]), buf);
   }
   if (plan_enqueue_values(p, ctx->stream, &e)) {
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = stream_wait(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "query result copy");
   }
   return plan_finish(p);

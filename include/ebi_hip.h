@@ -239,8 +239,9 @@ int ebi_hip_batch_run(ebi_hip_ctx *ctx, ebi_hip_batch *b, int eager);
 int ebi_hip_batch_config_get(const ebi_hip_batch *b, ebi_batch_config *cfg);
 int ebi_hip_batch_destroy(ebi_hip_ctx *ctx, ebi_hip_batch *b);
 
-/* Comparator: the plain per-chunk-launch form -- one kernel launch per
- * chunk and a host synchronisation after each. */
+/* Comparator: the plain per-chunk-launch form -- one launch of the decode
+ * kernel per chunk (over that chunk's own tile or segment jobs) and a host
+ * synchronisation after each. */
 int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col, double *d_out);
 
 /* Decoder::matmul_cuda (api/decoder.rs:659-724, over

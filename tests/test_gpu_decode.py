@@ -76,6 +76,8 @@ def test_decode_xor_adversarial(ctx, codec):
 def test_decode_per_chunk_comparator(ctx, codec):
     vals = G.random_walk(6000, seed=11)
     _check(ctx, vals, codec, E.ChunkOption.record_count(1000), per_chunk=True)
+    # several segment jobs per chunk (XOR codecs), ragged last chunk
+    _check(ctx, G.random_walk(50001, seed=12), codec, E.ChunkOption.record_count(8192), per_chunk=True)
 
 
 @pytest.mark.parametrize("codec", CODECS)
