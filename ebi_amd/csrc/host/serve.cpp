@@ -11,9 +11,11 @@
 //    and pinned arenas carved once at create), issues every query admitted in
 //    one pass as one group on the next stream (one batched decode launch for
 //    the group), and polls the groups' completion events;
-//  * the finisher thread copies materialized records into the callers'
-//    buffers, folds per-chunk partials, releases the arenas and completes the
-//    tickets, so the scheduler never blocks on a host copy.
+//  * kFinishers finisher threads (each with its own copy stream) copy
+//    materialized records into the callers' buffers, fold per-chunk
+//    partials, release the arenas and complete the tickets, so the
+//    scheduler never blocks on a host copy and one large result does not
+//    hold up the completions behind it.
 // Streams keep the default priority.
 #include <algorithm>
 #include <chrono>
@@ -118,7 +120,7 @@ struct ebi_hip_server {
   uint8_t *pin_base = nullptr;
   Arena dev, pin;
   std::vector<hipStream_t> streams;
-  hipStream_t copy_stream = nullptr;
+  std::vector<hipStream_t> copy_streams;  // one per finisher
   std::vector<hipEvent_t> free_events;
   std::mutex mu;
   std::condition_variable cv_sched, cv_fin, cv_done;
@@ -126,7 +128,8 @@ struct ebi_hip_server {
   std::unordered_map<uint64_t, Ticket *> tickets;
   std::vector<Group *> groups;     // on the device
   std::deque<Ticket *> finish_q;   // device work complete, host finish pending
-  std::thread sched, finisher;
+  std::thread sched;
+  std::vector<std::thread> finishers;
   bool stop = false, sched_done = false;
   uint64_t next_id = 1;
   uint32_t inflight = 0, rr = 0;
@@ -258,7 +261,7 @@ void scheduler(ebi_hip_server *s) {
       s->groups.erase(s->groups.begin() + i);
       progressed = true;
     }
-    if (progressed) s->cv_fin.notify_one();
+    if (progressed) s->cv_fin.notify_all();
     if (s->stop && s->pending.empty() && s->groups.empty()) break;
     if (!s->groups.empty()) {
       // device work outstanding: short poll interval
@@ -276,7 +279,9 @@ void scheduler(ebi_hip_server *s) {
   s->cv_fin.notify_all();
 }
 
-void finisher(ebi_hip_server *s) {
+constexpr int kFinishers = 4;
+
+void finisher(ebi_hip_server *s, hipStream_t copy_stream) {
   hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
@@ -291,7 +296,7 @@ void finisher(ebi_hip_server *s) {
     int status = t->plan.status;
     if (!status) {
       hipError_t e = hipSuccess;
-      if (plan_enqueue_values(t->plan, s->copy_stream, &e) && e == hipSuccess) e = hipStreamSynchronize(s->copy_stream);
+      if (plan_enqueue_values(t->plan, copy_stream, &e) && e == hipSuccess) e = hipStreamSynchronize(copy_stream);
       status = e != hipSuccess ? EBI_HIP : plan_finish(t->plan);
     }
     lk.lock();
@@ -307,6 +312,7 @@ void finisher(ebi_hip_server *s) {
     complete(s, t, status);
     s->cv_done.notify_all();
     s->cv_sched.notify_one();
+    if (s->sched_done && s->inflight == 0) s->cv_fin.notify_all();  // the other finishers end too
   }
 }
 
@@ -329,7 +335,7 @@ extern "C" int ebi_hip_serve_create(ebi_hip_ctx *ctx, const ebi_serve_opts *opts
   s->cu_count = ctx->cu_count;
   auto bail = [&](int code, hipError_t e, const char *what) {
     for (hipStream_t st : s->streams) hipStreamDestroy(st);
-    if (s->copy_stream) hipStreamDestroy(s->copy_stream);
+    for (hipStream_t st : s->copy_streams) hipStreamDestroy(st);
     if (s->dev_base) hipFree(s->dev_base);
     if (s->pin_base) hipHostFree(s->pin_base);
     delete s;
@@ -350,10 +356,14 @@ extern "C" int ebi_hip_serve_create(ebi_hip_ctx *ctx, const ebi_serve_opts *opts
     if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return bail(EBI_HIP, e, "serve: stream");
     s->streams.push_back(st);
   }
-  if ((e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking)) != hipSuccess)
-    return bail(EBI_HIP, e, "serve: copy stream");
+  for (int i = 0; i < kFinishers; ++i) {
+    hipStream_t st;
+    if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
+      return bail(EBI_HIP, e, "serve: copy stream");
+    s->copy_streams.push_back(st);
+  }
   s->sched = std::thread(scheduler, s);
-  s->finisher = std::thread(finisher, s);
+  for (hipStream_t st : s->copy_streams) s->finishers.emplace_back(finisher, s, st);
   *out = s;
   return EBI_OK;
 }
@@ -445,10 +455,11 @@ extern "C" int ebi_hip_serve_destroy(ebi_hip_server *s) {
   s->cv_sched.notify_all();
   if (s->sched.joinable()) s->sched.join();
   s->cv_fin.notify_all();
-  if (s->finisher.joinable()) s->finisher.join();
+  for (std::thread &f : s->finishers)
+    if (f.joinable()) f.join();
   for (auto &kv : s->tickets) delete kv.second;
   for (hipStream_t st : s->streams) hipStreamDestroy(st);
-  if (s->copy_stream) hipStreamDestroy(s->copy_stream);
+  for (hipStream_t st : s->copy_streams) hipStreamDestroy(st);
   for (hipEvent_t e : s->free_events) hipEventDestroy(e);
   hipFree(s->dev_base);
   hipHostFree(s->pin_base);
