@@ -123,11 +123,13 @@ Layout layout_of(const Plan &p) {
   const uint64_t nb = launch::compact_blocks(p.words) + 1;
   L.mask = take(wbytes);
   L.bits = take(wbytes);
+  // the per-chunk results (sel, acc, csum) back to back, laid out as in the
+  // pinned staging, so that one copy brings them all to the host
   L.sel = take((uint64_t)p.nc * 4);
-  L.cpred = take((uint64_t)p.nc * sizeof(ChunkPred));
   L.acc = take((uint64_t)p.nc * 32);
-  L.part = take((uint64_t)p.c->n_rtiles * 16);
   L.csum = take((uint64_t)p.nc * 16);
+  L.cpred = take((uint64_t)p.nc * sizeof(ChunkPred));
+  L.part = take((uint64_t)p.c->n_rtiles * 16);
   L.block = take(nb * 8);
   // DECODE writes the caller's buffer; fused plans keep the records in registers
   L.values = p.decode && !p.fused && p.q.op != kOpDecode ? take(N * 8) : ~0ull;
@@ -151,9 +153,14 @@ PinLayout pin_layout_of(const Plan &p) {
   };
   L.mask = p.has_mask ? take(p.words * 8) : ~0ull;
   L.bits = p.q.op == kOpFilter ? take(p.words * 8) : ~0ull;
-  L.sel = take((uint64_t)p.nc * 4);
-  L.acc = take((uint64_t)p.nc * 32);
-  L.csum = take((uint64_t)p.nc * 16);
+  // the per-chunk results at the device layout's spacing (kAlign)
+  off = round_up(off, kAlign);
+  L.sel = off;
+  off += round_up(std::max<uint64_t>((uint64_t)p.nc * 4, 8), kAlign);
+  L.acc = off;
+  off += round_up(std::max<uint64_t>((uint64_t)p.nc * 32, 8), kAlign);
+  L.csum = off;
+  off += round_up(std::max<uint64_t>((uint64_t)p.nc * 16, 8), kAlign);
   L.n = take(8);
   L.cpred = p.cpred ? take((uint64_t)p.nc * sizeof(ChunkPred)) : ~0ull;
   L.total = off;
@@ -202,6 +209,16 @@ hipError_t plan_enqueue_pre(Plan &p, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The per-chunk results (sel, acc and, with `csum`, the chunk sums) to the
+// pinned staging in one copy: both layouts hold them back to back at the
+// same offsets.
+void copy_results(const Plan &p, hipStream_t s, bool csum) {
+  const uint8_t *a = reinterpret_cast<const uint8_t *>(p.sel);
+  const uint8_t *b = csum ? reinterpret_cast<const uint8_t *>(p.chunk_sum + 2ull * p.nc)
+                          : reinterpret_cast<const uint8_t *>(p.acc + 4ull * p.nc);
+  if (p.nc) hipMemcpyAsync(p.h_sel, p.sel, (size_t)(b - a), hipMemcpyDeviceToHost, s);
+}
+
 // Async part after the decode: query kernels and small results to pinned.
 // err: the group's device error flags; max_blocks: the persistent grid.
 hipError_t plan_enqueue_post(Plan &p, hipStream_t s, uint32_t *err, uint32_t max_blocks) {
@@ -219,9 +236,7 @@ hipError_t plan_enqueue_post(Plan &p, hipStream_t s, uint32_t *err, uint32_t max
     }
     if (op == kOpFilterSum) launch::chunk_select(s, p.d_view, p.bits, p.q.chunk_id, p.sel, p.nc);
     if (agg == 0) launch::chunk_dd(s, p.d_view, p.sel, p.partials, p.chunk_sum, p.nc);
-    hipMemcpyAsync(p.h_sel, p.sel, (size_t)p.nc * 4, hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(p.h_acc, p.acc, (size_t)p.nc * 32, hipMemcpyDeviceToHost, s);
-    if (agg == 0) hipMemcpyAsync(p.h_csum, p.chunk_sum, (size_t)p.nc * 16, hipMemcpyDeviceToHost, s);
+    copy_results(p, s, agg == 0);
     return hipGetLastError();
   }
   if (is_agg(op)) {
@@ -236,9 +251,7 @@ hipError_t plan_enqueue_post(Plan &p, hipStream_t s, uint32_t *err, uint32_t max
     launch::aggregate(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, a, p.nc);
     const bool f64_sum = !p.insitu && (op == kOpSum || op == kOpFilterSum);
     if (f64_sum) launch::chunk_dd(s, p.d_view, p.sel, p.partials, p.chunk_sum, p.nc);
-    hipMemcpyAsync(p.h_sel, p.sel, (size_t)p.nc * 4, hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(p.h_acc, p.acc, (size_t)p.nc * 32, hipMemcpyDeviceToHost, s);
-    if (f64_sum) hipMemcpyAsync(p.h_csum, p.chunk_sum, (size_t)p.nc * 16, hipMemcpyDeviceToHost, s);
+    copy_results(p, s, f64_sum);
   } else if (op == kOpFilter) {
     launch::filter(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, args_of(p), p.filter_mode);
     if (p.words) hipMemcpyAsync(p.h_bits, p.bits, p.words * 8, hipMemcpyDeviceToHost, s);
