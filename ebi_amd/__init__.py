@@ -413,8 +413,16 @@ def ids_to_bitset(ids, n_records: int) -> np.ndarray:
 
 
 def bitset_to_ids(words: np.ndarray, n_records: int) -> np.ndarray:
-    bits = np.unpackbits(words.view(np.uint8), bitorder="little")[:n_records]
-    return np.flatnonzero(bits).astype(np.uint32)
+    """Dense u64 words -> ascending global record ids below n_records (only
+    the nonzero words are unpacked)."""
+    w = np.ascontiguousarray(np.asarray(words, dtype=np.uint64)[: (n_records + 63) // 64])
+    nz = np.flatnonzero(w)
+    if nz.size == 0:
+        return np.empty(0, dtype=np.uint32)
+    bits = np.unpackbits(w[nz].view(np.uint8), bitorder="little").reshape(nz.size, 64)
+    r, c = np.nonzero(bits)
+    ids = (nz[r].astype(np.int64) << 6) + c
+    return ids[ids < n_records].astype(np.uint32)
 
 
 # ------------------------------------------------------------------ device
