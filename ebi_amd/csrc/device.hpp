@@ -167,7 +167,11 @@ struct alignas(16) ColView {
   // 128 records before its first record (the ring a job starts from; zeros
   // before record 0), at c128_ring[128 * Job::col]
   const uint64_t *c128_ring;
-  uint64_t pad0, pad1;
+  // per chunk: records before the chunk's first NaN end marker, written by
+  // the load-time passes only when that is below the footer's record count
+  // (the reference stops a chunk at its first NaN, gorilla.rs:149-152)
+  uint32_t *chunk_end;
+  uint64_t pad1;
   const uint4 *ilv;  // lane-interleaved segment streams (Gorilla, Chimp, Elf; null before they are built)
 };
 static_assert(sizeof(ColView) == 112, "ColView layout");
@@ -178,6 +182,9 @@ enum DevError : uint32_t {
   kErrOverrun = 1,      // a reader went past the chunk's bytes
   kErrCount = 2,        // terminator or run total disagrees with the record count
   kErrCorrupt = 4,      // field combination the reference panics on
+  kNanEnd = 8,          // not an error: a NaN end marker before the footer's record
+                        // count ends the chunk there (returned by the serial decoders
+                        // with the marker's record index in XorState::end)
 };
 
 }  // namespace ebi

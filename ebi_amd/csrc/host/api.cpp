@@ -87,6 +87,7 @@ ColView view_of(const Column &c, double *out) {
   v.n_serial_jobs = c.n_serial_jobs;
   v.ilv = c.d_ilv;
   v.c128_ring = c.d_c128_ring;
+  v.chunk_end = c.d_chunk_end;
   return v;
 }
 
@@ -156,6 +157,7 @@ static void free_column(Column &c) {
   hipFree(c.d_ilv);
   hipFree(c.d_c128_ring);
   hipFree(c.d_chunk_err);
+  hipFree(c.d_chunk_end);
   hipFree(c.d_view);
   hipFree(c.d_prefix);
   hipFree(c.d_tile_jobs);
@@ -163,7 +165,36 @@ static void free_column(Column &c) {
   hipFree(c.d_seg_jobs);
   hipFree(c.d_single_jobs);
   hipFree(c.d_rec_jobs);
-  c = Column();
+}
+
+// Chunks that end at a NaN before their footer record count (the load-time
+// passes wrote the NaN's record index to d_chunk_end, which starts as the
+// footer counts): their descriptors get the shortened count, on the host
+// and in the device table.  The reference's XOR readers stop at the first
+// NaN and return the records before it without an error
+// (chunk_reader/gorilla.rs:149-152, chimp.rs:119, chimp_n.rs:219-265,
+// elf.rs:105-258), so every later query sees the shortened chunk.
+static int apply_chunk_ends(ebi_hip_ctx *ctx, Column &col) {
+  const size_t nc = col.chunks.size();
+  if (!nc || !col.d_chunk_end) return EBI_OK;
+  std::vector<uint32_t> ends(nc);
+  hipError_t e = hipMemcpy(ends.data(), col.d_chunk_end, nc * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(ctx, e, "chunk ends");
+  bool changed = false;
+  for (size_t k = 0; k < nc; ++k) {
+    DevChunk &d = col.chunks[k].dev;
+    if (ends[k] < d.n && !col.chunks[k].err) {
+      d.n = ends[k];
+      changed = true;
+    }
+  }
+  if (!changed) return EBI_OK;
+  col.truncated = true;
+  std::vector<DevChunk> table(nc);
+  for (size_t k = 0; k < nc; ++k) table[k] = col.chunks[k].dev;
+  e = hipMemcpy(col.d_chunks, table.data(), nc * sizeof(DevChunk), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(ctx, e, "chunk table (NaN ends)");
+  return EBI_OK;
 }
 
 template <class T>
@@ -396,6 +427,11 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
     return hip_fail(ctx, e, "hipMalloc(checkpoints)");
   if ((e = hipMalloc((void **)&col.d_chunk_err, std::max<size_t>(nc, 1) * 4)) != hipSuccess)
     return hip_fail(ctx, e, "hipMalloc(chunk errors)");
+  if (col.xor_codec || col.chimp128) {
+    std::vector<uint32_t> ends(nc);
+    for (size_t k = 0; k < nc; ++k) ends[k] = col.chunks[k].dev.n;
+    if ((e = upload_vec(&col.d_chunk_end, ends)) != hipSuccess) return hip_fail(ctx, e, "chunk ends");
+  }
   if ((e = hipMalloc((void **)&col.d_view, sizeof(ColView))) != hipSuccess)
     return hip_fail(ctx, e, "hipMalloc(view)");
 
@@ -474,8 +510,12 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return hip_fail(ctx, e, "index pass");
   for (size_t k = 0; k < nc; ++k) col.chunks[k].err = errs[k];
+  // Gorilla / Chimp / Elf: the index pass found the NaN ends; Chimp128's
+  // parse-only index cannot see values, its ring build's decode finds them
+  if (col.xor_codec && (rc = apply_chunk_ends(ctx, col))) return rc;
   if (col.xor_codec && col.n_seg_jobs && (rc = build_ilv(ctx, col))) return rc;
   if (col.chimp128 && col.n_seg_jobs && (rc = build_c128_ring(ctx, col))) return rc;
+  if (col.chimp128 && (rc = apply_chunk_ends(ctx, col))) return rc;
   if (scheme == kRLE && nc) {
     // Balance RLE decode jobs by output: a job expands up to kTilesPerJob
     // steps of kTile output records of one field tile (runs of up to 255
@@ -636,6 +676,8 @@ extern "C" int ebi_hip_column_load(ebi_hip_ctx *ctx, const uint8_t *file, size_t
 extern "C" int ebi_hip_column_free(ebi_hip_ctx *ctx, ebi_hip_col *col) {
   EBI_GUARD(ctx);
   if (!col) return EBI_INVALID_ARGUMENT;
+  if (col->c.users.load())
+    return fail(ctx, EBI_PRECONDITION, "column in use by a queued or running served query or a prepared batch");
   hipStreamSynchronize(ctx->stream);
   free_column(col->c);
   delete col;
@@ -784,9 +826,9 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
     if (!cols[i] || (!d_outs[i] && cols[i]->c.meta.number_of_records)) return EBI_INVALID_ARGUMENT;
     if ((rc = check_chunk_errors(ctx, cols[i]->c, -1))) return rc;
   }
-  // arguments: prefix over tile jobs, prefix over Chimp128 chunks, prefix
-  // over segment jobs, then the ColView array (256-byte aligned)
-  const size_t pre_bytes = ((3 * (n_cols + 1) * 4 + 255) / 256) * 256;
+  // arguments: prefix over tile jobs, prefix over segment jobs, then the
+  // ColView array (256-byte aligned)
+  const size_t pre_bytes = ((2 * (n_cols + 1) * 4 + 255) / 256) * 256;
   const size_t bytes = pre_bytes + n_cols * sizeof(ColView);
   if (bytes > ctx->h_batch_cap) {
     hipHostFree(ctx->h_batch);
@@ -798,22 +840,20 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   }
   if (ctx->batch_args.ensure(bytes)) return fail(ctx, EBI_OOM, "batch arguments");
   uint32_t *pre = reinterpret_cast<uint32_t *>(ctx->h_batch);
-  uint32_t *pre128 = pre + (n_cols + 1), *preseg = pre128 + (n_cols + 1);
+  uint32_t *preseg = pre + (n_cols + 1);
   ColView *views = reinterpret_cast<ColView *>(ctx->h_batch + pre_bytes);
-  uint64_t jobs = 0, jobs128 = 0, segs = 0;
+  uint64_t jobs = 0, segs = 0;
   for (size_t i = 0; i < n_cols; ++i) {
     const Column &c = cols[i]->c;
     pre[i] = (uint32_t)jobs;
-    pre128[i] = (uint32_t)jobs128;
     preseg[i] = (uint32_t)segs;
     views[i] = view_of(c, d_outs[i]);
     jobs += c.n_tile_jobs;
     segs += c.n_seg_jobs;
   }
   pre[n_cols] = (uint32_t)jobs;
-  pre128[n_cols] = (uint32_t)jobs128;
   preseg[n_cols] = (uint32_t)segs;
-  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32) || segs >= (1ull << 32))
+  if (jobs >= (1ull << 32) || segs >= (1ull << 32))
     return fail(ctx, EBI_UNSUPPORTED, "batch too large");
   uint8_t *d = (uint8_t *)ctx->batch_args.p;
   hipError_t e = hipMemcpyAsync(d, ctx->h_batch, bytes, hipMemcpyHostToDevice, ctx->stream);
@@ -822,7 +862,7 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   hipEventRecord(ctx->ev0, ctx->stream);
   const ColView *dv = (const ColView *)(d + pre_bytes);
   launch::decode(ctx->stream, dv, (uint32_t)n_cols, (const uint32_t *)d, (uint32_t)jobs,
-                 (const uint32_t *)d + 2 * (n_cols + 1), (uint32_t)segs, ctx->d_flags,
+                 (const uint32_t *)d + (n_cols + 1), (uint32_t)segs, ctx->d_flags,
                  (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg);
   hipEventRecord(ctx->ev_mid, ctx->stream);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "batch launch");

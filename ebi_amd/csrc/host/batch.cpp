@@ -18,9 +18,9 @@ using namespace ebi;
 struct ebi_hip_batch {
   std::vector<Column *> cols;
   std::vector<double *> outs;
-  DevBuf args;     // prefix[n + 1] | prefix128[n + 1] | prefix_seg[n + 1] | pad | ColView[n]
+  DevBuf args;     // prefix[n + 1] | prefix_seg[n + 1] | pad | ColView[n]
   uint64_t pre_bytes = 0;
-  uint32_t n_cols = 0, jobs = 0, jobs128 = 0, segs = 0;
+  uint32_t n_cols = 0, jobs = 0, segs = 0;
   ebi_batch_config cfg{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -39,7 +39,7 @@ void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool
   hipStream_t s = ctx->stream;
   launch::zero_u32(s, ctx->d_flags, 2);
   if (timed) hipEventRecord(ctx->ev0, s);
-  launch::decode(s, views_of(b), b->n_cols, prefix_of(b), b->jobs, prefix_of(b) + 2 * (b->n_cols + 1), b->segs,
+  launch::decode(s, views_of(b), b->n_cols, prefix_of(b), b->jobs, prefix_of(b) + (b->n_cols + 1), b->segs,
                  ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr,
                  c.mixed != 0);
   if (timed) hipEventRecord(ctx->ev_mid, s);
@@ -90,6 +90,7 @@ int capture(ebi_hip_ctx *ctx, ebi_hip_batch *b) {
 }
 
 void destroy(ebi_hip_batch *b) {
+  for (Column *c : b->cols) c->users.fetch_sub(1);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   b->args.release();
@@ -111,32 +112,29 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   ebi_hip_batch *b = new (std::nothrow) ebi_hip_batch();
   if (!b) return EBI_OOM;
   b->n_cols = (uint32_t)n_cols;
-  b->pre_bytes = ((3 * (n_cols + 1) * 4 + 255) / 256) * 256;
+  b->pre_bytes = ((2 * (n_cols + 1) * 4 + 255) / 256) * 256;
   std::vector<uint8_t> host(b->pre_bytes + n_cols * sizeof(ColView));
-  uint32_t *pre = reinterpret_cast<uint32_t *>(host.data()), *pre128 = pre + (n_cols + 1),
-           *preseg = pre128 + (n_cols + 1);
+  uint32_t *pre = reinterpret_cast<uint32_t *>(host.data()), *preseg = pre + (n_cols + 1);
   ColView *views = reinterpret_cast<ColView *>(host.data() + b->pre_bytes);
-  uint64_t jobs = 0, jobs128 = 0, segs = 0;
+  uint64_t jobs = 0, segs = 0;
   for (size_t i = 0; i < n_cols; ++i) {
     Column &c = const_cast<ebi_hip_col *>(cols[i])->c;
+    c.users.fetch_add(1);  // the captured graph holds the column's addresses
     b->cols.push_back(&c);
     b->outs.push_back(d_outs[i]);
     pre[i] = (uint32_t)jobs;
-    pre128[i] = (uint32_t)jobs128;
     preseg[i] = (uint32_t)segs;
     views[i] = view_of(c, d_outs[i]);
     jobs += c.n_tile_jobs;
     segs += c.n_seg_jobs;
   }
   pre[n_cols] = (uint32_t)jobs;
-  pre128[n_cols] = (uint32_t)jobs128;
   preseg[n_cols] = (uint32_t)segs;
-  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32) || segs >= (1ull << 32)) {
+  if (jobs >= (1ull << 32) || segs >= (1ull << 32)) {
     destroy(b);
     return fail(ctx, EBI_UNSUPPORTED, "batch too large");
   }
   b->jobs = (uint32_t)jobs;
-  b->jobs128 = (uint32_t)jobs128;
   b->segs = (uint32_t)segs;
   hipError_t e;
   if (b->args.ensure(host.size()) ||

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -46,6 +47,7 @@ struct Column {
   uint64_t *d_c128_ring = nullptr;  // Chimp128: per segment job the 128 values before it
   uint64_t ilv_rows = 0;     // 1 KiB rows of d_ilv
   uint32_t *d_chunk_err = nullptr;
+  uint32_t *d_chunk_end = nullptr;  // load-time passes: records before a chunk's first NaN
   ColView *d_view = nullptr;  // slot used by single-column launches
   uint32_t *d_prefix = nullptr;  // {0, tile jobs, 0, serial jobs, 0, chunks, 0, seg jobs}: single-column launches
   // resident job lists
@@ -64,6 +66,13 @@ struct Column {
   uint64_t tile_index_entries = 0, ckpt_entries = 0;
   bool xor_codec = false;
   bool chimp128 = false;
+  // some chunk ends at a NaN before its footer record count (its DevChunk n
+  // is shortened to the records before the NaN): the column's records are no
+  // longer contiguous, so a whole-column materialize compacts them
+  bool truncated = false;
+  // plans and prepared batches that use the column (column_free refuses
+  // while this is nonzero)
+  std::atomic<uint32_t> users{0};
 };
 
 }  // namespace ebi

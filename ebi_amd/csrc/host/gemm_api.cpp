@@ -62,6 +62,9 @@ extern "C" int ebi_hip_matmul(ebi_hip_ctx *ctx, const ebi_hip_col *colc, const d
       mul_overflows(data_rows, data_cols, &data_size) || mul_overflows(target_rows, target_cols, &target_size) ||
       N % data_size != 0)
     return fail(ctx, EBI_PRECONDITION, "matmul: matrix shapes do not match the column");
+  // a chunk that ends at a NaN holds fewer records than its data matrices
+  // (the reference's slice of the decoded chunk, chunk_reader.rs:271, panics)
+  if (c.truncated) return fail(ctx, EBI_PRECONDITION, "matmul: a chunk ends at a NaN before its record count");
   if (data_rows > UINT32_MAX || data_cols > UINT32_MAX || target_cols > UINT32_MAX)
     return fail(ctx, EBI_UNSUPPORTED, "matmul: matrix dimension above 2^32");
   const uint64_t batch = N / data_size;
@@ -101,6 +104,8 @@ extern "C" int ebi_hip_gemm_decoded(ebi_hip_ctx *ctx, const ebi_hip_col *col_a, 
   if (!m || !n || !k || mul_overflows(m, k, &mk) || mul_overflows(k, n, &kn) || mul_overflows(m, n, &mn) ||
       a.meta.number_of_records != mk || b.meta.number_of_records != kn)
     return fail(ctx, EBI_PRECONDITION, "gemm_decoded: column sizes are not m*k and k*n");
+  if (a.truncated || b.truncated)
+    return fail(ctx, EBI_PRECONDITION, "gemm_decoded: a chunk ends at a NaN before its record count");
   if (m > UINT32_MAX || n > UINT32_MAX || k > UINT32_MAX)
     return fail(ctx, EBI_UNSUPPORTED, "gemm_decoded: matrix dimension above 2^32");
   if (ctx->gemm_a.ensure(mk * 8) || ctx->gemm_b.ensure(kn * 8) || (!c_device && ctx->gemm_c.ensure(mn * 8)))

@@ -102,7 +102,7 @@ bool has_pred(int op) { return op == kOpFilter || op == kOpFilterMaterialize || 
 // ops whose filter writes the bitset `bits`
 bool writes_bits(const Plan &p) {
   return p.q.op == kOpFilter || p.q.op == kOpFilterMaterialize || p.q.op == kOpFilterSum ||
-         (p.q.op == kOpMaterialize && p.has_mask);
+         (p.q.op == kOpMaterialize && p.compact);
 }
 
 // Offsets of the device scratch of a plan (all 256-byte aligned).
@@ -133,7 +133,7 @@ Layout layout_of(const Plan &p) {
   L.block = take(nb * 8);
   // DECODE writes the caller's buffer; fused plans keep the records in registers
   L.values = p.decode && !p.fused && p.q.op != kOpDecode ? take(N * 8) : ~0ull;
-  L.out = (p.q.op == kOpFilterMaterialize || (p.q.op == kOpMaterialize && p.has_mask)) ? take(N * 8) : ~0ull;
+  L.out = (p.q.op == kOpFilterMaterialize || (p.q.op == kOpMaterialize && p.compact)) ? take(N * 8) : ~0ull;
   L.total = off;
   return L;
 }
@@ -255,7 +255,7 @@ hipError_t plan_enqueue_post(Plan &p, hipStream_t s, uint32_t *err, uint32_t max
   } else if (op == kOpFilter) {
     launch::filter(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, args_of(p), p.filter_mode);
     if (p.words) hipMemcpyAsync(p.h_bits, p.bits, p.words * 8, hipMemcpyDeviceToHost, s);
-  } else if (op == kOpFilterMaterialize || (op == kOpMaterialize && p.has_mask)) {
+  } else if (op == kOpFilterMaterialize || (op == kOpMaterialize && p.compact)) {
     launch::filter(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, args_of(p), p.filter_mode);
     launch::compact(s, p.bits, p.words, p.values, p.out, p.block, p.d_n);
     hipMemcpyAsync(p.h_n, p.d_n, 8, hipMemcpyDeviceToHost, s);
@@ -302,6 +302,8 @@ int plan_init(Plan &p, const ebi_query &q) {
   const int scheme = p.c->meta.scheme;
   p.insitu = scheme == kBUFF || scheme == kDeltaSprintz;
   p.has_mask = q.bitset != nullptr;
+  // one chunk's records stay contiguous (its DevChunk n is the shortened count)
+  p.compact = p.has_mask || (p.c->truncated && q.chunk_id < 0);
   p.words = (N + 63) / 64;
   p.nc = (uint32_t)p.c->chunks.size();
   switch (q.op) {
@@ -391,7 +393,7 @@ bool plan_enqueue_values(Plan &p, hipStream_t s, hipError_t *err) {
   *err = hipSuccess;
   const int op = p.q.op;
   const Column &c = *p.c;
-  if (op == kOpMaterialize && !p.has_mask) {
+  if (op == kOpMaterialize && !p.compact) {
     // every record of the selected chunks, in chunk order
     uint64_t lo = 0, hi = c.meta.number_of_records;
     if (p.q.chunk_id >= 0) {
@@ -492,17 +494,17 @@ int plan_finish(Plan &p) {
   return EBI_OK;
 }
 
-// Group block: views[n] | prefix[n + 1] | prefix128[n + 1] | prefix_seg[n + 1] | flags[4],
+// Group block: views[n] | prefix[n + 1] | prefix_seg[n + 1] | flags[4],
 // each part 256-byte aligned.
 uint64_t group_block_bytes(size_t n) {
-  return round_up(n * sizeof(ColView), kAlign) + 3 * round_up((n + 1) * 4, kAlign) + kAlign;
+  return round_up(n * sizeof(ColView), kAlign) + 2 * round_up((n + 1) * 4, kAlign) + kAlign;
 }
 
 void group_bind(PlanGroup &g, uint8_t *dev, uint8_t *pin) {
   g.dev = dev;
   g.pin = pin;
   const size_t n = g.plans.size();
-  const uint64_t flags = round_up(n * sizeof(ColView), kAlign) + 3 * round_up((n + 1) * 4, kAlign);
+  const uint64_t flags = round_up(n * sizeof(ColView), kAlign) + 2 * round_up((n + 1) * 4, kAlign);
   g.d_flags = reinterpret_cast<uint32_t *>(dev + flags);
   g.h_flags = reinterpret_cast<uint32_t *>(pin + flags);
   for (size_t i = 0; i < n; ++i) g.plans[i]->d_view = reinterpret_cast<const ColView *>(dev) + i;
@@ -515,14 +517,12 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   const uint64_t pbytes = round_up((n + 1) * 4, kAlign);
   ColView *views = reinterpret_cast<ColView *>(g.pin);
   uint32_t *pre = reinterpret_cast<uint32_t *>(g.pin + vbytes);
-  uint32_t *pre128 = reinterpret_cast<uint32_t *>(g.pin + vbytes + pbytes);
-  uint32_t *preseg = reinterpret_cast<uint32_t *>(g.pin + vbytes + 2 * pbytes);
-  uint64_t jobs = 0, jobs128 = 0, segs = 0;
+  uint32_t *preseg = reinterpret_cast<uint32_t *>(g.pin + vbytes + pbytes);
+  uint64_t jobs = 0, segs = 0;
   for (size_t i = 0; i < n; ++i) {
     Plan &p = *g.plans[i];
     views[i] = view_of(*p.c, p.values);
     pre[i] = (uint32_t)jobs;
-    pre128[i] = (uint32_t)jobs128;
     preseg[i] = (uint32_t)segs;
     if (p.decode) {
       jobs += p.c->n_tile_jobs;
@@ -530,17 +530,16 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
     }
   }
   pre[n] = (uint32_t)jobs;
-  pre128[n] = (uint32_t)jobs128;
   preseg[n] = (uint32_t)segs;
-  if (jobs >= (1ull << 32) || jobs128 >= (1ull << 32) || segs >= (1ull << 32)) return hipErrorInvalidValue;
+  if (jobs >= (1ull << 32) || segs >= (1ull << 32)) return hipErrorInvalidValue;
   g.h_flags[0] = g.h_flags[1] = 0;
-  hipError_t e = hipMemcpyAsync(g.dev, g.pin, vbytes + 3 * pbytes + 8, hipMemcpyHostToDevice, s);
+  hipError_t e = hipMemcpyAsync(g.dev, g.pin, vbytes + 2 * pbytes + 8, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return e;
   for (Plan *p : g.plans)
     if ((e = plan_enqueue_pre(*p, s)) != hipSuccess) return e;
   const ColView *dv = reinterpret_cast<const ColView *>(g.dev);
   const uint32_t *dpre = reinterpret_cast<const uint32_t *>(g.dev + vbytes);
-  launch::decode(s, dv, (uint32_t)n, dpre, (uint32_t)jobs, dpre + 2 * (pbytes / 4), (uint32_t)segs, g.d_flags,
+  launch::decode(s, dv, (uint32_t)n, dpre, (uint32_t)jobs, dpre + pbytes / 4, (uint32_t)segs, g.d_flags,
                  (uint32_t)cu_count * blocks_per_cu);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   for (Plan *p : g.plans)

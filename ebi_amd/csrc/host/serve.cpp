@@ -93,6 +93,8 @@ struct Ticket {
   uint64_t id = 0;
   Plan plan;
   int status = EBI_PENDING;
+  bool uses_column = false;  // counted in plan.c->users until it completes
+  bool claimed = false;      // a serve_wait call is waiting on it
   Clock::time_point t_submit, t_done;
   uint64_t dev_off = 0, dev_len = 0, pin_off = 0, pin_len = 0;
   Group *group = nullptr;
@@ -140,6 +142,10 @@ namespace {
 
 // Completes `t` (lock held).
 void complete(ebi_hip_server *s, Ticket *t, int status) {
+  if (t->uses_column) {
+    t->plan.c->users.fetch_sub(1);
+    t->uses_column = false;
+  }
   t->status = status;
   t->t_done = Clock::now();
   s->stats.completed++;
@@ -247,12 +253,20 @@ void scheduler(ebi_hip_server *s) {
     bool progressed = false;
     for (size_t i = 0; i < s->groups.size();) {
       Group *q = s->groups[i];
-      if (hipEventQuery(q->done) != hipSuccess) {
+      const hipError_t qe = hipEventQuery(q->done);
+      if (qe == hipErrorNotReady) {
         ++i;
         continue;
       }
-      const uint32_t flags = q->pg.h_flags[0];
+      // any other result than success is a device or runtime error (it is
+      // sticky, so the event would never report completion): the group's
+      // queries fail with it, and the finishers skip their result copies
+      const uint32_t flags = qe == hipSuccess ? q->pg.h_flags[0] : 0u;
       for (Ticket *t : q->tickets) {
+        if (qe != hipSuccess && !t->plan.status) {
+          t->plan.status = EBI_HIP;
+          t->plan.error = hipGetErrorString(qe);
+        }
         if (flags && !t->plan.status) t->plan.status = map_dev_err(flags);
         s->finish_q.push_back(t);
       }
@@ -381,13 +395,20 @@ extern "C" int ebi_hip_serve_submit(ebi_hip_server *s, const ebi_query *q, size_
     }
     t->t_submit = now;
     t->status = plan_init(t->plan, q[i]);  // host-side validation only
-    if (t->status == EBI_OK) t->status = EBI_PENDING;
+    if (t->status == EBI_OK) {
+      t->status = EBI_PENDING;
+      t->plan.c->users.fetch_add(1);  // the column must stay loaded until it completes
+      t->uses_column = true;
+    }
     made.push_back(t);
   }
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) {
-      for (Ticket *m : made) delete m;
+      for (Ticket *m : made) {
+        if (m->uses_column) m->plan.c->users.fetch_sub(1);
+        delete m;
+      }
       return EBI_INVALID_ARGUMENT;
     }
     for (size_t i = 0; i < n; ++i) {
@@ -426,15 +447,20 @@ extern "C" int ebi_hip_serve_wait(ebi_hip_server *s, uint64_t ticket, int64_t ti
   auto it = s->tickets.find(ticket);
   if (it == s->tickets.end()) return EBI_INVALID_ARGUMENT;
   Ticket *t = it->second;
+  // one waiter per ticket: the waiter that returns EBI_OK releases it, so a
+  // second concurrent waiter would be left holding a freed ticket
+  if (t->claimed) return EBI_INVALID_ARGUMENT;
+  t->claimed = true;
   auto done = [&] { return t->status != EBI_PENDING; };
   if (timeout_ms < 0) {
     s->cv_done.wait(lk, done);
   } else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done)) {
+    t->claimed = false;
     return EBI_PENDING;
   }
   if (status) *status = t->status;
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(t->t_done - t->t_submit).count();
-  s->tickets.erase(it);
+  s->tickets.erase(ticket);
   delete t;
   return EBI_OK;
 }
@@ -457,7 +483,10 @@ extern "C" int ebi_hip_serve_destroy(ebi_hip_server *s) {
   s->cv_fin.notify_all();
   for (std::thread &f : s->finishers)
     if (f.joinable()) f.join();
-  for (auto &kv : s->tickets) delete kv.second;
+  for (auto &kv : s->tickets) {
+    if (kv.second->uses_column) kv.second->plan.c->users.fetch_sub(1);
+    delete kv.second;
+  }
   for (hipStream_t st : s->streams) hipStreamDestroy(st);
   for (hipStream_t st : s->copy_streams) hipStreamDestroy(st);
   for (hipEvent_t e : s->free_events) hipEventDestroy(e);

@@ -156,6 +156,7 @@ struct XorState {
   uint64_t v;
   uint32_t a, b;
   int32_t beta;
+  uint32_t end;  // set with kNanEnd: the record index of the NaN end marker
 };
 
 // Decoder state (all but the window) at a checkpoint written by the index
@@ -212,7 +213,10 @@ __host__ __device__ __forceinline__ uint32_t gorilla_run(XorState &s, uint32_t s
     r.refill();
     if (r.pos > size_bits) return kErrOverrun;
     if (n == 0) return v == kNan ? kErrNone : kErrCount;
-    if (v == kNan) return kErrCount;
+    if (v == kNan) {  // the chunk ends at record 0
+      s.end = 0;
+      return kNanEnd;
+    }
     if (kWrite) out.put(0, as_f64(v));
     i = 1;
   }
@@ -236,7 +240,10 @@ __host__ __device__ __forceinline__ uint32_t gorilla_run(XorState &s, uint32_t s
     r.refill();
     if (r.pos > size_bits) return kErrOverrun;
     if (i == n) return v == kNan ? kErrNone : kErrCount;
-    if (v == kNan) return kErrCount;
+    if (v == kNan) {  // the chunk ends at record i
+      s.end = i;
+      return kNanEnd;
+    }
     if (kWrite) out.put(i, as_f64(v));
   }
   s.v = v;
@@ -263,7 +270,10 @@ __host__ __device__ __forceinline__ uint32_t chimp_run(XorState &s, uint32_t siz
     r.refill();
     if (r.pos > size_bits) return kErrOverrun;
     if (n == 0) return v == kNan ? kErrNone : kErrCount;
-    if (v == kNan) return kErrCount;
+    if (v == kNan) {  // the chunk ends at record 0
+      s.end = 0;
+      return kNanEnd;
+    }
     if (kWrite) out.put(0, as_f64(v));
     i = 1;
   }
@@ -291,7 +301,10 @@ __host__ __device__ __forceinline__ uint32_t chimp_run(XorState &s, uint32_t siz
     r.refill();
     if (r.pos > size_bits) return kErrOverrun;
     if (i == n) return v == kNan ? kErrNone : kErrCount;
-    if (v == kNan) return kErrCount;
+    if (v == kNan) {  // the chunk ends at record i
+      s.end = i;
+      return kNanEnd;
+    }
     if (kWrite) out.put(i, as_f64(v));
   }
   s.v = v;
@@ -373,7 +386,10 @@ __host__ __device__ __forceinline__ uint32_t elf_run(XorState &s, uint32_t size_
     r.refill();
     if (r.pos > size_bits) return kErrOverrun;
     if (i == n) return term ? kErrNone : kErrCount;
-    if (term) return kErrCount;
+    if (term) {  // the chunk ends at record i
+      s.end = i;
+      return kNanEnd;
+    }
     if (kWrite) {
       const double v = as_f64(stored);
       out.put(i, recover ? elf_recover(v, beta) : v);
@@ -725,9 +741,10 @@ __device__ __forceinline__ void stage_flush(const double *stage, double *dst, ui
 // ----------------------------------------------------------------- Chimp128
 // decoder/chunk_reader/chimp_n.rs:173-285.  `ring` is this lane's 128-slot
 // column in LDS, stride kWave (slot s of lane l at ring[s * 64]).
+// A NaN before record n ends the chunk there: kNanEnd, its index in *end.
 template <bool kWrite>
 __device__ uint32_t chimp128_run(const uint8_t *data, uint32_t size_bits, uint32_t n, double *out,
-                                 uint64_t *ring) {
+                                 uint64_t *ring, uint32_t *end) {
   StreamReader r;
   r.init(data, 0);
   for (int s = 0; s < 128; ++s) ring[s * kWave] = 0;
@@ -771,7 +788,10 @@ __device__ uint32_t chimp128_run(const uint8_t *data, uint32_t size_bits, uint32
     }
     if (r.pos > size_bits) return kErrOverrun;
     if (i == n) return term ? kErrNone : kErrCount;
-    if (term) return kErrCount;
+    if (term) {
+      *end = i;
+      return kNanEnd;
+    }
     if (kWrite) out[i] = as_f64(v);
   }
   return kErrNone;

@@ -28,6 +28,25 @@ __device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
+// State of a lane with no segment in a lockstep wave: every field defined
+// (zeros), position 0, so refill() and field() compute on known values.
+__device__ __forceinline__ void dead_window(BitWindow &r) {
+  r.p = nullptr;
+  r.c0 = r.c1 = r.n0 = r.n1 = 0;
+  r.nxt = make_uint4(0, 0, 0, 0);
+  r.b = 0;
+  r.stride = 0;
+  r.hi = r.lo = 0;
+  r.pos = 0;
+}
+__device__ __forceinline__ void dead_state(XorState &s) {
+  dead_window(s.r);
+  s.v = 0;
+  s.a = s.b = 0;
+  s.beta = 0;
+  s.end = 0;
+}
+
 // Checkpoint segments first .. first + 63 of chunk c (Gorilla, Chimp or
 // Elf), segment first + l on lane l, decoded in lockstep runs of kRun
 // records from the checkpoints into `out`: out.row(i, k, v, act) takes each
@@ -47,16 +66,10 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
     xor_state_set(s, kCodec, k);
   }
   const uint32_t bits = c.size * 8u;
-  // a lane without a segment has an empty window that it never advances
-  // (it takes part in no record)
-  if (!live) {
-    s.r.b = 0;
-    s.r.pos = 0;
-    s.r.hi = s.r.lo = 0;
-    s.v = 0;
-    s.a = s.b = 0;
-    s.beta = 0;
-  }
+  // a lane without a segment has an empty, fully defined window that it
+  // never advances (it takes part in no record, but the lockstep loop still
+  // runs refill() and the field extraction on it)
+  if (!live) dead_state(s);
   const uint32_t last = live && i1 == c.n ? c.n + 1 : i1;  // the chunk's last segment parses the terminator
   uint32_t from = i0, e = 0;
   if (live && i0 == 0) {
@@ -120,10 +133,8 @@ __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &
     const XorCkpt k = reinterpret_cast<const XorCkpt *>(c.xidx)[first + lane];
     r.init(reinterpret_cast<const uint8_t *>(c.data), k.bitpos);
     lead = k.a == 255 ? 0xffffffffu : k.a;
-  } else {  // an empty window that is never advanced
-    r.b = 0;
-    r.pos = 0;
-    r.hi = r.lo = 0;
+  } else {  // an empty, fully defined window that is never advanced
+    dead_window(r);
   }
   const StageSink sink{stage, dists};
   const uint32_t bits = c.size * 8u;
@@ -536,6 +547,11 @@ __global__ void __launch_bounds__(64) k_serial(const ColView *cols, const uint32
     XorState s;
     xor_state_start(s, c.codec, data);
     e = xor_run<kWrite>(c.codec, s, bits, c.n, 0, c.n, out, ck);
+    if (e == kNanEnd) {
+      // the chunk ends at its first NaN: the host shortens it to s.end records
+      if (cv.chunk_end) cv.chunk_end[ci] = s.end;
+      e = kErrNone;
+    }
   }
   if (e) {
     atomicOr(err, e);
@@ -562,8 +578,13 @@ __global__ void __launch_bounds__(64) k_chimp128(const ColView *cols, const uint
   const uint32_t ci = jb.chunk + lane;
   const DevChunk c = cv.chunks[ci];
   if (kWrite && cv.chunk_err && cv.chunk_err[ci]) return;
-  const uint32_t e = chimp128_run<kWrite>(reinterpret_cast<const uint8_t *>(c.data), c.size * 8u, c.n,
-                                          cv.out + c.rec_off, ring + lane);
+  uint32_t end = 0;
+  uint32_t e = chimp128_run<kWrite>(reinterpret_cast<const uint8_t *>(c.data), c.size * 8u, c.n,
+                                    cv.out + c.rec_off, ring + lane, &end);
+  if (e == kNanEnd) {  // the chunk ends at its first NaN (see k_serial)
+    if (cv.chunk_end) cv.chunk_end[ci] = end;
+    e = kErrNone;
+  }
   if (e) {
     atomicOr(err, e);
     if (cv.chunk_err) atomicOr(cv.chunk_err + ci, e);
