@@ -169,6 +169,7 @@ _SIGS = {
     "ebi_hip_min": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
     "ebi_hip_max": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double)]),
     "ebi_hip_query": (C.c_int, [_P, C.POINTER(_Query)]),
+    "ebi_hip_query_noalloc": (C.c_int, [_P, C.POINTER(_Query)]),
     "ebi_hip_batch_create": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_int, C.POINTER(_P)]),
     "ebi_hip_batch_run": (C.c_int, [_P, _P, C.c_int]),
     "ebi_hip_batch_config_get": (C.c_int, [_P, C.POINTER(_BatchCfg)]),
@@ -664,9 +665,11 @@ class Query:
         return float(self._scalar.value)
 
 
-def run_query(ctx: "Context", q: Query):
-    """ebi_hip_query: one query synchronously; returns its result."""
-    _check(lib().ebi_hip_query(ctx._h, C.byref(q.c)), ctx, "query")
+def run_query(ctx: "Context", q: Query, noalloc: bool = False):
+    """ebi_hip_query: one query synchronously; returns its result.  With
+    noalloc, ebi_hip_query_noalloc (fixed per-GPU scratch, no allocation)."""
+    fn = lib().ebi_hip_query_noalloc if noalloc else lib().ebi_hip_query
+    _check(fn(ctx._h, C.byref(q.c)), ctx, "query")
     return q.result()
 
 

@@ -592,7 +592,7 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
     std::snprintf(buf, sizeof buf, "1.0E-%d", i);
     n10[i] = std::strtod(buf, nullptr);
   }
-  if (launch::set_pow10(p10, n10) != hipSuccess) {
+  if (launch::set_pow10(p10, n10) != hipSuccess || noalloc_init(c) != EBI_OK) {
     ebi_hip_ctx_destroy(c);
     return EBI_HIP;
   }

@@ -109,4 +109,5 @@ struct ebi_hip_ctx {
   uint32_t blocks_per_cu = 8;   // persistent k_decode grid = cu_count * blocks_per_cu
   uint32_t batch_mixed = 0;     // untuned prepared batches: 1 = one k_decode_mix launch
   int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks (Gorilla, Chimp, Elf)
+  uint8_t *noalloc_dev = nullptr;  // this device's scratch of ebi_hip_query_noalloc
 };

@@ -34,5 +34,7 @@ int sync_and_check(ebi_hip_ctx *ctx, const char *what);
 int check_chunk_errors(ebi_hip_ctx *ctx, const Column &c, int64_t chunk_id);
 // Queues the kernels that decode every chunk of `c` into the view's output.
 int enqueue_decode(ebi_hip_ctx *ctx, Column &c);
+// Looks up the ctx device's scratch of ebi_hip_query_noalloc (device current).
+int noalloc_init(ebi_hip_ctx *ctx);
 
 }  // namespace ebi

@@ -151,8 +151,8 @@ PinLayout pin_layout_of(const Plan &p) {
     off += round_up(std::max<uint64_t>(bytes, 8), 64);
     return o;
   };
-  L.mask = p.has_mask ? take(p.words * 8) : ~0ull;
-  L.bits = p.q.op == kOpFilter ? take(p.words * 8) : ~0ull;
+  L.mask = p.has_mask && !p.direct ? take(p.words * 8) : ~0ull;
+  L.bits = p.q.op == kOpFilter && !p.direct ? take(p.words * 8) : ~0ull;
   // the per-chunk results at the device layout's spacing (kAlign)
   off = round_up(off, kAlign);
   L.sel = off;
@@ -368,8 +368,8 @@ void plan_bind(Plan &p, uint8_t *dev, uint8_t *pin) {
   p.values = p.q.op == kOpDecode ? p.q.out_values : at<double>(dev, L.values);
   p.out = at<double>(dev, L.out);
   const PinLayout P = pin_layout_of(p);
-  p.h_mask = at<uint64_t>(pin, P.mask);
-  p.h_bits = at<uint64_t>(pin, P.bits);
+  p.h_mask = p.direct ? const_cast<uint64_t *>(p.q.bitset) : at<uint64_t>(pin, P.mask);
+  p.h_bits = p.direct ? p.q.out_bitset : at<uint64_t>(pin, P.bits);
   p.h_sel = at<uint32_t>(pin, P.sel);
   p.h_acc = at<uint64_t>(pin, P.acc);
   p.h_csum = at<double>(pin, P.csum);
@@ -378,7 +378,7 @@ void plan_bind(Plan &p, uint8_t *dev, uint8_t *pin) {
 }
 
 void plan_stage(Plan &p) {
-  if (p.has_mask && p.words) std::memcpy(p.h_mask, p.q.bitset, p.words * 8);
+  if (p.has_mask && p.words && !p.direct) std::memcpy(p.h_mask, p.q.bitset, p.words * 8);
   if (p.cpred) {
     const bool buff = p.c->meta.scheme == kBUFF;
     for (uint32_t k = 0; k < p.nc; ++k) {
@@ -427,7 +427,7 @@ int plan_finish(Plan &p) {
     return EBI_OK;
   }
   if (op == kOpFilter) {
-    if (p.words) std::memcpy(p.q.out_bitset, p.h_bits, p.words * 8);
+    if (p.words && !p.direct) std::memcpy(p.q.out_bitset, p.h_bits, p.words * 8);
     return EBI_OK;
   }
   if (op == kOpDecode) return EBI_OK;

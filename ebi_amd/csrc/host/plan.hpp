@@ -54,6 +54,8 @@ struct Plan {
   bool insitu = false;     // BUFF / DeltaSprintz operate on codes
   bool decode = false;     // needs the column's records decoded into `values`
   bool has_mask = false;
+  bool direct = false;     // the bitmask and the filter result move between the caller's host
+                           // buffers and the device directly (no staging of them)
   bool compact = false;    // materialize through the bitset and compaction (a bitmask, or a
                            // column whose chunks end at NaNs: its records are not contiguous)
   bool cpred = false;      // per-chunk predicate parameters are uploaded

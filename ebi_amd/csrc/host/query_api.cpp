@@ -8,7 +8,9 @@
 // DeltaSprintz) or decode into HBM and run the record-tile kernels (other
 // codecs), then copy the result back.  Per-chunk partial results are
 // combined on the host in chunk order, as the Decoder loop does.
+#include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "../../../include/ebi_hip.h"
 #include "column.hpp"
@@ -16,6 +18,34 @@
 #include "plan.hpp"
 
 namespace ebi {
+
+// Scratch of ebi_hip_query_noalloc: one array per GPU for the life of the
+// process.  Host code never uses its name as an address (it names a
+// host-side stand-in): noalloc_init looks up this GPU's copy once, with the
+// device current, and the calls pass that address to the kernels.
+constexpr uint64_t kNoAllocDeviceBytes = 64ull << 20;
+constexpr uint64_t kNoAllocHostBytes = 4ull << 20;
+constexpr int kNoAllocMaxDevices = 16;
+static __device__ __align__(256) uint8_t scratch[kNoAllocDeviceBytes];
+// host staging of the same calls (per-chunk results, chunk predicates, the
+// launch's views and flags), one region per device
+alignas(256) static uint8_t g_noalloc_host[kNoAllocMaxDevices][kNoAllocHostBytes];
+static uint8_t *g_noalloc_dev[kNoAllocMaxDevices];
+static std::mutex g_noalloc_mu;
+
+int noalloc_init(ebi_hip_ctx *ctx) {
+  if (ctx->device < 0 || ctx->device >= kNoAllocMaxDevices) return EBI_OK;  // the entry point refuses
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_dev[ctx->device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));  // ctx->device is current
+    if (e != hipSuccess || !ptr) return hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "scratch address");
+    g_noalloc_dev[ctx->device] = static_cast<uint8_t *>(ptr);
+  }
+  ctx->noalloc_dev = g_noalloc_dev[ctx->device];
+  return EBI_OK;
+}
+
 namespace {
 
 int ensure_pinned(ebi_hip_ctx *ctx, size_t bytes) {
@@ -64,6 +94,53 @@ int run_query(ebi_hip_ctx *ctx, const ebi_query &q) {
   return plan_finish(p);
 }
 
+// ebi_hip_query without allocations: the plan's device scratch is this
+// GPU's `scratch` array, its host staging the device's static region, and
+// the bitmask / filter result move between the caller's buffers and the
+// device directly.
+int run_query_noalloc(ebi_hip_ctx *ctx, const ebi_query &q) {
+  if (!ctx->noalloc_dev) return fail(ctx, EBI_UNSUPPORTED, "query_noalloc: no scratch on this device");
+  Plan p;
+  int rc = plan_init(p, q);
+  p.direct = true;
+  if (rc) return fail(ctx, rc, p.error.empty() ? "invalid query arguments" : p.error);
+  const PlanNeed need = plan_need(p);
+  const uint64_t gb = group_block_bytes(1);
+  if (need.dev + gb > kNoAllocDeviceBytes || need.pin + gb > kNoAllocHostBytes) {
+    char buf[200];
+    std::snprintf(buf, sizeof buf,
+                  "query_noalloc: the query needs %llu device and %llu host scratch bytes, above the fixed "
+                  "%llu / %llu",
+                  (unsigned long long)(need.dev + gb), (unsigned long long)(need.pin + gb),
+                  (unsigned long long)kNoAllocDeviceBytes, (unsigned long long)kNoAllocHostBytes);
+    return fail(ctx, EBI_INVALID_ARGUMENT, buf);
+  }
+  PlanGroup g;
+  g.plans.push_back(&p);
+  uint8_t *dev = ctx->noalloc_dev, *pin = g_noalloc_host[ctx->device];
+  group_bind(g, dev, pin);
+  plan_bind(p, dev + gb, pin + gb);
+  plan_stage(p);
+  hipEventRecord(ctx->ev0, ctx->stream);
+  hipError_t e = group_enqueue(g, ctx->stream, ctx->cu_count, ctx->blocks_per_cu);
+  if (e != hipSuccess) return hip_fail(ctx, e, "query_noalloc launch");
+  hipEventRecord(ctx->ev1, ctx->stream);
+  if ((e = stream_wait(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "query_noalloc");
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  ctx->last_ms = ms;
+  if (g.h_flags[0]) {
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "query_noalloc: device reported chunk error flags 0x%x", g.h_flags[0]);
+    return fail(ctx, map_dev_err(g.h_flags[0]), buf);
+  }
+  if (plan_enqueue_values(p, ctx->stream, &e)) {
+    if (e == hipSuccess) e = stream_wait(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "query_noalloc result copy");
+  }
+  return plan_finish(p);
+}
+
 ebi_query make_query(int op, const ebi_hip_col *col, const ebi_pred *pred, const uint64_t *bitset,
                      int64_t chunk_id) {
   ebi_query q{};
@@ -84,6 +161,12 @@ extern "C" int ebi_hip_query(ebi_hip_ctx *ctx, const ebi_query *q) {
   EBI_GUARD(ctx);
   if (!q) return EBI_INVALID_ARGUMENT;
   return run_query(ctx, *q);
+}
+
+extern "C" int ebi_hip_query_noalloc(ebi_hip_ctx *ctx, const ebi_query *q) {
+  EBI_GUARD(ctx);
+  if (!q) return EBI_INVALID_ARGUMENT;
+  return run_query_noalloc(ctx, *q);
 }
 
 extern "C" int ebi_hip_sum(ebi_hip_ctx *ctx, const ebi_hip_col *col, const uint64_t *bitset, int64_t chunk_id,

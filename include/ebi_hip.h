@@ -201,6 +201,23 @@ typedef struct ebi_query {
  * ... ebi_hip_max entry points are this call with the matching op). */
 int ebi_hip_query(ebi_hip_ctx *ctx, const ebi_query *q);
 
+/* ebi_hip_query without allocations: no counterpart in the reference; for
+ * callers that cannot allocate device or pinned memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * query's device scratch is a fixed array of 64 MiB per GPU, held from the
+ * first context on that GPU until the process ends, and its host staging a
+ * fixed 4 MiB region per device (devices 0-15); the bitmask and the FILTER
+ * result move between the caller's buffers and the device directly.  A query
+ * that needs more scratch (about: 2 x ceil(N/64) x 8 bytes of bitsets, 16
+ * bytes per 512 records of partial sums, ~150 bytes per chunk, and for
+ * MATERIALIZE / FILTER_MATERIALIZE of codecs decoded to HBM 8 or 16 bytes
+ * per record) fails with EBI_INVALID_ARGUMENT and a message naming both
+ * sizes.  Every call on one GPU shares its scratch (each call clears what it
+ * needs), so calls that overlap on one GPU -- from two contexts on the same
+ * device -- race; calls on one ctx are serialised as for every entry point.
+ * Results equal ebi_hip_query's. */
+int ebi_hip_query_noalloc(ebi_hip_ctx *ctx, const ebi_query *q);
+
 /* Device-resident decode: writes all N records of the column to the device
  * buffer d_out (N doubles) on the ctx stream and waits for completion.
  * This is the whole-column decode the batched and DGEMM paths build on. */
