@@ -155,6 +155,7 @@ _SIGS = {
     "ebi_hip_column_free": (C.c_int, [_P, _P]),
     "ebi_hip_column_info": (C.c_int, [_P, C.POINTER(_FileInfo)]),
     "ebi_hip_column_compressed_bytes": (C.c_uint64, [_P]),
+    "ebi_hip_column_set_precision": (C.c_int, [_P, C.c_int32]),
     "ebi_hip_decode_device": (C.c_int, [_P, _P, _P]),
     "ebi_hip_decode_per_chunk": (C.c_int, [_P, _P, _P]),
     "ebi_hip_decode_batch": (C.c_int, [_P, _P, C.c_size_t, _P]),
@@ -269,6 +270,10 @@ class CompressorConfig:
     @staticmethod
     def elf(capacity: int = 8192):
         return CompressorConfig(7, 1, capacity)
+
+    @staticmethod
+    def elf_on_chimp(capacity: int = 8192):
+        return CompressorConfig(6, 1, capacity)
 
     @staticmethod
     def delta_sprintz(scale: int = 1, capacity: int = 8192):
@@ -531,6 +536,12 @@ class Decoder:
     @property
     def compressed_bytes(self) -> int:
         return int(lib().ebi_hip_column_compressed_bytes(self._h))
+
+    def with_precision(self, precision: int) -> None:
+        """Decoder::with_precision (api/decoder.rs:159-161): BUFF decodes keep
+        `precision` decimal digits (controlled_precision.rs:5-75); -1 restores
+        full precision."""
+        _check(lib().ebi_hip_column_set_precision(self._h, int(precision)), self.ctx, "with_precision")
 
     def _mask(self, bitmask):
         if bitmask is None:

@@ -42,6 +42,7 @@ enum Codec : uint32_t {
   kCodecGorilla = 3,
   kCodecChimp = 4,
   kCodecChimp128 = 5,
+  kCodecElfOnChimp = 6,
   kCodecElf = 7,
   kCodecSprintz = 8,
 };
@@ -171,7 +172,10 @@ struct alignas(16) ColView {
   // the load-time passes only when that is below the footer's record count
   // (the reference stops a chunk at its first NaN, gorilla.rs:149-152)
   uint32_t *chunk_end;
-  uint64_t pad1;
+  // BUFF decode precision of this view (Decoder::with_precision,
+  // api/decoder.rs:159-161): -1 full, else 0..12 (buff_tile)
+  int32_t precision;
+  uint32_t pad1;
   const uint4 *ilv;  // lane-interleaved segment streams (Gorilla, Chimp, Elf; null before they are built)
 };
 static_assert(sizeof(ColView) == 112, "ColView layout");

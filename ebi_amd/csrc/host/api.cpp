@@ -68,7 +68,7 @@ static uint32_t be32(const uint8_t *p) {
 }
 
 bool is_xor_codec(int scheme) {
-  return scheme == kGorilla || scheme == kChimp || scheme == kElf;
+  return scheme == kGorilla || scheme == kChimp || scheme == kElf || scheme == kElfOnChimp;
 }
 
 ColView view_of(const Column &c, double *out) {
@@ -88,6 +88,7 @@ ColView view_of(const Column &c, double *out) {
   v.ilv = c.d_ilv;
   v.c128_ring = c.d_c128_ring;
   v.chunk_end = c.d_chunk_end;
+  v.precision = c.precision;
   return v;
 }
 
@@ -288,7 +289,8 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   if (rc) return fail(ctx, rc, err);
   const int scheme = col.meta.scheme;
   if (!(scheme == kUncompressed || scheme == kRLE || scheme == kBUFF || scheme == kGorilla ||
-        scheme == kChimp || scheme == kChimp128 || scheme == kElf || scheme == kDeltaSprintz))
+        scheme == kChimp || scheme == kChimp128 || scheme == kElfOnChimp || scheme == kElf ||
+        scheme == kDeltaSprintz))
     return fail(ctx, EBI_UNSUPPORTED, "compression scheme outside the accelerated set");
   col.xor_codec = is_xor_codec(scheme);
   col.chimp128 = scheme == kChimp128;
@@ -697,6 +699,12 @@ extern "C" int ebi_hip_column_info(const ebi_hip_col *col, ebi_file_info *info) 
   info->chunk_option_kind = m.chunk_kind;
   for (int i = 0; i < 3; ++i) info->version[i] = m.version[i];
   info->reserved = 0;
+  return EBI_OK;
+}
+
+extern "C" int ebi_hip_column_set_precision(ebi_hip_col *col, int32_t precision) {
+  if (!col || precision < -1 || precision > 12) return EBI_INVALID_ARGUMENT;
+  col->c.precision = precision;
   return EBI_OK;
 }
 

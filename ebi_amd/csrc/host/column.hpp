@@ -70,6 +70,9 @@ struct Column {
   // is shortened to the records before the NaN): the column's records are no
   // longer contiguous, so a whole-column materialize compacts them
   bool truncated = false;
+  // Decoder::with_precision (api/decoder.rs:159-161): -1 full, else 0..12;
+  // BUFF decodes read only the planes that precision needs
+  int32_t precision = -1;
   // plans and prepared batches that use the column (column_free refuses
   // while this is nonzero)
   std::atomic<uint32_t> users{0};

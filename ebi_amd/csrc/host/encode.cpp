@@ -133,13 +133,16 @@ std::vector<uint8_t> enc_gorilla(const double *v, size_t n) {
 }
 
 // ------------------------------------------------------------------ Chimp
-// compressor/chimp.rs:99-146.
-std::vector<uint8_t> enc_chimp(const double *v, size_t n) {
-  MsbWriter w;
+// compressor/chimp.rs:99-146: the first value as 64 raw bits, then a 2-bit
+// flag per value (00 same, 01 lead code + sig + payload, 10 payload with the
+// stored lead, 11 lead code + payload).
+struct ChimpEnc {
   uint64_t stored = 0;
   uint32_t stored_lead = UINT32_MAX;
-  auto add = [&](uint64_t b, bool first) {
+  bool first = true;
+  void add(MsbWriter &w, uint64_t b) {
     if (first) {
+      first = false;
       stored = b;
       w.put(b, 64);
       return;
@@ -169,9 +172,14 @@ std::vector<uint8_t> enc_chimp(const double *v, size_t n) {
       }
     }
     stored = b;
-  };
-  for (size_t i = 0; i < n; ++i) add(f64_bits(v[i]), i == 0);
-  add(kNanBits, n == 0);
+  }
+};
+
+std::vector<uint8_t> enc_chimp(const double *v, size_t n) {
+  MsbWriter w;
+  ChimpEnc c;
+  for (size_t i = 0; i < n; ++i) c.add(w, f64_bits(v[i]));
+  c.add(w, kNanBits);  // close: the NaN end marker, then a 0 bit
   w.bit(false);
   return w.finish();
 }
@@ -372,10 +380,13 @@ struct ElfXorEnc {
 };
 
 // Elf wrapper, compressor/elf.rs:91-160: erase low mantissa bits when the
-// decimal significand allows it and tag the value with a beta* prefix.
-std::vector<uint8_t> enc_elf(const double *v, size_t n) {
+// decimal significand allows it and tag the value with a beta* prefix; the
+// erased values go to an XOR encoder, ElfXor (Elf) or Chimp (ElfOnChimp,
+// compressor/elf.rs:36), which is closed after a case-10 prefix.
+template <class Xor>
+std::vector<uint8_t> enc_elf_over(const double *v, size_t n) {
   MsbWriter w;
-  ElfXorEnc x;
+  Xor x;
   int32_t last_beta_star = INT32_MAX;
   for (size_t k = 0; k < n; ++k) {
     double d = v[k];
@@ -420,12 +431,14 @@ std::vector<uint8_t> enc_elf(const double *v, size_t n) {
     }
     x.add(w, vp);
   }
-  // close: case 10 prefix, then the ElfXor terminator (NaN and a 0 bit).
+  // close: case 10 prefix, then the XOR encoder's terminator (NaN and a 0 bit).
   w.put(2, 2);
   x.add(w, kNanBits);
   w.bit(false);
   return w.finish();
 }
+std::vector<uint8_t> enc_elf(const double *v, size_t n) { return enc_elf_over<ElfXorEnc>(v, n); }
+std::vector<uint8_t> enc_elf_on_chimp(const double *v, size_t n) { return enc_elf_over<ChimpEnc>(v, n); }
 
 // ------------------------------------------------------------------- BUFF
 const uint64_t kPrecisionMap[13] = {0, 5, 8, 11, 15, 18, 21, 25, 28, 31, 35, 38, 50};
@@ -560,6 +573,7 @@ int encode_chunk(int scheme, const double *v, size_t n, const ebi_encoder_config
     case kChimp: *out = enc_chimp(v, n); return 0;
     case kChimp128: *out = enc_chimp128(v, n); return 0;
     case kElf: *out = enc_elf(v, n); return 0;
+    case kElfOnChimp: *out = enc_elf_on_chimp(v, n); return 0;
     case kBUFF: *out = enc_buff(v, n, buff_frac); return 0;
     case kDeltaSprintz: *out = enc_sprintz(v, n, cfg.scale); return 0;
     default: return EBI_UNSUPPORTED;
@@ -650,7 +664,7 @@ extern "C" int ebi_encode(const double *values, size_t n, const ebi_encoder_conf
   *out = nullptr;
   *out_len = 0;
   int scheme = cfg->scheme;
-  if (scheme < 0 || scheme > kDeltaSprintz || scheme == kElfOnChimp) return EBI_UNSUPPORTED;
+  if (scheme < 0 || scheme > kDeltaSprintz) return EBI_UNSUPPORTED;
   uint32_t frac = 0;
   if (scheme == kBUFF && buff_frac_bits(cfg->scale, &frac) != 0) return EBI_INVALID_ARGUMENT;
 

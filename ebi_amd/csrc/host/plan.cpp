@@ -339,7 +339,8 @@ int plan_init(Plan &p, const ebi_query &q) {
   }
   // Gorilla, Chimp, Elf and Chimp128 filters and aggregates run fused: the
   // segment decode feeds the query in registers
-  const bool seg_codec = scheme == kGorilla || scheme == kChimp || scheme == kElf || scheme == kChimp128;
+  const bool seg_codec = scheme == kGorilla || scheme == kChimp || scheme == kElf || scheme == kElfOnChimp ||
+                         scheme == kChimp128;
   if (seg_codec && p.c->n_seg_jobs && (q.op == kOpFilter || q.op == kOpFilterSum || is_agg(q.op))) {
     p.fused = true;
     p.decode = false;

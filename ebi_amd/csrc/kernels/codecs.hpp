@@ -402,6 +402,71 @@ __host__ __device__ __forceinline__ uint32_t elf_run(XorState &s, uint32_t size_
   return kErrNone;
 }
 
+// -------------------------------------------------------------- ElfOnChimp
+// ElfDecoderWrapper over ChimpDecoder (decoder/chunk_reader/elf.rs:27-93
+// with chimp.rs:59-125): per record the Elf prefix (0: erased, reuse beta*;
+// 10: not erased; 11 + 4 bits: erased, new beta*), then one Chimp value
+// (record 0: 64 raw bits; after it a 2-bit flag: 00 same, 01 lead code + sig
+// + payload, 10 payload with the stored lead, 11 lead code + payload).  The
+// chunk ends at the first Chimp value equal to NaN.  State: v = Chimp's
+// stored value (the erased bits v'), a = stored lead (0xffffffff unset),
+// beta = last beta* (INT32_MAX unset).
+template <bool kWrite, class Sink>
+__host__ __device__ __forceinline__ uint32_t elfchimp_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+                                                 uint32_t i1, const Sink &out, XorCkpt *ck) {
+  BitWindow &r = s.r;
+  uint64_t v = s.v;
+  uint32_t lead = s.a;
+  int32_t beta = s.beta;
+  const uint32_t last = i1 == n ? n + 1 : i1;
+  for (uint32_t i = i0; i < last; ++i) {
+    if (ck && i < n && (i % kSeg) == 0)
+      ck[i / kSeg] = XorCkpt{v, r.pos, (uint8_t)(lead > 64 ? 255 : lead), 0,
+                             (int8_t)(beta == INT32_MAX ? -1 : beta), 0};
+    const uint32_t p2 = (uint32_t)(r.hi >> 62);
+    const uint32_t pre = (p2 & 2) ? (p2 == 2 ? 2u : 6u) : 1u;
+    const bool recover = p2 != 2;
+    if (p2 == 3) beta = (int32_t)((r.hi >> 58) & 15);
+    uint32_t used;
+    if (i == 0) {
+      v = r.field(pre, 64);
+      used = pre + 64;
+    } else {
+      const uint32_t f = (uint32_t)r.field(pre, 2);
+      const uint32_t h9 = (uint32_t)r.field(pre + 2, 9);
+      const uint32_t nl = (f == 3 || f == 1) ? lead_repr(h9 >> 6) : lead;
+      if (f == 2 && lead > 64) return kErrCorrupt;
+      uint32_t sig = 64 - nl;
+      if (f == 1) {
+        sig = (h9 & 63) ? (h9 & 63) : 64;
+        if (sig + nl > 64) return kErrCorrupt;
+      }
+      const uint32_t o = pre + (f == 3 ? 5u : f == 1 ? 11u : 2u);
+      if (f) {
+        v ^= r.field(o, sig) << (f == 1 ? 64 - sig - nl : 0);
+        lead = nl;
+      }
+      used = f ? o + sig : pre + 2;
+    }
+    r.consume(used);
+    r.refill();
+    if (r.pos > size_bits) return kErrOverrun;
+    if (i == n) return v == kNan ? kErrNone : kErrCount;
+    if (v == kNan) {  // the chunk ends at record i
+      s.end = i;
+      return kNanEnd;
+    }
+    if (kWrite) {
+      const double d = as_f64(v);
+      out.put(i, recover ? elf_recover(d, beta) : d);
+    }
+  }
+  s.v = v;
+  s.a = lead;
+  s.beta = beta;
+  return kErrNone;
+}
+
 // Dispatch on the codec (wave-uniform).
 template <bool kWrite, class Sink>
 __host__ __device__ __forceinline__ uint32_t xor_run(uint32_t codec, XorState &s, uint32_t size_bits, uint32_t n,
@@ -410,6 +475,7 @@ __host__ __device__ __forceinline__ uint32_t xor_run(uint32_t codec, XorState &s
     case kCodecGorilla: return gorilla_run<kWrite>(s, size_bits, n, i0, i1, out, ck);
     case kCodecChimp: return chimp_run<kWrite>(s, size_bits, n, i0, i1, out, ck);
     case kCodecElf: return elf_run<kWrite>(s, size_bits, n, i0, i1, out, ck);
+    case kCodecElfOnChimp: return elfchimp_run<kWrite>(s, size_bits, n, i0, i1, out, ck);
     default: return kErrCorrupt;
   }
 }
@@ -604,6 +670,69 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
   s.v = stored;
   s.a = L;
   s.b = T;
+  s.beta = beta;
+  return e;
+}
+
+// Record 0 of an ElfOnChimp chunk: prefix, then Chimp's 64 raw bits.
+template <class Sink>
+__host__ __device__ __forceinline__ uint32_t elfchimp_seg_first(XorState &s, uint32_t size_bits, Sink &out,
+                                                       const double *tab) {
+  BitWindow &r = s.r;
+  const uint32_t p2 = (uint32_t)(r.hi >> 62);
+  const uint32_t pre = (p2 & 2) ? (p2 == 2 ? 2u : 6u) : 1u;
+  if (p2 == 3) s.beta = (int32_t)((r.hi >> 58) & 15);
+  s.v = r.field(pre, 64);
+  r.consume(pre + 64);
+  r.refill();
+  const double v = as_f64(s.v);
+  out.row(0, 0, p2 != 2 ? elf_recover_tab(v, s.beta, tab) : v, true);
+  if (r.pos > size_bits) return kErrOverrun;
+  return s.v == kNan ? kErrCount : kErrNone;  // n >= 1: record 0 is not the end marker
+}
+
+// ElfOnChimp, the record step of elfchimp_run for records i >= 1 in the
+// lockstep form: the Elf prefix, then chimp_seg's step at offset `pre`
+// (the 2-bit flag and the 9-bit header lie in the window's first 17 bits).
+template <class Sink>
+__host__ __device__ __forceinline__ uint32_t elfchimp_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
+                                                 uint32_t cnt, uint32_t from, uint32_t lim, Sink &out,
+                                                 uint32_t e, const double *tab) {
+  BitWindow &r = s.r;
+  uint64_t v = s.v;
+  uint32_t lead = s.a;
+  int32_t beta = s.beta;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t i = a + k;
+    const bool act = e == 0 && i >= from && i < lim;
+    const uint32_t h32 = (uint32_t)(r.hi >> 32);
+    const uint32_t p2 = h32 >> 30;
+    const uint32_t pre = p2 == 3 ? 6u : p2 == 2 ? 2u : 1u;
+    const int32_t nbeta = p2 == 3 ? (int32_t)((h32 >> 26) & 15) : beta;
+    const uint32_t f = (h32 >> (30 - pre)) & 3;
+    const uint32_t h9 = (h32 >> (21 - pre)) & 0x1ff;
+    const uint32_t nl = (f & 1) ? lead_repr(h9 >> 6) : lead;  // flags 11 and 01 carry a lead code
+    const uint32_t s6 = h9 & 63;
+    const uint32_t sig = f == 1 ? (s6 ? s6 : 64u) : 64 - nl;
+    const bool bad = (f == 2 && lead > 64) || (f == 1 && sig + nl > 64);
+    const uint32_t o = pre + (f == 3 ? 5u : f == 1 ? 11u : 2u);
+    const uint64_t x = r.field(o, sig) << (f == 1 ? (64 - sig - nl) & 63 : 0u);
+    const bool upd = act && !bad;
+    v ^= upd && f != 0 ? x : 0;
+    lead = upd && f != 0 ? nl : lead;
+    beta = upd ? nbeta : beta;
+    r.consume(upd ? (f ? o + sig : pre + 2) : 0u);
+    r.refill();
+    const uint32_t code = bad                      ? (uint32_t)kErrCorrupt
+                          : r.pos > size_bits      ? (uint32_t)kErrOverrun
+                          : (i == n) != (v == kNan) ? (uint32_t)kErrCount
+                                                   : 0u;
+    e = act ? code : e;
+    const double d = as_f64(v);
+    out.row(i, k, p2 != 2 ? elf_recover_tab(d, beta, tab) : d, act);
+  }
+  s.v = v;
+  s.a = lead;
   s.beta = beta;
   return e;
 }
@@ -884,16 +1013,28 @@ __device__ __forceinline__ void plain_tile(const DevChunk &c, uint32_t t, double
 // BUFF deltas of this lane's 8 records: planes MSB-first (each byte XOR 0x80),
 // then the LSB-first tail of r bits (compression_common/buff.rs:85-287,
 // chunk_reader/buff/decode.rs:5-70).  Lanes whose records lie past n get
-// garbage-free zeros.
-__device__ __forceinline__ void buff_deltas(const DevChunk &c, uint32_t i0, uint64_t d[8]) {
+// garbage-free zeros.  Controlled precision reads only the first np planes
+// and the tail if `tail` (the bits not read are 0).
+__device__ __forceinline__ void buff_deltas(const DevChunk &c, uint32_t i0, uint64_t d[8], uint32_t np, bool tail) {
 #pragma unroll
   for (int j = 0; j < 8; ++j) d[j] = 0;
   if (i0 >= c.n) return;
   const EBI_GLOBAL uint8_t *base = as_global(reinterpret_cast<const uint8_t *>(c.data));
-  for (uint32_t p = 0; p < c.n_planes; ++p) {
+  for (uint32_t p = 0; p < np; ++p) {
     const uint64_t w = *reinterpret_cast<const EBI_GLOBAL uint64_t *>(base + (size_t)p * c.plane_stride + i0);
 #pragma unroll
     for (int j = 0; j < 8; ++j) d[j] = (d[j] << 8) | (((w >> (8 * j)) & 0xff) ^ 0x80);
+  }
+  if (np < c.n_planes) {  // np >= 1 here, so the shift is below 64
+    const uint32_t sh = 8 * (c.n_planes - np) + c.r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) d[j] <<= sh;
+    return;
+  }
+  if (c.r && !tail) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) d[j] <<= c.r;
+    return;
   }
   if (c.r) {
     // 8 records x r bits = r bytes starting at byte i0*r/8 (byte aligned).
@@ -959,10 +1100,45 @@ __device__ __forceinline__ void tile_store(double *dst, const double v[8], uint3
   wave_sync();  // the stage is reused by the wave's next job
 }
 
-__device__ __forceinline__ void buff_tile(const DevChunk &c, uint32_t t, double *out, double *stage) {
+__device__ __forceinline__ void buff_deltas(const DevChunk &c, uint32_t i0, uint64_t d[8]) {
+  buff_deltas(c, i0, d, c.n_planes, true);
+}
+
+// PRECISION_MAP (compression_common/buff.rs:36): fractional bits kept for
+// 0..12 decimal digits, six bits each.
+__host__ __device__ __forceinline__ uint32_t buff_precision_bits(uint32_t p) {
+  constexpr uint64_t lo = 0ull | (5ull << 6) | (8ull << 12) | (11ull << 18) | (15ull << 24) | (18ull << 30) |
+                          (21ull << 36) | (25ull << 42) | (28ull << 48) | (31ull << 54);
+  constexpr uint64_t hi = 35ull | (38ull << 6) | (50ull << 12);
+  return (uint32_t)((p < 10 ? lo >> (6 * p) : hi >> (6 * (p - 10))) & 63);
+}
+
+// Planes and tail that decode_with_precision reads
+// (chunk_reader/buff/controlled_precision.rs:21-75): with U = frac minus the
+// bits kept for `precision` (0 if fewer), plane 0 always, plane p >= 1 while
+// the B - 8p bits below it exceed U, the tail (r bits) only after every
+// plane and if r > U.  B < 8 decodes every bit.  precision < 0: full.
+__host__ __device__ __forceinline__ void buff_precision_reads(const DevChunk &c, int32_t precision, uint32_t *np,
+                                                              bool *tail) {
+  *np = c.n_planes;
+  *tail = true;
+  if (precision < 0 || c.bits < 8) return;
+  const uint32_t keep = buff_precision_bits((uint32_t)precision);
+  const uint32_t u = c.frac > keep ? c.frac - keep : 0u;
+  uint32_t p = 1;
+  while (p < c.n_planes && c.bits - 8 * p > u) ++p;
+  *np = p;
+  *tail = p == c.n_planes && c.r > u;
+}
+
+__device__ __forceinline__ void buff_tile(const DevChunk &c, uint32_t t, double *out, double *stage,
+                                          int32_t precision) {
   const uint32_t i0 = t * kTile + lane_id() * 8;
+  uint32_t np;
+  bool tail;
+  buff_precision_reads(c, precision, &np, &tail);
   uint64_t d[8];
-  buff_deltas(c, i0, d);  // zeros past the chunk's end
+  buff_deltas(c, i0, d, np, tail);  // zeros past the chunk's end
   double v[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) v[j] = buff_value(c, d[j]);

@@ -73,7 +73,9 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
   const uint32_t last = live && i1 == c.n ? c.n + 1 : i1;  // the chunk's last segment parses the terminator
   uint32_t from = i0, e = 0;
   if (live && i0 == 0) {
-    e = kCodec == kCodecElf ? elf_seg_first(s, bits, out, elf_tab) : xor_seg_first(s, bits, out);
+    e = kCodec == kCodecElf          ? elf_seg_first(s, bits, out, elf_tab)
+        : kCodec == kCodecElfOnChimp ? elfchimp_seg_first(s, bits, out, elf_tab)
+                                     : xor_seg_first(s, bits, out);
     from = 1;
   }
 #pragma unroll 1  // one copy of the codec loop per codec (instruction cache)
@@ -87,6 +89,8 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
       e = gorilla_seg(s, bits, c.n, a, cnt, from, lim, out, e);
     else if (kCodec == kCodecChimp)
       e = chimp_seg(s, bits, c.n, a, cnt, from, lim, out, e);
+    else if (kCodec == kCodecElfOnChimp)
+      e = elfchimp_seg(s, bits, c.n, a, cnt, from, lim, out, e, elf_tab);
     else
       e = elf_seg(s, bits, c.n, a, cnt, from, lim, out, e, elf_tab);
     from = max(from, lim);  // a terminator taken as record a + kRun is not parsed again
@@ -206,6 +210,7 @@ __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, 
     case kCodecGorilla: return segment_wave<kCodecGorilla>(cv, c, first, row, elf_tab, out);
     case kCodecChimp: return segment_wave<kCodecChimp>(cv, c, first, row, elf_tab, out);
     case kCodecElf: return segment_wave<kCodecElf>(cv, c, first, row, elf_tab, out);
+    case kCodecElfOnChimp: return segment_wave<kCodecElfOnChimp>(cv, c, first, row, elf_tab, out);
     case kCodecChimp128: {
       C128Out o{cv.out + c.rec_off, c.n};
       return c128_job(cv, c, first, row, stage, dists, o);
@@ -234,7 +239,7 @@ __device__ __forceinline__ void tile_job(const ColView *cols, const uint32_t *pr
     for (uint32_t t = jb.tile; t < jb.tile + jb.kind; ++t) {
       switch (c.codec) {
         case kCodecPlain: plain_tile(c, t, cv.out); break;
-        case kCodecBUFF: buff_tile(c, t, cv.out, stage); break;
+        case kCodecBUFF: buff_tile(c, t, cv.out, stage, cv.precision); break;
         case kCodecSprintz: sprintz_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, stage); break;
         default:
           if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
@@ -476,6 +481,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       case kCodecGorilla: e = segment_wave<kCodecGorilla>(cv, c, jb.tile, jb.col, elf_tab, q); break;
       case kCodecChimp: e = segment_wave<kCodecChimp>(cv, c, jb.tile, jb.col, elf_tab, q); break;
       case kCodecElf: e = segment_wave<kCodecElf>(cv, c, jb.tile, jb.col, elf_tab, q); break;
+      case kCodecElfOnChimp: e = segment_wave<kCodecElfOnChimp>(cv, c, jb.tile, jb.col, elf_tab, q); break;
       default: e = kErrCorrupt; break;
     }
     if (e) atomicOr(err, e);

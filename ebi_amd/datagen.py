@@ -43,7 +43,10 @@ GENERATORS = {
     "sin4": (lambda n, seed=4: sinusoid(n, 1000.0, 0.01, 4, seed), 4),
 }
 
+# the eight codecs of the plain-decode and batched workloads
 CODECS = ["uncompressed", "rle", "gorilla", "chimp", "chimp128", "elf", "buff", "delta_sprintz"]
+# every codec the GPU path decodes (ElfOnChimp: the Elf wrapper over Chimp)
+ALL_CODECS = CODECS + ["elf_on_chimp"]
 
 
 def xor_values() -> np.ndarray:

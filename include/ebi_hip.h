@@ -144,6 +144,16 @@ int ebi_hip_column_load(ebi_hip_ctx *ctx, const uint8_t *file, size_t len,
                         ebi_hip_col **out);
 int ebi_hip_column_free(ebi_hip_ctx *ctx, ebi_hip_col *col);
 int ebi_hip_column_info(const ebi_hip_col *col, ebi_file_info *info);
+/* Decoder::with_precision (api/decoder.rs:159-161): decodes of a BUFF column
+ * (materialize, filter_materialize, the device decodes and the matmul
+ * operands) use decode_with_precision (chunk_reader/buff/
+ * controlled_precision.rs:5-75) with this many decimal digits, reading only
+ * the byte planes those digits need; in-situ filters and aggregates are
+ * unaffected, as in the reference.  precision -1 restores full precision;
+ * other codecs ignore it.  0..12 (PRECISION_MAP's range), else
+ * EBI_INVALID_ARGUMENT.  Prepared batches keep the precision the column had
+ * when they were created. */
+int ebi_hip_column_set_precision(ebi_hip_col *col, int32_t precision);
 /* Bytes of chunk data resident for the column (excluding padding). */
 uint64_t ebi_hip_column_compressed_bytes(const ebi_hip_col *col);
 

@@ -164,43 +164,61 @@ static int dec_gorilla(msb_t *r, dvec *o) {
     }
 }
 
-/* Chimp, decoder/chunk_reader/chimp.rs:59-125. */
-static int dec_chimp(msb_t *r, dvec *o) {
-    uint64_t v, x;
-    uint32_t lead = UINT32_MAX;
-    RD(r, 64, &v);
-    for (;;) {
-        if (v == NAN_BITS) return ORC_OK;
-        if (dpush(o, bits2d(v))) return ORC_NOMEM;
-        uint64_t flag, t;
+/* ChimpDecoder::decompress_float, decoder/chunk_reader/chimp.rs:59-125:
+ * the first value is 64 raw bits, then a 2-bit flag per value; *has = 0
+ * once the value is the NaN end marker. */
+typedef struct { uint64_t v; uint32_t lead; int first, end; } chimp_t;
+static int chimp_next(chimp_t *s, msb_t *r, uint64_t *out, int *has) {
+    *has = 0;
+    if (s->end) return ORC_OK;
+    uint64_t x, flag, t;
+    if (s->first) {
+        s->first = 0;
+        RD(r, 64, &s->v);
+    } else {
         RD(r, 2, &flag);
         switch (flag) {
         case 3:
             RD(r, 3, &t);
-            lead = LEAD_REPR[t];
-            RD(r, 64 - lead, &x);
-            v ^= x;
+            s->lead = LEAD_REPR[t];
+            RD(r, 64 - s->lead, &x);
+            s->v ^= x;
             break;
         case 2:
-            if (lead > 64) return ORC_CORRUPT;
-            RD(r, 64 - lead, &x);
-            v ^= x;
+            if (s->lead > 64) return ORC_CORRUPT;
+            RD(r, 64 - s->lead, &x);
+            s->v ^= x;
             break;
         case 1: {
             uint64_t sig;
             RD(r, 3, &t);
-            lead = LEAD_REPR[t];
+            s->lead = LEAD_REPR[t];
             RD(r, 6, &sig);
             if (sig == 0) sig = 64;
-            if (sig + lead > 64) return ORC_CORRUPT;
-            uint32_t tz = 64 - (uint32_t)sig - lead;
-            RD(r, 64 - lead - tz, &x);
-            v ^= tz >= 64 ? 0 : x << tz;
+            if (sig + s->lead > 64) return ORC_CORRUPT;
+            uint32_t tz = 64 - (uint32_t)sig - s->lead;
+            RD(r, 64 - s->lead - tz, &x);
+            s->v ^= tz >= 64 ? 0 : x << tz;
             break;
         }
         default:
             break;
         }
+    }
+    if (s->v == NAN_BITS) { s->end = 1; return ORC_OK; }
+    *out = s->v; *has = 1;
+    return ORC_OK;
+}
+
+/* Chimp, decoder/chunk_reader/chimp.rs:59-125 via general_xor.rs:118-133. */
+static int dec_chimp(msb_t *r, dvec *o) {
+    chimp_t s = {0, UINT32_MAX, 1, 0};
+    for (;;) {
+        uint64_t v;
+        int has, rc;
+        if ((rc = chimp_next(&s, r, &v, &has))) return rc;
+        if (!has) return ORC_OK;
+        if (dpush(o, bits2d(v))) return ORC_NOMEM;
     }
 }
 
@@ -356,9 +374,12 @@ static int elfxor_next(elfxor_t *s, msb_t *r, uint64_t *out, int *has) {
     return ORC_OK;
 }
 
-/* ElfDecoderWrapper, decoder/chunk_reader/elf.rs:46-93. */
-static int dec_elf(msb_t *r, dvec *o) {
+/* ElfDecoderWrapper, decoder/chunk_reader/elf.rs:46-93, over the ElfXor
+ * decoder (Elf, scheme 7) or the Chimp decoder (ElfOnChimp, scheme 6,
+ * elf.rs:27). */
+static int dec_elf(msb_t *r, dvec *o, int on_chimp) {
     elfxor_t s = {0, UINT32_MAX, UINT32_MAX, 1, 0};
+    chimp_t c = {0, UINT32_MAX, 1, 0};
     int32_t last_beta_star = INT32_MAX;
     for (;;) {
         int b, has, rc;
@@ -378,7 +399,8 @@ static int dec_elf(msb_t *r, dvec *o) {
                 recover = 1;
             }
         }
-        if ((rc = elfxor_next(&s, r, &vp, &has))) return rc;
+        rc = on_chimp ? chimp_next(&c, r, &vp, &has) : elfxor_next(&s, r, &vp, &has);
+        if (rc) return rc;
         if (!has) return ORC_OK;
         double v = bits2d(vp);
         if (recover) {                                    /* recover_v_by_beta_star :66-81 */
@@ -477,6 +499,39 @@ static int dec_buff(const uint8_t *p, size_t size, dvec *o) {
         uint64_t d = b.B ? buff_delta(&b, i) : 0;
         if (dpush(o, (double)(int64_t)((uint64_t)b.base + d) / b.scale)) return ORC_NOMEM;
     }
+    return ORC_OK;
+}
+
+/* decode_with_precision, decoder/chunk_reader/buff/controlled_precision.rs:5-75
+ * (Decoder::with_precision, api/decoder.rs:159-161): the first byte plane
+ * always, further planes while the bits below them are still needed, the
+ * LSB tail only if every plane was read and its bits are needed. */
+static const uint32_t PRECISION_MAP[13] = {0, 5, 8, 11, 15, 18, 21, 25, 28, 31, 35, 38, 50}; /* buff.rs:36 */
+static int dec_buff_precision(const uint8_t *p, size_t size, uint32_t precision, dvec *o) {
+    buff_t b;
+    int rc = buff_open(p, size, &b);
+    if (rc) return rc;
+    if (precision > 12) return ORC_CORRUPT;       /* PRECISION_MAP[precision] panics */
+    uint32_t keep = PRECISION_MAP[precision];
+    uint32_t unnecessary = b.frac > keep ? b.frac - keep : 0; /* saturating_sub */
+    uint32_t remaining = b.B;
+    if (remaining < 8) return dec_buff(p, size, o); /* B = 0: base; 1..7: every tail bit */
+    uint64_t *fixed = malloc((b.n ? b.n : 1) * sizeof(uint64_t));
+    if (!fixed) return ORC_NOMEM;
+    remaining -= 8;
+    unsigned pl = 0;
+    for (uint32_t i = 0; i < b.n; ++i) fixed[i] = (uint64_t)plane_byte(&b, pl, i) << remaining;
+    while (remaining >= 8 && remaining > unnecessary) {
+        remaining -= 8;
+        ++pl;
+        for (uint32_t i = 0; i < b.n; ++i) fixed[i] |= (uint64_t)plane_byte(&b, pl, i) << remaining;
+    }
+    int tail = !(remaining == 0 || remaining <= unnecessary); /* then every plane was read */
+    for (uint32_t i = 0; i < b.n; ++i) {
+        uint64_t d = fixed[i] | (tail ? buff_tail(&b, i) : 0);
+        if (dpush(o, (double)(int64_t)((uint64_t)b.base + d) / b.scale)) { free(fixed); return ORC_NOMEM; }
+    }
+    free(fixed);
     return ORC_OK;
 }
 
@@ -708,7 +763,15 @@ static double rmax(double a, double b) { return isnan(a) ? b : isnan(b) ? a : (b
 /* ------------------------------------------------------------------ */
 /* Chunk-level readers                                                 */
 /* ------------------------------------------------------------------ */
+/* precision >= 0: Decoder::with_precision (only BUFF decodes differently,
+ * chunk_reader.rs:73-77). */
+static int chunk_decompress_p(int scheme, const uint8_t *p, size_t size, uint64_t nrec, int32_t precision,
+                              dvec *o);
 static int chunk_decompress(int scheme, const uint8_t *p, size_t size, uint64_t nrec, dvec *o) {
+    return chunk_decompress_p(scheme, p, size, nrec, -1, o);
+}
+static int chunk_decompress_p(int scheme, const uint8_t *p, size_t size, uint64_t nrec, int32_t precision,
+                              dvec *o) {
     msb_t r = {p, size, 0, 0};
     switch (scheme) {
     case 0: /* chunk_reader/uncompressed.rs:56-71 (read_exact of n records) */
@@ -728,11 +791,12 @@ static int chunk_decompress(int scheme, const uint8_t *p, size_t size, uint64_t 
         }
         return ORC_OK;
     }
-    case 2: return dec_buff(p, size, o);
+    case 2: return precision >= 0 ? dec_buff_precision(p, size, (uint32_t)precision, o) : dec_buff(p, size, o);
     case 3: return dec_gorilla(&r, o);
     case 4: return dec_chimp(&r, o);
     case 5: return dec_chimp128(&r, o);
-    case 7: return dec_elf(&r, o);
+    case 6: return dec_elf(&r, o, 1);
+    case 7: return dec_elf(&r, o, 0);
     case 8: {
         spz_hdr h;
         int64_t *q = malloc((nrec ? nrec : 1) * sizeof(int64_t));
@@ -802,6 +866,10 @@ int orc_file_records(const uint8_t *file, size_t len, uint64_t *n_records, uint6
 
 int orc_materialize(const uint8_t *file, size_t len, const uint64_t *bs, int64_t chunk_id,
                     double *out, size_t cap, size_t *n_out) {
+    return orc_materialize_p(file, len, bs, chunk_id, -1, out, cap, n_out);
+}
+int orc_materialize_p(const uint8_t *file, size_t len, const uint64_t *bs, int64_t chunk_id, int32_t precision,
+                      double *out, size_t cap, size_t *n_out) {
     ctx_t c;
     int rc = ctx_open(&c, file, len);
     if (rc) return rc;
@@ -811,7 +879,7 @@ int orc_materialize(const uint8_t *file, size_t len, const uint64_t *bs, int64_t
         if (!chunk_selected(&c, k, bs, chunk_id, &m)) continue;
         free(m);
         dvec d = {0};
-        rc = chunk_decompress(c.f.scheme, file + c.f.phys[k], c.f.size[k], c.f.nrec[k], &d);
+        rc = chunk_decompress_p(c.f.scheme, file + c.f.phys[k], c.f.size[k], c.f.nrec[k], precision, &d);
         for (size_t i = 0; !rc && i < d.n; ++i) {
             if (!in_mask(bs, c.f.n_records, c.f.logi[k] + i)) continue;
             if (no >= cap) { rc = ORC_CAPACITY; break; }
@@ -885,6 +953,10 @@ int orc_filter(const uint8_t *file, size_t len, const orc_pred *p, const uint64_
 
 int orc_filter_materialize(const uint8_t *file, size_t len, const orc_pred *p, const uint64_t *bs,
                            int64_t chunk_id, double *out, size_t cap, size_t *n_out) {
+    return orc_filter_materialize_p(file, len, p, bs, chunk_id, -1, out, cap, n_out);
+}
+int orc_filter_materialize_p(const uint8_t *file, size_t len, const orc_pred *p, const uint64_t *bs,
+                             int64_t chunk_id, int32_t precision, double *out, size_t cap, size_t *n_out) {
     ctx_t c;
     int rc = ctx_open(&c, file, len);
     if (rc) return rc;
@@ -894,7 +966,7 @@ int orc_filter_materialize(const uint8_t *file, size_t len, const orc_pred *p, c
         if (!chunk_selected(&c, k, bs, chunk_id, &m)) continue;
         uint64_t n = c.f.nrec[k];
         dvec d = {0};
-        rc = chunk_decompress(c.f.scheme, file + c.f.phys[k], c.f.size[k], n, &d);
+        rc = chunk_decompress_p(c.f.scheme, file + c.f.phys[k], c.f.size[k], n, precision, &d);
         uint8_t *res = NULL;
         if (!rc && c.f.scheme == 2) {
             /* BUFFReader::filter_materialize :230-241: in-situ filter, then

@@ -47,6 +47,13 @@ int orc_file_records(const uint8_t *file, size_t len, uint64_t *n_records, uint6
  * ceil(N/64) over global ids; chunk_id < 0 means all chunks. */
 int orc_materialize(const uint8_t *file, size_t len, const uint64_t *bitset, int64_t chunk_id,
                     double *out, size_t cap, size_t *n_out);
+/* The same after Decoder::with_precision(precision) (api/decoder.rs:159-161;
+ * BUFF decodes with decode_with_precision, chunk_reader/buff/
+ * controlled_precision.rs:5-75); precision < 0: full precision. */
+int orc_materialize_p(const uint8_t *file, size_t len, const uint64_t *bitset, int64_t chunk_id,
+                      int32_t precision, double *out, size_t cap, size_t *n_out);
+int orc_filter_materialize_p(const uint8_t *file, size_t len, const orc_pred *pred, const uint64_t *bitset,
+                             int64_t chunk_id, int32_t precision, double *out, size_t cap, size_t *n_out);
 /* Decoder::filter (api/decoder.rs:242-281); out has ceil(N/64) words. */
 int orc_filter(const uint8_t *file, size_t len, const orc_pred *pred, const uint64_t *bitset,
                int64_t chunk_id, uint64_t *out_bitset);

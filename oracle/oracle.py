@@ -54,6 +54,10 @@ def lib():
         L.orc_filter.argtypes = [P, C.c_size_t, C.POINTER(_Pred), P, C.c_int64, P]
         L.orc_filter_materialize.argtypes = [P, C.c_size_t, C.POINTER(_Pred), P, C.c_int64, P, C.c_size_t,
                                              C.POINTER(C.c_size_t)]
+        L.orc_materialize_p.argtypes = [P, C.c_size_t, P, C.c_int64, C.c_int32, P, C.c_size_t,
+                                        C.POINTER(C.c_size_t)]
+        L.orc_filter_materialize_p.argtypes = [P, C.c_size_t, C.POINTER(_Pred), P, C.c_int64, C.c_int32, P,
+                                               C.c_size_t, C.POINTER(C.c_size_t)]
         L.orc_file_records.argtypes = [P, C.c_size_t, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.orc_zigzag.restype = C.c_uint64
         L.orc_zigzag.argtypes = [C.c_int64]
@@ -82,6 +86,7 @@ class OracleDecoder:
     """CPU restatement of Decoder over one EBI1 file held in memory."""
 
     def __init__(self, data: bytes):
+        self.precision = -1  # Decoder::with_precision (api/decoder.rs:159-161); -1: full
         self._data = data  # keep the caller's buffer alive; it is only read
         self._buf = np.frombuffer(data, dtype=np.uint8)
         self._p = self._buf.ctypes.data
@@ -113,9 +118,12 @@ class OracleDecoder:
         cap = max(self.number_of_records * 2 + 64, 64)
         out = np.empty(cap, dtype=np.float64)
         n = C.c_size_t()
-        _ck(lib().orc_materialize(self._p, self._n, mp, self._cid(chunk_id), out.ctypes.data, cap, C.byref(n)),
-            "materialize")
+        _ck(lib().orc_materialize_p(self._p, self._n, mp, self._cid(chunk_id), self.precision, out.ctypes.data, cap,
+                                    C.byref(n)), "materialize")
         return out[: n.value].copy()
+
+    def with_precision(self, precision: int) -> None:
+        self.precision = int(precision)
 
     def filter_bitset(self, predicate, bitmask=None, chunk_id=None) -> np.ndarray:
         _, mp = self._mask(bitmask)
@@ -135,8 +143,8 @@ class OracleDecoder:
         out = np.empty(cap, dtype=np.float64)
         n = C.c_size_t()
         p = _pred(predicate)
-        _ck(lib().orc_filter_materialize(self._p, self._n, C.byref(p), mp, self._cid(chunk_id), out.ctypes.data, cap,
-                                         C.byref(n)), "filter_materialize")
+        _ck(lib().orc_filter_materialize_p(self._p, self._n, C.byref(p), mp, self._cid(chunk_id), self.precision,
+                                           out.ctypes.data, cap, C.byref(n)), "filter_materialize")
         return out[: n.value].copy()
 
     def _agg(self, fn, bitmask, chunk_id, what):

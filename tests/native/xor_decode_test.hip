@@ -1,4 +1,4 @@
-// Host run of the XOR decoders (Gorilla, Chimp, Elf, Chimp128) on files
+// Host run of the XOR decoders (Gorilla, Chimp, Elf, ElfOnChimp, Chimp128) on files
 // from this library's encoder.  Every Gorilla / Chimp / Elf chunk is decoded
 // whole from the stream start, and in the segment kernel's forms: the index
 // pass writes a checkpoint every kSeg records, then each segment restarts
@@ -84,6 +84,27 @@ struct RowSink {
   }
 };
 
+// One lockstep run of a Gorilla / Chimp / Elf / ElfOnChimp lane, and a
+// lane's record 0.
+template <class Sink>
+static uint32_t seg_step(int codec, XorState &s, uint32_t bits, uint32_t n, uint32_t a, uint32_t cnt, uint32_t from,
+                         uint32_t lim, Sink &sink, uint32_t e) {
+  switch (codec) {
+    case kCodecGorilla: return gorilla_seg(s, bits, n, a, cnt, from, lim, sink, e);
+    case kCodecChimp: return chimp_seg(s, bits, n, a, cnt, from, lim, sink, e);
+    case kCodecElfOnChimp: return elfchimp_seg(s, bits, n, a, cnt, from, lim, sink, e, g_elf_tab);
+    default: return elf_seg(s, bits, n, a, cnt, from, lim, sink, e, g_elf_tab);
+  }
+}
+template <class Sink>
+static uint32_t seg_first(int codec, XorState &s, uint32_t bits, Sink &sink) {
+  switch (codec) {
+    case kCodecElf: return elf_seg_first(s, bits, sink, g_elf_tab);
+    case kCodecElfOnChimp: return elfchimp_seg_first(s, bits, sink, g_elf_tab);
+    default: return xor_seg_first(s, bits, sink);
+  }
+}
+
 static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t bits, uint32_t cn, const XorCkpt *ck,
                                       uint32_t nseg, double *out, const Ilv *ilv = nullptr) {
   RowSink sink{out, cn};
@@ -108,7 +129,7 @@ static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t b
       from[l] = i0[l];
       e[l] = 0;
       if (live && i0[l] == 0) {
-        e[l] = codec == kCodecElf ? elf_seg_first(st[l], bits, sink, g_elf_tab) : xor_seg_first(st[l], bits, sink);
+        e[l] = seg_first(codec, st[l], bits, sink);
         from[l] = 1;
       }
     }
@@ -119,9 +140,7 @@ static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t b
       for (uint32_t l = 0; l < kWave; ++l) {
         const uint32_t a = i0[l] + k0;
         const uint32_t lim = last[l] == a + kRun + 1 ? last[l] : std::min(last[l], a + kRun);
-        e[l] = codec == kCodecGorilla ? gorilla_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l])
-               : codec == kCodecChimp ? chimp_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l])
-                                      : elf_seg(st[l], bits, cn, a, cnt, from[l], lim, sink, e[l], g_elf_tab);
+        e[l] = seg_step(codec, st[l], bits, cn, a, cnt, from[l], lim, sink, e[l]);
         from[l] = std::max(from[l], lim);
       }
     }
@@ -284,7 +303,7 @@ static void nan_cases(int codec) {
           from[l] = i0[l];
           ee[l] = 0;
           if (live && i0[l] == 0) {
-            ee[l] = codec == kCodecElf ? elf_seg_first(st[l], bits, rs, g_elf_tab) : xor_seg_first(st[l], bits, rs);
+            ee[l] = seg_first(codec, st[l], bits, rs);
             from[l] = 1;
           }
         }
@@ -295,9 +314,7 @@ static void nan_cases(int codec) {
           for (uint32_t l = 0; l < kWave; ++l) {
             const uint32_t a = i0[l] + k0;
             const uint32_t lim = last[l] == a + kRun + 1 ? last[l] : std::min(last[l], a + kRun);
-            ee[l] = codec == kCodecGorilla ? gorilla_seg(st[l], bits, end, a, cnt, from[l], lim, rs, ee[l])
-                    : codec == kCodecChimp ? chimp_seg(st[l], bits, end, a, cnt, from[l], lim, rs, ee[l])
-                                           : elf_seg(st[l], bits, end, a, cnt, from[l], lim, rs, ee[l], g_elf_tab);
+            ee[l] = seg_step(codec, st[l], bits, end, a, cnt, from[l], lim, rs, ee[l]);
             from[l] = std::max(from[l], lim);
           }
         }
@@ -324,7 +341,7 @@ int main() {
     int kind, dec;
     double p0, p1;
   } gens[] = {{0, 2, 100.0, 0.1}, {1, 3, -1000.0, 1000.0}, {2, 2, 0.01, 0.0}, {3, 4, 1000.0, 0.01}};
-  const int codecs[] = {kCodecGorilla, kCodecChimp, kCodecElf, kCodecChimp128};
+  const int codecs[] = {kCodecGorilla, kCodecChimp, kCodecElf, kCodecElfOnChimp, kCodecChimp128};
   const uint64_t chunks[] = {1, 3, 5, 64, 65, 80, 96, 112, 127, 1003, 4096};  // 80 / 96 / 112: terminator after a full run
   const uint64_t n = 10003;
   std::vector<double> v(n);
@@ -379,7 +396,7 @@ int main() {
             for (uint32_t i = 0; i < cn; ++i)
               expect(as_u64(out[i]) == as_u64(want[i]), "segment value", codec, g, chunk, k, i);
             const Ilv ilv = build_ilv(data, bits, ck.data(), nseg);
-            if (codec == kCodecElf) {  // the per-record form through the ilv (the lockstep form follows)
+            if (codec == kCodecElf || codec == kCodecElfOnChimp) {  // the per-record form through the ilv
               std::fill(out.begin(), out.end(), 0.0);
               for (uint32_t seg = 0; seg < nseg; ++seg) {
                 const uint32_t i0 = seg * kSeg, i1 = std::min(i0 + kSeg, cn);

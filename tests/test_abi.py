@@ -39,6 +39,6 @@ def test_encode_rejects_bad_config():
     import pytest
 
     with pytest.raises(E.EbiError):
-        E.encode(np.arange(10.0), E.CompressorConfig(6, 1, 8192), E.ChunkOption.record_count(4))  # ElfOnChimp
+        E.encode(np.arange(10.0), E.CompressorConfig(9, 1, 8192), E.ChunkOption.record_count(4))  # Zstd
     with pytest.raises(E.EbiError):
         E.encode(np.arange(10.0), E.CompressorConfig.gorilla(), E.ChunkOption.record_count(0))

@@ -10,7 +10,7 @@ from oracle.oracle import OracleDecoder
 
 pytestmark = pytest.mark.gpu
 
-CODECS = G.CODECS
+CODECS = G.ALL_CODECS
 
 
 def _gpu_decode(ctx, data, per_chunk=False):

@@ -14,7 +14,7 @@ from tests.helpers import chunk_of, known_answers, pred_of, round_at
 
 pytestmark = pytest.mark.gpu
 KA = known_answers()
-CODECS = G.CODECS
+CODECS = G.ALL_CODECS
 
 
 def enc(values, codec, scale, chunk):

@@ -1,7 +1,7 @@
 """GPU parity on the reference's quirks that ordinary data reaches:
 
 * a canonical NaN (0x7ff8000000000000, numpy's nan) in a Gorilla / Chimp /
-  Chimp128 / Elf column ends its chunk there -- the reference's readers stop
+  Chimp128 / Elf / ElfOnChimp column ends its chunk there -- the reference's readers stop
   at the first NaN and return the records before it without an error
   (chunk_reader/gorilla.rs:149-152, chimp.rs:119, chimp_n.rs:219-265,
   elf.rs:105-258, consumed by decoder/query.rs:15-154);
@@ -21,7 +21,7 @@ from oracle.oracle import OracleDecoder
 
 pytestmark = pytest.mark.gpu
 
-XOR = ["gorilla", "chimp", "chimp128", "elf"]
+XOR = ["gorilla", "chimp", "chimp128", "elf", "elf_on_chimp"]
 
 
 def same_bits(a, b):

@@ -10,7 +10,7 @@ from oracle.oracle import OracleDecoder
 from tests.helpers import chunk_of, known_answers, pred_of, round_at
 
 KA = known_answers()
-CODECS = G.CODECS
+CODECS = G.ALL_CODECS
 
 
 def enc(values, codec, scale, chunk):

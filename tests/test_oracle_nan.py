@@ -1,5 +1,5 @@
 """The oracle's NaN end semantics, pinned by a closed form: a chunk of a
-Gorilla / Chimp / Chimp128 / Elf column decodes to its records before its
+Gorilla / Chimp / Chimp128 / Elf / ElfOnChimp column decodes to its records before its
 first canonical NaN (the reference's readers stop there without an error,
 chunk_reader/gorilla.rs:149-152, chimp.rs:119, chimp_n.rs:219-265,
 elf.rs:105-258), and the queries see exactly those records
@@ -12,7 +12,7 @@ from ebi_amd import datagen as G
 from oracle.oracle import OracleDecoder
 
 
-@pytest.mark.parametrize("codec", ["gorilla", "chimp", "chimp128", "elf"])
+@pytest.mark.parametrize("codec", ["gorilla", "chimp", "chimp128", "elf", "elf_on_chimp"])
 def test_oracle_nan_truncates_chunks(codec):
     n, chunk = 3000, 700
     vals = G.random_walk(n, seed=9)
