@@ -90,6 +90,26 @@ class _Query(C.Structure):
     ]
 
 
+class _KnnResult(C.Structure):
+    _fields_ = [("index", C.c_uint64), ("distance", C.c_double)]
+
+
+class _ExprNode(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_uint32), ("literal", C.c_double)]
+
+
+class _Aggregation(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("n_nodes", C.c_uint32),
+        ("nodes", C.c_void_p),
+        ("out_scalar", C.c_void_p),
+        ("out_values", C.c_void_p),
+        ("capacity", C.c_size_t),
+        ("n_out", C.c_void_p),
+    ]
+
+
 class _ServeOpts(C.Structure):
     _fields_ = [
         ("device_budget", C.c_uint64),
@@ -184,6 +204,11 @@ _SIGS = {
     "ebi_hip_matmul": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _P]),
     "ebi_hip_gemm_decoded": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t, _P, _P]),
     "ebi_hip_dgemm_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t]),
+    "ebi_hip_knn1": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(_KnnResult)]),
+    "ebi_hip_distance_squared": (C.c_int, [_P, _P, C.c_uint64, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_double)]),
+    "ebi_hip_dot_product": (C.c_int, [_P, _P, C.c_uint64, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_double)]),
+    "ebi_hip_matmul_dot": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, _P]),
+    "ebi_hip_aggregate": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, _P]),
 }
 
 
@@ -628,6 +653,144 @@ class Decoder:
         _check(lib().ebi_hip_matmul(self.ctx._h, self._h, t.ctypes.data, k, n, m, kd, out.ctypes.data), self.ctx,
                "matmul")
         return out[: batch * m * n]
+
+
+    # ---- ML queries (api/decoder.rs:469-656) ----
+    def knn1(self, target):
+        """Decoder::knn1: (index, squared distance) of the vector nearest to
+        `target` (KnnResult; index 2**64 - 1 when none)."""
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        r = _KnnResult()
+        _check(lib().ebi_hip_knn1(self.ctx._h, self._h, t.ctypes.data if t.size else None, t.size, C.byref(r)),
+               self.ctx, "knn1")
+        return int(r.index), float(r.distance)
+
+    def _fold(self, fn, chunk_id, offset, target):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        r = C.c_double()
+        _check(fn(self.ctx._h, self._h, int(chunk_id), int(offset), t.ctypes.data if t.size else None, t.size,
+                  C.byref(r)), self.ctx, "chunk fold")
+        return float(r.value)
+
+    def distance_squared(self, chunk_id, offset_in_chunk, target) -> float:
+        """GeneralChunkReader::distance_squared of chunk `chunk_id`."""
+        return self._fold(lib().ebi_hip_distance_squared, chunk_id, offset_in_chunk, target)
+
+    def dot_product(self, chunk_id, offset_in_chunk, target) -> float:
+        """GeneralChunkReader::dot_product of chunk `chunk_id`."""
+        return self._fold(lib().ebi_hip_dot_product, chunk_id, offset_in_chunk, target)
+
+    def matmul(self, target_matrix, target_matrix_shape, data_matrix_shape) -> np.ndarray:
+        """Decoder::matmul (api/decoder.rs:606-656): target column-first,
+        data matrices row-first; the flat row-first products."""
+        tr, tc = (int(x) for x in target_matrix_shape)
+        dr, dc = (int(x) for x in data_matrix_shape)
+        t = np.ascontiguousarray(target_matrix, dtype=np.float64).ravel()
+        if t.size < tr * tc:
+            raise EbiError(INVALID_ARGUMENT, "target shorter than its shape")
+        batch = self.number_of_records // (dr * dc) if dr * dc else 0
+        out = np.empty(max(batch * dr * tc, 1), dtype=np.float64)
+        _check(lib().ebi_hip_matmul_dot(self.ctx._h, self._h, t.ctypes.data, tr, tc, dr, dc, out.ctypes.data),
+               self.ctx, "matmul")
+        return out[: batch * dr * tc]
+
+
+# --------------------------------------------------- aggregations (Expr)
+EXPR_INDEX, EXPR_LITERAL, EXPR_ADD, EXPR_SUB, EXPR_MUL, EXPR_DIV = range(6)
+AGG_NONE, AGG_MAX, AGG_MIN, AGG_SUM = range(4)
+
+
+class Expr:
+    """decoder::expr::Expr (decoder/expr.rs:31-40): Index(i), Literal(v) or
+    Binary {op, lhs, rhs}; kept as a postfix program for the C ABI."""
+
+    def __init__(self, nodes):
+        self.nodes = list(nodes)
+
+    @staticmethod
+    def index(i: int) -> "Expr":
+        return Expr([(EXPR_INDEX, int(i), 0.0)])
+
+    @staticmethod
+    def literal(v: float) -> "Expr":
+        return Expr([(EXPR_LITERAL, 0, float(v))])
+
+    @staticmethod
+    def binary(op: str, lhs: "Expr", rhs: "Expr") -> "Expr":
+        code = {"add": EXPR_ADD, "sub": EXPR_SUB, "mul": EXPR_MUL, "div": EXPR_DIV}[op]
+        return Expr(lhs.nodes + rhs.nodes + [(code, 0, 0.0)])
+
+    def __add__(self, o):
+        return Expr.binary("add", self, _as_expr(o))
+
+    def __sub__(self, o):
+        return Expr.binary("sub", self, _as_expr(o))
+
+    def __mul__(self, o):
+        return Expr.binary("mul", self, _as_expr(o))
+
+    def __truediv__(self, o):
+        return Expr.binary("div", self, _as_expr(o))
+
+
+def _as_expr(x):
+    return x if isinstance(x, Expr) else Expr.literal(float(x))
+
+
+class Aggregation:
+    """api::decoder::Aggregation (api/decoder.rs:835-917)."""
+
+    def __init__(self, kind: int, expr: Expr):
+        self.kind, self.expr = int(kind), expr
+
+    @staticmethod
+    def sum(expr: Expr) -> "Aggregation":
+        return Aggregation(AGG_SUM, expr)
+
+    @staticmethod
+    def min(expr: Expr) -> "Aggregation":
+        return Aggregation(AGG_MIN, expr)
+
+    @staticmethod
+    def max(expr: Expr) -> "Aggregation":
+        return Aggregation(AGG_MAX, expr)
+
+    @staticmethod
+    def none(expr: Expr) -> "Aggregation":
+        return Aggregation(AGG_NONE, expr)
+
+    def compute(self, decoders, bitmask=None):
+        return Aggregation.compute_multiple([self], decoders, bitmask)[0]
+
+    @staticmethod
+    def compute_multiple(aggregations, decoders, bitmask=None) -> list:
+        """Scalars for MAX / MIN / SUM, arrays for NONE."""
+        decs = list(decoders)
+        ctx = decs[0].ctx
+        n_max = max(d.number_of_records for d in decs)
+        mask = None
+        if bitmask is not None:
+            mask = np.ascontiguousarray(bitmask) if isinstance(bitmask, np.ndarray) and bitmask.dtype == np.uint64 \
+                else ids_to_bitset(bitmask, n_max)
+        keep, arr = [], (_Aggregation * len(aggregations))()
+        for a, agg in zip(arr, aggregations):
+            nodes = (_ExprNode * len(agg.expr.nodes))(*[_ExprNode(k, i, v) for k, i, v in agg.expr.nodes])
+            scalar = C.c_double(0.0)
+            n_out = C.c_size_t(0)
+            vals = np.empty(max(n_max, 1), dtype=np.float64) if agg.kind == AGG_NONE else None
+            a.kind, a.n_nodes, a.nodes = agg.kind, len(agg.expr.nodes), C.cast(nodes, C.c_void_p)
+            a.out_scalar = C.addressof(scalar)
+            a.out_values = vals.ctypes.data if vals is not None else None
+            a.capacity = vals.size if vals is not None else 0
+            a.n_out = C.addressof(n_out)
+            keep.append((nodes, scalar, n_out, vals))
+        cols = (_P * len(decs))(*[d._h for d in decs])
+        _check(lib().ebi_hip_aggregate(ctx._h, cols, len(decs), arr, len(aggregations),
+                                       mask.ctypes.data if mask is not None else None), ctx, "aggregate")
+        out = []
+        for agg, (_, scalar, n_out, vals) in zip(aggregations, keep):
+            out.append(vals[: n_out.value].copy() if agg.kind == AGG_NONE else float(scalar.value))
+        return out
 
 
 class Query:

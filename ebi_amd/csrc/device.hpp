@@ -180,6 +180,39 @@ struct alignas(16) ColView {
 };
 static_assert(sizeof(ColView) == 112, "ColView layout");
 
+// knn1 candidates of one chunk (Decoder::knn1, api/decoder.rs:469-533):
+// `straddle` when the previous chunk ended inside a vector (its partial
+// covered target[0 .. m1) from record prev_off of that chunk), then `k`
+// whole vectors from record 0.  n_eff: the chunk's decoded records.
+struct alignas(16) KnnChunk {
+  uint64_t rec_off, prev_rec_off;
+  uint32_t n_eff, prev_n_eff;
+  uint32_t prev_off, m1;
+  uint32_t straddle, k;
+};
+static_assert(sizeof(KnnChunk) == 40 || sizeof(KnnChunk) == 48, "KnnChunk layout");
+
+// Decoder::matmul batches of one chunk (api/decoder.rs:606-656): they start
+// at record off0 of the chunk, one data matrix apart.
+struct alignas(16) DotChunk {
+  uint64_t rec_off;
+  uint32_t n_eff, off0;
+};
+
+// Postfix program of an Expr (decoder/expr.rs): kind 0 column `index`,
+// 1 literal, 2 add, 3 sub, 4 mul, 5 div.
+constexpr uint32_t kExprNodes = 32;
+constexpr int kExprStack = 16;
+struct ExprNode {
+  int32_t kind;
+  uint32_t index;
+  double literal;
+};
+struct ExprProgram {
+  ExprNode nodes[kExprNodes];
+  uint32_t n;
+};
+
 // Device error flags (OR-ed by kernels).
 enum DevError : uint32_t {
   kErrNone = 0,

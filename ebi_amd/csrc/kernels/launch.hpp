@@ -63,6 +63,21 @@ uint32_t compact_blocks(uint64_t n_words);
 void compact(hipStream_t s, const uint64_t *bits, uint64_t n_words, const double *values, double *out,
              uint64_t *block_scratch, uint64_t *n_out);
 
+// Analytics over decoded columns (analytics.hip).  knn: candidate distances
+// to d[], then red[0] = min distance key, red[1] = first candidate at it,
+// red[2] = 1 + last `<=` candidate at it (0: none).
+void knn(hipStream_t s, const double *values, const KnnChunk *chunks, uint32_t nc, const uint64_t *cand0,
+         uint64_t n_cand, const double *target, uint32_t L, double *d, unsigned long long *red);
+void dot(hipStream_t s, const double *values, const DotChunk *chunks, uint32_t nc, const uint64_t *batch0,
+         uint64_t n_batch, uint32_t stride, uint32_t rows, uint32_t data_cols, const double *target, uint32_t k,
+         uint32_t cols, double *out);
+void dist_one(hipStream_t s, const double *values, uint64_t rec_off, uint32_t n_eff, uint32_t off,
+              const double *target, uint32_t len, int dot, double *out);
+void chunk_count(hipStream_t s, const DevChunk *chunks, uint32_t nc, const uint64_t *bits, uint64_t *counts);
+void expr(hipStream_t s, const ExprProgram &prog, const double *const *cols, const uint64_t *col_off, uint32_t nc,
+          const uint64_t *len, const uint64_t *out_off, const uint2 *tiles, uint32_t n_tiles, int kind,
+          double *partials, unsigned long long *acc, double *out);
+
 // FP64 DGEMM on the matrix cores: C_b = A_b * B_b, row-major, b < batch,
 // element strides sA / sB / sC between batches (0 = shared operand).
 hipError_t dgemm(hipStream_t s, const double *A, const double *B, double *C, uint32_t m, uint32_t n, uint32_t k,

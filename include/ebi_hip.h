@@ -300,6 +300,89 @@ int ebi_hip_dgemm_device(ebi_hip_ctx *ctx, const double *d_a, const double *d_b,
                          double *d_c, size_t m, size_t n, size_t k);
 
 /* ------------------------------------------------------------------ */
+/* ML queries on compressed columns (decoded on the device).  Every fold */
+/* the reference computes as a sequential f64 sum runs in the same order */
+/* on the GPU, so results are bit-identical to it.                       */
+/* ------------------------------------------------------------------ */
+typedef struct ebi_knn_result {
+    uint64_t index;   /* usize::MAX (UINT64_MAX) when no vector was accepted */
+    double distance;  /* squared distance; INFINITY when none */
+} ebi_knn_result;
+
+/* Decoder::knn1 (api/decoder.rs:469-533): the column read as consecutive
+ * vectors of `len` records; the nearest to `target` by squared distance,
+ * with the reference's walk reproduced exactly: a vector begun at the end
+ * of a chunk is completed from the next chunk's first records (and then
+ * compared with `<`), and that chunk's own vectors still start at its record
+ * 0; vectors inside a chunk are compared with `<=`.  len 0 (which never ends
+ * the reference's loop) is EBI_INVALID_ARGUMENT. */
+int ebi_hip_knn1(ebi_hip_ctx *ctx, const ebi_hip_col *col, const double *target, size_t len,
+                 ebi_knn_result *out);
+/* GeneralChunkReader::{distance_squared, dot_product} (chunk_reader.rs:233-260,
+ * decoder/query.rs:157-192) on chunk `chunk_id`: the records from
+ * offset_in_chunk (clamped to the chunk's decoded records) zipped with
+ * target. */
+int ebi_hip_distance_squared(ebi_hip_ctx *ctx, const ebi_hip_col *col, uint64_t chunk_id,
+                             size_t offset_in_chunk, const double *target, size_t len, double *out);
+int ebi_hip_dot_product(ebi_hip_ctx *ctx, const ebi_hip_col *col, uint64_t chunk_id,
+                        size_t offset_in_chunk, const double *target, size_t len, double *out);
+/* Decoder::matmul (api/decoder.rs:606-656): out = the batch row-major
+ * products of the column's data matrices (data_rows x data_cols, row-major,
+ * back to back) with the target matrix given column-first (target_cols
+ * columns of target_rows values); each element is the reference's
+ * sequential dot product (bit-identical), each batch read where the
+ * reference's walk reads it.  EBI_PRECONDITION as the reference
+ * (data_cols != target_rows, N not a multiple of the data matrix size, or
+ * more matrices than the chunks hold). */
+int ebi_hip_matmul_dot(ebi_hip_ctx *ctx, const ebi_hip_col *col, const double *target,
+                       size_t target_rows, size_t target_cols, size_t data_rows, size_t data_cols,
+                       double *out);
+
+/* ------------------------------------------------------------------ */
+/* Multi-column aggregations (Aggregation / Expr, api/decoder.rs:755-917,
+ * decoder/expr.rs): expressions over the records of several columns with
+ * the same chunking, in postfix form. */
+/* ------------------------------------------------------------------ */
+enum {
+    EBI_EXPR_INDEX = 0,   /* Expr::Index(index): the column cols[index] */
+    EBI_EXPR_LITERAL = 1, /* Expr::Literal(literal) */
+    EBI_EXPR_ADD = 2,     /* Op::Add of the two entries below (lhs deeper) */
+    EBI_EXPR_SUB = 3,
+    EBI_EXPR_MUL = 4,
+    EBI_EXPR_DIV = 5
+};
+typedef struct ebi_expr_node {
+    int32_t kind;
+    uint32_t index;
+    double literal;
+} ebi_expr_node;
+/* AggregationKind (api/decoder.rs:756-761). */
+enum { EBI_AGG_NONE = 0, EBI_AGG_MAX = 1, EBI_AGG_MIN = 2, EBI_AGG_SUM = 3 };
+typedef struct ebi_aggregation {
+    int32_t kind;                /* EBI_AGG_* */
+    uint32_t n_nodes;            /* postfix program, at most 32 nodes, stack depth 16 */
+    const ebi_expr_node *nodes;
+    double *out_scalar;          /* MAX / MIN / SUM */
+    double *out_values;          /* NONE: the expression's records, chunk by chunk */
+    size_t capacity;             /* NONE: room in out_values */
+    size_t *n_out;               /* NONE: records written */
+} ebi_aggregation;
+/* Aggregation::compute_multiple (api/decoder.rs:874-917) over cols[0..n_cols)
+ * with the optional bitmask (global ids, words for the longest column).  An
+ * aggregation of one bare column runs that column's own sum / min / max
+ * (in-situ for BUFF and DeltaSprintz); every other one is evaluated per
+ * chunk i < decoder 0's chunk count, record k of chunk i taking the k-th
+ * selected record of chunk i of each column it reads (the reference's
+ * zips), up to the shortest.  SUM: within 1e-12 relative of a compensated
+ * sum (double-double per 512 records); MAX / MIN: f64::max / f64::min, from
+ * -INFINITY / INFINITY.  EBI_PRECONDITION for a literal alone or a binary
+ * node of two literals (decoder/expr.rs:73, :97) or a column with fewer
+ * chunks than cols[0]; EBI_UNSUPPORTED for NONE over a bare column (the
+ * reference's todo!()). */
+int ebi_hip_aggregate(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols, size_t n_cols,
+                      ebi_aggregation *aggs, size_t n_aggs, const uint64_t *bitset);
+
+/* ------------------------------------------------------------------ */
 /* Serving: continuous batching of concurrent queries over resident     */
 /* columns.  No counterpart in the reference (its Decoder is             */
 /* single-threaded and synchronous, api/decoder.rs:202-439); each query  */

@@ -1090,3 +1090,224 @@ int orc_sum_compensated(const uint8_t *file, size_t len, const uint64_t *bs, int
     file_free(&c.f);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* ML queries and multi-column aggregations                            */
+/* ------------------------------------------------------------------ */
+
+/* One decoded chunk (the reference's GeneralChunkReader for chunk k). */
+static int chunk_values(ctx_t *c, size_t k, int32_t precision, dvec *d) {
+    memset(d, 0, sizeof *d);
+    return chunk_decompress_p(c->f.scheme, c->b + c->f.phys[k], c->f.size[k], c->f.nrec[k], precision, d);
+}
+
+/* default_distance_squared / default_dot_product, decoder/query.rs:157-192:
+ * decompressed[min(len, offset)..] zipped with target, folded from -0.0 as
+ * core's f64 Sum does. */
+static double fold_zip(const dvec *d, size_t offset, const double *t, size_t tn, int dot) {
+    size_t o = offset < d->n ? offset : d->n;
+    size_t cnt = d->n - o < tn ? d->n - o : tn;
+    double s = -0.0;
+    for (size_t i = 0; i < cnt; ++i) {
+        double a = d->v[o + i], b = t[i];
+        if (dot) s += a * b;
+        else { double x = a - b; s += x * x; }
+    }
+    return s;
+}
+
+int orc_chunk_fold(const uint8_t *file, size_t len, uint64_t chunk_id, size_t offset, const double *target,
+                   size_t tn, int dot, double *out) {
+    ctx_t c;
+    int rc = ctx_open(&c, file, len);
+    if (rc) return rc;
+    if (chunk_id >= c.f.n_chunks) { file_free(&c.f); return ORC_CORRUPT; }
+    dvec d;
+    rc = chunk_values(&c, chunk_id, -1, &d);
+    if (!rc) *out = fold_zip(&d, offset, target, tn, dot);
+    free(d.v);
+    file_free(&c.f);
+    return rc;
+}
+
+/* Decoder::knn1, api/decoder.rs:469-533, statement by statement (including
+ * `offset += remaining_records` after remaining_records was reset, :503-505). */
+int orc_knn1(const uint8_t *file, size_t len, const double *target, size_t tlen, uint64_t *index, double *dist) {
+    ctx_t c;
+    int rc = ctx_open(&c, file, len);
+    if (rc) return rc;
+    if (tlen == 0) { file_free(&c.f); return ORC_CORRUPT; } /* the loop would never end */
+    uint64_t pidx = 0;
+    double pdist = 0.0;
+    uint64_t midx = UINT64_MAX;
+    double mdist = INFINITY;
+    size_t remaining = 0;
+    for (size_t k = 0; k < c.f.n_chunks && !rc; ++k) {
+        size_t offset = 0, n = (size_t)c.f.nrec[k];
+        dvec d;
+        rc = chunk_values(&c, k, -1, &d);
+        if (rc) { free(d.v); break; }
+        if (remaining != 0) {
+            size_t start = tlen - remaining;
+            pdist += fold_zip(&d, 0, target + start, tlen - start, 0);
+            if (pdist < mdist) { midx = pidx; mdist = pdist; }
+            remaining = 0;
+            pidx += 1;
+            offset += remaining;
+        }
+        while (offset < n) {
+            size_t nin = tlen < n - offset ? tlen : n - offset;
+            pdist = fold_zip(&d, offset, target, nin, 0);
+            if (nin == tlen) {
+                if (pdist <= mdist) { midx = pidx; mdist = pdist; }
+                pidx += 1;
+            } else {
+                remaining = tlen - nin;
+            }
+            offset += nin;
+        }
+        free(d.v);
+    }
+    *index = midx;
+    *dist = mdist;
+    file_free(&c.f);
+    return rc;
+}
+
+/* Decoder::matmul, api/decoder.rs:606-656 (target column-first). */
+int orc_matmul_dot(const uint8_t *file, size_t len, const double *target, size_t trows, size_t tcols,
+                   size_t drows, size_t dcols, double *out) {
+    ctx_t c;
+    int rc = ctx_open(&c, file, len);
+    if (rc) return rc;
+    if (!drows || !dcols) { file_free(&c.f); return ORC_CORRUPT; }
+    if (dcols != trows || c.f.n_records % (drows * dcols) != 0) { file_free(&c.f); return 3; }
+    uint64_t batch = c.f.n_records / (drows * dcols);
+    size_t off = 0, ci = 0, no = 0;
+    dvec d;
+    if (c.f.n_chunks == 0) { file_free(&c.f); return batch ? 3 : ORC_OK; }
+    rc = chunk_values(&c, 0, -1, &d);
+    for (uint64_t b = 0; b < batch && !rc; ++b) {
+        if (off >= c.f.nrec[ci]) {
+            off = 0;
+            ci += 1;
+            free(d.v);
+            if (ci >= c.f.n_chunks) { d.v = NULL; rc = 3; break; }
+            rc = chunk_values(&c, ci, -1, &d);
+            if (rc) break;
+        }
+        for (size_t r = 0; r < drows; ++r)
+            for (size_t j = 0; j < tcols; ++j)
+                out[no++] = fold_zip(&d, off + r * dcols, target + j * trows, trows, 1);
+        off += drows * dcols;
+    }
+    free(d.v);
+    file_free(&c.f);
+    return rc;
+}
+
+/* Expr::compute_chunk (decoder/expr.rs:67-123) over postfix nodes (kind 0
+ * column, 1 literal, 2 add, 3 sub, 4 mul, 5 div), recursively as the
+ * reference: a column is its chunk's records selected by the bitmask, a
+ * binary node zips its operands (or maps a literal over one). */
+typedef struct { int32_t kind; uint32_t index; double literal; } orc_node;
+typedef struct { double *v; size_t n; int lit; double litv; } orc_val;
+
+static int expr_eval(const orc_node *nodes, size_t n_nodes, dvec *chunk_sel, orc_val *res) {
+    orc_val st[64];
+    size_t sp = 0;
+    for (size_t p = 0; p < n_nodes; ++p) {
+        const orc_node *e = &nodes[p];
+        if (e->kind == 0) {
+            orc_val v = {malloc((chunk_sel[e->index].n ? chunk_sel[e->index].n : 1) * sizeof(double)),
+                         chunk_sel[e->index].n, 0, 0.0};
+            memcpy(v.v, chunk_sel[e->index].v, v.n * sizeof(double));
+            st[sp++] = v;
+        } else if (e->kind == 1) {
+            orc_val v = {NULL, 0, 1, e->literal};
+            st[sp++] = v;
+        } else {
+            orc_val b = st[--sp], a = st[--sp];
+            if (a.lit && b.lit) return 3; /* PreconditionsNotMet */
+            size_t n = a.lit ? b.n : b.lit ? a.n : (a.n < b.n ? a.n : b.n);
+            orc_val r = {malloc((n ? n : 1) * sizeof(double)), n, 0, 0.0};
+            for (size_t i = 0; i < n; ++i) {
+                double x = a.lit ? a.litv : a.v[i], y = b.lit ? b.litv : b.v[i];
+                r.v[i] = e->kind == 2 ? x + y : e->kind == 3 ? x - y : e->kind == 4 ? x * y : x / y;
+            }
+            free(a.v);
+            free(b.v);
+            st[sp++] = r;
+        }
+    }
+    *res = st[0];
+    return res->lit ? 3 : ORC_OK;
+}
+
+/* Aggregation::compute_multiple (api/decoder.rs:874-917) of one
+ * aggregation over files[0..n_files): kind 0 none (values), 1 max, 2 min,
+ * 3 sum.  A bare column takes the decoder's own sum / min / max.  sum_comp:
+ * a compensated (Neumaier) sum of the same records. */
+int orc_aggregate(const uint8_t *const *files, const size_t *lens, size_t n_files, const void *nodes_v,
+                  size_t n_nodes, int kind, const uint64_t *bs, double *out_scalar, double *sum_comp,
+                  double *out_values, size_t cap, size_t *n_out) {
+    const orc_node *nodes = (const orc_node *)nodes_v;
+    if (n_nodes == 1 && nodes[0].kind == 0) {
+        if (kind == 0) return ORC_UNSUPPORTED;
+        int op = kind == 3 ? 0 : kind == 2 ? 1 : 2;
+        int rc = orc_agg(files[nodes[0].index], lens[nodes[0].index], bs, -1, op, out_scalar);
+        if (!rc && sum_comp) rc = orc_sum_compensated(files[nodes[0].index], lens[nodes[0].index], bs, -1, sum_comp);
+        return rc;
+    }
+    ctx_t *cs = calloc(n_files, sizeof(ctx_t));
+    int rc = ORC_OK;
+    for (size_t i = 0; i < n_files && !rc; ++i) rc = ctx_open(&cs[i], files[i], lens[i]);
+    size_t nc = rc ? 0 : (size_t)cs[0].f.n_chunks;
+    double result = kind == 1 ? -INFINITY : kind == 2 ? INFINITY : 0.0;
+    double s = 0.0, comp = 0.0;
+    size_t no = 0;
+    for (size_t k = 0; k < nc && !rc; ++k) {
+        dvec *sel = calloc(n_files, sizeof(dvec));
+        for (size_t i = 0; i < n_files && !rc; ++i) {
+            if (k >= cs[i].f.n_chunks) { rc = 3; break; }
+            dvec d;
+            rc = chunk_values(&cs[i], k, -1, &d);
+            for (size_t j = 0; !rc && j < d.n; ++j) /* default_materialize :15-34 */
+                if (in_mask(bs, cs[i].f.n_records, cs[i].f.logi[k] + j)) dpush(&sel[i], d.v[j]);
+            free(d.v);
+        }
+        orc_val r = {0};
+        if (!rc) rc = expr_eval(nodes, n_nodes, sel, &r);
+        if (!rc) {
+            if (kind == 0) {
+                for (size_t j = 0; j < r.n && !rc; ++j) {
+                    if (no >= cap) rc = ORC_CAPACITY;
+                    else out_values[no++] = r.v[j];
+                }
+            } else if (kind == 3) {
+                double cs_ = -0.0;  /* iter().sum::<f64>() */
+                for (size_t j = 0; j < r.n; ++j) {
+                    cs_ += r.v[j];
+                    double x = r.v[j], t = s + x;
+                    if (fabs(s) >= fabs(x)) comp += (s - t) + x;
+                    else comp += (x - t) + s;
+                    s = t;
+                }
+                result += cs_;
+            } else {
+                double m = kind == 1 ? -INFINITY : INFINITY;
+                for (size_t j = 0; j < r.n; ++j) m = kind == 1 ? rmax(m, r.v[j]) : rmin(m, r.v[j]);
+                result = kind == 1 ? rmax(result, m) : rmin(result, m);
+            }
+        }
+        free(r.v);
+        for (size_t i = 0; i < n_files; ++i) free(sel[i].v);
+        free(sel);
+    }
+    if (kind == 0) *n_out = no;
+    else *out_scalar = result;
+    if (sum_comp) *sum_comp = s + comp;
+    for (size_t i = 0; i < n_files; ++i) file_free(&cs[i].f);
+    free(cs);
+    return rc;
+}

@@ -70,6 +70,23 @@ int orc_max(const uint8_t *file, size_t len, const uint64_t *bitset, int64_t chu
 int orc_sum_compensated(const uint8_t *file, size_t len, const uint64_t *bitset, int64_t chunk_id,
                         double *out);
 
+/* GeneralChunkReader::{distance_squared (dot = 0), dot_product (dot = 1)}
+ * on chunk chunk_id (decoder/query.rs:157-192). */
+int orc_chunk_fold(const uint8_t *file, size_t len, uint64_t chunk_id, size_t offset, const double *target,
+                   size_t tlen, int dot, double *out);
+/* Decoder::knn1 (api/decoder.rs:469-533). */
+int orc_knn1(const uint8_t *file, size_t len, const double *target, size_t tlen, uint64_t *index, double *dist);
+/* Decoder::matmul (api/decoder.rs:606-656); status 3 = PreconditionsNotMet. */
+int orc_matmul_dot(const uint8_t *file, size_t len, const double *target, size_t trows, size_t tcols,
+                   size_t drows, size_t dcols, double *out);
+/* Aggregation::compute_multiple of one aggregation (api/decoder.rs:874-917,
+ * decoder/expr.rs:67-123); nodes as ebi_expr_node; kind 0 none, 1 max, 2 min,
+ * 3 sum; status 3 = PreconditionsNotMet. */
+typedef struct orc_node_ { int32_t kind; uint32_t index; double literal; } orc_node_t;
+int orc_aggregate(const uint8_t *const *files, const size_t *lens, size_t n_files, const void *nodes,
+                  size_t n_nodes, int kind, const uint64_t *bitset, double *out_scalar, double *sum_comp,
+                  double *out_values, size_t cap, size_t *n_out);
+
 /* Helpers exposed for known-answer tests. */
 uint64_t orc_zigzag(int64_t v);
 int64_t orc_unzigzag(uint64_t v);

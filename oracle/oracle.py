@@ -58,6 +58,12 @@ def lib():
                                         C.POINTER(C.c_size_t)]
         L.orc_filter_materialize_p.argtypes = [P, C.c_size_t, C.POINTER(_Pred), P, C.c_int64, C.c_int32, P,
                                                C.c_size_t, C.POINTER(C.c_size_t)]
+        L.orc_chunk_fold.argtypes = [P, C.c_size_t, C.c_uint64, C.c_size_t, P, C.c_size_t, C.c_int,
+                                     C.POINTER(C.c_double)]
+        L.orc_knn1.argtypes = [P, C.c_size_t, P, C.c_size_t, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+        L.orc_matmul_dot.argtypes = [P, C.c_size_t, P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, P]
+        L.orc_aggregate.argtypes = [P, P, C.c_size_t, P, C.c_size_t, C.c_int, P, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), P, C.c_size_t, C.POINTER(C.c_size_t)]
         L.orc_file_records.argtypes = [P, C.c_size_t, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.orc_zigzag.restype = C.c_uint64
         L.orc_zigzag.argtypes = [C.c_int64]
@@ -164,3 +170,64 @@ class OracleDecoder:
 
     def sum_compensated(self, bitmask=None, chunk_id=None):
         return self._agg(lib().orc_sum_compensated, bitmask, chunk_id, "sum_compensated")
+
+
+    # ---- ML queries (api/decoder.rs:469-656, decoder/query.rs:157-192) ----
+    def knn1(self, target):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        idx, dist = C.c_uint64(), C.c_double()
+        _ck(lib().orc_knn1(self._p, self._n, t.ctypes.data, t.size, C.byref(idx), C.byref(dist)), "knn1")
+        return int(idx.value), float(dist.value)
+
+    def _fold(self, chunk_id, offset, target, dot):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        r = C.c_double()
+        _ck(lib().orc_chunk_fold(self._p, self._n, int(chunk_id), int(offset), t.ctypes.data, t.size, dot,
+                                 C.byref(r)), "chunk fold")
+        return float(r.value)
+
+    def distance_squared(self, chunk_id, offset, target):
+        return self._fold(chunk_id, offset, target, 0)
+
+    def dot_product(self, chunk_id, offset, target):
+        return self._fold(chunk_id, offset, target, 1)
+
+    def matmul(self, target, target_shape, data_shape):
+        """Decoder::matmul: target column-first (target_shape[1] columns of
+        target_shape[0] values); returns the flat row-first products."""
+        tr, tc = (int(x) for x in target_shape)
+        dr, dc = (int(x) for x in data_shape)
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        batch = self.number_of_records // (dr * dc) if dr * dc else 0
+        out = np.zeros(max(batch * dr * tc, 1), dtype=np.float64)
+        _ck(lib().orc_matmul_dot(self._p, self._n, t.ctypes.data, tr, tc, dr, dc, out.ctypes.data), "matmul")
+        return out[: batch * dr * tc]
+
+
+class _Node(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_uint32), ("literal", C.c_double)]
+
+
+def aggregate(files, nodes, kind, bitmask=None):
+    """Aggregation::compute of one aggregation over several EBI1 files:
+    nodes = postfix [(kind, index, literal)], kind 0 none / 1 max / 2 min /
+    3 sum.  Returns (result, compensated_sum) -- result an array for kind 0."""
+    bufs = [np.frombuffer(f, dtype=np.uint8) for f in files]
+    ptrs = (C.c_void_p * len(bufs))(*[b.ctypes.data for b in bufs])
+    lens = (C.c_size_t * len(bufs))(*[len(b) for b in bufs])
+    arr = (_Node * len(nodes))(*[_Node(int(k), int(i), float(v)) for k, i, v in nodes])
+    n_max = max(OracleDecoder(f).number_of_records for f in files)
+    words = None
+    if bitmask is not None:
+        words = np.zeros((n_max + 63) // 64 + 1, dtype=np.uint64)
+        ids = np.asarray(list(bitmask), dtype=np.int64)
+        if ids.size:
+            np.bitwise_or.at(words, ids >> 6, np.left_shift(np.uint64(1), (ids & 63).astype(np.uint64)))
+    out = np.zeros(max(n_max, 1), dtype=np.float64)
+    sc, comp, n = C.c_double(), C.c_double(), C.c_size_t()
+    _ck(lib().orc_aggregate(ptrs, lens, len(bufs), arr, len(nodes), int(kind),
+                            None if words is None else words.ctypes.data, C.byref(sc), C.byref(comp),
+                            out.ctypes.data, out.size, C.byref(n)), "aggregate")
+    if kind == 0:
+        return out[: n.value].copy(), float(comp.value)
+    return float(sc.value), float(comp.value)

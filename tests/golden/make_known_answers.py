@@ -7,6 +7,10 @@ Sources (lemolatoon/ebi):
   sum / max / min ebi/tests/aggregation.rs:195-251, 303-316, 351-364
   matmul_cuda     ebi/tests/ml_query.rs:455-576 (run on uncompressed, rle,
                   gorilla, chimp, chimp128, elf: ml_query.rs:605-617)
+  knn1            ebi/tests/ml_query.rs:266-290 (same codecs, and ElfOnChimp)
+  matmul (CPU)    ebi/tests/ml_query.rs:292-450 (target column-first)
+  expr            ebi/tests/aggregation.rs:472-750 (uncompressed, 3 records
+                  per chunk: aggregation.rs:172-188)
 The reference runs the filter / materialize vectors on every codec, with BUFF
 at scale 100 (filters) or 10 (materialize) and DeltaSprintz at scale 10
 (query.rs:1280-1367); aggregations use scale 100 for the scaled codecs.
@@ -135,6 +139,76 @@ MATMUL = [
 ]
 MATMUL_CODECS = ["uncompressed", "rle", "gorilla", "chimp", "chimp128", "elf"]
 
+# knn1: data, chunk records, target, expected index (ml_query.rs:266-290)
+KNN_DATA = [1.0, 2.0, 3.0, 4.0, 5.0, 6.0]
+KNN1 = [{"name": f"Test {t}: {i}", "data": KNN_DATA, "chunk": RC(c), "target": KNN_DATA[2 * i:2 * i + 2],
+         "expected": i} for t, c in ((1, 2), (2, 3)) for i in range(3)]
+
+
+def mmc(name, data, chunk, target, expected):
+    """data: batch x H x W; target: columns (target_w lists of target_h
+    values, column-first); expected: batch x H x target_w."""
+    return {"name": name, "data": data, "chunk": RC(chunk), "target": target, "expected": expected}
+
+
+MATMUL_CPU = [
+    mmc("Test 1: Simple matmul", [[[1.0, 2.0], [3.0, 4.0], [0.0, 1.0]], [[3.0, 2.0], [-1.0, 8.0], [0.0, 1.0]]], 6,
+        [[1.0, 1.0], [5.0, 4.0], [6.0, 3.0], [2.0, 1.0]],
+        [[[3.0, 13.0, 12.0, 4.0], [7.0, 31.0, 30.0, 10.0], [1.0, 4.0, 3.0, 1.0]],
+         [[5.0, 23.0, 24.0, 8.0], [7.0, 27.0, 18.0, 6.0], [1.0, 4.0, 3.0, 1.0]]]),
+    mmc("Test 2: Updated matmul with four matrices",
+        [[[1.0, 2.0], [3.0, 4.0], [0.0, 1.0]], [[3.0, 2.0], [-1.0, 8.0], [0.0, 1.0]],
+         [[2.0, 1.0], [0.0, 3.0], [1.0, 0.0]], [[4.0, -1.0], [2.0, 2.0], [0.0, 0.5]]], 6,
+        [[1.0, 1.0], [5.0, 4.0], [6.0, 3.0], [2.0, 1.0]],
+        [[[3.0, 13.0, 12.0, 4.0], [7.0, 31.0, 30.0, 10.0], [1.0, 4.0, 3.0, 1.0]],
+         [[5.0, 23.0, 24.0, 8.0], [7.0, 27.0, 18.0, 6.0], [1.0, 4.0, 3.0, 1.0]],
+         [[3.0, 14.0, 15.0, 5.0], [3.0, 12.0, 9.0, 3.0], [1.0, 5.0, 6.0, 2.0]],
+         [[3.0, 16.0, 21.0, 7.0], [4.0, 18.0, 18.0, 6.0], [0.5, 2.0, 1.5, 0.5]]]),
+    mmc("Test 3: Updated matmul with 5x3 matrices and target 3x9",
+        [[[1.0, 2.0, 3.0], [4.0, 5.0, 6.0], [7.0, 8.0, 9.0], [0.0, 1.0, 2.0], [3.0, 4.0, 5.0]],
+         [[2.0, 1.0, 0.0], [3.0, 2.0, 1.0], [4.0, 3.0, 2.0], [5.0, 4.0, 3.0], [6.0, 5.0, 4.0]],
+         [[0.0, 1.0, 2.0], [1.0, 0.0, 3.0], [2.0, 1.0, 4.0], [3.0, 2.0, 5.0], [4.0, 3.0, 6.0]],
+         [[3.0, 2.0, 1.0], [4.0, 3.0, 2.0], [5.0, 4.0, 3.0], [6.0, 5.0, 4.0], [7.0, 6.0, 5.0]]], 30,
+        [[float(k * j) for j in (1, 2, 3)] for k in range(1, 10)],
+        [[[14.0 * m for m in range(1, 10)], [32.0 * m for m in range(1, 10)], [50.0 * m for m in range(1, 10)],
+          [8.0 * m for m in range(1, 10)], [26.0 * m for m in range(1, 10)]],
+         [[4.0 * m for m in range(1, 10)], [10.0 * m for m in range(1, 10)], [16.0 * m for m in range(1, 10)],
+          [22.0 * m for m in range(1, 10)], [28.0 * m for m in range(1, 10)]],
+         [[8.0 * m for m in range(1, 10)], [10.0 * m for m in range(1, 10)], [16.0 * m for m in range(1, 10)],
+          [22.0 * m for m in range(1, 10)], [28.0 * m for m in range(1, 10)]],
+         [[10.0 * m for m in range(1, 10)], [16.0 * m for m in range(1, 10)], [22.0 * m for m in range(1, 10)],
+          [28.0 * m for m in range(1, 10)], [34.0 * m for m in range(1, 10)]]]),
+]
+
+# expressions in postfix form: ["i", k] column k, ["l", v] literal, ["+"|"-"|"*"|"/"] operators
+def ex(name, columns, aggs, expected, bitmask=None):
+    return {"name": name, "columns": columns, "aggs": aggs, "expected": expected, "bitmask": bitmask}
+
+
+A01 = [["i", 0], ["i", 1], ["*"], ["i", 1], ["+"]]
+EXPR = [
+    ex("test_expr1", [[0.0, 2.0, 3.0, 4.0, 5.0]], [["sum", [["i", 0]]]], [14.0]),
+    ex("test_expr2", [[0.0, 2.0, -3.0, 4.0, 7.0], [1.0, 2.0, 3.0, 4.0, 5.0]], [["sum", A01]], [61.0]),
+    ex("test_expr3", [[0.0, 2.0, 3.0, 4.0, 5.0]], [["sum", [["i", 0]]]], [14.0]),
+    ex("test_expr4", [[0.0, 2.0, -3.0, 4.0, 7.0], [1.0, 2.0, 3.0, 4.0, 5.0]], [["sum", A01]], [61.0]),
+    ex("test_expr5", [[1.0, 5.0, 2.0, 4.0]], [["max", [["i", 0]]]], [5.0]),
+    ex("test_expr6", [[1.0, 2.0, 3.0], [2.0, 1.0, 4.0]], [["max", [["i", 0], ["i", 1], ["*"]]]], [12.0]),
+    ex("test_expr7", [[1.0, 5.0, 2.0, 4.0]], [["min", [["i", 0]]]], [1.0]),
+    ex("test_expr8", [[3.0, 1.0, 4.0], [2.0, 2.0, 1.0]], [["min", [["i", 0], ["i", 1], ["-"]]]], [-1.0]),
+    ex("test_expr9", [[0.0, 0.8, 0.3], [1.0, 1.0, 20.0]], [["sum", [["i", 0]]]], [0.8], bitmask=[0, 1]),
+    ex("test_expr10", [[1.0, 2.2, 5.0, 4.0]], [["sum", [["i", 0]]]], [6.0], bitmask=[0, 2]),
+    ex("test_multiple_sum_index_and_binary", [[1.0, 2.0, 3.0], [10.0, 20.0, 30.0]],
+       [["sum", [["i", 0]]], ["sum", [["i", 0], ["i", 1], ["+"]]]], [6.0, 66.0]),
+    ex("test_multiple_max_index_and_binary", [[3.0, 8.0, 5.0], [2.0, 1.0, 7.0]],
+       [["max", [["i", 1]]], ["max", [["i", 0], ["i", 1], ["*"]]]], [7.0, 35.0]),
+    ex("test_multiple_min_index_and_binary", [[6.0, 2.0, 9.0], [1.0, 2.0, 3.0]],
+       [["min", [["i", 0]]], ["min", [["i", 0], ["i", 1], ["-"]]]], [2.0, 0.0]),
+    ex("test_multiple_count_index_and_binary", [[1.0, 0.0, 3.0, 4.0], [5.0, 6.0, 7.0, 8.0]],
+       [["sum", [["i", 0]]], ["max", [["i", 0], ["i", 1], ["+"]]]], [4.0, 10.0], bitmask=[0, 2]),
+]
+EXPR_CHUNK = RC(3)
+ML_CODECS = ["uncompressed", "rle", "gorilla", "chimp", "chimp128", "elf_on_chimp", "elf"]
+
 # scales the reference tests use per codec
 SCALES = {"filter": {"buff": 100, "delta_sprintz": 10}, "materialize": {"buff": 10, "delta_sprintz": 10},
           "aggregate": {"buff": 100, "delta_sprintz": 100}}
@@ -142,5 +216,7 @@ SCALES = {"filter": {"buff": 100, "delta_sprintz": 10}, "materialize": {"buff": 
 if __name__ == "__main__":
     out = Path(__file__).resolve().parent / "known_answers.json"
     out.write_text(json.dumps({"filter": FILTER, "materialize": MATERIALIZE, "aggregate": AGGREGATE,
-                               "matmul": MATMUL, "matmul_codecs": MATMUL_CODECS, "scales": SCALES}, indent=1))
+                               "matmul": MATMUL, "matmul_codecs": MATMUL_CODECS, "scales": SCALES,
+                               "knn1": KNN1, "matmul_cpu": MATMUL_CPU, "expr": EXPR, "expr_chunk": EXPR_CHUNK,
+                               "ml_codecs": ML_CODECS}, indent=1))
     print(out)
