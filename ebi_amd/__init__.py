@@ -182,6 +182,10 @@ _SIGS = {
     "ebi_hip_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "ebi_hip_device_free": (C.c_int, [_P, _P]),
     "ebi_hip_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "ebi_hip_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "ebi_hip_encode_device": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(_EncCfg), C.POINTER(_P),
+                                        C.POINTER(C.c_size_t)]),
+    "ebi_hip_column_encode": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(_EncCfg), C.POINTER(_P)]),
     "ebi_hip_device_zero": (C.c_int, [_P, _P, C.c_size_t]),
     "ebi_hip_materialize": (C.c_int, [_P, _P, _P, C.c_int64, _P, C.POINTER(C.c_size_t)]),
     "ebi_hip_filter": (C.c_int, [_P, _P, C.POINTER(_Pred), _P, C.c_int64, _P]),
@@ -499,6 +503,31 @@ class Context:
         _check(lib().ebi_hip_device_alloc(self._h, int(nbytes), C.byref(p)), self, "device_alloc")
         return DeviceBuffer(self, p, int(nbytes))
 
+    def upload(self, values) -> "DeviceBuffer":
+        """f64 values copied into a new device buffer."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        buf = self.alloc(max(v.nbytes, 8))
+        _check(lib().ebi_hip_copy_to_device(self._h, buf.ptr, v.ctypes.data, v.nbytes), self, "copy_to_device")
+        return buf
+
+
+def _enc_cfg(config: "CompressorConfig", chunk_option: "ChunkOption") -> "_EncCfg":
+    return _EncCfg(config.scheme, chunk_option.kind, chunk_option.value, config.capacity, config.scale, 1, 0)
+
+
+def encode_device(ctx: Context, values: "DeviceBuffer", n: int, config: "CompressorConfig",
+                  chunk_option: "ChunkOption") -> np.ndarray:
+    """ebi_hip_encode_device: n f64 values already in device memory encoded
+    by GPU kernels (BUFF, DeltaSprintz); the EBI1 file as a uint8 array."""
+    cfg = _enc_cfg(config, chunk_option)
+    out, ln = _P(), C.c_size_t()
+    L = lib()
+    _check(L.ebi_hip_encode_device(ctx._h, values.ptr, int(n), C.byref(cfg), C.byref(out), C.byref(ln)), ctx,
+           "encode_device")
+    arr = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint8)), shape=(ln.value,))
+    weakref.finalize(arr, L.ebi_buffer_free, out.value)
+    return arr
+
 
 class DeviceBuffer:
     def __init__(self, ctx: Context, ptr, nbytes: int):
@@ -535,17 +564,32 @@ class Decoder:
     arrays.  ``chunk_id`` selects one chunk (ChunkId, api/decoder.rs:97-106).
     """
 
-    def __init__(self, ctx: Context, data: bytes):
+    def __init__(self, ctx: Context, data: bytes = None, _handle=None):
         self.ctx = ctx
-        self._data = np.frombuffer(data, dtype=np.uint8)
-        h = _P()
-        _check(lib().ebi_hip_column_load(ctx._h, self._data.ctypes.data, len(data), C.byref(h)), ctx, "column_load")
+        if _handle is not None:
+            h = _handle
+        else:
+            self._data = np.frombuffer(data, dtype=np.uint8)
+            h = _P()
+            _check(lib().ebi_hip_column_load(ctx._h, self._data.ctypes.data, len(data), C.byref(h)), ctx,
+                   "column_load")
         self._h = h
         fi = _FileInfo()
         _check(lib().ebi_hip_column_info(h, C.byref(fi)), ctx, "column_info")
         self.number_of_records = int(fi.number_of_records)
         self.number_of_chunks = int(fi.number_of_chunks)
         self.scheme = int(fi.scheme)
+
+    @staticmethod
+    def encode_on_device(ctx: Context, values: "DeviceBuffer", n: int, config: "CompressorConfig",
+                         chunk_option: "ChunkOption") -> "Decoder":
+        """ebi_hip_column_encode: device values encoded by GPU kernels and
+        loaded as a resident column."""
+        cfg = _enc_cfg(config, chunk_option)
+        h = _P()
+        _check(lib().ebi_hip_column_encode(ctx._h, values.ptr, int(n), C.byref(cfg), C.byref(h)), ctx,
+               "column_encode")
+        return Decoder(ctx, _handle=h)
 
     def close(self):
         if getattr(self, "_h", None) and self.ctx._h:

@@ -213,6 +213,19 @@ struct ExprProgram {
   uint32_t n;
 };
 
+// GPU encoder (BUFF, DeltaSprintz): per-chunk statistics (BUFF: a = min and
+// b = max fixed-point value; DeltaSprintz: b = OR of the zigzagged deltas)
+// and the layout the host derives from them (body offset in the file body,
+// BUFF base / DeltaSprintz unused, bit width B or W, BUFF fractional bits).
+struct EncStat {
+  int64_t a, b;
+};
+struct EncChunk {
+  uint64_t off;
+  int64_t base;
+  uint32_t bits, frac;
+};
+
 // Device error flags (OR-ed by kernels).
 enum DevError : uint32_t {
   kErrNone = 0,

@@ -443,7 +443,7 @@ std::vector<uint8_t> enc_elf_on_chimp(const double *v, size_t n) { return enc_el
 // ------------------------------------------------------------------- BUFF
 const uint64_t kPrecisionMap[13] = {0, 5, 8, 11, 15, 18, 21, 25, 28, 31, 35, 38, 50};
 
-int buff_frac_bits(uint64_t scale, uint32_t *frac) {
+int buff_frac_bits_impl(uint64_t scale, uint32_t *frac) {
   size_t precision = 0;
   if (scale != 0) {
     float l = std::log10((float)scale);
@@ -582,6 +582,71 @@ int encode_chunk(int scheme, const double *v, size_t n, const ebi_encoder_config
 
 }  // namespace
 
+int buff_frac_bits(uint64_t scale, uint32_t *frac) { return buff_frac_bits_impl(scale, frac); }
+
+int encode_geometry(const ebi_encoder_config &cfg, size_t n, EncodeGeom *g) {
+  // Chunk partition, encoder.rs:317-346.
+  size_t per_chunk;
+  switch (cfg.chunk_kind) {
+    case EBI_CHUNK_FULL: per_chunk = n ? n : 1; break;
+    case EBI_CHUNK_RECORD_COUNT: per_chunk = (size_t)cfg.chunk_value; break;
+    case EBI_CHUNK_BYTE_SIZE:
+      per_chunk = (size_t)std::ceil(((double)cfg.chunk_value / 8.0) * 1.5);
+      break;
+    default: return EBI_INVALID_ARGUMENT;
+  }
+  if (per_chunk == 0) return EBI_INVALID_ARGUMENT;
+  g->per_chunk = per_chunk;
+  g->n_chunks = (n + per_chunk - 1) / per_chunk;
+  return EBI_OK;
+}
+
+size_t encoded_file_size(int scheme, uint64_t body, size_t n_chunks) {
+  return kFileHeaderSize + body + kFooter0Size + n_chunks * kChunkFooterSize + 1 + config_size_of(scheme) +
+         kFooter3Size;
+}
+
+void write_header_footer(uint8_t *buf, const ebi_encoder_config &cfg, size_t n, const EncodeGeom &g,
+                         const std::vector<uint64_t> &phys) {
+  const int scheme = cfg.scheme;
+  const uint64_t body = phys[g.n_chunks] - kFileHeaderSize;
+  const int cfg_size = config_size_of(scheme);
+  // FileHeader, written with footer_offset already known (the reference
+  // back-patches the same slot, encoder.rs:236-241).
+  std::vector<uint8_t> h;
+  h.insert(h.end(), {'E', 'B', 'I', '1'});
+  h.push_back(0);
+  h.push_back((uint8_t)scheme);
+  h.push_back((uint8_t)cfg.chunk_kind);
+  h.push_back(0);
+  put_u64(h, cfg.chunk_kind == EBI_CHUNK_FULL ? 0 : cfg.chunk_value);
+  put_u64(h, kFileHeaderSize + body);
+  for (int i = 0; i < 3; ++i) put_u16(h, kVersion[i]);
+  std::memcpy(buf, h.data(), h.size());
+  std::vector<uint8_t> f;
+  f.reserve(kFooter0Size + g.n_chunks * kChunkFooterSize + 1 + cfg_size + kFooter3Size);
+  put_u64(f, n);
+  put_u64(f, g.n_chunks);
+  for (size_t c = 0; c < g.n_chunks; ++c) {
+    put_u64(f, phys[c]);
+    put_u64(f, c * g.per_chunk);
+  }
+  f.push_back((uint8_t)cfg_size);
+  const uint64_t capacity = cfg.capacity ? cfg.capacity : kDefaultCapacity;
+  if (scheme == kBUFF) {
+    put_u64(f, cfg.scale);
+  } else if (scheme == kDeltaSprintz) {
+    put_u64(f, capacity);
+    put_u64(f, cfg.scale);
+  } else {
+    put_u64(f, capacity);
+  }
+  // FileFooter3: compression time, 9 phase timers, crc -- all zero here so
+  // that files are byte-reproducible (the reference stores wall-clock times).
+  f.resize(f.size() + kFooter3Size, 0);
+  std::memcpy(buf + kFileHeaderSize + body, f.data(), f.size());
+}
+
 // ----------------------------------------------------------------- parser
 int parse_file(const uint8_t *file, size_t len, FileMeta *m, std::string *err) {
   if (len < kFileHeaderSize) {
@@ -668,18 +733,10 @@ extern "C" int ebi_encode(const double *values, size_t n, const ebi_encoder_conf
   uint32_t frac = 0;
   if (scheme == kBUFF && buff_frac_bits(cfg->scale, &frac) != 0) return EBI_INVALID_ARGUMENT;
 
-  // Chunk partition, encoder.rs:317-346.
-  size_t per_chunk;
-  switch (cfg->chunk_kind) {
-    case EBI_CHUNK_FULL: per_chunk = n ? n : 1; break;
-    case EBI_CHUNK_RECORD_COUNT: per_chunk = (size_t)cfg->chunk_value; break;
-    case EBI_CHUNK_BYTE_SIZE:
-      per_chunk = (size_t)std::ceil(((double)cfg->chunk_value / 8.0) * 1.5);
-      break;
-    default: return EBI_INVALID_ARGUMENT;
-  }
-  if (per_chunk == 0) return EBI_INVALID_ARGUMENT;
-  size_t n_chunks = (n + per_chunk - 1) / per_chunk;
+  EncodeGeom geom;
+  if (int rc = encode_geometry(*cfg, n, &geom)) return rc;
+  const size_t per_chunk = geom.per_chunk;
+  const size_t n_chunks = geom.n_chunks;
   std::vector<std::vector<uint8_t>> bodies(n_chunks);
   std::atomic<size_t> next{0};
   std::atomic<int> failed{0};
@@ -704,23 +761,9 @@ extern "C" int ebi_encode(const double *values, size_t n, const ebi_encoder_conf
   phys[0] = kFileHeaderSize;
   for (size_t c = 0; c < n_chunks; ++c) phys[c + 1] = phys[c] + bodies[c].size();
   const uint64_t body = phys[n_chunks] - kFileHeaderSize;
-  int cfg_size = config_size_of(scheme);
-  size_t total = kFileHeaderSize + body + kFooter0Size + n_chunks * kChunkFooterSize + 1 +
-                 cfg_size + kFooter3Size;
+  const size_t total = encoded_file_size(scheme, body, n_chunks);
   uint8_t *buf = (uint8_t *)std::malloc(total);
   if (!buf) return EBI_OOM;
-  // FileHeader, written with footer_offset already known (the reference
-  // back-patches the same slot, encoder.rs:236-241).
-  std::vector<uint8_t> h;
-  h.insert(h.end(), {'E', 'B', 'I', '1'});
-  h.push_back(0);
-  h.push_back((uint8_t)scheme);
-  h.push_back((uint8_t)cfg->chunk_kind);
-  h.push_back(0);
-  put_u64(h, cfg->chunk_kind == EBI_CHUNK_FULL ? 0 : cfg->chunk_value);
-  put_u64(h, kFileHeaderSize + body);
-  for (int i = 0; i < 3; ++i) put_u16(h, kVersion[i]);
-  std::memcpy(buf, h.data(), h.size());
   // chunk bodies, copied in parallel
   next = 0;
   auto copy = [&]() {
@@ -735,28 +778,7 @@ extern "C" int ebi_encode(const double *values, size_t n, const ebi_encoder_conf
   for (int t = 1; t < nt; ++t) pool.emplace_back(copy);
   copy();
   for (auto &t : pool) t.join();
-  std::vector<uint8_t> f;
-  f.reserve(kFooter0Size + n_chunks * kChunkFooterSize + 1 + cfg_size + kFooter3Size);
-  put_u64(f, n);
-  put_u64(f, n_chunks);
-  for (size_t c = 0; c < n_chunks; ++c) {
-    put_u64(f, phys[c]);
-    put_u64(f, c * per_chunk);
-  }
-  f.push_back((uint8_t)cfg_size);
-  uint64_t capacity = cfg->capacity ? cfg->capacity : kDefaultCapacity;
-  if (scheme == kBUFF) {
-    put_u64(f, cfg->scale);
-  } else if (scheme == kDeltaSprintz) {
-    put_u64(f, capacity);
-    put_u64(f, cfg->scale);
-  } else {
-    put_u64(f, capacity);
-  }
-  // FileFooter3: compression time, 9 phase timers, crc -- all zero here so
-  // that files are byte-reproducible (the reference stores wall-clock times).
-  f.resize(f.size() + kFooter3Size, 0);
-  std::memcpy(buf + kFileHeaderSize + body, f.data(), f.size());
+  write_header_footer(buf, *cfg, n, geom, phys);
   *out = buf;
   *out_len = total;
   return EBI_OK;

@@ -11,11 +11,15 @@
 //                          FileFooter3 (u128 + 9 x u128 + u32 crc = 164 B)
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include "../../../include/ebi_hip.h"
 
 namespace ebi {
 
@@ -102,8 +106,8 @@ inline int config_size_of(int scheme) {
 // mantissa shifted to `frac` fractional bits (truncating), negated for
 // negative values; 0 below 2^-frac.  Shift amounts wrap as in the reference's
 // release build.
-inline int64_t buff_fixed(double v, int32_t frac) {
-  const uint64_t b = f64_bits(v);
+__host__ __device__ inline int64_t buff_fixed(double v, int32_t frac) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
   const int32_t exp = (int32_t)((b >> 52) & 0x7ff) - 1023;
   if (exp < -frac) return 0;
   const int32_t sh = 63 - exp - frac;
@@ -133,5 +137,21 @@ struct FileMeta {
 
 // Returns 0 on success or an ebi_status code; *err receives a message.
 int parse_file(const uint8_t *file, size_t len, FileMeta *meta, std::string *err);
+
+// Write-path helpers shared by the host encoder and the GPU encoder.
+struct EncodeGeom {
+  size_t per_chunk = 0;  // records per chunk (the last chunk may hold fewer)
+  size_t n_chunks = 0;
+};
+// The chunk partition of encoder.rs:317-346 (0 or an ebi_status).
+int encode_geometry(const ebi_encoder_config &cfg, size_t n, EncodeGeom *g);
+// Total file bytes for chunk bodies of `body` bytes.
+size_t encoded_file_size(int scheme, uint64_t body, size_t n_chunks);
+// Writes the FileHeader at buf and the footer after the chunk bodies, which
+// start at phys[0] = 30 and end at phys[n_chunks].
+void write_header_footer(uint8_t *buf, const ebi_encoder_config &cfg, size_t n, const EncodeGeom &g,
+                         const std::vector<uint64_t> &phys);
+// BUFF fractional bits for a scale (compressor/buff.rs:131-135), 0 or an ebi_status.
+int buff_frac_bits(uint64_t scale, uint32_t *frac);
 
 }  // namespace ebi

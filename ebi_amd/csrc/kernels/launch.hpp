@@ -78,6 +78,14 @@ void expr(hipStream_t s, const ExprProgram &prog, const double *const *cols, con
           const uint64_t *len, const uint64_t *out_off, const uint2 *tiles, uint32_t n_tiles, int kind,
           double *partials, unsigned long long *acc, double *out);
 
+// GPU encoders (encode.hip): per-chunk statistics of n values (chunks of
+// per_chunk records), then the chunk bodies packed into `file` at each
+// EncChunk::off.  codec kCodecBUFF or kCodecSprintz.
+void enc_stats(hipStream_t s, const double *v, uint64_t n, uint64_t per_chunk, uint64_t nc, int codec, int32_t frac,
+               double scale, EncStat *stats);
+void enc_pack(hipStream_t s, const double *v, uint64_t n, uint64_t per_chunk, uint64_t nc, int codec,
+              const EncChunk *chunks, double scale, uint64_t scale_bits, uint8_t *file);
+
 // FP64 DGEMM on the matrix cores: C_b = A_b * B_b, row-major, b < batch,
 // element strides sA / sB / sC between batches (0 = shared operand).
 hipError_t dgemm(hipStream_t s, const double *A, const double *B, double *C, uint32_t m, uint32_t n, uint32_t k,

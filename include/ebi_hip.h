@@ -432,6 +432,22 @@ int ebi_hip_serve_destroy(ebi_hip_server *srv);
 int ebi_hip_device_alloc(ebi_hip_ctx *ctx, size_t bytes, void **d_ptr);
 int ebi_hip_device_free(ebi_hip_ctx *ctx, void *d_ptr);
 int ebi_hip_copy_to_host(ebi_hip_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int ebi_hip_copy_to_device(ebi_hip_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+
+/* The same write path on the GPU for BUFF and DeltaSprintz (the two codecs
+ * whose chunks are fixed-width packings): d_values are n f64 values in
+ * device memory (HBM); the chunk bodies are encoded by kernels and the file
+ * comes back in *out (free with ebi_buffer_free), byte-identical to
+ * ebi_encode's.  Other schemes return EBI_UNSUPPORTED.  Replaces
+ * ChunkWriter::write (encoder.rs:317-405) for those codecs. */
+int ebi_hip_encode_device(ebi_hip_ctx *ctx, const double *d_values, size_t n,
+                          const ebi_encoder_config *cfg, uint8_t **out, size_t *out_len);
+/* Encodes d_values on the GPU as ebi_hip_encode_device and loads the result
+ * as a resident column (e.g. a decoded column, changed or appended in HBM,
+ * re-encoded without moving its records through the host). */
+int ebi_hip_column_encode(ebi_hip_ctx *ctx, const double *d_values, size_t n,
+                          const ebi_encoder_config *cfg, ebi_hip_col **out);
+
 /* Zeroes `bytes` (a multiple of 4) of device memory with a fill kernel. */
 int ebi_hip_device_zero(ebi_hip_ctx *ctx, void *d_ptr, size_t bytes);
 
