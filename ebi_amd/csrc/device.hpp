@@ -226,6 +226,53 @@ struct EncChunk {
   uint32_t bits, frac;
 };
 
+// One query of a group as the group kernels see it (query plans, host/
+// plan.cpp): its arguments, its column, its scratch and what to run.
+enum PlanFlags : uint32_t {
+  kPfTiles = 1u << 0,    // record-tile pass (in-situ BUFF / DeltaSprintz, decoded records)
+  kPfSegs = 1u << 1,     // fused decode + query over the column's segment jobs
+  kPfPred = 1u << 2,     // evaluate the predicate (filter)
+  kPfBits = 1u << 3,     // write the selected records' bits to a.out_bits
+  kPfAggSel = 1u << 6,   // the aggregate runs over the predicate's selection (FILTER_SUM)
+  kPfFold = 1u << 7,     // per-chunk results folded on the device into result->scalar
+  kPfCompact = 1u << 8,  // materialize: compaction of a.values by a.out_bits into `out`
+  kPfMask = 1u << 9,     // a bitmask was given
+  kPfF64Sum = 1u << 10,  // the sum is over f64 records (double-double tile partials)
+};
+constexpr uint32_t kPfAggShift = 4;  // bits 4-5: 0 none, 1 sum, 2 min, 3 max
+
+struct alignas(16) PlanResult {
+  double scalar;   // SUM / MIN / MAX / FILTER_SUM
+  uint64_t n_out;  // MATERIALIZE / FILTER_MATERIALIZE through compaction
+};
+
+struct alignas(16) PlanDesc {
+  QueryArgs a;
+  const ColView *view;
+  const Job *rec_jobs;   // the column's record-tile jobs
+  double *chunk_sum;     // per chunk: (hi, lo) of an f64 sum, or (value, 0) to fold
+  double *out;           // compaction output
+  uint64_t *block;       // compaction: per-block counts, then offsets
+  PlanResult *result;
+  uint64_t words;        // ceil(N / 64)
+  int64_t chunk_id;
+  uint32_t n_chunks, n_rec_jobs, n_seg_jobs, n_blocks;
+  uint32_t flags;
+  int32_t filter_mode;   // 0 select by the bitmask, 1 code domain (BUFF / DeltaSprintz) or f64, 2 f64
+  uint32_t scheme;
+  uint32_t pad;
+};
+
+// A fill of a group's scratch (k_group_fill): n words at p, word i = v[i % period].
+struct alignas(16) GroupFill {
+  uint64_t *p;
+  uint64_t n;
+  uint64_t v[4];
+  uint32_t period;
+  uint32_t pad0;
+  uint64_t pad1;
+};
+
 // Device error flags (OR-ed by kernels).
 enum DevError : uint32_t {
   kErrNone = 0,

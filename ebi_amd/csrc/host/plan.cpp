@@ -99,15 +99,10 @@ ChunkPred sprintz_chunk_pred(const DevChunk &d, const ebi_pred &p) {
 
 bool is_agg(int op) { return op == kOpSum || op == kOpMin || op == kOpMax || op == kOpFilterSum; }
 bool has_pred(int op) { return op == kOpFilter || op == kOpFilterMaterialize || op == kOpFilterSum; }
-// ops whose filter writes the bitset `bits`
-bool writes_bits(const Plan &p) {
-  return p.q.op == kOpFilter || p.q.op == kOpFilterMaterialize || p.q.op == kOpFilterSum ||
-         (p.q.op == kOpMaterialize && p.compact);
-}
 
-// Offsets of the device scratch of a plan (all 256-byte aligned).
+// Offsets of the device scratch of a plan (all 256-byte aligned; ~0 = none).
 struct Layout {
-  uint64_t mask, bits, sel, cpred, acc, part, csum, block, values, out, total;
+  uint64_t mask, bits, sel, acc, csum, cpred, part, block, values, out, total;
 };
 
 Layout layout_of(const Plan &p) {
@@ -120,27 +115,25 @@ Layout layout_of(const Plan &p) {
   };
   const uint64_t N = p.c->meta.number_of_records;
   const uint64_t wbytes = (p.words + 2) * 8;
-  const uint64_t nb = launch::compact_blocks(p.words) + 1;
-  L.mask = take(wbytes);
-  L.bits = take(wbytes);
-  // the per-chunk results (sel, acc, csum) back to back, laid out as in the
-  // pinned staging, so that one copy brings them all to the host
+  const bool agg = ((p.flags >> kPfAggShift) & 3u) != 0;
+  L.mask = p.has_mask ? take(wbytes) : ~0ull;
+  L.bits = (p.flags & kPfBits) ? take(wbytes) : ~0ull;
   L.sel = take((uint64_t)p.nc * 4);
-  L.acc = take((uint64_t)p.nc * 32);
-  L.csum = take((uint64_t)p.nc * 16);
-  L.cpred = take((uint64_t)p.nc * sizeof(ChunkPred));
-  L.part = take((uint64_t)p.c->n_rtiles * 16);
-  L.block = take(nb * 8);
+  L.acc = agg ? take((uint64_t)p.nc * 32) : ~0ull;
+  L.csum = (p.flags & kPfFold) ? take((uint64_t)p.nc * 16) : ~0ull;
+  L.cpred = p.cpred ? take((uint64_t)p.nc * sizeof(ChunkPred)) : ~0ull;
+  L.part = (p.flags & kPfF64Sum) ? take((uint64_t)p.c->n_rtiles * 16) : ~0ull;
+  L.block = p.compact ? take(((uint64_t)p.n_blocks + 1) * 8) : ~0ull;
   // DECODE writes the caller's buffer; fused plans keep the records in registers
-  L.values = p.decode && !p.fused && p.q.op != kOpDecode ? take(N * 8) : ~0ull;
-  L.out = (p.q.op == kOpFilterMaterialize || (p.q.op == kOpMaterialize && p.compact)) ? take(N * 8) : ~0ull;
+  L.values = p.decode && p.q.op != kOpDecode ? take(N * 8) : ~0ull;
+  L.out = p.compact ? take(N * 8) : ~0ull;
   L.total = off;
   return L;
 }
 
-// Pinned staging: mask copy, bits result, sel, acc, csum, count, cpred.
+// Pinned staging: the bitmask copy, the FILTER result, the chunk predicates.
 struct PinLayout {
-  uint64_t mask, bits, sel, acc, csum, n, cpred, total;
+  uint64_t mask, bits, cpred, total;
 };
 
 PinLayout pin_layout_of(const Plan &p) {
@@ -153,18 +146,14 @@ PinLayout pin_layout_of(const Plan &p) {
   };
   L.mask = p.has_mask && !p.direct ? take(p.words * 8) : ~0ull;
   L.bits = p.q.op == kOpFilter && !p.direct ? take(p.words * 8) : ~0ull;
-  // the per-chunk results at the device layout's spacing (kAlign)
-  off = round_up(off, kAlign);
-  L.sel = off;
-  off += round_up(std::max<uint64_t>((uint64_t)p.nc * 4, 8), kAlign);
-  L.acc = off;
-  off += round_up(std::max<uint64_t>((uint64_t)p.nc * 32, 8), kAlign);
-  L.csum = off;
-  off += round_up(std::max<uint64_t>((uint64_t)p.nc * 16, 8), kAlign);
-  L.n = take(8);
   L.cpred = p.cpred ? take((uint64_t)p.nc * sizeof(ChunkPred)) : ~0ull;
   L.total = off;
   return L;
+}
+
+template <class T>
+T *at(uint8_t *base, uint64_t off) {
+  return off == ~0ull ? nullptr : reinterpret_cast<T *>(base + off);
 }
 
 QueryArgs args_of(const Plan &p) {
@@ -186,81 +175,6 @@ QueryArgs args_of(const Plan &p) {
     a.end = p.q.pred.end;
   }
   return a;
-}
-
-template <class T>
-T *at(uint8_t *base, uint64_t off) {
-  return off == ~0ull ? nullptr : reinterpret_cast<T *>(base + off);
-}
-
-// Async part before the decode: inputs and chunk selection.
-hipError_t plan_enqueue_pre(Plan &p, hipStream_t s) {
-  if (p.has_mask) {
-    launch::zero_u32(s, (uint32_t *)p.mask, (uint32_t)((p.words + 2) * 2));
-    hipError_t e = hipMemcpyAsync(p.mask, p.h_mask, p.words * 8, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-  }
-  if (p.cpred && p.nc) {
-    hipError_t e = hipMemcpyAsync(p.d_cpred, p.h_cpred, (size_t)p.nc * sizeof(ChunkPred), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-  }
-  if (writes_bits(p)) launch::zero_u32(s, (uint32_t *)p.bits, (uint32_t)((p.words + 2) * 2));
-  launch::chunk_select(s, p.d_view, p.has_mask ? p.mask : nullptr, p.q.chunk_id, p.sel, p.nc);
-  return hipGetLastError();
-}
-
-// The per-chunk results (sel, acc and, with `csum`, the chunk sums) to the
-// pinned staging in one copy: both layouts hold them back to back at the
-// same offsets.
-void copy_results(const Plan &p, hipStream_t s, bool csum) {
-  const uint8_t *a = reinterpret_cast<const uint8_t *>(p.sel);
-  const uint8_t *b = csum ? reinterpret_cast<const uint8_t *>(p.chunk_sum + 2ull * p.nc)
-                          : reinterpret_cast<const uint8_t *>(p.acc + 4ull * p.nc);
-  if (p.nc) hipMemcpyAsync(p.h_sel, p.sel, (size_t)(b - a), hipMemcpyDeviceToHost, s);
-}
-
-// Async part after the decode: query kernels and small results to pinned.
-// err: the group's device error flags; max_blocks: the persistent grid.
-hipError_t plan_enqueue_post(Plan &p, hipStream_t s, uint32_t *err, uint32_t max_blocks) {
-  Column &c = *p.c;
-  const int op = p.q.op;
-  if (p.fused) {
-    // decode + filter / aggregate in one pass; afterwards the same chunk
-    // selection, per-chunk fold and copies as below
-    QueryArgs a = args_of(p);
-    const int agg = op == kOpSum || op == kOpFilterSum ? 0 : op == kOpMin ? 1 : op == kOpMax ? 2 : -1;
-    launch::query_seg(s, p.d_view, c.n_seg_jobs, a, has_pred(op), writes_bits(p), agg, p.nc, err, max_blocks);
-    if (op == kOpFilter) {
-      if (p.words) hipMemcpyAsync(p.h_bits, p.bits, p.words * 8, hipMemcpyDeviceToHost, s);
-      return hipGetLastError();
-    }
-    if (op == kOpFilterSum) launch::chunk_select(s, p.d_view, p.bits, p.q.chunk_id, p.sel, p.nc);
-    if (agg == 0) launch::chunk_dd(s, p.d_view, p.sel, p.partials, p.chunk_sum, p.nc);
-    copy_results(p, s, agg == 0);
-    return hipGetLastError();
-  }
-  if (is_agg(op)) {
-    QueryArgs a = args_of(p);
-    if (op == kOpFilterSum) {
-      // filter into `bits`, then sum over them exactly as sum(Some(bits))
-      // does: chunks re-selected (pruned) by the filter's bitset
-      launch::filter(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, a, p.filter_mode);
-      launch::chunk_select(s, p.d_view, p.bits, p.q.chunk_id, p.sel, p.nc);
-      a.mask = p.bits;
-    }
-    launch::aggregate(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, a, p.nc);
-    const bool f64_sum = !p.insitu && (op == kOpSum || op == kOpFilterSum);
-    if (f64_sum) launch::chunk_dd(s, p.d_view, p.sel, p.partials, p.chunk_sum, p.nc);
-    copy_results(p, s, f64_sum);
-  } else if (op == kOpFilter) {
-    launch::filter(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, args_of(p), p.filter_mode);
-    if (p.words) hipMemcpyAsync(p.h_bits, p.bits, p.words * 8, hipMemcpyDeviceToHost, s);
-  } else if (op == kOpFilterMaterialize || (op == kOpMaterialize && p.compact)) {
-    launch::filter(s, p.d_view, c.d_rec_jobs, c.n_rec_jobs, args_of(p), p.filter_mode);
-    launch::compact(s, p.bits, p.words, p.values, p.out, p.block, p.d_n);
-    hipMemcpyAsync(p.h_n, p.d_n, 8, hipMemcpyDeviceToHost, s);
-  }
-  return hipGetLastError();
 }
 
 }  // namespace
@@ -303,16 +217,28 @@ int plan_init(Plan &p, const ebi_query &q) {
   p.insitu = scheme == kBUFF || scheme == kDeltaSprintz;
   p.has_mask = q.bitset != nullptr;
   // one chunk's records stay contiguous (its DevChunk n is the shortened count)
-  p.compact = p.has_mask || (p.c->truncated && q.chunk_id < 0);
+  p.compact = (q.op == kOpMaterialize && (p.has_mask || (p.c->truncated && q.chunk_id < 0))) ||
+              q.op == kOpFilterMaterialize;
   p.words = (N + 63) / 64;
   p.nc = (uint32_t)p.c->chunks.size();
+  p.n_blocks = p.compact ? launch::compact_blocks(p.words) : 0;
+  // Gorilla, Chimp, Elf, ElfOnChimp and Chimp128 filters and aggregates run
+  // fused: the segment decode feeds the query in registers
+  const bool seg_codec = scheme == kGorilla || scheme == kChimp || scheme == kElf || scheme == kElfOnChimp ||
+                         scheme == kChimp128;
+  const bool fused = seg_codec && p.c->n_seg_jobs;
+  uint32_t f = p.has_mask ? kPfMask : 0u;
+  const uint32_t agg = q.op == kOpSum || q.op == kOpFilterSum ? 1u : q.op == kOpMin ? 2u : q.op == kOpMax ? 3u : 0u;
+  f |= agg << kPfAggShift;
   switch (q.op) {
     case kOpMaterialize:
       p.decode = true;
       p.filter_mode = 0;  // select by the bitmask
+      if (p.compact) f |= kPfTiles | kPfBits | kPfCompact;
       break;
     case kOpFilter:
-      p.decode = !p.insitu;
+      f |= kPfPred | kPfBits | (p.insitu || !fused ? kPfTiles : kPfSegs);
+      p.decode = !p.insitu && !fused;
       p.cpred = p.insitu;
       p.filter_mode = 1;  // code domain for BUFF / DeltaSprintz, f64 otherwise
       break;
@@ -321,30 +247,29 @@ int plan_init(Plan &p, const ebi_query &q) {
       // filtered records (chunk_reader/buff.rs:230-241); every other codec,
       // DeltaSprintz included, evaluates the f64 predicate on decoded
       // records (query.rs:64-88).
+      f |= kPfTiles | kPfPred | kPfBits | kPfCompact;
       p.decode = true;
       p.cpred = scheme == kBUFF;
       p.filter_mode = scheme == kBUFF ? 1 : 2;
       break;
     case kOpFilterSum:
-      p.decode = !p.insitu;
+      // the sum over the records the filter selects, in one pass: as
+      // sum(Some(filter(..))), chunks without a selected record are pruned
+      f |= kPfPred | kPfAggSel | kPfFold | (p.insitu || !fused ? kPfTiles : kPfSegs);
+      p.decode = !p.insitu && !fused;
       p.cpred = p.insitu;
-      p.filter_mode = 1;  // as kOpFilter
+      p.filter_mode = 1;
       break;
     case kOpDecode:
       p.decode = true;
       break;
-    default:
-      p.decode = !p.insitu;
+    default:  // SUM / MIN / MAX
+      f |= kPfFold | (p.insitu || !fused ? kPfTiles : kPfSegs);
+      p.decode = !p.insitu && !fused;
       break;
   }
-  // Gorilla, Chimp, Elf and Chimp128 filters and aggregates run fused: the
-  // segment decode feeds the query in registers
-  const bool seg_codec = scheme == kGorilla || scheme == kChimp || scheme == kElf || scheme == kElfOnChimp ||
-                         scheme == kChimp128;
-  if (seg_codec && p.c->n_seg_jobs && (q.op == kOpFilter || q.op == kOpFilterSum || is_agg(q.op))) {
-    p.fused = true;
-    p.decode = false;
-  }
+  if (agg == 1 && !p.insitu) f |= kPfF64Sum;
+  p.flags = f;
   return EBI_OK;
 }
 
@@ -360,21 +285,16 @@ void plan_bind(Plan &p, uint8_t *dev, uint8_t *pin) {
   p.mask = at<uint64_t>(dev, L.mask);
   p.bits = at<uint64_t>(dev, L.bits);
   p.sel = at<uint32_t>(dev, L.sel);
-  p.d_cpred = at<ChunkPred>(dev, L.cpred);
   p.acc = at<uint64_t>(dev, L.acc);
-  p.partials = at<double>(dev, L.part);
   p.chunk_sum = at<double>(dev, L.csum);
+  p.d_cpred = at<ChunkPred>(dev, L.cpred);
+  p.partials = at<double>(dev, L.part);
   p.block = at<uint64_t>(dev, L.block);
-  p.d_n = p.block + launch::compact_blocks(p.words);
   p.values = p.q.op == kOpDecode ? p.q.out_values : at<double>(dev, L.values);
   p.out = at<double>(dev, L.out);
   const PinLayout P = pin_layout_of(p);
   p.h_mask = p.direct ? const_cast<uint64_t *>(p.q.bitset) : at<uint64_t>(pin, P.mask);
   p.h_bits = p.direct ? p.q.out_bitset : at<uint64_t>(pin, P.bits);
-  p.h_sel = at<uint32_t>(pin, P.sel);
-  p.h_acc = at<uint64_t>(pin, P.acc);
-  p.h_csum = at<double>(pin, P.csum);
-  p.h_n = at<uint64_t>(pin, P.n);
   p.h_cpred = at<ChunkPred>(pin, P.cpred);
 }
 
@@ -387,7 +307,7 @@ void plan_stage(Plan &p) {
       p.h_cpred[k] = buff ? buff_chunk_pred(d, p.q.pred, p.has_mask) : sprintz_chunk_pred(d, p.q.pred);
     }
   }
-  *p.h_n = 0;
+  p.n_out = 0;
 }
 
 bool plan_enqueue_values(Plan &p, hipStream_t s, hipError_t *err) {
@@ -405,15 +325,15 @@ bool plan_enqueue_values(Plan &p, hipStream_t s, hipError_t *err) {
         hi = lo + c.chunks[p.q.chunk_id].dev.n;
       }
     }
-    *p.h_n = hi - lo;
+    p.n_out = hi - lo;
     if (hi == lo) return false;
     *err = hipMemcpyAsync(p.q.out_values, p.values + lo, (hi - lo) * 8, hipMemcpyDeviceToHost, s);
     return true;
   }
-  if (op == kOpFilterMaterialize || op == kOpMaterialize) {
-    if (p.words == 0) *p.h_n = 0;
-    if (!*p.h_n) return false;
-    *err = hipMemcpyAsync(p.q.out_values, p.out, *p.h_n * 8, hipMemcpyDeviceToHost, s);
+  if (p.compact) {
+    p.n_out = p.words ? p.h_result->n_out : 0;
+    if (!p.n_out) return false;
+    *err = hipMemcpyAsync(p.q.out_values, p.out, p.n_out * 8, hipMemcpyDeviceToHost, s);
     return true;
   }
   return false;
@@ -422,9 +342,8 @@ bool plan_enqueue_values(Plan &p, hipStream_t s, hipError_t *err) {
 int plan_finish(Plan &p) {
   if (p.status) return p.status;
   const int op = p.q.op;
-  const Column &c = *p.c;
   if (op == kOpMaterialize || op == kOpFilterMaterialize) {
-    *p.q.n_out = (size_t)*p.h_n;
+    *p.q.n_out = (size_t)p.n_out;
     return EBI_OK;
   }
   if (op == kOpFilter) {
@@ -432,120 +351,174 @@ int plan_finish(Plan &p) {
     return EBI_OK;
   }
   if (op == kOpDecode) return EBI_OK;
-  // Decoder fold over the selected chunks, api/decoder.rs:323-439.
-  const int agg = op == kOpMin ? 1 : op == kOpMax ? 2 : 0;
-  const bool mask = p.has_mask || op == kOpFilterSum;
-  double result = agg == 0 ? 0.0 : agg == 1 ? INFINITY : -INFINITY;
-  double hi = 0.0, lo = 0.0;  // double-double total for the default codecs
-  for (uint32_t k = 0; k < p.nc; ++k) {
-    if (!p.h_sel[k]) continue;
-    const DevChunk &d = c.chunks[k].dev;
-    const uint64_t *ak = p.h_acc + 4ull * k;
-    double v;
-    if (c.meta.scheme == kBUFF) {
-      const double scale = std::ldexp(1.0, (int)d.frac);  // 2.0f64.powi(frac)
-      if (agg == 0) {
-        if (d.bits == 0)  // buff/sum.rs:80-84: value * n, bitmask ignored
-          v = ((double)d.base / scale) * (double)d.n;
-        else
-          v = (double)(int64_t)(ak[0] + (uint64_t)d.base * ak[1]) / scale;
-      } else if (d.bits == 0 || (agg == 1 && !mask)) {
-        v = (double)d.base / scale;  // min_max.rs:316-341 and the B = 0 exits
-      } else {
-        const uint64_t dd = agg == 1 ? ak[2] : ak[3];
-        v = (double)(int64_t)((uint64_t)d.base + dd) / scale;
-      }
-    } else if (c.meta.scheme == kDeltaSprintz) {
-      const double sc = (double)d.scale;
-      if (mask && ak[1] == 0 && agg == 0) v = NAN;
-      else if (agg == 0) v = (double)(int64_t)ak[0] / sc;
-      else v = (double)(int64_t)((agg == 1 ? ak[2] : ak[3]) ^ 0x8000000000000000ull) / sc;
-    } else {
-      if (agg == 0) {
-        // keep full precision across chunks (two-sum)
-        for (int t = 0; t < 2; ++t) {
-          const double b = p.h_csum[2ull * k + t];
-          const double sum = hi + b;
-          if (!std::isfinite(sum)) {
-            hi = sum;
-            lo = 0.0;
-            continue;
-          }
-          const double bv = sum - hi;
-          const double e = (hi - (sum - bv)) + (b - bv);
-          hi = sum;
-          lo += e;
-        }
-        continue;
-      }
-      const uint64_t key = agg == 1 ? ak[2] : ak[3];
-      if ((agg == 1 && key == ~0ull) || (agg == 2 && key == 0)) {
-        v = agg == 1 ? INFINITY : -INFINITY;
-      } else {
-        const uint64_t b = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
-        std::memcpy(&v, &b, 8);
-      }
-    }
-    if (agg == 0) result += v;
-    else if (agg == 1) result = rmin(result, v);
-    else result = rmax(result, v);
-  }
-  if (!p.insitu && agg == 0) result = std::isfinite(hi) ? hi + lo : hi;
-  *p.q.out_scalar = result;
+  *p.q.out_scalar = p.h_result->scalar;  // the Decoder's fold, computed on the device (k_group_finish)
   return EBI_OK;
 }
 
-// Group block: views[n] | prefix[n + 1] | prefix_seg[n + 1] | flags[4],
-// each part 256-byte aligned.
-uint64_t group_block_bytes(size_t n) {
-  return round_up(n * sizeof(ColView), kAlign) + 2 * round_up((n + 1) * 4, kAlign) + kAlign;
+// Group block, each part 256-byte aligned: views[n] | decode tile prefix[n+1] |
+// decode segment prefix[n+1] | PlanDesc[n] | item prefixes (chunks, record
+// tiles, segment jobs, compaction blocks)[4][n+1] | fills[3n] | results[n] | flags[4].
+namespace {
+struct GroupLayout {
+  uint64_t views, dpre, dseg, descs, ipre, fills, results, flags, total;
+};
+GroupLayout group_layout(size_t n) {
+  GroupLayout L{};
+  uint64_t off = 0;
+  auto take = [&](uint64_t bytes) {
+    const uint64_t o = off;
+    off += round_up(std::max<uint64_t>(bytes, 8), kAlign);
+    return o;
+  };
+  L.views = take(n * sizeof(ColView));
+  L.dpre = take((n + 1) * 4);
+  L.dseg = take((n + 1) * 4);
+  L.descs = take(n * sizeof(PlanDesc));
+  L.ipre = take(4 * (n + 1) * 4);
+  L.fills = take(3 * n * sizeof(GroupFill));
+  // results and flags back to back: one copy brings both to the host
+  L.results = off;
+  off += n * sizeof(PlanResult);
+  L.flags = off;
+  off += 16;
+  L.total = round_up(off, kAlign);
+  return L;
 }
+}  // namespace
 
-void group_bind(PlanGroup &g, uint8_t *dev, uint8_t *pin) {
+uint64_t group_block_bytes(size_t n) { return group_layout(n).total; }
+
+void group_bind(PlanGroup &g, uint8_t *dev, uint8_t *pin, size_t cap) {
   g.dev = dev;
   g.pin = pin;
-  const size_t n = g.plans.size();
-  const uint64_t flags = round_up(n * sizeof(ColView), kAlign) + 2 * round_up((n + 1) * 4, kAlign);
-  g.d_flags = reinterpret_cast<uint32_t *>(dev + flags);
-  g.h_flags = reinterpret_cast<uint32_t *>(pin + flags);
-  for (size_t i = 0; i < n; ++i) g.plans[i]->d_view = reinterpret_cast<const ColView *>(dev) + i;
+  g.cap = cap;
+  const GroupLayout L = group_layout(cap);
+  g.d_flags = reinterpret_cast<uint32_t *>(dev + L.flags);
+  g.h_flags = reinterpret_cast<uint32_t *>(pin + L.flags);
+  for (size_t i = 0; i < g.plans.size(); ++i) {
+    Plan &p = *g.plans[i];
+    p.d_view = reinterpret_cast<const ColView *>(dev + L.views) + i;
+    p.d_result = reinterpret_cast<PlanResult *>(dev + L.results) + i;
+    p.h_result = reinterpret_cast<PlanResult *>(pin + L.results) + i;
+  }
 }
 
 hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blocks_per_cu) {
   const size_t n = g.plans.size();
   if (!n) return hipSuccess;
-  const uint64_t vbytes = round_up(n * sizeof(ColView), kAlign);
-  const uint64_t pbytes = round_up((n + 1) * 4, kAlign);
-  ColView *views = reinterpret_cast<ColView *>(g.pin);
-  uint32_t *pre = reinterpret_cast<uint32_t *>(g.pin + vbytes);
-  uint32_t *preseg = reinterpret_cast<uint32_t *>(g.pin + vbytes + pbytes);
-  uint64_t jobs = 0, segs = 0;
+  const GroupLayout L = group_layout(g.cap);
+  ColView *views = reinterpret_cast<ColView *>(g.pin + L.views);
+  uint32_t *dpre = reinterpret_cast<uint32_t *>(g.pin + L.dpre);
+  uint32_t *dseg = reinterpret_cast<uint32_t *>(g.pin + L.dseg);
+  PlanDesc *descs = reinterpret_cast<PlanDesc *>(g.pin + L.descs);
+  uint32_t *ipre = reinterpret_cast<uint32_t *>(g.pin + L.ipre);
+  uint32_t *pchunk = ipre, *ptile = ipre + (n + 1), *pseg = ipre + 2 * (n + 1), *pblock = ipre + 3 * (n + 1);
+  GroupFill *fills = reinterpret_cast<GroupFill *>(g.pin + L.fills);
+  PlanResult *results = reinterpret_cast<PlanResult *>(g.pin + L.results);
+  uint64_t jobs = 0, segs = 0, chunks = 0, tiles = 0, qsegs = 0, blocks = 0, max_words = 0;
+  uint32_t n_fills = 0;
+  bool any_fold = false;
+  auto fill = [&](uint64_t *ptr, uint64_t words, uint64_t v0, uint64_t v1, uint64_t v2, uint64_t v3,
+                  uint32_t period) {
+    if (!ptr || !words) return;
+    GroupFill &f = fills[n_fills++];
+    f = GroupFill{};
+    f.p = ptr;
+    f.n = words;
+    f.v[0] = v0;
+    f.v[1] = v1;
+    f.v[2] = v2;
+    f.v[3] = v3;
+    f.period = period;
+    max_words = std::max(max_words, words);
+  };
   for (size_t i = 0; i < n; ++i) {
     Plan &p = *g.plans[i];
-    views[i] = view_of(*p.c, p.values);
-    pre[i] = (uint32_t)jobs;
-    preseg[i] = (uint32_t)segs;
+    const Column &c = *p.c;
+    views[i] = view_of(c, p.values);
+    dpre[i] = (uint32_t)jobs;
+    dseg[i] = (uint32_t)segs;
     if (p.decode) {
-      jobs += p.c->n_tile_jobs;
-      segs += p.c->n_seg_jobs;
+      jobs += c.n_tile_jobs;
+      segs += c.n_seg_jobs;
     }
+    PlanDesc &d = descs[i];
+    d = PlanDesc{};
+    d.a = args_of(p);
+    d.view = p.d_view;
+    d.rec_jobs = c.d_rec_jobs;
+    d.chunk_sum = p.chunk_sum;
+    d.out = p.out;
+    d.block = p.block;
+    d.result = p.d_result;
+    d.words = p.words;
+    d.chunk_id = p.q.chunk_id;
+    d.n_chunks = p.nc;
+    d.n_rec_jobs = c.n_rec_jobs;
+    d.n_seg_jobs = c.n_seg_jobs;
+    d.n_blocks = p.n_blocks;
+    d.flags = p.flags;
+    d.filter_mode = p.filter_mode;
+    d.scheme = (uint32_t)c.meta.scheme;
+    pchunk[i] = (uint32_t)chunks;
+    ptile[i] = (uint32_t)tiles;
+    pseg[i] = (uint32_t)qsegs;
+    pblock[i] = (uint32_t)blocks;
+    const bool runs = p.q.op != kOpDecode && !(p.q.op == kOpMaterialize && !p.compact);
+    if (runs) chunks += p.nc;
+    if (p.flags & kPfTiles) tiles += c.n_rec_jobs;
+    if (p.flags & kPfSegs) qsegs += c.n_seg_jobs;
+    if (p.flags & kPfCompact) blocks += p.n_blocks;
+    if (p.flags & kPfFold) any_fold = true;
+    results[i] = PlanResult{0.0, 0};
+    // scratch that must start cleared: the bitset, the bitmask's two spare
+    // words (after the copied ones), the per-chunk accumulators
+    if (p.flags & kPfBits) fill(p.bits, p.words + 2, 0, 0, 0, 0, 1);
+    if (p.has_mask) fill(p.mask + p.words, 2, 0, 0, 0, 0, 1);
+    if (p.acc) fill(p.acc, 4ull * p.nc, 0, 0, ~0ull, 0, 4);
   }
-  pre[n] = (uint32_t)jobs;
-  preseg[n] = (uint32_t)segs;
-  if (jobs >= (1ull << 32) || segs >= (1ull << 32)) return hipErrorInvalidValue;
+  dpre[n] = (uint32_t)jobs;
+  dseg[n] = (uint32_t)segs;
+  pchunk[n] = (uint32_t)chunks;
+  ptile[n] = (uint32_t)tiles;
+  pseg[n] = (uint32_t)qsegs;
+  pblock[n] = (uint32_t)blocks;
+  if (jobs >= (1ull << 32) || segs >= (1ull << 32) || chunks >= (1ull << 32) || tiles >= (1ull << 32) ||
+      qsegs >= (1ull << 32) || blocks >= (1ull << 31))
+    return hipErrorInvalidValue;
   g.h_flags[0] = g.h_flags[1] = 0;
-  hipError_t e = hipMemcpyAsync(g.dev, g.pin, vbytes + 2 * pbytes + 8, hipMemcpyHostToDevice, s);
+  hipError_t e = hipMemcpyAsync(g.dev, g.pin, L.total, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return e;
-  for (Plan *p : g.plans)
-    if ((e = plan_enqueue_pre(*p, s)) != hipSuccess) return e;
-  const ColView *dv = reinterpret_cast<const ColView *>(g.dev);
-  const uint32_t *dpre = reinterpret_cast<const uint32_t *>(g.dev + vbytes);
-  launch::decode(s, dv, (uint32_t)n, dpre, (uint32_t)jobs, dpre + pbytes / 4, (uint32_t)segs, g.d_flags,
-                 (uint32_t)cu_count * blocks_per_cu);
+  for (Plan *pp : g.plans) {
+    Plan &p = *pp;
+    if (p.has_mask && p.words &&
+        (e = hipMemcpyAsync(p.mask, p.h_mask, p.words * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
+      return e;
+    if (p.cpred && p.nc &&
+        (e = hipMemcpyAsync(p.d_cpred, p.h_cpred, (size_t)p.nc * sizeof(ChunkPred), hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+      return e;
+  }
+  const PlanDesc *dd = reinterpret_cast<const PlanDesc *>(g.dev + L.descs);
+  const uint32_t *di = reinterpret_cast<const uint32_t *>(g.dev + L.ipre);
+  const uint32_t np = (uint32_t)n;
+  const uint32_t max_blocks = (uint32_t)cu_count * blocks_per_cu;
+  launch::group_fill(s, reinterpret_cast<const GroupFill *>(g.dev + L.fills), n_fills, max_words);
+  launch::group_select(s, dd, di, np, (uint32_t)chunks);
+  launch::decode(s, reinterpret_cast<const ColView *>(g.dev + L.views), np,
+                 reinterpret_cast<const uint32_t *>(g.dev + L.dpre), (uint32_t)jobs,
+                 reinterpret_cast<const uint32_t *>(g.dev + L.dseg), (uint32_t)segs, g.d_flags, max_blocks);
+  launch::group_tiles(s, dd, di + (n + 1), np, (uint32_t)tiles);
+  launch::query_seg(s, dd, di + 2 * (n + 1), np, (uint32_t)qsegs, g.d_flags, max_blocks);
+  launch::group_fold(s, dd, di, np, (uint32_t)chunks, any_fold);
+  launch::group_compact(s, dd, di + 3 * (n + 1), np, (uint32_t)blocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  for (Plan *p : g.plans)
-    if ((e = plan_enqueue_post(*p, s, g.d_flags, (uint32_t)cu_count * blocks_per_cu)) != hipSuccess) return e;
-  return hipMemcpyAsync(g.h_flags, g.d_flags, 4, hipMemcpyDeviceToHost, s);
+  for (Plan *pp : g.plans)
+    if (pp->q.op == kOpFilter && pp->words &&
+        (e = hipMemcpyAsync(pp->h_bits, pp->bits, pp->words * 8, hipMemcpyDeviceToHost, s)) != hipSuccess)
+      return e;
+  // every plan's result record and the error flags in one copy
+  return hipMemcpyAsync(g.pin + L.results, g.dev + L.results, L.flags + 16 - L.results, hipMemcpyDeviceToHost, s);
 }
 
 }  // namespace ebi

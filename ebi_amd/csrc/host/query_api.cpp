@@ -71,7 +71,7 @@ int run_query(ebi_hip_ctx *ctx, const ebi_query &q) {
   if (ctx->plan_dev.ensure(need.dev + gb)) return fail(ctx, EBI_OOM, "query scratch allocation");
   if (ensure_pinned(ctx, need.pin + gb)) return fail(ctx, EBI_OOM, "pinned staging allocation");
   uint8_t *dev = (uint8_t *)ctx->plan_dev.p, *pin = ctx->plan_pin;
-  group_bind(g, dev, pin);
+  group_bind(g, dev, pin, 1);
   plan_bind(p, dev + gb, pin + gb);
   plan_stage(p);
   hipEventRecord(ctx->ev0, ctx->stream);
@@ -118,7 +118,7 @@ int run_query_noalloc(ebi_hip_ctx *ctx, const ebi_query &q) {
   PlanGroup g;
   g.plans.push_back(&p);
   uint8_t *dev = ctx->noalloc_dev, *pin = g_noalloc_host[ctx->device];
-  group_bind(g, dev, pin);
+  group_bind(g, dev, pin, 1);
   plan_bind(p, dev + gb, pin + gb);
   plan_stage(p);
   hipEventRecord(ctx->ev0, ctx->stream);

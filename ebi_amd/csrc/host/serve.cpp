@@ -219,7 +219,7 @@ Group *admit(ebi_hip_server *s) {
 
 // Issues a group on its stream (no lock: touches only the group's memory).
 hipError_t issue(ebi_hip_server *s, Group *g, hipStream_t st) {
-  group_bind(g->pg, s->dev_base + g->dev_off, s->pin_base + g->pin_off);
+  group_bind(g->pg, s->dev_base + g->dev_off, s->pin_base + g->pin_off, s->opts.max_batch);
   for (Ticket *t : g->tickets) {
     plan_bind(t->plan, s->dev_base + t->dev_off, s->pin_base + t->pin_off);
     plan_stage(t->plan);
@@ -344,6 +344,7 @@ extern "C" int ebi_hip_serve_create(ebi_hip_ctx *ctx, const ebi_serve_opts *opts
   if (!s->opts.n_streams) s->opts.n_streams = 4;
   if (!s->opts.max_batch) s->opts.max_batch = 64;
   s->opts.n_streams = std::min<uint32_t>(s->opts.n_streams, 16);
+  s->opts.max_batch = std::min<uint32_t>(s->opts.max_batch, 4096);  // fills of a group: 3 per plan, one grid row each
   s->ctx = ctx;
   s->device = ctx->device;
   s->cu_count = ctx->cu_count;

@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // or min / max order keys (agg 1 / 2, NaN skipped as Rust's f64::min / max
 // do).  No record leaves the registers.
 struct QueryOut {
-  const QueryArgs *a;
+  PredF64 pr;
   uint32_t i0, n;   // the lane's first record, the chunk's record count
   bool pred;
   int agg;          // -1 none, 0 sum, 1 min, 2 max
@@ -346,7 +346,7 @@ struct QueryOut {
   uint64_t kmin, kmax;
   __device__ __forceinline__ void row(uint32_t i, uint32_t, double v, bool act) {
     const uint32_t p = (i - i0) & 63;  // the terminator (i == n) may be record i0 + 64: never selected
-    const bool sel = act && i < n && ((mbits >> p) & 1) && (!pred || pred_f64(*a, v));
+    const bool sel = act && i < n && ((mbits >> p) & 1) && (!pred || pred_f64(pr, v));
     obits |= (uint64_t)sel << p;
     if (agg == 0) {
       if (sel) dd_add(hi, lo, v, 0.0);
@@ -364,25 +364,30 @@ struct QueryOut {
 // double-double partial, the butterfly sum of the lanes' sums; min / max
 // keys per lane.
 struct C128Query {
-  const QueryArgs *a;
+  PredF64 pr;
+  const uint64_t *mask;
+  uint64_t *out_bits;
+  double *partials;
   const DevChunk *c;
   bool pred, want_bits;
   int agg;
   double hi, lo;
   uint64_t kmin, kmax;
+  uint32_t cnt;  // selected records (wave-uniform)
   __device__ __forceinline__ void put(uint32_t i, uint64_t vb, uint32_t base) {
     const uint32_t n = c->n;
     const uint64_t gi = c->rec_off + i;
     const double v = as_f64(vb);
-    const bool m = i < n && (!a->mask || ((a->mask[gi >> 6] >> (gi & 63)) & 1));
-    const bool sel = m && (!pred || pred_f64(*a, v));
+    const bool m = i < n && (!mask || ((mask[gi >> 6] >> (gi & 63)) & 1));
+    const bool sel = m && (!pred || pred_f64(pr, v));
+    if (agg == 0) cnt += (uint32_t)__popcll(__ballot(sel));
     if (want_bits) {
       const uint64_t bal = __ballot(sel);
       if (lane_id() == 0 && base < n) {
         // the step's word: the wave's own when aligned and inside the
         // chunk, atomic ORs when it is shared
         const uint64_t g0 = c->rec_off + base;
-        unsigned long long *w = reinterpret_cast<unsigned long long *>(a->out_bits) + (g0 >> 6);
+        unsigned long long *w = reinterpret_cast<unsigned long long *>(out_bits) + (g0 >> 6);
         const uint32_t sh = (uint32_t)(g0 & 63);
         if (sh == 0 && base + kWave <= n) {
           *w = bal;
@@ -410,7 +415,7 @@ struct C128Query {
           }
         }
         if (lane == 0) {
-          double *pp = a->partials + 2ull * (c->rtile0 + base / kTile);
+          double *pp = partials + 2ull * (c->rtile0 + base / kTile);
           pp[0] = hi;
           pp[1] = lo;
         }
@@ -436,46 +441,60 @@ __device__ __forceinline__ void keys_to_acc(int agg, uint64_t kmin, uint64_t kma
   }
 }
 
-// Fused queries over the checkpoint-segment jobs of one Gorilla / Chimp /
-// Elf / Chimp128 column (cols[0]): the records are decoded as in
-// k_decode_seg and consumed in registers.  flags: bit 0 apply the f64 predicate, bit 1 write the
-// selected records' bits to a.out_bits (zeroed before), bits 2-3 aggregate
-// (0 none, 1 sum, 2 min, 3 max).  Sums go to the per-record-tile
-// double-double partials (k_chunk_dd folds them per chunk), each the
-// butterfly sum of its 8 segments' lane sums; min / max to the chunk
-// accumulators (k_acc_init before).  Chunks with a.chunk_sel 0 are skipped.
+// Fused queries over the checkpoint-segment jobs of every plan of a group
+// with kPfSegs (Gorilla / Chimp / Elf / ElfOnChimp / Chimp128 columns; the
+// plans' segment jobs concatenated, prefix over the plans): the records are
+// decoded as in k_decode_seg and consumed in registers.  Per plan: kPfPred
+// applies the f64 predicate, kPfBits writes the selected records' bits to
+// a.out_bits (zeroed before), the aggregate (flag bits 4-5) goes to the
+// per-record-tile double-double partials (sum; each the butterfly sum of its
+// 8 segments' lane sums) or the chunk accumulators (min / max), and with
+// kPfAggSel the selected records are counted per chunk (acc[1]).  Chunks
+// with a.chunk_sel 0 are skipped.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_query_seg(
-    const ColView *cols, uint32_t n_jobs, QueryArgs a, uint32_t flags, uint32_t *err) {
+    const PlanDesc *__restrict__ plans, const uint32_t *__restrict__ prefix, uint32_t n_plans, uint32_t n_jobs,
+    uint32_t *err) {
   __shared__ __align__(16) double lds[4][(kRun + 1) * kWave];  // Chimp128 jobs: the payload / value stage
   __shared__ uint8_t dists[4][kJob128];                         // and the source distances
   __shared__ double elf_tab[kElfTab];
   if (threadIdx.x < kElfTab) elf_tab_fill(elf_tab, threadIdx.x, c_p10, c_n10);
   __syncthreads();
-  const ColView cv = cols[0];
   const uint32_t lane = lane_id();
-  const bool want_bits = (flags >> 1) & 1;
-  const int agg = (int)((flags >> 2) & 3) - 1;
   const uint32_t stride = gridDim.x * 4;
   for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / kWave); j < n_jobs; j += stride) {
-    const Job jb = cv.seg_jobs[j];
-    if (!a.chunk_sel[jb.chunk]) continue;
+    const uint32_t pi = plan_of(prefix, n_plans, j);
+    const PlanDesc &P = plans[pi];
+    const PredF64 pr = pred_of(P.a);
+    const uint64_t *mask = P.a.mask;
+    uint64_t *out_bits = P.a.out_bits;
+    double *partials = P.a.partials;
+    uint64_t *acc = P.a.acc;
+    const uint32_t flags = P.flags;
+    const bool want_bits = (flags & kPfBits) != 0;
+    const bool pred = (flags & kPfPred) != 0;
+    const int agg = (int)((flags >> kPfAggShift) & 3u) - 1;  // -1 none, 0 sum, 1 min, 2 max
+    const bool count = agg == 0 && (flags & kPfAggSel) != 0;
+    const ColView cv = *static_cast<const ColView *__restrict__>(P.view);
+    const Job jb = cv.seg_jobs[j - prefix[pi]];
+    if (!P.a.chunk_sel[jb.chunk]) continue;
     const DevChunk c = cv.chunks[jb.chunk];
     if (c.codec == kCodecChimp128) {
-      C128Query q{&a, &c, (flags & 1) != 0, want_bits, agg, 0.0, 0.0, ~0ull, 0};
+      C128Query q{pr, mask, out_bits, partials, &c, pred, want_bits, agg, 0.0, 0.0, ~0ull, 0, 0};
       const uint32_t e = c128_job(cv, c, jb.tile, jb.col, lds[threadIdx.x / kWave], dists[threadIdx.x / kWave], q);
       if (e) atomicOr(err, e);
-      if (agg > 0) keys_to_acc(agg, q.kmin, q.kmax, a.acc + 4ull * jb.chunk);
+      if (agg > 0) keys_to_acc(agg, q.kmin, q.kmax, acc + 4ull * jb.chunk);
+      if (count && lane == 0 && q.cnt) atomicAdd((unsigned long long *)(acc + 4ull * jb.chunk) + 1, (unsigned long long)q.cnt);
       continue;
     }
     const uint32_t i0 = (jb.tile + lane) * kSeg;
     const uint64_t g = c.rec_off + i0;
     uint64_t m = ~0ull;
-    if (a.mask && i0 < c.n) {  // the 64 bitmask bits from record g (the bitmask has a spare word)
+    if (mask && i0 < c.n) {  // the 64 bitmask bits from record g (the bitmask has a spare word)
       const uint32_t sh = (uint32_t)(g & 63);
-      m = a.mask[g >> 6] >> sh;
-      if (sh) m |= a.mask[(g >> 6) + 1] << (64 - sh);
+      m = mask[g >> 6] >> sh;
+      if (sh) m |= mask[(g >> 6) + 1] << (64 - sh);
     }
-    QueryOut q{&a, i0, c.n, (flags & 1) != 0, agg, m, 0, 0.0, 0.0, ~0ull, 0};
+    QueryOut q{pr, i0, c.n, pred, agg, m, 0, 0.0, 0.0, ~0ull, 0};
     uint32_t e;
     switch (c.codec) {
       case kCodecGorilla: e = segment_wave<kCodecGorilla>(cv, c, jb.tile, jb.col, elf_tab, q); break;
@@ -488,7 +507,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     if (want_bits && i0 < c.n) {
       // a word wholly inside this segment is the lane's own; one shared
       // with another segment or chunk takes atomic ORs
-      unsigned long long *w = reinterpret_cast<unsigned long long *>(a.out_bits) + (g >> 6);
+      unsigned long long *w = reinterpret_cast<unsigned long long *>(out_bits) + (g >> 6);
       const uint32_t sh = (uint32_t)(g & 63);
       if (sh == 0 && i0 + kSeg <= c.n) {
         *w = q.obits;
@@ -512,12 +531,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         }
       }
       if ((lane & 7) == 0 && i0 < c.n) {
-        double *p = a.partials + 2ull * (c.rtile0 + (jb.tile + lane) / 8);
-        p[0] = q.hi;
-        p[1] = q.lo;
+        double *pp = partials + 2ull * (c.rtile0 + (jb.tile + lane) / 8);
+        pp[0] = q.hi;
+        pp[1] = q.lo;
+      }
+      if (count) {
+        const uint64_t n = wave_sum_u64(i0 < c.n ? (uint64_t)__popcll(q.obits) : 0ull);
+        if (lane == 0 && n) atomicAdd((unsigned long long *)(acc + 4ull * jb.chunk) + 1, (unsigned long long)n);
       }
     } else if (agg > 0) {
-      keys_to_acc(agg, q.kmin, q.kmax, a.acc + 4ull * jb.chunk);
+      keys_to_acc(agg, q.kmin, q.kmax, acc + 4ull * jb.chunk);
     }
   }
 }
@@ -754,15 +777,12 @@ void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t 
                        n_seg_jobs, err);
 }
 
-void query_seg(hipStream_t s, const ColView *cols, uint32_t n_seg_jobs, const QueryArgs &a, bool pred, bool bits,
-               int agg, uint32_t n_chunks, uint32_t *err, uint32_t max_blocks) {
-  if (agg > 0 && n_chunks)
-    hipLaunchKernelGGL(k_acc_init, dim3((n_chunks + 255) / 256), dim3(256), 0, s, a.acc, n_chunks);
-  if (!n_seg_jobs) return;
-  const uint32_t flags = (pred ? 1u : 0u) | (bits ? 2u : 0u) | ((uint32_t)(agg + 1) << 2);
-  uint32_t blocks = (n_seg_jobs + 3) / 4;
+void query_seg(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
+               uint32_t *err, uint32_t max_blocks) {
+  if (!n_jobs) return;
+  uint32_t blocks = (n_jobs + 3) / 4;
   if (max_blocks && blocks > max_blocks) blocks = max_blocks;
-  hipLaunchKernelGGL(k_query_seg, dim3(blocks), dim3(256), 0, s, cols, n_seg_jobs, a, flags, err);
+  hipLaunchKernelGGL(k_query_seg, dim3(blocks), dim3(256), 0, s, plans, prefix, n_plans, n_jobs, err);
 }
 
 void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,

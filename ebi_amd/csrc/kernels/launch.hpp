@@ -49,14 +49,21 @@ void chunk_select(hipStream_t s, const ColView *cols, const uint64_t *mask, int6
 void filter(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a, int mode);
 void aggregate(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_jobs, const QueryArgs &a,
                uint32_t n_chunks);
-// Fused decode + query over the segment jobs of one Gorilla / Chimp / Elf /
-// Chimp128 column (cols[0]; records never reach HBM): with `pred` the f64 predicate
-// of `a` on top of a.mask, with `bits` the selection to a.out_bits (zeroed
-// before), agg 0 sum (per-tile partials for chunk_dd), 1 min, 2 max (chunk
-// accumulators, initialized here), -1 none.  Chunks with a.chunk_sel 0 are
-// skipped.
-void query_seg(hipStream_t s, const ColView *cols, uint32_t n_seg_jobs, const QueryArgs &a, bool pred, bool bits,
-               int agg, uint32_t n_chunks, uint32_t *err, uint32_t max_blocks);
+// The kernels of a group of query plans (PlanDesc array in device memory,
+// flat work items mapped to plans by prefix arrays of n_plans + 1 entries):
+// scratch fills, chunk selection (items: chunks), the record-tile pass
+// (items: record tiles), the fused decode + query over segment jobs (items:
+// segment jobs; persistent grid of at most max_blocks), the per-chunk fold
+// and the final fold (items: chunks), and stream compaction (items: 32,768-
+// record blocks).
+void group_fill(hipStream_t s, const GroupFill *fills, uint32_t n_fills, uint64_t max_words);
+void group_select(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_items);
+void group_tiles(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs);
+void query_seg(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
+               uint32_t *err, uint32_t max_blocks);
+void group_fold(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_items,
+                bool any);
+void group_compact(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_blocks);
 void chunk_dd(hipStream_t s, const ColView *cols, const uint32_t *sel, const double *partials, double *chunk_sum,
               uint32_t n_chunks);
 uint32_t compact_blocks(uint64_t n_words);
