@@ -33,6 +33,7 @@ constexpr uint32_t kSeg = 64;    // records per checkpoint of the XOR index
 constexpr uint32_t kSeg128 = 16; // records per checkpoint of the Chimp128 index
 constexpr uint32_t kJob128 = kSeg128 * 64;  // records of a Chimp128 segment job (64 segments)
 constexpr uint32_t kTilesPerJob = 4;  // consecutive tiles of one chunk per decode job
+constexpr uint32_t kSpanTiles = 16;   // consecutive tiles of one chunk per query span job
 constexpr uint32_t kWaveLanes = 64;  // gfx950 wavefront size
 
 enum Codec : uint32_t {
@@ -246,17 +247,21 @@ struct alignas(16) PlanResult {
   uint64_t n_out;  // MATERIALIZE / FILTER_MATERIALIZE through compaction
 };
 
+// The record-tile kernels of a group: BUFF codes, DeltaSprintz codes, and
+// the bitmask / decoded records of every other plan.
+enum SpanKind : int { kSpanBuff = 0, kSpanSprintz = 1, kSpanValues = 2 };
+
 struct alignas(16) PlanDesc {
   QueryArgs a;
   const ColView *view;
-  const Job *rec_jobs;   // the column's record-tile jobs
+  const Job *span_jobs;  // the column's record-tile span jobs (Job::kind = tiles)
   double *chunk_sum;     // per chunk: (hi, lo) of an f64 sum, or (value, 0) to fold
   double *out;           // compaction output
   uint64_t *block;       // compaction: per-block counts, then offsets
   PlanResult *result;
   uint64_t words;        // ceil(N / 64)
   int64_t chunk_id;
-  uint32_t n_chunks, n_rec_jobs, n_seg_jobs, n_blocks;
+  uint32_t n_chunks, n_span_jobs, n_seg_jobs, n_blocks;
   uint32_t flags;
   int32_t filter_mode;   // 0 select by the bitmask, 1 code domain (BUFF / DeltaSprintz) or f64, 2 f64
   uint32_t scheme;

@@ -166,6 +166,7 @@ static void free_column(Column &c) {
   hipFree(c.d_seg_jobs);
   hipFree(c.d_single_jobs);
   hipFree(c.d_rec_jobs);
+  hipFree(c.d_span_jobs);
 }
 
 // Chunks that end at a NaN before their footer record count (the load-time
@@ -466,11 +467,14 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
       col.h_seg_jobs.push_back(Job{kJobSeg, (uint32_t)col.h_seg_jobs.size(), (uint32_t)k, s});
   }
   col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
-  std::vector<Job> rec_jobs;
-  for (size_t k = 0; k < nc; ++k)
-    for (uint32_t t = 0; t < (col.chunks[k].dev.n + kTile - 1) / kTile; ++t)
-      rec_jobs.push_back(Job{kJobTile, 0, (uint32_t)k, t});
+  std::vector<Job> rec_jobs, span_jobs;
+  for (size_t k = 0; k < nc; ++k) {
+    const uint32_t nt = (col.chunks[k].dev.n + kTile - 1) / kTile;
+    for (uint32_t t = 0; t < nt; ++t) rec_jobs.push_back(Job{kJobTile, 0, (uint32_t)k, t});
+    for (uint32_t t = 0; t < nt; t += kSpanTiles) span_jobs.push_back(Job{std::min(kSpanTiles, nt - t), 0, (uint32_t)k, t});
+  }
   col.n_rec_jobs = (uint32_t)rec_jobs.size();
+  col.n_span_jobs = (uint32_t)span_jobs.size();
   std::vector<Job> single;
   if (col.xor_codec || col.chimp128) {
     for (size_t k = 0; k < nc; k += kWaveLanes)
@@ -484,7 +488,8 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
       (e = upload_vec(&col.d_serial_jobs, col.h_serial_jobs)) != hipSuccess ||
       (e = upload_vec(&col.d_seg_jobs, col.h_seg_jobs)) != hipSuccess ||
       (e = upload_vec(&col.d_single_jobs, single)) != hipSuccess ||
-      (e = upload_vec(&col.d_rec_jobs, rec_jobs)) != hipSuccess)
+      (e = upload_vec(&col.d_rec_jobs, rec_jobs)) != hipSuccess ||
+      (e = upload_vec(&col.d_span_jobs, span_jobs)) != hipSuccess)
     return hip_fail(ctx, e, "upload job lists");
 
   {

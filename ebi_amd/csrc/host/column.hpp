@@ -57,8 +57,10 @@ struct Column {
   uint32_t n_serial_jobs = 0;
   Job *d_single_jobs = nullptr;  // one single-lane serial job per XOR chunk (comparator)
   Job *d_seg_jobs = nullptr;     // Gorilla/Chimp/Elf checkpoint segments
-  Job *d_rec_jobs = nullptr;     // record tiles of every chunk (queries)
+  Job *d_rec_jobs = nullptr;     // record tiles of every chunk (select filter)
   uint32_t n_rec_jobs = 0;
+  Job *d_span_jobs = nullptr;    // up to kSpanTiles record tiles of one chunk (query plans)
+  uint32_t n_span_jobs = 0;
   uint64_t n_rtiles = 0;
   uint32_t n_seg_jobs = 0;
   std::vector<Job> h_tile_jobs, h_serial_jobs, h_seg_jobs;

@@ -270,6 +270,10 @@ int plan_init(Plan &p, const ebi_query &q) {
   }
   if (agg == 1 && !p.insitu) f |= kPfF64Sum;
   p.flags = f;
+  // the record-tile kernel of the plan: on BUFF / DeltaSprintz codes when it
+  // filters in their domain or aggregates, else on the bitmask / decoded records
+  const bool code = p.insitu && (((f & kPfPred) && p.filter_mode == 1) || agg);
+  p.tile_kind = !code ? kSpanValues : scheme == kBUFF ? kSpanBuff : kSpanSprintz;
   return EBI_OK;
 }
 
@@ -356,9 +360,11 @@ int plan_finish(Plan &p) {
 }
 
 // Group block, each part 256-byte aligned: views[n] | decode tile prefix[n+1] |
-// decode segment prefix[n+1] | PlanDesc[n] | item prefixes (chunks, record
-// tiles, segment jobs, compaction blocks)[4][n+1] | fills[3n] | results[n] | flags[4].
+// decode segment prefix[n+1] | PlanDesc[n] | item prefixes (chunks, span
+// jobs of each SpanKind, segment jobs, compaction blocks)[kNumPre][n+1] |
+// fills[3n] | results[n] | flags[4].
 namespace {
+enum GroupPrefix : int { kPreChunk = 0, kPreSpan = 1, kPreSeg = 4, kPreBlock = 5, kNumPre = 6 };
 struct GroupLayout {
   uint64_t views, dpre, dseg, descs, ipre, fills, results, flags, total;
 };
@@ -374,7 +380,7 @@ GroupLayout group_layout(size_t n) {
   L.dpre = take((n + 1) * 4);
   L.dseg = take((n + 1) * 4);
   L.descs = take(n * sizeof(PlanDesc));
-  L.ipre = take(4 * (n + 1) * 4);
+  L.ipre = take((uint64_t)kNumPre * (n + 1) * 4);
   L.fills = take(3 * n * sizeof(GroupFill));
   // results and flags back to back: one copy brings both to the host
   L.results = off;
@@ -412,10 +418,12 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   uint32_t *dseg = reinterpret_cast<uint32_t *>(g.pin + L.dseg);
   PlanDesc *descs = reinterpret_cast<PlanDesc *>(g.pin + L.descs);
   uint32_t *ipre = reinterpret_cast<uint32_t *>(g.pin + L.ipre);
-  uint32_t *pchunk = ipre, *ptile = ipre + (n + 1), *pseg = ipre + 2 * (n + 1), *pblock = ipre + 3 * (n + 1);
+  auto pre = [&](int k) { return ipre + (size_t)k * (n + 1); };
+  uint32_t *pchunk = pre(kPreChunk), *pseg = pre(kPreSeg), *pblock = pre(kPreBlock);
   GroupFill *fills = reinterpret_cast<GroupFill *>(g.pin + L.fills);
   PlanResult *results = reinterpret_cast<PlanResult *>(g.pin + L.results);
-  uint64_t jobs = 0, segs = 0, chunks = 0, tiles = 0, qsegs = 0, blocks = 0, max_words = 0;
+  uint64_t jobs = 0, segs = 0, chunks = 0, qsegs = 0, blocks = 0, max_words = 0;
+  uint64_t tiles[3] = {0, 0, 0};  // span jobs per SpanKind
   uint32_t n_fills = 0;
   bool any_fold = false;
   auto fill = [&](uint64_t *ptr, uint64_t words, uint64_t v0, uint64_t v1, uint64_t v2, uint64_t v3,
@@ -446,7 +454,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
     d = PlanDesc{};
     d.a = args_of(p);
     d.view = p.d_view;
-    d.rec_jobs = c.d_rec_jobs;
+    d.span_jobs = c.d_span_jobs;
     d.chunk_sum = p.chunk_sum;
     d.out = p.out;
     d.block = p.block;
@@ -454,19 +462,19 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
     d.words = p.words;
     d.chunk_id = p.q.chunk_id;
     d.n_chunks = p.nc;
-    d.n_rec_jobs = c.n_rec_jobs;
+    d.n_span_jobs = c.n_span_jobs;
     d.n_seg_jobs = c.n_seg_jobs;
     d.n_blocks = p.n_blocks;
     d.flags = p.flags;
     d.filter_mode = p.filter_mode;
     d.scheme = (uint32_t)c.meta.scheme;
     pchunk[i] = (uint32_t)chunks;
-    ptile[i] = (uint32_t)tiles;
+    for (int k = 0; k < 3; ++k) pre(kPreSpan + k)[i] = (uint32_t)tiles[k];
     pseg[i] = (uint32_t)qsegs;
     pblock[i] = (uint32_t)blocks;
     const bool runs = p.q.op != kOpDecode && !(p.q.op == kOpMaterialize && !p.compact);
     if (runs) chunks += p.nc;
-    if (p.flags & kPfTiles) tiles += c.n_rec_jobs;
+    if (p.flags & kPfTiles) tiles[p.tile_kind] += c.n_span_jobs;
     if (p.flags & kPfSegs) qsegs += c.n_seg_jobs;
     if (p.flags & kPfCompact) blocks += p.n_blocks;
     if (p.flags & kPfFold) any_fold = true;
@@ -480,10 +488,10 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   dpre[n] = (uint32_t)jobs;
   dseg[n] = (uint32_t)segs;
   pchunk[n] = (uint32_t)chunks;
-  ptile[n] = (uint32_t)tiles;
+  for (int k = 0; k < 3; ++k) pre(kPreSpan + k)[n] = (uint32_t)tiles[k];
   pseg[n] = (uint32_t)qsegs;
   pblock[n] = (uint32_t)blocks;
-  if (jobs >= (1ull << 32) || segs >= (1ull << 32) || chunks >= (1ull << 32) || tiles >= (1ull << 32) ||
+  if (jobs >= (1ull << 32) || segs >= (1ull << 32) || chunks >= (1ull << 32) || tiles[0] + tiles[1] + tiles[2] >= (1ull << 32) ||
       qsegs >= (1ull << 32) || blocks >= (1ull << 31))
     return hipErrorInvalidValue;
   g.h_flags[0] = g.h_flags[1] = 0;
@@ -508,10 +516,10 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   launch::decode(s, reinterpret_cast<const ColView *>(g.dev + L.views), np,
                  reinterpret_cast<const uint32_t *>(g.dev + L.dpre), (uint32_t)jobs,
                  reinterpret_cast<const uint32_t *>(g.dev + L.dseg), (uint32_t)segs, g.d_flags, max_blocks);
-  launch::group_tiles(s, dd, di + (n + 1), np, (uint32_t)tiles);
-  launch::query_seg(s, dd, di + 2 * (n + 1), np, (uint32_t)qsegs, g.d_flags, max_blocks);
+  for (int k = 0; k < 3; ++k) launch::group_tiles(s, dd, di + (size_t)(kPreSpan + k) * (n + 1), np, (uint32_t)tiles[k], k);
+  launch::query_seg(s, dd, di + (size_t)kPreSeg * (n + 1), np, (uint32_t)qsegs, g.d_flags, max_blocks);
   launch::group_fold(s, dd, di, np, (uint32_t)chunks, any_fold);
-  launch::group_compact(s, dd, di + 3 * (n + 1), np, (uint32_t)blocks);
+  launch::group_compact(s, dd, di + (size_t)kPreBlock * (n + 1), np, (uint32_t)blocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   for (Plan *pp : g.plans)
     if (pp->q.op == kOpFilter && pp->words &&

@@ -71,6 +71,7 @@ struct Plan {
   bool cpred = false;      // per-chunk predicate parameters are uploaded
   uint32_t flags = 0;      // PlanFlags (device.hpp)
   int filter_mode = 0;     // record-tile filter mode (0 select, 1 code domain / f64, 2 f64)
+  int tile_kind = 0;       // SpanKind (device.hpp): which record-tile kernel runs the plan
   uint64_t words = 0;      // ceil(N / 64)
   uint32_t nc = 0;         // chunks
   uint32_t n_blocks = 0;   // compaction blocks

@@ -58,7 +58,8 @@ void aggregate(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_j
 // record blocks).
 void group_fill(hipStream_t s, const GroupFill *fills, uint32_t n_fills, uint64_t max_words);
 void group_select(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_items);
-void group_tiles(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs);
+void group_tiles(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
+                 int kind);
 void query_seg(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
                uint32_t *err, uint32_t max_blocks);
 void group_fold(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_items,

@@ -505,104 +505,252 @@ __global__ void __launch_bounds__(256) k_group_select(const PlanDesc *plans, con
   if (lane_id() == 0) const_cast<uint32_t *>(P.a.chunk_sel)[ci] = s;
 }
 
-// One record tile of one plan: the filter (bits of the predicate, or of the
-// bitmask when the plan only selects) and the aggregate, in one pass over
-// the tile's codes or records -- BUFF planes and DeltaSprintz codes are read
-// once even when the plan filters and then sums (FILTER_SUM).  The filter
-// follows k_filter, the aggregate k_aggregate; with kPfAggSel the aggregate
-// takes the predicate's selection instead of the bitmask (as sum(Some(
-// filter(..))) does) and counts it per chunk (acc[1]) for the re-pruning.
-__device__ __forceinline__ void group_tile(const PlanDesc &P, const Job jb) {
+// ------------------------------------------------------- record-tile spans
+// The record-tile pass works on spans: up to kSpanTiles consecutive tiles of
+// one chunk per wave (the column's span jobs, Job::kind = tile count).  The
+// wave loads the chunk's descriptor and predicate once, issues the loads of
+// tile k + 1 before it works on tile k (two tiles in flight per wave), keeps
+// its aggregates in lane registers across the span and adds them to the
+// chunk's accumulators once per span.
+//
+// Decoding BUFF and DeltaSprintz codes costs few bytes per record, so these
+// kernels are bound by vector ALU work rather than HBM: the predicate is
+// folded once per chunk into an interval test on the code (CodeSel), only
+// the requested aggregate is kept, and BUFF chunks of B < 24 bits work on
+// 32-bit deltas.
+
+// Lane-local aggregates of one wave over a span.
+struct SpanAcc {
+  uint64_t sum = 0, n = 0, mn = ~0ull, mx = 0;
+};
+
+// Which records of a chunk a code plan selects, from the code x of each
+// record (BUFF delta, unsigned; DeltaSprintz quantized value, signed):
+// none: no record; otherwise the record's bitmask bit (use_mask) and,
+// unless all, ((x - lo) <= (hi - lo) as u64) != inv -- an interval test that
+// holds for signed codes too (lo <= hi).
+struct CodeSel {
+  uint64_t lo, hi;
+  bool none, use_mask, all, inv;
+};
+
+// buff_pred (chunk_reader/buff.rs:128-228, filter_cmp.rs:298-451,
+// filter_eq.rs:73-201) folded per chunk: the early exits (kSideAll: every
+// record, bitmask ignored; kSideEmpty: none) and the comparisons as bounds.
+__device__ __forceinline__ CodeSel code_sel_buff(const PlanDesc &P, const ChunkPred &p) {
+  CodeSel s{0, ~0ull, false, true, true, false};
+  if (!(P.flags & kPfPred) || P.filter_mode != 1) return s;  // the bitmask alone
+  if (P.a.kind <= 1) {
+    const uint64_t v = p.eq_u8 ? (p.ldp & 0xff) : p.ldp;
+    if (P.a.kind == 0) {  // Eq: all(), none, or m && d == v
+      if (p.lmode == kSideAll) s.use_mask = false;
+      else if (p.lmode == kSideEmpty) s.none = true;
+      else s = CodeSel{v, v, false, true, false, false};
+    } else {  // NotEq = m && !eq: none, m, or m && d != v
+      if (p.lmode == kSideAll) s.none = true;
+      else if (p.lmode != kSideEmpty) s = CodeSel{v, v, false, true, false, true};
+    }
+    return s;
+  }
+  const bool cl = p.lmode == kSideCmp, cr = p.rmode == kSideCmp;
+  if (p.lmode == kSideNone && p.rmode == kSideNone) return s;  // m
+  if (p.lmode == kSideEmpty || p.rmode == kSideEmpty) {
+    s.none = true;
+    return s;
+  }
+  s.use_mask = cl || cr;  // kSideAll on both sides, or one and None: every record
+  s.all = !(cl || cr);
+  if (cl) {
+    if (!p.lincl && p.ldp == ~0ull) s.none = true;
+    s.lo = p.lincl ? p.ldp : p.ldp + 1;
+  }
+  if (cr) {
+    if (!p.rincl && p.rdp == 0) s.none = true;
+    s.hi = p.rincl ? p.rdp : p.rdp - 1;
+  }
+  if (s.lo > s.hi) s.none = true;
+  return s;
+}
+
+// pred_q (chunk_reader/sprintz.rs:116) as bounds on the signed quantized
+// value; the bitmask always applies.
+__device__ __forceinline__ CodeSel code_sel_sprintz(const PlanDesc &P, const ChunkPred &p) {
+  CodeSel s{0x8000000000000000ull, 0x7fffffffffffffffull, false, true, true, false};
+  if (!(P.flags & kPfPred) || P.filter_mode != 1) return s;
+  if (P.a.kind <= 1) {
+    s = CodeSel{p.ldp, p.ldp, false, true, false, P.a.kind == 1};
+    return s;
+  }
+  if (p.lmode == 0 && p.rmode == 0) return s;
+  int64_t lo = INT64_MIN, hi = INT64_MAX;
+  if (p.lmode == 1) lo = (int64_t)p.ldp;
+  else if (p.lmode == 2) {
+    if ((int64_t)p.ldp == INT64_MAX) s.none = true;
+    lo = (int64_t)(p.ldp + 1);
+  }
+  if (p.rmode == 1) hi = (int64_t)p.rdp;
+  else if (p.rmode == 2) {
+    if ((int64_t)p.rdp == INT64_MIN) s.none = true;
+    hi = (int64_t)(p.rdp - 1);
+  }
+  if (lo > hi) s.none = true;
+  s.lo = (uint64_t)lo;
+  s.hi = (uint64_t)hi;
+  s.all = false;
+  return s;
+}
+
+// The records of this lane in tile t: count, global id of the first, and
+// the bitmask bits (1 if no bitmask).
+struct TileLane {
+  uint32_t cnt, valid, m;
+  uint64_t g;
+};
+__device__ __forceinline__ TileLane tile_lane(const PlanDesc &P, const DevChunk &c, uint32_t t) {
+  TileLane L;
+  const uint32_t i0 = t * kTile + lane_id() * 8;
+  L.cnt = i0 < c.n ? min(8u, c.n - i0) : 0u;
+  L.valid = 0xffu >> (8 - L.cnt);
+  L.g = c.rec_off + i0;
+  L.m = L.cnt ? mask_byte(P.a.mask, L.g, L.cnt) : 0u;
+  return L;
+}
+
+// One record tile of a code plan: the filter (predicate or bitmask bits),
+// then the one aggregate the plan asks for (the bitmask's records, or with
+// kPfAggSel the predicate's -- as sum(Some(filter(..))) does -- counted for
+// the re-pruning).  X: uint32_t (BUFF, B < 24) or uint64_t codes.
+template <int KIND, class X>
+__device__ __forceinline__ void code_tile(const PlanDesc &P, const CodeSel &cs, const TileLane &L, const X x[8],
+                                          SpanAcc &acc) {
+  uint32_t sel = cs.use_mask ? L.m : L.valid;
+  if (!cs.all) {
+    // 32-bit codes (B < 24) lie below 2^24: bounds above 2^32 - 1 clamp to it
+    constexpr uint64_t top = sizeof(X) == 4 ? 0xffffffffull : ~0ull;
+    const X lo = (X)(cs.lo < top ? cs.lo : top);
+    const X span = (X)(cs.hi < top ? cs.hi : top) - lo;
+    uint32_t in = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) in |= (X)(x[k] - lo) <= span ? 1u << k : 0u;
+    sel &= cs.inv ? ~in : in;
+  }
+  const uint32_t flags = P.flags;
+  if ((flags & kPfBits) && L.cnt) put_bits(P.a.out_bits, L.g, sel, (L.g & 7) == 0 && L.cnt == 8);
+  const int agg = (int)((flags >> kPfAggShift) & 3u);
+  if (agg == 1) {
+    X s = 0;  // B < 24: 8 deltas sum below 2^27
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += ((sel >> k) & 1) ? x[k] : (X)0;
+    acc.sum += (uint64_t)s;
+    acc.n += (uint64_t)__builtin_popcount(sel);
+  } else if (agg && KIND == kSpanBuff) {  // keys: the deltas themselves
+    X mn = (X)~0ull, mx = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const bool on = (sel >> k) & 1;
+      if (agg == 2) mn = min(mn, on ? x[k] : (X)~0ull);
+      else mx = max(mx, on ? x[k] : (X)0);
+    }
+    if (agg == 2 && sel) acc.mn = min(acc.mn, (uint64_t)mn);
+    if (agg == 3) acc.mx = max(acc.mx, (uint64_t)mx);
+  } else if (agg) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if ((sel >> k) & 1) {
+        const uint64_t key = i64_key((int64_t)x[k]);
+        if (agg == 2) acc.mn = min(acc.mn, key);
+        else acc.mx = max(acc.mx, key);
+      }
+  }
+}
+
+// One record tile of a DeltaSprintz plan with W <= 16: record j of the lane
+// has q = base + s[j] (mod 2^64), s[j] an exact 32-bit prefix below 2^21 in
+// magnitude, so the interval test, the sum and min / max work on s and the
+// lane's base: (q - lo) = (s - (lo - base)) mod 2^64; the selected q sum to
+// count * base + sum(s); min / max of base + s are base + min / max of s
+// unless base lies within 2^26 of the i64 range's ends (then per record).
+__device__ __forceinline__ void sprintz_tile(const PlanDesc &P, const CodeSel &cs, const TileLane &L, int64_t base,
+                                             const int32_t s[8], SpanAcc &acc) {
+  uint32_t sel = cs.use_mask ? L.m : L.valid;
+  if (!cs.all) {
+    const uint64_t lo = cs.lo - (uint64_t)base, span = cs.hi - cs.lo;
+    uint32_t in = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) in |= (uint64_t)(int64_t)s[k] - lo <= span ? 1u << k : 0u;
+    sel &= cs.inv ? ~in : in;
+  }
+  const uint32_t flags = P.flags;
+  if ((flags & kPfBits) && L.cnt) put_bits(P.a.out_bits, L.g, sel, (L.g & 7) == 0 && L.cnt == 8);
+  const int agg = (int)((flags >> kPfAggShift) & 3u);
+  if (agg == 1) {
+    int32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t += ((sel >> k) & 1) ? s[k] : 0;
+    const uint32_t n = (uint32_t)__builtin_popcount(sel);
+    acc.sum += (uint64_t)n * (uint64_t)base + (uint64_t)(int64_t)t;
+    acc.n += n;
+  } else if (agg && sel) {
+    constexpr int64_t kEdge = 1ll << 26;
+    if (base > INT64_MIN + kEdge && base < INT64_MAX - kEdge) {
+      int32_t m = agg == 2 ? INT32_MAX : INT32_MIN;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const bool on = (sel >> k) & 1;
+        if (agg == 2) m = min(m, on ? s[k] : INT32_MAX);
+        else m = max(m, on ? s[k] : INT32_MIN);
+      }
+      const uint64_t key = i64_key(base + m);
+      if (agg == 2) acc.mn = min(acc.mn, key);
+      else acc.mx = max(acc.mx, key);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if ((sel >> k) & 1) {
+          const uint64_t key = i64_key((int64_t)((uint64_t)base + (uint64_t)(int64_t)s[k]));
+          if (agg == 2) acc.mn = min(acc.mn, key);
+          else acc.mx = max(acc.mx, key);
+        }
+    }
+  }
+}
+
+// One record tile of a plan on decoded records (v) or on the bitmask alone
+// (filter_mode 0, no aggregate): the filter follows k_filter, the aggregate
+// k_aggregate (f64 sums as a double-double partial per tile, folded in tile
+// order later; min / max as order-preserving keys, NaN ignored as Rust's
+// f64::min / max do).
+__device__ __forceinline__ void values_tile(const PlanDesc &P, const DevChunk &c, uint32_t t, const double v[8],
+                                            SpanAcc &acc) {
   const QueryArgs &a = P.a;
-  if (!a.chunk_sel[jb.chunk]) return;  // wave-uniform
-  const ColView *cv = P.view;
-  const DevChunk c = cv->chunks[jb.chunk];
-  const uint32_t lane = lane_id();
-  const uint32_t i0 = jb.tile * kTile + lane * 8;
-  const uint32_t cnt = i0 < c.n ? min(8u, c.n - i0) : 0u;
-  const uint64_t g = c.rec_off + i0;
-  const uint32_t m = cnt ? mask_byte(a.mask, g, cnt) : 0u;
+  const TileLane L = tile_lane(P, c, t);
   const uint32_t flags = P.flags;
   const int agg = (int)((flags >> kPfAggShift) & 3u);
-  const bool pred = (flags & kPfPred) != 0;
-  const bool insitu = c.codec == kCodecBUFF || c.codec == kCodecSprintz;
-  const bool code = insitu && ((pred && P.filter_mode == 1) || agg);
-  uint64_t d[8];
-  int64_t q[8];
-  if (code && c.codec == kCodecSprintz) sprintz_q(c, jb.tile, cv->tile_index[c.tile0 + jb.tile], q);  // every lane
-  if (code && c.codec == kCodecBUFF) buff_deltas(c, i0, d);  // zeros past the chunk's end
-  uint32_t sel = m;
-  if (pred) {
-    uint32_t bits = 0;
-    if (P.filter_mode == 1 && c.codec == kCodecBUFF) {
-      const ChunkPred cp = a.cpred[jb.chunk];
+  uint32_t sel = L.m;
+  if (flags & kPfPred) {
+    uint32_t bits = L.m;
+    if (P.filter_mode != 0) {
+      const PredF64 pr = pred_of(a);
+      bits = 0;
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if ((uint32_t)k < cnt && buff_pred(a, cp, d[k], (m >> k) & 1)) bits |= 1u << k;
-    } else if (P.filter_mode == 1 && c.codec == kCodecSprintz) {
-      const ChunkPred cp = a.cpred[jb.chunk];
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if ((uint32_t)k < cnt && ((m >> k) & 1) && pred_q(a, cp, q[k])) bits |= 1u << k;
-    } else if (P.filter_mode == 0) {
-      bits = m;
-    } else {
-      const double *v = a.values + g;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if ((uint32_t)k < cnt && ((m >> k) & 1) && pred_f64(a, v[k])) bits |= 1u << k;
+        if ((uint32_t)k < L.cnt && ((L.m >> k) & 1) && pred_f64(pr, v[k])) bits |= 1u << k;
     }
     sel = bits;
   }
-  if ((flags & kPfBits) && cnt) put_bits(a.out_bits, g, sel, (g & 7) == 0 && cnt == 8);
+  if ((flags & kPfBits) && L.cnt) put_bits(a.out_bits, L.g, sel, (L.g & 7) == 0 && L.cnt == 8);
   if (!agg) return;  // wave-uniform
-  const uint32_t use = (flags & kPfAggSel) ? sel : m;
-  uint64_t *acc = a.acc + 4ull * jb.chunk;
-  if (insitu) {
-    uint64_t sum = 0, n = 0, mn = ~0ull, mx = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if ((use >> k) & 1) {
-        const uint64_t x = c.codec == kCodecBUFF ? d[k] : (uint64_t)q[k];
-        const uint64_t key = c.codec == kCodecBUFF ? d[k] : i64_key(q[k]);
-        sum += x;
-        ++n;
-        mn = min(mn, key);
-        mx = max(mx, key);
-      }
-    if (agg == 1) {
-      sum = wave_sum_u64(sum);
-      n = wave_sum_u64(n);
-      if (lane == 0 && n) {
-        atomicAdd((unsigned long long *)acc, (unsigned long long)sum);
-        atomicAdd((unsigned long long *)acc + 1, (unsigned long long)n);
-      }
-    } else {
-#pragma unroll
-      for (int o = 32; o >= 1; o >>= 1) {
-        mn = min(mn, (uint64_t)__shfl_xor((unsigned long long)mn, o, kWave));
-        mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, o, kWave));
-      }
-      if (lane == 0) {
-        if (agg == 2) atomicMin((unsigned long long *)acc + 2, (unsigned long long)mn);
-        else atomicMax((unsigned long long *)acc + 3, (unsigned long long)mx);
-      }
-    }
-    return;
-  }
-  const double *v = a.values + g;
+  const uint32_t use = (flags & kPfAggSel) ? sel : L.m;
   if (agg == 1) {
     double hi = 0.0, lo = 0.0;
-    uint32_t n = 0;
+#pragma unroll
     for (int k = 0; k < 8; ++k)
-      if ((use >> k) & 1) {
-        dd_add(hi, lo, v[k], 0.0);
-        ++n;
-      }
+      if ((use >> k) & 1) dd_add(hi, lo, v[k], 0.0);
+    acc.n += (uint64_t)__builtin_popcount(use);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
       const double oh = __shfl_xor(hi, o, kWave), ol = __shfl_xor(lo, o, kWave);
-      if ((lane & o) == 0) {
+      if ((lane_id() & o) == 0) {
         dd_add(hi, lo, oh, ol);
       } else {
         double h2 = oh, l2 = ol;
@@ -611,41 +759,307 @@ __device__ __forceinline__ void group_tile(const PlanDesc &P, const Job jb) {
         lo = l2;
       }
     }
-    if (flags & kPfAggSel) n = (uint32_t)wave_sum_u64(n);
-    if (lane == 0) {
-      double *pp = a.partials + 2ull * (c.rtile0 + jb.tile);
+    if (lane_id() == 0) {
+      double *pp = a.partials + 2ull * (c.rtile0 + t);
       pp[0] = hi;
       pp[1] = lo;
-      if ((flags & kPfAggSel) && n) atomicAdd((unsigned long long *)acc + 1, (unsigned long long)n);
     }
   } else {
-    uint64_t mn = ~0ull, mx = 0;
-    for (int k = 0; k < 8; ++k)
-      if (((use >> k) & 1) && !(v[k] != v[k])) {  // Rust f64::min/max ignore NaN
-        mn = min(mn, f64_key(v[k]));
-        mx = max(mx, f64_key(v[k]));
-      }
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-      mn = min(mn, (uint64_t)__shfl_xor((unsigned long long)mn, o, kWave));
-      mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, o, kWave));
+    for (int k = 0; k < 8; ++k)
+      if (((use >> k) & 1) && !(v[k] != v[k])) {
+        acc.mn = min(acc.mn, f64_key(v[k]));
+        acc.mx = max(acc.mx, f64_key(v[k]));
+      }
+  }
+}
+
+// The span's aggregates into the chunk's accumulators (one atomic each).
+template <int KIND>
+__device__ __forceinline__ void span_flush(const PlanDesc &P, uint32_t chunk, SpanAcc &acc) {
+  const uint32_t flags = P.flags;
+  const int agg = (int)((flags >> kPfAggShift) & 3u);
+  if (!agg) return;
+  uint64_t *ak = P.a.acc + 4ull * chunk;
+  if (agg == 1) {
+    if (KIND == kSpanValues && !(flags & kPfAggSel)) return;  // the partials hold the sums
+    const uint64_t n = wave_sum_u64(acc.n);
+    const uint64_t sum = KIND == kSpanValues ? 0 : wave_sum_u64(acc.sum);
+    if (lane_id() == 0 && n) {
+      if (KIND != kSpanValues) atomicAdd((unsigned long long *)ak, (unsigned long long)sum);
+      atomicAdd((unsigned long long *)ak + 1, (unsigned long long)n);
     }
-    if (lane == 0) {
-      if (agg == 2) atomicMin((unsigned long long *)acc + 2, (unsigned long long)mn);
-      else atomicMax((unsigned long long *)acc + 3, (unsigned long long)mx);
+    return;
+  }
+  uint64_t mn = acc.mn, mx = acc.mx;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    mn = min(mn, (uint64_t)__shfl_xor((unsigned long long)mn, o, kWave));
+    mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, o, kWave));
+  }
+  if (lane_id() == 0) {
+    if (agg == 2) atomicMin((unsigned long long *)ak + 2, (unsigned long long)mn);
+    else atomicMax((unsigned long long *)ak + 3, (unsigned long long)mx);
+  }
+}
+
+// BUFF raw words of one lane's 8 records: NP plane words and, with TAIL, the
+// two aligned u64 words holding its r-bit tail fields.  Every load is
+// issued unconditionally (lanes past the chunk's end read its first group),
+// so a tile's loads are in flight while the wave works on the previous one.
+template <uint32_t NP>
+struct BuffRaw {
+  uint64_t w[NP ? NP : 1];
+  uint64_t lo, hi;
+};
+template <uint32_t NP, bool TAIL>
+__device__ __forceinline__ void buff_load(const DevChunk &c, uint32_t i0, BuffRaw<NP> &r) {
+  const uint32_t i = i0 < c.n ? i0 : 0u;
+  const EBI_GLOBAL uint8_t *base = as_global(reinterpret_cast<const uint8_t *>(c.data));
+#pragma unroll
+  for (uint32_t p = 0; p < NP; ++p)
+    r.w[p] = *reinterpret_cast<const EBI_GLOBAL uint64_t *>(base + (size_t)p * c.plane_stride + i);
+  if (TAIL) {
+    const EBI_GLOBAL uint8_t *tail = base + (size_t)NP * c.plane_stride;
+    const uint32_t a = ((i / 8) * c.r) & ~7u;
+    r.lo = *reinterpret_cast<const EBI_GLOBAL uint64_t *>(tail + a);
+    r.hi = *reinterpret_cast<const EBI_GLOBAL uint64_t *>(tail + a + 8);
+  }
+}
+// The deltas of buff_deltas from the raw words, as 32-bit values (NP <= 2:
+// B < 24); zeros past the chunk's end.
+template <uint32_t NP, bool TAIL>
+__device__ __forceinline__ void buff_assemble32(const DevChunk &c, uint32_t i0, const BuffRaw<NP> &r,
+                                                uint32_t d[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) d[j] = 0;
+  if (i0 >= c.n) return;
+#pragma unroll
+  for (uint32_t p = 0; p < NP; ++p) {
+    const uint64_t w = r.w[p] ^ 0x8080808080808080ull;  // planes store each byte XOR 0x80
+#pragma unroll
+    for (int j = 0; j < 8; ++j) d[j] = (d[j] << 8) | (uint32_t)((w >> (8 * j)) & 0xff);
+  }
+  if (TAIL) {
+    const uint32_t sh = (((i0 / 8) * c.r) & 7u) * 8;
+    const uint64_t bits = sh ? (r.lo >> sh) | (r.hi << (64 - sh)) : r.lo;
+    const uint32_t mk = (1u << c.r) - 1;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) d[j] = (d[j] << c.r) | ((uint32_t)(bits >> (j * c.r)) & mk);
+  }
+}
+
+template <uint32_t NP, bool TAIL>
+__device__ __forceinline__ void span_buff(const PlanDesc &P, const DevChunk &c, const CodeSel &cs, uint32_t t0,
+                                          uint32_t nt, SpanAcc &acc) {
+  const uint32_t lane = lane_id();
+  BuffRaw<NP> cur, nxt;
+  buff_load<NP, TAIL>(c, t0 * kTile + lane * 8, cur);
+  for (uint32_t k = 0; k < nt; ++k) {
+    const uint32_t t = t0 + k, tn = k + 1 < nt ? t + 1 : t;  // the last tile reloads itself
+    buff_load<NP, TAIL>(c, tn * kTile + lane * 8, nxt);
+    uint32_t d[8];
+    buff_assemble32<NP, TAIL>(c, t * kTile + lane * 8, cur, d);
+    code_tile<kSpanBuff>(P, cs, tile_lane(P, c, t), d, acc);
+    cur = nxt;
+  }
+}
+
+// DeltaSprintz with W <= 16: each lane's 8 fields (W bytes from byte
+// (i0 / 8) * W, MSB first) lie in three aligned u64 words; the words of tile
+// k + 1 load during tile k.
+struct SprRaw {
+  uint64_t w0, w1, w2;
+};
+__device__ __forceinline__ void sprintz_load16(const DevChunk &c, uint32_t i0, SprRaw &r) {
+  const uint32_t i = i0 < c.n ? i0 : 0u;
+  const EBI_GLOBAL uint64_t *p =
+      as_global(reinterpret_cast<const uint64_t *>(c.data)) + (((i / 8) * c.bits) >> 3);
+  r.w0 = p[0];
+  r.w1 = p[1];
+  r.w2 = p[2];
+}
+// The 8 fields of this lane for a fixed W <= 16 as exact 32-bit inclusive
+// prefix sums of their zigzag-decoded deltas: after the byte-aligned funnel
+// shift every field sits at a fixed bit of four 32-bit words.  Fields past
+// the chunk's end decode padding; they only feed the prefixes of records
+// that are not selected (later fields of the last lane, empty lanes).
+template <uint32_t W>
+__device__ __forceinline__ void sprintz_local(uint32_t i0, const SprRaw &r, int32_t s[8]) {
+  const uint32_t sh = (((i0 / 8) * W) & 7u) * 8;
+  const uint64_t b0 = __builtin_bswap64(r.w0), b1 = __builtin_bswap64(r.w1), b2 = __builtin_bswap64(r.w2);
+  const uint64_t hi = sh ? (b0 << sh) | (b1 >> (64 - sh)) : b0;  // the group's 8W bits, MSB first
+  const uint64_t lo = sh ? (b1 << sh) | (b2 >> (64 - sh)) : b1;
+  const uint32_t w[4] = {(uint32_t)(hi >> 32), (uint32_t)hi, (uint32_t)(lo >> 32), (uint32_t)lo};
+  constexpr uint32_t mk = (1u << W) - 1;
+  int32_t a = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    const uint32_t b = j * W, k = b >> 5, o = b & 31;
+    const uint32_t z = o + W <= 32 ? (w[k] >> (32 - o - W)) & mk
+                                   : ((w[k] << (o + W - 32)) | (w[k + 1 < 4 ? k + 1 : 3] >> (64 - o - W))) & mk;
+    a += (int32_t)(z >> 1) ^ -(int32_t)(z & 1);
+    s[j] = a;
+  }
+}
+
+// DeltaSprintz with W > 16: field j of the lane's records starts at bit
+// (i0 + j) * W (MSB first) and lies in the two aligned u64 words from its
+// first bit's word; the loads of four fields are issued before they are cut
+// out.
+__device__ __forceinline__ void sprintz_q_wide(const DevChunk &c, uint32_t i0, int64_t tile_base, int64_t q[8]) {
+  const uint32_t W = c.bits;
+  const uint32_t cnt = i0 < c.n ? min(8u, c.n - i0) : 0u;
+  const uint64_t b0 = (uint64_t)(cnt ? i0 : 0u) * W;
+  const EBI_GLOBAL uint64_t *p = as_global(reinterpret_cast<const uint64_t *>(c.data));
+  int64_t s = 0;
+#pragma unroll
+  for (int h = 0; h < 8; h += 4) {
+    uint64_t w[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t wi = (b0 + (uint64_t)(h + j) * W) >> 6;
+      w[2 * j] = p[wi];
+      w[2 * j + 1] = p[wi + 1];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t o = (uint32_t)((b0 + (uint64_t)(h + j) * W) & 63);
+      const uint64_t hi = __builtin_bswap64(w[2 * j]), lo = __builtin_bswap64(w[2 * j + 1]);
+      const uint64_t x = o ? (hi << o) | (lo >> (64 - o)) : hi;
+      const uint64_t z = x >> (64 - W);  // W <= 64: a shift of 0 .. 47
+      const int64_t dj = (uint32_t)(h + j) < cnt ? (int64_t)(z >> 1) ^ -(int64_t)(z & 1) : 0;
+      s = (int64_t)((uint64_t)s + (uint64_t)dj);
+      q[h + j] = s;
+    }
+  }
+  const int64_t incl = wave_incl_scan_i64(s);
+  const int64_t excl = (int64_t)((uint64_t)incl - (uint64_t)s) + tile_base;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) q[j] = (int64_t)((uint64_t)q[j] + (uint64_t)excl);
+}
+
+template <uint32_t W>
+__device__ __forceinline__ void span_sprintz_w(const PlanDesc &P, const DevChunk &c, const CodeSel &cs,
+                                               uint32_t t0, uint32_t nt, SpanAcc &acc) {
+  const uint32_t lane = lane_id();
+  const int64_t *tidx = P.view->tile_index + c.tile0;
+  SprRaw cur, nxt;
+  sprintz_load16(c, t0 * kTile + lane * 8, cur);
+  for (uint32_t k = 0; k < nt; ++k) {
+    const uint32_t t = t0 + k, tn = k + 1 < nt ? t + 1 : t;  // the last tile reloads itself
+    sprintz_load16(c, tn * kTile + lane * 8, nxt);
+    int32_t s[8];
+    sprintz_local<W>(t * kTile + lane * 8, cur, s);
+    const int32_t excl = (int32_t)wave_incl_scan_u32((uint32_t)s[7]) - s[7];  // every lane
+    const int64_t base = (int64_t)((uint64_t)tidx[t] + (uint64_t)(int64_t)excl);
+    sprintz_tile(P, cs, tile_lane(P, c, t), base, s, acc);
+    cur = nxt;
+  }
+}
+
+__device__ __forceinline__ void span_sprintz(const PlanDesc &P, const DevChunk &c, const CodeSel &cs, uint32_t t0,
+                                             uint32_t nt, SpanAcc &acc) {
+  const uint32_t lane = lane_id();
+  const int64_t *tidx = P.view->tile_index + c.tile0;
+  switch (c.bits) {
+#define EBI_SPAN_SPR(w) \
+  case w: span_sprintz_w<w>(P, c, cs, t0, nt, acc); return;
+    EBI_SPAN_SPR(1) EBI_SPAN_SPR(2) EBI_SPAN_SPR(3) EBI_SPAN_SPR(4)
+    EBI_SPAN_SPR(5) EBI_SPAN_SPR(6) EBI_SPAN_SPR(7) EBI_SPAN_SPR(8)
+    EBI_SPAN_SPR(9) EBI_SPAN_SPR(10) EBI_SPAN_SPR(11) EBI_SPAN_SPR(12)
+    EBI_SPAN_SPR(13) EBI_SPAN_SPR(14) EBI_SPAN_SPR(15) EBI_SPAN_SPR(16)
+#undef EBI_SPAN_SPR
+    default: break;
+  }
+  {  // W = 0 or W > 16
+    for (uint32_t t = t0; t < t0 + nt; ++t) {
+      int64_t q[8];
+      if (c.bits == 0) {  // every delta 0: q = the tile's starting prefix
+#pragma unroll
+        for (int j = 0; j < 8; ++j) q[j] = tidx[t];
+      } else {
+        sprintz_q_wide(c, t * kTile + lane * 8, tidx[t], q);  // every lane (wave scan)
+      }
+      code_tile<kSpanSprintz>(P, cs, tile_lane(P, c, t), reinterpret_cast<const uint64_t *>(q), acc);
     }
   }
 }
 
-// The record-tile pass of every plan with kPfTiles: one wave per (plan,
-// record tile), plans' tile jobs concatenated.
-__global__ void __launch_bounds__(256) k_group_tiles(const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans,
+// Decoded records of one lane's 8 records (addresses clamped into the chunk,
+// so every load is issued; values past the chunk's end are not used).
+__device__ __forceinline__ void values_load(const DevChunk &c, const double *values, uint32_t i0, double v[8]) {
+  const uint32_t cnt = i0 < c.n ? min(8u, c.n - i0) : 0u;
+  const double *src = values + c.rec_off + (cnt ? i0 : 0u);
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) v[k] = src[cnt ? min(k, cnt - 1) : 0u];
+}
+
+__device__ __forceinline__ void span_values(const PlanDesc &P, const DevChunk &c, uint32_t t0, uint32_t nt,
+                                            SpanAcc &acc) {
+  const uint32_t lane = lane_id();
+  const uint32_t flags = P.flags;
+  const bool load = ((flags & kPfPred) && P.filter_mode != 0) || ((flags >> kPfAggShift) & 3u);
+  if (!load) {
+    for (uint32_t t = t0; t < t0 + nt; ++t) values_tile(P, c, t, nullptr, acc);
+    return;
+  }
+  double cur[8], nxt[8];
+  values_load(c, P.a.values, t0 * kTile + lane * 8, cur);
+  for (uint32_t k = 0; k < nt; ++k) {
+    const uint32_t t = t0 + k, tn = k + 1 < nt ? t + 1 : t;
+    values_load(c, P.a.values, tn * kTile + lane * 8, nxt);
+    values_tile(P, c, t, cur, acc);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) cur[j] = nxt[j];
+  }
+}
+
+// The record-tile pass of the plans of one SpanKind (their span jobs
+// concatenated): one wave per (plan, span job).  One kernel per kind, so
+// each gets the registers its own path needs.
+template <int KIND>
+__global__ void __launch_bounds__(256) k_group_tiles(const PlanDesc *__restrict__ plans,
+                                                    const uint32_t *__restrict__ prefix, uint32_t n_plans,
                                                     uint32_t n_jobs) {
   const uint32_t j = blockIdx.x * 4 + threadIdx.x / kWave;
   if (j >= n_jobs) return;
   const uint32_t p = plan_of(prefix, n_plans, j);
   const PlanDesc &P = plans[p];
-  group_tile(P, P.rec_jobs[j - prefix[p]]);
+  const Job jb = P.span_jobs[j - prefix[p]];
+  if (!P.a.chunk_sel[jb.chunk]) return;  // wave-uniform
+  const DevChunk c = P.view->chunks[jb.chunk];
+  SpanAcc acc;
+  if (KIND == kSpanValues) {
+    span_values(P, c, jb.tile, jb.kind, acc);
+    span_flush<KIND>(P, jb.chunk, acc);
+    return;
+  }
+  ChunkPred cp{};
+  if ((P.flags & kPfPred) && P.filter_mode == 1) cp = P.a.cpred[jb.chunk];
+  const CodeSel cs = KIND == kSpanBuff ? code_sel_buff(P, cp) : code_sel_sprintz(P, cp);
+  if (cs.none) return;  // no record: the bitset is zeroed, no aggregate
+  if (KIND == kSpanSprintz) {
+    span_sprintz(P, c, cs, jb.tile, jb.kind, acc);
+  } else {
+    switch (c.n_planes * 2 + (c.r ? 1u : 0u)) {
+#define EBI_SPAN_BUFF(np)                                                    \
+  case 2 * np: span_buff<np, false>(P, c, cs, jb.tile, jb.kind, acc); break; \
+  case 2 * np + 1: span_buff<np, true>(P, c, cs, jb.tile, jb.kind, acc); break;
+      EBI_SPAN_BUFF(0)
+      EBI_SPAN_BUFF(1)
+      EBI_SPAN_BUFF(2)
+#undef EBI_SPAN_BUFF
+      default:  // B >= 24: 64-bit deltas, a plane loop per tile, one tile in flight
+        for (uint32_t t = jb.tile; t < jb.tile + jb.kind; ++t) {
+          uint64_t d[8];
+          buff_deltas(c, t * kTile + lane_id() * 8, d);
+          code_tile<kSpanBuff>(P, cs, tile_lane(P, c, t), d, acc);
+        }
+        break;
+    }
+  }
+  span_flush<KIND>(P, jb.chunk, acc);
 }
 
 // Rust f64::min / f64::max: a NaN operand yields the other operand.
@@ -729,56 +1143,106 @@ __global__ void __launch_bounds__(256) k_group_fold(const PlanDesc *plans, const
 }
 
 // The Decoder's fold (api/decoder.rs:323-439) of every plan with kPfFold,
-// one block per plan: the chunks' results, staged through LDS, folded in
-// chunk order by one lane -- `result += v` (the reference's f64 order), the
-// double-double total of an f64 sum, f64::min / f64::max from +-INFINITY.
-constexpr uint32_t kFoldStage = 2048;
+// one block per plan.  The block stages the chunks' results in LDS (all
+// loads in flight at once), then:
+//  * BUFF / DeltaSprintz sums in the reference's order: `result += v` chunk
+//    after chunk, by one lane reading LDS 16 values ahead of its adds (an
+//    untaken chunk is staged as -0.0, which leaves every sum as it is);
+//  * f64 sums: lane l of wave 0 folds the double-double results of chunks
+//    l, l + 64, ... then a butterfly in a lane-independent order
+//    (deterministic);
+//  * min / max: f64::min / f64::max from +-INFINITY, in any order.
+constexpr uint32_t kFinStage = 2048;  // chunks per LDS stage (4096 for the in-order sums)
 __global__ void __launch_bounds__(256) k_group_finish(const PlanDesc *plans) {
-  __shared__ double vs[2 * kFoldStage];
-  __shared__ uint32_t take[kFoldStage];
+  __shared__ double2 buf[kFinStage];
+  __shared__ uint32_t tk[kFinStage];
   const PlanDesc &P = plans[blockIdx.x];
   if (!(P.flags & kPfFold)) return;  // block-uniform
   const int agg = (int)((P.flags >> kPfAggShift) & 3u);
+  const uint32_t tid = threadIdx.x, lane = lane_id();
+  const uint32_t nc = P.n_chunks;
+  const double *cs = P.chunk_sum;
+  const uint32_t *take = P.a.chunk_sel;
+  if (agg == 1 && !(P.flags & kPfF64Sum)) {
+    double *seq = reinterpret_cast<double *>(buf);
+    constexpr uint32_t cap = 2 * kFinStage;
+    double r = 0.0;
+    for (uint32_t k0 = 0; k0 < nc; k0 += cap) {
+      const uint32_t m = min(cap, nc - k0), mp = (m + 15) & ~15u;
+#pragma unroll 8
+      for (uint32_t i = tid; i < mp; i += blockDim.x)
+        seq[i] = i < m && take[k0 + i] ? cs[2ull * (k0 + i)] : -0.0;
+      __syncthreads();
+      if (tid == 0) {
+        const double2 *s2 = reinterpret_cast<const double2 *>(seq);
+        const uint32_t nb = mp / 16;
+        double2 x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = s2[j];
+        for (uint32_t b = 0; b < nb; ++b) {
+          const uint32_t b1 = b + 1 < nb ? b + 1 : b;
+          double2 y[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) y[j] = s2[8 * b1 + j];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            r += x[j].x;
+            r += x[j].y;
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) x[j] = y[j];
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) P.result->scalar = r;
+    return;
+  }
   const bool f64sum = (P.flags & kPfF64Sum) != 0;
-  double result = agg == 1 ? 0.0 : agg == 2 ? INFINITY : -INFINITY;
   double hi = 0.0, lo = 0.0;
-  for (uint32_t k0 = 0; k0 < P.n_chunks; k0 += kFoldStage) {
-    const uint32_t m = min(kFoldStage, P.n_chunks - k0);
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-      take[i] = P.a.chunk_sel[k0 + i];
-      vs[2 * i] = P.chunk_sum[2ull * (k0 + i)];
-      vs[2 * i + 1] = P.chunk_sum[2ull * (k0 + i) + 1];
+  double r = agg == 2 ? INFINITY : -INFINITY;
+  for (uint32_t k0 = 0; k0 < nc; k0 += kFinStage) {
+    const uint32_t m = min(kFinStage, nc - k0);
+#pragma unroll 8
+    for (uint32_t i = tid; i < m; i += blockDim.x) {
+      tk[i] = take[k0 + i];
+      buf[i] = make_double2(cs[2ull * (k0 + i)], cs[2ull * (k0 + i) + 1]);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      for (uint32_t i = 0; i < m; ++i) {
-        if (!take[i]) continue;
-        if (f64sum) {  // keep full precision across chunks (two-sum)
-          for (int t = 0; t < 2; ++t) {
-            const double b = vs[2 * i + t];
-            const double sum = hi + b;
-            if (!isfinite(sum)) {
-              hi = sum;
-              lo = 0.0;
-              continue;
-            }
-            const double bv = sum - hi;
-            const double e = (hi - (sum - bv)) + (b - bv);
-            hi = sum;
-            lo += e;
-          }
-        } else if (agg == 1) {
-          result += vs[2 * i];
-        } else if (agg == 2) {
-          result = rmin_d(result, vs[2 * i]);
-        } else {
-          result = rmax_d(result, vs[2 * i]);
-        }
+    if (tid < kWave) {
+      for (uint32_t i = lane; i < m; i += kWave) {
+        if (!tk[i]) continue;
+        if (f64sum) dd_add(hi, lo, buf[i].x, buf[i].y);
+        else r = agg == 2 ? rmin_d(r, buf[i].x) : rmax_d(r, buf[i].x);
       }
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) P.result->scalar = f64sum ? (isfinite(hi) ? hi + lo : hi) : result;
+  if (tid >= kWave) return;
+  double out;
+  if (f64sum) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const double oh = __shfl_xor(hi, o, kWave), ol = __shfl_xor(lo, o, kWave);
+      if ((lane & o) == 0) {
+        dd_add(hi, lo, oh, ol);
+      } else {
+        double h2 = oh, l2 = ol;
+        dd_add(h2, l2, hi, lo);
+        hi = h2;
+        lo = l2;
+      }
+    }
+    out = isfinite(hi) ? hi + lo : hi;
+  } else {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double x = __shfl_xor(r, o, kWave);
+      r = agg == 2 ? rmin_d(r, x) : rmax_d(r, x);
+    }
+    out = r;
+  }
+  if (lane == 0) P.result->scalar = out;
 }
 
 // Stream compaction of every plan with kPfCompact (materialize through a
@@ -818,8 +1282,16 @@ void group_select(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, 
   if (n_items)
     hipLaunchKernelGGL(k_group_select, dim3((n_items + 3) / 4), dim3(256), 0, s, plans, prefix, n_plans, n_items);
 }
-void group_tiles(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs) {
-  if (n_jobs) hipLaunchKernelGGL(k_group_tiles, dim3((n_jobs + 3) / 4), dim3(256), 0, s, plans, prefix, n_plans, n_jobs);
+void group_tiles(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
+                 int kind) {
+  if (!n_jobs) return;
+  const dim3 grid((n_jobs + 3) / 4), block(256);
+  if (kind == kSpanBuff)
+    hipLaunchKernelGGL(k_group_tiles<kSpanBuff>, grid, block, 0, s, plans, prefix, n_plans, n_jobs);
+  else if (kind == kSpanSprintz)
+    hipLaunchKernelGGL(k_group_tiles<kSpanSprintz>, grid, block, 0, s, plans, prefix, n_plans, n_jobs);
+  else
+    hipLaunchKernelGGL(k_group_tiles<kSpanValues>, grid, block, 0, s, plans, prefix, n_plans, n_jobs);
 }
 void group_fold(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_items,
                 bool any) {

@@ -1,4 +1,4 @@
-"""The decode kernels are built without scratch (register spills): a build of
+"""The decode and query kernels are built without scratch (register spills): a build of
 the segment kernel that spilled decoded Elf wrongly in workgroups sharing a
 CU with an earlier one (the GPU test test_decode_grid_shapes covers that
 case).  Compiles the decode kernels for gfx950 with hipcc's resource-usage
@@ -15,6 +15,10 @@ KERNELS = {  # mangled-name prefix -> minimum occupancy (waves per SIMD)
     "_ZN3ebi12k_decode_segE": 4,
     "_ZN3ebi12k_decode_mixE": 4,  # both job lists in one launch
     "_ZN3ebi11k_query_segE": 4,  # the segment decode with the fused queries
+    "_ZN3ebi13k_group_tilesILi0E": 6,  # query spans on BUFF codes
+    "_ZN3ebi13k_group_tilesILi1E": 6,  # query spans on DeltaSprintz codes
+    "_ZN3ebi13k_group_tilesILi2E": 7,  # query spans on the bitmask / decoded records
+    "_ZN3ebi14k_group_finishE": 4,  # the per-plan fold
 }
 
 
