@@ -271,45 +271,61 @@ def pct(xs, q):
     return round(float(np.percentile(np.asarray(xs), q)), 4) if len(xs) else None
 
 
+QUERY_CODECS = ("buff", "delta_sprintz", "rle", "uncompressed", "gorilla", "chimp", "elf", "chimp128")
+
+
 def query_leg(E, ctx, cols, sample_q, n, reps=20):
     """In-situ query latency on the resident 2^log2n random-walk columns:
-    range-filter + sum (greater-than at the (1 - s) quantile, s = 0.1, 1, 10,
-    50 %) and max, one synchronous query at a time; BUFF and DeltaSprintz run
-    on their codes, Gorilla, Chimp and Elf decode and query in one kernel
-    (records stay in registers), Chimp128 decodes into HBM first.  Latency is
-    host wall time of the call; GB/s counts compressed bytes + 2 x ceil(N/8)
-    (bitset written and read) + 8."""
+    one-pass range-filter + sum (greater-than at the (1 - s) quantile, s =
+    0.1, 1, 10, 50 %) and max, one synchronous query at a time.  BUFF and
+    DeltaSprintz run on their codes (BUFF chunks whose range excludes the
+    predicate are skipped by the reference's own early exits), RLE per run
+    (on the random walk, nearly one record per run, and on the step series),
+    Uncompressed in place; Gorilla, Chimp and Elf decode and query in one
+    kernel (records stay in registers), Chimp128 decodes into HBM first.
+    p50 is host wall time of the call; device_p50 the library's HIP-event
+    time of the query's launches (block upload through the result copy).
+    GB/s count the column's compressed bytes + 8 (no bitset leaves the
+    device in a filter + sum), at wall and at device time; for BUFF's
+    filters the bytes of skipped chunks are counted too, so their GB/s are
+    effective rates (max reads every chunk)."""
     out = {}
-    sq = sample_q.get("walk2")
-    if sq is None:
+    if sample_q.get("walk2") is None:
         return None
-    for g, codec, d, cb in cols:
-        if g != "walk2" or codec not in ("buff", "delta_sprintz", "gorilla", "chimp", "elf", "chimp128"):
-            continue
-        res = {}
-        for s in (0.001, 0.01, 0.1, 0.5):
-            thr = float(sq[int((1.0 - s) * (len(sq) - 1))])
-            p = E.Predicate.range(E.RangeValue.exclusive(thr), E.RangeValue.none())
-            q = E.Query(E.OP_FILTER_SUM, d, predicate=p)
-            E.run_query(ctx, q)
-            lat = []
-            for _ in range(reps):
-                t = time.perf_counter()
-                E.run_query(ctx, q)
-                lat.append((time.perf_counter() - t) * 1e3)
-            byt = cb + 2 * ((n + 7) // 8) + 8
-            res[f"filter_sum_sel{s * 100:g}%"] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                                                  "gbs_at_p50": round(byt / (pct(lat, 50) / 1e3) / 1e9, 1)}
-        q = E.Query(E.OP_MAX, d)
+
+    def run(q):
         E.run_query(ctx, q)
-        lat = []
+        lat, dev = [], []
         for _ in range(reps):
             t = time.perf_counter()
             E.run_query(ctx, q)
             lat.append((time.perf_counter() - t) * 1e3)
-        res["max"] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                      "gbs_at_p50": round((cb + 8) / (pct(lat, 50) / 1e3) / 1e9, 1)}
-        out[codec] = res
+            dev.append(ctx.last_device_ms)
+        return lat, dev
+
+    for g, codec, d, cb in cols:
+        # every codec on the random walk; RLE also on the step series, whose
+        # runs (~Geom(0.01), at most 255) are what RLE is for
+        if not ((g == "walk2" and codec in QUERY_CODECS) or (g == "step2" and codec == "rle")):
+            continue
+        sq = sample_q[g]
+        res = {}
+        byt = cb + 8
+        for s in (0.001, 0.01, 0.1, 0.5):
+            thr = float(sq[int((1.0 - s) * (len(sq) - 1))])
+            p = E.Predicate.range(E.RangeValue.exclusive(thr), E.RangeValue.none())
+            lat, dev = run(E.Query(E.OP_FILTER_SUM, d, predicate=p))
+            res[f"filter_sum_sel{s * 100:g}%"] = {
+                "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "device_p50_ms": pct(dev, 50),
+                "gbs_at_p50": round(byt / (pct(lat, 50) / 1e3) / 1e9, 1),
+                "gbs_device": round(byt / (pct(dev, 50) / 1e3) / 1e9, 1)}
+        lat, dev = run(E.Query(E.OP_MAX, d))
+        res["max"] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "device_p50_ms": pct(dev, 50),
+                      "gbs_at_p50": round(byt / (pct(lat, 50) / 1e3) / 1e9, 1),
+                      "gbs_device": round(byt / (pct(dev, 50) / 1e3) / 1e9, 1),
+                      "device_hbm_frac": round(byt / (pct(dev, 50) / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+        res["compressed_bytes"] = int(cb)
+        out[codec if g == "walk2" else f"{codec}_{g}"] = res
         log(f"[query] {codec} done")
     return {"workload": f"2^{int(np.log2(n))} f64 random walk (2 decimals) per column, one query at a time",
             "codecs": out}

@@ -239,6 +239,7 @@ enum PlanFlags : uint32_t {
   kPfCompact = 1u << 8,  // materialize: compaction of a.values by a.out_bits into `out`
   kPfMask = 1u << 9,     // a bitmask was given
   kPfF64Sum = 1u << 10,  // the sum is over f64 records (double-double tile partials)
+  kPfFieldTiles = 1u << 11,  // RLE queried per run: the tile pass walks field tiles (partials per field tile)
 };
 constexpr uint32_t kPfAggShift = 4;  // bits 4-5: 0 none, 1 sum, 2 min, 3 max
 
@@ -247,9 +248,11 @@ struct alignas(16) PlanResult {
   uint64_t n_out;  // MATERIALIZE / FILTER_MATERIALIZE through compaction
 };
 
-// The record-tile kernels of a group: BUFF codes, DeltaSprintz codes, and
-// the bitmask / decoded records of every other plan.
-enum SpanKind : int { kSpanBuff = 0, kSpanSprintz = 1, kSpanValues = 2 };
+// The record-tile kernels of a group: BUFF codes, DeltaSprintz codes, the
+// bitmask / decoded records (Uncompressed: the chunk body itself), and RLE
+// runs.
+enum SpanKind : int { kSpanBuff = 0, kSpanSprintz = 1, kSpanValues = 2, kSpanRle = 3 };
+constexpr int kSpanKinds = 4;
 
 struct alignas(16) PlanDesc {
   QueryArgs a;

@@ -167,6 +167,7 @@ static void free_column(Column &c) {
   hipFree(c.d_single_jobs);
   hipFree(c.d_rec_jobs);
   hipFree(c.d_span_jobs);
+  hipFree(c.d_field_spans);
 }
 
 // Chunks that end at a NaN before their footer record count (the load-time
@@ -467,14 +468,22 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
       col.h_seg_jobs.push_back(Job{kJobSeg, (uint32_t)col.h_seg_jobs.size(), (uint32_t)k, s});
   }
   col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
-  std::vector<Job> rec_jobs, span_jobs;
+  std::vector<Job> rec_jobs, span_jobs, field_spans;
+  col.rle_direct = scheme == kRLE;
   for (size_t k = 0; k < nc; ++k) {
     const uint32_t nt = (col.chunks[k].dev.n + kTile - 1) / kTile;
     for (uint32_t t = 0; t < nt; ++t) rec_jobs.push_back(Job{kJobTile, 0, (uint32_t)k, t});
     for (uint32_t t = 0; t < nt; t += kSpanTiles) span_jobs.push_back(Job{std::min(kSpanTiles, nt - t), 0, (uint32_t)k, t});
+    if (scheme == kRLE) {  // field tiles; their f64 partials use the chunk's record-tile slots
+      const uint32_t ft = col.chunks[k].n_tiles;
+      if (col.chunks[k].dev.fields > col.chunks[k].dev.n) col.rle_direct = false;
+      for (uint32_t t = 0; t < ft; t += kSpanTiles)
+        field_spans.push_back(Job{std::min(kSpanTiles, ft - t), 0, (uint32_t)k, t});
+    }
   }
   col.n_rec_jobs = (uint32_t)rec_jobs.size();
   col.n_span_jobs = (uint32_t)span_jobs.size();
+  col.n_field_spans = (uint32_t)field_spans.size();
   std::vector<Job> single;
   if (col.xor_codec || col.chimp128) {
     for (size_t k = 0; k < nc; k += kWaveLanes)
@@ -489,7 +498,8 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
       (e = upload_vec(&col.d_seg_jobs, col.h_seg_jobs)) != hipSuccess ||
       (e = upload_vec(&col.d_single_jobs, single)) != hipSuccess ||
       (e = upload_vec(&col.d_rec_jobs, rec_jobs)) != hipSuccess ||
-      (e = upload_vec(&col.d_span_jobs, span_jobs)) != hipSuccess)
+      (e = upload_vec(&col.d_span_jobs, span_jobs)) != hipSuccess ||
+      (e = upload_vec(&col.d_field_spans, field_spans)) != hipSuccess)
     return hip_fail(ctx, e, "upload job lists");
 
   {

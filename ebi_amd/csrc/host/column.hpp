@@ -61,6 +61,9 @@ struct Column {
   uint32_t n_rec_jobs = 0;
   Job *d_span_jobs = nullptr;    // up to kSpanTiles record tiles of one chunk (query plans)
   uint32_t n_span_jobs = 0;
+  Job *d_field_spans = nullptr;  // RLE: up to kSpanTiles field tiles of one chunk (per-run queries)
+  uint32_t n_field_spans = 0;
+  bool rle_direct = false;       // RLE whose chunks have no more fields than records: queried per run
   uint64_t n_rtiles = 0;
   uint32_t n_seg_jobs = 0;
   std::vector<Job> h_tile_jobs, h_serial_jobs, h_seg_jobs;

@@ -227,6 +227,9 @@ int plan_init(Plan &p, const ebi_query &q) {
   const bool seg_codec = scheme == kGorilla || scheme == kChimp || scheme == kElf || scheme == kElfOnChimp ||
                          scheme == kChimp128;
   const bool fused = seg_codec && p.c->n_seg_jobs;
+  // Uncompressed chunks are read in place and RLE is queried per run: no
+  // decode for filters and aggregates
+  const bool direct = scheme == kUncompressed || (scheme == kRLE && p.c->rle_direct);
   uint32_t f = p.has_mask ? kPfMask : 0u;
   const uint32_t agg = q.op == kOpSum || q.op == kOpFilterSum ? 1u : q.op == kOpMin ? 2u : q.op == kOpMax ? 3u : 0u;
   f |= agg << kPfAggShift;
@@ -238,7 +241,7 @@ int plan_init(Plan &p, const ebi_query &q) {
       break;
     case kOpFilter:
       f |= kPfPred | kPfBits | (p.insitu || !fused ? kPfTiles : kPfSegs);
-      p.decode = !p.insitu && !fused;
+      p.decode = !p.insitu && !fused && !direct;
       p.cpred = p.insitu;
       p.filter_mode = 1;  // code domain for BUFF / DeltaSprintz, f64 otherwise
       break;
@@ -256,7 +259,7 @@ int plan_init(Plan &p, const ebi_query &q) {
       // the sum over the records the filter selects, in one pass: as
       // sum(Some(filter(..))), chunks without a selected record are pruned
       f |= kPfPred | kPfAggSel | kPfFold | (p.insitu || !fused ? kPfTiles : kPfSegs);
-      p.decode = !p.insitu && !fused;
+      p.decode = !p.insitu && !fused && !direct;
       p.cpred = p.insitu;
       p.filter_mode = 1;
       break;
@@ -265,15 +268,18 @@ int plan_init(Plan &p, const ebi_query &q) {
       break;
     default:  // SUM / MIN / MAX
       f |= kPfFold | (p.insitu || !fused ? kPfTiles : kPfSegs);
-      p.decode = !p.insitu && !fused;
+      p.decode = !p.insitu && !fused && !direct;
       break;
   }
   if (agg == 1 && !p.insitu) f |= kPfF64Sum;
   p.flags = f;
   // the record-tile kernel of the plan: on BUFF / DeltaSprintz codes when it
-  // filters in their domain or aggregates, else on the bitmask / decoded records
+  // filters in their domain or aggregates, on RLE runs when it needs no
+  // decode, else on the bitmask / decoded records (Uncompressed: in place)
   const bool code = p.insitu && (((f & kPfPred) && p.filter_mode == 1) || agg);
-  p.tile_kind = !code ? kSpanValues : scheme == kBUFF ? kSpanBuff : kSpanSprintz;
+  p.tile_kind = code ? (scheme == kBUFF ? kSpanBuff : kSpanSprintz)
+                     : (scheme == kRLE && direct && !p.decode && (f & kPfTiles)) ? kSpanRle : kSpanValues;
+  if (p.tile_kind == kSpanRle) p.flags |= kPfFieldTiles;
   return EBI_OK;
 }
 
@@ -364,7 +370,7 @@ int plan_finish(Plan &p) {
 // jobs of each SpanKind, segment jobs, compaction blocks)[kNumPre][n+1] |
 // fills[3n] | results[n] | flags[4].
 namespace {
-enum GroupPrefix : int { kPreChunk = 0, kPreSpan = 1, kPreSeg = 4, kPreBlock = 5, kNumPre = 6 };
+enum GroupPrefix : int { kPreChunk = 0, kPreSpan = 1, kPreSeg = 1 + kSpanKinds, kPreBlock = 2 + kSpanKinds, kNumPre = 3 + kSpanKinds };
 struct GroupLayout {
   uint64_t views, dpre, dseg, descs, ipre, fills, results, flags, total;
 };
@@ -423,7 +429,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   GroupFill *fills = reinterpret_cast<GroupFill *>(g.pin + L.fills);
   PlanResult *results = reinterpret_cast<PlanResult *>(g.pin + L.results);
   uint64_t jobs = 0, segs = 0, chunks = 0, qsegs = 0, blocks = 0, max_words = 0;
-  uint64_t tiles[3] = {0, 0, 0};  // span jobs per SpanKind
+  uint64_t tiles[kSpanKinds] = {};  // span jobs per SpanKind
   uint32_t n_fills = 0;
   bool any_fold = false;
   auto fill = [&](uint64_t *ptr, uint64_t words, uint64_t v0, uint64_t v1, uint64_t v2, uint64_t v3,
@@ -454,7 +460,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
     d = PlanDesc{};
     d.a = args_of(p);
     d.view = p.d_view;
-    d.span_jobs = c.d_span_jobs;
+    d.span_jobs = p.tile_kind == kSpanRle ? c.d_field_spans : c.d_span_jobs;
     d.chunk_sum = p.chunk_sum;
     d.out = p.out;
     d.block = p.block;
@@ -462,19 +468,19 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
     d.words = p.words;
     d.chunk_id = p.q.chunk_id;
     d.n_chunks = p.nc;
-    d.n_span_jobs = c.n_span_jobs;
+    d.n_span_jobs = p.tile_kind == kSpanRle ? c.n_field_spans : c.n_span_jobs;
     d.n_seg_jobs = c.n_seg_jobs;
     d.n_blocks = p.n_blocks;
     d.flags = p.flags;
     d.filter_mode = p.filter_mode;
     d.scheme = (uint32_t)c.meta.scheme;
     pchunk[i] = (uint32_t)chunks;
-    for (int k = 0; k < 3; ++k) pre(kPreSpan + k)[i] = (uint32_t)tiles[k];
+    for (int k = 0; k < kSpanKinds; ++k) pre(kPreSpan + k)[i] = (uint32_t)tiles[k];
     pseg[i] = (uint32_t)qsegs;
     pblock[i] = (uint32_t)blocks;
     const bool runs = p.q.op != kOpDecode && !(p.q.op == kOpMaterialize && !p.compact);
     if (runs) chunks += p.nc;
-    if (p.flags & kPfTiles) tiles[p.tile_kind] += c.n_span_jobs;
+    if (p.flags & kPfTiles) tiles[p.tile_kind] += p.tile_kind == kSpanRle ? c.n_field_spans : c.n_span_jobs;
     if (p.flags & kPfSegs) qsegs += c.n_seg_jobs;
     if (p.flags & kPfCompact) blocks += p.n_blocks;
     if (p.flags & kPfFold) any_fold = true;
@@ -488,10 +494,10 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   dpre[n] = (uint32_t)jobs;
   dseg[n] = (uint32_t)segs;
   pchunk[n] = (uint32_t)chunks;
-  for (int k = 0; k < 3; ++k) pre(kPreSpan + k)[n] = (uint32_t)tiles[k];
+  for (int k = 0; k < kSpanKinds; ++k) pre(kPreSpan + k)[n] = (uint32_t)tiles[k];
   pseg[n] = (uint32_t)qsegs;
   pblock[n] = (uint32_t)blocks;
-  if (jobs >= (1ull << 32) || segs >= (1ull << 32) || chunks >= (1ull << 32) || tiles[0] + tiles[1] + tiles[2] >= (1ull << 32) ||
+  if (jobs >= (1ull << 32) || segs >= (1ull << 32) || chunks >= (1ull << 32) || tiles[0] + tiles[1] + tiles[2] + tiles[3] >= (1ull << 32) ||
       qsegs >= (1ull << 32) || blocks >= (1ull << 31))
     return hipErrorInvalidValue;
   g.h_flags[0] = g.h_flags[1] = 0;
@@ -516,7 +522,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   launch::decode(s, reinterpret_cast<const ColView *>(g.dev + L.views), np,
                  reinterpret_cast<const uint32_t *>(g.dev + L.dpre), (uint32_t)jobs,
                  reinterpret_cast<const uint32_t *>(g.dev + L.dseg), (uint32_t)segs, g.d_flags, max_blocks);
-  for (int k = 0; k < 3; ++k) launch::group_tiles(s, dd, di + (size_t)(kPreSpan + k) * (n + 1), np, (uint32_t)tiles[k], k);
+  for (int k = 0; k < kSpanKinds; ++k) launch::group_tiles(s, dd, di + (size_t)(kPreSpan + k) * (n + 1), np, (uint32_t)tiles[k], k);
   launch::query_seg(s, dd, di + (size_t)kPreSeg * (n + 1), np, (uint32_t)qsegs, g.d_flags, max_blocks);
   launch::group_fold(s, dd, di, np, (uint32_t)chunks, any_fold);
   launch::group_compact(s, dd, di + (size_t)kPreBlock * (n + 1), np, (uint32_t)blocks);

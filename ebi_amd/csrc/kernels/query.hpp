@@ -667,17 +667,34 @@ __device__ __forceinline__ void code_tile(const PlanDesc &P, const CodeSel &cs, 
 // One record tile of a DeltaSprintz plan with W <= 16: record j of the lane
 // has q = base + s[j] (mod 2^64), s[j] an exact 32-bit prefix below 2^21 in
 // magnitude, so the interval test, the sum and min / max work on s and the
-// lane's base: (q - lo) = (s - (lo - base)) mod 2^64; the selected q sum to
-// count * base + sum(s); min / max of base + s are base + min / max of s
-// unless base lies within 2^26 of the i64 range's ends (then per record).
+// lane's base.  Unless base lies within 2^26 of the i64 range's ends (then
+// per record, modulo 2^64), q = base + s exactly: lo <= q <= hi is
+// lo - base <= s <= hi - base with both bounds clamped to 32 bits; the
+// selected q sum to count * base + sum(s) (mod 2^64); min / max of base + s
+// are base + min / max of s.
+__device__ __forceinline__ int32_t clamp_i32(int64_t x) {
+  return x < INT32_MIN ? INT32_MIN : x > INT32_MAX ? INT32_MAX : (int32_t)x;
+}
 __device__ __forceinline__ void sprintz_tile(const PlanDesc &P, const CodeSel &cs, const TileLane &L, int64_t base,
                                              const int32_t s[8], SpanAcc &acc) {
+  constexpr int64_t kEdge = 1ll << 26;
+  const bool inner = base > INT64_MIN + kEdge && base < INT64_MAX - kEdge;
   uint32_t sel = cs.use_mask ? L.m : L.valid;
   if (!cs.all) {
-    const uint64_t lo = cs.lo - (uint64_t)base, span = cs.hi - cs.lo;
     uint32_t in = 0;
+    if (inner) {
+      // lo - base and hi - base, saturated: |base| < 2^63 - 2^26 keeps one side in range
+      int64_t a, b;
+      if (__builtin_sub_overflow((int64_t)cs.lo, base, &a)) a = base > 0 ? INT64_MIN : INT64_MAX;
+      if (__builtin_sub_overflow((int64_t)cs.hi, base, &b)) b = base > 0 ? INT64_MIN : INT64_MAX;
+      const int32_t a32 = clamp_i32(a), b32 = clamp_i32(b);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) in |= (uint64_t)(int64_t)s[k] - lo <= span ? 1u << k : 0u;
+      for (int k = 0; k < 8; ++k) in |= (s[k] >= a32 && s[k] <= b32) ? 1u << k : 0u;  // |s| < 2^21
+    } else {
+      const uint64_t lo = cs.lo - (uint64_t)base, span = cs.hi - cs.lo;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) in |= (uint64_t)(int64_t)s[k] - lo <= span ? 1u << k : 0u;
+    }
     sel &= cs.inv ? ~in : in;
   }
   const uint32_t flags = P.flags;
@@ -691,8 +708,7 @@ __device__ __forceinline__ void sprintz_tile(const PlanDesc &P, const CodeSel &c
     acc.sum += (uint64_t)n * (uint64_t)base + (uint64_t)(int64_t)t;
     acc.n += n;
   } else if (agg && sel) {
-    constexpr int64_t kEdge = 1ll << 26;
-    if (base > INT64_MIN + kEdge && base < INT64_MAX - kEdge) {
+    if (inner) {
       int32_t m = agg == 2 ? INT32_MAX : INT32_MIN;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
@@ -782,11 +798,12 @@ __device__ __forceinline__ void span_flush(const PlanDesc &P, uint32_t chunk, Sp
   if (!agg) return;
   uint64_t *ak = P.a.acc + 4ull * chunk;
   if (agg == 1) {
-    if (KIND == kSpanValues && !(flags & kPfAggSel)) return;  // the partials hold the sums
+    constexpr bool f64 = KIND == kSpanValues || KIND == kSpanRle;
+    if (f64 && !(flags & kPfAggSel)) return;  // the partials hold the sums
     const uint64_t n = wave_sum_u64(acc.n);
-    const uint64_t sum = KIND == kSpanValues ? 0 : wave_sum_u64(acc.sum);
+    const uint64_t sum = f64 ? 0 : wave_sum_u64(acc.sum);
     if (lane_id() == 0 && n) {
-      if (KIND != kSpanValues) atomicAdd((unsigned long long *)ak, (unsigned long long)sum);
+      if (!f64) atomicAdd((unsigned long long *)ak, (unsigned long long)sum);
       atomicAdd((unsigned long long *)ak + 1, (unsigned long long)n);
     }
     return;
@@ -987,10 +1004,12 @@ __device__ __forceinline__ void span_sprintz(const PlanDesc &P, const DevChunk &
 }
 
 // Decoded records of one lane's 8 records (addresses clamped into the chunk,
-// so every load is issued; values past the chunk's end are not used).
+// so every load is issued; values past the chunk's end are not used).  An
+// Uncompressed chunk's body is its records: read in place.
 __device__ __forceinline__ void values_load(const DevChunk &c, const double *values, uint32_t i0, double v[8]) {
   const uint32_t cnt = i0 < c.n ? min(8u, c.n - i0) : 0u;
-  const double *src = values + c.rec_off + (cnt ? i0 : 0u);
+  const double *src = (c.codec == kCodecPlain ? reinterpret_cast<const double *>(c.data) : values + c.rec_off) +
+                      (cnt ? i0 : 0u);
 #pragma unroll
   for (uint32_t k = 0; k < 8; ++k) v[k] = src[cnt ? min(k, cnt - 1) : 0u];
 }
@@ -1015,6 +1034,114 @@ __device__ __forceinline__ void span_values(const PlanDesc &P, const DevChunk &c
   }
 }
 
+// Bits of global bitset word w inside records [g, g + r).
+__device__ __forceinline__ uint64_t range_word(uint64_t w, uint64_t g, uint32_t r) {
+  const uint64_t lo = w << 6, a = max(g, lo), b = min(g + r, lo + 64);
+  if (a >= b) return 0;
+  const uint32_t n = (uint32_t)(b - a);
+  return (n == 64 ? ~0ull : (1ull << n) - 1) << (uint32_t)(a - lo);
+}
+
+// RLE queried per run (chunk_reader/run_length.rs:68-91 expands runs; the
+// Decoder's default filter / sum / min / max then see each record of a run):
+// a run's value is tested once, its records' bits are set a word at a time
+// (ANDed with the bitmask), and the run adds value x count (an exact
+// two-product into the tile's double-double partial) or its key once.
+// Field tiles: lane l of tile t owns fields t*kTile + 8l .. +7, whose first
+// record follows from the tile's start (tile index) and a wave scan of the
+// run lengths.  A lane's 8 run bytes and 8 values are read as one u64 and
+// four 16-byte loads, all issued before the first is used.
+struct RleRaw {
+  uint64_t runs;
+  double2 v[4];
+};
+__device__ __forceinline__ void rle_load(const DevChunk &c, uint32_t f0, RleRaw &r) {
+  const uint32_t f = f0 < c.fields ? f0 : 0u;  // f0 is a multiple of 8: 8-byte runs, 64-byte values
+  const EBI_GLOBAL uint8_t *base = as_global(reinterpret_cast<const uint8_t *>(c.data));
+  r.runs = *reinterpret_cast<const EBI_GLOBAL uint64_t *>(base + f);
+  const EBI_GLOBAL double2 *v = reinterpret_cast<const EBI_GLOBAL double2 *>(base + c.values_off) + f / 2;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r.v[k] = v[k];
+}
+
+__device__ __forceinline__ void span_rle(const PlanDesc &P, const DevChunk &c, uint32_t t0, uint32_t nt,
+                                         SpanAcc &acc) {
+  const QueryArgs &a = P.a;
+  const uint32_t lane = lane_id(), flags = P.flags;
+  const int agg = (int)((flags >> kPfAggShift) & 3u);
+  const bool pred = (flags & kPfPred) && P.filter_mode != 0;
+  const bool bits_out = (flags & kPfBits) != 0;
+  const PredF64 pr = pred_of(a);
+  const int64_t *tidx = P.view->tile_index + c.tile0;
+  for (uint32_t t = t0; t < t0 + nt; ++t) {
+    const uint32_t f0 = t * kTile + lane * 8;
+    RleRaw cur;
+    rle_load(c, f0, cur);
+    uint32_t run[8];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {  // 0 past the chunk's fields
+      run[j] = f0 + j < c.fields ? (uint32_t)(cur.runs >> (8 * j)) & 0xffu : 0u;
+      s += run[j];
+    }
+    const double v8[8] = {cur.v[0].x, cur.v[0].y, cur.v[1].x, cur.v[1].y,
+                          cur.v[2].x, cur.v[2].y, cur.v[3].x, cur.v[3].y};
+    const uint32_t incl = wave_incl_scan_u32(s);
+    uint32_t pos = (uint32_t)tidx[t] + (incl - s);
+    double hi = 0.0, lo = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t r = pos < c.n ? min(run[j], c.n - pos) : 0u;  // run totals are checked at load
+      if (r) {
+        const double v = v8[j];
+        if (!pred || pred_f64(pr, v)) {
+          const uint64_t g = c.rec_off + pos;
+          uint32_t cnt = r;
+          if (a.mask || bits_out) {
+            cnt = 0;
+            for (uint64_t w = g >> 6; w <= (g + r - 1) >> 6; ++w) {
+              uint64_t b = range_word(w, g, r);
+              if (a.mask) b &= a.mask[w];
+              cnt += (uint32_t)__popcll(b);
+              if (bits_out && b) atomicOr(reinterpret_cast<unsigned long long *>(a.out_bits + w), b);
+            }
+          }
+          if (agg == 1) {
+            if (cnt) {
+              const double p = v * (double)cnt, e = fma(v, (double)cnt, -p);  // v x cnt exactly
+              dd_add(hi, lo, p, e);
+            }
+            acc.n += cnt;
+          } else if (agg && cnt && !(v != v)) {  // Rust f64::min / max ignore NaN
+            if (agg == 2) acc.mn = min(acc.mn, f64_key(v));
+            else acc.mx = max(acc.mx, f64_key(v));
+          }
+        }
+      }
+      pos += run[j];
+    }
+    if (agg == 1) {  // wave-uniform
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const double oh = __shfl_xor(hi, o, kWave), ol = __shfl_xor(lo, o, kWave);
+        if ((lane & o) == 0) {
+          dd_add(hi, lo, oh, ol);
+        } else {
+          double h2 = oh, l2 = ol;
+          dd_add(h2, l2, hi, lo);
+          hi = h2;
+          lo = l2;
+        }
+      }
+      if (lane == 0) {
+        double *pp = a.partials + 2ull * (c.rtile0 + t);
+        pp[0] = hi;
+        pp[1] = lo;
+      }
+    }
+  }
+}
+
 // The record-tile pass of the plans of one SpanKind (their span jobs
 // concatenated): one wave per (plan, span job).  One kernel per kind, so
 // each gets the registers its own path needs.
@@ -1030,8 +1157,9 @@ __global__ void __launch_bounds__(256) k_group_tiles(const PlanDesc *__restrict_
   if (!P.a.chunk_sel[jb.chunk]) return;  // wave-uniform
   const DevChunk c = P.view->chunks[jb.chunk];
   SpanAcc acc;
-  if (KIND == kSpanValues) {
-    span_values(P, c, jb.tile, jb.kind, acc);
+  if (KIND == kSpanValues || KIND == kSpanRle) {
+    if (KIND == kSpanValues) span_values(P, c, jb.tile, jb.kind, acc);
+    else span_rle(P, c, jb.tile, jb.kind, acc);
     span_flush<KIND>(P, jb.chunk, acc);
     return;
   }
@@ -1074,6 +1202,13 @@ __device__ __forceinline__ double rmax_d(double a, double b) { return a != a ? b
 // reader's rules (BUFF buff/sum.rs:60-119, min_max.rs:42-341 with the B = 0
 // and unmasked-min exits; DeltaSprintz sprintz.rs:176-446, NaN for an empty
 // in-chunk bitmask; default readers' f64::min / max keys).
+// BUFF chunk sums are integers times 2^-frac.  While every integer and the
+// sum of their magnitudes stay below 2^53, each partial sum of the Decoder's
+// `result += v` is exact, so the in-order sum equals the integer total times
+// 2^-frac and can be computed in any order.
+constexpr uint64_t kExactLimit = 1ull << 53;
+constexpr int64_t kNotExact = INT64_MIN;
+
 __global__ void __launch_bounds__(256) k_group_fold(const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans,
                                                    uint32_t n_items) {
   const uint32_t j = blockIdx.x * 4 + threadIdx.x / kWave;
@@ -1091,7 +1226,8 @@ __global__ void __launch_bounds__(256) k_group_fold(const PlanDesc *plans, const
     double hi = 0.0, lo = 0.0;
     if (take) {  // wave-uniform
       const DevChunk c = P.view->chunks[ci];
-      const uint32_t nt = (c.n + kTile - 1) / kTile;
+      const uint32_t nt = (P.flags & kPfFieldTiles) ? (uint32_t)((c.fields + kTile - 1) / kTile)
+                                                    : (c.n + kTile - 1) / kTile;
       for (uint32_t t = lane_id(); t < nt; t += kWave)
         dd_add(hi, lo, P.a.partials[2ull * (c.rtile0 + t)], P.a.partials[2ull * (c.rtile0 + t) + 1]);
       for (int d = kWave / 2; d >= 1; d >>= 1) {
@@ -1108,16 +1244,22 @@ __global__ void __launch_bounds__(256) k_group_fold(const PlanDesc *plans, const
   }
   if (lane_id() != 0) return;
   double v = 0.0;
+  int64_t exact = kNotExact;  // BUFF sums: the chunk's value is exact * 2^-frac
   if (take) {
     const DevChunk c = P.view->chunks[ci];
     const bool mask = (P.flags & (kPfMask | kPfAggSel)) != 0;
     if (c.codec == kCodecBUFF) {
       const double scale = ldexp(1.0, (int)c.frac);  // 2.0f64.powi(frac)
       if (agg == 1) {
-        if (c.bits == 0)  // buff/sum.rs:80-84: value * n, bitmask ignored
+        if (c.bits == 0) {  // buff/sum.rs:80-84: value * n, bitmask ignored
           v = ((double)c.base / scale) * (double)c.n;
-        else
-          v = (double)(int64_t)(ak[0] + (uint64_t)c.base * ak[1]) / scale;
+          const uint64_t ab = c.base < 0 ? 0ull - (uint64_t)c.base : (uint64_t)c.base;
+          if (ab < kExactLimit && (c.n == 0 || ab <= (kExactLimit - 1) / c.n)) exact = c.base * (int64_t)c.n;
+        } else {
+          const int64_t s = (int64_t)(ak[0] + (uint64_t)c.base * ak[1]);
+          v = (double)s / scale;
+          if (s > -(int64_t)kExactLimit && s < (int64_t)kExactLimit) exact = s;
+        }
       } else if (c.bits == 0 || (agg == 2 && !mask)) {
         v = (double)c.base / scale;  // min_max.rs:316-341 and the B = 0 exits
       } else {
@@ -1138,7 +1280,7 @@ __global__ void __launch_bounds__(256) k_group_fold(const PlanDesc *plans, const
     }
   }
   cs[0] = v;
-  cs[1] = 0.0;
+  cs[1] = as_f64((uint64_t)exact);
   const_cast<uint32_t *>(P.a.chunk_sel)[ci] = take;
 }
 
@@ -1155,7 +1297,7 @@ __global__ void __launch_bounds__(256) k_group_fold(const PlanDesc *plans, const
 constexpr uint32_t kFinStage = 2048;  // chunks per LDS stage (4096 for the in-order sums)
 __global__ void __launch_bounds__(256) k_group_finish(const PlanDesc *plans) {
   __shared__ double2 buf[kFinStage];
-  __shared__ uint32_t tk[kFinStage];
+  __shared__ uint8_t tk[kFinStage];
   const PlanDesc &P = plans[blockIdx.x];
   if (!(P.flags & kPfFold)) return;  // block-uniform
   const int agg = (int)((P.flags >> kPfAggShift) & 3u);
@@ -1163,6 +1305,61 @@ __global__ void __launch_bounds__(256) k_group_finish(const PlanDesc *plans) {
   const uint32_t nc = P.n_chunks;
   const double *cs = P.chunk_sum;
   const uint32_t *take = P.a.chunk_sel;
+  if (agg == 1 && !(P.flags & kPfF64Sum) && P.scheme == kCodecBUFF) {
+    // the exact integer total when it applies (one frac, magnitudes < 2^53)
+    __shared__ uint64_t r_sum[4], r_abs[4];
+    __shared__ uint32_t r_fmin[4], r_fmax[4], r_bad[4];
+    uint64_t sum = 0;  // wrapping; exact whenever the magnitudes stay below 2^53
+    uint64_t mag = 0;
+    uint32_t fmin = ~0u, fmax = 0, bad = 0;
+    for (uint32_t k = tid; k < nc; k += blockDim.x) {
+      if (!take[k]) continue;
+      const int64_t x = (int64_t)as_u64(cs[2ull * k + 1]);
+      if (x == kNotExact) {
+        bad = 1;
+        continue;
+      }
+      const uint32_t f = P.view->chunks[k].frac;
+      fmin = min(fmin, f);
+      fmax = max(fmax, f);
+      sum += (uint64_t)x;
+      mag += x < 0 ? 0ull - (uint64_t)x : (uint64_t)x;
+      if (mag >= kExactLimit) {
+        bad = 1;
+        mag = kExactLimit;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      sum += (uint64_t)__shfl_xor((unsigned long long)sum, o, kWave);
+      mag = min(mag + (uint64_t)__shfl_xor((unsigned long long)mag, o, kWave), kExactLimit);
+      fmin = min(fmin, (uint32_t)__shfl_xor((int)fmin, o, kWave));
+      fmax = max(fmax, (uint32_t)__shfl_xor((int)fmax, o, kWave));
+      bad |= (uint32_t)__shfl_xor((int)bad, o, kWave);
+    }
+    if (lane == 0) {
+      const uint32_t w = tid / kWave;
+      r_sum[w] = sum;
+      r_abs[w] = mag;
+      r_fmin[w] = fmin;
+      r_fmax[w] = fmax;
+      r_bad[w] = bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (uint32_t w = 1; w < blockDim.x / kWave; ++w) {
+        sum += r_sum[w];
+        mag = min(mag + r_abs[w], kExactLimit);
+        fmin = min(fmin, r_fmin[w]);
+        fmax = max(fmax, r_fmax[w]);
+        bad |= r_bad[w];
+      }
+      r_bad[0] = bad || mag >= kExactLimit || (fmin != ~0u && fmin != fmax);
+      if (!r_bad[0]) P.result->scalar = fmin == ~0u ? 0.0 : ldexp((double)(int64_t)sum, -(int)fmin);
+    }
+    __syncthreads();
+    if (!r_bad[0]) return;  // block-uniform
+  }
   if (agg == 1 && !(P.flags & kPfF64Sum)) {
     double *seq = reinterpret_cast<double *>(buf);
     constexpr uint32_t cap = 2 * kFinStage;
@@ -1205,7 +1402,7 @@ __global__ void __launch_bounds__(256) k_group_finish(const PlanDesc *plans) {
     const uint32_t m = min(kFinStage, nc - k0);
 #pragma unroll 8
     for (uint32_t i = tid; i < m; i += blockDim.x) {
-      tk[i] = take[k0 + i];
+      tk[i] = take[k0 + i] != 0;
       buf[i] = make_double2(cs[2ull * (k0 + i)], cs[2ull * (k0 + i) + 1]);
     }
     __syncthreads();
@@ -1290,6 +1487,8 @@ void group_tiles(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, u
     hipLaunchKernelGGL(k_group_tiles<kSpanBuff>, grid, block, 0, s, plans, prefix, n_plans, n_jobs);
   else if (kind == kSpanSprintz)
     hipLaunchKernelGGL(k_group_tiles<kSpanSprintz>, grid, block, 0, s, plans, prefix, n_plans, n_jobs);
+  else if (kind == kSpanRle)
+    hipLaunchKernelGGL(k_group_tiles<kSpanRle>, grid, block, 0, s, plans, prefix, n_plans, n_jobs);
   else
     hipLaunchKernelGGL(k_group_tiles<kSpanValues>, grid, block, 0, s, plans, prefix, n_plans, n_jobs);
 }

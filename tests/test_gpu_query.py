@@ -98,8 +98,17 @@ def test_random_parity(ctx, codec, gen, n, chunk):
                 c = o.sum_compensated(m, cid)
                 assert abs(g - c) <= 1e-12 * max(abs(c), 1e-300) or g == c, ("sum", g, c)
             for p in _preds(vals, rng):
-                assert np.array_equal(d.filter(p, m, cid), o.filter(p, m, cid)), (p, cid)
+                ids = o.filter(p, m, cid)
+                assert np.array_equal(d.filter(p, m, cid), ids), (p, cid)
                 assert same_bits(d.filter_materialize(p, m, cid), o.filter_materialize(p, m, cid)), (p, cid)
+                # the fused filter + sum: sum(Some(filter(..))) of the reference's Decoder
+                g = E.run_query(ctx, E.Query(E.OP_FILTER_SUM, d, predicate=p, bitmask=m, chunk_id=cid))
+                w = o.sum(ids, cid)
+                if codec in ("buff", "delta_sprintz"):
+                    assert same_bits([g], [w]), ("filter_sum", p, g, w)
+                else:
+                    c = o.sum_compensated(ids, cid)
+                    assert abs(g - c) <= 1e-12 * max(abs(c), 1e-300) or g == c, ("filter_sum", p, g, c)
 
 
 @pytest.mark.parametrize("codec", ["uncompressed", "gorilla", "buff", "chimp128"])
@@ -122,3 +131,23 @@ def test_materialize_many_blocks(ctx, codec):
     p = E.Predicate.range(E.RangeValue.inclusive(float(q[0])), E.RangeValue.exclusive(float(q[1])))
     assert same_bits(d.filter_materialize(p, None), o.filter_materialize(p, None))
     d.close()
+
+
+@pytest.mark.parametrize("center,spread", [(5e11, 1e9), (1e9, 1e6)])
+def test_buff_sum_past_exact_integers(ctx, center, spread):
+    """BUFF sums whose chunk integers (first case) or whose total magnitude
+    (second) pass 2^53 take the in-order f64 fold instead of the exact
+    integer total; both must equal the reference's `result += v`, bit for
+    bit, with and without a bitmask, through sum and the fused filter + sum.
+    The first case also runs the wide-delta (B >= 24) path."""
+    rng = np.random.default_rng(int(center) % 1000)
+    vals = np.round(center + rng.uniform(-spread, spread, 60000), 3)
+    data = enc(vals, "buff", 1000, E.ChunkOption.record_count(1000))
+    d, o = E.Decoder(ctx, data), OracleDecoder(data)
+    m = np.flatnonzero(rng.random(vals.size) < 0.4)
+    p = E.Predicate.range(E.RangeValue.inclusive(center - spread / 2), E.RangeValue.none())
+    for mask in (None, m):
+        assert same_bits([d.sum(mask)], [o.sum(mask)])
+        ids = o.filter(p, mask)
+        got = E.run_query(ctx, E.Query(E.OP_FILTER_SUM, d, predicate=p, bitmask=mask))
+        assert same_bits([got], [o.sum(ids)])

@@ -18,6 +18,7 @@ KERNELS = {  # mangled-name prefix -> minimum occupancy (waves per SIMD)
     "_ZN3ebi13k_group_tilesILi0E": 6,  # query spans on BUFF codes
     "_ZN3ebi13k_group_tilesILi1E": 6,  # query spans on DeltaSprintz codes
     "_ZN3ebi13k_group_tilesILi2E": 7,  # query spans on the bitmask / decoded records
+    "_ZN3ebi13k_group_tilesILi3E": 6,  # query spans on RLE runs
     "_ZN3ebi14k_group_finishE": 4,  # the per-plan fold
 }
 
