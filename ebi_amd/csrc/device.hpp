@@ -240,6 +240,7 @@ enum PlanFlags : uint32_t {
   kPfMask = 1u << 9,     // a bitmask was given
   kPfF64Sum = 1u << 10,  // the sum is over f64 records (double-double tile partials)
   kPfFieldTiles = 1u << 11,  // RLE queried per run: the tile pass walks field tiles (partials per field tile)
+  kPfCpred = 1u << 12,       // the selection pass writes each chunk's ChunkPred (BUFF / DeltaSprintz)
 };
 constexpr uint32_t kPfAggShift = 4;  // bits 4-5: 0 none, 1 sum, 2 min, 3 max
 

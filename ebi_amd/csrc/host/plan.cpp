@@ -16,86 +16,9 @@ namespace {
 
 uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-int64_t sat_i64(double d) {
-  if (std::isnan(d)) return 0;
-  if (d >= 9223372036854775808.0) return INT64_MAX;
-  if (d < -9223372036854775808.0) return INT64_MIN;
-  return (int64_t)d;
-}
 // Rust f64::min / f64::max: a NaN operand yields the other operand.
 double rmin(double a, double b) { return std::isnan(a) ? b : std::isnan(b) ? a : (b < a ? b : a); }
 double rmax(double a, double b) { return std::isnan(a) ? b : std::isnan(b) ? a : (b > a ? b : a); }
-
-// Per-chunk BUFF predicate parameters: the early exits of
-// buff/filter_cmp.rs:328-351 and buff/filter_eq.rs:103-112, then the
-// predicate delta in the chunk's fixed-point domain.
-ChunkPred buff_chunk_pred(const DevChunk &d, const ebi_pred &p, bool has_mask) {
-  ChunkPred cp{};
-  const int32_t frac = (int32_t)d.frac;
-  const uint64_t wmask = d.bits >= 64 ? ~0ull : ((1ull << d.bits) - 1);
-  if (p.kind == EBI_PRED_EQ || p.kind == EBI_PRED_NE) {
-    const int64_t fp = buff_fixed(p.value, frac);
-    if (fp < d.base) {
-      cp.lmode = kSideEmpty;
-    } else if (d.bits == 0) {
-      cp.lmode = fp == d.base ? kSideAll : kSideEmpty;
-    } else {
-      cp.lmode = kSideCmp;
-      const uint64_t dp = (uint64_t)fp - (uint64_t)d.base;
-      if (d.bits < 8 && !has_mask) {
-        cp.eq_u8 = 1;  // filter_eq.rs:188-195 compares with `dp as u8`
-        cp.ldp = dp;
-      } else {
-        cp.ldp = dp & wmask;  // planes and tail see only the low B bits
-      }
-    }
-    return cp;
-  }
-  auto side = [&](int kind, double v, bool greater, uint32_t *mode, uint64_t *dpo, uint32_t *incl) {
-    if (kind == EBI_BOUND_NONE) {
-      *mode = kSideNone;
-      return;
-    }
-    const bool inc = kind == EBI_BOUND_INCLUSIVE;
-    *incl = inc;
-    const int64_t fp = buff_fixed(v, frac);
-    if (fp < d.base) {
-      *mode = greater ? kSideAll : kSideEmpty;
-      return;
-    }
-    if (d.bits == 0) {
-      const bool all = greater ? (inc ? d.base >= fp : d.base > fp) : (inc ? d.base <= fp : d.base < fp);
-      *mode = all ? kSideAll : kSideEmpty;
-      return;
-    }
-    const uint64_t dp = (uint64_t)fp - (uint64_t)d.base;
-    const uint32_t pbits = dp == 0 ? 0 : 64 - (uint32_t)__builtin_clzll(dp);
-    if (d.bits < pbits) {
-      *mode = greater ? kSideEmpty : kSideAll;
-      return;
-    }
-    *mode = kSideCmp;
-    *dpo = dp;
-  };
-  side(p.start_kind, p.start, true, &cp.lmode, &cp.ldp, &cp.lincl);
-  side(p.end_kind, p.end, false, &cp.rmode, &cp.rdp, &cp.rincl);
-  return cp;
-}
-
-// DeltaSprintz: predicate.map(|v| (v * scale).round() as i64), sprintz.rs:116.
-ChunkPred sprintz_chunk_pred(const DevChunk &d, const ebi_pred &p) {
-  ChunkPred cp{};
-  const double sc = (double)d.scale;
-  if (p.kind == EBI_PRED_EQ || p.kind == EBI_PRED_NE) {
-    cp.ldp = (uint64_t)sat_i64(std::round(p.value * sc));
-    return cp;
-  }
-  cp.lmode = (uint32_t)p.start_kind;
-  cp.rmode = (uint32_t)p.end_kind;
-  cp.ldp = (uint64_t)sat_i64(std::round(p.start * sc));
-  cp.rdp = (uint64_t)sat_i64(std::round(p.end * sc));
-  return cp;
-}
 
 bool is_agg(int op) { return op == kOpSum || op == kOpMin || op == kOpMax || op == kOpFilterSum; }
 bool has_pred(int op) { return op == kOpFilter || op == kOpFilterMaterialize || op == kOpFilterSum; }
@@ -121,7 +44,7 @@ Layout layout_of(const Plan &p) {
   L.sel = take((uint64_t)p.nc * 4);
   L.acc = agg ? take((uint64_t)p.nc * 32) : ~0ull;
   L.csum = (p.flags & kPfFold) ? take((uint64_t)p.nc * 16) : ~0ull;
-  L.cpred = p.cpred ? take((uint64_t)p.nc * sizeof(ChunkPred)) : ~0ull;
+  L.cpred = p.cpred ? take((uint64_t)p.nc * sizeof(ChunkPred)) : ~0ull;  // written by the selection pass
   L.part = (p.flags & kPfF64Sum) ? take((uint64_t)p.c->n_rtiles * 16) : ~0ull;
   L.block = p.compact ? take(((uint64_t)p.n_blocks + 1) * 8) : ~0ull;
   // DECODE writes the caller's buffer; fused plans keep the records in registers
@@ -131,9 +54,9 @@ Layout layout_of(const Plan &p) {
   return L;
 }
 
-// Pinned staging: the bitmask copy, the FILTER result, the chunk predicates.
+// Pinned staging: the bitmask copy, the FILTER result.
 struct PinLayout {
-  uint64_t mask, bits, cpred, total;
+  uint64_t mask, bits, total;
 };
 
 PinLayout pin_layout_of(const Plan &p) {
@@ -146,7 +69,6 @@ PinLayout pin_layout_of(const Plan &p) {
   };
   L.mask = p.has_mask && !p.direct ? take(p.words * 8) : ~0ull;
   L.bits = p.q.op == kOpFilter && !p.direct ? take(p.words * 8) : ~0ull;
-  L.cpred = p.cpred ? take((uint64_t)p.nc * sizeof(ChunkPred)) : ~0ull;
   L.total = off;
   return L;
 }
@@ -272,6 +194,7 @@ int plan_init(Plan &p, const ebi_query &q) {
       break;
   }
   if (agg == 1 && !p.insitu) f |= kPfF64Sum;
+  if (p.cpred) f |= kPfCpred;
   p.flags = f;
   // the record-tile kernel of the plan: on BUFF / DeltaSprintz codes when it
   // filters in their domain or aggregates, on RLE runs when it needs no
@@ -305,18 +228,10 @@ void plan_bind(Plan &p, uint8_t *dev, uint8_t *pin) {
   const PinLayout P = pin_layout_of(p);
   p.h_mask = p.direct ? const_cast<uint64_t *>(p.q.bitset) : at<uint64_t>(pin, P.mask);
   p.h_bits = p.direct ? p.q.out_bitset : at<uint64_t>(pin, P.bits);
-  p.h_cpred = at<ChunkPred>(pin, P.cpred);
 }
 
 void plan_stage(Plan &p) {
   if (p.has_mask && p.words && !p.direct) std::memcpy(p.h_mask, p.q.bitset, p.words * 8);
-  if (p.cpred) {
-    const bool buff = p.c->meta.scheme == kBUFF;
-    for (uint32_t k = 0; k < p.nc; ++k) {
-      const DevChunk &d = p.c->chunks[k].dev;
-      p.h_cpred[k] = buff ? buff_chunk_pred(d, p.q.pred, p.has_mask) : sprintz_chunk_pred(d, p.q.pred);
-    }
-  }
   p.n_out = 0;
 }
 
@@ -508,17 +423,13 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
     if (p.has_mask && p.words &&
         (e = hipMemcpyAsync(p.mask, p.h_mask, p.words * 8, hipMemcpyHostToDevice, s)) != hipSuccess)
       return e;
-    if (p.cpred && p.nc &&
-        (e = hipMemcpyAsync(p.d_cpred, p.h_cpred, (size_t)p.nc * sizeof(ChunkPred), hipMemcpyHostToDevice, s)) !=
-            hipSuccess)
-      return e;
   }
   const PlanDesc *dd = reinterpret_cast<const PlanDesc *>(g.dev + L.descs);
   const uint32_t *di = reinterpret_cast<const uint32_t *>(g.dev + L.ipre);
   const uint32_t np = (uint32_t)n;
   const uint32_t max_blocks = (uint32_t)cu_count * blocks_per_cu;
-  launch::group_fill(s, reinterpret_cast<const GroupFill *>(g.dev + L.fills), n_fills, max_words);
-  launch::group_select(s, dd, di, np, (uint32_t)chunks);
+  launch::group_prep(s, reinterpret_cast<const GroupFill *>(g.dev + L.fills), n_fills, max_words, dd, di, np,
+                     (uint32_t)chunks);
   launch::decode(s, reinterpret_cast<const ColView *>(g.dev + L.views), np,
                  reinterpret_cast<const uint32_t *>(g.dev + L.dpre), (uint32_t)jobs,
                  reinterpret_cast<const uint32_t *>(g.dev + L.dseg), (uint32_t)segs, g.d_flags, max_blocks);

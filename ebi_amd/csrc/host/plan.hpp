@@ -13,8 +13,7 @@
 //   plan_init   validate the request, derive what it runs (host only)
 //   plan_need   device and pinned bytes of its scratch
 //   plan_bind   carve that scratch out of caller-provided memory
-//   plan_stage  host staging into pinned memory (input bitmask copy, BUFF /
-//               DeltaSprintz per-chunk predicate parameters)
+//   plan_stage  host staging into pinned memory (input bitmask copy)
 //   group_enqueue (per group, on one stream): one upload of the group block
 //               (views, descriptors, prefixes, fills), each plan's bitmask /
 //               predicate upload, then the group kernels: scratch fills,
@@ -68,7 +67,7 @@ struct Plan {
                            // buffers and the device directly (no staging of them)
   bool compact = false;    // materialize through the bitset and compaction (a bitmask, or a
                            // column whose chunks end at NaNs: its records are not contiguous)
-  bool cpred = false;      // per-chunk predicate parameters are uploaded
+  bool cpred = false;      // per-chunk predicate parameters (computed by the selection pass)
   uint32_t flags = 0;      // PlanFlags (device.hpp)
   int filter_mode = 0;     // record-tile filter mode (0 select, 1 code domain / f64, 2 f64)
   int tile_kind = 0;       // SpanKind (device.hpp): which record-tile kernel runs the plan
@@ -82,7 +81,6 @@ struct Plan {
   double *partials = nullptr, *chunk_sum = nullptr, *values = nullptr, *out = nullptr;
   // pinned staging
   uint64_t *h_mask = nullptr, *h_bits = nullptr;
-  ChunkPred *h_cpred = nullptr;
   // in the group block (set by group_bind)
   const ColView *d_view = nullptr;
   PlanResult *d_result = nullptr, *h_result = nullptr;

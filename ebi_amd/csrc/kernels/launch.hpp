@@ -51,13 +51,13 @@ void aggregate(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_j
                uint32_t n_chunks);
 // The kernels of a group of query plans (PlanDesc array in device memory,
 // flat work items mapped to plans by prefix arrays of n_plans + 1 entries):
-// scratch fills, chunk selection (items: chunks), the record-tile pass
-// (items: record tiles), the fused decode + query over segment jobs (items:
+// scratch fills and chunk selection in one launch (items: chunks), the
+// record-tile span pass of one SpanKind (items: span jobs), the fused decode + query over segment jobs (items:
 // segment jobs; persistent grid of at most max_blocks), the per-chunk fold
 // and the final fold (items: chunks), and stream compaction (items: 32,768-
 // record blocks).
-void group_fill(hipStream_t s, const GroupFill *fills, uint32_t n_fills, uint64_t max_words);
-void group_select(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_items);
+void group_prep(hipStream_t s, const GroupFill *fills, uint32_t n_fills, uint64_t max_words, const PlanDesc *plans,
+                const uint32_t *prefix, uint32_t n_plans, uint32_t n_items);
 void group_tiles(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
                  int kind);
 void query_seg(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,

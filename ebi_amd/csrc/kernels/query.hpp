@@ -8,6 +8,7 @@
 // readers cited on each function (paths relative to lemolatoon/ebi).
 #pragma once
 
+#include "../chunk_pred.hpp"
 #include "codecs.hpp"
 
 namespace ebi {
@@ -469,20 +470,25 @@ __device__ __forceinline__ uint32_t plan_of(const uint32_t *prefix, uint32_t n_p
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-// Scratch fills (zeroed bitsets, accumulator patterns): block row y = fill y.
-__global__ void __launch_bounds__(256) k_group_fill(const GroupFill *fills) {
-  const GroupFill f = fills[blockIdx.y];
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t r = f.period == 4 ? (uint32_t)(i & 3) : 0u;
-    f.p[i] = r == 0 ? f.v[0] : r == 1 ? f.v[1] : r == 2 ? f.v[2] : f.v[3];
+// The group's first launch, two independent parts in one 1-D grid:
+//  * blocks [0, n_fills * fbx): scratch fills (zeroed bitsets, accumulator
+//    patterns), fbx blocks per fill;
+//  * the rest: chunk selection of every plan (chunk_id, bitmask pruning;
+//    api/decoder.rs:216-226), one wave per (plan, chunk), which also writes
+//    the chunk's in-situ predicate parameters (ChunkPred) for kPfCpred plans.
+__global__ void __launch_bounds__(256) k_group_prep(const GroupFill *fills, uint32_t n_fills, uint32_t fbx,
+                                                   const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans,
+                                                   uint32_t n_items) {
+  if (blockIdx.x < n_fills * fbx) {  // block-uniform
+    const GroupFill f = fills[blockIdx.x / fbx];
+    const uint64_t step = (uint64_t)fbx * blockDim.x;
+    for (uint64_t i = (uint64_t)(blockIdx.x % fbx) * blockDim.x + threadIdx.x; i < f.n; i += step) {
+      const uint32_t r = f.period == 4 ? (uint32_t)(i & 3) : 0u;
+      f.p[i] = r == 0 ? f.v[0] : r == 1 ? f.v[1] : r == 2 ? f.v[2] : f.v[3];
+    }
+    return;
   }
-}
-
-// Chunk selection of every plan (chunk_id, bitmask pruning; api/decoder.rs:
-// 216-226): one wave per (plan, chunk).
-__global__ void __launch_bounds__(256) k_group_select(const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans,
-                                                     uint32_t n_items) {
-  const uint32_t j = blockIdx.x * 4 + threadIdx.x / kWave;
+  const uint32_t j = (blockIdx.x - n_fills * fbx) * 4 + threadIdx.x / kWave;
   if (j >= n_items) return;
   const uint32_t p = plan_of(prefix, n_plans, j);
   const PlanDesc &P = plans[p];
@@ -502,7 +508,14 @@ __global__ void __launch_bounds__(256) k_group_select(const PlanDesc *plans, con
     }
     s = __any(any != 0) ? 1u : 0u;
   }
-  if (lane_id() == 0) const_cast<uint32_t *>(P.a.chunk_sel)[ci] = s;
+  if (lane_id() != 0) return;
+  const_cast<uint32_t *>(P.a.chunk_sel)[ci] = s;
+  if (P.flags & kPfCpred) {
+    const DevChunk c = P.view->chunks[ci];
+    const ebi_pred pr{P.a.kind, P.a.start_kind, P.a.end_kind, 0, P.a.value, P.a.start, P.a.end};
+    const_cast<ChunkPred *>(P.a.cpred)[ci] =
+        P.scheme == kCodecBUFF ? buff_chunk_pred(c, pr, mask != nullptr) : sprintz_chunk_pred(c, pr);
+  }
 }
 
 // ------------------------------------------------------- record-tile spans
@@ -1470,14 +1483,13 @@ __global__ void __launch_bounds__(256) k_group_scatter(const PlanDesc *plans, co
 
 namespace launch {
 
-void group_fill(hipStream_t s, const GroupFill *fills, uint32_t n_fills, uint64_t max_words) {
-  if (!n_fills) return;
-  const uint32_t bx = (uint32_t)std::min<uint64_t>((max_words + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_group_fill, dim3(std::max(bx, 1u), n_fills), dim3(256), 0, s, fills);
-}
-void group_select(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_items) {
-  if (n_items)
-    hipLaunchKernelGGL(k_group_select, dim3((n_items + 3) / 4), dim3(256), 0, s, plans, prefix, n_plans, n_items);
+void group_prep(hipStream_t s, const GroupFill *fills, uint32_t n_fills, uint64_t max_words, const PlanDesc *plans,
+                const uint32_t *prefix, uint32_t n_plans, uint32_t n_items) {
+  const uint32_t fbx = n_fills ? (uint32_t)std::max<uint64_t>(std::min<uint64_t>((max_words + 255) / 256, 1024), 1) : 0;
+  const uint64_t blocks = (uint64_t)n_fills * fbx + (n_items + 3) / 4;
+  if (blocks)
+    hipLaunchKernelGGL(k_group_prep, dim3((uint32_t)blocks), dim3(256), 0, s, fills, n_fills, std::max(fbx, 1u), plans,
+                       prefix, n_plans, n_items);
 }
 void group_tiles(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
                  int kind) {
