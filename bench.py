@@ -94,6 +94,7 @@ def main():
            "par_seconds": 0.0, "par_bytes": 0, "par_chunks": 0, "par_threads": cpu_threads(), "core": None}
     comparator_files = {}
     sample_q = {}  # generator -> every 64th value, for predicate quantiles
+    cold = {}  # codec -> the random walk column's cold load and first decode
     t_setup = time.time()
     for g in gens:
         fn, dec = G.GENERATORS[g]
@@ -102,8 +103,13 @@ def main():
         for codec in codecs:
             cfg = E.CompressorConfig.by_name(codec, scale=10 ** dec)
             data = E.encode(vals, cfg, E.ChunkOption.record_count(args.chunk), threads=threads)
+            t = time.perf_counter()
             d = E.Decoder(ctx, data)
+            load_ms = (time.perf_counter() - t) * 1e3
+            phases_ms = [ctx.last_phase_ms(i) for i in range(3)]
             cols.append((g, codec, d, d.compressed_bytes))
+            if g == "walk2":
+                cold[codec] = cold_entry(E, ctx, d, n, load_ms, phases_ms)
             log(f"[setup] {g:6s} {codec:14s} {len(data) / n:6.3f} B/value  loaded")
             if g == "walk2" and codec in ("gorilla", "chimp"):
                 comparator_files[codec] = E.encode(vals[: args.chunk * args.comparator_chunks], cfg,
@@ -185,6 +191,8 @@ def main():
         o.free()
     outs = []
     query = serving = matmul = c1 = None
+    cold_obj = {"workload": f"2^{args.log2n} f64 2-decimal random walk per codec, {args.chunk} records/chunk, "
+                            "file bytes in host memory", "codecs": cold} if cold else None
     if not args.no_query:
         query = query_leg(E, ctx, cols, sample_q, n)
         c1 = c1_leg(E, G, ctx)
@@ -238,6 +246,7 @@ def main():
                        for k, v in per_kernel.items()},
         "per_chunk_launch_comparator": comparator,
         "query": query,
+        "cold": cold_obj,
         "cpu_plain_case": c1,
         "serving": serving,
         "matmul": matmul,
@@ -269,6 +278,28 @@ def main():
 
 def pct(xs, q):
     return round(float(np.percentile(np.asarray(xs), q)), 4) if len(xs) else None
+
+
+def cold_entry(E, ctx, d, n, load_ms, phases_ms):
+    """The cold path of one column, as the reference's Decoder::new +
+    materialize meets it: the file's bytes in host memory are parsed,
+    repacked and uploaded, the load-time index is built (tile prefixes;
+    XOR checkpoints by a lane per chunk; the lane-interleaved stream copy;
+    Chimp128's ring checkpoints by a wavefront over its jobs), then the first
+    decode of the whole column into HBM.  index_bytes_per_record: the
+    resident bytes beside the chunk bodies (checkpoints, interleaved copy,
+    ring, tile prefixes, descriptors, job lists) per record."""
+    out = ctx.alloc(8 * n)
+    E.run_query(ctx, E.Query(E.OP_DECODE, d, out=out))
+    dec_ms = ctx.last_device_ms
+    out.free()
+    cb = d.compressed_bytes
+    dev_ms = phases_ms[1] + phases_ms[2]
+    return {"load_ms": round(load_ms, 2), "upload_ms": round(phases_ms[0], 2), "index_ms": round(phases_ms[1], 2),
+            "copy_ring_ms": round(phases_ms[2], 2), "first_decode_ms": round(dec_ms, 3),
+            "index_over_decode": round(dev_ms / dec_ms, 2) if dec_ms > 0 else None,
+            "index_bytes_per_record": round((d.device_bytes - cb) / n, 3),
+            "cold_gbs_device": round((cb + 8 * n) / ((dev_ms + dec_ms) / 1e3) / 1e9, 1)}
 
 
 QUERY_CODECS = ("buff", "delta_sprintz", "rle", "uncompressed", "gorilla", "chimp", "elf", "chimp128")

@@ -175,6 +175,7 @@ _SIGS = {
     "ebi_hip_column_free": (C.c_int, [_P, _P]),
     "ebi_hip_column_info": (C.c_int, [_P, C.POINTER(_FileInfo)]),
     "ebi_hip_column_compressed_bytes": (C.c_uint64, [_P]),
+    "ebi_hip_column_device_bytes": (C.c_uint64, [_P]),
     "ebi_hip_column_set_precision": (C.c_int, [_P, C.c_int32]),
     "ebi_hip_decode_device": (C.c_int, [_P, _P, _P]),
     "ebi_hip_decode_per_chunk": (C.c_int, [_P, _P, _P]),
@@ -605,6 +606,11 @@ class Decoder:
     @property
     def compressed_bytes(self) -> int:
         return int(lib().ebi_hip_column_compressed_bytes(self._h))
+
+    @property
+    def device_bytes(self) -> int:
+        """Device bytes the resident column holds (bodies + load-time index)."""
+        return int(lib().ebi_hip_column_device_bytes(self._h))
 
     def with_precision(self, precision: int) -> None:
         """Decoder::with_precision (api/decoder.rs:159-161): BUFF decodes keep

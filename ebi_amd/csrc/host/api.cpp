@@ -7,6 +7,7 @@
 // does; decode_device is the whole-column materialize (api/decoder.rs:202-237)
 // written to HBM.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -247,28 +248,35 @@ static int build_ilv(ebi_hip_ctx *ctx, Column &col) {
   return EBI_OK;
 }
 
-// Ring checkpoints of a Chimp128 column (after the index pass): the column
-// is decoded once by the lane-per-chunk decoder into a temporary buffer, and
-// each segment job keeps the values of the 128 records before it
-// (128 * 8 bytes per 1024 records).
+// Ring checkpoints of a Chimp128 column (after the index pass): each
+// segment job keeps the values of the 128 records before it (128 * 8 bytes
+// per 1024 records).  Built as a wavefront over the jobs of every chunk:
+// step s decodes job s of each chunk (one wave per chunk) from the ring
+// step s - 1 wrote and writes the ring of job s + 1, so the column is
+// decoded once, in max(jobs per chunk) launches of up to one wave per chunk.
+// The decode validates the values too (the parse-only index cannot see a
+// wrong terminator) and lowers each chunk's end to its first NaN.
 static int build_c128_ring(ebi_hip_ctx *ctx, Column &col) {
-  const uint64_t N = col.meta.number_of_records;
-  double *tmp = nullptr;
-  hipError_t e = hipMalloc((void **)&col.d_c128_ring, (size_t)col.n_seg_jobs * 128 * 8);
-  if (e == hipSuccess) e = hipMalloc((void **)&tmp, std::max<uint64_t>(N, 1) * 8);
-  if (e != hipSuccess) {
-    hipFree(tmp);
-    return hip_fail(ctx, e, "hipMalloc(Chimp128 ring checkpoints)");
-  }
-  int rc = upload_view(ctx, col, tmp);
   const size_t nc = col.chunks.size();
+  std::vector<uint32_t> first(nc + 1, 0);
+  for (const Job &jb : col.h_seg_jobs) ++first[jb.chunk + 1];
+  uint32_t steps = 0;
+  for (size_t k = 0; k < nc; ++k) {
+    steps = std::max(steps, first[k + 1]);
+    first[k + 1] += first[k];
+  }
+  uint32_t *d_first = nullptr;
+  hipError_t e = hipMalloc((void **)&col.d_c128_ring, (size_t)col.n_seg_jobs * 128 * 8);
+  if (e == hipSuccess) e = upload_vec(&d_first, first);
+  if (e != hipSuccess) {
+    hipFree(d_first);
+    return hip_fail(ctx, e, "Chimp128 ring checkpoints: allocation");
+  }
+  int rc = upload_view(ctx, col, nullptr);
   std::vector<uint32_t> errs(nc);
   if (!rc) {
-    // the decode validates the values too (the parse-only index cannot see
-    // a wrong terminator): the chunks it rejects are marked like the index
-    // pass's, and the ring gather skips them
-    launch::chimp128(ctx->stream, col.d_view, col.d_prefix + 2, 1, col.n_serial_jobs, nullptr, ctx->d_flags, true);
-    launch::c128_ring(ctx->stream, col.d_view, col.n_seg_jobs, col.d_c128_ring);
+    for (uint32_t s = 0; s < steps; ++s)
+      launch::c128_ring_step(ctx->stream, col.d_view, d_first, (uint32_t)nc, s, col.d_c128_ring, ctx->d_flags);
     e = hipGetLastError();
     if (e == hipSuccess && nc)
       e = hipMemcpyAsync(errs.data(), col.d_chunk_err, nc * 4, hipMemcpyDeviceToHost, ctx->stream);
@@ -278,14 +286,19 @@ static int build_c128_ring(ebi_hip_ctx *ctx, Column &col) {
   }
   if (!rc)
     for (size_t k = 0; k < nc; ++k) col.chunks[k].err |= errs[k];
-  hipFree(tmp);
+  hipFree(d_first);
   if (!rc) col.device_bytes += (uint64_t)col.n_seg_jobs * 128 * 8;
   return rc;
 }
 
 // Parse, validate codec headers, repack into the device layout, upload and
 // build the tile / checkpoint index.
+static double ms_since(std::chrono::steady_clock::time_point t) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
+
 static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column &col) {
+  const auto t0 = std::chrono::steady_clock::now();
   std::string err;
   int rc = parse_file(file, len, &col.meta, &err);
   if (rc) return fail(ctx, rc, err);
@@ -507,6 +520,8 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
     if ((e = upload_vec(&col.d_prefix, pre)) != hipSuccess) return hip_fail(ctx, e, "upload prefix");
   }
   // Load-time index pass.
+  const double upload_ms = ms_since(t0);
+  const auto t1 = std::chrono::steady_clock::now();
   launch::zero_u32(ctx->stream, col.d_chunk_err, (uint32_t)nc);
   launch::zero_u32(ctx->stream, ctx->d_flags, 2);
   if ((rc = upload_view(ctx, col, nullptr))) return rc;
@@ -527,6 +542,8 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return hip_fail(ctx, e, "index pass");
   for (size_t k = 0; k < nc; ++k) col.chunks[k].err = errs[k];
+  const double index_ms = ms_since(t1);
+  const auto t2 = std::chrono::steady_clock::now();
   // Gorilla / Chimp / Elf: the index pass found the NaN ends; Chimp128's
   // parse-only index cannot see values, its ring build's decode finds them
   if (col.xor_codec && (rc = apply_chunk_ends(ctx, col))) return rc;
@@ -564,6 +581,18 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
         (e = hipMemcpy(col.d_prefix + 1, &jobs, 4, hipMemcpyHostToDevice)) != hipSuccess)
       return hip_fail(ctx, e, "RLE job lists");
   }
+  // resident bytes beside the chunk bodies (and the ilv / ring counted as built)
+  col.device_bytes += col.ckpt_entries * sizeof(XorCkpt) + col.tile_index_entries * 8 + nc * sizeof(DevChunk) +
+                      (uint64_t)(col.h_tile_jobs.size() + col.h_serial_jobs.size() + col.h_seg_jobs.size() +
+                                 col.n_rec_jobs + col.n_span_jobs + col.n_field_spans) *
+                          sizeof(Job);
+  // host clock of the load's phases: parse + repack + upload, the index
+  // pass, the lane-interleaved copy / Chimp128 ring / job sizing
+  ctx->phase_ms[0] = upload_ms;
+  ctx->phase_ms[1] = index_ms;
+  ctx->phase_ms[2] = ms_since(t2);
+  ctx->phase_ms[3] = 0.0;
+  ctx->last_ms = ms_since(t0);
   return EBI_OK;
 }
 
@@ -722,6 +751,8 @@ extern "C" int ebi_hip_column_set_precision(ebi_hip_col *col, int32_t precision)
   col->c.precision = precision;
   return EBI_OK;
 }
+
+extern "C" uint64_t ebi_hip_column_device_bytes(const ebi_hip_col *col) { return col ? col->c.device_bytes : 0; }
 
 extern "C" uint64_t ebi_hip_column_compressed_bytes(const ebi_hip_col *col) {
   return col ? col->c.compressed_bytes : 0;

@@ -151,17 +151,21 @@ __host__ __device__ __forceinline__ double elf_recover_tab(double v, int32_t bet
 // b = T; Chimp: v, a = stored lead; Elf: v = stored bits, a = L, b = T,
 // beta = last beta*).  A decoder called on records [i0, i1) leaves the
 // state at record i1, so a segment can be decoded in several runs.
-struct XorState {
-  BitWindow r;
+template <class W>
+struct XorStateT {
+  W r;
   uint64_t v;
   uint32_t a, b;
   int32_t beta;
   uint32_t end;  // set with kNanEnd: the record index of the NaN end marker
 };
+using XorState = XorStateT<BitWindow>;        // lockstep segments
+using SerialState = XorStateT<SerialWindow>;  // lane-per-chunk runs (deep prefetch)
 
 // Decoder state (all but the window) at a checkpoint written by the index
 // pass (255 / -1 encode "unset").
-__host__ __device__ __forceinline__ void xor_state_set(XorState &s, uint32_t codec, const XorCkpt &k) {
+template <class S>
+__host__ __device__ __forceinline__ void xor_state_set(S &s, uint32_t codec, const XorCkpt &k) {
   s.v = k.value;
   if (codec == kCodecGorilla) {
     s.a = k.a;
@@ -174,13 +178,15 @@ __host__ __device__ __forceinline__ void xor_state_set(XorState &s, uint32_t cod
 }
 
 // State at a checkpoint, reading the chunk's stream in place.
-__host__ __device__ __forceinline__ void xor_state_at(XorState &s, uint32_t codec, const uint8_t *data, const XorCkpt &k) {
+template <class S>
+__host__ __device__ __forceinline__ void xor_state_at(S &s, uint32_t codec, const uint8_t *data, const XorCkpt &k) {
   s.r.init(data, k.bitpos);
   xor_state_set(s, codec, k);
 }
 
 // State before record 0 of a chunk.
-__host__ __device__ __forceinline__ void xor_state_start(XorState &s, uint32_t codec, const uint8_t *data) {
+template <class S>
+__host__ __device__ __forceinline__ void xor_state_start(S &s, uint32_t codec, const uint8_t *data) {
   s.r.init(data, 0);
   s.v = 0;
   s.a = codec == kCodecGorilla ? 0u : 0xffffffffu;
@@ -198,10 +204,10 @@ struct DirectSink {  // one lane per chunk: records stored as they come
 // decoder/chunk_reader/gorilla.rs:77-157.  Decodes records [i0, i1) from
 // state s (v = record i0-1, L, T); when i1 == n also checks that the next
 // record is the NaN terminator.  ck (index pass): checkpoints every kSeg.
-template <bool kWrite, class Sink>
-__host__ __device__ __forceinline__ uint32_t gorilla_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+template <bool kWrite, class Sink, class S>
+__host__ __device__ __forceinline__ uint32_t gorilla_run(S &s, uint32_t size_bits, uint32_t n, uint32_t i0,
                                                 uint32_t i1, const Sink &out, XorCkpt *ck) {
-  BitWindow &r = s.r;
+  auto &r = s.r;
   uint64_t v = s.v;
   uint32_t L = s.a, T = s.b;
   const uint32_t last = i1 == n ? n + 1 : i1;
@@ -255,10 +261,10 @@ __host__ __device__ __forceinline__ uint32_t gorilla_run(XorState &s, uint32_t s
 // -------------------------------------------------------------------- Chimp
 // decoder/chunk_reader/chimp.rs:59-125.  State: v, stored lead (0xffffffff
 // before the first 11/01 record).
-template <bool kWrite, class Sink>
-__host__ __device__ __forceinline__ uint32_t chimp_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+template <bool kWrite, class Sink, class S>
+__host__ __device__ __forceinline__ uint32_t chimp_run(S &s, uint32_t size_bits, uint32_t n, uint32_t i0,
                                               uint32_t i1, const Sink &out, XorCkpt *ck) {
-  BitWindow &r = s.r;
+  auto &r = s.r;
   uint64_t v = s.v;
   uint32_t lead = s.a;
   const uint32_t last = i1 == n ? n + 1 : i1;
@@ -316,10 +322,10 @@ __host__ __device__ __forceinline__ uint32_t chimp_run(XorState &s, uint32_t siz
 // ElfDecoderWrapper + ElfXorDecoder, decoder/chunk_reader/elf.rs:46-258.
 // State: stored v' bits, L, T (0xffffffff before the first 10/11 record),
 // last beta* (INT32_MAX before the first 11 prefix).
-template <bool kWrite, class Sink>
-__host__ __device__ __forceinline__ uint32_t elf_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+template <bool kWrite, class Sink, class S>
+__host__ __device__ __forceinline__ uint32_t elf_run(S &s, uint32_t size_bits, uint32_t n, uint32_t i0,
                                             uint32_t i1, const Sink &out, XorCkpt *ck) {
-  BitWindow &r = s.r;
+  auto &r = s.r;
   uint64_t stored = s.v;
   uint32_t L = s.a, T = s.b;
   int32_t beta = s.beta;
@@ -411,10 +417,10 @@ __host__ __device__ __forceinline__ uint32_t elf_run(XorState &s, uint32_t size_
 // chunk ends at the first Chimp value equal to NaN.  State: v = Chimp's
 // stored value (the erased bits v'), a = stored lead (0xffffffff unset),
 // beta = last beta* (INT32_MAX unset).
-template <bool kWrite, class Sink>
-__host__ __device__ __forceinline__ uint32_t elfchimp_run(XorState &s, uint32_t size_bits, uint32_t n, uint32_t i0,
+template <bool kWrite, class Sink, class S>
+__host__ __device__ __forceinline__ uint32_t elfchimp_run(S &s, uint32_t size_bits, uint32_t n, uint32_t i0,
                                                  uint32_t i1, const Sink &out, XorCkpt *ck) {
-  BitWindow &r = s.r;
+  auto &r = s.r;
   uint64_t v = s.v;
   uint32_t lead = s.a;
   int32_t beta = s.beta;
@@ -468,8 +474,8 @@ __host__ __device__ __forceinline__ uint32_t elfchimp_run(XorState &s, uint32_t 
 }
 
 // Dispatch on the codec (wave-uniform).
-template <bool kWrite, class Sink>
-__host__ __device__ __forceinline__ uint32_t xor_run(uint32_t codec, XorState &s, uint32_t size_bits, uint32_t n,
+template <bool kWrite, class Sink, class S>
+__host__ __device__ __forceinline__ uint32_t xor_run(uint32_t codec, S &s, uint32_t size_bits, uint32_t n,
                                             uint32_t i0, uint32_t i1, const Sink &out, XorCkpt *ck) {
   switch (codec) {
     case kCodecGorilla: return gorilla_run<kWrite>(s, size_bits, n, i0, i1, out, ck);
@@ -937,7 +943,8 @@ __device__ uint32_t chimp128_run(const uint8_t *data, uint32_t size_bits, uint32
 // lead; 01 = 16-bit header (index, lead code, sig), sig bits.  Sets the
 // flag, ring index, XOR payload (aligned) and bits used; false on a header
 // the reference would reject.
-__host__ __device__ __forceinline__ bool chimp128_rec(const BitWindow &r, uint32_t &lead, uint32_t &f, uint32_t &idx,
+template <class W>
+__host__ __device__ __forceinline__ bool chimp128_rec(const W &r, uint32_t &lead, uint32_t &f, uint32_t &idx,
                                              uint64_t &x, uint32_t &used) {
   f = (uint32_t)(r.hi >> 62);
   const uint32_t h = (uint32_t)(r.hi >> 46) & 0xffff;  // bits 2..17
@@ -970,7 +977,7 @@ __host__ __device__ __forceinline__ bool chimp128_rec(const BitWindow &r, uint32
 }
 
 __host__ __device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_bits, uint32_t n, XorCkpt *ck) {
-  BitWindow r;
+  SerialWindow r;
   r.init(data, 0);
   uint32_t lead = 0xffffffffu;
   for (uint32_t i = 0; i <= n; ++i) {

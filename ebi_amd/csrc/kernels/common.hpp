@@ -222,12 +222,15 @@ struct PrefetchReader {
 // current and the block after it -- requested one block earlier -- is
 // byte-swapped in while the following one is requested.  So a record of up
 // to 128 bits decodes from registers with shifts and selects, and a lane
-// waits on memory only for a load issued a whole block earlier.  Reads at
-// most 48 bytes past the current position.
-struct BitWindow {
+// waits on memory only for a load issued kDepth blocks earlier.  Reads at
+// most 16 (kDepth + 2) bytes past the current position.  The lockstep
+// segment kernels use kDepth 1 (many waves hide the latency); the lane-per-
+// chunk passes, one long chain per lane with few waves, a deeper queue.
+template <int kDepth>
+struct BitWindowT {
   const EBI_GLOBAL uint4 *p;  // next block to request
   uint64_t c0, c1, n0, n1; // current and next block
-  uint4 nxt;               // requested block (after the next)
+  uint4 q[kDepth];         // requested blocks (after the next), oldest first
   uint32_t b;              // bit offset of pos in the current block, [0, 128)
   uint32_t stride;         // blocks from one stream block to the next
   uint64_t hi, lo;
@@ -244,9 +247,11 @@ struct BitWindow {
   __host__ __device__ __forceinline__ void advance() {
     c0 = n0;
     c1 = n1;
-    n0 = be64(nxt.x, nxt.y);
-    n1 = be64(nxt.z, nxt.w);
-    nxt = *p;
+    n0 = be64(q[0].x, q[0].y);
+    n1 = be64(q[0].z, q[0].w);
+#pragma unroll
+    for (int k = 0; k < kDepth - 1; ++k) q[k] = q[k + 1];
+    q[kDepth - 1] = *p;
     p += stride;
     b -= 128;
   }
@@ -268,8 +273,9 @@ struct BitWindow {
     stride = step;
     p = as_global(blk);
     const uint4 c = p[0], n = p[step];
-    nxt = p[2 * step];
-    p += 3 * step;
+#pragma unroll
+    for (int k = 0; k < kDepth; ++k) q[k] = p[(2 + k) * step];
+    p += (2 + kDepth) * step;
     c0 = be64(c.x, c.y);
     c1 = be64(c.z, c.w);
     n0 = be64(n.x, n.y);
@@ -288,6 +294,9 @@ struct BitWindow {
     pos += k;
   }
 };
+using BitWindow = BitWindowT<1>;
+constexpr int kSerialDepth = 8;  // blocks in flight per lane in the lane-per-chunk passes
+using SerialWindow = BitWindowT<kSerialDepth>;
 
 #ifndef EBI_XOR_DEPTH
 #define EBI_XOR_DEPTH 2

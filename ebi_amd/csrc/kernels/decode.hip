@@ -33,7 +33,7 @@ __device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_
 __device__ __forceinline__ void dead_window(BitWindow &r) {
   r.p = nullptr;
   r.c0 = r.c1 = r.n0 = r.n1 = 0;
-  r.nxt = make_uint4(0, 0, 0, 0);
+  r.q[0] = make_uint4(0, 0, 0, 0);
   r.b = 0;
   r.stride = 0;
   r.hi = r.lo = 0;
@@ -182,7 +182,7 @@ __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &
     if (i < total) *slot = as_f64(v);
     wave_sync();  // this step's values are in the stage before the next step reads them
     if (i < c.n) {
-      if (v == kNan) e |= kErrCount;
+      if (v == kNan && !Out::kNanEnds) e |= kErrCount;
     } else if (i == c.n && v != kNan) {
       e |= kErrCount;
     }
@@ -194,6 +194,7 @@ __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &
 // Decode output of c128_job: record i < n to cv.out (a step's 64 records are
 // one 512-byte store).
 struct C128Out {
+  static constexpr bool kNanEnds = false;  // chunks end at their first NaN since load
   double *dst;  // record 0 of the chunk
   uint32_t n;
   __device__ __forceinline__ void put(uint32_t i, uint64_t v, uint32_t) {
@@ -364,6 +365,7 @@ struct QueryOut {
 // double-double partial, the butterfly sum of the lanes' sums; min / max
 // keys per lane.
 struct C128Query {
+  static constexpr bool kNanEnds = false;
   PredF64 pr;
   const uint64_t *mask;
   uint64_t *out_bits;
@@ -573,7 +575,7 @@ __global__ void __launch_bounds__(64) k_serial(const ColView *cols, const uint32
   if (c.codec == kCodecChimp128) {
     e = kIndex ? chimp128_index(data, bits, c.n, ck) : kErrCorrupt;
   } else {
-    XorState s;
+    SerialState s;
     xor_state_start(s, c.codec, data);
     e = xor_run<kWrite>(c.codec, s, bits, c.n, 0, c.n, out, ck);
     if (e == kNanEnd) {
@@ -635,6 +637,58 @@ __global__ void __launch_bounds__(256) k_c128_ring(const ColView *cols, uint32_t
   for (uint32_t u = lane_id(); u < 128; u += kWave) {
     const int64_t r = j0 - 128 + u;
     ring[128ull * jb.col + u] = ok && r >= 0 ? as_u64(cv.out[c.rec_off + r]) : 0;
+  }
+}
+
+// Output of c128_job for the load-time ring build: the values of the job's
+// last 128 records become the ring checkpoint of the chunk's next job, and
+// the first NaN before the chunk's footer count is where the reference ends
+// the chunk (chimp_n.rs:219-266), so it is kept instead of flagged.
+struct C128Ring {
+  static constexpr bool kNanEnds = true;
+  uint64_t *next;  // ring of the chunk's next job, null for its last job
+  uint32_t tail0;  // first record held by that ring
+  uint32_t n;
+  uint32_t nan;    // smallest record < n whose value is NaN (this lane's)
+  __device__ __forceinline__ void put(uint32_t i, uint64_t v, uint32_t) {
+    if (next && i - tail0 < 128u) next[i - tail0] = v;
+    if (i < n && v == kNan) nan = min(nan, i);
+  }
+};
+
+// One step of the load-time Chimp128 ring build over the column cols[0], a
+// wavefront over job indexes: wave w decodes job `step` of chunk w (its ring
+// written by the previous step; job 0 reads none), writes the next job's
+// ring and lowers the chunk's end to its first NaN.  job_first: per chunk its
+// first segment job, nc + 1 entries.  Chunks the index pass rejected keep
+// zero rings and are never decoded.
+__global__ void __launch_bounds__(256) k_c128_ring_step(const ColView *cols, const uint32_t *job_first, uint32_t nc,
+                                                        uint32_t step, uint64_t *ring, uint32_t *err) {
+  __shared__ __align__(16) double lds[4][(kRun + 1) * kWave];
+  __shared__ uint8_t dists[4][kJob128];
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t ci = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
+  if (ci >= nc) return;
+  const uint32_t j = job_first[ci] + step;
+  if (j >= job_first[ci + 1]) return;  // wave-uniform
+  const ColView cv = cols[0];
+  if (cv.chunk_err && cv.chunk_err[ci]) return;
+  const Job jb = cv.seg_jobs[j];
+  const DevChunk c = cv.chunks[ci];
+  C128Ring o{j + 1 < job_first[ci + 1] ? ring + 128ull * (jb.col + 1) : nullptr,
+             jb.tile * kSeg128 + kJob128 - 128, c.n, ~0u};
+  uint32_t e = c128_job(cv, c, jb.tile, jb.col, lds[wave], dists[wave], o);
+  uint32_t nan = o.nan;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) nan = min(nan, (uint32_t)__shfl_xor((int)nan, d, kWave));
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) e |= (uint32_t)__shfl_xor((int)e, d, kWave);
+  if (lane_id() == 0) {
+    if (nan != ~0u && cv.chunk_end) atomicMin(cv.chunk_end + ci, nan);
+    if (e) {
+      atomicOr(err, e);
+      if (cv.chunk_err) atomicOr(cv.chunk_err + ci, e);
+    }
   }
 }
 
@@ -796,6 +850,10 @@ void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
     hipLaunchKernelGGL((k_serial<false, true>), dim3(n_jobs), dim3(64), 0, s, cols, prefix, n_cols, single, err);
 }
 
+void c128_ring_step(hipStream_t s, const ColView *cols, const uint32_t *job_first, uint32_t nc, uint32_t step,
+                    uint64_t *ring, uint32_t *err) {
+  if (nc) hipLaunchKernelGGL(k_c128_ring_step, dim3((nc + 3) / 4), dim3(256), 0, s, cols, job_first, nc, step, ring, err);
+}
 void c128_ring(hipStream_t s, const ColView *cols, uint32_t n_jobs, uint64_t *ring) {
   if (n_jobs) hipLaunchKernelGGL(k_c128_ring, dim3((n_jobs + 3) / 4), dim3(256), 0, s, cols, n_jobs, ring);
 }

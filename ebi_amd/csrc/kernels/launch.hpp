@@ -32,6 +32,10 @@ void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t
 // Load-time ring checkpoints of the Chimp128 column cols[0] (decoded into
 // cols[0].out): per segment job the values of the 128 records before it.
 void c128_ring(hipStream_t s, const ColView *cols, uint32_t n_jobs, uint64_t *ring);
+// One step of the wavefront ring build: job `step` of every chunk of the
+// Chimp128 column cols[0] (job_first: nc + 1 prefix of its segment jobs).
+void c128_ring_step(hipStream_t s, const ColView *cols, const uint32_t *job_first, uint32_t nc, uint32_t step,
+                    uint64_t *ring, uint32_t *err);
 void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,
               const Job *single, uint32_t *err, bool write);
 // Load-time lane-interleaved segment streams of the XOR column cols[0]:

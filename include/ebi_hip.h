@@ -156,6 +156,14 @@ int ebi_hip_column_info(const ebi_hip_col *col, ebi_file_info *info);
 int ebi_hip_column_set_precision(ebi_hip_col *col, int32_t precision);
 /* Bytes of chunk data resident for the column (excluding padding). */
 uint64_t ebi_hip_column_compressed_bytes(const ebi_hip_col *col);
+/* Device bytes the resident column holds: chunk bodies, descriptors, job
+ * lists and the load-time index (XOR checkpoints, the lane-interleaved
+ * stream copy, Chimp128 ring checkpoints, tile prefixes).  No counterpart in
+ * the reference (its Decoder reads the file in host memory); for capacity
+ * planning.  ebi_hip_last_phase_ms after ebi_hip_column_load: phase 0 =
+ * parse, repack and upload, 1 = the index pass, 2 = the interleaved copy /
+ * ring checkpoints (host clock, each phase ends in a stream sync). */
+uint64_t ebi_hip_column_device_bytes(const ebi_hip_col *col);
 
 /* Query entry points.  They mirror Decoder::{materialize, filter,
  * filter_materialize, sum, min, max} (api/decoder.rs:202-439):

@@ -271,7 +271,7 @@ static void nan_cases(int codec) {
       }
       // index pass over the footer count: kNanEnd at the first NaN
       std::vector<XorCkpt> ck((cn + kSeg - 1) / kSeg + 1);
-      XorState s{};
+      SerialState s{};  // the device index pass's form (deep prefetch window)
       xor_state_start(s, codec, data);
       const DirectSink sink{out.data()};
       const uint32_t e = xor_run<false>(codec, s, bits, cn, 0, cn, sink, ck.data());
@@ -375,13 +375,19 @@ int main() {
           std::vector<XorCkpt> ck(nseg + 1);
           XorState s;
           if (codec != kCodecChimp128) {
-            xor_state_start(s, codec, data);
-            uint32_t e = xor_run<true>(codec, s, bits, cn, 0, cn, sink, nullptr);
+            // whole chunk and index pass in the lane-per-chunk form (deep
+            // prefetch window), both windows over the same padded body
+            SerialState ss;
+            xor_state_start(ss, codec, data);
+            uint32_t e = xor_run<true>(codec, ss, bits, cn, 0, cn, sink, nullptr);
             expect(e == kErrNone, "whole-chunk decode error", codec, g, chunk, k, 0);
             for (uint32_t i = 0; i < cn; ++i)
               expect(as_u64(out[i]) == as_u64(want[i]), "whole-chunk value", codec, g, chunk, k, i);
             xor_state_start(s, codec, data);
-            e = xor_run<false>(codec, s, bits, cn, 0, cn, sink, ck.data());
+            e = xor_run<true>(codec, s, bits, cn, 0, cn, sink, nullptr);
+            expect(e == kErrNone, "whole-chunk decode error (depth 1)", codec, g, chunk, k, 0);
+            xor_state_start(ss, codec, data);
+            e = xor_run<false>(codec, ss, bits, cn, 0, cn, sink, ck.data());
             expect(e == kErrNone, "index pass error", codec, g, chunk, k, 0);
             std::fill(out.begin(), out.end(), 0.0);
             for (uint32_t seg = 0; seg < nseg; ++seg) {
