@@ -55,8 +55,6 @@ def main():
     ap.add_argument("--codecs", default="all")
     ap.add_argument("--gens", default="all")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
-    ap.add_argument("--comparator-chunks", type=int, default=128,
-                    help="chunks of the Gorilla/Chimp column used for the per-chunk-launch comparator")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-query", action="store_true", help="skip the in-situ query latency leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson stream) leg")
@@ -92,9 +90,10 @@ def main():
     cols = []  # (gen, codec, decoder, compressed_bytes)
     cpu = {"seconds": 0.0, "bytes": 0, "values": 0, "chunks": 0,
            "par_seconds": 0.0, "par_bytes": 0, "par_chunks": 0, "par_threads": cpu_threads(), "core": None}
-    comparator_files = {}
     sample_q = {}  # generator -> every 64th value, for predicate quantiles
     cold = {}  # codec -> the random walk column's cold load and first decode
+    checks = []  # (column, first record, oracle's records): sampled chunks of every column
+    rng_check = np.random.default_rng(5)
     t_setup = time.time()
     for g in gens:
         fn, dec = G.GENERATORS[g]
@@ -110,10 +109,8 @@ def main():
             cols.append((g, codec, d, d.compressed_bytes))
             if g == "walk2":
                 cold[codec] = cold_entry(E, ctx, d, n, load_ms, phases_ms)
+            checks += sampled_chunks(data, len(cols) - 1, n, args.chunk, rng_check)
             log(f"[setup] {g:6s} {codec:14s} {len(data) / n:6.3f} B/value  loaded")
-            if g == "walk2" and codec in ("gorilla", "chimp"):
-                comparator_files[codec] = E.encode(vals[: args.chunk * args.comparator_chunks], cfg,
-                                                   E.ChunkOption.record_count(args.chunk), threads=threads)
             if not args.no_cpu:
                 cpu_sample(data, n, args.chunk, cpu, args.cpu_seconds / (len(gens) * len(codecs)))
             del data
@@ -152,6 +149,8 @@ def main():
         phase_ms[2].append(p2)
         log(f"[timed] step {s + 1}/{args.steps} done")
     t1 = time.perf_counter()
+    output_check = check_outputs(outs, checks)
+    log(f"[check] {output_check}")
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = total_bytes / (ms_per_step / 1000.0) / 1e9
     graph_ms = []
@@ -186,7 +185,7 @@ def main():
     per_codec, per_kernel, comparator = {}, {}, {}
     if not args.no_breakdown:
         per_codec, per_kernel = breakdown(E, ctx, cols, decs, outs, n, kernel_of)
-        comparator = run_comparator(E, ctx, comparator_files)
+        comparator = run_comparator(E, ctx, cols, n)
     for o in outs:
         o.free()
     outs = []
@@ -245,6 +244,7 @@ def main():
                            "ms_per_step": round(v["ms"], 3), "launches": v["launches"]}
                        for k, v in per_kernel.items()},
         "per_chunk_launch_comparator": comparator,
+        "output_check": output_check,
         "query": query,
         "cold": cold_obj,
         "cpu_plain_case": c1,
@@ -278,6 +278,30 @@ def main():
 
 def pct(xs, q):
     return round(float(np.percentile(np.asarray(xs), q)), 4) if len(xs) else None
+
+
+def sampled_chunks(data, col, n, chunk, rng, per_column=2):
+    """Two random chunks of a column decoded by the oracle (the checker)."""
+    from oracle.oracle import OracleDecoder
+
+    od = OracleDecoder(data)
+    nch = (n + chunk - 1) // chunk
+    out = []
+    for k in rng.choice(nch, min(per_column, nch), replace=False):
+        out.append((col, int(k) * chunk, np.asarray(od.materialize(chunk_id=int(k)))))
+    return out
+
+
+def check_outputs(outs, checks):
+    """The timed step's decoded columns against the oracle on the sampled
+    chunks: f64 bit patterns compared."""
+    mism, recs = 0, 0
+    for ci, r0, want in checks:
+        got = outs[ci].to_host(count=len(want), offset=r0)
+        recs += len(want)
+        mism += int(np.count_nonzero(got.view(np.uint64) != want.view(np.uint64)))
+    return {"chunks": len(checks), "records": recs, "mismatched_records": mism, "bit_exact": mism == 0,
+            "checker": "oracle (C restatement of the reference decoders), 2 random chunks per column"}
 
 
 def cold_entry(E, ctx, d, n, load_ms, phases_ms):
@@ -502,10 +526,13 @@ def serve_leg(E, G, ctx, args, threads, n_cols=64):
 def batched_queries(E, ctx, cols):
     """The batched workload over the same 64 resident columns: (1) every column
     decoded by one tuned prepared batch replayed as a hipGraph; (2) per
-    selectivity s (0.1, 1, 10, 50 %), a greater-than filter + sum at the
-    (1 - s) quantile and a max on every column, all 64 submitted together to
-    the server (in-situ on BUFF / DeltaSprintz codes).  GB/s counts compressed
-    bytes (+ 8 per decoded record for the decode)."""
+    selectivity s (0.1, 1, 10, 50 %), on every column: a greater-than filter
+    + sum at the (1 - s) quantile, a range filter [q(a), q(a + s)] (the
+    experimenter's two-sided predicate), an eq filter on a sampled value,
+    max and min, and for s <= 1 % the range filter + materialize (results to
+    the host), all submitted together to the server (in-situ on BUFF /
+    DeltaSprintz codes, per run on RLE).  GB/s counts the compressed bytes
+    each query scans (+ 8 per decoded record for the decode)."""
     decs = [d for d, _, _ in cols]
     nrec = sum(nv for _, _, nv in cols)
     comp = sum(d.compressed_bytes for d in decs)
@@ -525,21 +552,31 @@ def batched_queries(E, ctx, cols):
                       "gbs": round((comp + 8 * nrec) / (float(np.median(ms)) / 1e3) / 1e9, 1),
                       "launch_shape": cfg}}
     with E.Server(ctx, device_budget=16 << 30, max_batch=64, n_streams=4) as srv:
+        rng = np.random.default_rng(11)
         for sel in (0.001, 0.01, 0.1, 0.5):
-            qs = []
+            qs, scanned = [], 0
             for d, sq, nv in cols:
                 thr = float(sq[int((1.0 - sel) * (len(sq) - 1))])
-                p = E.Predicate.range(E.RangeValue.exclusive(thr), E.RangeValue.none())
-                qs += [E.Query(E.OP_FILTER_SUM, d, predicate=p), E.Query(E.OP_MAX, d)]
+                a = float(rng.uniform(0, 1 - sel))
+                lo, hi = float(sq[int(a * (len(sq) - 1))]), float(sq[int((a + sel) * (len(sq) - 1))])
+                gt = E.Predicate.range(E.RangeValue.exclusive(thr), E.RangeValue.none())
+                rg = E.Predicate.range(E.RangeValue.inclusive(lo), E.RangeValue.inclusive(hi))
+                eq = E.Predicate.eq(float(sq[int(rng.integers(len(sq)))]))
+                qs += [E.Query(E.OP_FILTER_SUM, d, predicate=gt), E.Query(E.OP_FILTER_SUM, d, predicate=rg),
+                       E.Query(E.OP_FILTER, d, predicate=eq), E.Query(E.OP_MAX, d), E.Query(E.OP_MIN, d)]
+                if sel <= 0.01:
+                    qs.append(E.Query(E.OP_FILTER_MATERIALIZE, d, predicate=rg))
+                scanned += (6 if sel <= 0.01 else 5) * d.compressed_bytes
             for x in srv.submit(qs[:2]):  # warm
                 srv.wait(x)
             t = time.perf_counter()
             ts = srv.submit(qs)
             lat = [srv.wait(x)[1] for x in ts]
             wall = time.perf_counter() - t
-            res[f"filter_sum+max_sel{sel * 100:g}%"] = {
-                "queries": len(qs), "wall_ms": round(wall * 1e3, 3), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-                "gbs": round(2 * comp / wall / 1e9, 1)}
+            ops = "filter_sum(gt,range)+filter(eq)+max+min" + ("+filter_materialize(range)" if sel <= 0.01 else "")
+            res[f"mix_sel{sel * 100:g}%"] = {
+                "ops": ops, "queries": len(qs), "wall_ms": round(wall * 1e3, 3), "p50_ms": pct(lat, 50),
+                "p99_ms": pct(lat, 99), "gbs": round(scanned / wall / 1e9, 1)}
     return res
 
 
@@ -631,14 +668,16 @@ def breakdown(E, ctx, cols, decs, outs, n, kernel_of):
     return per_codec, per_kernel
 
 
-def run_comparator(E, ctx, comparator_files):
-    """Batched decode vs the plain form (one launch + host sync per chunk) on
-    the Gorilla / Chimp random-walk columns."""
-    comparator = {}
-    for codec, data in comparator_files.items():
-        d = E.Decoder(ctx, data)
-        m = d.number_of_records
-        buf = ctx.alloc(8 * m)
+def run_comparator(E, ctx, cols, n):
+    """Batched decode vs the plain form (the same decode kernel launched once
+    per chunk over that chunk's jobs, a host sync after each) on the full
+    Gorilla and Chimp columns of every generator (C2: 4,096 chunks of 65,536
+    records per column at 2^28)."""
+    per, tot = {}, {"chunks": 0, "batched_ms": 0.0, "per_chunk_launch_ms": 0.0}
+    buf = ctx.alloc(8 * n)
+    for g, codec, d, _ in cols:
+        if codec not in ("gorilla", "chimp"):
+            continue
         d.decode_device(buf)
         t = time.perf_counter()
         for _ in range(3):
@@ -647,12 +686,17 @@ def run_comparator(E, ctx, comparator_files):
         t = time.perf_counter()
         d.decode_per_chunk(buf)
         plain = time.perf_counter() - t
-        comparator[codec] = {"chunks": d.number_of_chunks, "batched_ms": batched * 1e3,
-                             "per_chunk_launch_ms": plain * 1e3, "speedup": plain / batched}
-        buf.free()
-        d.close()
-
-    return comparator
+        per[f"{codec}_{g}"] = {"chunks": d.number_of_chunks, "batched_ms": round(batched * 1e3, 3),
+                               "per_chunk_launch_ms": round(plain * 1e3, 3), "speedup": round(plain / batched, 1)}
+        tot["chunks"] += d.number_of_chunks
+        tot["batched_ms"] += batched * 1e3
+        tot["per_chunk_launch_ms"] += plain * 1e3
+    buf.free()
+    if tot["chunks"]:
+        tot["speedup"] = round(tot["per_chunk_launch_ms"] / tot["batched_ms"], 1)
+        tot["batched_ms"] = round(tot["batched_ms"], 3)
+        tot["per_chunk_launch_ms"] = round(tot["per_chunk_launch_ms"], 3)
+    return {"columns": per, "total": tot}
 
 
 def cpu_threads():

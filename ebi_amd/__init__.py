@@ -534,11 +534,15 @@ class DeviceBuffer:
     def __init__(self, ctx: Context, ptr, nbytes: int):
         self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
 
-    def to_host(self, dtype=np.float64, count: Optional[int] = None) -> np.ndarray:
+    def to_host(self, dtype=np.float64, count: Optional[int] = None, offset: int = 0) -> np.ndarray:
+        """count elements from element `offset` copied to a new host array."""
         it = np.dtype(dtype).itemsize
-        n = self.nbytes // it if count is None else count
+        n = self.nbytes // it - offset if count is None else count
+        if offset < 0 or n < 0 or (offset + n) * it > self.nbytes:
+            raise EbiError(INVALID_ARGUMENT, "to_host: range outside the buffer")
         out = np.empty(n, dtype=dtype)
-        _check(lib().ebi_hip_copy_to_host(self.ctx._h, out.ctypes.data, self.ptr, n * it), self.ctx, "copy_to_host")
+        src = C.c_void_p((self.ptr.value or 0) + offset * it)
+        _check(lib().ebi_hip_copy_to_host(self.ctx._h, out.ctypes.data, src, n * it), self.ctx, "copy_to_host")
         return out
 
     def zero(self):
