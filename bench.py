@@ -240,6 +240,7 @@ def main():
         },
         "batch_kernels": batch_kernels,
         "per_codec_gbs": {c: round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 2) for c, v in per_codec.items()},
+        "per_codec_tuned_shape": {c: v["shape"] for c, v in per_codec.items()},
         "per_kernel": {k: {"gbs": round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 2),
                            "ms_per_step": round(v["ms"], 3), "launches": v["launches"]}
                        for k, v in per_kernel.items()},
@@ -657,10 +658,12 @@ def breakdown(E, ctx, cols, decs, outs, n, kernel_of):
         for _ in range(3):
             b.run(eager=True)
             ms.append(ctx.last_device_ms)
+        shape = b.config
         b.close()
         t = float(np.median(ms))
         byt = sum(cols[i][3] + 8 * n for i in idx)
-        per_codec[codec] = {"bytes": byt, "ms": t}
+        per_codec[codec] = {"bytes": byt, "ms": t,
+                            "shape": {k: shape[k] for k in ("blocks_per_cu", "mixed", "tiles_per_job", "cached")}}
         pk = per_kernel.setdefault(kernel_of[codec], {"bytes": 0, "ms": 0.0, "launches": 0})
         pk["bytes"] += byt
         pk["ms"] += t

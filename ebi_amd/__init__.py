@@ -127,8 +127,8 @@ class _ServeStats(C.Structure):
 
 
 class _BatchCfg(C.Structure):
-    _fields_ = [("blocks_per_cu", C.c_uint32), ("mixed", C.c_uint32), ("reserved1", C.c_uint32),
-                ("reserved2", C.c_uint32), ("tuned_ms", C.c_double)]
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("mixed", C.c_uint32), ("tiles_per_job", C.c_uint32),
+                ("cached", C.c_uint32), ("tuned_ms", C.c_double)]
 
 
 class _EncCfg(C.Structure):
@@ -199,6 +199,7 @@ _SIGS = {
     "ebi_hip_batch_create": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_int, C.POINTER(_P)]),
     "ebi_hip_batch_run": (C.c_int, [_P, _P, C.c_int]),
     "ebi_hip_batch_config_get": (C.c_int, [_P, C.POINTER(_BatchCfg)]),
+    "ebi_hip_batch_config_set": (C.c_int, [_P, _P, C.POINTER(_BatchCfg)]),
     "ebi_hip_batch_destroy": (C.c_int, [_P, _P]),
     "ebi_hip_serve_create": (C.c_int, [_P, C.POINTER(_ServeOpts), C.POINTER(_P)]),
     "ebi_hip_serve_submit": (C.c_int, [_P, C.POINTER(_Query), C.c_size_t, C.POINTER(C.c_uint64)]),
@@ -1020,7 +1021,13 @@ class Batch:
     def config(self) -> dict:
         c = _BatchCfg()
         _check(lib().ebi_hip_batch_config_get(self._h, C.byref(c)), None, "batch_config")
-        return {"blocks_per_cu": c.blocks_per_cu, "mixed": bool(c.mixed), "tuned_ms": c.tuned_ms}
+        return {"blocks_per_cu": c.blocks_per_cu, "mixed": bool(c.mixed), "tiles_per_job": c.tiles_per_job,
+                "cached": bool(c.cached), "tuned_ms": c.tuned_ms}
+
+    def set_config(self, blocks_per_cu: int, mixed: bool, tiles_per_job: int) -> None:
+        """Forces a launch shape (ebi_hip_batch_config_set); re-captures the graph."""
+        c = _BatchCfg(blocks_per_cu=blocks_per_cu, mixed=1 if mixed else 0, tiles_per_job=tiles_per_job)
+        _check(lib().ebi_hip_batch_config_set(self.ctx._h, self._h, C.byref(c)), self.ctx, "batch_config_set")
 
     def close(self):
         if getattr(self, "_h", None) and self.ctx._h:

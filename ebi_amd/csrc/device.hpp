@@ -33,7 +33,8 @@ constexpr uint32_t kSeg = 64;    // records per checkpoint of the XOR index
 constexpr uint32_t kSeg128 = 16; // records per checkpoint of the Chimp128 index
 constexpr uint32_t kJob128 = kSeg128 * 64;  // records of a Chimp128 segment job (64 segments)
 constexpr uint32_t kTilesPerJob = 4;  // consecutive tiles of one chunk per decode job
-constexpr uint32_t kSpanTiles = 16;   // consecutive tiles of one chunk per query span job
+constexpr uint32_t kSpanTiles = 16;   // most consecutive tiles of one chunk per query span job
+constexpr uint32_t kSpanMinWaves = 8192;  // span jobs a column's query aims for at least
 constexpr uint32_t kWaveLanes = 64;  // gfx950 wavefront size
 
 enum Codec : uint32_t {

@@ -72,6 +72,30 @@ bool is_xor_codec(int scheme) {
   return scheme == kGorilla || scheme == kChimp || scheme == kElf || scheme == kElfOnChimp;
 }
 
+std::vector<Job> tile_jobs_of(const Column &c, uint32_t per_job, std::vector<uint32_t> *first) {
+  std::vector<Job> jobs;
+  const size_t nc = c.chunks.size();
+  if (first) first->assign(nc + 1, 0);
+  for (size_t k = 0; k < nc; ++k) {
+    if (first) (*first)[k] = (uint32_t)jobs.size();
+    const HostChunk &h = c.chunks[k];
+    if (c.xor_codec || c.chimp128) continue;
+    if (c.meta.scheme == kRLE) {
+      if (h.err) continue;  // never decoded
+      for (uint32_t t = 0; t < h.n_tiles; ++t) {
+        const uint32_t steps = c.rle_steps[h.dev.tile0 + t];
+        for (uint32_t s0 = 0; s0 < steps; s0 += per_job)
+          jobs.push_back(Job{std::min(per_job, steps - s0), 0, (uint32_t)k, t | (s0 << 24)});
+      }
+    } else {
+      for (uint32_t t = 0; t < h.n_tiles; t += per_job)
+        jobs.push_back(Job{std::min(per_job, h.n_tiles - t), 0, (uint32_t)k, t});
+    }
+  }
+  if (first) (*first)[nc] = (uint32_t)jobs.size();
+  return jobs;
+}
+
 ColView view_of(const Column &c, double *out) {
   ColView v{};
   v.chunks = c.d_chunks;
@@ -465,33 +489,42 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   // consecutive tiles of one chunk (Job::kind = tile count), so a wave pays
   // the descriptor lookups once per up to 16 KiB of output; the load-time
   // index pass walks single tiles.
-  col.tile_job_first.assign(nc + 1, 0);
   std::vector<Job> index_jobs;
+  if (scheme != kRLE) col.h_tile_jobs = tile_jobs_of(col, kTilesPerJob, &col.tile_job_first);  // RLE: after the index
   for (size_t k = 0; k < nc; ++k) {
-    col.tile_job_first[k] = (uint32_t)col.h_tile_jobs.size();
-    if (!col.xor_codec && !col.chimp128) {
-      const uint32_t nt = col.chunks[k].n_tiles;
-      for (uint32_t t = 0; t < nt; ++t) index_jobs.push_back(Job{1, 0, (uint32_t)k, t});
-      for (uint32_t t = 0; t < nt; t += kTilesPerJob)
-        col.h_tile_jobs.push_back(Job{std::min(kTilesPerJob, nt - t), 0, (uint32_t)k, t});
-    }
+    if (!col.xor_codec && !col.chimp128)
+      for (uint32_t t = 0; t < col.chunks[k].n_tiles; ++t) index_jobs.push_back(Job{1, 0, (uint32_t)k, t});
     // Job::col: the job's index (Chimp128's ring checkpoints); the other
     // codecs' first ilv row, set once the ilv is built
     for (uint32_t s = 0; s < col.chunks[k].n_segs; s += kWaveLanes)
       col.h_seg_jobs.push_back(Job{kJobSeg, (uint32_t)col.h_seg_jobs.size(), (uint32_t)k, s});
   }
-  col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
+  if (scheme == kRLE) col.tile_job_first.assign(nc + 1, 0);
   std::vector<Job> rec_jobs, span_jobs, field_spans;
   col.rle_direct = scheme == kRLE;
+  // query span length: up to kSpanTiles tiles per wave, fewer when the
+  // column would otherwise give under kSpanMinWaves waves (small columns
+  // keep the GPU filled; large ones amortize the per-span set-up)
+  uint64_t all_tiles = 0, all_fields = 0;
   for (size_t k = 0; k < nc; ++k) {
-    const uint32_t nt = (col.chunks[k].dev.n + kTile - 1) / kTile;
+    all_tiles += (col.chunks[k].dev.n + kTile - 1) / kTile;
+    if (scheme == kRLE) all_fields += col.chunks[k].n_tiles;
+  }
+  auto span_of = [](uint64_t tiles) {
+    uint32_t s = kSpanTiles;
+    while (s > 1 && tiles / s < kSpanMinWaves) s /= 2;
+    return s;
+  };
+  col.span_tiles = span_of(all_tiles);
+  col.field_span_tiles = span_of(all_fields);
+  for (size_t k = 0; k < nc; ++k) {
+    const uint32_t nt = (col.chunks[k].dev.n + kTile - 1) / kTile, st = col.span_tiles;
     for (uint32_t t = 0; t < nt; ++t) rec_jobs.push_back(Job{kJobTile, 0, (uint32_t)k, t});
-    for (uint32_t t = 0; t < nt; t += kSpanTiles) span_jobs.push_back(Job{std::min(kSpanTiles, nt - t), 0, (uint32_t)k, t});
+    for (uint32_t t = 0; t < nt; t += st) span_jobs.push_back(Job{std::min(st, nt - t), 0, (uint32_t)k, t});
     if (scheme == kRLE) {  // field tiles; their f64 partials use the chunk's record-tile slots
-      const uint32_t ft = col.chunks[k].n_tiles;
+      const uint32_t ft = col.chunks[k].n_tiles, fs = col.field_span_tiles;
       if (col.chunks[k].dev.fields > col.chunks[k].dev.n) col.rle_direct = false;
-      for (uint32_t t = 0; t < ft; t += kSpanTiles)
-        field_spans.push_back(Job{std::min(kSpanTiles, ft - t), 0, (uint32_t)k, t});
+      for (uint32_t t = 0; t < ft; t += fs) field_spans.push_back(Job{std::min(fs, ft - t), 0, (uint32_t)k, t});
     }
   }
   col.n_rec_jobs = (uint32_t)rec_jobs.size();
@@ -559,20 +592,16 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
     std::vector<int64_t> starts(n_tiles);
     if (n_tiles && (e = hipMemcpy(starts.data(), col.d_tile_index, n_tiles * 8, hipMemcpyDeviceToHost)) != hipSuccess)
       return hip_fail(ctx, e, "RLE tile index");
-    col.h_tile_jobs.clear();
+    col.rle_steps.assign(n_tiles, 0);
     for (size_t k = 0; k < nc; ++k) {
-      col.tile_job_first[k] = (uint32_t)col.h_tile_jobs.size();
       const HostChunk &h = col.chunks[k];
-      if (h.err) continue;  // never decoded
       for (uint32_t t = 0; t < h.n_tiles; ++t) {
         const int64_t a = starts[h.dev.tile0 + t];
         const int64_t b = t + 1 < h.n_tiles ? starts[h.dev.tile0 + t + 1] : (int64_t)h.dev.n;
-        const uint32_t steps = b > a ? (uint32_t)((b - a + kTile - 1) / kTile) : 0;
-        for (uint32_t s0 = 0; s0 < steps; s0 += kTilesPerJob)
-          col.h_tile_jobs.push_back(Job{std::min(kTilesPerJob, steps - s0), 0, (uint32_t)k, t | (s0 << 24)});
+        col.rle_steps[h.dev.tile0 + t] = b > a ? (uint16_t)((b - a + kTile - 1) / kTile) : 0;
       }
     }
-    col.tile_job_first[nc] = (uint32_t)col.h_tile_jobs.size();
+    col.h_tile_jobs = tile_jobs_of(col, kTilesPerJob, &col.tile_job_first);
     col.n_tile_jobs = (uint32_t)col.h_tile_jobs.size();
     hipFree(col.d_tile_jobs);
     col.d_tile_jobs = nullptr;

@@ -1,11 +1,17 @@
 // Prepared batched decode: a fixed set of resident columns decoded into fixed
 // device outputs, replayed many times.  The job-space arguments are uploaded
-// once; an autotuner times candidate persistent grid sizes on this very
-// batch and keeps the fastest; the launch sequence is captured in a hipGraph
-// and replayed with one launch.  No counterpart in lemolatoon/ebi,
-// whose Decoder decodes chunk after chunk on the CPU (api/decoder.rs:202-241).
+// once per candidate shape; an autotuner times candidate launch shapes on
+// this very batch (persistent grid size, one mixed launch or two kernels,
+// tiles per tile-codec job) and keeps the fastest, remembering it per batch
+// signature (codec, chunk-size bucket, bit-width bucket of each column) in
+// the context so an equal batch reuses it; the launch sequence is captured
+// in a hipGraph and replayed with one launch.  No counterpart in
+// lemolatoon/ebi, whose Decoder decodes chunk after chunk on the CPU
+// (api/decoder.rs:202-241).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../../include/ebi_hip.h"
@@ -15,32 +21,45 @@
 
 using namespace ebi;
 
+// Tile-codec decode jobs of 1, 2, 4 or 8 tiles (RLE: output steps).
+constexpr uint32_t kShapeTiles[] = {1, 2, 4, 8};
+constexpr int kShapes = 4;
+
 struct ebi_hip_batch {
   std::vector<Column *> cols;
   std::vector<double *> outs;
-  DevBuf args;     // prefix[n + 1] | prefix_seg[n + 1] | pad | ColView[n]
+  // per tile-job shape: prefix[n + 1] | prefix_seg[n + 1] | pad | ColView[n] | pad | Job[jobs]
+  DevBuf args[kShapes];
+  uint32_t jobs[kShapes] = {};
   uint64_t pre_bytes = 0;
-  uint32_t n_cols = 0, jobs = 0, segs = 0;
+  uint32_t n_cols = 0, segs = 0;
   ebi_batch_config cfg{};
+  std::string key;  // tuning-cache signature
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 namespace {
 
-const ColView *views_of(const ebi_hip_batch *b) {
-  return reinterpret_cast<const ColView *>(static_cast<uint8_t *>(b->args.p) + b->pre_bytes);
+int shape_of(uint32_t tiles) {
+  for (int i = 0; i < kShapes; ++i)
+    if (kShapeTiles[i] == tiles) return i;
+  return -1;
 }
-const uint32_t *prefix_of(const ebi_hip_batch *b) { return static_cast<const uint32_t *>(b->args.p); }
+const ColView *views_of(const ebi_hip_batch *b, int sh) {
+  return reinterpret_cast<const ColView *>(static_cast<uint8_t *>(b->args[sh].p) + b->pre_bytes);
+}
+const uint32_t *prefix_of(const ebi_hip_batch *b, int sh) { return static_cast<const uint32_t *>(b->args[sh].p); }
 
 // Issues the batch on ctx->stream with configuration c.  timed: record the
 // phase events (eager runs only; a captured sequence carries no timing).
 void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool timed) {
   hipStream_t s = ctx->stream;
+  const int sh = shape_of(c.tiles_per_job);
   launch::zero_u32(s, ctx->d_flags, 2);
   if (timed) hipEventRecord(ctx->ev0, s);
-  launch::decode(s, views_of(b), b->n_cols, prefix_of(b), b->jobs, prefix_of(b) + (b->n_cols + 1), b->segs,
-                 ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr,
+  launch::decode(s, views_of(b, sh), b->n_cols, prefix_of(b, sh), b->jobs[sh], prefix_of(b, sh) + (b->n_cols + 1),
+                 b->segs, ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr,
                  c.mixed != 0);
   if (timed) hipEventRecord(ctx->ev_mid, s);
   if (timed) hipEventRecord(ctx->ev1, s);
@@ -93,8 +112,18 @@ void destroy(ebi_hip_batch *b) {
   for (Column *c : b->cols) c->users.fetch_sub(1);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
-  b->args.release();
+  for (DevBuf &a : b->args) a.release();
   delete b;
+}
+
+// Tuning-cache signature of one column: codec, log2 of its mean chunk
+// records, compressed bits per record (BUFF / DeltaSprintz: the width B / W
+// is what sets a tile's work; XOR codecs: the stream length) in steps of 4.
+std::string column_key(const Column &c) {
+  const uint64_t n = c.meta.number_of_records, nc = std::max<size_t>(c.chunks.size(), 1);
+  const int chunk_log2 = n ? (int)std::lround(std::log2((double)n / (double)nc)) : 0;
+  const int bits = n ? (int)std::lround(8.0 * (double)c.compressed_bytes / (double)n / 4.0) * 4 : 0;
+  return std::to_string(c.meta.scheme) + ":" + std::to_string(chunk_log2) + ":" + std::to_string(bits);
 }
 
 }  // namespace
@@ -113,62 +142,110 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   if (!b) return EBI_OOM;
   b->n_cols = (uint32_t)n_cols;
   b->pre_bytes = ((2 * (n_cols + 1) * 4 + 255) / 256) * 256;
-  std::vector<uint8_t> host(b->pre_bytes + n_cols * sizeof(ColView));
-  uint32_t *pre = reinterpret_cast<uint32_t *>(host.data()), *preseg = pre + (n_cols + 1);
-  ColView *views = reinterpret_cast<ColView *>(host.data() + b->pre_bytes);
-  uint64_t jobs = 0, segs = 0;
+  const uint64_t view_bytes = ((n_cols * sizeof(ColView) + 255) / 256) * 256;
+  uint64_t segs = 0;
+  std::vector<std::string> keys;
   for (size_t i = 0; i < n_cols; ++i) {
     Column &c = const_cast<ebi_hip_col *>(cols[i])->c;
     c.users.fetch_add(1);  // the captured graph holds the column's addresses
     b->cols.push_back(&c);
     b->outs.push_back(d_outs[i]);
-    pre[i] = (uint32_t)jobs;
-    preseg[i] = (uint32_t)segs;
-    views[i] = view_of(c, d_outs[i]);
-    jobs += c.n_tile_jobs;
     segs += c.n_seg_jobs;
+    keys.push_back(column_key(c));
   }
-  pre[n_cols] = (uint32_t)jobs;
-  preseg[n_cols] = (uint32_t)segs;
-  if (jobs >= (1ull << 32) || segs >= (1ull << 32)) {
+  std::sort(keys.begin(), keys.end());
+  for (const std::string &k : keys) b->key += k + ";";
+  if (segs >= (1ull << 32)) {
     destroy(b);
     return fail(ctx, EBI_UNSUPPORTED, "batch too large");
   }
-  b->jobs = (uint32_t)jobs;
   b->segs = (uint32_t)segs;
-  hipError_t e;
-  if (b->args.ensure(host.size()) ||
-      (e = hipMemcpy(b->args.p, host.data(), host.size(), hipMemcpyHostToDevice)) != hipSuccess) {
-    destroy(b);
-    return fail(ctx, EBI_OOM, "batch arguments");
+  // one argument block per tile-job shape: the columns' tile jobs rebuilt
+  // with that many tiles per job, views pointing at them
+  for (int sh = 0; sh < kShapes; ++sh) {
+    std::vector<std::vector<Job>> lists(n_cols);
+    uint64_t jobs = 0;
+    for (size_t i = 0; i < n_cols; ++i) {
+      lists[i] = tile_jobs_of(*b->cols[i], kShapeTiles[sh], nullptr);
+      jobs += lists[i].size();
+    }
+    if (jobs >= (1ull << 32)) {
+      destroy(b);
+      return fail(ctx, EBI_UNSUPPORTED, "batch too large");
+    }
+    std::vector<uint8_t> host(b->pre_bytes + view_bytes + std::max<uint64_t>(jobs, 1) * sizeof(Job));
+    if (b->args[sh].ensure(host.size())) {
+      destroy(b);
+      return fail(ctx, EBI_OOM, "batch arguments");
+    }
+    uint32_t *pre = reinterpret_cast<uint32_t *>(host.data()), *preseg = pre + (n_cols + 1);
+    ColView *views = reinterpret_cast<ColView *>(host.data() + b->pre_bytes);
+    Job *hj = reinterpret_cast<Job *>(host.data() + b->pre_bytes + view_bytes);
+    const Job *dj = reinterpret_cast<const Job *>(static_cast<uint8_t *>(b->args[sh].p) + b->pre_bytes + view_bytes);
+    uint64_t at = 0, sg = 0;
+    for (size_t i = 0; i < n_cols; ++i) {
+      pre[i] = (uint32_t)at;
+      preseg[i] = (uint32_t)sg;
+      views[i] = view_of(*b->cols[i], d_outs[i]);
+      views[i].tile_jobs = lists[i].empty() ? nullptr : dj + at;
+      views[i].n_tile_jobs = (uint32_t)lists[i].size();
+      std::copy(lists[i].begin(), lists[i].end(), hj + at);
+      at += lists[i].size();
+      sg += b->cols[i]->n_seg_jobs;
+    }
+    pre[n_cols] = (uint32_t)at;
+    preseg[n_cols] = (uint32_t)sg;
+    b->jobs[sh] = (uint32_t)at;
+    if (hipMemcpy(b->args[sh].p, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      destroy(b);
+      return fail(ctx, EBI_HIP, "batch arguments upload");
+    }
   }
   b->cfg.blocks_per_cu = ctx->blocks_per_cu;
-  b->cfg.mixed = ctx->batch_mixed && b->jobs && b->segs ? 1u : 0u;
+  b->cfg.tiles_per_job = kTilesPerJob;
+  b->cfg.mixed = ctx->batch_mixed && b->jobs[shape_of(kTilesPerJob)] && b->segs ? 1u : 0u;
   if (autotune) {
-    // persistent grid size of the decode kernels, and (with both job kinds)
-    // one mixed launch or the two kernels in turn
-    double best = time_config(ctx, b, b->cfg, 2);
-    if (best < 0) {
-      destroy(b);
-      return EBI_HIP;
+    auto hit = ctx->tune_cache.find(b->key);
+    if (hit != ctx->tune_cache.end()) {  // an equal batch was tuned on this context
+      b->cfg = hit->second;
+      b->cfg.cached = 1;
+    } else {
+      // coordinate search: grid size x launch form at the default tiles per
+      // job, then the tiles per job at the best of those
+      double best = time_config(ctx, b, b->cfg, 2);
+      if (best < 0) {
+        destroy(b);
+        return EBI_HIP;
+      }
+      auto consider = [&](ebi_batch_config c) {
+        const double t = time_config(ctx, b, c, 2);
+        if (t >= 0 && t < best) {
+          best = t;
+          b->cfg = c;
+        }
+      };
+      const ebi_batch_config first = b->cfg;
+      const bool both = b->jobs[shape_of(kTilesPerJob)] && b->segs;
+      for (uint32_t mixed = 0; mixed < (both ? 2u : 1u); ++mixed)
+        for (uint32_t bpc : {4u, 8u, 16u}) {
+          ebi_batch_config c = first;
+          c.blocks_per_cu = bpc;
+          c.mixed = mixed;
+          if (bpc != first.blocks_per_cu || mixed != first.mixed) consider(c);
+        }
+      if (b->jobs[shape_of(kTilesPerJob)]) {
+        const ebi_batch_config mid = b->cfg;
+        for (uint32_t tpj : kShapeTiles)
+          if (tpj != mid.tiles_per_job) {
+            ebi_batch_config c = mid;
+            c.tiles_per_job = tpj;
+            consider(c);
+          }
+      }
+      b->cfg.tuned_ms = best;
+      b->cfg.cached = 0;
+      ctx->tune_cache[b->key] = b->cfg;
     }
-    auto consider = [&](ebi_batch_config c) {
-      const double t = time_config(ctx, b, c, 2);
-      if (t >= 0 && t < best) {
-        best = t;
-        b->cfg = c;
-      }
-    };
-    const ebi_batch_config first = b->cfg;
-    const uint32_t n_mixed = b->jobs && b->segs ? 2u : 1u;
-    for (uint32_t mixed = 0; mixed < n_mixed; ++mixed)
-      for (uint32_t bpc : {4u, 8u, 16u}) {
-        ebi_batch_config c = first;
-        c.blocks_per_cu = bpc;
-        c.mixed = mixed;
-        if (bpc != first.blocks_per_cu || mixed != first.mixed) consider(c);
-      }
-    b->cfg.tuned_ms = best;
   }
   if ((rc = capture(ctx, b))) {
     destroy(b);
@@ -205,6 +282,18 @@ extern "C" int ebi_hip_batch_config_get(const ebi_hip_batch *b, ebi_batch_config
   if (!b || !cfg) return EBI_INVALID_ARGUMENT;
   *cfg = b->cfg;
   return EBI_OK;
+}
+
+extern "C" int ebi_hip_batch_config_set(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config *cfg) {
+  EBI_GUARD(ctx);
+  if (!b || !cfg || shape_of(cfg->tiles_per_job) < 0 || cfg->blocks_per_cu == 0 || cfg->blocks_per_cu > 64 ||
+      cfg->mixed > 1)
+    return EBI_INVALID_ARGUMENT;
+  hipStreamSynchronize(ctx->stream);
+  b->cfg = *cfg;
+  b->cfg.cached = 0;
+  b->cfg.tuned_ms = 0;
+  return capture(ctx, b);
 }
 
 extern "C" int ebi_hip_batch_destroy(ebi_hip_ctx *ctx, ebi_hip_batch *b) {

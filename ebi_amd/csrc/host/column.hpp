@@ -5,6 +5,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -62,12 +63,14 @@ struct Column {
   Job *d_span_jobs = nullptr;    // up to kSpanTiles record tiles of one chunk (query plans)
   uint32_t n_span_jobs = 0;
   Job *d_field_spans = nullptr;  // RLE: up to kSpanTiles field tiles of one chunk (per-run queries)
+  uint32_t span_tiles = kSpanTiles, field_span_tiles = kSpanTiles;  // chosen span lengths
   uint32_t n_field_spans = 0;
   bool rle_direct = false;       // RLE whose chunks have no more fields than records: queried per run
   uint64_t n_rtiles = 0;
   uint32_t n_seg_jobs = 0;
   std::vector<Job> h_tile_jobs, h_serial_jobs, h_seg_jobs;
   std::vector<uint32_t> tile_job_first;  // per chunk: first tile job index (comparator)
+  std::vector<uint16_t> rle_steps;       // RLE: output steps (kTile records) of each field tile
   uint64_t tile_index_entries = 0, ckpt_entries = 0;
   bool xor_codec = false;
   bool chimp128 = false;
@@ -116,6 +119,7 @@ struct ebi_hip_ctx {
   // launch shape of the decode kernels (defaults; a prepared batch tunes its own)
   uint32_t blocks_per_cu = 8;   // persistent k_decode grid = cu_count * blocks_per_cu
   uint32_t batch_mixed = 0;     // untuned prepared batches: 1 = one k_decode_mix launch
+  std::map<std::string, ebi_batch_config> tune_cache;  // prepared-batch shapes by batch signature
   int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks (Gorilla, Chimp, Elf)
   uint8_t *noalloc_dev = nullptr;  // this device's scratch of ebi_hip_query_noalloc
 };

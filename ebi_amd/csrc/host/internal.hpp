@@ -26,6 +26,11 @@ bool is_xor_codec(int scheme);
 hipError_t stream_wait(hipStream_t s);
 // The ColView of `c` with output base `out`.
 ColView view_of(const Column &c, double *out);
+// Decode tile jobs of a tile-codec column with up to per_job tiles (RLE:
+// output steps of one field tile) per job; first (nullable) gets each
+// chunk's first job, nc + 1 entries.  Chunks rejected at load get none
+// (RLE) or are skipped by the kernel.
+std::vector<Job> tile_jobs_of(const Column &c, uint32_t per_job, std::vector<uint32_t> *first);
 // Uploads the single-column ColView of `c` with output base `out` and waits.
 int upload_view(ebi_hip_ctx *ctx, Column &c, double *out);
 // Copies the device error word back (synchronising the stream) and maps it.

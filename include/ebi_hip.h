@@ -249,23 +249,32 @@ int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols,
 
 /* Prepared batch: the same decode for a fixed set of columns and outputs,
  * replayed many times.  The job-space arguments are uploaded once; with
- * autotune != 0 the persistent grid size and (when the batch holds both tile
- * and segment jobs) the mixed or two-kernel launch are timed on this batch
- * and the fastest kept (without: the context's defaults, the two kernels
- * unless EBI_HIP_BATCH_MIXED=1 was set when it was created); the launch sequence is
- * captured in a hipGraph.  Columns and outputs must outlive the batch.  No
+ * autotune != 0 the persistent grid size, (when the batch holds both tile
+ * and segment jobs) the mixed or two-kernel launch and the tiles per
+ * tile-codec job are timed on this batch and the fastest kept, and
+ * remembered per batch signature (codec, chunk-size and bit-width bucket of
+ * each column) on the context, so an equal batch reuses it without timing
+ * (without autotune: the context's defaults, 4 tiles per job, the two
+ * kernels unless EBI_HIP_BATCH_MIXED=1 was set when it was created); the
+ * launch sequence is captured in a hipGraph.  Columns and outputs must outlive the batch.  No
  * counterpart in the reference. */
 typedef struct ebi_hip_batch ebi_hip_batch;
 typedef struct ebi_batch_config {
     uint32_t blocks_per_cu; /* persistent decode grids = CUs x blocks_per_cu */
     uint32_t mixed;         /* 1: tile and segment jobs in one launch (k_decode_mix),
                                0: k_decode then k_decode_seg */
-    uint32_t reserved1;     /* 0 */
-    uint32_t reserved2;     /* 0 */
+    uint32_t tiles_per_job; /* 1, 2, 4 or 8: 512-record tiles (RLE: output steps) per
+                               tile-codec decode job */
+    uint32_t cached;        /* 1: taken from the context's tuning cache (an equal
+                               batch was tuned before), no timing */
     double tuned_ms;        /* device ms of the chosen shape (0 if not tuned) */
 } ebi_batch_config;
 int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *cols, size_t n_cols,
                          double *const *d_outs, int autotune, ebi_hip_batch **out);
+/* Forces a launch shape (and re-captures the graph): tiles_per_job 1/2/4/8,
+ * blocks_per_cu 1..64, mixed 0/1; EBI_INVALID_ARGUMENT otherwise.  No
+ * counterpart in the reference (tuning and shape sweeps). */
+int ebi_hip_batch_config_set(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config *cfg);
 /* Replays the batch and waits.  eager = 0: one hipGraph launch
  * (ebi_hip_last_device_ms only); eager != 0: the same kernels launched one by
  * one with per-kernel events (ebi_hip_last_phase_ms as for

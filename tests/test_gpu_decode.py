@@ -249,3 +249,70 @@ def test_damaged_chunk(ctx, codec, how):
     finally:
         buf.free()
         d.close()
+
+
+@pytest.mark.parametrize("codec", CODECS)
+def test_prepared_batch_shapes(ctx, codec):
+    """Every launch shape the batch tuner may pick decodes bit-exactly: tiles
+    per tile-codec job 1 / 2 / 4 / 8 (RLE: output steps per job), grids of 4
+    and 16 blocks per CU, one mixed launch and the two kernels -- on two
+    columns of the codec (random walk, step series: RLE runs up to 255) with
+    chunk sizes that leave ragged last tiles and a short last chunk."""
+    decs, outs, wants = [], [], []
+    for i, (gen, chunk) in enumerate((("walk2", 5000), ("step2", 70000))):
+        fn, dec = G.GENERATORS[gen]
+        vals = fn(150_001, seed=40 + i)
+        data = E.encode(vals, E.CompressorConfig.by_name(codec, scale=10 ** dec), E.ChunkOption.record_count(chunk))
+        decs.append(E.Decoder(ctx, data))
+        outs.append(ctx.alloc(8 * decs[-1].number_of_records))
+        wants.append(OracleDecoder(data).materialize())
+    b = E.Batch(ctx, decs, outs, autotune=False)
+    try:
+        for tpj, bpc, mixed in ((1, 4, False), (2, 16, True), (8, 4, True), (8, 16, False), (4, 8, False)):
+            b.set_config(bpc, mixed, tpj)
+            for o in outs:
+                o.zero()
+            b.run()
+            for d, o, w in zip(decs, outs, wants):
+                got = o.to_host(np.float64, d.number_of_records)
+                assert np.array_equal(got.view(np.uint64), w.view(np.uint64)), (codec, tpj, bpc, mixed)
+        with pytest.raises(E.EbiError):
+            b.set_config(8, False, 3)  # not a tile-job shape
+    finally:
+        b.close()
+        for o in outs:
+            o.free()
+        for d in decs:
+            d.close()
+
+
+def test_prepared_batch_tuning_cache(ctx):
+    """A second batch of the same columns reuses the first one's tuned shape
+    from the context's cache (no timing) and decodes the same records."""
+    rng = np.random.default_rng(9)
+    decs, outs = [], []
+    for codec in ("buff", "gorilla", "rle"):
+        vals = G.random_walk(200_000, seed=int(rng.integers(1000)))
+        data = E.encode(vals, E.CompressorConfig.by_name(codec, scale=100), E.ChunkOption.record_count(20000))
+        decs.append(E.Decoder(ctx, data))
+        outs.append(ctx.alloc(8 * 200_000))
+    b1 = E.Batch(ctx, decs, outs, autotune=True)
+    c1 = b1.config
+    b1.run()
+    first = [o.to_host(np.float64, 200_000) for o in outs]
+    b1.close()
+    b2 = E.Batch(ctx, decs, outs, autotune=True)
+    c2 = b2.config
+    for o in outs:
+        o.zero()
+    b2.run()
+    assert c2["cached"]
+    assert (c1["blocks_per_cu"], c1["mixed"], c1["tiles_per_job"]) == (c2["blocks_per_cu"], c2["mixed"],
+                                                                      c2["tiles_per_job"])
+    for o, f in zip(outs, first):
+        assert np.array_equal(o.to_host(np.float64, 200_000).view(np.uint64), f.view(np.uint64))
+    b2.close()
+    for o in outs:
+        o.free()
+    for d in decs:
+        d.close()
