@@ -418,25 +418,6 @@ class Predicate:
     def to_c(self) -> _Pred:
         return _Pred(self.kind, self.start.kind, self.end.kind, 0, self.value, self.start.value, self.end.value)
 
-    def eval(self, x: np.ndarray) -> np.ndarray:
-        """Predicate::eval on f64 values (decoder/query.rs:425-433)."""
-        x = np.asarray(x, dtype=np.float64)
-        if self.kind == 0:
-            return x == self.value
-        if self.kind == 1:
-            return x != self.value
-        s = np.ones(x.shape, bool)
-        if self.start.kind == 1:
-            s = x >= self.start.value
-        elif self.start.kind == 2:
-            s = x > self.start.value
-        e = np.ones(x.shape, bool)
-        if self.end.kind == 1:
-            e = x <= self.end.value
-        elif self.end.kind == 2:
-            e = x < self.end.value
-        return s & e
-
 
 # ----------------------------------------------------------------- bitmasks
 def ids_to_bitset(ids, n_records: int) -> np.ndarray:

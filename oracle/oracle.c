@@ -105,10 +105,19 @@ static int msb_bit(msb_t *r, int *bit) {            /* read_bit :291-299 */
     if (++r->bit_pos == 8) { r->bit_pos = 0; r->byte_pos++; }
     return 0;
 }
-static int msb_bits(msb_t *r, unsigned n, uint64_t *out) { /* read_bits :301-325 */
+static uint8_t msb_byte(msb_t *r) {                 /* read_byte_unchecked :143-159 */
+    uint8_t hi = r->buf[r->byte_pos++];
+    if (r->bit_pos == 0) return hi;
+    uint8_t lo = r->byte_pos < r->len ? (uint8_t)(r->buf[r->byte_pos] >> (8 - r->bit_pos)) : 0;
+    return (uint8_t)(hi << r->bit_pos) | lo;
+}
+/* read_bits :293-318: whole bytes first, then the remaining bits one by one. */
+static int msb_bits(msb_t *r, unsigned n, uint64_t *out) {
     if ((r->len - r->byte_pos) * 8 - r->bit_pos < n) return -1;
     uint64_t v = 0;
-    for (unsigned i = 0; i < n; ++i) {
+    unsigned rem = n;
+    for (; rem >= 8; rem -= 8) v = (v << 8) | msb_byte(r);
+    for (; rem; --rem) {
         int b = 0;
         msb_bit(r, &b);
         v = (v << 1) | (uint64_t)b;
@@ -472,13 +481,15 @@ static int buff_open(const uint8_t *p, size_t size, buff_t *b) {
     if (need > size) return ORC_CORRUPT;
     return ORC_OK;
 }
-/* BitPack::read_bits, LSB first (compression_common/buff.rs:233-268). */
+/* BitPack::read_bits, LSB first, a byte's worth of bits at a time
+ * (compression_common/buff.rs:206-245); fields are under 8 bits wide. */
 static uint32_t buff_tail(const buff_t *b, uint32_t i) {
     uint64_t bit = (uint64_t)i * b->r;
-    uint32_t v = 0;
-    for (unsigned k = 0; k < b->r; ++k, ++bit)
-        v |= (uint32_t)((b->tail[bit >> 3] >> (bit & 7)) & 1) << k;
-    return v;
+    const uint8_t *q = b->tail + (bit >> 3);
+    unsigned at = (unsigned)(bit & 7), left = 8 - at;
+    uint32_t v = (uint32_t)(q[0] >> at);
+    if (b->r > left) v |= (uint32_t)q[1] << left;
+    return v & ((1u << b->r) - 1);
 }
 static uint8_t flip(uint8_t x) { return x ^ 0x80; } /* buff.rs:2-5 */
 static uint8_t plane_byte(const buff_t *b, unsigned pl, uint32_t i) {
@@ -490,15 +501,23 @@ static uint64_t buff_delta(const buff_t *b, uint32_t i) {
     if (b->r) d = (d << b->r) | buff_tail(b, i);
     return d;
 }
-/* buff_simd256_decode, decoder/chunk_reader/buff/decode.rs:5-70. */
+/* buff_simd256_decode, decoder/chunk_reader/buff/decode.rs:5-70: plane by
+ * plane into a fixed-point vector, then the tail fields, then the floats. */
 static int dec_buff(const uint8_t *p, size_t size, dvec *o) {
     buff_t b;
     int rc = buff_open(p, size, &b);
     if (rc) return rc;
-    for (uint32_t i = 0; i < b.n; ++i) {
-        uint64_t d = b.B ? buff_delta(&b, i) : 0;
-        if (dpush(o, (double)(int64_t)((uint64_t)b.base + d) / b.scale)) return ORC_NOMEM;
+    uint64_t *fx = calloc(b.n ? b.n : 1, sizeof(uint64_t));
+    if (!fx) return ORC_NOMEM;
+    for (unsigned pl = 0; pl < b.n_planes; ++pl) {
+        const uint8_t *x = b.planes + (size_t)pl * b.n;
+        for (uint32_t i = 0; i < b.n; ++i) fx[i] = (fx[i] << 8) | flip(x[i]);
     }
+    if (b.r)
+        for (uint32_t i = 0; i < b.n; ++i) fx[i] = (fx[i] << b.r) | buff_tail(&b, i);
+    for (uint32_t i = 0; i < b.n; ++i)
+        if (dpush(o, (double)(int64_t)((uint64_t)b.base + fx[i]) / b.scale)) { free(fx); return ORC_NOMEM; }
+    free(fx);
     return ORC_OK;
 }
 
@@ -545,8 +564,56 @@ int64_t orc_buff_fixed(double v, int32_t frac) { /* compression_common/buff.rs:1
     return (int64_t)fixed;
 }
 
+/* The first plane of a filter without a bitmask, 32 records per step as
+ * comp_simd (filter_cmp.rs:484-543) and the equal filter's SIMD form do:
+ * signed byte compares of the stored (flipped) bytes with flip(pred) order
+ * the unflipped bytes as unsigned.  op 0: equal -> chk; 1: greater -> res,
+ * equal -> chk; 2: less -> res, equal -> chk.  Returns the size of chk. */
+#if defined(__x86_64__)
+#include <immintrin.h>
+__attribute__((target("avx2")))
+static uint32_t plane_first_avx2(const uint8_t *x, uint32_t n, uint8_t ps, int op,
+                                 uint8_t *res, uint32_t *chk) {
+    const __m256i pw = _mm256_set1_epi8((char)flip(ps));
+    uint32_t m = 0, i = 0;
+    for (; i + 32 <= n; i += 32) {
+        __m256i w = _mm256_loadu_si256((const __m256i *)(x + i));
+        uint32_t eq = (uint32_t)_mm256_movemask_epi8(_mm256_cmpeq_epi8(w, pw));
+        if (op) {
+            uint32_t q = (uint32_t)_mm256_movemask_epi8(op == 1 ? _mm256_cmpgt_epi8(w, pw)
+                                                                : _mm256_cmpgt_epi8(pw, w));
+            for (; q; q &= q - 1) res[i + (uint32_t)__builtin_ctz(q)] = 1;
+        }
+        for (; eq; eq &= eq - 1) chk[m++] = i + (uint32_t)__builtin_ctz(eq);
+    }
+    for (; i < n; ++i) {
+        uint8_t v = flip(x[i]);
+        if (v == ps) chk[m++] = i;
+        else if (op && ((op == 1) == (v > ps))) res[i] = 1;
+    }
+    return m;
+}
+#endif
+static uint32_t plane_first(const uint8_t *x, uint32_t n, uint8_t ps, int op, uint8_t *res, uint32_t *chk) {
+#if defined(__x86_64__)
+    static int avx2 = -1;
+    if (avx2 < 0) { __builtin_cpu_init(); avx2 = __builtin_cpu_supports("avx2") ? 1 : 0; }
+    if (avx2) return plane_first_avx2(x, n, ps, op, res, chk);
+#endif
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint8_t v = flip(x[i]);
+        if (v == ps) chk[m++] = i;
+        else if (op && ((op == 1) == (v > ps))) res[i] = 1;
+    }
+    return m;
+}
+
 /* buff_simd256_cmp_filter::<IS_GREATER, INCLUSIVE>, filter_cmp.rs:298-451.
- * mask (nullable) = the bitmask restricted to the chunk; res[i] out. */
+ * mask (nullable) = the bitmask restricted to the chunk; res[i] out.  The
+ * records still undecided after each plane are kept as an ascending index
+ * list (the reference's to_check bitmap), so later planes and the tail
+ * visit only those. */
 static int buff_cmp(const buff_t *b, double pred, int greater, int inclusive,
                     const uint8_t *mask, uint8_t *res) {
     uint32_t n = b->n;
@@ -568,42 +635,50 @@ static int buff_cmp(const buff_t *b, double pred, int greater, int inclusive,
         if (!greater) memset(res, 1, n);
         return ORC_OK;
     }
-    uint8_t *check = malloc(n ? n : 1);
-    if (!check) return ORC_NOMEM;
-    for (uint32_t i = 0; i < n; ++i) check[i] = mask ? mask[i] : 1;
-    unsigned rem = b->B;
-    unsigned pl = 0;
+    uint32_t *chk = malloc((n ? n : 1) * sizeof(uint32_t));
+    if (!chk) return ORC_NOMEM;
+    uint32_t nchk = 0;
+    unsigned rem = b->B, pl = 0;
+    if (!mask && rem >= 8) {                                /* :365-371, no bitmask */
+        rem -= 8;
+        nchk = plane_first(b->planes, n, (uint8_t)(dp >> rem), greater ? 1 : 2, res, chk);
+        pl = 1;
+        if (!nchk) { free(chk); return ORC_OK; }
+    } else {
+        for (uint32_t i = 0; i < n; ++i) if (!mask || mask[i]) chk[nchk++] = i;
+    }
     while (rem >= 8) {                                      /* :357-385 */
         rem -= 8;
         uint8_t ps = (uint8_t)(dp >> rem);
-        int any = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            if (!check[i]) continue;
-            uint8_t s = plane_byte(b, pl, i);
-            if (s > ps) { if (greater) res[i] = 1; check[i] = 0; }
-            else if (s < ps) { if (!greater) res[i] = 1; check[i] = 0; }
-            else any = 1;
+        const uint8_t *x = b->planes + (size_t)pl * n;
+        uint32_t m = 0;
+        for (uint32_t k = 0; k < nchk; ++k) {
+            uint32_t i = chk[k];
+            uint8_t v = flip(x[i]);
+            if (v == ps) chk[m++] = i;
+            else if (greater == (v > ps)) res[i] = 1;
         }
+        nchk = m;
         ++pl;
-        if (!any) { free(check); return ORC_OK; }
+        if (!nchk) { free(chk); return ORC_OK; }
     }
     if (rem > 0) {                                          /* :388-444 */
         uint32_t ps = (uint32_t)(dp & ((1u << rem) - 1));
-        for (uint32_t i = 0; i < n; ++i) {
-            if (!check[i]) continue;
-            uint32_t s = buff_tail(b, i);
-            if (s > ps) { if (greater) res[i] = 1; }
-            else if (s == ps) { if (inclusive) res[i] = 1; }
+        for (uint32_t k = 0; k < nchk; ++k) {
+            uint32_t i = chk[k], v = buff_tail(b, i);
+            if (v > ps) { if (greater) res[i] = 1; }
+            else if (v == ps) { if (inclusive) res[i] = 1; }
             else { if (!greater) res[i] = 1; }
         }
     } else if (inclusive) {                                 /* :445-449 */
-        for (uint32_t i = 0; i < n; ++i) if (check[i]) res[i] = 1;
+        for (uint32_t k = 0; k < nchk; ++k) res[chk[k]] = 1;
     }
-    free(check);
+    free(chk);
     return ORC_OK;
 }
 
-/* buff_simd256_equal_filter, filter_eq.rs:73-201. */
+/* buff_simd256_equal_filter, filter_eq.rs:73-201; the candidates after each
+ * plane are an ascending index list (the reference's result_bitmask). */
 static int buff_eq(const buff_t *b, double pred, const uint8_t *mask, uint8_t *res) {
     uint32_t n = b->n;
     memset(res, 0, n);
@@ -614,33 +689,42 @@ static int buff_eq(const buff_t *b, double pred, const uint8_t *mask, uint8_t *r
         return ORC_OK;
     }
     uint64_t dp = (uint64_t)fp - (uint64_t)b->base;
-    /* result_bitmask: None (all candidates) until a plane has been scanned */
-    int have = mask != NULL;
-    for (uint32_t i = 0; i < n; ++i) res[i] = mask ? mask[i] : 1;
     unsigned rem = b->B, pl = 0;
+    if (!mask && rem < 8) {                 /* no bitmask, no plane: :181-191 */
+        uint8_t ps = (uint8_t)dp;           /* `delta_fixed_pred as u8`, unmasked */
+        for (uint32_t i = 0; i < n; ++i) res[i] = buff_tail(b, i) == ps;
+        return ORC_OK;
+    }
+    uint32_t *chk = malloc((n ? n : 1) * sizeof(uint32_t));
+    if (!chk) return ORC_NOMEM;
+    uint32_t nchk = 0;
+    if (!mask) {                            /* first plane without a bitmask */
+        rem -= 8;
+        nchk = plane_first(b->planes, n, (uint8_t)(dp >> rem), 0, res, chk);
+        pl = 1;
+        if (!nchk) { free(chk); return ORC_OK; }
+    } else {
+        for (uint32_t i = 0; i < n; ++i) if (mask[i]) chk[nchk++] = i;
+    }
     while (rem >= 8) {
         rem -= 8;
         uint8_t ps = (uint8_t)(dp >> rem);  /* truncating cast: quirk, no width check */
-        int any = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            if (!res[i]) continue;
-            if (plane_byte(b, pl, i) != ps) res[i] = 0; else any = 1;
-        }
+        const uint8_t *x = b->planes + (size_t)pl * n;
+        uint32_t m = 0;
+        for (uint32_t k = 0; k < nchk; ++k)
+            if (flip(x[chk[k]]) == ps) chk[m++] = chk[k];
+        nchk = m;
         ++pl;
-        have = 1;
-        if (!any) return ORC_OK;
+        if (!nchk) { free(chk); return ORC_OK; }
     }
     if (rem > 0) {
-        if (have) {
-            uint32_t ps = (uint32_t)(dp & ((1u << rem) - 1));
-            for (uint32_t i = 0; i < n; ++i)
-                if (res[i] && buff_tail(b, i) != ps) res[i] = 0;
-        } else {
-            uint8_t ps = (uint8_t)dp;       /* `delta_fixed_pred as u8`, unmasked */
-            for (uint32_t i = 0; i < n; ++i)
-                res[i] = buff_tail(b, i) == ps;
-        }
+        uint32_t ps = (uint32_t)(dp & ((1u << rem) - 1));
+        for (uint32_t k = 0; k < nchk; ++k)
+            if (buff_tail(b, chk[k]) == ps) res[chk[k]] = 1;
+    } else {
+        for (uint32_t k = 0; k < nchk; ++k) res[chk[k]] = 1;
     }
+    free(chk);
     return ORC_OK;
 }
 
@@ -878,7 +962,9 @@ int orc_materialize_p(const uint8_t *file, size_t len, const uint64_t *bs, int64
         uint8_t *m;
         if (!chunk_selected(&c, k, bs, chunk_id, &m)) continue;
         free(m);
-        dvec d = {0};
+        dvec d = {0};                   /* sized for the chunk up front, as Vec::with_capacity */
+        d.cap = c.f.nrec[k] ? c.f.nrec[k] : 1;
+        if (!(d.v = malloc(d.cap * sizeof(double)))) { rc = ORC_NOMEM; break; }
         rc = chunk_decompress_p(c.f.scheme, file + c.f.phys[k], c.f.size[k], c.f.nrec[k], precision, &d);
         for (size_t i = 0; !rc && i < d.n; ++i) {
             if (!in_mask(bs, c.f.n_records, c.f.logi[k] + i)) continue;
