@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <thread>
 
 #include "../../../include/ebi_hip.h"
 #include "../kernels/launch.hpp"
@@ -321,6 +322,90 @@ static double ms_since(std::chrono::steady_clock::time_point t) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
 
+// A piece of a column's device image taken from the file: kPieceCopy
+// copies len bytes; kPieceRuns gathers len run bytes src[9 i]; kPieceValues
+// gathers len f64 values src[9 i .. 9 i + 8) (RLE's 9-byte fields).
+enum : uint8_t { kPieceCopy = 0, kPieceRuns = 1, kPieceValues = 2 };
+struct Piece {
+  uint64_t dst;
+  const uint8_t *src;
+  uint64_t len;
+  uint8_t kind;
+  uint64_t end() const { return dst + (kind == kPieceValues ? 8 * len : len); }
+};
+
+constexpr uint64_t kStageBytes = 32ull << 20;  // one half of the upload double buffer
+constexpr uint64_t kStageSplit = 4096;         // repack threads split a window at multiples of this
+
+// Bytes [t0, t1) of the image into dst (= image byte t0): the pieces'
+// bytes, zeros between them.  Piece boundaries and t0 / t1 are multiples of
+// 8 where they cut a kPieceValues piece (pieces start kAlign-aligned and
+// windows are cut at multiples of kStageSplit).
+static void fill_image(const std::vector<Piece> &pieces, uint64_t t0, uint64_t t1, uint8_t *dst) {
+  size_t i = std::upper_bound(pieces.begin(), pieces.end(), t0,
+                              [](uint64_t v, const Piece &p) { return v < p.end(); }) - pieces.begin();
+  uint64_t at = t0;
+  for (; i < pieces.size() && pieces[i].dst < t1; ++i) {
+    const Piece &p = pieces[i];
+    const uint64_t a = std::max(t0, p.dst), b = std::min(t1, p.end());
+    if (a > at) std::memset(dst + (at - t0), 0, a - at);
+    uint8_t *o = dst + (a - t0);
+    if (p.kind == kPieceCopy) {
+      std::memcpy(o, p.src + (a - p.dst), b - a);
+    } else if (p.kind == kPieceRuns) {
+      for (uint64_t j = a - p.dst; j < b - p.dst; ++j) *o++ = p.src[9 * j];
+    } else {
+      for (uint64_t j = (a - p.dst) / 8; j < (b - p.dst) / 8; ++j, o += 8) std::memcpy(o, p.src + 9 * j, 8);
+    }
+    at = b;
+  }
+  if (t1 > at) std::memset(dst + (at - t0), 0, t1 - at);
+}
+
+// Build the image [0, bytes) of `pieces` in device memory d_dst on the
+// context's stream: windows of kStageBytes, alternating between the two
+// pinned buffers, each repacked on up to 8 host threads and copied with
+// hipMemcpyAsync; a buffer is refilled once its previous copy is done.
+// Returns with the copies queued (later work on the stream is ordered
+// after them).
+static int upload_image(ebi_hip_ctx *ctx, const std::vector<Piece> &pieces, uint8_t *d_dst, uint64_t bytes) {
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < 2 && e == hipSuccess; ++s) {
+    if (!ctx->stage[s]) e = hipHostMalloc((void **)&ctx->stage[s], kStageBytes, hipHostMallocDefault);
+    if (e == hipSuccess && !ctx->stage_ev[s]) e = hipEventCreateWithFlags(&ctx->stage_ev[s], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) return hip_fail(ctx, e, "column upload buffers");
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  for (uint64_t w0 = 0, w = 0; w0 < bytes; w0 += kStageBytes, ++w) {
+    const int s = (int)(w & 1);
+    const uint64_t w1 = std::min(bytes, w0 + kStageBytes);
+    // the buffer's last copy (of this or an earlier upload) has ended; an
+    // event never recorded counts as complete
+    if ((e = hipEventSynchronize(ctx->stage_ev[s])) != hipSuccess) break;
+    uint8_t *buf = ctx->stage[s];
+    const uint64_t parts = std::min<uint64_t>(std::min(8u, hw), (w1 - w0 + (1u << 20) - 1) >> 20);
+    if (parts <= 1) {
+      fill_image(pieces, w0, w1, buf);
+    } else {
+      const uint64_t step = (((w1 - w0) / parts) + kStageSplit - 1) / kStageSplit * kStageSplit;
+      std::vector<std::thread> pool;
+      for (uint64_t t0 = w0; t0 < w1; t0 += step) {
+        const uint64_t t1 = std::min(w1, t0 + step);
+        pool.emplace_back([&pieces, t0, t1, buf, w0] { fill_image(pieces, t0, t1, buf + (t0 - w0)); });
+      }
+      for (std::thread &t : pool) t.join();
+    }
+    if ((e = hipMemcpyAsync(d_dst + w0, buf, w1 - w0, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
+        (e = hipEventRecord(ctx->stage_ev[s], ctx->stream)) != hipSuccess)
+      break;
+  }
+  if (e != hipSuccess) {
+    hipStreamSynchronize(ctx->stream);
+    return hip_fail(ctx, e, "column upload");
+  }
+  return EBI_OK;
+}
+
 static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column &col) {
   const auto t0 = std::chrono::steady_clock::now();
   std::string err;
@@ -430,37 +515,32 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
   hipError_t e;
   if ((e = hipMalloc((void **)&col.d_body, std::max<uint64_t>(dev_bytes, kAlign))) != hipSuccess)
     return hip_fail(ctx, e, "hipMalloc(column body)");
-  // Repack into the device layout in a host staging buffer, then one copy.
+  // The device image as pieces of the file (everything between them is
+  // zero), repacked window by window into the context's pinned double
+  // buffer and copied asynchronously, so the copy of one window overlaps the
+  // repack of the next.
   {
-    uint8_t *stage = nullptr;
-    e = hipHostMalloc((void **)&stage, std::max<uint64_t>(dev_bytes, kAlign), hipHostMallocDefault);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipHostMalloc(staging)");
-    std::memset(stage, 0, dev_bytes);
+    std::vector<Piece> pieces;
+    pieces.reserve(nc * 2);
     for (size_t k = 0; k < nc; ++k) {
       const Body &b = bodies[k];
-      HostChunk &h = col.chunks[k];
-      uint8_t *dst = stage + b.off;
-      const uint8_t *p = file + h.geom.physical_offset;
-      if (scheme == kRLE) {
-        const uint64_t f = h.dev.fields;
-        for (uint64_t i = 0; i < f; ++i) {
-          dst[i] = p[8 + 9 * i];
-          std::memcpy(dst + h.dev.values_off + 8 * i, p + 8 + 9 * i + 1, 8);
+      const DevChunk &d = col.chunks[k].dev;
+      const uint8_t *p = file + col.chunks[k].geom.physical_offset;
+      if (scheme == kRLE) {  // u8 runs, then the f64 values, from 9-byte fields
+        if (d.fields) {
+          pieces.push_back(Piece{b.off, p + 8, d.fields, kPieceRuns});
+          pieces.push_back(Piece{b.off + d.values_off, p + 9, d.fields, kPieceValues});
         }
-      } else if (scheme == kBUFF) {
-        const uint32_t n = h.dev.n;
-        for (uint32_t pl = 0; pl < h.dev.n_planes; ++pl)
-          std::memcpy(dst + (size_t)pl * h.dev.plane_stride, p + 20 + (size_t)pl * n, n);
-        const uint64_t tail = ((uint64_t)n * h.dev.r + 7) / 8;
-        std::memcpy(dst + (size_t)h.dev.n_planes * h.dev.plane_stride, p + 20 + (size_t)h.dev.n_planes * n,
-                    tail);
+      } else if (scheme == kBUFF) {  // planes at plane_stride, then the LSB tail
+        for (uint32_t pl = 0; pl < d.n_planes; ++pl)
+          if (d.n) pieces.push_back(Piece{b.off + (uint64_t)pl * d.plane_stride, p + 20 + (size_t)pl * d.n, d.n, kPieceCopy});
+        const uint64_t tail = ((uint64_t)d.n * d.r + 7) / 8;
+        if (tail) pieces.push_back(Piece{b.off + (uint64_t)d.n_planes * d.plane_stride, p + 20 + (size_t)d.n_planes * d.n, tail, kPieceCopy});
       } else if (b.len) {
-        std::memcpy(dst, b.src, b.len);
+        pieces.push_back(Piece{b.off, b.src, b.len, kPieceCopy});
       }
     }
-    e = hipMemcpy(col.d_body, stage, dev_bytes, hipMemcpyHostToDevice);
-    hipHostFree(stage);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(column body)");
+    if ((rc = upload_image(ctx, pieces, col.d_body, dev_bytes))) return rc;
   }
   if (tiles && (e = hipMalloc((void **)&col.d_tile_index, tiles * 8)) != hipSuccess)
     return hip_fail(ctx, e, "hipMalloc(tile index)");
@@ -694,6 +774,10 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   if (ctx->d_flags) hipFree(ctx->d_flags);
   if (ctx->h_flags) hipHostFree(ctx->h_flags);
   if (ctx->h_view) hipHostFree(ctx->h_view);
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->stage[i]) hipHostFree(ctx->stage[i]);
+    if (ctx->stage_ev[i]) hipEventDestroy(ctx->stage_ev[i]);
+  }
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   if (ctx->ev_mid) hipEventDestroy(ctx->ev_mid);

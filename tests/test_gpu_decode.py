@@ -333,3 +333,16 @@ def test_decode_hand_derived_vectors(ctx):
         data = E.encode(np.array(vals), E.CompressorConfig.buff(scale=scale), E.ChunkOption.record_count(8))
         want = OracleDecoder(data).materialize()
         assert np.array_equal(_gpu_decode(ctx, data).view(np.uint64), want.view(np.uint64)), vals
+
+
+@pytest.mark.parametrize("codec,scale", [("rle", 100), ("buff", 10000), ("gorilla", 100), ("uncompressed", 1)])
+def test_upload_spans_several_windows(ctx, codec, scale):
+    """Columns whose device image (RLE: 12 Mi fields of 9 bytes; BUFF ~3 B
+    per record; Gorilla; Uncompressed 96 MiB) is repacked and copied over
+    several 32 MiB windows of the pinned upload buffer: windows cut through
+    RLE run and value arrays, BUFF planes and XOR streams."""
+    vals = G.uniform(12 << 20, -1000.0, 1000.0, 4 if codec == "buff" else 2, seed=9)
+    data = E.encode(vals, E.CompressorConfig.by_name(codec, scale=scale), E.ChunkOption.record_count(1 << 20))
+    want = OracleDecoder(data).materialize()
+    got = _gpu_decode(ctx, data)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
