@@ -182,6 +182,62 @@ struct alignas(16) ColView {
 };
 static_assert(sizeof(ColView) == 112, "ColView layout");
 
+// Parallel XOR index (kernels/xindex.hpp): a chunk's stream is cut into
+// spans of kSpanBits bits (span_count of them: the last one also owns a
+// record that would start at the stream's end); one candidate walk covers
+// kSpansPerLane spans; Gorilla's candidate walks start in the state the
+// chunk's walker reaches at span kHintSpans.
+constexpr uint32_t kSpanBits = 4096;
+constexpr uint32_t kSpansPerLane = 4;
+constexpr uint32_t kHintSpans = 4;
+__host__ __device__ inline uint32_t span_count(uint32_t bits) { return bits / kSpanBits + 1; }
+// Per span of a chunk's stream:
+// the result of parsing it exactly from an entry (the next span's entry,
+// the XOR of its records' payloads, its record count, its last beta* or
+// -1, the error that ended the exact walk or 0) and its true entry with the
+// record index, value and beta* before it; per chunk the walker's state
+// (true entry, value, record index and beta* at its frontier span).
+struct SpanSum {
+  uint64_t exit;
+  uint64_t xr;
+  uint32_t cnt;
+  int16_t lbeta;
+  uint16_t err;
+};
+static_assert(sizeof(SpanSum) == 24, "SpanSum layout");
+struct SpanPick {
+  uint64_t ent;
+  uint64_t v0;
+  uint32_t g0;
+  int32_t beta0;
+};
+static_assert(sizeof(SpanPick) == 24, "SpanPick layout");
+struct ChunkWalk {
+  uint64_t cur;
+  uint64_t v;
+  uint32_t frontier;  // the span count: done
+  uint32_t g;
+  int32_t beta;       // INT32_MAX: unset
+  uint32_t pad;
+};
+static_assert(sizeof(ChunkWalk) == 32, "ChunkWalk layout");
+// Arguments of the index kernels (column cols[0]): span_first (per chunk
+// its first span in the span arrays), groups (chunk, first span) of
+// kSpansPerLane spans, per span cand / res / pick, per chunk walk and ev
+// (its first event, record << 8 | code, ~0 none), the load's error flags.
+struct XIndexArgs {
+  const ColView *cols;
+  const uint32_t *span_first;
+  const uint2 *groups;
+  uint64_t *cand;
+  SpanSum *res;
+  SpanPick *pick;
+  ChunkWalk *walk;
+  uint64_t *ev;
+  uint32_t *err;
+  uint32_t n_groups, nc;
+};
+
 // knn1 candidates of one chunk (Decoder::knn1, api/decoder.rs:469-533):
 // `straddle` when the previous chunk ended inside a vector (its partial
 // covered target[0 .. m1) from record prev_off of that chunk), then `k`

@@ -235,6 +235,64 @@ static hipError_t upload_vec(T **dst, const std::vector<T> &v) {
   return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
+// Parallel checkpoint index of an XOR column (Gorilla, Chimp, Chimp128, Elf,
+// ElfOnChimp; kernels/xindex.hpp): candidates over spans of every chunk's
+// stream (Gorilla: after its walkers have parsed the first kHintSpans
+// spans), the per-chunk walkers, then the checkpoints and per-chunk
+// events.  The view must be uploaded; errors land in ctx->d_flags[0] and
+// d_chunk_err, NaN ends in d_chunk_end, as with the lane-per-chunk pass.
+static int build_xindex(ebi_hip_ctx *ctx, Column &col) {
+  const size_t nc = col.chunks.size();
+  std::vector<uint32_t> first(nc + 1, 0);
+  std::vector<uint2> groups;
+  for (size_t k = 0; k < nc; ++k) {
+    const uint32_t ns = span_count(col.chunks[k].dev.size * 8u);
+    for (uint32_t j = 0; j < ns; j += kSpansPerLane) groups.push_back(make_uint2((uint32_t)k, j));
+    if ((uint64_t)first[k] + ns > UINT32_MAX) return fail(ctx, EBI_OOM, "XOR index: too many spans");
+    first[k + 1] = first[k] + ns;
+  }
+  const uint64_t spans = first[nc];
+  if (groups.size() > UINT32_MAX) return fail(ctx, EBI_OOM, "XOR index: too many span groups");
+  XIndexArgs a{};
+  a.cols = col.d_view;
+  a.err = ctx->d_flags;
+  a.nc = (uint32_t)nc;
+  a.n_groups = (uint32_t)groups.size();
+  // one allocation: span_first, groups, cand, res, pick, walk, ev
+  const size_t o_first = 0, o_groups = round_up((nc + 1) * 4, 256),
+               o_cand = o_groups + round_up(groups.size() * 8, 256), o_res = o_cand + round_up(spans * 8, 256),
+               o_pick = o_res + round_up(spans * sizeof(SpanSum), 256),
+               o_walk = o_pick + round_up(spans * sizeof(SpanPick), 256),
+               o_ev = o_walk + round_up(nc * sizeof(ChunkWalk), 256), total = o_ev + round_up(nc * 8, 256) + 256;
+  uint8_t *d = nullptr;
+  hipError_t e = hipMalloc((void **)&d, total);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(XOR index scratch)");
+  a.span_first = reinterpret_cast<uint32_t *>(d + o_first);
+  a.groups = reinterpret_cast<uint2 *>(d + o_groups);
+  a.cand = reinterpret_cast<uint64_t *>(d + o_cand);
+  a.res = reinterpret_cast<SpanSum *>(d + o_res);
+  a.pick = reinterpret_cast<SpanPick *>(d + o_pick);
+  a.walk = reinterpret_cast<ChunkWalk *>(d + o_walk);
+  a.ev = reinterpret_cast<uint64_t *>(d + o_ev);
+  e = hipMemcpyAsync(d + o_first, first.data(), (nc + 1) * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && !groups.empty())
+    e = hipMemcpyAsync(d + o_groups, groups.data(), groups.size() * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    launch::xindex_init(ctx->stream, a, (uint32_t)spans);
+    const bool gorilla = nc && col.chunks[0].dev.codec == kCodecGorilla;
+    if (gorilla) launch::xindex_walk(ctx->stream, a, kHintSpans, false);  // the state its candidates start in
+    launch::xindex_cands(ctx->stream, a);
+    launch::xindex_walk(ctx->stream, a, UINT32_MAX, gorilla);
+    launch::xindex_emit(ctx->stream, a);
+    e = hipGetLastError();
+  }
+  const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) return hip_fail(ctx, e, "XOR index");
+  return EBI_OK;
+}
+
 // Lane-interleaved copy of an XOR column's segment streams (after the index
 // pass, whose checkpoints give each segment's bits): the row count of every
 // segment job, their prefix as each job's first row (Job::col), then the
@@ -644,10 +702,18 @@ static int load_column(ebi_hip_ctx *ctx, const uint8_t *file, size_t len, Column
     launch::tile_index(ctx->stream, col.d_view, 0, d_index_jobs, (uint32_t)index_jobs.size(), (uint32_t)nc,
                        ctx->d_flags);
   }
-  if (col.xor_codec || col.chimp128)
+  // XOR codecs: the parallel index, or (EBI_HIP_XOR_INDEX=serial) one lane
+  // decoding each chunk from its start
+  const bool xor_index = col.xor_codec || col.chimp128;
+  if (xor_index && ctx->xindex_serial)
     launch::serial(ctx->stream, col.d_view, col.d_prefix + 2, 1, col.n_serial_jobs, nullptr, ctx->d_flags, false,
                    true);
-  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "index launch");
+  if ((e = hipGetLastError()) == hipSuccess && xor_index && !ctx->xindex_serial) rc = build_xindex(ctx, col);
+  if (e != hipSuccess || rc) {
+    hipStreamSynchronize(ctx->stream);
+    if (d_index_jobs) hipFree(d_index_jobs);
+    return rc ? rc : hip_fail(ctx, e, "index launch");
+  }
   std::vector<uint32_t> errs(nc);
   if (nc) e = hipMemcpyAsync(errs.data(), col.d_chunk_err, nc * 4, hipMemcpyDeviceToHost, ctx->stream);
   const hipError_t e2 = hipStreamSynchronize(ctx->stream);
@@ -753,6 +819,8 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   }
   const char *mode = std::getenv("EBI_HIP_XOR_MODE");
   if (mode && std::strcmp(mode, "serial") == 0) c->xor_mode = 1;
+  const char *xi = std::getenv("EBI_HIP_XOR_INDEX");
+  if (xi && std::strcmp(xi, "serial") == 0) c->xindex_serial = 1;
   // launch-shape overrides (the batch tuner picks its own per batch)
   if (const char *v = std::getenv("EBI_HIP_BLOCKS_PER_CU")) c->blocks_per_cu = std::max(1, std::min(64, std::atoi(v)));
   if (const char *v = std::getenv("EBI_HIP_BATCH_MIXED")) c->batch_mixed = std::atoi(v) != 0;

@@ -121,6 +121,7 @@ struct ebi_hip_ctx {
   uint32_t batch_mixed = 0;     // untuned prepared batches: 1 = one k_decode_mix launch
   std::map<std::string, ebi_batch_config> tune_cache;  // prepared-batch shapes by batch signature
   int xor_mode = 0;  // 0: checkpoint segments, 1: lane-serial chunks (Gorilla, Chimp, Elf)
+  int xindex_serial = 0;  // 1: XOR load index by one lane per chunk instead of the parallel index
   // pinned double buffer of the column upload (allocated at the first load)
   uint8_t *stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};

@@ -976,7 +976,7 @@ __host__ __device__ __forceinline__ bool chimp128_rec(const W &r, uint32_t &lead
   return true;
 }
 
-__host__ __device__ uint32_t chimp128_index(const uint8_t *data, uint32_t size_bits, uint32_t n, XorCkpt *ck) {
+__host__ __device__ inline uint32_t chimp128_index(const uint8_t *data, uint32_t size_bits, uint32_t n, XorCkpt *ck) {
   SerialWindow r;
   r.init(data, 0);
   uint32_t lead = 0xffffffffu;

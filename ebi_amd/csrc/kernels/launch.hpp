@@ -43,6 +43,15 @@ void chimp128(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32
 // rows (each job's first row in its Job::col).
 void ilv_rows(hipStream_t s, const ColView *cols, uint32_t n_jobs, uint32_t *rows);
 void ilv_copy(hipStream_t s, const ColView *cols, uint32_t n_jobs, const uint32_t *rows, uint4 *ilv);
+// Load-time parallel index of the XOR column cols[0] (kernels/xindex.hpp):
+// initial state, candidates (Gorilla: from its walkers' frontiers on), the
+// per-chunk walkers up to span `stop` (blocks: Gorilla's walker blocks),
+// then checkpoints and per-chunk events (NaN ends into chunk_end, errors
+// into chunk_err and a.err).
+void xindex_init(hipStream_t s, const XIndexArgs &a, uint32_t n_spans);
+void xindex_cands(hipStream_t s, const XIndexArgs &a);
+void xindex_walk(hipStream_t s, const XIndexArgs &a, uint32_t stop, bool blocks);
+void xindex_emit(hipStream_t s, const XIndexArgs &a);
 // Load-time tile index for DeltaSprintz and RLE chunks of column slot `col`.
 void tile_index(hipStream_t s, const ColView *cols, uint32_t col, const Job *jobs, uint32_t n_jobs,
                 uint32_t n_chunks, uint32_t *err);
