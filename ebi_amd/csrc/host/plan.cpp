@@ -346,6 +346,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   uint64_t jobs = 0, segs = 0, chunks = 0, qsegs = 0, blocks = 0, max_words = 0;
   uint64_t tiles[kSpanKinds] = {};  // span jobs per SpanKind
   uint32_t n_fills = 0;
+  uint32_t qseg_codecs = 0;  // bit c: segment jobs of codec c to query
   bool any_fold = false;
   auto fill = [&](uint64_t *ptr, uint64_t words, uint64_t v0, uint64_t v1, uint64_t v2, uint64_t v3,
                   uint32_t period) {
@@ -396,7 +397,10 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
     const bool runs = p.q.op != kOpDecode && !(p.q.op == kOpMaterialize && !p.compact);
     if (runs) chunks += p.nc;
     if (p.flags & kPfTiles) tiles[p.tile_kind] += p.tile_kind == kSpanRle ? c.n_field_spans : c.n_span_jobs;
-    if (p.flags & kPfSegs) qsegs += c.n_seg_jobs;
+    if (p.flags & kPfSegs) {
+      qsegs += c.n_seg_jobs;
+      if (c.n_seg_jobs) qseg_codecs |= 1u << c.meta.scheme;
+    }
     if (p.flags & kPfCompact) blocks += p.n_blocks;
     if (p.flags & kPfFold) any_fold = true;
     results[i] = PlanResult{0.0, 0};
@@ -434,7 +438,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
                  reinterpret_cast<const uint32_t *>(g.dev + L.dpre), (uint32_t)jobs,
                  reinterpret_cast<const uint32_t *>(g.dev + L.dseg), (uint32_t)segs, g.d_flags, max_blocks);
   for (int k = 0; k < kSpanKinds; ++k) launch::group_tiles(s, dd, di + (size_t)(kPreSpan + k) * (n + 1), np, (uint32_t)tiles[k], k);
-  launch::query_seg(s, dd, di + (size_t)kPreSeg * (n + 1), np, (uint32_t)qsegs, g.d_flags, max_blocks);
+  launch::query_seg(s, dd, di + (size_t)kPreSeg * (n + 1), np, (uint32_t)qsegs, g.d_flags, max_blocks, qseg_codecs);
   launch::group_fold(s, dd, di, np, (uint32_t)chunks, any_fold);
   launch::group_compact(s, dd, di + (size_t)kPreBlock * (n + 1), np, (uint32_t)blocks);
   if ((e = hipGetLastError()) != hipSuccess) return e;

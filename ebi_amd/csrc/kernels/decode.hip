@@ -444,28 +444,48 @@ __device__ __forceinline__ void keys_to_acc(int agg, uint64_t kmin, uint64_t kma
 }
 
 // Fused queries over the checkpoint-segment jobs of every plan of a group
-// with kPfSegs (Gorilla / Chimp / Elf / ElfOnChimp / Chimp128 columns; the
-// plans' segment jobs concatenated, prefix over the plans): the records are
+// with kPfSegs whose column has codec kCodec (Gorilla / Chimp / Elf /
+// ElfOnChimp / Chimp128; the plans' segment jobs concatenated, prefix over
+// all plans of the group, one launch per codec present: a wave that reaches
+// a plan of another codec steps past its jobs at once): the records are
 // decoded as in k_decode_seg and consumed in registers.  Per plan: kPfPred
 // applies the f64 predicate, kPfBits writes the selected records' bits to
 // a.out_bits (zeroed before), the aggregate (flag bits 4-5) goes to the
 // per-record-tile double-double partials (sum; each the butterfly sum of its
 // 8 segments' lane sums) or the chunk accumulators (min / max), and with
 // kPfAggSel the selected records are counted per chunk (acc[1]).  Chunks
-// with a.chunk_sel 0 are skipped.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_query_seg(
+// with a.chunk_sel 0 are skipped.  One kernel per codec, so that each
+// keeps its own registers: Gorilla and Chimp records never touch LDS and run
+// at the waves per SIMD their registers allow; Chimp128 jobs need the
+// [kSeg128][64] stage and the distances (four blocks per CU by LDS).
+template <uint32_t kCodec>
+struct QuerySegWaves {
+  static constexpr int value = kCodec == kCodecGorilla || kCodec == kCodecChimp ? 5 : 4;
+};
+template <uint32_t kCodec>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QuerySegWaves<kCodec>::value))) k_query_seg(
     const PlanDesc *__restrict__ plans, const uint32_t *__restrict__ prefix, uint32_t n_plans, uint32_t n_jobs,
     uint32_t *err) {
-  __shared__ __align__(16) double lds[4][(kRun + 1) * kWave];  // Chimp128 jobs: the payload / value stage
-  __shared__ uint8_t dists[4][kJob128];                         // and the source distances
-  __shared__ double elf_tab[kElfTab];
-  if (threadIdx.x < kElfTab) elf_tab_fill(elf_tab, threadIdx.x, c_p10, c_n10);
-  __syncthreads();
+  constexpr bool kC128 = kCodec == kCodecChimp128;
+  constexpr bool kTab = kCodec == kCodecElf || kCodec == kCodecElfOnChimp;
+  __shared__ __align__(16) double lds[4][kC128 ? kSeg128 * kWave : 1];  // Chimp128 jobs: the payload / value stage
+  __shared__ uint8_t dists[4][kC128 ? kJob128 : 1];                    // and the source distances
+  __shared__ double elf_tab[kTab ? kElfTab : 1];
+  if (kTab) {
+    if (threadIdx.x < kElfTab) elf_tab_fill(elf_tab, threadIdx.x, c_p10, c_n10);
+    __syncthreads();
+  }
   const uint32_t lane = lane_id();
   const uint32_t stride = gridDim.x * 4;
   for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / kWave); j < n_jobs; j += stride) {
     const uint32_t pi = plan_of(prefix, n_plans, j);
     const PlanDesc &P = plans[pi];
+    if (P.scheme != kCodec) {
+      // wave-uniform: on to this wave's first job past the plan (the loop
+      // adds the last stride)
+      j += (prefix[pi + 1] - j - 1) / stride * stride;
+      continue;
+    }
     const PredF64 pr = pred_of(P.a);
     const uint64_t *mask = P.a.mask;
     uint64_t *out_bits = P.a.out_bits;
@@ -480,69 +500,63 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const Job jb = cv.seg_jobs[j - prefix[pi]];
     if (!P.a.chunk_sel[jb.chunk]) continue;
     const DevChunk c = cv.chunks[jb.chunk];
-    if (c.codec == kCodecChimp128) {
+    if constexpr (kC128) {
+      const uint32_t wave = threadIdx.x / kWave;
       C128Query q{pr, mask, out_bits, partials, &c, pred, want_bits, agg, 0.0, 0.0, ~0ull, 0, 0};
-      const uint32_t e = c128_job(cv, c, jb.tile, jb.col, lds[threadIdx.x / kWave], dists[threadIdx.x / kWave], q);
+      const uint32_t e = c128_job(cv, c, jb.tile, jb.col, lds[wave], dists[wave], q);
       if (e) atomicOr(err, e);
       if (agg > 0) keys_to_acc(agg, q.kmin, q.kmax, acc + 4ull * jb.chunk);
       if (count && lane == 0 && q.cnt) atomicAdd((unsigned long long *)(acc + 4ull * jb.chunk) + 1, (unsigned long long)q.cnt);
-      continue;
-    }
-    const uint32_t i0 = (jb.tile + lane) * kSeg;
-    const uint64_t g = c.rec_off + i0;
-    uint64_t m = ~0ull;
-    if (mask && i0 < c.n) {  // the 64 bitmask bits from record g (the bitmask has a spare word)
-      const uint32_t sh = (uint32_t)(g & 63);
-      m = mask[g >> 6] >> sh;
-      if (sh) m |= mask[(g >> 6) + 1] << (64 - sh);
-    }
-    QueryOut q{pr, i0, c.n, pred, agg, m, 0, 0.0, 0.0, ~0ull, 0};
-    uint32_t e;
-    switch (c.codec) {
-      case kCodecGorilla: e = segment_wave<kCodecGorilla>(cv, c, jb.tile, jb.col, elf_tab, q); break;
-      case kCodecChimp: e = segment_wave<kCodecChimp>(cv, c, jb.tile, jb.col, elf_tab, q); break;
-      case kCodecElf: e = segment_wave<kCodecElf>(cv, c, jb.tile, jb.col, elf_tab, q); break;
-      case kCodecElfOnChimp: e = segment_wave<kCodecElfOnChimp>(cv, c, jb.tile, jb.col, elf_tab, q); break;
-      default: e = kErrCorrupt; break;
-    }
-    if (e) atomicOr(err, e);
-    if (want_bits && i0 < c.n) {
-      // a word wholly inside this segment is the lane's own; one shared
-      // with another segment or chunk takes atomic ORs
-      unsigned long long *w = reinterpret_cast<unsigned long long *>(out_bits) + (g >> 6);
-      const uint32_t sh = (uint32_t)(g & 63);
-      if (sh == 0 && i0 + kSeg <= c.n) {
-        *w = q.obits;
-      } else if (q.obits) {
-        atomicOr(w, (unsigned long long)(q.obits << sh));
-        if (sh) atomicOr(w + 1, (unsigned long long)(q.obits >> (64 - sh)));
+    } else {
+      const uint32_t i0 = (jb.tile + lane) * kSeg;
+      const uint64_t g = c.rec_off + i0;
+      uint64_t m = ~0ull;
+      if (mask && i0 < c.n) {  // the 64 bitmask bits from record g (the bitmask has a spare word)
+        const uint32_t sh = (uint32_t)(g & 63);
+        m = mask[g >> 6] >> sh;
+        if (sh) m |= mask[(g >> 6) + 1] << (64 - sh);
       }
-    }
-    if (agg == 0) {
-      // the 8 segments of a record tile: butterfly in a lane-independent order
-#pragma unroll
-      for (int d = 1; d < 8; d <<= 1) {
-        const double oh = __shfl_xor(q.hi, d, kWave), ol = __shfl_xor(q.lo, d, kWave);
-        if ((lane & d) == 0) {
-          dd_add(q.hi, q.lo, oh, ol);
-        } else {
-          double h2 = oh, l2 = ol;
-          dd_add(h2, l2, q.hi, q.lo);
-          q.hi = h2;
-          q.lo = l2;
+      QueryOut q{pr, i0, c.n, pred, agg, m, 0, 0.0, 0.0, ~0ull, 0};
+      const uint32_t e = c.codec == kCodec ? segment_wave<kCodec>(cv, c, jb.tile, jb.col, elf_tab, q) : kErrCorrupt;
+      if (e) atomicOr(err, e);
+      if (want_bits && i0 < c.n) {
+        // a word wholly inside this segment is the lane's own; one shared
+        // with another segment or chunk takes atomic ORs
+        unsigned long long *w = reinterpret_cast<unsigned long long *>(out_bits) + (g >> 6);
+        const uint32_t sh = (uint32_t)(g & 63);
+        if (sh == 0 && i0 + kSeg <= c.n) {
+          *w = q.obits;
+        } else if (q.obits) {
+          atomicOr(w, (unsigned long long)(q.obits << sh));
+          if (sh) atomicOr(w + 1, (unsigned long long)(q.obits >> (64 - sh)));
         }
       }
-      if ((lane & 7) == 0 && i0 < c.n) {
-        double *pp = partials + 2ull * (c.rtile0 + (jb.tile + lane) / 8);
-        pp[0] = q.hi;
-        pp[1] = q.lo;
+      if (agg == 0) {
+        // the 8 segments of a record tile: butterfly in a lane-independent order
+#pragma unroll
+        for (int d = 1; d < 8; d <<= 1) {
+          const double oh = __shfl_xor(q.hi, d, kWave), ol = __shfl_xor(q.lo, d, kWave);
+          if ((lane & d) == 0) {
+            dd_add(q.hi, q.lo, oh, ol);
+          } else {
+            double h2 = oh, l2 = ol;
+            dd_add(h2, l2, q.hi, q.lo);
+            q.hi = h2;
+            q.lo = l2;
+          }
+        }
+        if ((lane & 7) == 0 && i0 < c.n) {
+          double *pp = partials + 2ull * (c.rtile0 + (jb.tile + lane) / 8);
+          pp[0] = q.hi;
+          pp[1] = q.lo;
+        }
+        if (count) {
+          const uint64_t n = wave_sum_u64(i0 < c.n ? (uint64_t)__popcll(q.obits) : 0ull);
+          if (lane == 0 && n) atomicAdd((unsigned long long *)(acc + 4ull * jb.chunk) + 1, (unsigned long long)n);
+        }
+      } else if (agg > 0) {
+        keys_to_acc(agg, q.kmin, q.kmax, acc + 4ull * jb.chunk);
       }
-      if (count) {
-        const uint64_t n = wave_sum_u64(i0 < c.n ? (uint64_t)__popcll(q.obits) : 0ull);
-        if (lane == 0 && n) atomicAdd((unsigned long long *)(acc + 4ull * jb.chunk) + 1, (unsigned long long)n);
-      }
-    } else if (agg > 0) {
-      keys_to_acc(agg, q.kmin, q.kmax, acc + 4ull * jb.chunk);
     }
   }
 }
@@ -832,11 +846,22 @@ void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t 
 }
 
 void query_seg(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
-               uint32_t *err, uint32_t max_blocks) {
+               uint32_t *err, uint32_t max_blocks, uint32_t codecs) {
   if (!n_jobs) return;
   uint32_t blocks = (n_jobs + 3) / 4;
   if (max_blocks && blocks > max_blocks) blocks = max_blocks;
-  hipLaunchKernelGGL(k_query_seg, dim3(blocks), dim3(256), 0, s, plans, prefix, n_plans, n_jobs, err);
+  if (!codecs) codecs = kSegCodecs;
+  const dim3 g(blocks);
+  if (codecs & (1u << kCodecGorilla))
+    hipLaunchKernelGGL(k_query_seg<kCodecGorilla>, g, dim3(256), 0, s, plans, prefix, n_plans, n_jobs, err);
+  if (codecs & (1u << kCodecChimp))
+    hipLaunchKernelGGL(k_query_seg<kCodecChimp>, g, dim3(256), 0, s, plans, prefix, n_plans, n_jobs, err);
+  if (codecs & (1u << kCodecElf))
+    hipLaunchKernelGGL(k_query_seg<kCodecElf>, g, dim3(256), 0, s, plans, prefix, n_plans, n_jobs, err);
+  if (codecs & (1u << kCodecElfOnChimp))
+    hipLaunchKernelGGL(k_query_seg<kCodecElfOnChimp>, g, dim3(256), 0, s, plans, prefix, n_plans, n_jobs, err);
+  if (codecs & (1u << kCodecChimp128))
+    hipLaunchKernelGGL(k_query_seg<kCodecChimp128>, g, dim3(256), 0, s, plans, prefix, n_plans, n_jobs, err);
 }
 
 void serial(hipStream_t s, const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs,

@@ -66,15 +66,18 @@ void aggregate(hipStream_t s, const ColView *cols, const Job *jobs, uint32_t n_j
 // flat work items mapped to plans by prefix arrays of n_plans + 1 entries):
 // scratch fills and chunk selection in one launch (items: chunks), the
 // record-tile span pass of one SpanKind (items: span jobs), the fused decode + query over segment jobs (items:
-// segment jobs; persistent grid of at most max_blocks), the per-chunk fold
+// segment jobs; persistent grid of at most max_blocks; one launch per codec
+// in `codecs`, bit c for codec c, 0: every segment codec), the per-chunk fold
 // and the final fold (items: chunks), and stream compaction (items: 32,768-
 // record blocks).
 void group_prep(hipStream_t s, const GroupFill *fills, uint32_t n_fills, uint64_t max_words, const PlanDesc *plans,
                 const uint32_t *prefix, uint32_t n_plans, uint32_t n_items);
 void group_tiles(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
                  int kind);
+constexpr uint32_t kSegCodecs = (1u << kCodecGorilla) | (1u << kCodecChimp) | (1u << kCodecChimp128) |
+                                (1u << kCodecElfOnChimp) | (1u << kCodecElf);
 void query_seg(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,
-               uint32_t *err, uint32_t max_blocks);
+               uint32_t *err, uint32_t max_blocks, uint32_t codecs = 0);
 void group_fold(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_items,
                 bool any);
 void group_compact(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_blocks);

@@ -14,7 +14,11 @@ KERNELS = {  # mangled-name prefix -> minimum occupancy (waves per SIMD)
     "_ZN3ebi8k_decodeE": 5,
     "_ZN3ebi12k_decode_segE": 4,
     "_ZN3ebi12k_decode_mixE": 4,  # both job lists in one launch
-    "_ZN3ebi11k_query_segE": 4,  # the segment decode with the fused queries
+    "_ZN3ebi11k_query_segILj3EE": 5,  # the segment decode with the fused queries: Gorilla
+    "_ZN3ebi11k_query_segILj4EE": 5,  # Chimp
+    "_ZN3ebi11k_query_segILj5EE": 4,  # Chimp128 (LDS: four blocks per CU)
+    "_ZN3ebi11k_query_segILj6EE": 4,  # ElfOnChimp
+    "_ZN3ebi11k_query_segILj7EE": 4,  # Elf
     "_ZN3ebi13k_group_tilesILi0E": 6,  # query spans on BUFF codes
     "_ZN3ebi13k_group_tilesILi1E": 6,  # query spans on DeltaSprintz codes
     "_ZN3ebi13k_group_tilesILi2E": 7,  # query spans on the bitmask / decoded records
