@@ -503,7 +503,10 @@ constexpr uint32_t kRun = 16;  // records of a run (between two flushes of the o
 // records it leaves in the sink are then meaningless.  sink.row(i, k, v,
 // act) receives record i as row k of the run, act = whether the lane takes
 // part in it (a sink may ignore the rows with act false).  Returns the
-// lane's error.
+// lane's error.  kChecked false drops the per-record checks (corrupt
+// header, overrun, terminator): for streams the load-time index has already
+// parsed whole and accepted, where none can fire; the returned error is
+// then the one passed in.
 
 // Record 0 of a Gorilla or Chimp chunk (the stream's first 64 bits).
 template <class Sink>
@@ -519,7 +522,7 @@ __host__ __device__ __forceinline__ uint32_t xor_seg_first(XorState &s, uint32_t
 
 // Gorilla, decoder/chunk_reader/gorilla.rs:77-157 (the record step of
 // gorilla_run).
-template <class Sink>
+template <bool kChecked, class Sink>
 __host__ __device__ __forceinline__ uint32_t gorilla_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
                                                 uint32_t cnt, uint32_t from, uint32_t lim, Sink &out,
                                                 uint32_t e) {
@@ -528,13 +531,13 @@ __host__ __device__ __forceinline__ uint32_t gorilla_seg(XorState &s, uint32_t s
   uint32_t L = s.a, T = s.b;
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t i = a + k;
-    const bool act = e == 0 && i >= from && i < lim;
+    const bool act = (!kChecked || e == 0) && i >= from && i < lim;
     const uint32_t c2 = (uint32_t)(r.hi >> 62);
     const bool nz = c2 >= 2, fresh = c2 == 3;
     const uint32_t h = (uint32_t)(r.hi >> 51) & 0x7ff;
     const uint32_t l = fresh ? h >> 6 : L;
     const uint32_t sig = fresh ? (h & 63) + 1 : 64 - L - T;
-    const bool bad = fresh && l + sig > 64;
+    const bool bad = kChecked && fresh && l + sig > 64;
     const uint32_t o = fresh ? 13 : 2;
     const uint32_t t = (64 - l - sig) & 63;
     const uint64_t x = r.field(o, sig) << t;
@@ -544,11 +547,13 @@ __host__ __device__ __forceinline__ uint32_t gorilla_seg(XorState &s, uint32_t s
     T = upd ? t : T;
     r.consume(act && !bad ? (nz ? o + sig : 1u) : 0u);
     r.refill();
-    const uint32_t code = bad                      ? (uint32_t)kErrCorrupt
-                          : r.pos > size_bits      ? (uint32_t)kErrOverrun
-                          : (i == n) != (v == kNan) ? (uint32_t)kErrCount
-                                                   : 0u;
-    e = act ? code : e;
+    if (kChecked) {
+      const uint32_t code = bad                      ? (uint32_t)kErrCorrupt
+                            : r.pos > size_bits      ? (uint32_t)kErrOverrun
+                            : (i == n) != (v == kNan) ? (uint32_t)kErrCount
+                                                     : 0u;
+      e = act ? code : e;
+    }
     out.row(i, k, as_f64(v), act);
   }
   s.v = v;
@@ -558,7 +563,7 @@ __host__ __device__ __forceinline__ uint32_t gorilla_seg(XorState &s, uint32_t s
 }
 
 // Chimp, decoder/chunk_reader/chimp.rs:59-125 (the record step of chimp_run).
-template <class Sink>
+template <bool kChecked, class Sink>
 __host__ __device__ __forceinline__ uint32_t chimp_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
                                               uint32_t cnt, uint32_t from, uint32_t lim, Sink &out,
                                               uint32_t e) {
@@ -567,13 +572,13 @@ __host__ __device__ __forceinline__ uint32_t chimp_seg(XorState &s, uint32_t siz
   uint32_t lead = s.a;
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t i = a + k;
-    const bool act = e == 0 && i >= from && i < lim;
+    const bool act = (!kChecked || e == 0) && i >= from && i < lim;
     const uint32_t f = (uint32_t)(r.hi >> 62);
     const uint32_t h9 = (uint32_t)(r.hi >> 53) & 0x1ff;
     const uint32_t nl = (f & 1) ? lead_repr(h9 >> 6) : lead;  // flags 11 and 01 carry a lead code
     const uint32_t s6 = h9 & 63;
     const uint32_t sig = f == 1 ? (s6 ? s6 : 64u) : 64 - nl;
-    const bool bad = (f == 2 && lead > 64) || (f == 1 && sig + nl > 64);
+    const bool bad = kChecked && ((f == 2 && lead > 64) || (f == 1 && sig + nl > 64));
     const uint32_t o = f == 3 ? 5u : f == 1 ? 11u : 2u;
     const uint64_t x = r.field(o, sig) << (f == 1 ? (64 - sig - nl) & 63 : 0u);
     const bool upd = act && f != 0 && !bad;
@@ -581,11 +586,13 @@ __host__ __device__ __forceinline__ uint32_t chimp_seg(XorState &s, uint32_t siz
     lead = upd ? nl : lead;
     r.consume(act && !bad ? (f ? o + sig : 2u) : 0u);
     r.refill();
-    const uint32_t code = bad                      ? (uint32_t)kErrCorrupt
-                          : r.pos > size_bits      ? (uint32_t)kErrOverrun
-                          : (i == n) != (v == kNan) ? (uint32_t)kErrCount
-                                                   : 0u;
-    e = act ? code : e;
+    if (kChecked) {
+      const uint32_t code = bad                      ? (uint32_t)kErrCorrupt
+                            : r.pos > size_bits      ? (uint32_t)kErrOverrun
+                            : (i == n) != (v == kNan) ? (uint32_t)kErrCount
+                                                     : 0u;
+      e = act ? code : e;
+    }
     out.row(i, k, as_f64(v), act);
   }
   s.v = v;
@@ -622,7 +629,7 @@ __host__ __device__ __forceinline__ uint32_t elf_seg_first(XorState &s, uint32_t
 // Elf, decoder/chunk_reader/elf.rs:46-258 (the record step of elf_run for
 // records i >= 1).  Only records the lane takes part in reach the sink.
 // tab: the powers of ten of elf_recover_tab.
-template <class Sink>
+template <bool kChecked, class Sink>
 __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
                                             uint32_t cnt, uint32_t from, uint32_t lim, Sink &out,
                                             uint32_t e, const double *tab) {
@@ -632,7 +639,7 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
   int32_t beta = s.beta;
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t i = a + k;
-    const bool act = e == 0 && i >= from && i < lim;
+    const bool act = (!kChecked || e == 0) && i >= from && i < lim;
     // prefix (0: erased, reuse beta*; 10: not erased; 11 + 4 bits: erased,
     // new beta*), then the 2-bit XOR flag and a 9-bit header, all within
     // the window's first 17 bits
@@ -649,7 +656,7 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
     const uint32_t nL = hdr ? lead_repr(h9 >> 6) : L;
     const uint32_t cb = f3 ? h9 & 63 : (h9 >> 2) & 15;
     const uint32_t c = cb ? cb : (f3 ? 64u : 16u);
-    const bool bad = (hdr & (nL + c > 64)) | (f0 & ((uint64_t)L + T > 64));
+    const bool bad = kChecked && ((hdr & (nL + c > 64)) | (f0 & ((uint64_t)L + T > 64)));
     const uint32_t nT = hdr ? (64 - nL - c) & 63 : T;
     const uint32_t len = hdr ? c - 1 : (f0 ? 64 - L - T : 0u);
     const uint32_t off = pre + (f3 ? 11u : hdr ? 9u : 2u);
@@ -664,11 +671,13 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
     beta = upd ? nbeta : beta;
     r.consume(upd ? (f == 1 ? pre + 2 : off + len) : 0u);
     r.refill();
-    const uint32_t code = bad                 ? (uint32_t)kErrCorrupt
-                          : r.pos > size_bits ? (uint32_t)kErrOverrun
-                          : (i == n) != term  ? (uint32_t)kErrCount
-                                              : 0u;
-    e = act ? code : e;
+    if (kChecked) {
+      const uint32_t code = bad                 ? (uint32_t)kErrCorrupt
+                            : r.pos > size_bits ? (uint32_t)kErrOverrun
+                            : (i == n) != term  ? (uint32_t)kErrCount
+                                                : 0u;
+      e = act ? code : e;
+    }
     const double v = as_f64(stored);
     const double w = p2 != 2 ? elf_recover_tab(v, beta, tab) : v;
     out.row(i, k, w, act);
@@ -700,7 +709,7 @@ __host__ __device__ __forceinline__ uint32_t elfchimp_seg_first(XorState &s, uin
 // ElfOnChimp, the record step of elfchimp_run for records i >= 1 in the
 // lockstep form: the Elf prefix, then chimp_seg's step at offset `pre`
 // (the 2-bit flag and the 9-bit header lie in the window's first 17 bits).
-template <class Sink>
+template <bool kChecked, class Sink>
 __host__ __device__ __forceinline__ uint32_t elfchimp_seg(XorState &s, uint32_t size_bits, uint32_t n, uint32_t a,
                                                  uint32_t cnt, uint32_t from, uint32_t lim, Sink &out,
                                                  uint32_t e, const double *tab) {
@@ -710,7 +719,7 @@ __host__ __device__ __forceinline__ uint32_t elfchimp_seg(XorState &s, uint32_t 
   int32_t beta = s.beta;
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t i = a + k;
-    const bool act = e == 0 && i >= from && i < lim;
+    const bool act = (!kChecked || e == 0) && i >= from && i < lim;
     const uint32_t h32 = (uint32_t)(r.hi >> 32);
     const uint32_t p2 = h32 >> 30;
     const uint32_t pre = p2 == 3 ? 6u : p2 == 2 ? 2u : 1u;
@@ -720,7 +729,7 @@ __host__ __device__ __forceinline__ uint32_t elfchimp_seg(XorState &s, uint32_t 
     const uint32_t nl = (f & 1) ? lead_repr(h9 >> 6) : lead;  // flags 11 and 01 carry a lead code
     const uint32_t s6 = h9 & 63;
     const uint32_t sig = f == 1 ? (s6 ? s6 : 64u) : 64 - nl;
-    const bool bad = (f == 2 && lead > 64) || (f == 1 && sig + nl > 64);
+    const bool bad = kChecked && ((f == 2 && lead > 64) || (f == 1 && sig + nl > 64));
     const uint32_t o = pre + (f == 3 ? 5u : f == 1 ? 11u : 2u);
     const uint64_t x = r.field(o, sig) << (f == 1 ? (64 - sig - nl) & 63 : 0u);
     const bool upd = act && !bad;
@@ -729,11 +738,13 @@ __host__ __device__ __forceinline__ uint32_t elfchimp_seg(XorState &s, uint32_t 
     beta = upd ? nbeta : beta;
     r.consume(upd ? (f ? o + sig : pre + 2) : 0u);
     r.refill();
-    const uint32_t code = bad                      ? (uint32_t)kErrCorrupt
-                          : r.pos > size_bits      ? (uint32_t)kErrOverrun
-                          : (i == n) != (v == kNan) ? (uint32_t)kErrCount
-                                                   : 0u;
-    e = act ? code : e;
+    if (kChecked) {
+      const uint32_t code = bad                      ? (uint32_t)kErrCorrupt
+                            : r.pos > size_bits      ? (uint32_t)kErrOverrun
+                            : (i == n) != (v == kNan) ? (uint32_t)kErrCount
+                                                     : 0u;
+      e = act ? code : e;
+    }
     const double d = as_f64(v);
     out.row(i, k, p2 != 2 ? elf_recover_tab(d, beta, tab) : d, act);
   }
@@ -775,14 +786,14 @@ __host__ __device__ __forceinline__ uint32_t chimp128_seg_first(BitWindow &r, ui
 
 // px / pd: x' and dist' of the record before `a` when it is in this segment
 // (have_prev), then of each record parsed.
-template <class Sink>
+template <bool kChecked, class Sink>
 __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, uint32_t &lead, uint32_t size_bits,
                                                        uint32_t a, uint32_t cnt, uint32_t from, uint32_t lim,
                                                        Sink &out, uint32_t e, uint64_t px, uint32_t pd,
                                                        bool have_prev) {
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t i = a + k;
-    const bool act = e == 0 && i >= from && i < lim;
+    const bool act = (!kChecked || e == 0) && i >= from && i < lim;
     // 00: 7-bit ring index; 11: lead code, 64 - lead bits; 10: 64 - lead
     // bits with the stored lead; 01: 16-bit header (index, lead code, sig)
     const uint32_t f = (uint32_t)(r.hi >> 62);
@@ -792,7 +803,7 @@ __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, ui
     const uint32_t nl = (f & 1) ? lc : lead;
     const uint32_t s6 = h & 63;
     const uint32_t len = f == 1 ? (s6 ? s6 : 64u) : 64 - nl;
-    const bool bad = ((f == 2) & (lead > 64)) | ((f == 1) & (len + nl > 64));
+    const bool bad = kChecked && (((f == 2) & (lead > 64)) | ((f == 1) & (len + nl > 64)));
     const uint32_t off = f == 3 ? 5u : f == 1 ? 18u : 2u;
     const uint64_t x = r.field(off, len) << (f == 1 ? (64 - len - nl) & 63 : 0u) & (f ? ~0ull : 0ull);
     const uint32_t back = f <= 1 ? (((i - 1) - idx) & 127u) + 1 : 1u;  // i - source
@@ -807,8 +818,10 @@ __host__ __device__ __forceinline__ uint32_t chimp128_parse_seg(BitWindow &r, ui
     lead = upd && f != 0 ? nl : lead;
     r.consume(upd ? (f == 0 ? 9u : off + len) : 0u);
     r.refill();
-    const uint32_t code = bad ? (uint32_t)kErrCorrupt : r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
-    e = act ? code : e;
+    if (kChecked) {
+      const uint32_t code = bad ? (uint32_t)kErrCorrupt : r.pos > size_bits ? (uint32_t)kErrOverrun : 0u;
+      e = act ? code : e;
+    }
     out.row(i, k, as_f64(xf), act);
     out.dist(k, dist, act);
   }

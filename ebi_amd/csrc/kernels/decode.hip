@@ -51,8 +51,11 @@ __device__ __forceinline__ void dead_state(XorState &s) {
 // Elf), segment first + l on lane l, decoded in lockstep runs of kRun
 // records from the checkpoints into `out`: out.row(i, k, v, act) takes each
 // record, out.run_end(k0) follows each run.  The loop over runs is
-// wave-uniform: lanes without a segment (past the chunk's end) or stopped by
-// an error stay in it.
+// wave-uniform: lanes without a segment (past the chunk's end) stay in it.
+// The chunks reaching here were parsed whole and accepted by the load-time
+// index (a rejected chunk fails the call before any launch), so the records
+// go through the codec loops without their per-record checks; record 0's
+// check remains.
 template <uint32_t kCodec, class Out>
 __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChunk &c, uint32_t first, uint32_t row,
                                                 const double *elf_tab, Out &out) {
@@ -86,13 +89,13 @@ __device__ __forceinline__ uint32_t segment_wave(const ColView &cv, const DevChu
     const uint32_t cnt = kRun + (__ballot(term_next) != 0);
     const uint32_t lim = term_next ? last : min(last, a + kRun);
     if (kCodec == kCodecGorilla)
-      e = gorilla_seg(s, bits, c.n, a, cnt, from, lim, out, e);
+      e = gorilla_seg<false>(s, bits, c.n, a, cnt, from, lim, out, e);
     else if (kCodec == kCodecChimp)
-      e = chimp_seg(s, bits, c.n, a, cnt, from, lim, out, e);
+      e = chimp_seg<false>(s, bits, c.n, a, cnt, from, lim, out, e);
     else if (kCodec == kCodecElfOnChimp)
-      e = elfchimp_seg(s, bits, c.n, a, cnt, from, lim, out, e, elf_tab);
+      e = elfchimp_seg<false>(s, bits, c.n, a, cnt, from, lim, out, e, elf_tab);
     else
-      e = elf_seg(s, bits, c.n, a, cnt, from, lim, out, e, elf_tab);
+      e = elf_seg<false>(s, bits, c.n, a, cnt, from, lim, out, e, elf_tab);
     from = max(from, lim);  // a terminator taken as record a + kRun is not parsed again
     out.run_end(k0);
   }
@@ -121,10 +124,13 @@ struct StageOut {
 // values replace its payloads), sources before the job from its ring
 // checkpoint (cv.c128_ring + 128 ring_idx).  out.put(i, v, base) takes
 // record i's value (base: the step's first record; every lane calls it).
-// Record n must be the NaN end marker and an earlier NaN is a count error:
-// a NaN can only come from flags 01 / 10 / 11 (the ring never holds one, as
-// the reference stops at the first).
-template <class Out>
+// With kChecked, record n must be the NaN end marker and an earlier NaN is
+// a count error (a NaN can only come from flags 01 / 10 / 11; the ring never
+// holds one, as the reference stops at the first), and the parse checks
+// each record: the load-time ring build, which is where a Chimp128 chunk is
+// validated (with the index's parse).  The decode and the fused queries run
+// on accepted chunks only and pass kChecked false.
+template <bool kChecked, class Out>
 __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &c, uint32_t first, uint32_t ring_idx,
                                             double *stage, uint8_t *dists, Out &out) {
   const uint32_t lane = lane_id();
@@ -149,8 +155,8 @@ __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &
     e = chimp128_seg_first(r, bits, sink, px, pd);
     from = 1;
   }
-  e = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, live ? min(i0 + kSeg128, total) : 0u, sink, e, px, pd,
-                         live && i0 == 0);
+  e = chimp128_parse_seg<kChecked>(r, lead, bits, i0, kSeg128, from, live ? min(i0 + kSeg128, total) : 0u, sink, e,
+                                   px, pd, live && i0 == 0);
   wave_sync();  // the stage holds every payload of the job
   const uint32_t j0 = first * kSeg128;  // the job's first record
   const EBI_GLOBAL uint64_t *ring = as_global(cv.c128_ring + 128ull * ring_idx);
@@ -181,10 +187,12 @@ __device__ __forceinline__ uint32_t c128_job(const ColView &cv, const DevChunk &
     }
     if (i < total) *slot = as_f64(v);
     wave_sync();  // this step's values are in the stage before the next step reads them
-    if (i < c.n) {
-      if (v == kNan && !Out::kNanEnds) e |= kErrCount;
-    } else if (i == c.n && v != kNan) {
-      e |= kErrCount;
+    if (kChecked) {
+      if (i < c.n) {
+        if (v == kNan && !Out::kNanEnds) e |= kErrCount;
+      } else if (i == c.n && v != kNan) {
+        e |= kErrCount;
+      }
     }
     out.put(i, v, base);
   }
@@ -214,7 +222,7 @@ __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, 
     case kCodecElfOnChimp: return segment_wave<kCodecElfOnChimp>(cv, c, first, row, elf_tab, out);
     case kCodecChimp128: {
       C128Out o{cv.out + c.rec_off, c.n};
-      return c128_job(cv, c, first, row, stage, dists, o);
+      return c128_job<false>(cv, c, first, row, stage, dists, o);
     }
     default: return kErrCorrupt;
   }
@@ -503,7 +511,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QueryS
     if constexpr (kC128) {
       const uint32_t wave = threadIdx.x / kWave;
       C128Query q{pr, mask, out_bits, partials, &c, pred, want_bits, agg, 0.0, 0.0, ~0ull, 0, 0};
-      const uint32_t e = c128_job(cv, c, jb.tile, jb.col, lds[wave], dists[wave], q);
+      const uint32_t e = c128_job<false>(cv, c, jb.tile, jb.col, lds[wave], dists[wave], q);
       if (e) atomicOr(err, e);
       if (agg > 0) keys_to_acc(agg, q.kmin, q.kmax, acc + 4ull * jb.chunk);
       if (count && lane == 0 && q.cnt) atomicAdd((unsigned long long *)(acc + 4ull * jb.chunk) + 1, (unsigned long long)q.cnt);
@@ -691,7 +699,7 @@ __global__ void __launch_bounds__(256) k_c128_ring_step(const ColView *cols, con
   const DevChunk c = cv.chunks[ci];
   C128Ring o{j + 1 < job_first[ci + 1] ? ring + 128ull * (jb.col + 1) : nullptr,
              jb.tile * kSeg128 + kJob128 - 128, c.n, ~0u};
-  uint32_t e = c128_job(cv, c, jb.tile, jb.col, lds[wave], dists[wave], o);
+  uint32_t e = c128_job<true>(cv, c, jb.tile, jb.col, lds[wave], dists[wave], o);
   uint32_t nan = o.nan;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) nan = min(nan, (uint32_t)__shfl_xor((int)nan, d, kWave));

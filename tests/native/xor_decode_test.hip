@@ -84,16 +84,17 @@ struct RowSink {
   }
 };
 
-// One lockstep run of a Gorilla / Chimp / Elf / ElfOnChimp lane, and a
-// lane's record 0.
-template <class Sink>
+// One lockstep run of a Gorilla / Chimp / Elf / ElfOnChimp lane (with the
+// per-record checks, or without them as the segment kernels run accepted
+// chunks), and a lane's record 0.
+template <bool kChecked, class Sink>
 static uint32_t seg_step(int codec, XorState &s, uint32_t bits, uint32_t n, uint32_t a, uint32_t cnt, uint32_t from,
                          uint32_t lim, Sink &sink, uint32_t e) {
   switch (codec) {
-    case kCodecGorilla: return gorilla_seg(s, bits, n, a, cnt, from, lim, sink, e);
-    case kCodecChimp: return chimp_seg(s, bits, n, a, cnt, from, lim, sink, e);
-    case kCodecElfOnChimp: return elfchimp_seg(s, bits, n, a, cnt, from, lim, sink, e, g_elf_tab);
-    default: return elf_seg(s, bits, n, a, cnt, from, lim, sink, e, g_elf_tab);
+    case kCodecGorilla: return gorilla_seg<kChecked>(s, bits, n, a, cnt, from, lim, sink, e);
+    case kCodecChimp: return chimp_seg<kChecked>(s, bits, n, a, cnt, from, lim, sink, e);
+    case kCodecElfOnChimp: return elfchimp_seg<kChecked>(s, bits, n, a, cnt, from, lim, sink, e, g_elf_tab);
+    default: return elf_seg<kChecked>(s, bits, n, a, cnt, from, lim, sink, e, g_elf_tab);
   }
 }
 template <class Sink>
@@ -105,6 +106,7 @@ static uint32_t seg_first(int codec, XorState &s, uint32_t bits, Sink &sink) {
   }
 }
 
+template <bool kChecked = true>
 static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t bits, uint32_t cn, const XorCkpt *ck,
                                       uint32_t nseg, double *out, const Ilv *ilv = nullptr) {
   RowSink sink{out, cn};
@@ -140,7 +142,7 @@ static std::vector<uint32_t> lockstep(int codec, const uint8_t *data, uint32_t b
       for (uint32_t l = 0; l < kWave; ++l) {
         const uint32_t a = i0[l] + k0;
         const uint32_t lim = last[l] == a + kRun + 1 ? last[l] : std::min(last[l], a + kRun);
-        e[l] = seg_step(codec, st[l], bits, cn, a, cnt, from[l], lim, sink, e[l]);
+        e[l] = seg_step<kChecked>(codec, st[l], bits, cn, a, cnt, from[l], lim, sink, e[l]);
         from[l] = std::max(from[l], lim);
       }
     }
@@ -256,7 +258,7 @@ static void nan_cases(int codec) {
             ee = chimp128_seg_first(r, bits, js, px, pd);
             from = 1;
           }
-          ee = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, std::min(i0 + kSeg128, end + 1), js, ee, px, pd,
+          ee = chimp128_parse_seg<true>(r, lead, bits, i0, kSeg128, from, std::min(i0 + kSeg128, end + 1), js, ee, px, pd,
                                   i0 == 0);
           expect(ee == kErrNone, "NaN: Chimp128 parse", codec, 9, chunk, k, i0);
           for (uint32_t t = 0; t < kSeg128 && i0 + t <= end; ++t) {
@@ -314,7 +316,7 @@ static void nan_cases(int codec) {
           for (uint32_t l = 0; l < kWave; ++l) {
             const uint32_t a = i0[l] + k0;
             const uint32_t lim = last[l] == a + kRun + 1 ? last[l] : std::min(last[l], a + kRun);
-            ee[l] = seg_step(codec, st[l], bits, end, a, cnt, from[l], lim, rs, ee[l]);
+            ee[l] = seg_step<true>(codec, st[l], bits, end, a, cnt, from[l], lim, rs, ee[l]);
             from[l] = std::max(from[l], lim);
           }
         }
@@ -425,6 +427,13 @@ int main() {
                 expect(le[sg] == kErrNone, "lockstep (ilv) error", codec, g, chunk, k, sg * kSeg);
               for (uint32_t i = 0; i < cn; ++i)
                 expect(as_u64(out[i]) == as_u64(want[i]), "lockstep (ilv) value", codec, g, chunk, k, i);
+              // the kernels' form: no per-record checks on the accepted chunk
+              std::fill(out.begin(), out.end(), 0.0);
+              const std::vector<uint32_t> lu = lockstep<false>(codec, data, bits, cn, ck.data(), nseg, out.data(), &ilv);
+              for (uint32_t sg = 0; sg < nseg; ++sg)
+                expect(lu[sg] == kErrNone, "unchecked lockstep error", codec, g, chunk, k, sg * kSeg);
+              for (uint32_t i = 0; i < cn; ++i)
+                expect(as_u64(out[i]) == as_u64(want[i]), "unchecked lockstep value", codec, g, chunk, k, i);
               // damaged streams (checkpoints of the intact one): a flipped
               // bit, then a stream cut short; both forms must stop each
               // segment with the same error, and agree on segments that
@@ -484,9 +493,21 @@ int main() {
                   ee = chimp128_seg_first(r, bits, js, px, pd);
                   from = 1;
                 }
-                ee = chimp128_parse_seg(r, lead, bits, i0, kSeg128, from, std::min(i0 + kSeg128, cn + 1), js, ee, px, pd,
-                                        i0 == 0);
+                // the kernels' form (no per-record checks) from the same state
+                BitWindow ru = r;
+                uint32_t leadu = lead;
+                std::vector<double> xu(kSeg128, 0.0);
+                std::vector<uint32_t> du(kSeg128, 0);
+                JobSink ju{xu.data(), du.data()};
+                const uint32_t eu = chimp128_parse_seg<false>(ru, leadu, bits, i0, kSeg128, from,
+                                                              std::min(i0 + kSeg128, cn + 1), ju, ee, px, pd, i0 == 0);
+                ee = chimp128_parse_seg<true>(r, lead, bits, i0, kSeg128, from, std::min(i0 + kSeg128, cn + 1), js, ee,
+                                              px, pd, i0 == 0);
                 expect(ee == kErrNone, "Chimp128 parse error", codec, g, chunk, k, i0);
+                expect(eu == kErrNone, "unchecked Chimp128 parse error", codec, g, chunk, k, i0);
+                for (uint32_t t = from - i0; t < kSeg128 && i0 + t <= cn; ++t)  // record 0: parsed before
+                  expect(as_u64(xu[t]) == as_u64(xs[l * kSeg128 + t]) && du[t] == ds[l * kSeg128 + t],
+                         "unchecked Chimp128 parse", codec, g, chunk, k, i0 + t);
               }
               std::vector<uint64_t> res(kJob128, 0);
               for (uint32_t t = 0; t < kJob128 && j0 + t <= cn; ++t) {
