@@ -177,7 +177,7 @@ struct alignas(16) ColView {
   // BUFF decode precision of this view (Decoder::with_precision,
   // api/decoder.rs:159-161): -1 full, else 0..12 (buff_tile)
   int32_t precision;
-  uint32_t pad1;
+  uint32_t codec;    // the column's scheme (the codec of every chunk)
   const uint4 *ilv;  // lane-interleaved segment streams (Gorilla, Chimp, Elf; null before they are built)
 };
 static_assert(sizeof(ColView) == 112, "ColView layout");

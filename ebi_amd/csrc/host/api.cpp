@@ -115,6 +115,7 @@ ColView view_of(const Column &c, double *out) {
   v.c128_ring = c.d_c128_ring;
   v.chunk_end = c.d_chunk_end;
   v.precision = c.precision;
+  v.codec = (uint32_t)c.meta.scheme;
   return v;
 }
 
@@ -167,7 +168,7 @@ int enqueue_decode(ebi_hip_ctx *ctx, Column &c) {
   // Chimp128 always decodes in segment jobs (parse and values in one pass)
   const uint32_t segs = ctx->xor_mode == 0 || c.chimp128 ? c.n_seg_jobs : 0;
   launch::decode(ctx->stream, c.d_view, 1, c.d_prefix, c.n_tile_jobs, c.d_prefix + 6, segs, ctx->d_flags,
-                 (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg);
+                 (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg, false, 1u << c.meta.scheme);
   hipEventRecord(ctx->ev_mid, ctx->stream);
   if (c.xor_codec && ctx->xor_mode != 0)
     launch::serial(ctx->stream, c.d_view, c.d_prefix + 2, 1, c.n_serial_jobs, nullptr, ctx->d_flags, true, false);
@@ -1002,7 +1003,7 @@ extern "C" int ebi_hip_decode_per_chunk(ebi_hip_ctx *ctx, const ebi_hip_col *col
       hipMemcpyAsync(ctx->batch_args.p, ctx->h_batch, 256 + sizeof v, hipMemcpyHostToDevice, ctx->stream);
       launch::decode(ctx->stream, (const ColView *)((uint8_t *)ctx->batch_args.p + 256), 1,
                      (const uint32_t *)ctx->batch_args.p, b - a, nullptr, 0, ctx->d_flags,
-                     (uint32_t)ctx->cu_count * 8);
+                     (uint32_t)ctx->cu_count * 8, nullptr, false, 1u << c.meta.scheme);
     }
     hipError_t e = stream_wait(ctx->stream);  // the same wait as the batched forms
     if (e != hipSuccess) return hip_fail(ctx, e, "per-chunk decode");
@@ -1078,6 +1079,7 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   uint32_t *preseg = pre + (n_cols + 1);
   ColView *views = reinterpret_cast<ColView *>(ctx->h_batch + pre_bytes);
   uint64_t jobs = 0, segs = 0;
+  uint32_t codecs = 0;
   for (size_t i = 0; i < n_cols; ++i) {
     const Column &c = cols[i]->c;
     pre[i] = (uint32_t)jobs;
@@ -1085,6 +1087,7 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
     views[i] = view_of(c, d_outs[i]);
     jobs += c.n_tile_jobs;
     segs += c.n_seg_jobs;
+    codecs |= 1u << c.meta.scheme;
   }
   pre[n_cols] = (uint32_t)jobs;
   preseg[n_cols] = (uint32_t)segs;
@@ -1098,7 +1101,7 @@ extern "C" int ebi_hip_decode_batch(ebi_hip_ctx *ctx, const ebi_hip_col *const *
   const ColView *dv = (const ColView *)(d + pre_bytes);
   launch::decode(ctx->stream, dv, (uint32_t)n_cols, (const uint32_t *)d, (uint32_t)jobs,
                  (const uint32_t *)d + (n_cols + 1), (uint32_t)segs, ctx->d_flags,
-                 (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg);
+                 (uint32_t)ctx->cu_count * ctx->blocks_per_cu, ctx->ev_seg, false, codecs);
   hipEventRecord(ctx->ev_mid, ctx->stream);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "batch launch");
   hipEventRecord(ctx->ev1, ctx->stream);

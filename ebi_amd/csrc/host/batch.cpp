@@ -33,6 +33,7 @@ struct ebi_hip_batch {
   uint32_t jobs[kShapes] = {};
   uint64_t pre_bytes = 0;
   uint32_t n_cols = 0, segs = 0;
+  uint32_t codecs = 0;  // bit c: some column has codec c
   ebi_batch_config cfg{};
   std::string key;  // tuning-cache signature
   hipGraph_t graph = nullptr;
@@ -60,7 +61,7 @@ void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool
   if (timed) hipEventRecord(ctx->ev0, s);
   launch::decode(s, views_of(b, sh), b->n_cols, prefix_of(b, sh), b->jobs[sh], prefix_of(b, sh) + (b->n_cols + 1),
                  b->segs, ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr,
-                 c.mixed != 0);
+                 c.mixed != 0, b->codecs);
   if (timed) hipEventRecord(ctx->ev_mid, s);
   if (timed) hipEventRecord(ctx->ev1, s);
 }
@@ -151,6 +152,7 @@ extern "C" int ebi_hip_batch_create(ebi_hip_ctx *ctx, const ebi_hip_col *const *
     b->cols.push_back(&c);
     b->outs.push_back(d_outs[i]);
     segs += c.n_seg_jobs;
+    b->codecs |= 1u << c.meta.scheme;
     keys.push_back(column_key(c));
   }
   std::sort(keys.begin(), keys.end());

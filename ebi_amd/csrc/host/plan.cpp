@@ -347,6 +347,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
   uint64_t tiles[kSpanKinds] = {};  // span jobs per SpanKind
   uint32_t n_fills = 0;
   uint32_t qseg_codecs = 0;  // bit c: segment jobs of codec c to query
+  uint32_t dec_codecs = 0;   // bit c: a column of codec c to decode
   bool any_fold = false;
   auto fill = [&](uint64_t *ptr, uint64_t words, uint64_t v0, uint64_t v1, uint64_t v2, uint64_t v3,
                   uint32_t period) {
@@ -371,6 +372,7 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
     if (p.decode) {
       jobs += c.n_tile_jobs;
       segs += c.n_seg_jobs;
+      dec_codecs |= 1u << c.meta.scheme;
     }
     PlanDesc &d = descs[i];
     d = PlanDesc{};
@@ -436,7 +438,8 @@ hipError_t group_enqueue(PlanGroup &g, hipStream_t s, int cu_count, uint32_t blo
                      (uint32_t)chunks);
   launch::decode(s, reinterpret_cast<const ColView *>(g.dev + L.views), np,
                  reinterpret_cast<const uint32_t *>(g.dev + L.dpre), (uint32_t)jobs,
-                 reinterpret_cast<const uint32_t *>(g.dev + L.dseg), (uint32_t)segs, g.d_flags, max_blocks);
+                 reinterpret_cast<const uint32_t *>(g.dev + L.dseg), (uint32_t)segs, g.d_flags, max_blocks, nullptr,
+                 false, dec_codecs);
   for (int k = 0; k < kSpanKinds; ++k) launch::group_tiles(s, dd, di + (size_t)(kPreSpan + k) * (n + 1), np, (uint32_t)tiles[k], k);
   launch::query_seg(s, dd, di + (size_t)kPreSeg * (n + 1), np, (uint32_t)qsegs, g.d_flags, max_blocks, qseg_codecs);
   launch::group_fold(s, dd, di, np, (uint32_t)chunks, any_fold);

@@ -228,25 +228,28 @@ __device__ __forceinline__ uint32_t segment_job(const ColView &cv, uint32_t ci, 
   }
 }
 
-// Tile job j of the launch (Uncompressed, BUFF, DeltaSprintz, RLE; the
-// columns' tile_jobs lists concatenated).  stage: the wave's store stage
+// Tile job jl of column cv (Uncompressed, BUFF, DeltaSprintz, RLE): kCodec
+// is the codec of every chunk of the launch (its tile loop alone is compiled
+// in), or kAnyCodec (each chunk's own).  stage: the wave's store stage
 // (kStageStride * kWave doubles; an RLE step's field marks before that).
-__device__ __forceinline__ void tile_job(const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t j,
-                                         double *stage, uint32_t *err) {
-  const uint32_t ci = find_col(prefix, n_cols, j);
-  const ColView cv = cols[ci];
+constexpr uint32_t kAnyCodec = 0xffu;
+template <uint32_t kCodec>
+__device__ __forceinline__ void tile_job_of(const ColView &cv, uint32_t jl, double *stage, uint32_t *err) {
   // jb.kind consecutive tiles from jb.tile; RLE: jb.kind output steps
   // from step jb.tile >> 24 of field tile jb.tile & 0xffffff
-  const Job jb = cv.tile_jobs[j - prefix[ci]];
+  const Job jb = cv.tile_jobs[jl];
   const DevChunk c = cv.chunks[jb.chunk];
+  const uint32_t codec = kCodec == kAnyCodec ? c.codec : kCodec;
   if (cv.chunk_err && cv.chunk_err[jb.chunk]) {
     // chunk rejected at load: nothing to decode
-  } else if (c.codec == kCodecRLE) {
+  } else if (codec != c.codec) {
+    if (lane_id() == 0) atomicOr(err, (uint32_t)kErrCorrupt);
+  } else if (codec == kCodecRLE) {
     const uint32_t t = jb.tile & 0xffffffu;
     rle_tile(c, t, jb.tile >> 24, jb.kind, cv.tile_index[c.tile0 + t], cv.out, stage);
   } else {
     for (uint32_t t = jb.tile; t < jb.tile + jb.kind; ++t) {
-      switch (c.codec) {
+      switch (codec) {
         case kCodecPlain: plain_tile(c, t, cv.out); break;
         case kCodecBUFF: buff_tile(c, t, cv.out, stage, cv.precision); break;
         case kCodecSprintz: sprintz_tile(c, t, cv.tile_index[c.tile0 + t], cv.out, stage); break;
@@ -256,6 +259,13 @@ __device__ __forceinline__ void tile_job(const ColView *cols, const uint32_t *pr
       }
     }
   }
+}
+
+// Tile job j of the launch (the columns' tile_jobs lists concatenated).
+__device__ __forceinline__ void tile_job(const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t j,
+                                         double *stage, uint32_t *err) {
+  const uint32_t ci = find_col(prefix, n_cols, j);
+  tile_job_of<kAnyCodec>(cols[ci], j - prefix[ci], stage, err);
 }
 
 // Segment job j of the launch (64 checkpoint segments of a Gorilla / Chimp /
@@ -271,18 +281,36 @@ __device__ __forceinline__ void seg_job(const ColView *cols, const uint32_t *pre
   }
 }
 
-// Persistent decode over the tile jobs of n_cols columns (mixed codecs and
-// uneven chunks in one launch).  Wave w takes jobs w, w + W, w + 2W, ...
-// (W = waves in the grid): a static interleave, so neighbouring jobs
-// (similar cost) spread over all waves and no job counter is contended.
-// amdgpu_waves_per_eu(5): the register budget of 5 waves per SIMD.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_decode(
+// Persistent decode over the tile jobs of the columns of one tile codec
+// among n_cols columns (uneven chunks in one launch; one launch per codec
+// present).  Wave w takes jobs w, w + W, w + 2W, ... of the launch's whole
+// job list (W = waves in the grid): a static interleave, so neighbouring
+// jobs (similar cost) spread over all waves and no job counter is
+// contended; a wave that reaches a column of another codec steps past its
+// jobs at once.  One kernel per codec so that each gets the waves per SIMD
+// its own registers allow (TileWaves: the memory-bound copy at 8, BUFF at 5,
+// RLE at 6, DeltaSprintz's wave scans at 5), to keep more loads in flight.
+template <uint32_t kCodec>
+struct TileWaves {
+  static constexpr int value = kCodec == kCodecSprintz ? 5 : kCodec == kCodecRLE ? 6 : kCodec == kCodecBUFF ? 5 : 8;
+};
+template <uint32_t kCodec>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileWaves<kCodec>::value))) k_decode(
     const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t n_jobs, uint32_t *err) {
   __shared__ __align__(16) double lds[4][kStageStride * kWave];
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
-  for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); j < n_jobs; j += stride)
-    tile_job(cols, prefix, n_cols, j, lds[wave], err);
+  for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); j < n_jobs; j += stride) {
+    const uint32_t ci = find_col(prefix, n_cols, j);
+    const ColView cv = cols[ci];
+    if (cv.codec != kCodec) {
+      // wave-uniform: on to this wave's first job past the column (the loop
+      // adds the last stride)
+      j += (prefix[ci + 1] - j - 1) / stride * stride;
+      continue;
+    }
+    tile_job_of<kCodec>(cv, j - prefix[ci], lds[wave], err);
+  }
 }
 
 // Persistent decode over the checkpoint-segment jobs of n_cols columns, with
@@ -832,7 +860,7 @@ void zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
 
 void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t *tile_prefix, uint32_t n_tile_jobs,
             const uint32_t *seg_prefix, uint32_t n_seg_jobs, uint32_t *err, uint32_t max_blocks, hipEvent_t mid,
-            bool mixed) {
+            bool mixed, uint32_t codecs) {
   auto grid = [&](uint32_t jobs) {
     const uint32_t blocks = (jobs + 3) / 4;
     return max_blocks && blocks > max_blocks ? max_blocks : blocks;
@@ -844,9 +872,18 @@ void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t 
     if (mid) hipEventRecord(mid, s);
     return;
   }
-  if (n_tile_jobs)
-    hipLaunchKernelGGL(k_decode, dim3(grid(n_tile_jobs)), dim3(256), 0, s, cols, tile_prefix, n_cols, n_tile_jobs,
-                       err);
+  if (!codecs) codecs = kTileCodecs;
+  if (n_tile_jobs) {
+    const dim3 g(grid(n_tile_jobs));
+    if (codecs & (1u << kCodecPlain))
+      hipLaunchKernelGGL(k_decode<kCodecPlain>, g, dim3(256), 0, s, cols, tile_prefix, n_cols, n_tile_jobs, err);
+    if (codecs & (1u << kCodecBUFF))
+      hipLaunchKernelGGL(k_decode<kCodecBUFF>, g, dim3(256), 0, s, cols, tile_prefix, n_cols, n_tile_jobs, err);
+    if (codecs & (1u << kCodecRLE))
+      hipLaunchKernelGGL(k_decode<kCodecRLE>, g, dim3(256), 0, s, cols, tile_prefix, n_cols, n_tile_jobs, err);
+    if (codecs & (1u << kCodecSprintz))
+      hipLaunchKernelGGL(k_decode<kCodecSprintz>, g, dim3(256), 0, s, cols, tile_prefix, n_cols, n_tile_jobs, err);
+  }
   if (mid) hipEventRecord(mid, s);
   if (n_seg_jobs)
     hipLaunchKernelGGL(k_decode_seg, dim3(grid(n_seg_jobs)), dim3(256), 0, s, cols, seg_prefix, n_cols,

@@ -21,9 +21,12 @@ void zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 // (nullable) is recorded between the two kernels.  mixed: when both lists
 // are non-empty, one k_decode_mix launch over both instead (`mid` recorded
 // after it).
+// codecs: bit c set when some column has codec c (one k_decode launch per
+// tile codec present; 0: every tile codec).
+constexpr uint32_t kTileCodecs = (1u << kCodecPlain) | (1u << kCodecRLE) | (1u << kCodecBUFF) | (1u << kCodecSprintz);
 void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t *tile_prefix, uint32_t n_tile_jobs,
             const uint32_t *seg_prefix, uint32_t n_seg_jobs, uint32_t *err, uint32_t max_blocks,
-            hipEvent_t mid = nullptr, bool mixed = false);
+            hipEvent_t mid = nullptr, bool mixed = false, uint32_t codecs = 0);
 // Lane-per-chunk serial XOR decode (Gorilla, Chimp, Elf); `index` writes
 // checkpoints.  prefix is over n_serial_jobs; `single` (device, nullable)
 // runs one explicit job on cols[0] instead.

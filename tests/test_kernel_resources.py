@@ -11,7 +11,10 @@ import pytest
 from ebi_amd import build as B
 
 KERNELS = {  # mangled-name prefix -> minimum occupancy (waves per SIMD)
-    "_ZN3ebi8k_decodeE": 5,
+    "_ZN3ebi8k_decodeILj0EE": 8,  # Uncompressed tile jobs
+    "_ZN3ebi8k_decodeILj1EE": 6,  # RLE
+    "_ZN3ebi8k_decodeILj2EE": 5,  # BUFF
+    "_ZN3ebi8k_decodeILj8EE": 5,  # DeltaSprintz
     "_ZN3ebi12k_decode_segE": 4,
     "_ZN3ebi12k_decode_mixE": 4,  # both job lists in one launch
     "_ZN3ebi11k_query_segILj3EE": 5,  # the segment decode with the fused queries: Gorilla
