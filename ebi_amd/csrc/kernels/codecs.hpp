@@ -663,7 +663,7 @@ __host__ __device__ __forceinline__ uint32_t elf_seg(XorState &s, uint32_t size_
     const uint64_t m = r.field(off, len ? len : 1u) & (len ? ~0ull : 0ull);
     const uint64_t x = f == 0 ? m << (T & 63) : ((m << 1) | 1) << nT;
     const uint64_t val = x ^ stored;
-    const bool term = f != 1 && val == kNan;
+    const bool term = kChecked && f != 1 && val == kNan;  // unchecked: only record n can be the NaN, and it is never output
     const bool upd = act && !bad;
     stored = upd && f != 1 && !term ? val : stored;
     L = upd ? nL : L;
