@@ -68,8 +68,9 @@ __device__ __forceinline__ void wave_fence_global() {
 // predecessor's exit (a ballot), and their first record indexes, values
 // and beta* are scans over the lanes.  A miss is parsed exactly by the
 // whole wave (the same walk on every lane); Gorilla's walker blocks (each
-// lane one block_cand) follow a miss whose state is not the one the next
-// span was guessed in.  Same results as chunk_walk.
+// lane one block_cands walk over kBlockSpansPerLane spans) follow a miss
+// whose state is not the one the next span was guessed in.  Same results
+// as chunk_walk.
 template <uint32_t kCodec>
 __device__ void wave_walk(const uint8_t *data, uint32_t bits, uint32_t ns, ChunkWalk &w, uint64_t *cand,
                           SpanSum *res, SpanPick *pick, uint32_t stop, bool blocks) {
@@ -131,9 +132,11 @@ __device__ void wave_walk(const uint8_t *data, uint32_t bits, uint32_t ns, Chunk
     ++i0;
     const uint32_t st = (uint32_t)(w.cur >> 32) & 0xffff;
     if (blocks && i0 < end && st != (i0 < bend ? bhint : chint)) {
-      bend = min(end, i0 + (uint32_t)kWave);
+      bend = min(end, i0 + (uint32_t)kWave * kBlockSpansPerLane);
       bhint = st;
-      if (i0 + l < bend) block_cand<kCodec, XWindow>(data, bits, ns, i0 + l, i0, w.cur, cand, res);
+      const uint32_t j = i0 + l * kBlockSpansPerLane;
+      if (j < bend)
+        block_cands<kCodec, XWindow>(data, bits, ns, j, min(j + kBlockSpansPerLane, bend), i0, w.cur, cand, res);
       wave_fence_global();
     }
   }

@@ -469,24 +469,32 @@ __host__ __device__ __forceinline__ void span_cands(const uint8_t *data, uint32_
 }
 
 // ------------------------------------------------------------ step 2
-// Candidate of span i of a Gorilla walker's block that starts at span b0
-// (true entry `from`): a walk in `from`'s state from kBlockLookback spans
-// before i, or the exact walk from `from` when that would start before b0.
+// Candidates of spans [j, j1) of a Gorilla walker's block that starts at
+// span b0 (true entry `from`): one walk in `from`'s state from
+// kBlockLookback spans before j, or the exact walk from `from` when that
+// would start before b0.  A block covers kWaveLanes * kBlockSpansPerLane
+// spans, kBlockSpansPerLane per walk: after a state change the stream
+// usually keeps the new state to the chunk's end, and every block boundary
+// is a miss (its span was guessed in the old state), so longer blocks save
+// boundaries at the price of longer walks.
 constexpr uint32_t kBlockLookback = 2;
+constexpr uint32_t kBlockSpansPerLane = 4;
 template <uint32_t kCodec, class W>
-__host__ __device__ __forceinline__ void block_cand(const uint8_t *data, uint32_t bits, uint32_t ns, uint32_t i,
-                                                uint32_t b0, uint64_t from, uint64_t *cand, SpanSum *res) {
-  if (i < b0 + kBlockLookback)
-    cand_walk<kCodec, W>(data, bits, ns, i, i + 1, b0, false, 0, cand, res, from);
+__host__ __device__ __forceinline__ void block_cands(const uint8_t *data, uint32_t bits, uint32_t ns, uint32_t j,
+                                                 uint32_t j1, uint32_t b0, uint64_t from, uint64_t *cand,
+                                                 SpanSum *res) {
+  if (j < b0 + kBlockLookback)
+    cand_walk<kCodec, W>(data, bits, ns, j, j1, b0, false, 0, cand, res, from);
   else
-    cand_walk<kCodec, W>(data, bits, ns, i, i + 1, i - kBlockLookback, true, from, cand, res);
+    cand_walk<kCodec, W>(data, bits, ns, j, j1, j - kBlockLookback, true, from, cand, res);
 }
 
 // Step 2 for one chunk from w.frontier up to span `stop` (Gorilla first
 // walks to span kHintSpans, whose state becomes the hint of its candidate
 // walks).  Gorilla (blocks): after a miss whose exact parse ends in a state
 // other than the one the next span's candidate was guessed in, the walker
-// guesses the next kWaveLanes spans itself in the new state (block_cand).
+// guesses the next kWaveLanes * kBlockSpansPerLane spans itself in the new
+// state (block_cands).
 // After a span whose true walk fails, the later spans get kEntDead entries.
 template <uint32_t kCodec, class W>
 __host__ __device__ __forceinline__ void chunk_walk(const uint8_t *data, uint32_t bits, uint32_t ns, ChunkWalk &w,
@@ -509,9 +517,10 @@ __host__ __device__ __forceinline__ void chunk_walk(const uint8_t *data, uint32_
     w.cur = s.exit;
     const uint32_t st = (uint32_t)(w.cur >> 32) & 0xffff, nx = i + 1;
     if (blocks && !hit && nx < end && st != (nx < bend ? bhint : chint)) {
-      bend = min(end, nx + kWaveLanes);
+      bend = min(end, nx + kWaveLanes * kBlockSpansPerLane);
       bhint = st;
-      for (uint32_t k = nx; k < bend; ++k) block_cand<kCodec, W>(data, bits, ns, k, nx, w.cur, cand, res);
+      for (uint32_t k = nx; k < bend; k += kBlockSpansPerLane)
+        block_cands<kCodec, W>(data, bits, ns, k, min(k + kBlockSpansPerLane, bend), nx, w.cur, cand, res);
     }
   }
   w.frontier = end;
