@@ -28,6 +28,23 @@ __device__ __forceinline__ uint32_t find_col(const uint32_t *prefix, uint32_t n_
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
+// The column of launch-wide job j, remembered across a persistent wave's
+// jobs: the wave's successive jobs (a grid stride apart) mostly fall in the
+// same column, so the binary search over the prefix -- a chain of dependent
+// loads at every job start -- runs only when j leaves that column's range.
+// All fields wave-uniform.  Also used over the plans of a query group.
+struct ColCursor {
+  uint32_t ci = 0, lo = 1, hi = 0;  // column ci holds the jobs [lo, hi); empty at first
+  __device__ __forceinline__ uint32_t at(const uint32_t *prefix, uint32_t n_cols, uint32_t j) {
+    if (j < lo || j >= hi) {
+      ci = find_col(prefix, n_cols, j);
+      lo = __builtin_amdgcn_readfirstlane(prefix[ci]);
+      hi = __builtin_amdgcn_readfirstlane(prefix[ci + 1]);
+    }
+    return ci;
+  }
+};
+
 // State of a lane with no segment in a lockstep wave: every field defined
 // (zeros), position 0, so refill() and field() compute on known values.
 __device__ __forceinline__ void dead_window(BitWindow &r) {
@@ -268,13 +285,11 @@ __device__ __forceinline__ void tile_job(const ColView *cols, const uint32_t *pr
   tile_job_of<kAnyCodec>(cols[ci], j - prefix[ci], stage, err);
 }
 
-// Segment job j of the launch (64 checkpoint segments of a Gorilla / Chimp /
-// Elf chunk, or a Chimp128 job; the columns' seg_jobs lists concatenated).
-__device__ __forceinline__ void seg_job(const ColView *cols, const uint32_t *prefix, uint32_t n_cols, uint32_t j,
-                                        double *stage, uint8_t *dists, const double *elf_tab, uint32_t *err) {
-  const uint32_t ci = find_col(prefix, n_cols, j);
-  const ColView cv = cols[ci];
-  const Job jb = cv.seg_jobs[j - prefix[ci]];
+// Segment job jl of column cv (64 checkpoint segments of a Gorilla / Chimp /
+// Elf chunk, or a Chimp128 job).
+__device__ __forceinline__ void seg_job_of(const ColView &cv, uint32_t jl, double *stage, uint8_t *dists,
+                                           const double *elf_tab, uint32_t *err) {
+  const Job jb = cv.seg_jobs[jl];
   if (!cv.chunk_err || cv.chunk_err[jb.chunk] == 0) {
     const uint32_t e = segment_job(cv, jb.chunk, jb.tile, jb.col, stage, dists, elf_tab);
     if (e) atomicOr(err, e);
@@ -300,16 +315,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileWa
   __shared__ __align__(16) double lds[4][kStageStride * kWave];
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
+  ColCursor cur;
   for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); j < n_jobs; j += stride) {
-    const uint32_t ci = find_col(prefix, n_cols, j);
-    const ColView cv = cols[ci];
+    const ColView cv = cols[cur.at(prefix, n_cols, j)];
     if (cv.codec != kCodec) {
       // wave-uniform: on to this wave's first job past the column (the loop
       // adds the last stride)
-      j += (prefix[ci + 1] - j - 1) / stride * stride;
+      j += (cur.hi - j - 1) / stride * stride;
       continue;
     }
-    tile_job_of<kCodec>(cv, j - prefix[ci], lds[wave], err);
+    tile_job_of<kCodec>(cv, j - cur.lo, lds[wave], err);
   }
 }
 
@@ -330,8 +345,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   __syncthreads();
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t stride = gridDim.x * 4;
-  for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); j < n_jobs; j += stride)
-    seg_job(cols, prefix, n_cols, j, lds[wave], dists[wave], elf_tab, err);
+  ColCursor cur;
+  for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); j < n_jobs; j += stride) {
+    const ColView cv = cols[cur.at(prefix, n_cols, j)];
+    seg_job_of(cv, j - cur.lo, lds[wave], dists[wave], elf_tab, err);
+  }
 }
 
 // Both job lists in one persistent launch: job j of the n_seg + n_tile
@@ -360,9 +378,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); j < n; j += stride) {
     const uint32_t a = (uint32_t)((uint64_t)j * n_seg_jobs / n);
     const uint32_t b = (uint32_t)((uint64_t)(j + 1) * n_seg_jobs / n);
-    if (b > a)
-      seg_job(cols, seg_prefix, n_cols, a, lds[wave], dists[wave], elf_tab, err);
-    else
+    if (b > a) {
+      const uint32_t ci = find_col(seg_prefix, n_cols, a);
+      seg_job_of(cols[ci], a - seg_prefix[ci], lds[wave], dists[wave], elf_tab, err);
+    } else
       tile_job(cols, tile_prefix, n_cols, j - a, lds[wave], err);
   }
 }
@@ -513,13 +532,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QueryS
   }
   const uint32_t lane = lane_id();
   const uint32_t stride = gridDim.x * 4;
+  ColCursor cur;  // over the plans
   for (uint32_t j = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / kWave); j < n_jobs; j += stride) {
-    const uint32_t pi = plan_of(prefix, n_plans, j);
+    const uint32_t pi = cur.at(prefix, n_plans, j);
     const PlanDesc &P = plans[pi];
     if (P.scheme != kCodec) {
       // wave-uniform: on to this wave's first job past the plan (the loop
       // adds the last stride)
-      j += (prefix[pi + 1] - j - 1) / stride * stride;
+      j += (cur.hi - j - 1) / stride * stride;
       continue;
     }
     const PredF64 pr = pred_of(P.a);
@@ -533,7 +553,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QueryS
     const int agg = (int)((flags >> kPfAggShift) & 3u) - 1;  // -1 none, 0 sum, 1 min, 2 max
     const bool count = agg == 0 && (flags & kPfAggSel) != 0;
     const ColView cv = *static_cast<const ColView *__restrict__>(P.view);
-    const Job jb = cv.seg_jobs[j - prefix[pi]];
+    const Job jb = cv.seg_jobs[j - cur.lo];
     if (!P.a.chunk_sel[jb.chunk]) continue;
     const DevChunk c = cv.chunks[jb.chunk];
     if constexpr (kC128) {
