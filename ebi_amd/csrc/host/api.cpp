@@ -794,6 +794,9 @@ extern "C" int ebi_hip_ctx_create(int device, ebi_hip_ctx **out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess || hipEventCreate(&c->ev_mid) != hipSuccess ||
       hipEventCreate(&c->ev_seg) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&c->ev_ph[0]) != hipSuccess || hipEventCreate(&c->ev_ph[1]) != hipSuccess ||
       hipEventCreate(&c->ev_ph[2]) != hipSuccess || hipEventCreate(&c->ev_ph[3]) != hipSuccess ||
       hipMalloc((void **)&c->d_flags, 64 * sizeof(uint32_t)) != hipSuccess ||
@@ -851,6 +854,9 @@ extern "C" int ebi_hip_ctx_destroy(ebi_hip_ctx *ctx) {
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   if (ctx->ev_mid) hipEventDestroy(ctx->ev_mid);
   if (ctx->ev_seg) hipEventDestroy(ctx->ev_seg);
+  if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+  if (ctx->side) hipStreamDestroy(ctx->side);
   for (hipEvent_t e : ctx->ev_ph)
     if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);

@@ -53,7 +53,8 @@ const ColView *views_of(const ebi_hip_batch *b, int sh) {
 const uint32_t *prefix_of(const ebi_hip_batch *b, int sh) { return static_cast<const uint32_t *>(b->args[sh].p); }
 
 // Issues the batch on ctx->stream with configuration c.  timed: record the
-// phase events (eager runs only; a captured sequence carries no timing).
+// phase events (eager runs only; a captured sequence carries no timing, and
+// runs the Uncompressed copy on ctx->side beside the segment kernel).
 void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool timed) {
   hipStream_t s = ctx->stream;
   const int sh = shape_of(c.tiles_per_job);
@@ -61,7 +62,8 @@ void enqueue(ebi_hip_ctx *ctx, ebi_hip_batch *b, const ebi_batch_config &c, bool
   if (timed) hipEventRecord(ctx->ev0, s);
   launch::decode(s, views_of(b, sh), b->n_cols, prefix_of(b, sh), b->jobs[sh], prefix_of(b, sh) + (b->n_cols + 1),
                  b->segs, ctx->d_flags, (uint32_t)ctx->cu_count * c.blocks_per_cu, timed ? ctx->ev_seg : nullptr,
-                 c.mixed != 0, b->codecs);
+                 c.mixed != 0, b->codecs, timed ? nullptr : ctx->side, ctx->ev_fork, ctx->ev_join,
+                 2u * (uint32_t)ctx->cu_count);
   if (timed) hipEventRecord(ctx->ev_mid, s);
   if (timed) hipEventRecord(ctx->ev1, s);
 }

@@ -98,6 +98,9 @@ struct ebi_hip_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_seg = nullptr;  // between the tile kernel and the segment kernel of a decode
   hipEvent_t ev_mid = nullptr;  // between the segment kernel and the Chimp128 kernel of a decode
+  // captured batches: the Uncompressed copy runs on `side` beside the segment kernel
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev_ph[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries of matmul calls
   double last_ms = 0.0;
   double phase_ms[4] = {0.0, 0.0, 0.0, 0.0};  // of the last decode_device / decode_batch / matmul call

@@ -880,7 +880,8 @@ void zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
 
 void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t *tile_prefix, uint32_t n_tile_jobs,
             const uint32_t *seg_prefix, uint32_t n_seg_jobs, uint32_t *err, uint32_t max_blocks, hipEvent_t mid,
-            bool mixed, uint32_t codecs) {
+            bool mixed, uint32_t codecs, hipStream_t side, hipEvent_t fork, hipEvent_t join,
+            uint32_t side_blocks) {
   auto grid = [&](uint32_t jobs) {
     const uint32_t blocks = (jobs + 3) / 4;
     return max_blocks && blocks > max_blocks ? max_blocks : blocks;
@@ -893,6 +894,9 @@ void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t 
     return;
   }
   if (!codecs) codecs = kTileCodecs;
+  const bool beside = side && fork && join && side_blocks && n_tile_jobs && n_seg_jobs &&
+                      (codecs & (1u << kCodecPlain));
+  if (beside) codecs &= ~(1u << kCodecPlain);
   if (n_tile_jobs) {
     const dim3 g(grid(n_tile_jobs));
     if (codecs & (1u << kCodecPlain))
@@ -905,9 +909,20 @@ void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t 
       hipLaunchKernelGGL(k_decode<kCodecSprintz>, g, dim3(256), 0, s, cols, tile_prefix, n_cols, n_tile_jobs, err);
   }
   if (mid) hipEventRecord(mid, s);
+  if (beside) {
+    hipEventRecord(fork, s);
+    hipStreamWaitEvent(side, fork, 0);
+    const uint32_t blocks = (n_tile_jobs + 3) / 4;
+    hipLaunchKernelGGL(k_decode<kCodecPlain>, dim3(blocks < side_blocks ? blocks : side_blocks), dim3(256), 0, side,
+                       cols, tile_prefix, n_cols, n_tile_jobs, err);
+  }
   if (n_seg_jobs)
     hipLaunchKernelGGL(k_decode_seg, dim3(grid(n_seg_jobs)), dim3(256), 0, s, cols, seg_prefix, n_cols,
                        n_seg_jobs, err);
+  if (beside) {
+    hipEventRecord(join, side);
+    hipStreamWaitEvent(s, join, 0);
+  }
 }
 
 void query_seg(hipStream_t s, const PlanDesc *plans, const uint32_t *prefix, uint32_t n_plans, uint32_t n_jobs,

@@ -23,10 +23,17 @@ void zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 // after it).
 // codecs: bit c set when some column has codec c (one k_decode launch per
 // tile codec present; 0: every tile codec).
+// side (nullable, with the events fork and join): when there are segment jobs
+// and Uncompressed columns, their copy kernel runs on `side` with side_blocks
+// blocks, forked after the other tile kernels and joined back into s after
+// k_decode_seg, so it fills the wave slots and VGPRs the segment kernel leaves
+// free (it needs no LDS); side_blocks stays small enough that the segment
+// kernel's 4 blocks per CU still fit beside it.
 constexpr uint32_t kTileCodecs = (1u << kCodecPlain) | (1u << kCodecRLE) | (1u << kCodecBUFF) | (1u << kCodecSprintz);
 void decode(hipStream_t s, const ColView *cols, uint32_t n_cols, const uint32_t *tile_prefix, uint32_t n_tile_jobs,
             const uint32_t *seg_prefix, uint32_t n_seg_jobs, uint32_t *err, uint32_t max_blocks,
-            hipEvent_t mid = nullptr, bool mixed = false, uint32_t codecs = 0);
+            hipEvent_t mid = nullptr, bool mixed = false, uint32_t codecs = 0, hipStream_t side = nullptr,
+            hipEvent_t fork = nullptr, hipEvent_t join = nullptr, uint32_t side_blocks = 0);
 // Lane-per-chunk serial XOR decode (Gorilla, Chimp, Elf); `index` writes
 // checkpoints.  prefix is over n_serial_jobs; `single` (device, nullable)
 // runs one explicit job on cols[0] instead.
